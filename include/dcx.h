@@ -1,0 +1,171 @@
+/* dcx — C-ABI of the MI355X-native forward-and-adjoint path for 2.5-D / 3-D
+ * DC-resistivity modelling.
+ *
+ * Every entry point returns 0 on success and a negative status on failure;
+ * dcx_last_error() then gives the reason (thread-local).  No C++ exception
+ * crosses this boundary.  Device pointers are plain device addresses (for
+ * example torch's tensor.data_ptr()); host pointers are marked "host".  Every
+ * call that launches work takes the HIP stream to order it on (void* so that
+ * the header needs no HIP include; pass torch.cuda.current_stream().cuda_stream).
+ * One plan must not be used from two threads or streams at once.
+ *
+ * Layouts: dense right-hand sides and solutions are row-major (n x k) blocks,
+ * rows = matrix rows, k columns contiguous; a batch of them is nbatch such
+ * blocks back to back.  Values of the nbatch matrices are (nbatch x nnz), in
+ * the order of the CSR pattern given to dcx_plan_create.
+ */
+#ifndef DCX_H_
+#define DCX_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct dcx_plan dcx_plan;
+typedef struct dcx_lmap dcx_lmap;
+typedef struct dcx_survey dcx_survey;
+
+enum {
+  DCX_OK = 0,
+  DCX_ERR_ARG = -1,
+  DCX_ERR_HIP = -2,
+  DCX_ERR_STATE = -3,
+  DCX_ERR_NUMERIC = -4,
+  DCX_ERR_INTERNAL = -5
+};
+
+/* factor precision */
+enum { DCX_FP64 = 0, DCX_FP32 = 1 };
+
+/* dcx_solve flags */
+enum {
+  DCX_SHARED_RHS = 1,  /* b is one (n x k) block shared by every batch member */
+  DCX_PERMUTED = 2,    /* b and x are in the plan's elimination order */
+  DCX_SPARSE_RHS = 4   /* b is zero outside the rows set by dcx_set_rhs_support */
+};
+
+typedef struct {
+  int n, nnz, nbatch, nsup, ndepth, max_front, max_ns, precision;
+  int64_t nnz_L;              /* stored factor entries (dense supernode blocks) */
+  double factor_flops;        /* per matrix */
+  double solve_flops_per_rhs; /* forward + backward sweep, per column */
+  int64_t device_bytes;       /* factor + front storage held by the plan */
+} dcx_plan_info;
+
+typedef struct dcx_symbolic dcx_symbolic;
+
+/* Read-only view of a host symbolic analysis (arrays owned by the object).
+ * Supernodes are in postorder; columns use the elimination numbering. */
+typedef struct {
+  int n, nnz, nsup, ndepth, max_front, max_ns;
+  const int32_t *perm, *first, *ns, *rs, *parent, *depth;
+  const int32_t *rptr, *ridx, *cptr, *cidx, *relpos;
+  const int32_t *aptr, *a_e, *a_row, *a_col, *lvl_ptr, *lvl_sup;
+  int64_t nnz_L;
+  double factor_flops, solve_flops_per_rhs;
+} dcx_symbolic_view;
+
+const char* dcx_last_error(void);
+
+/* Host-only symbolic analysis (no device work): the same ordering and
+ * supernode structure dcx_plan_create builds, for inspection and tests. */
+int dcx_symbolic_analyse(int n, const int32_t* crow, const int32_t* col,
+                         const double* coords, int dim, int leaf_max,
+                         dcx_symbolic** out);
+int dcx_symbolic_view_get(const dcx_symbolic* sym, dcx_symbolic_view* out);
+int dcx_symbolic_destroy(dcx_symbolic* sym);
+int dcx_device_count(int* out);
+
+/* Symbolic phase, once per sparsity pattern: nested-dissection ordering,
+ * supernodal front structure, device maps.  Replaces cuDSS plan() in
+ * CuDSSBatchSolver.__init__ (reference dctorch/solver.py:187-213).
+ * crow/col: host, the FULL symmetric pattern (n+1, nnz) with a stored
+ * diagonal.  coords: host (n x dim) node coordinates or NULL (graph-only
+ * ordering).  precision: DCX_FP64 or DCX_FP32 (FP32 factor; dcx_solve then
+ * refines in FP64).  Runs on the current HIP device. */
+int dcx_plan_create(int n, const int32_t* crow, const int32_t* col, int nbatch,
+                    const double* coords, int dim, int leaf_max, int precision,
+                    dcx_plan** out);
+int dcx_plan_destroy(dcx_plan* plan);
+int dcx_plan_get_info(const dcx_plan* plan, dcx_plan_info* out);
+/* host (n): perm[new] = old */
+int dcx_plan_get_perm(const dcx_plan* plan, int32_t* perm);
+/* number of FP64 refinement passes after the first FP32 solve (mixed mode) */
+int dcx_plan_set_refinement(dcx_plan* plan, int passes);
+
+/* Numeric factorisation of all nbatch matrices (values: device, FP64,
+ * nbatch x nnz).  Replaces CuDSSBatchSolver.factorize
+ * (dctorch/solver.py:215-221).  The values are copied (the FP64 copy feeds the
+ * refinement residual). */
+int dcx_factorize(dcx_plan* plan, const double* values, void* stream);
+
+/* Multi-RHS solve of every batch member, FP64 in and out.  Replaces
+ * CuDSSBatchSolver.solve (dctorch/solver.py:223-230).  b: device, (n x k)
+ * with DCX_SHARED_RHS, else (nbatch x n x k); x: device (nbatch x n x k).
+ * b and x may alias when not shared. */
+int dcx_solve(dcx_plan* plan, const double* b, double* x, int k, int flags,
+              void* stream);
+
+/* Rows (host, original numbering) outside of which right-hand sides passed
+ * with DCX_SPARSE_RHS are zero; lets the forward sweep skip the supernodes
+ * whose subtree holds none of them. */
+int dcx_set_rhs_support(dcx_plan* plan, const int32_t* rows, int nrows);
+
+/* Backward gather of the solve: gv[b][e] = -scale[b] * sum_c y[b][row_e][c] *
+ * x[b][col_e][c] over the pattern's entries e.  Replaces the chunked gather of
+ * _SparseSolveBatchFn.backward (dctorch/solver.py:268-276).  y, x: device
+ * (nbatch x n x k); scale: device (nbatch) or NULL (= 1); gv: device
+ * (nbatch x nnz).  With DCX_PERMUTED the blocks are in elimination order. */
+int dcx_grad_values(dcx_plan* plan, const double* y, const double* x, int k,
+                    const double* scale, double* gv, int flags, void* stream);
+
+/* Row permutations between the caller's order and the elimination order:
+ * out[b][new][c] = in[b][perm[new]][c] (to_elim=1) or the inverse. */
+int dcx_permute_rows(dcx_plan* plan, const double* in, double* out, int k,
+                     int nblocks, int to_elim, void* stream);
+
+/* ---- the sigma -> A-values assembly map (factored form) -------------------
+ * A_k(sigma).values[e] = sum_p (le[p] + ky_k^2 ln[p]) sigma[cell[p]]
+ *                        + sum_q bc[k][q] sigma[bc_cell[q]]  (bc_e[q] == e)
+ * Replaces the stacked COO map Lstack and torch.sparse.mm of
+ * TorchDC2D._extract / dpred (dctorch/simulation2d.py:249-260, :392). */
+int dcx_lmap_create(int nnz, int ncell, int nky, const double* ky2,
+                    const int32_t* ptr, const int32_t* cell, const double* le,
+                    const double* ln, int nbc, const int32_t* bc_e,
+                    const int32_t* bc_cell, const double* bc_coef,
+                    dcx_lmap** out);
+int dcx_lmap_destroy(dcx_lmap* map);
+/* values (device, nky x nnz) from sigma (device, ncell) */
+int dcx_assemble(dcx_lmap* map, const double* sigma, double* values, void* stream);
+/* grad_m[c] = -sigma[c] * (L^T gv)[c]: the transposed map applied to the value
+ * gradient (device, nky x nnz), chained through sigma = exp(-m).  Replaces the
+ * sparse-mm backward + exp backward (dctorch/simulation2d.py:392, autograd). */
+int dcx_lmap_transpose(dcx_lmap* map, const double* gv, const double* sigma,
+                       double* grad_m, void* stream);
+
+/* ---- receivers and adjoint sources ----------------------------------------
+ * Pd (nD x nN, CSR, host) and the source index of each datum.  Replaces the
+ * projection torch.sparse.mm(Pd, Ubar)[iD, sidx] and its backward
+ * (dctorch/simulation2d.py:394-395). */
+int dcx_survey_create(dcx_plan* plan, int nD, int nS, const int32_t* pd_ptr,
+                      const int32_t* pd_col, const double* pd_val,
+                      const int32_t* sidx, dcx_survey** out);
+int dcx_survey_destroy(dcx_survey* sv);
+/* d[i] = sum_k w[k] sum_j Pd[i,j] U[k][j][sidx_i - s0] for the data whose
+ * source lies in [s0, s1) (the others are left untouched); U device
+ * (nky x n x k), rows in elimination order (DCX_PERMUTED) or original order. */
+int dcx_survey_dpred(dcx_survey* sv, const double* U, int k, int nky,
+                     const double* w, int s0, int s1, double* d, int flags,
+                     void* stream);
+/* G[j][s - s0] = sum over data i of source s in [s0, s1) of Pd[i,j] v[i]:
+ * device (n x k), zero elsewhere; rows in elimination order (DCX_PERMUTED) or
+ * original order. */
+int dcx_survey_adjoint_rhs(dcx_survey* sv, const double* v, double* G, int k,
+                           int s0, int s1, int flags, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DCX_H_ */

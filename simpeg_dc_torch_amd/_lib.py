@@ -1,0 +1,164 @@
+"""ctypes binding of libdcx.so (the C-ABI declared in include/dcx.h).
+
+The library is loaded from this package directory only.  A missing or
+unloadable library raises ImportError with the reason the first time a GPU
+object is built; the CPU path of the package never needs it.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libdcx.so")
+
+DCX_FP64, DCX_FP32 = 0, 1
+DCX_SHARED_RHS, DCX_PERMUTED, DCX_SPARSE_RHS = 1, 2, 4
+
+_i32p = C.POINTER(C.c_int32)
+_f64p = C.POINTER(C.c_double)
+_vp = C.c_void_p
+
+
+class PlanInfo(C.Structure):
+    _fields_ = [("n", C.c_int), ("nnz", C.c_int), ("nbatch", C.c_int), ("nsup", C.c_int),
+                ("ndepth", C.c_int), ("max_front", C.c_int), ("max_ns", C.c_int),
+                ("precision", C.c_int), ("nnz_L", C.c_int64), ("factor_flops", C.c_double),
+                ("solve_flops_per_rhs", C.c_double), ("device_bytes", C.c_int64)]
+
+
+class SymbolicView(C.Structure):
+    _fields_ = [("n", C.c_int), ("nnz", C.c_int), ("nsup", C.c_int), ("ndepth", C.c_int),
+                ("max_front", C.c_int), ("max_ns", C.c_int)] + \
+        [(f, _i32p) for f in ("perm", "first", "ns", "rs", "parent", "depth", "rptr", "ridx",
+                              "cptr", "cidx", "relpos", "aptr", "a_e", "a_row", "a_col",
+                              "lvl_ptr", "lvl_sup")] + \
+        [("nnz_L", C.c_int64), ("factor_flops", C.c_double),
+         ("solve_flops_per_rhs", C.c_double)]
+
+
+# name -> (restype, argtypes); every entry point returns an int status
+PROTOTYPES = {
+    "dcx_last_error": (C.c_char_p, []),
+    "dcx_device_count": (C.c_int, [C.POINTER(C.c_int)]),
+    "dcx_symbolic_analyse": (C.c_int, [C.c_int, _i32p, _i32p, _f64p, C.c_int, C.c_int,
+                                       C.POINTER(_vp)]),
+    "dcx_symbolic_view_get": (C.c_int, [_vp, C.POINTER(SymbolicView)]),
+    "dcx_symbolic_destroy": (C.c_int, [_vp]),
+    "dcx_plan_create": (C.c_int, [C.c_int, _i32p, _i32p, C.c_int, _f64p, C.c_int, C.c_int,
+                                  C.c_int, C.POINTER(_vp)]),
+    "dcx_plan_destroy": (C.c_int, [_vp]),
+    "dcx_plan_get_info": (C.c_int, [_vp, C.POINTER(PlanInfo)]),
+    "dcx_plan_get_perm": (C.c_int, [_vp, _i32p]),
+    "dcx_plan_set_refinement": (C.c_int, [_vp, C.c_int]),
+    "dcx_factorize": (C.c_int, [_vp, _vp, _vp]),
+    "dcx_solve": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, _vp]),
+    "dcx_set_rhs_support": (C.c_int, [_vp, _i32p, C.c_int]),
+    "dcx_grad_values": (C.c_int, [_vp, _vp, _vp, C.c_int, _vp, _vp, C.c_int, _vp]),
+    "dcx_permute_rows": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, C.c_int, _vp]),
+    "dcx_lmap_create": (C.c_int, [C.c_int, C.c_int, C.c_int, _f64p, _i32p, _i32p, _f64p,
+                                  _f64p, C.c_int, _i32p, _i32p, _f64p, C.POINTER(_vp)]),
+    "dcx_lmap_destroy": (C.c_int, [_vp]),
+    "dcx_assemble": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "dcx_lmap_transpose": (C.c_int, [_vp, _vp, _vp, _vp, _vp]),
+    "dcx_survey_create": (C.c_int, [_vp, C.c_int, C.c_int, _i32p, _i32p, _f64p, _i32p,
+                                    C.POINTER(_vp)]),
+    "dcx_survey_destroy": (C.c_int, [_vp]),
+    "dcx_survey_dpred": (C.c_int, [_vp, _vp, C.c_int, C.c_int, _vp, C.c_int, C.c_int, _vp,
+                                   C.c_int, _vp]),
+    "dcx_survey_adjoint_rhs": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, C.c_int, C.c_int,
+                                         _vp]),
+}
+
+_lib = None
+_load_error = None
+
+
+def load():
+    """The loaded library; ImportError (with the reason) when it is absent."""
+    global _lib, _load_error
+    if _lib is not None:
+        return _lib
+    if _load_error is not None:
+        raise ImportError(_load_error)
+    if not os.path.exists(LIB_PATH):
+        _load_error = (f"{LIB_PATH} is not built: run __graft_entry__.build() "
+                       "(or python -m simpeg_dc_torch_amd._build)")
+        raise ImportError(_load_error)
+    try:
+        lib = C.CDLL(LIB_PATH)
+    except OSError as exc:
+        _load_error = f"cannot load {LIB_PATH}: {exc}"
+        raise ImportError(_load_error) from exc
+    for name, (res, args) in PROTOTYPES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(status, what=""):
+    if status != 0:
+        msg = _lib.dcx_last_error().decode(errors="replace") if _lib else "unknown"
+        raise RuntimeError(f"dcx {what} failed ({status}): {msg}")
+
+
+def i32(a):
+    a = np.ascontiguousarray(a, dtype=np.int32)
+    return a, a.ctypes.data_as(_i32p)
+
+
+def f64(a):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a, a.ctypes.data_as(_f64p)
+
+
+def ptr(t):
+    """Device address of a torch tensor (or None)."""
+    return None if t is None else C.c_void_p(t.data_ptr())
+
+
+def stream_of(device):
+    import torch
+    return C.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
+def symbolic(crow, col, coords=None, leaf_max=16):
+    """Host-only symbolic analysis: a dict of numpy arrays (copies)."""
+    lib = load()
+    crow_a, crow_p = i32(crow)
+    col_a, col_p = i32(col)
+    n = crow_a.size - 1
+    if coords is not None:
+        co, co_p = f64(coords)
+        dim = co.shape[1] if co.ndim == 2 else 1
+    else:
+        co_p, dim = None, 0
+    h = _vp()
+    check(lib.dcx_symbolic_analyse(n, crow_p, col_p, co_p, dim, int(leaf_max), C.byref(h)),
+          "symbolic analysis")
+    try:
+        v = SymbolicView()
+        check(lib.dcx_symbolic_view_get(h, C.byref(v)), "symbolic view")
+        ns = v.nsup
+
+        def arr(p, m):
+            return np.ctypeslib.as_array(p, shape=(m,)).copy() if m else np.zeros(0, np.int32)
+
+        rptr = arr(v.rptr, ns + 1)
+        cptr = arr(v.cptr, ns + 1)
+        aptr = arr(v.aptr, ns + 1)
+        lvl_ptr = arr(v.lvl_ptr, v.ndepth + 1)
+        out = dict(n=v.n, nnz=v.nnz, nsup=ns, ndepth=v.ndepth, max_front=v.max_front,
+                   max_ns=v.max_ns, nnz_L=v.nnz_L, factor_flops=v.factor_flops,
+                   solve_flops_per_rhs=v.solve_flops_per_rhs,
+                   perm=arr(v.perm, v.n), first=arr(v.first, ns), ns=arr(v.ns, ns),
+                   rs=arr(v.rs, ns), parent=arr(v.parent, ns), depth=arr(v.depth, ns),
+                   rptr=rptr, ridx=arr(v.ridx, rptr[-1]), cptr=cptr, cidx=arr(v.cidx, cptr[-1]),
+                   relpos=arr(v.relpos, rptr[-1]), aptr=aptr, a_e=arr(v.a_e, aptr[-1]),
+                   a_row=arr(v.a_row, aptr[-1]), a_col=arr(v.a_col, aptr[-1]),
+                   lvl_ptr=lvl_ptr, lvl_sup=arr(v.lvl_sup, lvl_ptr[-1]))
+        return out
+    finally:
+        lib.dcx_symbolic_destroy(h)

@@ -1,0 +1,1183 @@
+// dcx: supernodal multifrontal Cholesky, multi-RHS sweeps, gather-dot
+// backward, sigma->A assembly map and receiver projection for gfx950.
+//
+// Factor storage per supernode s (front F_s = J_s U R_s, nf = ns + rs):
+//   Q_s = [ L_JJ^{-1} ; L_RJ L_JJ^{-1} ]   (nf x ns, column-major)
+// so that one sweep step is one small GEMM:
+//   forward  (L y = b):   [y_J ; w] = Q_s f_J,  u_s = f_R - w  (passed up)
+//   backward (L^T x = y): x_J = Q_s^T [ y_J ; -x_R ]
+// Fronts and update rows are double-buffered by tree depth parity; each
+// depth level is one launch over (supernodes x batch [x column tiles]).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/dcx.h"
+#include "symbolic.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                          \
+  do {                                                                         \
+    hipError_t e_ = (expr);                                                    \
+    if (e_ != hipSuccess)                                                      \
+      return fail(DCX_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+struct SupDev {
+  int first, ns, rs, rptr;
+  int c_begin, c_end, a_begin, a_end;
+  int64_t panel_off, front_off, upd_off;
+};
+
+template <typename T>
+hipError_t upload(T** dst, const std::vector<T>& v) {
+  *dst = nullptr;
+  if (v.empty()) return hipSuccess;
+  hipError_t e = hipMalloc(dst, v.size() * sizeof(T));
+  if (e != hipSuccess) return e;
+  return hipMemcpy(*dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
+}
+
+constexpr int kThreads = 256;
+
+// ---------------------------------------------------------------------------
+// numeric factorisation, one launch per depth level (deepest first)
+// grid (supernodes at level, batch), 256 threads
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_factor_level(
+    const SupDev* __restrict__ sup, const int* __restrict__ lvl,
+    const int* __restrict__ relpos, const int* __restrict__ cidx,
+    const int* __restrict__ a_e, const int* __restrict__ a_row,
+    const int* __restrict__ a_col, const double* __restrict__ vals, int64_t nnz,
+    T* __restrict__ fcur, int64_t fcur_bs, const T* __restrict__ fprev,
+    int64_t fprev_bs, T* __restrict__ panels, int64_t panel_bs,
+    int* __restrict__ info) {
+  const int s = lvl[blockIdx.x];
+  const int b = blockIdx.y;
+  const SupDev S = sup[s];
+  const int ns = S.ns, nf = S.ns + S.rs;
+  const int tid = threadIdx.x;
+  T* F = fcur + b * fcur_bs + S.front_off;
+  const int64_t nf2 = static_cast<int64_t>(nf) * nf;
+  for (int64_t i = tid; i < nf2; i += kThreads) F[i] = T(0);
+  __syncthreads();
+  const double* v = vals + b * nnz;
+  for (int q = S.a_begin + tid; q < S.a_end; q += kThreads)
+    F[a_row[q] + static_cast<int64_t>(a_col[q]) * nf] = static_cast<T>(v[a_e[q]]);
+  __syncthreads();
+  // extend-add of the children's update (Schur complement) blocks
+  for (int cc = S.c_begin; cc < S.c_end; ++cc) {
+    const SupDev C = sup[cidx[cc]];
+    const int rc = C.rs, nfc = C.ns + C.rs;
+    const T* Fc = fprev + b * fprev_bs + C.front_off;
+    const int* rp = relpos + C.rptr;
+    const int64_t rc2 = static_cast<int64_t>(rc) * rc;
+    for (int64_t t = tid; t < rc2; t += kThreads) {
+      const int i = static_cast<int>(t % rc), j = static_cast<int>(t / rc);
+      if (i < j) continue;
+      F[rp[i] + static_cast<int64_t>(rp[j]) * nf] +=
+          Fc[(C.ns + i) + static_cast<int64_t>(C.ns + j) * nfc];
+    }
+    __syncthreads();
+  }
+  // right-looking partial Cholesky of the first ns columns; the trailing
+  // (R,R) block becomes this front's update block
+  for (int j = 0; j < ns; ++j) {
+    T d = F[j + static_cast<int64_t>(j) * nf];
+    if (!(d > T(0))) {
+      if (tid == 0) atomicMax(info, 1);
+      d = T(1);
+    }
+    d = sqrt(d);
+    __syncthreads();
+    T* colj = F + static_cast<int64_t>(j) * nf;
+    for (int i = j + tid; i < nf; i += kThreads) colj[i] = (i == j) ? d : colj[i] / d;
+    __syncthreads();
+    const int m = nf - j - 1;
+    // lower triangle of the trailing block, columns spread over warps
+    for (int l = tid / 64; l < m; l += kThreads / 64) {
+      const T lj = colj[j + 1 + l];
+      T* cl = F + static_cast<int64_t>(j + 1 + l) * nf;
+      for (int i = l + (tid & 63); i < m; i += 64) cl[j + 1 + i] -= colj[j + 1 + i] * lj;
+    }
+    __syncthreads();
+  }
+  // Q = [L_JJ^{-1}; L_RJ L_JJ^{-1}]
+  T* P = panels + b * panel_bs + S.panel_off;
+  for (int c = tid; c < ns; c += kThreads) {
+    T* pc = P + static_cast<int64_t>(c) * nf;
+    for (int i = 0; i < c; ++i) pc[i] = T(0);
+    pc[c] = T(1) / F[c + static_cast<int64_t>(c) * nf];
+    for (int i = c + 1; i < ns; ++i) {
+      T acc = T(0);
+      for (int k = c; k < i; ++k) acc += F[i + static_cast<int64_t>(k) * nf] * pc[k];
+      pc[i] = -acc / F[i + static_cast<int64_t>(i) * nf];
+    }
+  }
+  __syncthreads();
+  const int64_t rsn = static_cast<int64_t>(S.rs) * ns;
+  for (int64_t t = tid; t < rsn; t += kThreads) {
+    const int r = static_cast<int>(t % S.rs), c = static_cast<int>(t / S.rs);
+    const T* pc = P + static_cast<int64_t>(c) * nf;
+    T acc = T(0);
+    for (int k = c; k < ns; ++k) acc += F[(ns + r) + static_cast<int64_t>(k) * nf] * pc[k];
+    P[(ns + r) + static_cast<int64_t>(c) * nf] = acc;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// forward sweep, one launch per depth level (deepest first)
+// grid (supernodes at level, batch, column tiles of TC), 256 threads
+// ---------------------------------------------------------------------------
+template <typename T, int TC>
+__global__ __launch_bounds__(kThreads) void k_fwd_level(
+    const SupDev* __restrict__ sup, const int* __restrict__ lvl,
+    const int* __restrict__ relpos, const int* __restrict__ cidx,
+    const int* __restrict__ active, const T* __restrict__ panels,
+    int64_t panel_bs, T* __restrict__ W, int64_t w_bs, int k,
+    T* __restrict__ ucur, int64_t ucur_bs, const T* __restrict__ uprev,
+    int64_t uprev_bs) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* f = reinterpret_cast<T*>(smem_raw);
+  constexpr int NY = kThreads / TC;
+  const int s = lvl[blockIdx.x];
+  const int b = blockIdx.y;
+  const int tx = threadIdx.x % TC, ty = threadIdx.x / TC;
+  const int col = blockIdx.z * TC + tx;
+  const bool cok = col < k;
+  const SupDev S = sup[s];
+  const int ns = S.ns, nf = S.ns + S.rs;
+  T* Wb = W + b * w_bs;
+  for (int i = ty; i < nf; i += NY)
+    f[i * TC + tx] = (i < ns && cok) ? Wb[static_cast<int64_t>(S.first + i) * k + col] : T(0);
+  __syncthreads();
+  for (int cc = S.c_begin; cc < S.c_end; ++cc) {
+    const int c = cidx[cc];
+    if (active && !active[c]) continue;
+    const SupDev C = sup[c];
+    const T* u = uprev + b * uprev_bs + C.upd_off * k;
+    const int* rp = relpos + C.rptr;
+    for (int i = ty; i < C.rs; i += NY)
+      if (cok) f[rp[i] * TC + tx] += u[static_cast<int64_t>(i) * k + col];
+    __syncthreads();
+  }
+  const T* P = panels + b * panel_bs + S.panel_off;
+  T* uo = ucur + b * ucur_bs + S.upd_off * k;
+  for (int r = ty; r < nf; r += NY) {
+    T acc = T(0);
+    for (int kk = 0; kk < ns; ++kk) acc += P[r + static_cast<int64_t>(kk) * nf] * f[kk * TC + tx];
+    if (cok) {
+      if (r < ns)
+        Wb[static_cast<int64_t>(S.first + r) * k + col] = acc;
+      else
+        uo[static_cast<int64_t>(r - ns) * k + col] = f[r * TC + tx] - acc;
+    }
+  }
+}
+
+// backward sweep, one launch per depth level (root first)
+template <typename T, int TC>
+__global__ __launch_bounds__(kThreads) void k_bwd_level(
+    const SupDev* __restrict__ sup, const int* __restrict__ lvl,
+    const int* __restrict__ ridx, const int* __restrict__ active,
+    const T* __restrict__ panels, int64_t panel_bs, T* __restrict__ W,
+    int64_t w_bs, int k) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* g = reinterpret_cast<T*>(smem_raw);
+  constexpr int NY = kThreads / TC;
+  const int s = lvl[blockIdx.x];
+  const int b = blockIdx.y;
+  const int tx = threadIdx.x % TC, ty = threadIdx.x / TC;
+  const int col = blockIdx.z * TC + tx;
+  const bool cok = col < k;
+  const SupDev S = sup[s];
+  const int ns = S.ns, nf = S.ns + S.rs;
+  const bool act = !active || active[s];
+  T* Wb = W + b * w_bs;
+  for (int i = ty; i < nf; i += NY) {
+    T val = T(0);
+    if (cok) {
+      if (i < ns)
+        val = act ? Wb[static_cast<int64_t>(S.first + i) * k + col] : T(0);
+      else
+        val = -Wb[static_cast<int64_t>(ridx[S.rptr + i - ns]) * k + col];
+    }
+    g[i * TC + tx] = val;
+  }
+  __syncthreads();
+  const T* P = panels + b * panel_bs + S.panel_off;
+  for (int i = ty; i < ns; i += NY) {
+    const T* pc = P + static_cast<int64_t>(i) * nf;
+    T acc = T(0);
+    for (int kk = 0; kk < nf; ++kk) acc += pc[kk] * g[kk * TC + tx];
+    if (cok) Wb[static_cast<int64_t>(S.first + i) * k + col] = acc;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// elementwise / gather helpers (one wavefront per row, lanes over columns)
+// ---------------------------------------------------------------------------
+template <typename TI, typename TO>
+__global__ void k_permute(const TI* __restrict__ in, TO* __restrict__ out,
+                          const int* __restrict__ perm, int n, int k,
+                          int64_t in_bs, int64_t out_bs, int to_elim) {
+  const int row = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+  const int b = blockIdx.y;
+  if (row >= n) return;
+  const int src = to_elim ? perm[row] : row;
+  const int dst = to_elim ? row : perm[row];
+  const TI* pi = in + b * in_bs + static_cast<int64_t>(src) * k;
+  TO* po = out + b * out_bs + static_cast<int64_t>(dst) * k;
+  for (int c = threadIdx.x & 63; c < k; c += 64) po[c] = static_cast<TO>(pi[c]);
+}
+
+template <typename TI, typename TO>
+__global__ void k_convert(const TI* __restrict__ in, TO* __restrict__ out, int64_t n) {
+  for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n;
+       i += static_cast<int64_t>(gridDim.x) * blockDim.x)
+    out[i] = static_cast<TO>(in[i]);
+}
+
+__global__ void k_axpy_f32(const float* __restrict__ c, double* __restrict__ x, int64_t n) {
+  for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n;
+       i += static_cast<int64_t>(gridDim.x) * blockDim.x)
+    x[i] += static_cast<double>(c[i]);
+}
+
+__global__ void k_zero_int(int* p) {
+  if (threadIdx.x == 0) p[0] = 0;
+}
+
+__global__ void k_fill(double* __restrict__ x, int64_t n, double v) {
+  for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n;
+       i += static_cast<int64_t>(gridDim.x) * blockDim.x)
+    x[i] = v;
+}
+
+// r[b] = B[b or shared] - A_b x[b]  (FP64, elimination order)
+__global__ void k_residual(const int* __restrict__ crow, const int* __restrict__ ccol,
+                           const int* __restrict__ vidx, const double* __restrict__ vals,
+                           int64_t nnz, const double* __restrict__ B, int64_t b_bs,
+                           const double* __restrict__ X, double* __restrict__ R,
+                           int n, int k) {
+  const int row = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+  const int b = blockIdx.y;
+  if (row >= n) return;
+  const int64_t xb = static_cast<int64_t>(b) * n * k;
+  const double* vb = vals + b * nnz;
+  const int p0 = crow[row], p1 = crow[row + 1];
+  for (int c = threadIdx.x & 63; c < k; c += 64) {
+    double acc = B[b * b_bs + static_cast<int64_t>(row) * k + c];
+    for (int p = p0; p < p1; ++p) acc -= vb[vidx[p]] * X[xb + static_cast<int64_t>(ccol[p]) * k + c];
+    R[xb + static_cast<int64_t>(row) * k + c] = acc;
+  }
+}
+
+// gv[b][e] = -scale[b] sum_c Y[b][row][c] X[b][col_e][c]; one wavefront per row
+__global__ void k_grad_values(const int* __restrict__ crow, const int* __restrict__ ccol,
+                              const int* __restrict__ eidx, const double* __restrict__ Y,
+                              const double* __restrict__ X, int n, int k,
+                              const double* __restrict__ scale, double* __restrict__ gv,
+                              int64_t nnz) {
+  constexpr int MAXE = 8;
+  const int row = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+  const int b = blockIdx.y;
+  const int lane = threadIdx.x & 63;
+  if (row >= n) return;
+  const int64_t xb = static_cast<int64_t>(b) * n * k;
+  const double* y = Y + xb + static_cast<int64_t>(row) * k;
+  const double sc = scale ? scale[b] : 1.0;
+  for (int p0 = crow[row]; p0 < crow[row + 1]; p0 += MAXE) {
+    const int ne = min(MAXE, crow[row + 1] - p0);
+    double acc[MAXE];
+#pragma unroll
+    for (int q = 0; q < MAXE; ++q) acc[q] = 0.0;
+    for (int c = lane; c < k; c += 64) {
+      const double yv = y[c];
+#pragma unroll
+      for (int q = 0; q < MAXE; ++q)
+        if (q < ne) acc[q] += yv * X[xb + static_cast<int64_t>(ccol[p0 + q]) * k + c];
+    }
+#pragma unroll
+    for (int q = 0; q < MAXE; ++q) {
+      double a = acc[q];
+      for (int off = 32; off > 0; off >>= 1) a += __shfl_xor(a, off, 64);
+      if (lane == 0 && q < ne) gv[b * nnz + (eidx ? eidx[p0 + q] : p0 + q)] = -sc * a;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// assembly map
+// ---------------------------------------------------------------------------
+__global__ void k_assemble(int nnz, int nky, const double* __restrict__ ky2,
+                           const int* __restrict__ ptr, const int* __restrict__ cell,
+                           const double* __restrict__ le, const double* __restrict__ ln,
+                           const int* __restrict__ bptr, const int* __restrict__ bcell,
+                           const double* __restrict__ bcoef, int nbc,
+                           const double* __restrict__ sigma, double* __restrict__ vals) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nnz) return;
+  double ae = 0.0, an = 0.0;
+  for (int p = ptr[e]; p < ptr[e + 1]; ++p) {
+    const double s = sigma[cell[p]];
+    ae += le[p] * s;
+    an += ln[p] * s;
+  }
+  const int q0 = bptr[e], q1 = bptr[e + 1];
+  for (int k = 0; k < nky; ++k) {
+    double v = ae + ky2[k] * an;
+    for (int q = q0; q < q1; ++q) v += bcoef[static_cast<int64_t>(k) * nbc + q] * sigma[bcell[q]];
+    vals[static_cast<int64_t>(k) * nnz + e] = v;
+  }
+}
+
+// per entry: ge = sum_k gv[k][e], gn = sum_k ky_k^2 gv[k][e]
+__global__ void k_reduce_ky(int nnz, int nky, const double* __restrict__ ky2,
+                            const double* __restrict__ gv, double* __restrict__ ge,
+                            double* __restrict__ gn) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nnz) return;
+  double a = 0.0, c = 0.0;
+  for (int k = 0; k < nky; ++k) {
+    const double g = gv[static_cast<int64_t>(k) * nnz + e];
+    a += g;
+    c += ky2[k] * g;
+  }
+  ge[e] = a;
+  gn[e] = c;
+}
+
+// grad_m[c] = -sigma[c] * sum over the cell's map entries (transposed CSR)
+__global__ void k_lmap_t(int ncell, int nnz, int nky, const int* __restrict__ tptr,
+                         const int* __restrict__ tent, const int* __restrict__ te,
+                         const double* __restrict__ le, const double* __restrict__ ln,
+                         const int* __restrict__ tbptr, const int* __restrict__ tbq,
+                         const int* __restrict__ bc_e, const double* __restrict__ bcoef,
+                         int nbc, const double* __restrict__ ge, const double* __restrict__ gn,
+                         const double* __restrict__ gv, const double* __restrict__ sigma,
+                         double* __restrict__ grad_m) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncell) return;
+  double acc = 0.0;
+  for (int t = tptr[c]; t < tptr[c + 1]; ++t) {
+    const int p = tent[t], e = te[t];
+    acc += le[p] * ge[e] + ln[p] * gn[e];
+  }
+  for (int t = tbptr[c]; t < tbptr[c + 1]; ++t) {
+    const int q = tbq[t];
+    const int e = bc_e[q];
+    for (int k = 0; k < nky; ++k)
+      acc += bcoef[static_cast<int64_t>(k) * nbc + q] * gv[static_cast<int64_t>(k) * nnz + e];
+  }
+  grad_m[c] = -sigma[c] * acc;
+}
+
+// ---------------------------------------------------------------------------
+// survey: projection and adjoint sources
+// ---------------------------------------------------------------------------
+__global__ void k_dpred(int nD, const int* __restrict__ ptr, const int* __restrict__ rows,
+                        const double* __restrict__ val, const int* __restrict__ sidx,
+                        const double* __restrict__ U, int64_t bs, int k, int nky,
+                        const double* __restrict__ w, int s0, int s1, double* __restrict__ d) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nD) return;
+  const int s = sidx[i] - s0;
+  if (sidx[i] < s0 || sidx[i] >= s1) return;
+  double acc = 0.0;
+  for (int kk = 0; kk < nky; ++kk) {
+    double a = 0.0;
+    for (int p = ptr[i]; p < ptr[i + 1]; ++p)
+      a += val[p] * U[kk * bs + static_cast<int64_t>(rows[p]) * k + s];
+    acc += w[kk] * a;
+  }
+  d[i] = acc;
+}
+
+// G[row][src - s0] = sum coef * v[datum] for each (row, source) target
+__global__ void k_adjoint_rhs(int ntgt, const int* __restrict__ tptr,
+                              const int* __restrict__ trow, const int* __restrict__ tsrc,
+                              const int* __restrict__ tdat, const double* __restrict__ tcoef,
+                              const double* __restrict__ v, int k, int s0, int s1,
+                              double* __restrict__ G) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ntgt) return;
+  const int src = tsrc[t];
+  if (src < s0 || src >= s1) return;
+  double acc = 0.0;
+  for (int q = tptr[t]; q < tptr[t + 1]; ++q) acc += tcoef[q] * v[tdat[q]];
+  G[static_cast<int64_t>(trow[t]) * k + (src - s0)] = acc;
+}
+
+inline int grid1(int64_t n, int bs) { return static_cast<int>((n + bs - 1) / bs); }
+inline int grid_stride(int64_t n) {
+  return static_cast<int>(std::min<int64_t>((n + 255) / 256, 4096));
+}
+
+}  // namespace
+
+// ===========================================================================
+// plan
+// ===========================================================================
+struct dcx_plan {
+  dcx::Symbolic S;
+  int device = 0;
+  int nbatch = 1;
+  int precision = DCX_FP64;
+  int refine = 1;
+  size_t tsize = 8;
+  // device maps
+  SupDev* d_sup = nullptr;
+  int *d_lvl = nullptr, *d_relpos = nullptr, *d_cidx = nullptr, *d_ridx = nullptr;
+  int *d_a_e = nullptr, *d_a_row = nullptr, *d_a_col = nullptr;
+  int *d_perm = nullptr, *d_pcrow = nullptr, *d_pcol = nullptr, *d_pvidx = nullptr;
+  int *d_crow = nullptr, *d_col = nullptr;
+  int* d_info = nullptr;
+  // factor
+  void* d_panels = nullptr;
+  void* d_front[2] = {nullptr, nullptr};
+  double* d_vals = nullptr;
+  bool factored = false;
+  // sparse right-hand sides
+  int* d_active = nullptr;
+  std::vector<int> act_lvl_ptr;
+  int* d_act_lvl = nullptr;
+  // workspace (grown on demand)
+  void* d_upd[2] = {nullptr, nullptr};
+  int upd_k = 0;
+  void* d_w = nullptr;       // working block in factor precision
+  size_t w_bytes = 0;
+  double* d_r = nullptr;     // FP64 residual (mixed)
+  double* d_b = nullptr;     // permuted FP64 rhs
+  double* d_x = nullptr;     // permuted FP64 iterate (mixed)
+  size_t r_bytes = 0, b_bytes = 0, x_bytes = 0;
+  int64_t device_bytes = 0;
+
+  ~dcx_plan() {
+    void* ptrs[] = {d_sup, d_lvl, d_relpos, d_cidx, d_ridx, d_a_e, d_a_row, d_a_col,
+                    d_perm, d_pcrow, d_pcol, d_pvidx, d_crow, d_col, d_info, d_panels,
+                    d_front[0], d_front[1], d_vals, d_active, d_act_lvl, d_upd[0],
+                    d_upd[1], d_w, d_r, d_b, d_x};
+    for (void* p : ptrs)
+      if (p) (void)hipFree(p);
+  }
+};
+
+namespace {
+
+int ensure(void** p, size_t* cap, size_t bytes, hipStream_t st) {
+  if (*cap >= bytes && *p) return DCX_OK;
+  HIP_TRY(hipStreamSynchronize(st));
+  if (*p) HIP_TRY(hipFree(*p));
+  *p = nullptr;
+  *cap = 0;
+  HIP_TRY(hipMalloc(p, std::max<size_t>(bytes, 256)));
+  *cap = bytes;
+  return DCX_OK;
+}
+
+template <typename T>
+int launch_factor(dcx_plan* P, hipStream_t st) {
+  const auto& S = P->S;
+  T* panels = static_cast<T*>(P->d_panels);
+  T* fr[2] = {static_cast<T*>(P->d_front[0]), static_cast<T*>(P->d_front[1])};
+  const int64_t fbs[2] = {S.front_size[0], S.front_size[1]};
+  for (int d = S.ndepth - 1; d >= 0; --d) {
+    const int cnt = S.lvl_ptr[d + 1] - S.lvl_ptr[d];
+    if (!cnt) continue;
+    dim3 grid(cnt, P->nbatch);
+    hipLaunchKernelGGL(k_factor_level<T>, grid, dim3(kThreads), 0, st, P->d_sup,
+                       P->d_lvl + S.lvl_ptr[d], P->d_relpos, P->d_cidx, P->d_a_e,
+                       P->d_a_row, P->d_a_col, P->d_vals, static_cast<int64_t>(S.nnz),
+                       fr[d & 1], fbs[d & 1], fr[(d + 1) & 1], fbs[(d + 1) & 1], panels,
+                       S.panel_size, P->d_info);
+  }
+  HIP_TRY(hipGetLastError());
+  return DCX_OK;
+}
+
+template <typename T, int TC>
+void sweep_tc(dcx_plan* P, T* W, int k, int maxnf, bool fwd, int d, const int* lvl,
+              int cnt, const int* active, hipStream_t st) {
+  const auto& S = P->S;
+  dim3 grid(cnt, P->nbatch, (k + TC - 1) / TC);
+  const size_t lds = static_cast<size_t>(maxnf) * TC * sizeof(T);
+  T* panels = static_cast<T*>(P->d_panels);
+  const int64_t w_bs = static_cast<int64_t>(S.n) * k;
+  if (fwd) {
+    T* u[2] = {static_cast<T*>(P->d_upd[0]), static_cast<T*>(P->d_upd[1])};
+    const int64_t ubs[2] = {S.upd_rows[0] * k, S.upd_rows[1] * k};
+    hipLaunchKernelGGL((k_fwd_level<T, TC>), grid, dim3(kThreads), lds, st, P->d_sup, lvl,
+                       P->d_relpos, P->d_cidx, active, panels, S.panel_size, W, w_bs, k,
+                       u[d & 1], ubs[d & 1], u[(d + 1) & 1], ubs[(d + 1) & 1]);
+  } else {
+    hipLaunchKernelGGL((k_bwd_level<T, TC>), grid, dim3(kThreads), lds, st, P->d_sup, lvl,
+                       P->d_ridx, active, panels, S.panel_size, W, w_bs, k);
+  }
+}
+
+int pick_tc(int maxnf, size_t tsize) {
+  const size_t budget = 64 * 1024;
+  for (int tc : {64, 32, 16, 8})
+    if (static_cast<size_t>(maxnf) * tc * tsize <= budget) return tc;
+  return 0;
+}
+
+// full solve on the working block W (elimination order, factor precision)
+template <typename T>
+int sweeps(dcx_plan* P, T* W, int k, bool sparse_rhs, hipStream_t st) {
+  const auto& S = P->S;
+  if (P->upd_k < k) {
+    const size_t ub0 = static_cast<size_t>(P->nbatch) * S.upd_rows[0] * k * sizeof(T);
+    const size_t ub1 = static_cast<size_t>(P->nbatch) * S.upd_rows[1] * k * sizeof(T);
+    HIP_TRY(hipStreamSynchronize(st));
+    if (P->d_upd[0]) HIP_TRY(hipFree(P->d_upd[0]));
+    if (P->d_upd[1]) HIP_TRY(hipFree(P->d_upd[1]));
+    P->d_upd[0] = P->d_upd[1] = nullptr;
+    HIP_TRY(hipMalloc(&P->d_upd[0], std::max<size_t>(ub0, 256)));
+    HIP_TRY(hipMalloc(&P->d_upd[1], std::max<size_t>(ub1, 256)));
+    P->upd_k = k;
+  }
+  const bool use_act = sparse_rhs && P->d_active;
+  for (int pass = 0; pass < 2; ++pass) {
+    const bool fwd = pass == 0;
+    for (int dd = 0; dd < S.ndepth; ++dd) {
+      const int d = fwd ? S.ndepth - 1 - dd : dd;
+      const int* lvl;
+      int cnt;
+      if (fwd && use_act) {
+        lvl = P->d_act_lvl + P->act_lvl_ptr[d];
+        cnt = P->act_lvl_ptr[d + 1] - P->act_lvl_ptr[d];
+      } else {
+        lvl = P->d_lvl + S.lvl_ptr[d];
+        cnt = S.lvl_ptr[d + 1] - S.lvl_ptr[d];
+      }
+      if (!cnt) continue;
+      int maxnf = 0;
+      for (int q = S.lvl_ptr[d]; q < S.lvl_ptr[d + 1]; ++q) {
+        int s = S.lvl_sup[q];
+        maxnf = std::max(maxnf, S.ns[s] + S.rs[s]);
+      }
+      const int tc = pick_tc(maxnf, sizeof(T));
+      if (!tc) return fail(DCX_ERR_ARG, "front too large for the in-LDS sweep");
+      const int* act = use_act ? P->d_active : nullptr;
+      switch (tc) {
+        case 64: sweep_tc<T, 64>(P, W, k, maxnf, fwd, d, lvl, cnt, act, st); break;
+        case 32: sweep_tc<T, 32>(P, W, k, maxnf, fwd, d, lvl, cnt, act, st); break;
+        case 16: sweep_tc<T, 16>(P, W, k, maxnf, fwd, d, lvl, cnt, act, st); break;
+        default: sweep_tc<T, 8>(P, W, k, maxnf, fwd, d, lvl, cnt, act, st); break;
+      }
+    }
+  }
+  HIP_TRY(hipGetLastError());
+  return DCX_OK;
+}
+
+void launch_permute_d2T(dcx_plan* P, const double* in, void* out, int k, int nblocks,
+                        int64_t in_bs, int64_t out_bs, bool f32, hipStream_t st) {
+  const int n = P->S.n;
+  dim3 grid(grid1(n, 4), nblocks);
+  if (f32)
+    hipLaunchKernelGGL((k_permute<double, float>), grid, dim3(256), 0, st, in,
+                       static_cast<float*>(out), P->d_perm, n, k, in_bs, out_bs, 1);
+  else
+    hipLaunchKernelGGL((k_permute<double, double>), grid, dim3(256), 0, st, in,
+                       static_cast<double*>(out), P->d_perm, n, k, in_bs, out_bs, 1);
+}
+
+}  // namespace
+
+// ===========================================================================
+// C-ABI
+// ===========================================================================
+extern "C" {
+
+const char* dcx_last_error(void) { return g_err.c_str(); }
+
+int dcx_device_count(int* out) {
+  if (!out) return fail(DCX_ERR_ARG, "null output");
+  HIP_TRY(hipGetDeviceCount(out));
+  return DCX_OK;
+}
+
+int dcx_plan_create(int n, const int32_t* crow, const int32_t* col, int nbatch,
+                    const double* coords, int dim, int leaf_max, int precision,
+                    dcx_plan** out) {
+  try {
+    if (!out || !crow || !col) return fail(DCX_ERR_ARG, "null argument");
+    *out = nullptr;
+    if (nbatch < 1) return fail(DCX_ERR_ARG, "nbatch must be >= 1");
+    if (precision != DCX_FP64 && precision != DCX_FP32)
+      return fail(DCX_ERR_ARG, "precision must be DCX_FP64 or DCX_FP32");
+    if (coords && (dim < 1 || dim > 3)) return fail(DCX_ERR_ARG, "dim must be 1..3");
+    std::unique_ptr<dcx_plan> P(new dcx_plan());
+    std::string err = dcx::analyse(n, crow, col, coords, dim, leaf_max, P->S);
+    if (!err.empty()) return fail(DCX_ERR_ARG, err);
+    auto& S = P->S;
+    HIP_TRY(hipGetDevice(&P->device));
+    P->nbatch = nbatch;
+    P->precision = precision;
+    P->tsize = precision == DCX_FP32 ? 4 : 8;
+    std::vector<SupDev> sd(S.nsup);
+    for (int s = 0; s < S.nsup; ++s) {
+      sd[s] = SupDev{S.first[s], S.ns[s], S.rs[s], S.rptr[s], S.cptr[s], S.cptr[s + 1],
+                     S.aptr[s], S.aptr[s + 1], S.panel_off[s], S.front_off[s], S.upd_off[s]};
+    }
+    HIP_TRY(upload(&P->d_sup, sd));
+    HIP_TRY(upload(&P->d_lvl, S.lvl_sup));
+    HIP_TRY(upload(&P->d_relpos, S.relpos));
+    HIP_TRY(upload(&P->d_cidx, S.cidx));
+    HIP_TRY(upload(&P->d_ridx, S.ridx));
+    HIP_TRY(upload(&P->d_a_e, S.a_e));
+    HIP_TRY(upload(&P->d_a_row, S.a_row));
+    HIP_TRY(upload(&P->d_a_col, S.a_col));
+    HIP_TRY(upload(&P->d_perm, S.perm));
+    HIP_TRY(upload(&P->d_pcrow, S.pcrow));
+    HIP_TRY(upload(&P->d_pcol, S.pcol));
+    HIP_TRY(upload(&P->d_pvidx, S.pval_idx));
+    std::vector<int> vcrow(crow, crow + n + 1), vcol(col, col + crow[n]);
+    HIP_TRY(upload(&P->d_crow, vcrow));
+    HIP_TRY(upload(&P->d_col, vcol));
+    HIP_TRY(hipMalloc(&P->d_info, sizeof(int)));
+    const size_t pb = static_cast<size_t>(nbatch) * S.panel_size * P->tsize;
+    const size_t f0 = static_cast<size_t>(nbatch) * S.front_size[0] * P->tsize;
+    const size_t f1 = static_cast<size_t>(nbatch) * S.front_size[1] * P->tsize;
+    const size_t vb = static_cast<size_t>(nbatch) * S.nnz * sizeof(double);
+    HIP_TRY(hipMalloc(&P->d_panels, std::max<size_t>(pb, 256)));
+    HIP_TRY(hipMalloc(&P->d_front[0], std::max<size_t>(f0, 256)));
+    HIP_TRY(hipMalloc(&P->d_front[1], std::max<size_t>(f1, 256)));
+    HIP_TRY(hipMalloc(&P->d_vals, std::max<size_t>(vb, 256)));
+    P->device_bytes = static_cast<int64_t>(pb + f0 + f1 + vb);
+    *out = P.release();
+    return DCX_OK;
+  } catch (const std::exception& e) {
+    return fail(DCX_ERR_INTERNAL, e.what());
+  }
+}
+
+int dcx_plan_destroy(dcx_plan* plan) {
+  delete plan;
+  return DCX_OK;
+}
+
+int dcx_plan_get_info(const dcx_plan* P, dcx_plan_info* o) {
+  if (!P || !o) return fail(DCX_ERR_ARG, "null argument");
+  const auto& S = P->S;
+  o->n = S.n;
+  o->nnz = S.nnz;
+  o->nbatch = P->nbatch;
+  o->nsup = S.nsup;
+  o->ndepth = S.ndepth;
+  o->max_front = S.max_front;
+  o->max_ns = S.max_ns;
+  o->precision = P->precision;
+  o->nnz_L = S.nnz_L;
+  o->factor_flops = S.factor_flops;
+  o->solve_flops_per_rhs = S.solve_flops_per_rhs;
+  o->device_bytes = P->device_bytes;
+  return DCX_OK;
+}
+
+int dcx_plan_get_perm(const dcx_plan* P, int32_t* perm) {
+  if (!P || !perm) return fail(DCX_ERR_ARG, "null argument");
+  std::memcpy(perm, P->S.perm.data(), sizeof(int32_t) * P->S.n);
+  return DCX_OK;
+}
+
+int dcx_plan_set_refinement(dcx_plan* P, int passes) {
+  if (!P || passes < 0 || passes > 16) return fail(DCX_ERR_ARG, "passes must be 0..16");
+  P->refine = passes;
+  return DCX_OK;
+}
+
+int dcx_set_rhs_support(dcx_plan* P, const int32_t* rows, int nrows) {
+  if (!P || (nrows > 0 && !rows)) return fail(DCX_ERR_ARG, "null argument");
+  const auto& S = P->S;
+  std::vector<int> act(S.nsup, 0);
+  for (int i = 0; i < nrows; ++i) {
+    if (rows[i] < 0 || rows[i] >= S.n) return fail(DCX_ERR_ARG, "support row out of range");
+    for (int s = S.snode_of[S.iperm[rows[i]]]; s >= 0 && !act[s]; s = S.parent[s]) act[s] = 1;
+  }
+  std::vector<int> ptr(S.ndepth + 1, 0), lst;
+  for (int d = 0; d < S.ndepth; ++d) {
+    for (int q = S.lvl_ptr[d]; q < S.lvl_ptr[d + 1]; ++q)
+      if (act[S.lvl_sup[q]]) lst.push_back(S.lvl_sup[q]);
+    ptr[d + 1] = static_cast<int>(lst.size());
+  }
+  if (P->d_active) HIP_TRY(hipFree(P->d_active));
+  if (P->d_act_lvl) HIP_TRY(hipFree(P->d_act_lvl));
+  P->d_active = P->d_act_lvl = nullptr;
+  HIP_TRY(upload(&P->d_active, act));
+  if (lst.empty()) lst.push_back(0);
+  HIP_TRY(upload(&P->d_act_lvl, lst));
+  P->act_lvl_ptr = ptr;
+  return DCX_OK;
+}
+
+int dcx_factorize(dcx_plan* P, const double* values, void* stream) {
+  if (!P || !values) return fail(DCX_ERR_ARG, "null argument");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  HIP_TRY(hipSetDevice(P->device));
+  const size_t vb = static_cast<size_t>(P->nbatch) * P->S.nnz * sizeof(double);
+  HIP_TRY(hipMemcpyAsync(P->d_vals, values, vb, hipMemcpyDeviceToDevice, st));
+  hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, st, P->d_info);
+  int rc = P->precision == DCX_FP32 ? launch_factor<float>(P, st) : launch_factor<double>(P, st);
+  if (rc) return rc;
+  P->factored = true;
+  return DCX_OK;
+}
+
+int dcx_solve(dcx_plan* P, const double* b, double* x, int k, int flags, void* stream) {
+  if (!P || !b || !x) return fail(DCX_ERR_ARG, "null argument");
+  if (!P->factored) return fail(DCX_ERR_STATE, "dcx_solve before dcx_factorize");
+  if (k < 1) return fail(DCX_ERR_ARG, "k must be >= 1");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  HIP_TRY(hipSetDevice(P->device));
+  const auto& S = P->S;
+  const int nb = P->nbatch;
+  const bool shared = flags & DCX_SHARED_RHS;
+  const bool permuted = flags & DCX_PERMUTED;
+  const bool sparse = (flags & DCX_SPARSE_RHS) != 0;
+  const int64_t blk = static_cast<int64_t>(S.n) * k;
+  const int64_t b_bs = shared ? 0 : blk;
+  const int n = S.n;
+  dim3 rgrid(grid1(n, 4), nb);
+  if (P->precision == DCX_FP64) {
+    // W = x (permuted) or plan workspace
+    double* W = x;
+    if (!permuted) {
+      int rc = ensure(&P->d_w, &P->w_bytes, static_cast<size_t>(nb) * blk * 8, st);
+      if (rc) return rc;
+      W = static_cast<double*>(P->d_w);
+      launch_permute_d2T(P, b, W, k, nb, b_bs, blk, false, st);
+    } else if (shared || b != x) {
+      for (int i = 0; i < nb; ++i)
+        HIP_TRY(hipMemcpyAsync(W + i * blk, b + i * b_bs, blk * 8, hipMemcpyDeviceToDevice, st));
+    }
+    int rc = sweeps<double>(P, W, k, sparse, st);
+    if (rc) return rc;
+    if (!permuted)
+      hipLaunchKernelGGL((k_permute<double, double>), rgrid, dim3(256), 0, st, W, x,
+                         P->d_perm, n, k, blk, blk, 0);
+    HIP_TRY(hipGetLastError());
+    return DCX_OK;
+  }
+  // mixed: FP32 sweeps, FP64 refinement
+  int rc = ensure(&P->d_w, &P->w_bytes, static_cast<size_t>(nb) * blk * 4, st);
+  if (rc) return rc;
+  rc = ensure(reinterpret_cast<void**>(&P->d_b), &P->b_bytes,
+              static_cast<size_t>(shared ? 1 : nb) * blk * 8, st);
+  if (rc) return rc;
+  rc = ensure(reinterpret_cast<void**>(&P->d_r), &P->r_bytes, static_cast<size_t>(nb) * blk * 8, st);
+  if (rc) return rc;
+  float* W = static_cast<float*>(P->d_w);
+  double* Bp = P->d_b;
+  double* X = x;
+  if (!permuted) {
+    rc = ensure(reinterpret_cast<void**>(&P->d_x), &P->x_bytes, static_cast<size_t>(nb) * blk * 8, st);
+    if (rc) return rc;
+    X = P->d_x;
+    launch_permute_d2T(P, b, Bp, k, shared ? 1 : nb, b_bs, blk, false, st);
+  } else {
+    HIP_TRY(hipMemcpyAsync(Bp, b, (shared ? 1 : nb) * blk * 8, hipMemcpyDeviceToDevice, st));
+  }
+  const int64_t tot = nb * blk;
+  // first pass: W = (float) B
+  for (int i = 0; i < nb; ++i)
+    hipLaunchKernelGGL((k_convert<double, float>), dim3(grid_stride(blk)), dim3(256), 0, st,
+                       Bp + i * b_bs, W + i * blk, blk);
+  rc = sweeps<float>(P, W, k, sparse, st);
+  if (rc) return rc;
+  hipLaunchKernelGGL((k_convert<float, double>), dim3(grid_stride(tot)), dim3(256), 0, st, W, X, tot);
+  for (int it = 0; it < P->refine; ++it) {
+    hipLaunchKernelGGL(k_residual, rgrid, dim3(256), 0, st, P->d_pcrow, P->d_pcol, P->d_pvidx,
+                       P->d_vals, static_cast<int64_t>(S.nnz), Bp, b_bs, X, P->d_r, n, k);
+    hipLaunchKernelGGL((k_convert<double, float>), dim3(grid_stride(tot)), dim3(256), 0, st,
+                       P->d_r, W, tot);
+    rc = sweeps<float>(P, W, k, false, st);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_axpy_f32, dim3(grid_stride(tot)), dim3(256), 0, st, W, X, tot);
+  }
+  if (!permuted)
+    hipLaunchKernelGGL((k_permute<double, double>), rgrid, dim3(256), 0, st, X, x, P->d_perm,
+                       n, k, blk, blk, 0);
+  HIP_TRY(hipGetLastError());
+  return DCX_OK;
+}
+
+int dcx_grad_values(dcx_plan* P, const double* y, const double* x, int k,
+                    const double* scale, double* gv, int flags, void* stream) {
+  if (!P || !y || !x || !gv) return fail(DCX_ERR_ARG, "null argument");
+  if (k < 1) return fail(DCX_ERR_ARG, "k must be >= 1");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  HIP_TRY(hipSetDevice(P->device));
+  const auto& S = P->S;
+  dim3 grid(grid1(S.n, 4), P->nbatch);
+  if (flags & DCX_PERMUTED)
+    hipLaunchKernelGGL(k_grad_values, grid, dim3(256), 0, st, P->d_pcrow, P->d_pcol,
+                       P->d_pvidx, y, x, S.n, k, scale, gv, static_cast<int64_t>(S.nnz));
+  else
+    hipLaunchKernelGGL(k_grad_values, grid, dim3(256), 0, st, P->d_crow, P->d_col,
+                       static_cast<const int*>(nullptr), y, x, S.n, k, scale, gv,
+                       static_cast<int64_t>(S.nnz));
+  HIP_TRY(hipGetLastError());
+  return DCX_OK;
+}
+
+int dcx_permute_rows(dcx_plan* P, const double* in, double* out, int k, int nblocks,
+                     int to_elim, void* stream) {
+  if (!P || !in || !out || in == out) return fail(DCX_ERR_ARG, "bad arguments");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  HIP_TRY(hipSetDevice(P->device));
+  const int64_t blk = static_cast<int64_t>(P->S.n) * k;
+  dim3 grid(grid1(P->S.n, 4), nblocks);
+  hipLaunchKernelGGL((k_permute<double, double>), grid, dim3(256), 0, st, in, out, P->d_perm,
+                     P->S.n, k, blk, blk, to_elim);
+  HIP_TRY(hipGetLastError());
+  return DCX_OK;
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// assembly map
+// ===========================================================================
+struct dcx_lmap {
+  int nnz = 0, ncell = 0, nky = 0, nbc = 0, device = 0;
+  double* d_ky2 = nullptr;
+  int *d_ptr = nullptr, *d_cell = nullptr;
+  double *d_le = nullptr, *d_ln = nullptr;
+  int *d_bptr = nullptr, *d_bcell = nullptr, *d_bc_e = nullptr;
+  double* d_bcoef = nullptr;
+  int *d_tptr = nullptr, *d_tent = nullptr, *d_te = nullptr, *d_tbptr = nullptr, *d_tbq = nullptr;
+  double *d_ge = nullptr, *d_gn = nullptr;
+  ~dcx_lmap() {
+    void* ptrs[] = {d_ky2, d_ptr, d_cell, d_le, d_ln, d_bptr, d_bcell, d_bc_e, d_bcoef,
+                    d_tptr, d_tent, d_te, d_tbptr, d_tbq, d_ge, d_gn};
+    for (void* p : ptrs)
+      if (p) (void)hipFree(p);
+  }
+};
+
+extern "C" {
+
+int dcx_lmap_create(int nnz, int ncell, int nky, const double* ky2, const int32_t* ptr,
+                    const int32_t* cell, const double* le, const double* ln, int nbc,
+                    const int32_t* bc_e, const int32_t* bc_cell, const double* bc_coef,
+                    dcx_lmap** out) {
+  try {
+    if (!out || !ky2 || !ptr || !cell || !le || !ln) return fail(DCX_ERR_ARG, "null argument");
+    if (nbc > 0 && (!bc_e || !bc_cell || !bc_coef)) return fail(DCX_ERR_ARG, "null bc arrays");
+    *out = nullptr;
+    std::unique_ptr<dcx_lmap> M(new dcx_lmap());
+    M->nnz = nnz;
+    M->ncell = ncell;
+    M->nky = nky;
+    M->nbc = nbc;
+    HIP_TRY(hipGetDevice(&M->device));
+    const int np = ptr[nnz];
+    for (int p = 0; p < np; ++p)
+      if (cell[p] < 0 || cell[p] >= ncell) return fail(DCX_ERR_ARG, "map cell out of range");
+    // bc entries grouped by value entry (CSR over entries) and by cell
+    std::vector<int> bptr(nnz + 1, 0), order(nbc);
+    for (int q = 0; q < nbc; ++q) {
+      if (bc_e[q] < 0 || bc_e[q] >= nnz || bc_cell[q] < 0 || bc_cell[q] >= ncell)
+        return fail(DCX_ERR_ARG, "bc entry out of range");
+      bptr[bc_e[q] + 1]++;
+    }
+    for (int e = 0; e < nnz; ++e) bptr[e + 1] += bptr[e];
+    {
+      std::vector<int> fill(bptr.begin(), bptr.end() - 1);
+      for (int q = 0; q < nbc; ++q) order[fill[bc_e[q]]++] = q;
+    }
+    std::vector<int> bcell(nbc), be(nbc);
+    std::vector<double> bcoef(static_cast<size_t>(nky) * nbc);
+    for (int t = 0; t < nbc; ++t) {
+      const int q = order[t];
+      bcell[t] = bc_cell[q];
+      be[t] = bc_e[q];
+      for (int k = 0; k < nky; ++k) bcoef[static_cast<size_t>(k) * nbc + t] = bc_coef[static_cast<size_t>(k) * nbc + q];
+    }
+    // transposed CSR by cell
+    std::vector<int> tptr(ncell + 1, 0), tent(np), te(np);
+    std::vector<int> ent_of(np);
+    for (int e = 0; e < nnz; ++e)
+      for (int p = ptr[e]; p < ptr[e + 1]; ++p) ent_of[p] = e;
+    for (int p = 0; p < np; ++p) tptr[cell[p] + 1]++;
+    for (int c = 0; c < ncell; ++c) tptr[c + 1] += tptr[c];
+    {
+      std::vector<int> fill(tptr.begin(), tptr.end() - 1);
+      for (int p = 0; p < np; ++p) {
+        int t = fill[cell[p]]++;
+        tent[t] = p;
+        te[t] = ent_of[p];
+      }
+    }
+    std::vector<int> tbptr(ncell + 1, 0), tbq(nbc);
+    for (int t = 0; t < nbc; ++t) tbptr[bcell[t] + 1]++;
+    for (int c = 0; c < ncell; ++c) tbptr[c + 1] += tbptr[c];
+    {
+      std::vector<int> fill(tbptr.begin(), tbptr.end() - 1);
+      for (int t = 0; t < nbc; ++t) tbq[fill[bcell[t]]++] = t;
+    }
+    HIP_TRY(upload(&M->d_ky2, std::vector<double>(ky2, ky2 + nky)));
+    HIP_TRY(upload(&M->d_ptr, std::vector<int>(ptr, ptr + nnz + 1)));
+    HIP_TRY(upload(&M->d_cell, std::vector<int>(cell, cell + np)));
+    HIP_TRY(upload(&M->d_le, std::vector<double>(le, le + np)));
+    HIP_TRY(upload(&M->d_ln, std::vector<double>(ln, ln + np)));
+    HIP_TRY(upload(&M->d_bptr, bptr));
+    if (nbc) {
+      HIP_TRY(upload(&M->d_bcell, bcell));
+      HIP_TRY(upload(&M->d_bc_e, be));
+      HIP_TRY(upload(&M->d_bcoef, bcoef));
+      HIP_TRY(upload(&M->d_tbq, tbq));
+    }
+    HIP_TRY(upload(&M->d_tptr, tptr));
+    HIP_TRY(upload(&M->d_tent, tent));
+    HIP_TRY(upload(&M->d_te, te));
+    HIP_TRY(upload(&M->d_tbptr, tbptr));
+    HIP_TRY(hipMalloc(&M->d_ge, std::max<size_t>(8, nnz * sizeof(double))));
+    HIP_TRY(hipMalloc(&M->d_gn, std::max<size_t>(8, nnz * sizeof(double))));
+    *out = M.release();
+    return DCX_OK;
+  } catch (const std::exception& e) {
+    return fail(DCX_ERR_INTERNAL, e.what());
+  }
+}
+
+int dcx_lmap_destroy(dcx_lmap* m) {
+  delete m;
+  return DCX_OK;
+}
+
+int dcx_assemble(dcx_lmap* M, const double* sigma, double* values, void* stream) {
+  if (!M || !sigma || !values) return fail(DCX_ERR_ARG, "null argument");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  HIP_TRY(hipSetDevice(M->device));
+  hipLaunchKernelGGL(k_assemble, dim3(grid1(M->nnz, 256)), dim3(256), 0, st, M->nnz, M->nky,
+                     M->d_ky2, M->d_ptr, M->d_cell, M->d_le, M->d_ln, M->d_bptr, M->d_bcell,
+                     M->d_bcoef, M->nbc, sigma, values);
+  HIP_TRY(hipGetLastError());
+  return DCX_OK;
+}
+
+int dcx_lmap_transpose(dcx_lmap* M, const double* gv, const double* sigma, double* grad_m,
+                       void* stream) {
+  if (!M || !gv || !sigma || !grad_m) return fail(DCX_ERR_ARG, "null argument");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  HIP_TRY(hipSetDevice(M->device));
+  hipLaunchKernelGGL(k_reduce_ky, dim3(grid1(M->nnz, 256)), dim3(256), 0, st, M->nnz, M->nky,
+                     M->d_ky2, gv, M->d_ge, M->d_gn);
+  hipLaunchKernelGGL(k_lmap_t, dim3(grid1(M->ncell, 256)), dim3(256), 0, st, M->ncell, M->nnz,
+                     M->nky, M->d_tptr, M->d_tent, M->d_te, M->d_le, M->d_ln, M->d_tbptr,
+                     M->d_tbq, M->d_bc_e, M->d_bcoef, M->nbc, M->d_ge, M->d_gn, gv, sigma,
+                     grad_m);
+  HIP_TRY(hipGetLastError());
+  return DCX_OK;
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// survey
+// ===========================================================================
+struct dcx_survey {
+  int nD = 0, nS = 0, n = 0, device = 0;
+  int *d_ptr = nullptr, *d_rows = nullptr, *d_rows_p = nullptr, *d_sidx = nullptr;
+  double* d_val = nullptr;
+  // adjoint targets, per numbering (0 original, 1 elimination)
+  int ntgt = 0;
+  int *d_tptr = nullptr, *d_tdat = nullptr, *d_trow = nullptr, *d_trow_p = nullptr,
+      *d_tsrc = nullptr;
+  double* d_tcoef = nullptr;
+  std::vector<int> t_row, t_src;
+  ~dcx_survey() {
+    void* ptrs[] = {d_ptr, d_rows, d_rows_p, d_sidx, d_val, d_tptr, d_tdat, d_tcoef,
+                    d_trow, d_trow_p, d_tsrc};
+    for (void* p : ptrs)
+      if (p) (void)hipFree(p);
+  }
+};
+
+extern "C" {
+
+int dcx_survey_create(dcx_plan* P, int nD, int nS, const int32_t* pd_ptr, const int32_t* pd_col,
+                      const double* pd_val, const int32_t* sidx, dcx_survey** out) {
+  try {
+    if (!P || !out || !pd_ptr || !pd_col || !pd_val || !sidx) return fail(DCX_ERR_ARG, "null argument");
+    *out = nullptr;
+    std::unique_ptr<dcx_survey> V(new dcx_survey());
+    V->nD = nD;
+    V->nS = nS;
+    V->n = P->S.n;
+    HIP_TRY(hipGetDevice(&V->device));
+    const int np = pd_ptr[nD];
+    std::vector<int> rows(pd_col, pd_col + np), rows_p(np);
+    for (int p = 0; p < np; ++p) {
+      if (rows[p] < 0 || rows[p] >= V->n) return fail(DCX_ERR_ARG, "Pd column out of range");
+      rows_p[p] = P->S.iperm[rows[p]];
+    }
+    for (int i = 0; i < nD; ++i)
+      if (sidx[i] < 0 || sidx[i] >= nS) return fail(DCX_ERR_ARG, "source index out of range");
+    // targets (row, source) with their contributing data, sorted by (row, src)
+    std::vector<std::tuple<int, int, int, double>> trip;  // row, src, datum, coef
+    for (int i = 0; i < nD; ++i)
+      for (int p = pd_ptr[i]; p < pd_ptr[i + 1]; ++p) trip.emplace_back(rows[p], sidx[i], i, pd_val[p]);
+    std::sort(trip.begin(), trip.end(), [](const auto& a, const auto& b) {
+      if (std::get<0>(a) != std::get<0>(b)) return std::get<0>(a) < std::get<0>(b);
+      if (std::get<1>(a) != std::get<1>(b)) return std::get<1>(a) < std::get<1>(b);
+      return std::get<2>(a) < std::get<2>(b);
+    });
+    std::vector<int> tptr, tdat;
+    std::vector<double> tcoef;
+    for (size_t q = 0; q < trip.size(); ++q) {
+      const bool fresh = q == 0 || std::get<0>(trip[q]) != std::get<0>(trip[q - 1]) ||
+                         std::get<1>(trip[q]) != std::get<1>(trip[q - 1]);
+      if (fresh) {
+        tptr.push_back(static_cast<int>(q));
+        V->t_row.push_back(std::get<0>(trip[q]));
+        V->t_src.push_back(std::get<1>(trip[q]));
+      }
+      tdat.push_back(std::get<2>(trip[q]));
+      tcoef.push_back(std::get<3>(trip[q]));
+    }
+    tptr.push_back(static_cast<int>(trip.size()));
+    V->ntgt = static_cast<int>(V->t_row.size());
+    if (static_cast<int>(tptr.size()) != V->ntgt + 1) return fail(DCX_ERR_INTERNAL, "target table");
+    HIP_TRY(upload(&V->d_ptr, std::vector<int>(pd_ptr, pd_ptr + nD + 1)));
+    HIP_TRY(upload(&V->d_rows, rows));
+    HIP_TRY(upload(&V->d_rows_p, rows_p));
+    HIP_TRY(upload(&V->d_val, std::vector<double>(pd_val, pd_val + np)));
+    HIP_TRY(upload(&V->d_sidx, std::vector<int>(sidx, sidx + nD)));
+    HIP_TRY(upload(&V->d_tptr, tptr));
+    HIP_TRY(upload(&V->d_tdat, tdat));
+    HIP_TRY(upload(&V->d_tcoef, tcoef));
+    std::vector<int> trow_p(V->ntgt);
+    for (int t = 0; t < V->ntgt; ++t) trow_p[t] = P->S.iperm[V->t_row[t]];
+    HIP_TRY(upload(&V->d_trow, V->t_row));
+    HIP_TRY(upload(&V->d_trow_p, trow_p));
+    HIP_TRY(upload(&V->d_tsrc, V->t_src));
+    *out = V.release();
+    return DCX_OK;
+  } catch (const std::exception& e) {
+    return fail(DCX_ERR_INTERNAL, e.what());
+  }
+}
+
+int dcx_survey_destroy(dcx_survey* v) {
+  delete v;
+  return DCX_OK;
+}
+
+int dcx_survey_dpred(dcx_survey* V, const double* U, int k, int nky, const double* w, int s0,
+                     int s1, double* d, int flags, void* stream) {
+  if (!V || !U || !w || !d) return fail(DCX_ERR_ARG, "null argument");
+  if (s0 < 0 || s1 > V->nS || s0 > s1 || s1 - s0 > k)
+    return fail(DCX_ERR_ARG, "source window does not fit the block width");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  HIP_TRY(hipSetDevice(V->device));
+  const int64_t bs = static_cast<int64_t>(V->n) * k;
+  hipLaunchKernelGGL(k_dpred, dim3(grid1(V->nD, 256)), dim3(256), 0, st, V->nD, V->d_ptr,
+                     (flags & DCX_PERMUTED) ? V->d_rows_p : V->d_rows, V->d_val, V->d_sidx, U,
+                     bs, k, nky, w, s0, s1, d);
+  HIP_TRY(hipGetLastError());
+  return DCX_OK;
+}
+
+int dcx_survey_adjoint_rhs(dcx_survey* V, const double* v, double* G, int k, int s0, int s1,
+                           int flags, void* stream) {
+  if (!V || !v || !G) return fail(DCX_ERR_ARG, "null argument");
+  if (s0 < 0 || s1 > V->nS || s0 > s1 || s1 - s0 > k)
+    return fail(DCX_ERR_ARG, "source window does not fit the block width");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  HIP_TRY(hipSetDevice(V->device));
+  const int64_t tot = static_cast<int64_t>(V->n) * k;
+  hipLaunchKernelGGL(k_fill, dim3(grid_stride(tot)), dim3(256), 0, st, G, tot, 0.0);
+  if (V->ntgt)
+    hipLaunchKernelGGL(k_adjoint_rhs, dim3(grid1(V->ntgt, 256)), dim3(256), 0, st, V->ntgt,
+                       V->d_tptr, (flags & DCX_PERMUTED) ? V->d_trow_p : V->d_trow, V->d_tsrc,
+                       V->d_tdat, V->d_tcoef, v, k, s0, s1, G);
+  HIP_TRY(hipGetLastError());
+  return DCX_OK;
+}
+
+}  // extern "C"
+// ===========================================================================
+// host-only symbolic analysis
+// ===========================================================================
+struct dcx_symbolic {
+  dcx::Symbolic S;
+};
+
+extern "C" {
+
+int dcx_symbolic_analyse(int n, const int32_t* crow, const int32_t* col, const double* coords,
+                         int dim, int leaf_max, dcx_symbolic** out) {
+  try {
+    if (!out || !crow || !col) return fail(DCX_ERR_ARG, "null argument");
+    *out = nullptr;
+    if (coords && (dim < 1 || dim > 3)) return fail(DCX_ERR_ARG, "dim must be 1..3");
+    std::unique_ptr<dcx_symbolic> Y(new dcx_symbolic());
+    std::string err = dcx::analyse(n, crow, col, coords, dim, leaf_max, Y->S);
+    if (!err.empty()) return fail(DCX_ERR_ARG, err);
+    *out = Y.release();
+    return DCX_OK;
+  } catch (const std::exception& e) {
+    return fail(DCX_ERR_INTERNAL, e.what());
+  }
+}
+
+int dcx_symbolic_view_get(const dcx_symbolic* Y, dcx_symbolic_view* o) {
+  if (!Y || !o) return fail(DCX_ERR_ARG, "null argument");
+  const auto& S = Y->S;
+  o->n = S.n;
+  o->nnz = S.nnz;
+  o->nsup = S.nsup;
+  o->ndepth = S.ndepth;
+  o->max_front = S.max_front;
+  o->max_ns = S.max_ns;
+  o->perm = S.perm.data();
+  o->first = S.first.data();
+  o->ns = S.ns.data();
+  o->rs = S.rs.data();
+  o->parent = S.parent.data();
+  o->depth = S.depth.data();
+  o->rptr = S.rptr.data();
+  o->ridx = S.ridx.data();
+  o->cptr = S.cptr.data();
+  o->cidx = S.cidx.data();
+  o->relpos = S.relpos.data();
+  o->aptr = S.aptr.data();
+  o->a_e = S.a_e.data();
+  o->a_row = S.a_row.data();
+  o->a_col = S.a_col.data();
+  o->lvl_ptr = S.lvl_ptr.data();
+  o->lvl_sup = S.lvl_sup.data();
+  o->nnz_L = S.nnz_L;
+  o->factor_flops = S.factor_flops;
+  o->solve_flops_per_rhs = S.solve_flops_per_rhs;
+  return DCX_OK;
+}
+
+int dcx_symbolic_destroy(dcx_symbolic* Y) {
+  delete Y;
+  return DCX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,554 @@
+"""Differentiable 2.5-D DC-resistivity forward, drop-in for dctorch.TorchDC2D.
+
+API (dctorch/simulation2d.py:34-534): ``TorchDC2D(mesh, survey, nky=12,
+device, dtype, active_cells, val_inactive, chunk, singularity_removal,
+rho_primary, quadrature)`` with ``dpred``, ``misfit``, ``misfit_and_grad``,
+``Jtvec``, ``Jmatrix``, ``Jvec``/``linearize`` and the attributes ``nC, nN,
+nS, nD, nky, nnzA, kys, wq, q, Pd, sidx, iD, Lstack, solver,
+quadrature_info``.  The model is m = log(rho) (sigma = exp(-m)); the misfit
+has no 1/2.
+
+On a GPU device one evaluation is, all in libdcx.so kernels:
+
+1. sigma -> A-values of every wavenumber (factored assembly map);
+2. numeric multifrontal factorisation of all nky systems (one launch per
+   tree level, all wavenumbers in each launch);
+3. U_k = A_k^{-1} q for all wavenumbers and sources in one batched solve (the
+   forward sweep skips the subtrees holding no electrode);
+4. d = (Pd sum_k w_k U_k)[iD, sidx] by a direct gather per datum.
+
+The reverse pass (a torch.autograd.Function) reuses the factor:
+G = Pd^T (dL/dd) per source, Lam_k = A_k^{-1} G, then
+gv[k, e] = -w_k sum_s Lam_k[row_e, s] U_k[col_e, s] and
+dL/dm = -sigma * L^T gv.  The fields are kept in the elimination order
+throughout, so no permutation pass touches them.
+
+``precision="mixed"`` factors and sweeps in FP32 and recovers FP64 accuracy
+by residual refinement (FP64 residual on the FP64 values); ``"fp64"`` (the
+default) is FP64 end to end.  On the cpu device the reference's own chain
+runs: torch sparse assembly and SciPy SuperLU (dctorch/solver.py:314-374).
+"""
+import ctypes as C
+import os
+import warnings
+
+import numpy as np
+import scipy.sparse as sp
+import torch
+
+from . import _lib
+from .operators2d import Problem2DMaps
+from .quadrature import (amplification, electrode_distance_range, k0_quadrature,
+                         simpeg_quadrature, transform_error)
+from .solver import (HipBatchSolver, make_batch_solver, sparse_solve_batch,
+                     sparse_solve_batch_prefactored)
+
+
+def _s2t(M, device, dtype):
+    M = sp.coo_matrix(M)
+    return torch.sparse_coo_tensor(
+        torch.tensor(np.vstack((M.row, M.col)), dtype=torch.int64, device=device),
+        torch.tensor(M.data, dtype=dtype, device=device), M.shape).coalesce()
+
+
+class _GpuState:
+    """Device objects of one engine: plan, assembly map, survey map."""
+
+    def __init__(self, eng, maps, precision, refine, leaf):
+        dev = eng.device
+        self.solver = HipBatchSolver(
+            torch.tensor(maps.crow), torch.tensor(maps.col), maps.nN, maps.nS, eng.nky,
+            device=dev, dtype=torch.float64, precision=precision, refine=refine,
+            coords=maps.node_coords(), leaf=leaf)
+        lib = self.lib = self.solver._lib
+        rows = maps.electrode_rows()
+        self.solver.set_rhs_support(rows)
+        self.perm = torch.tensor(self.solver.perm().astype(np.int64), device=dev)
+        # assembly map
+        ky2, ky2p = _lib.f64(eng.kys ** 2)
+        ptr, ptrp = _lib.i32(maps.map_ptr)
+        cell, cellp = _lib.i32(maps.map_cell)
+        le, lep = _lib.f64(maps.map_le)
+        ln, lnp = _lib.f64(maps.map_ln)
+        be, bep = _lib.i32(maps.bc_e)
+        bcl, bclp = _lib.i32(maps.bc_cell)
+        bco, bcop = _lib.f64(maps.bc_coef)
+        h = C.c_void_p()
+        with torch.cuda.device(dev):
+            _lib.check(lib.dcx_lmap_create(maps.nnz, maps.nC, eng.nky, ky2p, ptrp, cellp, lep,
+                                           lnp, int(be.size), bep, bclp, bcop, C.byref(h)),
+                       "lmap_create")
+        self.lmap = h
+        # survey map
+        Pd = maps.Pd
+        pp, ppp = _lib.i32(Pd.indptr)
+        pc, pcp = _lib.i32(Pd.indices)
+        pv, pvp = _lib.f64(Pd.data)
+        si, sip = _lib.i32(maps.sidx)
+        h2 = C.c_void_p()
+        with torch.cuda.device(dev):
+            _lib.check(lib.dcx_survey_create(self.solver._h, maps.nD, maps.nS, ppp, pcp, pvp,
+                                             sip, C.byref(h2)), "survey_create")
+        self.survey = h2
+        # sources in elimination order, dense (nN x nS)
+        q = maps.q.tocsr()[self.solver.perm()].toarray()
+        self.q_perm = torch.tensor(q, dtype=torch.float64, device=dev)
+        self.wq = torch.tensor(eng.wq_np, dtype=torch.float64, device=dev)
+
+    def stream(self):
+        return self.solver._stream()
+
+    def assemble(self, sigma):
+        vals = torch.empty(self.solver.nbatch, self.solver.col.numel(), dtype=torch.float64,
+                           device=sigma.device)
+        _lib.check(self.lib.dcx_assemble(self.lmap, _lib.ptr(sigma), _lib.ptr(vals),
+                                         self.stream()), "assemble")
+        return vals
+
+    def lmap_t(self, gv, sigma):
+        out = torch.empty_like(sigma)
+        _lib.check(self.lib.dcx_lmap_transpose(self.lmap, _lib.ptr(gv), _lib.ptr(sigma),
+                                               _lib.ptr(out), self.stream()), "lmap_transpose")
+        return out
+
+    def solve_perm(self, b, k, sparse=True, shared=True):
+        sv = self.solver
+        x = torch.empty(sv.nbatch, sv.n, k, dtype=torch.float64, device=b.device)
+        flags = _lib.DCX_PERMUTED | (_lib.DCX_SHARED_RHS if shared else 0) | \
+            (_lib.DCX_SPARSE_RHS if sparse else 0)
+        _lib.check(self.lib.dcx_solve(sv._h, _lib.ptr(b), _lib.ptr(x), int(k), flags,
+                                      self.stream()), "solve")
+        return x
+
+    def dpred(self, U, k, s0, s1, d):
+        _lib.check(self.lib.dcx_survey_dpred(self.survey, _lib.ptr(U), int(k),
+                                             self.solver.nbatch, _lib.ptr(self.wq), int(s0),
+                                             int(s1), _lib.ptr(d), _lib.DCX_PERMUTED,
+                                             self.stream()), "survey_dpred")
+
+    def adjoint_rhs(self, v, k, s0, s1):
+        G = torch.empty(self.solver.n, k, dtype=torch.float64, device=v.device)
+        _lib.check(self.lib.dcx_survey_adjoint_rhs(self.survey, _lib.ptr(v), _lib.ptr(G),
+                                                   int(k), int(s0), int(s1),
+                                                   _lib.DCX_PERMUTED, self.stream()),
+                   "survey_adjoint_rhs")
+        return G
+
+    def free(self):
+        if getattr(self, "lmap", None):
+            self.lib.dcx_lmap_destroy(self.lmap)
+            self.lmap = None
+        if getattr(self, "survey", None):
+            self.lib.dcx_survey_destroy(self.survey)
+            self.survey = None
+        self.solver.free()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class _Forward2D(torch.autograd.Function):
+    """m_full -> dpred on the GPU; backward = one adjoint solve + gathers."""
+
+    @staticmethod
+    def forward(ctx, m_full, eng):
+        g = eng._gpu
+        with torch.cuda.device(eng.device):
+            sig = torch.exp(-m_full.detach())
+            vals = g.assemble(sig)
+            g.solver.factorize(vals)
+            U = g.solve_perm(g.q_perm, eng.nS)
+            d = torch.empty(eng.nD, dtype=torch.float64, device=eng.device)
+            g.dpred(U, eng.nS, 0, eng.nS, d)
+        ctx.eng = eng
+        ctx.stamp = g.solver.stamp
+        ctx.save_for_backward(sig, vals, U)
+        return d
+
+    @staticmethod
+    def backward(ctx, gd):
+        sig, vals, U = ctx.saved_tensors
+        eng = ctx.eng
+        g = eng._gpu
+        with torch.cuda.device(eng.device):
+            if g.solver.stamp != ctx.stamp:      # someone refactorised since
+                g.solver.factorize(vals)
+            v = gd.detach().to(torch.float64).contiguous()
+            G = g.adjoint_rhs(v, eng.nS, 0, eng.nS)
+            Lam = g.solve_perm(G, eng.nS)
+            gv = g.solver.grad_values(Lam, U, scale=g.wq, permuted=True)
+            grad_m = g.lmap_t(gv, sig)
+        return grad_m, None
+
+
+class TorchDC2D:
+    """Differentiable dpred/misfit for 2.5-D nodal DC on a fixed mesh+survey."""
+
+    def __init__(self, mesh, survey, nky=12, device="cuda", dtype=torch.float64,
+                 active_cells=None, val_inactive=None, chunk=None,
+                 singularity_removal=False, rho_primary=None, quadrature=None,
+                 precision="fp64", refine=1, leaf=None):
+        if singularity_removal:
+            raise NotImplementedError(
+                "singularity removal is not part of this engine (off by default in the "
+                "reference); use the total-field formulation")
+        if quadrature is None:
+            quadrature = os.environ.get("DCTORCH_QUADRATURE", "electrodes")
+        if quadrature not in ("auto", "simpeg", "electrodes"):
+            raise ValueError(f"quadrature must be auto|simpeg|electrodes, got {quadrature!r}")
+        self.device = torch.device(device)
+        if self.device.type == "cuda" and self.device.index is None:
+            self.device = torch.device("cuda", torch.cuda.current_device())
+        self.dtype = dtype
+        self.chunk = None if chunk is None else int(chunk)
+        self._sr = False
+        self.precision = precision
+
+        rmin, rmax = electrode_distance_range(survey)
+        sp_pts, sp_wts, fell_back = simpeg_quadrature(mesh, nky)
+        err_sp = transform_error(sp_pts, sp_wts, rmin, rmax)
+        self.quadrature_info = {"source": "simpeg", "simpeg_fell_back": fell_back,
+                                "r_range": (rmin, rmax), "max_rel_err": err_sp,
+                                "amplification": amplification(sp_wts),
+                                "n_used": int(np.size(sp_pts))}
+        pts, wts = sp_pts, sp_wts
+        if quadrature == "electrodes" or (quadrature == "auto" and fell_back):
+            pts, wts, info = k0_quadrature(rmin, rmax, nky)
+            self.quadrature_info.update(source="electrodes", max_rel_err=info["max_rel_err"],
+                                        amplification=info["amplification"],
+                                        n_used=info["n_used"], simpeg_max_rel_err=err_sp)
+            if fell_back:
+                warnings.warn(
+                    f"SimPEG's wavenumber fit failed at nky={nky} and fell back to a "
+                    f"trapezoid rule (transform error {err_sp:.1e}); replaced by the "
+                    f"electrode fit ({info['max_rel_err']:.1e}).", stacklevel=2)
+        elif fell_back:
+            warnings.warn(f"SimPEG's wavenumber fit failed at nky={nky}; its trapezoid "
+                          f"fallback has transform error {err_sp:.1e}.", stacklevel=2)
+        self.kys = np.asarray(pts, float)
+        self.wq_np = np.asarray(wts, float)
+        maps = Problem2DMaps(mesh, survey, self.kys)
+        self._maps = maps
+        self.nC, self.nN, self.nS, self.nD = maps.nC, maps.nN, maps.nS, maps.nD
+        self.nky = self.kys.size
+        self.nnzA = maps.nnz
+        dev, dt = self.device, self.dtype
+        self.wq = torch.tensor(self.wq_np, dtype=dt, device=dev)
+        self.sidx = torch.tensor(maps.sidx, dtype=torch.int64, device=dev)
+        self.iD = torch.arange(self.nD, dtype=torch.int64, device=dev)
+        self._crow = torch.tensor(maps.crow, dtype=torch.int32)
+        self._ccol = torch.tensor(maps.col, dtype=torch.int32)
+        self._Pd = None
+        self._q = None
+        self._Lstack = None
+        self._P_inj = None
+        if active_cells is not None:
+            act = np.asarray(active_cells, dtype=bool)
+            ia = np.where(act)[0]
+            self._act = torch.tensor(ia, dtype=torch.int64, device=dev)
+            v = np.full(self.nC, float(val_inactive))
+            v[act] = 0.0
+            self._v_inj = torch.tensor(v, dtype=dt, device=dev)
+            self._P_inj = True
+        self._mesh, self._survey = mesh, survey
+        self._solvers = {}
+        self._gpu = None
+        if self.device.type == "cuda":
+            self._gpu = _GpuState(self, maps, precision, refine, leaf)
+            self.solver = self._gpu.solver
+        else:
+            self.solver = self._solver_for(self.nS)
+
+    # ---------------- reference attributes (built on first use) -------------
+    @property
+    def q(self):
+        if self._q is None:
+            self._q = torch.tensor(self._maps.q.toarray(), dtype=self.dtype, device=self.device)
+        return self._q
+
+    @property
+    def Pd(self):
+        if self._Pd is None:
+            self._Pd = _s2t(self._maps.Pd, self.device, self.dtype)
+        return self._Pd
+
+    @property
+    def Lstack(self):
+        """(nky*nnzA, nC) stacked map, as the reference's COO tensor."""
+        if self._Lstack is None:
+            m = self._maps
+            ent = np.repeat(np.arange(m.nnz), np.diff(m.map_ptr))
+            blocks = []
+            for ky in self.kys:
+                L = sp.csr_matrix((m.map_le + ky ** 2 * m.map_ln, (ent, m.map_cell)),
+                                  shape=(m.nnz, m.nC))
+                blocks.append(L)
+            for k in range(self.nky):
+                blocks[k] = blocks[k] + sp.csr_matrix(
+                    (m.bc_coef[k], (m.bc_e, m.bc_cell)), shape=(m.nnz, m.nC))
+            self._Lstack = _s2t(sp.vstack(blocks), self.device, self.dtype)
+        return self._Lstack
+
+    # ---------------- helpers -------------------------------------------------
+    def _solver_for(self, n_rhs):
+        if n_rhs not in self._solvers:
+            self._solvers[n_rhs] = make_batch_solver(self._crow, self._ccol, self.nN, n_rhs,
+                                                     self.nky, device=self.device,
+                                                     dtype=self.dtype)
+        return self._solvers[n_rhs]
+
+    def _source_chunks(self, chunk=None):
+        w = self.chunk if chunk is None else chunk
+        if not w or w >= self.nS:
+            return [(0, self.nS)]
+        return [(s, min(s + w, self.nS)) for s in range(0, self.nS, w)]
+
+    def _rows_of(self, s0, s1):
+        key = (s0, s1)
+        cache = self.__dict__.setdefault("_rowcache", {})
+        if key not in cache:
+            sel = (self.sidx >= s0) & (self.sidx < s1)
+            cache[key] = (torch.nonzero(sel, as_tuple=True)[0], self.iD[sel], self.sidx[sel] - s0)
+        return cache[key]
+
+    def _model_to_full(self, m):
+        m_loc = m.to(device=self.device, dtype=self.dtype).reshape(-1)
+        if self._P_inj is not None:
+            full = self._v_inj.clone()
+            full = full.index_put((self._act,), m_loc, accumulate=True)
+            return full
+        return m_loc
+
+    def _model_from_full(self, g_full):
+        if self._P_inj is not None:
+            return g_full[self._act]
+        return g_full
+
+    # ---------------- runtime --------------------------------------------------
+    def dpred(self, m):
+        """Predicted data (nD,) for m = log(rho).  Autograd-traceable."""
+        dev0, dt0 = m.device, m.dtype
+        m_full = self._model_to_full(m)
+        if self._gpu is not None:
+            d = _Forward2D.apply(m_full.to(torch.float64), self)
+            return d.to(device=dev0, dtype=dt0)
+        if self.chunk and self.chunk < self.nS:
+            d = torch.empty(self.nD, device=self.device, dtype=self.dtype)
+            for s0, s1 in self._source_chunks():
+                rows, _, _ = self._rows_of(s0, s1)
+                d[rows] = self._dpred_chunk_cpu(m_full, s0, s1)
+            return d.to(device=dev0, dtype=dt0)
+        sig = torch.exp(-m_full).unsqueeze(1)
+        VALS = torch.sparse.mm(self.Lstack, sig).view(self.nky, self.nnzA)
+        U = sparse_solve_batch(VALS, self.q.contiguous(), self.solver)
+        Ubar = (self.wq.view(-1, 1, 1) * U).sum(0)
+        d = torch.sparse.mm(self.Pd, Ubar)[self.iD, self.sidx]
+        return d.to(device=dev0, dtype=dt0)
+
+    def _dpred_chunk_cpu(self, m_full, s0, s1):
+        sig = torch.exp(-m_full).unsqueeze(1)
+        VALS = torch.sparse.mm(self.Lstack, sig).view(self.nky, self.nnzA)
+        U = sparse_solve_batch(VALS, self.q[:, s0:s1].contiguous(), self._solver_for(s1 - s0))
+        Ubar = (self.wq.view(-1, 1, 1) * U).sum(0)
+        _, gi, li = self._rows_of(s0, s1)
+        return torch.sparse.mm(self.Pd, Ubar)[gi, li]
+
+    def misfit(self, m, dobs, w):
+        """phi_d = ||w (dpred - dobs)||^2 (no 1/2 factor)."""
+        r = w * (self.dpred(m) - dobs)
+        return (r ** 2).sum()
+
+    # -- chunked reverse passes ---------------------------------------------------
+    def _gpu_reverse(self, m, weights_of, chunk):
+        """Shared body of misfit_and_grad / Jtvec on the GPU: factor once, then
+        per source chunk solve, project, form the adjoint weights (weights_of
+        gets the chunk's data and rows), adjoint solve and gather."""
+        g = self._gpu
+        width = self.chunk if chunk is None else chunk
+        width = self.nS if not width else min(int(width), self.nS)
+        with torch.no_grad(), torch.cuda.device(self.device):
+            m_full = self._model_to_full(m.detach()).to(torch.float64)
+            sig = torch.exp(-m_full)
+            vals = g.assemble(sig)
+            g.solver.factorize(vals)
+            gv = torch.zeros(self.nky, self.nnzA, dtype=torch.float64, device=self.device)
+            total = 0.0
+            d = torch.empty(self.nD, dtype=torch.float64, device=self.device)
+            for s0, s1 in self._source_chunks(width):
+                k = s1 - s0
+                qb = g.q_perm[:, s0:s1].contiguous() if k != self.nS else g.q_perm
+                U = g.solve_perm(qb, k)
+                g.dpred(U, k, s0, s1, d)
+                rows, _, _ = self._rows_of(s0, s1)
+                v, f = weights_of(d, rows)
+                total += f
+                G = g.adjoint_rhs(v, k, s0, s1)
+                Lam = g.solve_perm(G, k)
+                gv += g.solver.grad_values(Lam, U, scale=g.wq, permuted=True)
+                del U, Lam, G
+            grad = self._model_from_full(g.lmap_t(gv, sig))
+        return total, grad
+
+    def misfit_and_grad(self, m, dobs, w, chunk=None):
+        """(phi_d, dphi_d/dm) one source chunk at a time, outside autograd."""
+        if self._gpu is not None:
+            dobs_t = dobs.to(self.device, torch.float64).reshape(-1)
+            w_t = w.to(self.device, torch.float64).reshape(-1)
+
+            def weights_of(d, rows):
+                vfull = torch.zeros(self.nD, dtype=torch.float64, device=self.device)
+                r = d[rows] - dobs_t[rows]
+                vfull[rows] = 2.0 * w_t[rows] ** 2 * r
+                return vfull, float(((w_t[rows] * r) ** 2).sum())
+
+            total, grad = self._gpu_reverse(m, weights_of, chunk)
+            return total, grad.to(m.device, m.dtype)
+        width = self.chunk if chunk is None else chunk
+        width = self.nS if not width else min(int(width), self.nS)
+        solver = self._solver_for(width)
+        with torch.no_grad():
+            sig = torch.exp(-self._model_to_full(m)).unsqueeze(1)
+            solver.factorize(torch.sparse.mm(self.Lstack, sig).view(self.nky, self.nnzA))
+        total, grad = 0.0, torch.zeros_like(m)
+        for s0, s1 in self._source_chunks(width):
+            rows, gi, li = self._rows_of(s0, s1)
+            mc = m.detach().clone().requires_grad_(True)
+            sig = torch.exp(-self._model_to_full(mc)).unsqueeze(1)
+            VALS = torch.sparse.mm(self.Lstack, sig).view(self.nky, self.nnzA)
+            q = self.q[:, s0:s1]
+            if width > s1 - s0:
+                q = torch.cat([q, q.new_zeros(self.nN, width - (s1 - s0))], dim=1)
+            U = sparse_solve_batch_prefactored(VALS, q.contiguous(), solver)
+            Ubar = (self.wq.view(-1, 1, 1) * U).sum(0)[:, :s1 - s0]
+            d = torch.sparse.mm(self.Pd, Ubar)[gi, li]
+            f = ((w[rows] * (d - dobs[rows])) ** 2).sum()
+            f.backward()
+            total += float(f)
+            grad += mc.grad
+        return total, grad
+
+    def Jtvec(self, m, v, chunk=None):
+        """J^T v without forming J (same reverse pass as the gradient)."""
+        if self._gpu is not None:
+            v_t = v.to(self.device, torch.float64).reshape(-1)
+
+            def weights_of(d, rows):
+                vfull = torch.zeros(self.nD, dtype=torch.float64, device=self.device)
+                vfull[rows] = v_t[rows]
+                return vfull, 0.0
+
+            _, g = self._gpu_reverse(m, weights_of, chunk)
+            return g.to(m.device, m.dtype)
+        width = self.chunk if chunk is None else chunk
+        width = self.nS if not width else min(int(width), self.nS)
+        solver = self._solver_for(width)
+        v_t = v.to(device=self.device, dtype=self.dtype).reshape(-1)
+        with torch.no_grad():
+            sig = torch.exp(-self._model_to_full(m)).unsqueeze(1)
+            solver.factorize(torch.sparse.mm(self.Lstack, sig).view(self.nky, self.nnzA))
+        out = torch.zeros_like(m)
+        for s0, s1 in self._source_chunks(width):
+            rows, gi, li = self._rows_of(s0, s1)
+            mc = m.detach().clone().requires_grad_(True)
+            sig = torch.exp(-self._model_to_full(mc)).unsqueeze(1)
+            VALS = torch.sparse.mm(self.Lstack, sig).view(self.nky, self.nnzA)
+            q = self.q[:, s0:s1]
+            if width > s1 - s0:
+                q = torch.cat([q, q.new_zeros(self.nN, width - (s1 - s0))], dim=1)
+            U = sparse_solve_batch_prefactored(VALS, q.contiguous(), solver)
+            Ubar = (self.wq.view(-1, 1, 1) * U).sum(0)[:, :s1 - s0]
+            d = torch.sparse.mm(self.Pd, Ubar)[gi, li]
+            d.backward(v_t[rows])
+            out = out + mc.grad
+        return out
+
+    def Jmatrix(self, m, block=32):
+        """The explicit Jacobian (nD, nM) from one factorisation (batched
+        adjoint: the adjoint right-hand sides are the columns of Pd^T)."""
+        block = max(1, min(int(block), self.nD))
+        dev0, dt0 = m.device, m.dtype
+        Lt = self.Lstack.t().coalesce()
+        with torch.no_grad():
+            m_full = self._model_to_full(m).to(torch.float64)
+            sig = torch.exp(-m_full)
+            if self._gpu is not None:
+                g = self._gpu
+                with torch.cuda.device(self.device):
+                    vals = g.assemble(sig)
+                    g.solver.factorize(vals)
+                    U = g.solve_perm(g.q_perm, self.nS)
+                    PdT = torch.tensor(self._maps.Pd.T.tocsr()[self.solver.perm()].toarray(),
+                                       dtype=torch.float64, device=self.device)
+                    # pattern entries in elimination numbering
+                    perm_inv = torch.empty_like(g.perm)
+                    perm_inv[g.perm] = torch.arange(self.nN, device=self.device)
+                    row = perm_inv[self.solver.row]
+                    col = perm_inv[self.solver.col.long()]
+                    solve = lambda rhs: g.solve_perm(rhs, rhs.shape[1], sparse=True)
+            else:
+                solver = self._solver_for(self.nS)
+                VALS = torch.sparse.mm(self.Lstack, sig.unsqueeze(1)).view(self.nky, self.nnzA)
+                solver.factorize(VALS.to(self.dtype))
+                U = solver.solve(self.q.to(self.dtype)).to(torch.float64)
+                PdT = self.Pd.t().coalesce().to_dense().to(torch.float64)
+                row, col = solver.row.long(), solver.col.long()
+                solve = lambda rhs: solver.solve(rhs.to(self.dtype)).to(torch.float64)
+            wq = self.wq.to(torch.float64)
+            Jm = torch.empty(self.nC, self.nD, dtype=torch.float64, device=self.device)
+            for a in range(0, self.nD, block):
+                b = min(a + block, self.nD)
+                Y = solve(PdT[:, a:b].contiguous())
+                cols = self.sidx[a:b]
+                per_ky = [(-wq[k]) * Y[k][row] * U[k][:, cols][col] for k in range(self.nky)]
+                Jm[:, a:b] = torch.sparse.mm(Lt.to(torch.float64), torch.cat(per_ky, 0))
+            Jm = -sig.unsqueeze(1) * Jm
+            Jm = self._model_from_full(Jm)
+            return Jm.T.contiguous().to(device=dev0, dtype=dt0)
+
+    def Jvec(self, m, v):
+        """J v (nD,) by finite-free linearisation: one forward + one extra solve."""
+        with torch.no_grad():
+            m_full = self._model_to_full(m).to(torch.float64)
+            sig = torch.exp(-m_full)
+            dm = self._model_to_full(v).to(torch.float64)
+            if self._P_inj is not None:
+                dm = dm - self._v_inj.to(torch.float64)
+            dsig = -sig * dm
+            if self._gpu is not None:
+                g = self._gpu
+                with torch.cuda.device(self.device):
+                    vals = g.assemble(sig)
+                    dvals = g.assemble(dsig)
+                    g.solver.factorize(vals)
+                    U = g.solve_perm(g.q_perm, self.nS)
+                    rhs = torch.empty_like(U)
+                    pinv = torch.empty_like(g.perm)
+                    pinv[g.perm] = torch.arange(self.nN, device=self.device)
+                    crow = self._crow.to(self.device, torch.int64)
+                    colp = pinv[self._ccol.to(self.device).long()]
+                    rowp = pinv[self.solver.row]
+                    for k in range(self.nky):
+                        A = torch.sparse_coo_tensor(torch.stack([rowp, colp]), -dvals[k],
+                                                    (self.nN, self.nN))
+                        rhs[k] = torch.sparse.mm(A, U[k])
+                    del crow
+                    dU = g.solve_perm(rhs, self.nS, sparse=False, shared=False)
+                    d = torch.empty(self.nD, dtype=torch.float64, device=self.device)
+                    g.dpred(dU, self.nS, 0, self.nS, d)
+                    return d.to(m.device, m.dtype)
+            VALS = torch.sparse.mm(self.Lstack, sig.unsqueeze(1)).view(self.nky, self.nnzA)
+            dV = torch.sparse.mm(self.Lstack, dsig.unsqueeze(1)).view(self.nky, self.nnzA)
+            solver = self._solver_for(self.nS)
+            solver.factorize(VALS)
+            U = solver.solve(self.q)
+            rhs = torch.empty_like(U)
+            for k in range(self.nky):
+                A = torch.sparse_csr_tensor(self._crow.long(), self._ccol.long(), -dV[k],
+                                            (self.nN, self.nN))
+                rhs[k] = A @ U[k]
+            dU = solver.solve(rhs)
+            Ubar = (self.wq.view(-1, 1, 1) * dU).sum(0)
+            return torch.sparse.mm(self.Pd, Ubar)[self.iD, self.sidx].to(m.device, m.dtype)

@@ -1,0 +1,159 @@
+"""Synthetic 2.5-D workloads: meshes, electrode arrays and models.
+
+* `regression_case`  -- the reference's 2.5-D regression problem
+  (dctorch/tests/test_simulation2d.py:24-56): 100 x 50 cells of 1 m, 21
+  electrodes at -30..30, dipole-dipole n = 1..6, a 10 ohm-m block in 100.
+* `suite_case`       -- the reference's 2d-tensor coverage case
+  (dctorch/tests/test_suite.py:62-87): 64 x 32 cells, 15 electrodes, DD
+  n = 1..4, rough model (seed 11, sigma_log 0.3).
+* `small_padded_case` -- 4.8k cells, 2.5 m core with 1.3x padding, 21
+  electrodes at 5 m, DD n = 1..6, 100 ohm-m with 10 and 1000 ohm-m Gaussian
+  blobs (seed 0).
+* `line_25k_case`    -- 25,088 cells (200 x 100 core of 1 m, 12 padding cells
+  growing 1.3x), 61 electrodes at 2 m, dipole-dipole (a = 1, 2, 3 electrode
+  spacings, n = 1..6) plus Wenner-Schlumberger (a = 1, 2, n = 1..8), sources
+  merged by electrode pair: 849 sources, 1,636 data; m = log(rho) a Gaussian
+  random field (mean log 100, std 1, correlation length 10 m, seed 1).
+"""
+import numpy as np
+
+from .mesh import TensorMesh
+from .survey import Survey, receivers, sources
+
+
+def dipole_dipole(xe, nmax, spacings=(1,), z=0.0):
+    """{(A, B): [(M, N), ...]} electrode-index pairs of a DD array."""
+    out = {}
+    ne = len(xe)
+    for a in spacings:
+        for i in range(ne):
+            for n in range(1, nmax + 1):
+                jm, jn = i + a + n * a, i + 2 * a + n * a
+                if jn >= ne:
+                    break
+                out.setdefault((i, i + a), []).append((jm, jn))
+    return out
+
+
+def wenner_schlumberger(xe, nmax, spacings=(1,)):
+    out = {}
+    ne = len(xe)
+    for a in spacings:
+        for n in range(1, nmax + 1):
+            for i in range(ne):
+                ib = i + (2 * n + 1) * a
+                if ib >= ne:
+                    break
+                im = i + n * a
+                out.setdefault((i, ib), []).append((im, im + a))
+    return out
+
+
+def build_survey(xe, pairs, z=0.0):
+    """Survey of dipole sources with merged receiver lists (sorted by A, B)."""
+    srcs = []
+    for (ia, ib) in sorted(pairs):
+        rx = sorted(set(pairs[(ia, ib)]))
+        M = np.array([[xe[m], z] for m, _ in rx])
+        N = np.array([[xe[n], z] for _, n in rx])
+        srcs.append(sources.Dipole([receivers.Dipole(M, N)], [xe[ia], z], [xe[ib], z]))
+    return Survey(srcs)
+
+
+def merge(*dicts):
+    out = {}
+    for d in dicts:
+        for k, v in d.items():
+            out.setdefault(k, []).extend(v)
+    return out
+
+
+def regression_case():
+    mesh = TensorMesh([[(1.0, 100)], [(1.0, 50)]], origin="CN")
+    xe = np.linspace(-30, 30, 21)
+    srcs = []
+    for i in range(len(xe) - 1):
+        lm, ln = [], []
+        for n in range(1, 7):
+            jm, jn = i + 1 + n, i + 2 + n
+            if jn >= len(xe):
+                break
+            lm.append([xe[jm], 0.0])
+            ln.append([xe[jn], 0.0])
+        if lm:
+            srcs.append(sources.Dipole([receivers.Dipole(np.array(lm), np.array(ln))],
+                                       np.r_[xe[i], 0.0], np.r_[xe[i + 1], 0.0]))
+    survey = Survey(srcs)
+    m = np.log(100.0) * np.ones(mesh.nC)
+    cc = mesh.cell_centers
+    blk = (np.abs(cc[:, 0] - 5) < 8) & (cc[:, 1] < -3) & (cc[:, 1] > -10)
+    m[blk] = np.log(10.0)
+    return mesh, survey, m
+
+
+def suite_case(seed=11):
+    rng = np.random.default_rng(seed)
+    mesh = TensorMesh([[(1.0, 64)], [(1.0, 32)]], origin="CN")
+    xe = np.linspace(-20, 20, 15)
+    srcs = []
+    for i in range(len(xe) - 1):
+        lm, ln = [], []
+        for n in range(1, 5):
+            jm, jn = i + 1 + n, i + 2 + n
+            if jn >= len(xe):
+                break
+            lm.append([xe[jm], 0.0])
+            ln.append([xe[jn], 0.0])
+        if lm:
+            srcs.append(sources.Dipole([receivers.Dipole(np.array(lm), np.array(ln))],
+                                       np.r_[xe[i], 0.0], np.r_[xe[i + 1], 0.0]))
+    m = np.log(100.0) * np.ones(mesh.nC) + 0.3 * rng.standard_normal(mesh.nC)
+    return mesh, Survey(srcs), m
+
+
+def small_padded_case(seed=0):
+    mesh = TensorMesh([[(2.5, 8, -1.3), (2.5, 80), (2.5, 8, 1.3)],
+                       [(2.5, 10, -1.3), (2.5, 40)]], origin="CN")
+    xe = np.linspace(-50, 50, 21)
+    survey = build_survey(xe, dipole_dipole(xe, 6))
+    rng = np.random.default_rng(seed)
+    cc = mesh.cell_centers
+    c1 = np.array([rng.uniform(-60, 60), rng.uniform(-60, -15)])
+    c2 = np.array([rng.uniform(-60, 60), rng.uniform(-60, -15)])
+    g1 = np.exp(-((cc - c1) ** 2).sum(1) / (2 * 8.0 ** 2))
+    g2 = np.exp(-((cc - c2) ** 2).sum(1) / (2 * 8.0 ** 2))
+    m = np.log(100.0) + np.log(10.0 / 100.0) * g1 + np.log(1000.0 / 100.0) * g2
+    return mesh, survey, m
+
+
+def gaussian_random_field(mesh, rng, corr_len, std=1.0, mean=np.log(100.0), dx=0.5):
+    """Smooth field at the cell centres: white noise on a dx grid over the mesh,
+    Gaussian filter of std corr_len/2 (covariance exp(-r^2/corr_len^2)),
+    standardised, scaled, sampled at the nearest grid point."""
+    from scipy.ndimage import gaussian_filter
+    lo = mesh.origin
+    ext = np.array([np.sum(h) for h in mesh.h])
+    shape = tuple(int(np.ceil(e / dx)) + 1 for e in ext)
+    noise = rng.standard_normal(shape)
+    f = gaussian_filter(noise, sigma=(corr_len / 2.0) / dx, mode="reflect")
+    f = (f - f.mean()) / f.std()
+    cc = mesh.cell_centers
+    idx = tuple(np.clip(np.rint((cc[:, i] - lo[i]) / dx).astype(int), 0, shape[i] - 1)
+                for i in range(cc.shape[1]))
+    return mean + std * f[idx]
+
+
+def line_25k_case(seed=1):
+    mesh = TensorMesh([[(1.0, 12, -1.3), (1.0, 200), (1.0, 12, 1.3)],
+                       [(1.0, 12, -1.3), (1.0, 100)]], origin="CN")
+    xe = np.arange(-60.0, 60.0 + 1e-9, 2.0)
+    pairs = merge(dipole_dipole(xe, 6, spacings=(1, 2, 3)),
+                  wenner_schlumberger(xe, 8, spacings=(1, 2)))
+    survey = build_survey(xe, pairs)
+    rng = np.random.default_rng(seed)
+    m = gaussian_random_field(mesh, rng, corr_len=10.0, std=1.0)
+    return mesh, survey, m, rng
+
+
+CASES = {"regression": regression_case, "suite": suite_case,
+         "small_padded": small_padded_case}
