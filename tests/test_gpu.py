@@ -1,0 +1,218 @@
+"""GPU parity: libdcx.so kernels (through the C-ABI) against the oracle.
+
+Tolerances follow the reference's tests: solver vs SciPy 1e-11
+(test_solver.py:39), value/RHS gradients vs FD 1e-5 (:78), narrow RHS 1e-13
+(:93), dpred vs the CPU path 1e-10 (test_simulation2d.py:80), gradient
+1e-9; mixed precision (FP32 factor + FP64 refinement) dpred <= 1e-6.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+import torch
+
+from helpers import case, laplacian_fixture, noisy_data, oracle_for, rel
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda"
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def _solver(A, n_rhs, nbatch=1, **kw):
+    from simpeg_dc_torch_amd import make_batch_solver
+    return make_batch_solver(torch.tensor(A.indptr, dtype=torch.int32),
+                             torch.tensor(A.indices, dtype=torch.int32), A.shape[0], n_rhs,
+                             nbatch, device=DEV, **kw)
+
+
+def test_solver_matches_scipy():
+    from simpeg_dc_torch_amd import CuDSSSolver, sparse_solve
+    _say("solver fixture nx=40, 5 rhs")
+    A, b = laplacian_fixture()
+    n = A.shape[0]
+    slv = CuDSSSolver(torch.tensor(A.indptr, dtype=torch.int32),
+                      torch.tensor(A.indices, dtype=torch.int32), n, b.shape[1], device=DEV)
+    x = sparse_solve(torch.tensor(A.data, device=DEV), torch.tensor(b, device=DEV), slv)
+    assert rel(x.cpu().numpy(), spla.spsolve(A.tocsc(), b)) < 1e-11
+    slv.free()
+
+
+def test_solver_gradients_vs_fd():
+    from simpeg_dc_torch_amd import CuDSSSolver, sparse_solve
+    A, b = laplacian_fixture(nx=12, n_rhs=3)
+    n = A.shape[0]
+    slv = CuDSSSolver(torch.tensor(A.indptr, dtype=torch.int32),
+                      torch.tensor(A.indices, dtype=torch.int32), n, 3, device=DEV)
+    rng = np.random.default_rng(1)
+    w = rng.standard_normal((n, 3))
+    vals = torch.tensor(A.data, device=DEV, requires_grad=True)
+    bt = torch.tensor(b, device=DEV, requires_grad=True)
+    (torch.tensor(w, device=DEV) * sparse_solve(vals, bt, slv)).sum().backward()
+
+    def loss(data, bb):
+        Am = sp.csr_matrix((data, A.indices, A.indptr), shape=A.shape)
+        return float((w * spla.spsolve(Am.tocsc(), bb)).sum())
+
+    eps = 1e-6
+    for p in rng.choice(A.nnz, 12, replace=False):
+        d = A.data.copy()
+        d[p] += eps
+        fd = (loss(d, b) - loss(A.data, b)) / eps
+        assert abs(fd - float(vals.grad[p])) / (abs(fd) + 1e-12) < 1e-5
+    for _ in range(6):
+        i, j = rng.integers(n), rng.integers(3)
+        bb = b.copy()
+        bb[i, j] += eps
+        fd = (loss(A.data, bb) - loss(A.data, b)) / eps
+        assert abs(fd - float(bt.grad[i, j])) / (abs(fd) + 1e-12) < 1e-5
+
+
+def test_narrow_rhs_through_wide_plan():
+    from simpeg_dc_torch_amd import CuDSSSolver, sparse_solve
+    A, b = laplacian_fixture(nx=95, n_rhs=140)
+    slv = CuDSSSolver(torch.tensor(A.indptr, dtype=torch.int32),
+                      torch.tensor(A.indices, dtype=torch.int32), A.shape[0], 140, device=DEV)
+    vals = torch.tensor(A.data, device=DEV)
+    bt = torch.tensor(b, device=DEV)
+    x = sparse_solve(vals, bt, slv)
+    xn = sparse_solve(vals, bt[:, :7], slv)
+    assert float(torch.linalg.norm(xn - x[:, :7]) / torch.linalg.norm(x[:, :7])) < 1e-13
+    assert rel(x.cpu().numpy(), spla.splu(A.tocsc()).solve(b)) < 1e-11
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-11), ("mixed", 1e-9)])
+def test_batch_solver_shared_and_batched(precision, tol):
+    A, b = laplacian_fixture(nx=30, n_rhs=9)
+    rng = np.random.default_rng(4)
+    nb = 3
+    scales = [1.0, 2.5, 0.3]
+    vals = np.stack([A.data * s + (i * 0.05) * (A.indices == np.repeat(
+        np.arange(A.shape[0]), np.diff(A.indptr))) for i, s in enumerate(scales)])
+    slv = _solver(A, 9, nbatch=nb, precision=precision, refine=2)
+    slv.factorize(torch.tensor(vals, device=DEV))
+    xs = slv.solve(torch.tensor(b, device=DEV)).cpu().numpy()
+    bb = rng.standard_normal((nb, A.shape[0], 9))
+    xb = slv.solve(torch.tensor(bb, device=DEV)).cpu().numpy()
+    for i in range(nb):
+        Ai = sp.csr_matrix((vals[i], A.indices, A.indptr), shape=A.shape)
+        lu = spla.splu(Ai.tocsc())
+        assert rel(xs[i], lu.solve(b)) < tol
+        assert rel(xb[i], lu.solve(bb[i])) < tol
+
+
+def test_sparse_rhs_support_matches_dense():
+    from simpeg_dc_torch_amd import _lib
+    A, _ = laplacian_fixture(nx=40, n_rhs=1)
+    n = A.shape[0]
+    rows = np.array([3, 17, 400, 1200, 1599])
+    b = np.zeros((n, 6))
+    rng = np.random.default_rng(2)
+    b[rows] = rng.standard_normal((rows.size, 6))
+    slv = _solver(A, 6, nbatch=2)
+    slv.factorize(torch.tensor(np.stack([A.data, 2 * A.data]), device=DEV))
+    slv.set_rhs_support(rows)
+    xd = slv.solve(torch.tensor(b, device=DEV)).cpu().numpy()
+    xs = slv.solve(torch.tensor(b, device=DEV), flags=_lib.DCX_SPARSE_RHS).cpu().numpy()
+    assert np.abs(xd - xs).max() <= 1e-14 * np.abs(xd).max()
+    assert rel(xd[0], spla.spsolve(A.tocsc(), b)) < 1e-11
+
+
+# ---------------------------------------------------------------------------
+# the engine
+# ---------------------------------------------------------------------------
+def _engine(name, nky=6, **kw):
+    from simpeg_dc_torch_amd import TorchDC2D
+    mesh, survey, m = case(name)
+    eng = TorchDC2D(mesh, survey, nky=nky, device=DEV, **kw)
+    O = oracle_for(mesh, survey, eng.kys, eng.wq_np)
+    return eng, O, m
+
+
+@pytest.mark.parametrize("name", ["suite", "regression", "small_padded"])
+def test_engine_fp64_dpred_and_gradient(name):
+    _say(f"engine fp64 {name}")
+    eng, O, m = _engine(name)
+    d_o = O.dpred(m)
+    mt = torch.tensor(m, device=DEV, requires_grad=True)
+    d = eng.dpred(mt)
+    assert rel(d.detach().cpu().numpy(), d_o) < 1e-10
+    dobs, w = noisy_data(d_o)
+    phi = eng.misfit(mt, torch.tensor(dobs, device=DEV), torch.tensor(w, device=DEV))
+    phi.backward()
+    phi_o, g_o, _ = O.misfit_and_grad(m, dobs, w)
+    assert float(phi.detach()) == pytest.approx(phi_o, rel=1e-10)
+    assert rel(mt.grad.cpu().numpy(), g_o) < 1e-9
+    f, g = eng.misfit_and_grad(torch.tensor(m, device=DEV), torch.tensor(dobs, device=DEV),
+                               torch.tensor(w, device=DEV), chunk=5)
+    assert f == pytest.approx(phi_o, rel=1e-10)
+    assert rel(g.cpu().numpy(), g_o) < 1e-9
+
+
+def test_engine_mixed_precision():
+    _say("engine mixed small_padded")
+    eng, O, m = _engine("small_padded", precision="mixed", refine=2)
+    d_o = O.dpred(m)
+    mt = torch.tensor(m, device=DEV, requires_grad=True)
+    d = eng.dpred(mt)
+    assert rel(d.detach().cpu().numpy(), d_o) < 1e-6
+    dobs, w = noisy_data(d_o)
+    eng.misfit(mt, torch.tensor(dobs, device=DEV), torch.tensor(w, device=DEV)).backward()
+    _, g_o, _ = O.misfit_and_grad(m, dobs, w)
+    assert rel(mt.grad.cpu().numpy(), g_o) < 1e-5
+
+
+def test_engine_jtvec_jmatrix_chunks():
+    _say("engine Jtvec/Jmatrix suite")
+    eng, O, m = _engine("suite", nky=4)
+    mt = torch.tensor(m, device=DEV)
+    J = eng.Jmatrix(mt, block=7).cpu().numpy()
+    assert rel(J, O.Jmatrix(m)) < 1e-9
+    rng = np.random.default_rng(2)
+    v = rng.standard_normal(eng.nD)
+    g1 = eng.Jtvec(mt, torch.tensor(v, device=DEV)).cpu().numpy()
+    g2 = eng.Jtvec(mt, torch.tensor(v, device=DEV), chunk=4).cpu().numpy()
+    assert rel(g1, J.T @ v) < 1e-10
+    assert rel(g2, g1) < 1e-12
+    u = rng.standard_normal(m.size)
+    assert rel(eng.Jvec(mt, torch.tensor(u, device=DEV)).cpu().numpy(), J @ u) < 1e-9
+
+
+def test_line25k_subset_against_oracle_and_fd():
+    """Full 25k-cell line: dpred of a subset of sources against the oracle
+    (same mesh and quadrature), and the gradient against central differences
+    of the GPU misfit itself."""
+    from simpeg_dc_torch_amd import TorchDC2D
+    from simpeg_dc_torch_amd.survey import Survey
+    _say("line 25k fp64")
+    mesh, survey, m = case("line25k")
+    eng = TorchDC2D(mesh, survey, nky=12, device=DEV)
+    mt = torch.tensor(m, device=DEV)
+    with torch.no_grad():
+        d = eng.dpred(mt).cpu().numpy()
+    pick = list(range(0, survey.nSrc, 53))
+    sub = Survey([survey.source_list[i] for i in pick])
+    O = oracle_for(mesh, sub, eng.kys, eng.wq_np)
+    sel = np.concatenate([np.where(eng._maps.sidx == i)[0] for i in pick])
+    assert rel(d[sel], O.dpred(m)) < 1e-10
+    dobs, w = noisy_data(d, rel_noise=0.05)
+    dobs_t, w_t = torch.tensor(dobs, device=DEV), torch.tensor(w, device=DEV)
+    f, g = eng.misfit_and_grad(mt, dobs_t, w_t)
+    rng = np.random.default_rng(9)
+    eps = 1e-4
+    for _ in range(2):
+        v = rng.standard_normal(m.size)
+        v /= np.linalg.norm(v)
+        vt = torch.tensor(v, device=DEV)
+        with torch.no_grad():
+            fp = float(eng.misfit(mt + eps * vt, dobs_t, w_t))
+            fm = float(eng.misfit(mt - eps * vt, dobs_t, w_t))
+        fd = (fp - fm) / (2 * eps)
+        assert abs(fd - float(g @ vt)) / abs(fd) < 1e-6
+    _say("line 25k mixed")
+    engm = TorchDC2D(mesh, survey, nky=12, device=DEV, precision="mixed", refine=2)
+    with torch.no_grad():
+        dm = engm.dpred(mt).cpu().numpy()
+    assert rel(dm, d) < 1e-6

@@ -1,0 +1,161 @@
+"""CPU checks of the product's host logic against the oracle and the fixtures."""
+import os
+import re
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+import torch
+
+import emulate
+from helpers import case, laplacian_fixture, noisy_data, oracle_for, quad, rel
+from simpeg_dc_torch_amd import TorchDC2D, _lib
+from simpeg_dc_torch_amd.operators2d import Problem2DMaps
+from simpeg_dc_torch_amd.quadrature import k0_quadrature, transform_error
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = os.path.join(ROOT, "tests", "golden", "quadrature_golden.npz")
+
+
+@pytest.mark.parametrize("name", ["suite", "small_padded", "regression"])
+def test_assembly_map_matches_oracle(name):
+    """Factored map == operator-form assembly at two models, every ky
+    (the reference validates its map the same way, simulation2d.py:211-229)."""
+    mesh, survey, m = case(name)
+    kys, wq = quad(survey, 6)
+    P = Problem2DMaps(mesh, survey, kys)
+    O = oracle_for(mesh, survey, kys, wq)
+    rng = np.random.default_rng(7)
+    for mm in (m, m + 0.5 * rng.standard_normal(m.size)):
+        sig = np.exp(-mm)
+        V = P.values(sig)
+        for k in range(len(kys)):
+            A = O.A(sig, k)
+            assert np.array_equal(A.indptr, P.crow) and np.array_equal(A.indices, P.col)
+            assert np.abs(A.data - V[k]).max() <= 1e-12 * np.abs(A.data).max()
+    assert np.abs(P.q.toarray() - O.q).max() == 0.0
+    assert abs(P.Pd - O.Pd).max() == 0.0
+    assert np.array_equal(P.sidx, O.sidx)
+
+
+def test_quadrature_golden_vectors():
+    """k0_quadrature against the reference's own outputs (tests/golden)."""
+    g = np.load(GOLDEN)
+    for i in range(int(g["ncases"][0])):
+        lo, hi, n = g[f"case{i}_in"]
+        p, w, info = k0_quadrature(lo, hi, int(n))
+        assert p.shape == g[f"case{i}_points"].shape
+        assert np.allclose(p, g[f"case{i}_points"], rtol=1e-10, atol=0)
+        assert np.allclose(w, g[f"case{i}_weights"], rtol=1e-10, atol=0)
+
+
+def test_quadrature_invariants():
+    """Error never grows with nky; weights positive; err(31) < 1e-5
+    (dctorch/tests/test_quadrature.py:27-40)."""
+    p, w, info = k0_quadrature(5.0, 120.0, 31)
+    eb = info["err_by_n"]
+    ns = sorted(eb)
+    assert max(eb[b] / eb[a] for a, b in zip(ns, ns[1:])) < 1.5
+    assert np.all(w > 0) and info["amplification"] < 1 + 1e-12
+    assert eb[31] < 1e-5
+    assert transform_error(p, w, 5.0, 120.0) == pytest.approx(eb[31])
+
+
+def test_library_exports_header():
+    """libdcx.so loads (no GPU needed) and exports every declared entry point."""
+    lib = _lib.load()
+    hdr = open(os.path.join(ROOT, "include", "dcx.h")).read()
+    names = set(re.findall(r"\b(dcx_[a-z_0-9]+)\s*\(", hdr))
+    assert len(names) >= 20
+    for nm in names:
+        assert hasattr(lib, nm), nm
+    assert set(_lib.PROTOTYPES) <= names
+
+
+@pytest.mark.parametrize("coords", [False, True])
+def test_symbolic_structure_and_emulated_solve(coords):
+    """Host analysis + a numpy run of the kernel algorithm == SciPy."""
+    mesh, survey, m = case("small_padded")
+    kys, wq = quad(survey, 3)
+    P = Problem2DMaps(mesh, survey, kys)
+    sym = _lib.symbolic(P.crow, P.col, P.node_coords() if coords else None, leaf_max=16)
+    perm = sym["perm"]
+    assert np.array_equal(np.sort(perm), np.arange(P.nN))
+    # the extend-add condition: child rows inside the parent front
+    for c in range(sym["nsup"]):
+        p = sym["parent"][c]
+        R = sym["ridx"][sym["rptr"][c]:sym["rptr"][c + 1]]
+        if p < 0:
+            assert R.size == 0
+            continue
+        F = np.r_[np.arange(sym["first"][p], sym["first"][p] + sym["ns"][p]),
+                  sym["ridx"][sym["rptr"][p]:sym["rptr"][p + 1]]]
+        rp = sym["relpos"][sym["rptr"][c]:sym["rptr"][c + 1]]
+        assert np.array_equal(F[rp], R)
+    V = P.values(np.exp(-m))
+    rng = np.random.default_rng(0)
+    B = rng.standard_normal((P.nN, 4))
+    A = sp.csr_matrix((V[0], P.col, P.crow), shape=(P.nN, P.nN))
+    Q = emulate.factor(sym, V[0])
+    X = emulate.solve(sym, Q, B[perm])
+    x = np.empty_like(X)
+    x[perm] = X
+    assert rel(x, spla.splu(A.tocsc()).solve(B)) < 1e-11
+
+
+def test_symbolic_laplacian_fixture():
+    A, b = laplacian_fixture()
+    sym = _lib.symbolic(A.indptr, A.indices, None, leaf_max=16)
+    Q = emulate.factor(sym, A.data)
+    X = emulate.solve(sym, Q, b[sym["perm"]])
+    x = np.empty_like(X)
+    x[sym["perm"]] = X
+    assert rel(x, np.linalg.solve(A.toarray(), b)) < 1e-11
+
+
+def test_symbolic_rejects_bad_patterns():
+    with pytest.raises(RuntimeError, match="symmetric"):
+        _lib.symbolic(np.array([0, 2, 3]), np.array([0, 1, 1]))
+    with pytest.raises(RuntimeError, match="diagonal"):
+        _lib.symbolic(np.array([0, 1, 2]), np.array([1, 0]))
+
+
+# ---------------------------------------------------------------------------
+# the cpu engine (reference chain: torch sparse + SuperLU) against the oracle
+# ---------------------------------------------------------------------------
+def _engine_and_oracle(name, nky=5, **kw):
+    mesh, survey, m = case(name)
+    eng = TorchDC2D(mesh, survey, nky=nky, device="cpu", **kw)
+    O = oracle_for(mesh, survey, eng.kys, eng.wq_np)
+    return eng, O, m
+
+
+def test_cpu_engine_dpred_and_gradient():
+    eng, O, m = _engine_and_oracle("suite")
+    mt = torch.tensor(m, requires_grad=True)
+    d = eng.dpred(mt)
+    d_o = O.dpred(m)
+    assert rel(d.detach().numpy(), d_o) < 1e-10
+    dobs, w = noisy_data(d_o)
+    phi = eng.misfit(mt, torch.tensor(dobs), torch.tensor(w))
+    phi.backward()
+    phi_o, g_o, _ = O.misfit_and_grad(m, dobs, w)
+    assert float(phi) == pytest.approx(phi_o, rel=1e-10)
+    assert rel(mt.grad.numpy(), g_o) < 1e-9
+    f, g = eng.misfit_and_grad(torch.tensor(m), torch.tensor(dobs), torch.tensor(w), chunk=5)
+    assert rel(g.numpy(), mt.grad.numpy()) < 1e-12
+
+
+def test_cpu_engine_jtvec_jmatrix_jvec():
+    eng, O, m = _engine_and_oracle("suite", nky=4)
+    mt = torch.tensor(m)
+    J = eng.Jmatrix(mt, block=7).numpy()
+    Jo = O.Jmatrix(m)
+    assert rel(J, Jo) < 1e-9
+    rng = np.random.default_rng(2)
+    v = rng.standard_normal(eng.nD)
+    assert rel(eng.Jtvec(mt, torch.tensor(v)).numpy(), J.T @ v) < 1e-10
+    assert rel(eng.Jtvec(mt, torch.tensor(v), chunk=4).numpy(), J.T @ v) < 1e-10
+    u = rng.standard_normal(m.size)
+    assert rel(eng.Jvec(mt, torch.tensor(u)).numpy(), J @ u) < 1e-9

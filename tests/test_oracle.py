@@ -1,0 +1,129 @@
+"""Pins for the CPU oracle (oracle/dc25d.py), independent of its own formulas.
+
+Each check is chosen so that a plausible slip in the oracle (a dropped term,
+a wrong sign, a swapped index) fails it: closed-form stencils, symmetry,
+linearity, the analytic half-space potential, dense linear algebra, finite
+differences and the adjoint identities.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from helpers import laplacian_fixture, oracle_for, quad, rel
+from oracle.dc25d import Grid2D, Problem2D, halfspace_dipole, splu_solve
+from simpeg_dc_torch_amd import workloads
+from simpeg_dc_torch_amd.mesh import TensorMesh
+from simpeg_dc_torch_amd.survey import Survey, receivers, sources
+
+
+def _tiny(nky=3):
+    mesh, survey, m = workloads.suite_case()
+    kys, wq = quad(survey, nky)
+    return mesh, survey, m, oracle_for(mesh, survey, kys, wq)
+
+
+def test_uniform_stencil_closed_form():
+    """h = 1, homogeneous sigma: interior rows are sigma*(4 + ky^2, -1 x4)."""
+    hx, hz = np.ones(12), np.ones(9)
+    sv = Survey([sources.Dipole([receivers.Dipole([[1.0, 9.0]], [[2.0, 9.0]])],
+                                [3.0, 9.0], [4.0, 9.0])])
+    ky = 0.37
+    O = Problem2D(hx, hz, (0.0, 0.0), sv, [ky], [1.0])
+    sigma = 0.02
+    A = O.A(np.full(O.g.nC, sigma), 0).toarray()
+    g = O.g
+    for ix, iz in ((5, 4), (2, 7), (9, 1)):
+        i = g.node(ix, iz)
+        assert A[i, i] == pytest.approx(sigma * (4 + ky ** 2), rel=1e-14)
+        for j in (g.node(ix - 1, iz), g.node(ix + 1, iz), g.node(ix, iz - 1), g.node(ix, iz + 1)):
+            assert A[i, j] == pytest.approx(-sigma, rel=1e-14)
+        assert np.count_nonzero(A[i]) == 5
+    # top-surface (Neumann) node: two half-weight boundary edges (one cell
+    # each), one full edge downwards, two cells of node mass, no Robin term
+    i = g.node(5, g.nz)
+    assert A[i, i] == pytest.approx(sigma * (2 + ky ** 2 / 2), rel=1e-14)
+    assert A[i, g.node(4, g.nz)] == pytest.approx(-sigma / 2, rel=1e-14)
+    assert A[i, g.node(5, g.nz - 1)] == pytest.approx(-sigma, rel=1e-14)
+
+
+def test_symmetry_linearity_row_sums():
+    mesh, survey, m, O = _tiny()
+    rng = np.random.default_rng(1)
+    s1, s2 = np.exp(-m), np.exp(-m + 0.3 * rng.standard_normal(m.size))
+    for k in range(O.nky):
+        A1, A2 = O.A(s1, k), O.A(s2, k)
+        assert abs(A1 - A1.T).max() <= 1e-15 * abs(A1).max()
+        A12 = O.A(0.7 * s1 + 1.9 * s2, k)
+        assert abs(A12 - (0.7 * A1 + 1.9 * A2)).max() <= 1e-13 * abs(A12).max()
+    L = (O.Grad.T @ sp.diags(O.Qe_ @ s1) @ O.Grad)
+    assert np.abs(L @ np.ones(O.g.nN)).max() < 1e-14 * abs(L).max()
+
+
+def test_edge_mass_and_node_mass_totals():
+    """Qe distributes vol/2 of each cell to each of its 4 edges (2 vol per cell
+    in total); Qn distributes each cell's volume over its 4 nodes."""
+    g = Grid2D(np.array([1.0, 2.0, 0.5]), np.array([3.0, 1.5]), (0.0, 0.0))
+    assert np.allclose(np.asarray(g.Qe().sum(0)).ravel(), 2 * g.vol)
+    assert np.allclose(np.asarray(g.Qn().sum(0)).ravel(), g.vol)
+
+
+def test_laplacian_fixture_solver():
+    A, b = laplacian_fixture()
+    x = splu_solve(A.indptr, A.indices, A.data, A.shape[0], b)
+    xd = np.linalg.solve(A.toarray(), b)
+    assert rel(x, xd) < 1e-11
+
+
+def _halfspace_error(h):
+    n = int(round(80 / h))
+    mesh = TensorMesh([[(h, 14, -1.3), (h, n), (h, 14, 1.3)],
+                       [(h, 14, -1.3), (h, n // 2)]], "CN")
+    xe = np.arange(-20.0, 21.0, 2.0)
+    survey = workloads.build_survey(xe, workloads.dipole_dipole(xe, 5))
+    kys, wq = quad(survey, 12)
+    O = oracle_for(mesh, survey, kys, wq)
+    rho = 100.0
+    d = O.dpred(np.full(mesh.nC, np.log(rho)))
+    ana = np.array([halfspace_dipole(rho, s.location_a, s.location_b, M, N)
+                    for s in survey.source_list
+                    for M, N in zip(s.receiver_list[0].locations_m,
+                                    s.receiver_list[0].locations_n)])
+    return np.abs(d - ana) / np.abs(ana)
+
+
+def test_halfspace_dipole_dipole_converges():
+    """Homogeneous 100 ohm-m: dpred converges to the analytic half-space value
+    (test_singularity.py:49-51) at second order in the cell size."""
+    e1 = _halfspace_error(1.0)
+    e2 = _halfspace_error(0.5)
+    assert np.median(e1) < 0.02, np.median(e1)
+    assert np.median(e2) < np.median(e1) / 3.0, (np.median(e1), np.median(e2))
+    assert e2.max() < 0.03, e2.max()
+
+
+def test_gradient_central_differences():
+    mesh, survey, m, O = _tiny(nky=4)
+    d = O.dpred(m)
+    rng = np.random.default_rng(3)
+    dobs = d * (1 + 0.03 * rng.standard_normal(d.size))
+    w = 1.0 / (0.03 * np.abs(d))
+    _, g, _ = O.misfit_and_grad(m, dobs, w)
+    eps = 1e-4
+    for _ in range(3):
+        v = rng.standard_normal(m.size)
+        v /= np.linalg.norm(v)
+        fd = (O.misfit(m + eps * v, dobs, w) - O.misfit(m - eps * v, dobs, w)) / (2 * eps)
+        assert abs(fd - g @ v) / abs(fd) < 1e-6
+
+
+def test_jmatrix_rows_and_jtvec():
+    mesh, survey, m, O = _tiny(nky=3)
+    J = O.Jmatrix(m)
+    rng = np.random.default_rng(5)
+    v = rng.standard_normal(O.nD)
+    assert rel(O.Jtvec(m, v), J.T @ v) < 1e-12
+    u = rng.standard_normal(m.size)
+    u /= np.linalg.norm(u)
+    eps = 1e-5
+    fd = (O.dpred(m + eps * u) - O.dpred(m - eps * u)) / (2 * eps)
+    assert rel(J @ u, fd) < 1e-6
