@@ -125,6 +125,11 @@ def main():
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(see DESIGN.md for how the path would shard)", file=sys.stderr)
         sys.exit(2)
+    pr = torch.cuda.get_device_properties(0)
+    log(f"device: {pr.name} CUs={pr.multi_processor_count} "
+        f"L2={getattr(pr, 'L2_cache_size', 0) >> 20} MiB "
+        f"lds/block={getattr(pr, 'shared_memory_per_block', 0) >> 10} KiB "
+        f"lds/CU={getattr(pr, 'shared_memory_per_multiprocessor', 0) >> 10} KiB")
     log("building engines (config: 25k-cell line, 12 ky)")
     mesh, survey, m, e64, emx, mt, dobs, w, t_build = build_engines(args.refine)
     info = e64.solver.info()

@@ -121,6 +121,12 @@ int dcx_set_rhs_support(dcx_plan* plan, const int32_t* rows, int nrows);
 int dcx_grad_values(dcx_plan* plan, const double* y, const double* x, int k,
                     const double* scale, double* gv, int flags, void* stream);
 
+/* Lane-layout probe of the 16x16x4 matrix instruction (fp32 = 0: FP64):
+ * C (16x16, row-major) = A (16x4, row-major) * B (4x16, row-major), one
+ * wavefront.  Device pointers.  A test of the instruction layout the sweep
+ * kernels assume; no counterpart in the reference. */
+int dcx_mma_probe(int fp32, const void* A, const void* B, void* C, void* stream);
+
 /* Row permutations between the caller's order and the elimination order:
  * out[b][new][c] = in[b][perm[new]][c] (to_elim=1) or the inverse. */
 int dcx_permute_rows(dcx_plan* plan, const double* in, double* out, int k,

@@ -55,6 +55,7 @@ PROTOTYPES = {
     "dcx_solve": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, _vp]),
     "dcx_set_rhs_support": (C.c_int, [_vp, _i32p, C.c_int]),
     "dcx_grad_values": (C.c_int, [_vp, _vp, _vp, C.c_int, _vp, _vp, C.c_int, _vp]),
+    "dcx_mma_probe": (C.c_int, [C.c_int, _vp, _vp, _vp, _vp]),
     "dcx_permute_rows": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, C.c_int, _vp]),
     "dcx_lmap_create": (C.c_int, [C.c_int, C.c_int, C.c_int, _f64p, _i32p, _i32p, _f64p,
                                   _f64p, C.c_int, _i32p, _i32p, _f64p, C.POINTER(_vp)]),

@@ -57,8 +57,18 @@ constexpr int kThreads = 256;
 
 // ---------------------------------------------------------------------------
 // numeric factorisation, one launch per depth level (deepest first)
-// grid (supernodes at level, batch), 256 threads
+// grid (supernodes at level, batch), 256 threads.
+// F (nf x nf, column-major, lower triangle used) is assembled in the depth
+// buffer, factored by panels of kNB columns staged in LDS with 4x4
+// register-tiled trailing updates, then the panel
+//     P_s = [ L_JJ^{-1} ; L_RJ ]   (nf x ns, column-major)
+// is written: the diagonal blocks of the inverse (kNBI x kNBI) in LDS, the
+// off-diagonal blocks by blocked substitution.  F's (R,R) block is left
+// holding the Schur complement (this front's update block).
 // ---------------------------------------------------------------------------
+constexpr int kNB = 16;   // panel width
+constexpr int kNBI = 64;  // diagonal block of the inverse
+
 template <typename T>
 __global__ __launch_bounds__(kThreads) void k_factor_level(
     const SupDev* __restrict__ sup, const int* __restrict__ lvl,
@@ -66,8 +76,10 @@ __global__ __launch_bounds__(kThreads) void k_factor_level(
     const int* __restrict__ a_e, const int* __restrict__ a_row,
     const int* __restrict__ a_col, const double* __restrict__ vals, int64_t nnz,
     T* __restrict__ fcur, int64_t fcur_bs, const T* __restrict__ fprev,
-    int64_t fprev_bs, T* __restrict__ panels, int64_t panel_bs,
+    int64_t fprev_bs, T* __restrict__ panels, int64_t panel_bs, int ldi,
     int* __restrict__ info) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* sm = reinterpret_cast<T*>(smem_raw);
   const int s = lvl[blockIdx.x];
   const int b = blockIdx.y;
   const SupDev S = sup[s];
@@ -75,13 +87,13 @@ __global__ __launch_bounds__(kThreads) void k_factor_level(
   const int tid = threadIdx.x;
   T* F = fcur + b * fcur_bs + S.front_off;
   const int64_t nf2 = static_cast<int64_t>(nf) * nf;
+  T* Xs = F + nf2;  // scratch of the blocked inverse (ns x ns)
   for (int64_t i = tid; i < nf2; i += kThreads) F[i] = T(0);
   __syncthreads();
   const double* v = vals + b * nnz;
   for (int q = S.a_begin + tid; q < S.a_end; q += kThreads)
     F[a_row[q] + static_cast<int64_t>(a_col[q]) * nf] = static_cast<T>(v[a_e[q]]);
   __syncthreads();
-  // extend-add of the children's update (Schur complement) blocks
   for (int cc = S.c_begin; cc < S.c_end; ++cc) {
     const SupDev C = sup[cidx[cc]];
     const int rc = C.rs, nfc = C.ns + C.rs;
@@ -96,55 +108,175 @@ __global__ __launch_bounds__(kThreads) void k_factor_level(
     }
     __syncthreads();
   }
-  // right-looking partial Cholesky of the first ns columns; the trailing
-  // (R,R) block becomes this front's update block
-  for (int j = 0; j < ns; ++j) {
-    T d = F[j + static_cast<int64_t>(j) * nf];
-    if (!(d > T(0))) {
-      if (tid == 0) atomicMax(info, 1);
-      d = T(1);
+  // ---- partial Cholesky by panels --------------------------------------
+  constexpr int LDP = kNB + 1;
+  for (int j0 = 0; j0 < ns; j0 += kNB) {
+    const int jb = min(kNB, ns - j0);
+    const int m = nf - j0;
+    for (int idx = tid; idx < m * jb; idx += kThreads) {
+      const int r = idx % m, t = idx / m;
+      sm[r * LDP + t] = F[(j0 + r) + static_cast<int64_t>(j0 + t) * nf];
     }
-    d = sqrt(d);
     __syncthreads();
-    T* colj = F + static_cast<int64_t>(j) * nf;
-    for (int i = j + tid; i < nf; i += kThreads) colj[i] = (i == j) ? d : colj[i] / d;
-    __syncthreads();
-    const int m = nf - j - 1;
-    // lower triangle of the trailing block, columns spread over warps
-    for (int l = tid / 64; l < m; l += kThreads / 64) {
-      const T lj = colj[j + 1 + l];
-      T* cl = F + static_cast<int64_t>(j + 1 + l) * nf;
-      for (int i = l + (tid & 63); i < m; i += 64) cl[j + 1 + i] -= colj[j + 1 + i] * lj;
+    for (int t = 0; t < jb; ++t) {
+      T d = sm[t * LDP + t];
+      if (!(d > T(0))) {
+        if (tid == 0) atomicMax(info, 1);
+        d = T(1);
+      }
+      d = sqrt(d);
+      __syncthreads();
+      for (int r = t + tid; r < m; r += kThreads) sm[r * LDP + t] = (r == t) ? d : sm[r * LDP + t] / d;
+      __syncthreads();
+      const int w = jb - t - 1;
+      for (int idx = tid; idx < m * w; idx += kThreads) {
+        const int r = idx % m, l = t + 1 + idx / m;
+        if (r >= l) sm[r * LDP + l] -= sm[r * LDP + t] * sm[l * LDP + t];
+      }
+      __syncthreads();
+    }
+    for (int idx = tid; idx < m * jb; idx += kThreads) {
+      const int r = idx % m, t = idx / m;
+      if (r >= t) F[(j0 + r) + static_cast<int64_t>(j0 + t) * nf] = sm[r * LDP + t];
+    }
+    const int mt = m - jb;
+    const int nt = (mt + 3) >> 2;
+    for (int tt = tid; tt < nt * nt; tt += kThreads) {
+      const int ti = tt % nt, tj = tt / nt;
+      if (ti < tj) continue;
+      T acc[4][4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int p = 0; p < 4; ++p) acc[q][p] = T(0);
+      for (int t = 0; t < jb; ++t) {
+        T av[4], bv[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int ri = ti * 4 + q, rj = tj * 4 + q;
+          av[q] = ri < mt ? sm[(jb + ri) * LDP + t] : T(0);
+          bv[q] = rj < mt ? sm[(jb + rj) * LDP + t] : T(0);
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+          for (int p = 0; p < 4; ++p) acc[q][p] += av[q] * bv[p];
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int p = 0; p < 4; ++p) {
+          const int i = ti * 4 + q, l = tj * 4 + p;
+          if (i < mt && l < mt && i >= l)
+            F[(j0 + jb + i) + static_cast<int64_t>(j0 + jb + l) * nf] -= acc[q][p];
+        }
     }
     __syncthreads();
   }
-  // Q = [L_JJ^{-1}; L_RJ L_JJ^{-1}]
+  // ---- panel: L_RJ copy, then L_JJ^{-1} -----------------------------------
   T* P = panels + b * panel_bs + S.panel_off;
-  for (int c = tid; c < ns; c += kThreads) {
-    T* pc = P + static_cast<int64_t>(c) * nf;
-    for (int i = 0; i < c; ++i) pc[i] = T(0);
-    pc[c] = T(1) / F[c + static_cast<int64_t>(c) * nf];
-    for (int i = c + 1; i < ns; ++i) {
-      T acc = T(0);
-      for (int k = c; k < i; ++k) acc += F[i + static_cast<int64_t>(k) * nf] * pc[k];
-      pc[i] = -acc / F[i + static_cast<int64_t>(i) * nf];
+  const int rs = S.rs;
+  for (int64_t idx = tid; idx < static_cast<int64_t>(rs) * ns; idx += kThreads) {
+    const int r = static_cast<int>(idx % rs), c = static_cast<int>(idx / rs);
+    P[(ns + r) + static_cast<int64_t>(c) * nf] = F[(ns + r) + static_cast<int64_t>(c) * nf];
+  }
+  T* Lb = sm;  // ldi = min(kNBI, widest supernode of the level)
+  T* Xb = sm + ldi * ldi;
+  const int nblk = (ns + kNBI - 1) / kNBI;
+  for (int ib = 0; ib < nblk; ++ib) {
+    const int o = ib * kNBI, nb = min(kNBI, ns - o);
+    __syncthreads();
+    for (int idx = tid; idx < nb * nb; idx += kThreads) {
+      const int r = idx % nb, c = idx / nb;
+      Lb[r * ldi + c] = (r >= c) ? F[(o + r) + static_cast<int64_t>(o + c) * nf] : T(0);
+    }
+    __syncthreads();
+    if (tid < nb) {  // one column per thread, rows in lockstep
+      const int c = tid;
+      for (int i = 0; i < c; ++i) Xb[i * ldi + c] = T(0);
+      Xb[c * ldi + c] = T(1) / Lb[c * ldi + c];
+      for (int i = c + 1; i < nb; ++i) {
+        T acc = T(0);
+        for (int k = c; k < i; ++k) acc += Lb[i * ldi + k] * Xb[k * ldi + c];
+        Xb[i * ldi + c] = -acc / Lb[i * ldi + i];
+      }
+    }
+    __syncthreads();
+    for (int idx = tid; idx < nb * (ns - o); idx += kThreads) {
+      const int r = idx % nb, c = o + idx / nb;
+      P[(o + r) + static_cast<int64_t>(c) * nf] = (c < o + nb) ? Xb[r * ldi + (c - o)] : T(0);
     }
   }
-  __syncthreads();
-  const int64_t rsn = static_cast<int64_t>(S.rs) * ns;
-  for (int64_t t = tid; t < rsn; t += kThreads) {
-    const int r = static_cast<int>(t % S.rs), c = static_cast<int>(t / S.rs);
-    const T* pc = P + static_cast<int64_t>(c) * nf;
-    T acc = T(0);
-    for (int k = c; k < ns; ++k) acc += F[(ns + r) + static_cast<int64_t>(k) * nf] * pc[k];
-    P[(ns + r) + static_cast<int64_t>(c) * nf] = acc;
+  // off-diagonal blocks: X_ib,<o = -X_ib,ib * (L_ib,<o X_<o,<o)
+  for (int ib = 1; ib < nblk; ++ib) {
+    const int o = ib * kNBI, nb = min(kNBI, ns - o);
+    __syncthreads();
+    for (int idx = tid; idx < nb * o; idx += kThreads) {
+      const int r = idx % nb, c = idx / nb;
+      T acc = T(0);
+      for (int k = c; k < o; ++k)
+        acc += F[(o + r) + static_cast<int64_t>(k) * nf] * P[k + static_cast<int64_t>(c) * nf];
+      Xs[r + static_cast<int64_t>(c) * nb] = acc;
+    }
+    __syncthreads();
+    for (int idx = tid; idx < nb * o; idx += kThreads) {
+      const int r = idx % nb, c = idx / nb;
+      T acc = T(0);
+      for (int q = 0; q <= r; ++q)
+        acc += P[(o + r) + static_cast<int64_t>(o + q) * nf] * Xs[q + static_cast<int64_t>(c) * nb];
+      P[(o + r) + static_cast<int64_t>(c) * nf] = -acc;
+    }
   }
 }
 
 // ---------------------------------------------------------------------------
-// forward sweep, one launch per depth level (deepest first)
-// grid (supernodes at level, batch, column tiles of TC), 256 threads
+// multi-RHS sweeps on MFMA.  One wavefront computes 16x16 output tiles with
+// the 16x16x4 matrix instruction of the element type:
+//   A operand: lane l holds A[r0 + l%16][k0 + l/16]
+//   B operand: lane l holds B[k0 + l/16][c0 + l%16]
+//   result   : lane l holds C[r0 + 4*(l/16) + i][c0 + l%16], i = 0..3
 // ---------------------------------------------------------------------------
+template <typename T>
+struct Mma;
+template <>
+struct Mma<double> {
+  typedef double v4 __attribute__((ext_vector_type(4)));
+  __device__ static inline v4 op(double a, double b, v4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+};
+template <>
+struct Mma<float> {
+  typedef float v4 __attribute__((ext_vector_type(4)));
+  __device__ static inline v4 op(float a, float b, v4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+};
+
+constexpr int kMaxTiles = 8;  // 16x16 register tiles a wave keeps across a barrier
+
+// C tile = A(rows r0.., K) * B(K, cols c0..), A read through acc(r, k) (global),
+// B from LDS with row stride TC.
+template <typename T, int TC, typename AFn>
+__device__ inline typename Mma<T>::v4 mma_tile(AFn A, int M, int K, const T* Bs, int r0,
+                                               int c0, int lane) {
+  typename Mma<T>::v4 acc = {T(0), T(0), T(0), T(0)};
+  const int rr = r0 + (lane & 15);
+  const int kq = lane >> 4;
+  const int cc = c0 + (lane & 15);
+  const bool rok = rr < M;
+  for (int k0 = 0; k0 < K; k0 += 4) {
+    const int kk = k0 + kq;
+    const bool kok = kk < K;
+    const T a = (rok && kok) ? A(rr, kk) : T(0);
+    const T bv = kok ? Bs[kk * TC + cc] : T(0);
+    acc = Mma<T>::op(a, bv, acc);
+  }
+  return acc;
+}
+
+// forward sweep (deepest level first): f = [b_J; sum children updates] in LDS,
+// y_J = L_JJ^{-1} f_J (to W), u_s = f_R - L_RJ y_J (to the update buffer)
 template <typename T, int TC>
 __global__ __launch_bounds__(kThreads) void k_fwd_level(
     const SupDev* __restrict__ sup, const int* __restrict__ lvl,
@@ -155,17 +287,17 @@ __global__ __launch_bounds__(kThreads) void k_fwd_level(
     int64_t uprev_bs) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T* f = reinterpret_cast<T*>(smem_raw);
-  constexpr int NY = kThreads / TC;
   const int s = lvl[blockIdx.x];
   const int b = blockIdx.y;
-  const int tx = threadIdx.x % TC, ty = threadIdx.x / TC;
-  const int col = blockIdx.z * TC + tx;
-  const bool cok = col < k;
+  const int col0 = blockIdx.z * TC;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const SupDev S = sup[s];
-  const int ns = S.ns, nf = S.ns + S.rs;
+  const int ns = S.ns, rs = S.rs, nf = S.ns + S.rs;
   T* Wb = W + b * w_bs;
-  for (int i = ty; i < nf; i += NY)
-    f[i * TC + tx] = (i < ns && cok) ? Wb[static_cast<int64_t>(S.first + i) * k + col] : T(0);
+  for (int idx = tid; idx < nf * TC; idx += kThreads) {
+    const int r = idx / TC, c = idx % TC;
+    f[idx] = (r < ns && col0 + c < k) ? Wb[static_cast<int64_t>(S.first + r) * k + col0 + c] : T(0);
+  }
   __syncthreads();
   for (int cc = S.c_begin; cc < S.c_end; ++cc) {
     const int c = cidx[cc];
@@ -173,25 +305,58 @@ __global__ __launch_bounds__(kThreads) void k_fwd_level(
     const SupDev C = sup[c];
     const T* u = uprev + b * uprev_bs + C.upd_off * k;
     const int* rp = relpos + C.rptr;
-    for (int i = ty; i < C.rs; i += NY)
-      if (cok) f[rp[i] * TC + tx] += u[static_cast<int64_t>(i) * k + col];
+    for (int idx = tid; idx < C.rs * TC; idx += kThreads) {
+      const int r = idx / TC, cl = idx % TC;
+      if (col0 + cl < k) f[rp[r] * TC + cl] += u[static_cast<int64_t>(r) * k + col0 + cl];
+    }
     __syncthreads();
   }
   const T* P = panels + b * panel_bs + S.panel_off;
+  auto Linv = [=](int r, int kk) { return P[r + static_cast<int64_t>(kk) * nf]; };
+  auto Lrj = [=](int r, int kk) { return P[(ns + r) + static_cast<int64_t>(kk) * nf]; };
+  constexpr int NCT = TC / 16;
+  const int nt1 = ((ns + 15) / 16) * NCT;
+  typename Mma<T>::v4 acc[kMaxTiles];
+#pragma unroll
+  for (int q = 0; q < kMaxTiles; ++q) {
+    const int t = wave + 4 * q;
+    if (t < nt1) acc[q] = mma_tile<T, TC>(Linv, ns, ns, f, (t / NCT) * 16, (t % NCT) * 16, lane);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < kMaxTiles; ++q) {
+    const int t = wave + 4 * q;
+    if (t < nt1) {
+      const int r0 = (t / NCT) * 16 + 4 * (lane >> 4), c = (t % NCT) * 16 + (lane & 15);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int r = r0 + i;
+        if (r < ns) {
+          f[r * TC + c] = acc[q][i];
+          if (col0 + c < k) Wb[static_cast<int64_t>(S.first + r) * k + col0 + c] = acc[q][i];
+        }
+      }
+    }
+  }
+  if (rs == 0) return;
+  __syncthreads();
   T* uo = ucur + b * ucur_bs + S.upd_off * k;
-  for (int r = ty; r < nf; r += NY) {
-    T acc = T(0);
-    for (int kk = 0; kk < ns; ++kk) acc += P[r + static_cast<int64_t>(kk) * nf] * f[kk * TC + tx];
-    if (cok) {
-      if (r < ns)
-        Wb[static_cast<int64_t>(S.first + r) * k + col] = acc;
-      else
-        uo[static_cast<int64_t>(r - ns) * k + col] = f[r * TC + tx] - acc;
+  const int nt2 = ((rs + 15) / 16) * NCT;
+  for (int t = wave; t < nt2; t += 4) {
+    const int tr = (t / NCT) * 16, tc = (t % NCT) * 16;
+    typename Mma<T>::v4 a2 = mma_tile<T, TC>(Lrj, rs, ns, f, tr, tc, lane);
+    const int r0 = tr + 4 * (lane >> 4), c = tc + (lane & 15);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int r = r0 + i;
+      if (r < rs && col0 + c < k)
+        uo[static_cast<int64_t>(r) * k + col0 + c] = f[(ns + r) * TC + c] - a2[i];
     }
   }
 }
 
-// backward sweep, one launch per depth level (root first)
+// backward sweep (root first): g = [y_J; x_R] in LDS,
+// t = y_J - L_RJ^T x_R, x_J = L_JJ^{-T} t (to W)
 template <typename T, int TC>
 __global__ __launch_bounds__(kThreads) void k_bwd_level(
     const SupDev* __restrict__ sup, const int* __restrict__ lvl,
@@ -200,34 +365,71 @@ __global__ __launch_bounds__(kThreads) void k_bwd_level(
     int64_t w_bs, int k) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T* g = reinterpret_cast<T*>(smem_raw);
-  constexpr int NY = kThreads / TC;
   const int s = lvl[blockIdx.x];
   const int b = blockIdx.y;
-  const int tx = threadIdx.x % TC, ty = threadIdx.x / TC;
-  const int col = blockIdx.z * TC + tx;
-  const bool cok = col < k;
+  const int col0 = blockIdx.z * TC;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const SupDev S = sup[s];
-  const int ns = S.ns, nf = S.ns + S.rs;
+  const int ns = S.ns, rs = S.rs, nf = S.ns + S.rs;
   const bool act = !active || active[s];
   T* Wb = W + b * w_bs;
-  for (int i = ty; i < nf; i += NY) {
+  for (int idx = tid; idx < nf * TC; idx += kThreads) {
+    const int r = idx / TC, c = idx % TC;
     T val = T(0);
-    if (cok) {
-      if (i < ns)
-        val = act ? Wb[static_cast<int64_t>(S.first + i) * k + col] : T(0);
+    if (col0 + c < k) {
+      if (r < ns)
+        val = act ? Wb[static_cast<int64_t>(S.first + r) * k + col0 + c] : T(0);
       else
-        val = -Wb[static_cast<int64_t>(ridx[S.rptr + i - ns]) * k + col];
+        val = Wb[static_cast<int64_t>(ridx[S.rptr + r - ns]) * k + col0 + c];
     }
-    g[i * TC + tx] = val;
+    g[idx] = val;
   }
   __syncthreads();
   const T* P = panels + b * panel_bs + S.panel_off;
-  for (int i = ty; i < ns; i += NY) {
-    const T* pc = P + static_cast<int64_t>(i) * nf;
-    T acc = T(0);
-    for (int kk = 0; kk < nf; ++kk) acc += pc[kk] * g[kk * TC + tx];
-    if (cok) Wb[static_cast<int64_t>(S.first + i) * k + col] = acc;
+  auto LrjT = [=](int i, int kk) { return P[(ns + kk) + static_cast<int64_t>(i) * nf]; };
+  auto LinvT = [=](int i, int kk) { return P[kk + static_cast<int64_t>(i) * nf]; };
+  constexpr int NCT = TC / 16;
+  const int nt1 = ((ns + 15) / 16) * NCT;
+  if (rs > 0) {
+    typename Mma<T>::v4 acc[kMaxTiles];
+#pragma unroll
+    for (int q = 0; q < kMaxTiles; ++q) {
+      const int t = wave + 4 * q;
+      if (t < nt1) acc[q] = mma_tile<T, TC>(LrjT, ns, rs, g + ns * TC, (t / NCT) * 16, (t % NCT) * 16, lane);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < kMaxTiles; ++q) {
+      const int t = wave + 4 * q;
+      if (t < nt1) {
+        const int r0 = (t / NCT) * 16 + 4 * (lane >> 4), c = (t % NCT) * 16 + (lane & 15);
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+          if (r0 + i < ns) g[(r0 + i) * TC + c] -= acc[q][i];
+      }
+    }
+    __syncthreads();
   }
+  for (int t = wave; t < nt1; t += 4) {
+    const int tr = (t / NCT) * 16, tc = (t % NCT) * 16;
+    typename Mma<T>::v4 a2 = mma_tile<T, TC>(LinvT, ns, ns, g, tr, tc, lane);
+    const int r0 = tr + 4 * (lane >> 4), c = tc + (lane & 15);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int r = r0 + i;
+      if (r < ns && col0 + c < k) Wb[static_cast<int64_t>(S.first + r) * k + col0 + c] = a2[i];
+    }
+  }
+}
+
+// lane-layout probe of the 16x16x4 matrix instruction: C = A(16x4) B(4x16)
+template <typename T>
+__global__ void k_mma_probe(const T* __restrict__ A, const T* __restrict__ B, T* __restrict__ C) {
+  const int lane = threadIdx.x;
+  typename Mma<T>::v4 acc = {T(0), T(0), T(0), T(0)};
+  acc = Mma<T>::op(A[(lane & 15) * 4 + (lane >> 4)], B[(lane >> 4) * 16 + (lane & 15)], acc);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) C[(4 * (lane >> 4) + i) * 16 + (lane & 15)] = acc[i];
 }
 
 // ---------------------------------------------------------------------------
@@ -289,22 +491,37 @@ __global__ void k_residual(const int* __restrict__ crow, const int* __restrict__
   }
 }
 
-// gv[b][e] = -scale[b] sum_c Y[b][row][c] X[b][col_e][c]; one wavefront per row
+// gv[b][e] = -scale[b] sum_c Y[b][row][c] X[b][col_e][c]; one wavefront per
+// row of the caller's pattern, rows taken in the pattern's own (geometric)
+// order so that neighbouring rows share their X rows in L2.  The hardware
+// deals workgroups round-robin to the 8 XCDs; the block index is remapped so
+// that each XCD walks one contiguous range of rows (its own L2).  gridDim.x
+// must be a multiple of 8.  iperm (may be null) maps rows to the storage
+// order of Y and X.
 __global__ void k_grad_values(const int* __restrict__ crow, const int* __restrict__ ccol,
-                              const int* __restrict__ eidx, const double* __restrict__ Y,
+                              const int* __restrict__ iperm, const double* __restrict__ Y,
                               const double* __restrict__ X, int n, int k,
                               const double* __restrict__ scale, double* __restrict__ gv,
                               int64_t nnz) {
   constexpr int MAXE = 8;
-  const int row = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+  const int per = gridDim.x >> 3;
+  const int lb = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+  const int row = lb * (blockDim.x / 64) + threadIdx.x / 64;
   const int b = blockIdx.y;
   const int lane = threadIdx.x & 63;
   if (row >= n) return;
   const int64_t xb = static_cast<int64_t>(b) * n * k;
-  const double* y = Y + xb + static_cast<int64_t>(row) * k;
+  const int prow = iperm ? iperm[row] : row;
+  const double* y = Y + xb + static_cast<int64_t>(prow) * k;
   const double sc = scale ? scale[b] : 1.0;
   for (int p0 = crow[row]; p0 < crow[row + 1]; p0 += MAXE) {
     const int ne = min(MAXE, crow[row + 1] - p0);
+    const double* xr[MAXE];
+#pragma unroll
+    for (int q = 0; q < MAXE; ++q) {
+      const int cq = q < ne ? ccol[p0 + q] : ccol[p0];
+      xr[q] = X + xb + static_cast<int64_t>(iperm ? iperm[cq] : cq) * k;
+    }
     double acc[MAXE];
 #pragma unroll
     for (int q = 0; q < MAXE; ++q) acc[q] = 0.0;
@@ -312,13 +529,13 @@ __global__ void k_grad_values(const int* __restrict__ crow, const int* __restric
       const double yv = y[c];
 #pragma unroll
       for (int q = 0; q < MAXE; ++q)
-        if (q < ne) acc[q] += yv * X[xb + static_cast<int64_t>(ccol[p0 + q]) * k + c];
+        if (q < ne) acc[q] += yv * xr[q][c];
     }
 #pragma unroll
     for (int q = 0; q < MAXE; ++q) {
       double a = acc[q];
       for (int off = 32; off > 0; off >>= 1) a += __shfl_xor(a, off, 64);
-      if (lane == 0 && q < ne) gv[b * nnz + (eidx ? eidx[p0 + q] : p0 + q)] = -sc * a;
+      if (lane == 0 && q < ne) gv[b * nnz + p0 + q] = -sc * a;
     }
   }
 }
@@ -446,7 +663,8 @@ struct dcx_plan {
   SupDev* d_sup = nullptr;
   int *d_lvl = nullptr, *d_relpos = nullptr, *d_cidx = nullptr, *d_ridx = nullptr;
   int *d_a_e = nullptr, *d_a_row = nullptr, *d_a_col = nullptr;
-  int *d_perm = nullptr, *d_pcrow = nullptr, *d_pcol = nullptr, *d_pvidx = nullptr;
+  int *d_perm = nullptr, *d_iperm = nullptr, *d_pcrow = nullptr, *d_pcol = nullptr,
+      *d_pvidx = nullptr;
   int *d_crow = nullptr, *d_col = nullptr;
   int* d_info = nullptr;
   // factor
@@ -471,7 +689,7 @@ struct dcx_plan {
 
   ~dcx_plan() {
     void* ptrs[] = {d_sup, d_lvl, d_relpos, d_cidx, d_ridx, d_a_e, d_a_row, d_a_col,
-                    d_perm, d_pcrow, d_pcol, d_pvidx, d_crow, d_col, d_info, d_panels,
+                    d_perm, d_iperm, d_pcrow, d_pcol, d_pvidx, d_crow, d_col, d_info, d_panels,
                     d_front[0], d_front[1], d_vals, d_active, d_act_lvl, d_upd[0],
                     d_upd[1], d_w, d_r, d_b, d_x};
     for (void* p : ptrs)
@@ -492,6 +710,8 @@ int ensure(void** p, size_t* cap, size_t bytes, hipStream_t st) {
   return DCX_OK;
 }
 
+constexpr size_t kLdsBudget = 64 * 1024;
+
 template <typename T>
 int launch_factor(dcx_plan* P, hipStream_t st) {
   const auto& S = P->S;
@@ -501,12 +721,22 @@ int launch_factor(dcx_plan* P, hipStream_t st) {
   for (int d = S.ndepth - 1; d >= 0; --d) {
     const int cnt = S.lvl_ptr[d + 1] - S.lvl_ptr[d];
     if (!cnt) continue;
+    int maxnf = 0, maxns = 0;
+    for (int q = S.lvl_ptr[d]; q < S.lvl_ptr[d + 1]; ++q) {
+      const int s = S.lvl_sup[q];
+      maxnf = std::max(maxnf, S.ns[s] + S.rs[s]);
+      maxns = std::max(maxns, S.ns[s]);
+    }
+    const int ldi = std::min(kNBI, maxns);
+    const size_t lds = std::max<size_t>(static_cast<size_t>(maxnf) * (kNB + 1),
+                                        2 * static_cast<size_t>(ldi) * ldi) * sizeof(T);
+    if (lds > kLdsBudget) return fail(DCX_ERR_ARG, "front too large for the factor kernel");
     dim3 grid(cnt, P->nbatch);
-    hipLaunchKernelGGL(k_factor_level<T>, grid, dim3(kThreads), 0, st, P->d_sup,
+    hipLaunchKernelGGL(k_factor_level<T>, grid, dim3(kThreads), lds, st, P->d_sup,
                        P->d_lvl + S.lvl_ptr[d], P->d_relpos, P->d_cidx, P->d_a_e,
                        P->d_a_row, P->d_a_col, P->d_vals, static_cast<int64_t>(S.nnz),
                        fr[d & 1], fbs[d & 1], fr[(d + 1) & 1], fbs[(d + 1) & 1], panels,
-                       S.panel_size, P->d_info);
+                       S.panel_size, ldi, P->d_info);
   }
   HIP_TRY(hipGetLastError());
   return DCX_OK;
@@ -532,10 +762,13 @@ void sweep_tc(dcx_plan* P, T* W, int k, int maxnf, bool fwd, int d, const int* l
   }
 }
 
-int pick_tc(int maxnf, size_t tsize) {
-  const size_t budget = 64 * 1024;
-  for (int tc : {64, 32, 16, 8})
-    if (static_cast<size_t>(maxnf) * tc * tsize <= budget) return tc;
+// widest column tile whose LDS block fits and whose register tiles fit
+int pick_tc(int maxnf, int maxns, size_t tsize) {
+  for (int tc : {64, 32, 16}) {
+    const int tiles = ((maxns + 15) / 16) * (tc / 16);
+    if (static_cast<size_t>(maxnf) * tc * tsize <= kLdsBudget && (tiles + 3) / 4 <= kMaxTiles)
+      return tc;
+  }
   return 0;
 }
 
@@ -569,19 +802,19 @@ int sweeps(dcx_plan* P, T* W, int k, bool sparse_rhs, hipStream_t st) {
         cnt = S.lvl_ptr[d + 1] - S.lvl_ptr[d];
       }
       if (!cnt) continue;
-      int maxnf = 0;
+      int maxnf = 0, maxns = 0;
       for (int q = S.lvl_ptr[d]; q < S.lvl_ptr[d + 1]; ++q) {
         int s = S.lvl_sup[q];
         maxnf = std::max(maxnf, S.ns[s] + S.rs[s]);
+        maxns = std::max(maxns, S.ns[s]);
       }
-      const int tc = pick_tc(maxnf, sizeof(T));
+      const int tc = pick_tc(maxnf, maxns, sizeof(T));
       if (!tc) return fail(DCX_ERR_ARG, "front too large for the in-LDS sweep");
       const int* act = use_act ? P->d_active : nullptr;
       switch (tc) {
         case 64: sweep_tc<T, 64>(P, W, k, maxnf, fwd, d, lvl, cnt, act, st); break;
         case 32: sweep_tc<T, 32>(P, W, k, maxnf, fwd, d, lvl, cnt, act, st); break;
-        case 16: sweep_tc<T, 16>(P, W, k, maxnf, fwd, d, lvl, cnt, act, st); break;
-        default: sweep_tc<T, 8>(P, W, k, maxnf, fwd, d, lvl, cnt, act, st); break;
+        default: sweep_tc<T, 16>(P, W, k, maxnf, fwd, d, lvl, cnt, act, st); break;
       }
     }
   }
@@ -648,6 +881,7 @@ int dcx_plan_create(int n, const int32_t* crow, const int32_t* col, int nbatch,
     HIP_TRY(upload(&P->d_a_row, S.a_row));
     HIP_TRY(upload(&P->d_a_col, S.a_col));
     HIP_TRY(upload(&P->d_perm, S.perm));
+    HIP_TRY(upload(&P->d_iperm, S.iperm));
     HIP_TRY(upload(&P->d_pcrow, S.pcrow));
     HIP_TRY(upload(&P->d_pcol, S.pcol));
     HIP_TRY(upload(&P->d_pvidx, S.pval_idx));
@@ -828,14 +1062,27 @@ int dcx_grad_values(dcx_plan* P, const double* y, const double* x, int k,
   hipStream_t st = static_cast<hipStream_t>(stream);
   HIP_TRY(hipSetDevice(P->device));
   const auto& S = P->S;
-  dim3 grid(grid1(S.n, 4), P->nbatch);
-  if (flags & DCX_PERMUTED)
-    hipLaunchKernelGGL(k_grad_values, grid, dim3(256), 0, st, P->d_pcrow, P->d_pcol,
-                       P->d_pvidx, y, x, S.n, k, scale, gv, static_cast<int64_t>(S.nnz));
+  const int blocks = ((grid1(S.n, 4) + 7) / 8) * 8;  // XCD remap needs a multiple of 8
+  dim3 grid(blocks, P->nbatch);
+  hipLaunchKernelGGL(k_grad_values, grid, dim3(256), 0, st, P->d_crow, P->d_col,
+                     (flags & DCX_PERMUTED) ? static_cast<const int*>(P->d_iperm)
+                                            : static_cast<const int*>(nullptr),
+                     y, x, S.n, k, scale, gv, static_cast<int64_t>(S.nnz));
+  HIP_TRY(hipGetLastError());
+  return DCX_OK;
+}
+
+int dcx_mma_probe(int fp32, const void* A, const void* B, void* C, void* stream) {
+  if (!A || !B || !C) return fail(DCX_ERR_ARG, "null argument");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (fp32)
+    hipLaunchKernelGGL(k_mma_probe<float>, dim3(1), dim3(64), 0, st,
+                       static_cast<const float*>(A), static_cast<const float*>(B),
+                       static_cast<float*>(C));
   else
-    hipLaunchKernelGGL(k_grad_values, grid, dim3(256), 0, st, P->d_crow, P->d_col,
-                       static_cast<const int*>(nullptr), y, x, S.n, k, scale, gv,
-                       static_cast<int64_t>(S.nnz));
+    hipLaunchKernelGGL(k_mma_probe<double>, dim3(1), dim3(64), 0, st,
+                       static_cast<const double*>(A), static_cast<const double*>(B),
+                       static_cast<double*>(C));
   HIP_TRY(hipGetLastError());
   return DCX_OK;
 }

@@ -435,7 +435,7 @@ std::string analyse(int n, const int* crow, const int* col, const double* coords
     double a = S.ns[s], r = S.rs[s];
     S.nnz_L += static_cast<int64_t>(a * (a + 1) / 2 + r * a);
     S.factor_flops += a * a * a / 3.0 + r * a * a + r * r * a  // partial Cholesky
-                      + a * a * a / 3.0 + r * a * a;           // inverse and Q_R
+                      + a * a * a / 3.0;                       // inverse of L_JJ
     S.solve_flops_per_rhs += 4.0 * (a + r) * a;
   }
   S.panel_size = po;
@@ -445,7 +445,7 @@ std::string analyse(int n, const int* crow, const int* col, const double* coords
       int s = S.lvl_sup[q];
       int64_t nf = S.ns[s] + S.rs[s];
       S.front_off[s] = fo;
-      fo += nf * nf;
+      fo += nf * nf + static_cast<int64_t>(S.ns[s]) * S.ns[s];  // front + inverse scratch
       S.upd_off[s] = uo;
       uo += S.rs[s];
     }

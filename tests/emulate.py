@@ -1,7 +1,7 @@
 """Numpy emulation of the GPU factor and sweep kernels, step for step.
 
 Test helper: it runs the multifrontal factorisation (extend-add, partial
-Cholesky, Q = [L_JJ^-1; L_RJ L_JJ^-1]) and the two sweeps exactly as the
+Cholesky, panel P = [L_JJ^-1; L_RJ]) and the two sweeps exactly as the
 kernels in csrc/dcx.hip do, level by level, on the structure that the host
 symbolic analysis returns.  A mistake in the structure (fronts, relative
 positions, entry maps) or in the sweep algebra shows up here, on the CPU.
@@ -32,7 +32,7 @@ def factor(sym, vals):
                 F[j + 1:, j + 1:] -= np.tril(np.outer(F[j + 1:, j], F[j + 1:, j]))
             L_JJ = np.tril(F[:ns, :ns])
             Linv = np.linalg.inv(L_JJ)
-            Q[s] = np.vstack([np.tril(Linv), F[ns:, :ns] @ np.tril(Linv)])
+            Q[s] = np.vstack([np.tril(Linv), F[ns:, :ns]])
             fronts[s] = F
     return Q
 
@@ -50,13 +50,13 @@ def solve(sym, Q, B):
             for c in sym["cidx"][sym["cptr"][s]:sym["cptr"][s + 1]]:
                 rp = sym["relpos"][sym["rptr"][c]:sym["rptr"][c + 1]]
                 f[rp] += upd[c]
-            out = Q[s] @ f[:ns]
-            W[first[s]:first[s] + ns] = out[:ns]
-            upd[s] = f[ns:] - out[ns:]
+            y = Q[s][:ns] @ f[:ns]
+            W[first[s]:first[s] + ns] = y
+            upd[s] = f[ns:] - Q[s][ns:] @ y
     for d in range(sym["ndepth"]):
         for s in sym["lvl_sup"][sym["lvl_ptr"][d]:sym["lvl_ptr"][d + 1]]:
             ns, rs = ns_[s], rs_[s]
             R = sym["ridx"][sym["rptr"][s]:sym["rptr"][s + 1]]
-            g = np.vstack([W[first[s]:first[s] + ns], -W[R]])
-            W[first[s]:first[s] + ns] = Q[s].T @ g
+            t = W[first[s]:first[s] + ns] - Q[s][ns:].T @ W[R]
+            W[first[s]:first[s] + ns] = Q[s][:ns].T @ t
     return W

@@ -28,6 +28,24 @@ def _solver(A, n_rhs, nbatch=1, **kw):
                              nbatch, device=DEV, **kw)
 
 
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_mfma_lane_layout(dtype):
+    """The 16x16x4 matrix instruction computes A(16x4) B(4x16) with the lane
+    layout the sweep kernels assume (checked in bounds, one wavefront)."""
+    from simpeg_dc_torch_amd import _lib
+    lib = _lib.load()
+    g = torch.Generator().manual_seed(0)
+    A = torch.randn(16, 4, generator=g, dtype=dtype).to(DEV)
+    B = torch.randn(4, 16, generator=g, dtype=dtype).to(DEV)
+    C = torch.full((16, 16), float("nan"), dtype=dtype, device=DEV)
+    _lib.check(lib.dcx_mma_probe(int(dtype == torch.float32), _lib.ptr(A), _lib.ptr(B),
+                                 _lib.ptr(C), _lib.stream_of(torch.device(DEV))), "mma_probe")
+    torch.cuda.synchronize()
+    ref = A.double().cpu() @ B.double().cpu()
+    tol = 1e-13 if dtype == torch.float64 else 1e-5
+    assert float((C.double().cpu() - ref).abs().max()) < tol
+
+
 def test_solver_matches_scipy():
     from simpeg_dc_torch_amd import CuDSSSolver, sparse_solve
     _say("solver fixture nx=40, 5 rhs")
