@@ -234,7 +234,9 @@ __global__ __launch_bounds__(kThreads) void k_factor_level(
 // the 16x16x4 matrix instruction of the element type:
 //   A operand: lane l holds A[r0 + l%16][k0 + l/16]
 //   B operand: lane l holds B[k0 + l/16][c0 + l%16]
-//   result   : lane l holds C[r0 + 4*(l/16) + i][c0 + l%16], i = 0..3
+//   result   : lane l, register i holds C[r0 + Mma<T>::row(l, i)][c0 + l%16]
+//              (FP32: row 4*(l/16) + i; FP64: row l/16 + 4*i, as measured by
+//              the dcx_mma_probe layout test)
 // ---------------------------------------------------------------------------
 template <typename T>
 struct Mma;
@@ -244,6 +246,7 @@ struct Mma<double> {
   __device__ static inline v4 op(double a, double b, v4 c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
+  __device__ static inline int row(int lane, int i) { return (lane >> 4) + 4 * i; }
 };
 template <>
 struct Mma<float> {
@@ -251,6 +254,7 @@ struct Mma<float> {
   __device__ static inline v4 op(float a, float b, v4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
+  __device__ static inline int row(int lane, int i) { return 4 * (lane >> 4) + i; }
 };
 
 constexpr int kMaxTiles = 8;  // 16x16 register tiles a wave keeps across a barrier
@@ -327,10 +331,10 @@ __global__ __launch_bounds__(kThreads) void k_fwd_level(
   for (int q = 0; q < kMaxTiles; ++q) {
     const int t = wave + 4 * q;
     if (t < nt1) {
-      const int r0 = (t / NCT) * 16 + 4 * (lane >> 4), c = (t % NCT) * 16 + (lane & 15);
+      const int r0 = (t / NCT) * 16, c = (t % NCT) * 16 + (lane & 15);
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
-        const int r = r0 + i;
+        const int r = r0 + Mma<T>::row(lane, i);
         if (r < ns) {
           f[r * TC + c] = acc[q][i];
           if (col0 + c < k) Wb[static_cast<int64_t>(S.first + r) * k + col0 + c] = acc[q][i];
@@ -345,10 +349,10 @@ __global__ __launch_bounds__(kThreads) void k_fwd_level(
   for (int t = wave; t < nt2; t += 4) {
     const int tr = (t / NCT) * 16, tc = (t % NCT) * 16;
     typename Mma<T>::v4 a2 = mma_tile<T, TC>(Lrj, rs, ns, f, tr, tc, lane);
-    const int r0 = tr + 4 * (lane >> 4), c = tc + (lane & 15);
+    const int c = tc + (lane & 15);
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      const int r = r0 + i;
+      const int r = tr + Mma<T>::row(lane, i);
       if (r < rs && col0 + c < k)
         uo[static_cast<int64_t>(r) * k + col0 + c] = f[(ns + r) * TC + c] - a2[i];
     }
@@ -402,10 +406,12 @@ __global__ __launch_bounds__(kThreads) void k_bwd_level(
     for (int q = 0; q < kMaxTiles; ++q) {
       const int t = wave + 4 * q;
       if (t < nt1) {
-        const int r0 = (t / NCT) * 16 + 4 * (lane >> 4), c = (t % NCT) * 16 + (lane & 15);
+        const int r0 = (t / NCT) * 16, c = (t % NCT) * 16 + (lane & 15);
 #pragma unroll
-        for (int i = 0; i < 4; ++i)
-          if (r0 + i < ns) g[(r0 + i) * TC + c] -= acc[q][i];
+        for (int i = 0; i < 4; ++i) {
+          const int r = r0 + Mma<T>::row(lane, i);
+          if (r < ns) g[r * TC + c] -= acc[q][i];
+        }
       }
     }
     __syncthreads();
@@ -413,10 +419,10 @@ __global__ __launch_bounds__(kThreads) void k_bwd_level(
   for (int t = wave; t < nt1; t += 4) {
     const int tr = (t / NCT) * 16, tc = (t % NCT) * 16;
     typename Mma<T>::v4 a2 = mma_tile<T, TC>(LinvT, ns, ns, g, tr, tc, lane);
-    const int r0 = tr + 4 * (lane >> 4), c = tc + (lane & 15);
+    const int c = tc + (lane & 15);
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      const int r = r0 + i;
+      const int r = tr + Mma<T>::row(lane, i);
       if (r < ns && col0 + c < k) Wb[static_cast<int64_t>(S.first + r) * k + col0 + c] = a2[i];
     }
   }
@@ -429,7 +435,7 @@ __global__ void k_mma_probe(const T* __restrict__ A, const T* __restrict__ B, T*
   typename Mma<T>::v4 acc = {T(0), T(0), T(0), T(0)};
   acc = Mma<T>::op(A[(lane & 15) * 4 + (lane >> 4)], B[(lane >> 4) * 16 + (lane & 15)], acc);
 #pragma unroll
-  for (int i = 0; i < 4; ++i) C[(4 * (lane >> 4) + i) * 16 + (lane & 15)] = acc[i];
+  for (int i = 0; i < 4; ++i) C[Mma<T>::row(lane, i) * 16 + (lane & 15)] = acc[i];
 }
 
 // ---------------------------------------------------------------------------
