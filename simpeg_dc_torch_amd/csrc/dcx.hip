@@ -259,25 +259,36 @@ struct Mma<float> {
 
 constexpr int kMaxTiles = 8;  // 16x16 register tiles a wave keeps across a barrier
 
-// C tile = A(rows r0.., K) * B(K, cols c0..), A read through acc(r, k) (global),
-// B from LDS with row stride TC.
-template <typename T, int TC, typename AFn>
-__device__ inline typename Mma<T>::v4 mma_tile(AFn A, int M, int K, const T* Bs, int r0,
-                                               int c0, int lane) {
+// C tile = A(rows r0.., K) * B(K, cols c0..), A read through A(r, k) (global),
+// B from LDS rows brow(k) with row stride TC.  Operands are fetched eight
+// k-steps at a time so that the global loads of a chunk are in flight together.
+template <typename T, int TC, typename AFn, typename BRow>
+__device__ inline typename Mma<T>::v4 mma_tile(AFn A, int M, int K, const T* Bs, BRow brow,
+                                               int r0, int c0, int lane) {
   typename Mma<T>::v4 acc = {T(0), T(0), T(0), T(0)};
   const int rr = r0 + (lane & 15);
   const int kq = lane >> 4;
   const int cc = c0 + (lane & 15);
   const bool rok = rr < M;
-  for (int k0 = 0; k0 < K; k0 += 4) {
-    const int kk = k0 + kq;
-    const bool kok = kk < K;
-    const T a = (rok && kok) ? A(rr, kk) : T(0);
-    const T bv = kok ? Bs[kk * TC + cc] : T(0);
-    acc = Mma<T>::op(a, bv, acc);
+  for (int k0 = 0; k0 < K; k0 += 32) {
+    T av[8], bv[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int kk = k0 + 4 * j + kq;
+      const bool kok = kk < K;
+      av[j] = (rok && kok) ? A(rr, kk) : T(0);
+      bv[j] = kok ? Bs[brow(kk) * TC + cc] : T(0);
+    }
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+      if (k0 + 4 * j < K) acc = Mma<T>::op(av[j], bv[j], acc);
   }
   return acc;
 }
+
+struct RowId {
+  __device__ inline int operator()(int k) const { return k; }
+};
 
 // forward sweep (deepest level first): f = [b_J; sum children updates] in LDS,
 // y_J = L_JJ^{-1} f_J (to W), u_s = f_R - L_RJ y_J (to the update buffer)
@@ -286,7 +297,8 @@ __global__ __launch_bounds__(kThreads) void k_fwd_level(
     const SupDev* __restrict__ sup, const int* __restrict__ lvl,
     const int* __restrict__ relpos, const int* __restrict__ cidx,
     const int* __restrict__ active, const T* __restrict__ panels,
-    int64_t panel_bs, T* __restrict__ W, int64_t w_bs, int k,
+    int64_t panel_bs, const double* __restrict__ Bsrc, int64_t b_bs,
+    const int* __restrict__ brow, T* __restrict__ W, int64_t w_bs, int k,
     T* __restrict__ ucur, int64_t ucur_bs, const T* __restrict__ uprev,
     int64_t uprev_bs) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -298,9 +310,15 @@ __global__ __launch_bounds__(kThreads) void k_fwd_level(
   const SupDev S = sup[s];
   const int ns = S.ns, rs = S.rs, nf = S.ns + S.rs;
   T* Wb = W + b * w_bs;
+  const double* Bb = Bsrc + b * b_bs;
   for (int idx = tid; idx < nf * TC; idx += kThreads) {
     const int r = idx / TC, c = idx % TC;
-    f[idx] = (r < ns && col0 + c < k) ? Wb[static_cast<int64_t>(S.first + r) * k + col0 + c] : T(0);
+    T v = T(0);
+    if (r < ns && col0 + c < k) {
+      const int gr = brow ? brow[S.first + r] : S.first + r;
+      v = static_cast<T>(Bb[static_cast<int64_t>(gr) * k + col0 + c]);
+    }
+    f[idx] = v;
   }
   __syncthreads();
   for (int cc = S.c_begin; cc < S.c_end; ++cc) {
@@ -324,7 +342,7 @@ __global__ __launch_bounds__(kThreads) void k_fwd_level(
 #pragma unroll
   for (int q = 0; q < kMaxTiles; ++q) {
     const int t = wave + 4 * q;
-    if (t < nt1) acc[q] = mma_tile<T, TC>(Linv, ns, ns, f, (t / NCT) * 16, (t % NCT) * 16, lane);
+    if (t < nt1) acc[q] = mma_tile<T, TC>(Linv, ns, ns, f, RowId(), (t / NCT) * 16, (t % NCT) * 16, lane);
   }
   __syncthreads();
 #pragma unroll
@@ -348,7 +366,7 @@ __global__ __launch_bounds__(kThreads) void k_fwd_level(
   const int nt2 = ((rs + 15) / 16) * NCT;
   for (int t = wave; t < nt2; t += 4) {
     const int tr = (t / NCT) * 16, tc = (t % NCT) * 16;
-    typename Mma<T>::v4 a2 = mma_tile<T, TC>(Lrj, rs, ns, f, tr, tc, lane);
+    typename Mma<T>::v4 a2 = mma_tile<T, TC>(Lrj, rs, ns, f, RowId(), tr, tc, lane);
     const int c = tc + (lane & 15);
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
@@ -366,7 +384,8 @@ __global__ __launch_bounds__(kThreads) void k_bwd_level(
     const SupDev* __restrict__ sup, const int* __restrict__ lvl,
     const int* __restrict__ ridx, const int* __restrict__ active,
     const T* __restrict__ panels, int64_t panel_bs, T* __restrict__ W,
-    int64_t w_bs, int k) {
+    int64_t w_bs, int k, double* __restrict__ X64, const int* __restrict__ xrow,
+    int xmode) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T* g = reinterpret_cast<T*>(smem_raw);
   const int s = lvl[blockIdx.x];
@@ -399,7 +418,7 @@ __global__ __launch_bounds__(kThreads) void k_bwd_level(
 #pragma unroll
     for (int q = 0; q < kMaxTiles; ++q) {
       const int t = wave + 4 * q;
-      if (t < nt1) acc[q] = mma_tile<T, TC>(LrjT, ns, rs, g + ns * TC, (t / NCT) * 16, (t % NCT) * 16, lane);
+      if (t < nt1) acc[q] = mma_tile<T, TC>(LrjT, ns, rs, g + ns * TC, RowId(), (t / NCT) * 16, (t % NCT) * 16, lane);
     }
     __syncthreads();
 #pragma unroll
@@ -418,12 +437,19 @@ __global__ __launch_bounds__(kThreads) void k_bwd_level(
   }
   for (int t = wave; t < nt1; t += 4) {
     const int tr = (t / NCT) * 16, tc = (t % NCT) * 16;
-    typename Mma<T>::v4 a2 = mma_tile<T, TC>(LinvT, ns, ns, g, tr, tc, lane);
+    typename Mma<T>::v4 a2 = mma_tile<T, TC>(LinvT, ns, ns, g, RowId(), tr, tc, lane);
     const int c = tc + (lane & 15);
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const int r = tr + Mma<T>::row(lane, i);
-      if (r < ns && col0 + c < k) Wb[static_cast<int64_t>(S.first + r) * k + col0 + c] = a2[i];
+      if (r < ns && col0 + c < k) {
+        Wb[static_cast<int64_t>(S.first + r) * k + col0 + c] = a2[i];
+        if (xmode) {  // 1: store, 2: accumulate the FP64 solution
+          const int gr = xrow ? xrow[S.first + r] : S.first + r;
+          double* xp = X64 + b * w_bs + static_cast<int64_t>(gr) * k + col0 + c;
+          *xp = (xmode == 2 ? *xp : 0.0) + static_cast<double>(a2[i]);
+        }
+      }
     }
   }
 }
@@ -492,7 +518,8 @@ __global__ void k_residual(const int* __restrict__ crow, const int* __restrict__
   const int p0 = crow[row], p1 = crow[row + 1];
   for (int c = threadIdx.x & 63; c < k; c += 64) {
     double acc = B[b * b_bs + static_cast<int64_t>(row) * k + c];
-    for (int p = p0; p < p1; ++p) acc -= vb[vidx[p]] * X[xb + static_cast<int64_t>(ccol[p]) * k + c];
+    for (int p = p0; p < p1; ++p)
+      acc -= vb[vidx ? vidx[p] : p] * X[xb + static_cast<int64_t>(ccol[p]) * k + c];
     R[xb + static_cast<int64_t>(row) * k + c] = acc;
   }
 }
@@ -748,9 +775,20 @@ int launch_factor(dcx_plan* P, hipStream_t st) {
   return DCX_OK;
 }
 
+// where the forward sweep reads its right-hand side and where the backward
+// sweep leaves the FP64 solution
+struct SweepIO {
+  const double* B = nullptr;  // FP64 right-hand side (batch stride b_bs)
+  int64_t b_bs = 0;
+  const int* brow = nullptr;  // B row of each elimination row (null: same)
+  double* X64 = nullptr;      // FP64 solution (null: only the working block)
+  const int* xrow = nullptr;  // X64 row of each elimination row (null: same)
+  int xmode = 0;              // 0 none, 1 store, 2 accumulate
+};
+
 template <typename T, int TC>
 void sweep_tc(dcx_plan* P, T* W, int k, int maxnf, bool fwd, int d, const int* lvl,
-              int cnt, const int* active, hipStream_t st) {
+              int cnt, const int* active, const SweepIO& io, hipStream_t st) {
   const auto& S = P->S;
   dim3 grid(cnt, P->nbatch, (k + TC - 1) / TC);
   const size_t lds = static_cast<size_t>(maxnf) * TC * sizeof(T);
@@ -760,11 +798,13 @@ void sweep_tc(dcx_plan* P, T* W, int k, int maxnf, bool fwd, int d, const int* l
     T* u[2] = {static_cast<T*>(P->d_upd[0]), static_cast<T*>(P->d_upd[1])};
     const int64_t ubs[2] = {S.upd_rows[0] * k, S.upd_rows[1] * k};
     hipLaunchKernelGGL((k_fwd_level<T, TC>), grid, dim3(kThreads), lds, st, P->d_sup, lvl,
-                       P->d_relpos, P->d_cidx, active, panels, S.panel_size, W, w_bs, k,
-                       u[d & 1], ubs[d & 1], u[(d + 1) & 1], ubs[(d + 1) & 1]);
+                       P->d_relpos, P->d_cidx, active, panels, S.panel_size, io.B, io.b_bs,
+                       io.brow, W, w_bs, k, u[d & 1], ubs[d & 1], u[(d + 1) & 1],
+                       ubs[(d + 1) & 1]);
   } else {
     hipLaunchKernelGGL((k_bwd_level<T, TC>), grid, dim3(kThreads), lds, st, P->d_sup, lvl,
-                       P->d_ridx, active, panels, S.panel_size, W, w_bs, k);
+                       P->d_ridx, active, panels, S.panel_size, W, w_bs, k, io.X64, io.xrow,
+                       io.xmode);
   }
 }
 
@@ -780,7 +820,7 @@ int pick_tc(int maxnf, int maxns, size_t tsize) {
 
 // full solve on the working block W (elimination order, factor precision)
 template <typename T>
-int sweeps(dcx_plan* P, T* W, int k, bool sparse_rhs, hipStream_t st) {
+int sweeps(dcx_plan* P, T* W, int k, bool sparse_rhs, const SweepIO& io, hipStream_t st) {
   const auto& S = P->S;
   if (P->upd_k < k) {
     const size_t ub0 = static_cast<size_t>(P->nbatch) * S.upd_rows[0] * k * sizeof(T);
@@ -818,9 +858,9 @@ int sweeps(dcx_plan* P, T* W, int k, bool sparse_rhs, hipStream_t st) {
       if (!tc) return fail(DCX_ERR_ARG, "front too large for the in-LDS sweep");
       const int* act = use_act ? P->d_active : nullptr;
       switch (tc) {
-        case 64: sweep_tc<T, 64>(P, W, k, maxnf, fwd, d, lvl, cnt, act, st); break;
-        case 32: sweep_tc<T, 32>(P, W, k, maxnf, fwd, d, lvl, cnt, act, st); break;
-        default: sweep_tc<T, 16>(P, W, k, maxnf, fwd, d, lvl, cnt, act, st); break;
+        case 64: sweep_tc<T, 64>(P, W, k, maxnf, fwd, d, lvl, cnt, act, io, st); break;
+        case 32: sweep_tc<T, 32>(P, W, k, maxnf, fwd, d, lvl, cnt, act, io, st); break;
+        default: sweep_tc<T, 16>(P, W, k, maxnf, fwd, d, lvl, cnt, act, io, st); break;
       }
     }
   }
@@ -995,68 +1035,61 @@ int dcx_solve(dcx_plan* P, const double* b, double* x, int k, int flags, void* s
   const bool permuted = flags & DCX_PERMUTED;
   const bool sparse = (flags & DCX_SPARSE_RHS) != 0;
   const int64_t blk = static_cast<int64_t>(S.n) * k;
-  const int64_t b_bs = shared ? 0 : blk;
   const int n = S.n;
-  dim3 rgrid(grid1(n, 4), nb);
+  SweepIO io;
+  io.B = b;
+  io.b_bs = shared ? 0 : blk;
+  io.brow = permuted ? nullptr : P->d_perm;
+  io.xrow = permuted ? nullptr : P->d_perm;
   if (P->precision == DCX_FP64) {
-    // W = x (permuted) or plan workspace
     double* W = x;
     if (!permuted) {
       int rc = ensure(&P->d_w, &P->w_bytes, static_cast<size_t>(nb) * blk * 8, st);
       if (rc) return rc;
       W = static_cast<double*>(P->d_w);
-      launch_permute_d2T(P, b, W, k, nb, b_bs, blk, false, st);
-    } else if (shared || b != x) {
-      for (int i = 0; i < nb; ++i)
-        HIP_TRY(hipMemcpyAsync(W + i * blk, b + i * b_bs, blk * 8, hipMemcpyDeviceToDevice, st));
+      io.X64 = x;
+      io.xmode = 1;
     }
-    int rc = sweeps<double>(P, W, k, sparse, st);
-    if (rc) return rc;
-    if (!permuted)
-      hipLaunchKernelGGL((k_permute<double, double>), rgrid, dim3(256), 0, st, W, x,
-                         P->d_perm, n, k, blk, blk, 0);
-    HIP_TRY(hipGetLastError());
-    return DCX_OK;
+    return sweeps<double>(P, W, k, sparse, io, st);
   }
-  // mixed: FP32 sweeps, FP64 refinement
+  // mixed: FP32 sweeps, FP64 residual refinement; the solution accumulates in x
   int rc = ensure(&P->d_w, &P->w_bytes, static_cast<size_t>(nb) * blk * 4, st);
   if (rc) return rc;
-  rc = ensure(reinterpret_cast<void**>(&P->d_b), &P->b_bytes,
-              static_cast<size_t>(shared ? 1 : nb) * blk * 8, st);
-  if (rc) return rc;
-  rc = ensure(reinterpret_cast<void**>(&P->d_r), &P->r_bytes, static_cast<size_t>(nb) * blk * 8, st);
-  if (rc) return rc;
   float* W = static_cast<float*>(P->d_w);
-  double* Bp = P->d_b;
-  double* X = x;
-  if (!permuted) {
-    rc = ensure(reinterpret_cast<void**>(&P->d_x), &P->x_bytes, static_cast<size_t>(nb) * blk * 8, st);
+  if (P->refine > 0 && b == x) {  // the residual needs the right-hand side intact
+    rc = ensure(reinterpret_cast<void**>(&P->d_b), &P->b_bytes,
+                static_cast<size_t>(shared ? 1 : nb) * blk * 8, st);
     if (rc) return rc;
-    X = P->d_x;
-    launch_permute_d2T(P, b, Bp, k, shared ? 1 : nb, b_bs, blk, false, st);
-  } else {
-    HIP_TRY(hipMemcpyAsync(Bp, b, (shared ? 1 : nb) * blk * 8, hipMemcpyDeviceToDevice, st));
+    HIP_TRY(hipMemcpyAsync(P->d_b, b, (shared ? 1 : nb) * blk * 8, hipMemcpyDeviceToDevice, st));
+    io.B = P->d_b;
   }
-  const int64_t tot = nb * blk;
-  // first pass: W = (float) B
-  for (int i = 0; i < nb; ++i)
-    hipLaunchKernelGGL((k_convert<double, float>), dim3(grid_stride(blk)), dim3(256), 0, st,
-                       Bp + i * b_bs, W + i * blk, blk);
-  rc = sweeps<float>(P, W, k, sparse, st);
+  const double* Borig = io.B;
+  const int64_t Bbs = io.b_bs;
+  io.X64 = x;
+  io.xmode = 1;
+  rc = sweeps<float>(P, W, k, sparse, io, st);
   if (rc) return rc;
-  hipLaunchKernelGGL((k_convert<float, double>), dim3(grid_stride(tot)), dim3(256), 0, st, W, X, tot);
-  for (int it = 0; it < P->refine; ++it) {
-    hipLaunchKernelGGL(k_residual, rgrid, dim3(256), 0, st, P->d_pcrow, P->d_pcol, P->d_pvidx,
-                       P->d_vals, static_cast<int64_t>(S.nnz), Bp, b_bs, X, P->d_r, n, k);
-    hipLaunchKernelGGL((k_convert<double, float>), dim3(grid_stride(tot)), dim3(256), 0, st,
-                       P->d_r, W, tot);
-    rc = sweeps<float>(P, W, k, false, st);
+  if (P->refine > 0) {
+    rc = ensure(reinterpret_cast<void**>(&P->d_r), &P->r_bytes, static_cast<size_t>(nb) * blk * 8, st);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_axpy_f32, dim3(grid_stride(tot)), dim3(256), 0, st, W, X, tot);
   }
-  if (!permuted)
-    hipLaunchKernelGGL((k_permute<double, double>), rgrid, dim3(256), 0, st, X, x, P->d_perm,
-                       n, k, blk, blk, 0);
+  dim3 rgrid(grid1(n, 4), nb);
+  for (int it = 0; it < P->refine; ++it) {
+    if (permuted)
+      hipLaunchKernelGGL(k_residual, rgrid, dim3(256), 0, st, P->d_pcrow, P->d_pcol,
+                         P->d_pvidx, P->d_vals, static_cast<int64_t>(S.nnz), Borig, Bbs, x,
+                         P->d_r, n, k);
+    else
+      hipLaunchKernelGGL(k_residual, rgrid, dim3(256), 0, st, P->d_crow, P->d_col,
+                         static_cast<const int*>(nullptr), P->d_vals,
+                         static_cast<int64_t>(S.nnz), Borig, Bbs, x, P->d_r, n, k);
+    SweepIO ir = io;
+    ir.B = P->d_r;
+    ir.b_bs = blk;
+    ir.xmode = 2;
+    rc = sweeps<float>(P, W, k, false, ir, st);
+    if (rc) return rc;
+  }
   HIP_TRY(hipGetLastError());
   return DCX_OK;
 }
