@@ -65,6 +65,10 @@ typedef struct {
   const int32_t *aptr, *a_e, *a_row, *a_col, *lvl_ptr, *lvl_sup;
   int64_t nnz_L;
   double factor_flops, solve_flops_per_rhs;
+  /* fused bottom subtrees of the sweeps */
+  int nfused, nfused_lv, nfused_mem, nfused_grp, f_max_lds_rows;
+  const int32_t *f_root, *f_lo, *f_hi, *f_lv_begin, *f_lv_end, *f_tb_rows;
+  const int32_t *f_lv, *f_mem, *f_gptr, *f_gb, *toff, *lridx, *ulvl_ptr, *ulvl_sup;
 } dcx_symbolic_view;
 
 const char* dcx_last_error(void);
@@ -72,7 +76,7 @@ const char* dcx_last_error(void);
 /* Host-only symbolic analysis (no device work): the same ordering and
  * supernode structure dcx_plan_create builds, for inspection and tests. */
 int dcx_symbolic_analyse(int n, const int32_t* crow, const int32_t* col,
-                         const double* coords, int dim, int leaf_max,
+                         const double* coords, int dim, int leaf_max, int fuse_rows,
                          dcx_symbolic** out);
 int dcx_symbolic_view_get(const dcx_symbolic* sym, dcx_symbolic_view* out);
 int dcx_symbolic_destroy(dcx_symbolic* sym);
@@ -92,6 +96,9 @@ int dcx_plan_destroy(dcx_plan* plan);
 int dcx_plan_get_info(const dcx_plan* plan, dcx_plan_info* out);
 /* host (n): perm[new] = old */
 int dcx_plan_get_perm(const dcx_plan* plan, int32_t* perm);
+/* LDS row budget of the fused bottom subtrees of the sweeps (0: none);
+ * rebuilds the fused structure (host) and its device tables. */
+int dcx_plan_set_fusion(dcx_plan* plan, int fuse_rows);
 /* number of FP64 refinement passes after the first FP32 solve (mixed mode) */
 int dcx_plan_set_refinement(dcx_plan* plan, int passes);
 

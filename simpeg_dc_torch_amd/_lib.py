@@ -34,7 +34,12 @@ class SymbolicView(C.Structure):
                               "cptr", "cidx", "relpos", "aptr", "a_e", "a_row", "a_col",
                               "lvl_ptr", "lvl_sup")] + \
         [("nnz_L", C.c_int64), ("factor_flops", C.c_double),
-         ("solve_flops_per_rhs", C.c_double)]
+         ("solve_flops_per_rhs", C.c_double)] + \
+        [(f, C.c_int) for f in ("nfused", "nfused_lv", "nfused_mem", "nfused_grp",
+                                "f_max_lds_rows")] + \
+        [(f, _i32p) for f in ("f_root", "f_lo", "f_hi", "f_lv_begin", "f_lv_end", "f_tb_rows",
+                              "f_lv", "f_mem", "f_gptr", "f_gb", "toff", "lridx", "ulvl_ptr",
+                              "ulvl_sup")]
 
 
 # name -> (restype, argtypes); every entry point returns an int status
@@ -42,7 +47,7 @@ PROTOTYPES = {
     "dcx_last_error": (C.c_char_p, []),
     "dcx_device_count": (C.c_int, [C.POINTER(C.c_int)]),
     "dcx_symbolic_analyse": (C.c_int, [C.c_int, _i32p, _i32p, _f64p, C.c_int, C.c_int,
-                                       C.POINTER(_vp)]),
+                                       C.c_int, C.POINTER(_vp)]),
     "dcx_symbolic_view_get": (C.c_int, [_vp, C.POINTER(SymbolicView)]),
     "dcx_symbolic_destroy": (C.c_int, [_vp]),
     "dcx_plan_create": (C.c_int, [C.c_int, _i32p, _i32p, C.c_int, _f64p, C.c_int, C.c_int,
@@ -51,6 +56,7 @@ PROTOTYPES = {
     "dcx_plan_get_info": (C.c_int, [_vp, C.POINTER(PlanInfo)]),
     "dcx_plan_get_perm": (C.c_int, [_vp, _i32p]),
     "dcx_plan_set_refinement": (C.c_int, [_vp, C.c_int]),
+    "dcx_plan_set_fusion": (C.c_int, [_vp, C.c_int]),
     "dcx_factorize": (C.c_int, [_vp, _vp, _vp]),
     "dcx_solve": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, _vp]),
     "dcx_set_rhs_support": (C.c_int, [_vp, _i32p, C.c_int]),
@@ -125,7 +131,7 @@ def stream_of(device):
     return C.c_void_p(torch.cuda.current_stream(device).cuda_stream)
 
 
-def symbolic(crow, col, coords=None, leaf_max=16):
+def symbolic(crow, col, coords=None, leaf_max=16, fuse_rows=0):
     """Host-only symbolic analysis: a dict of numpy arrays (copies)."""
     lib = load()
     crow_a, crow_p = i32(crow)
@@ -137,8 +143,8 @@ def symbolic(crow, col, coords=None, leaf_max=16):
     else:
         co_p, dim = None, 0
     h = _vp()
-    check(lib.dcx_symbolic_analyse(n, crow_p, col_p, co_p, dim, int(leaf_max), C.byref(h)),
-          "symbolic analysis")
+    check(lib.dcx_symbolic_analyse(n, crow_p, col_p, co_p, dim, int(leaf_max),
+                                   int(fuse_rows), C.byref(h)), "symbolic analysis")
     try:
         v = SymbolicView()
         check(lib.dcx_symbolic_view_get(h, C.byref(v)), "symbolic view")
@@ -160,6 +166,15 @@ def symbolic(crow, col, coords=None, leaf_max=16):
                    relpos=arr(v.relpos, rptr[-1]), aptr=aptr, a_e=arr(v.a_e, aptr[-1]),
                    a_row=arr(v.a_row, aptr[-1]), a_col=arr(v.a_col, aptr[-1]),
                    lvl_ptr=lvl_ptr, lvl_sup=arr(v.lvl_sup, lvl_ptr[-1]))
+        nf, nlv = v.nfused, v.nfused_lv
+        out.update(nfused=nf, f_max_lds_rows=v.f_max_lds_rows,
+                   f_root=arr(v.f_root, nf), f_lo=arr(v.f_lo, nf), f_hi=arr(v.f_hi, nf),
+                   f_lv_begin=arr(v.f_lv_begin, nf), f_lv_end=arr(v.f_lv_end, nf),
+                   f_tb_rows=arr(v.f_tb_rows, nf), f_lv=arr(v.f_lv, nlv + 1),
+                   f_mem=arr(v.f_mem, v.nfused_mem), f_gptr=arr(v.f_gptr, nlv + 1),
+                   f_gb=arr(v.f_gb, v.nfused_grp + 1), toff=arr(v.toff, ns),
+                   lridx=arr(v.lridx, rptr[-1]), ulvl_ptr=arr(v.ulvl_ptr, v.ndepth + 1),
+                   ulvl_sup=arr(v.ulvl_sup, int(arr(v.ulvl_ptr, v.ndepth + 1)[-1])))
         return out
     finally:
         lib.dcx_symbolic_destroy(h)

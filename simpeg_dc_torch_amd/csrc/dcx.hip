@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -42,6 +43,12 @@ struct SupDev {
   int first, ns, rs, rptr;
   int c_begin, c_end, a_begin, a_end;
   int64_t panel_off, front_off, upd_off;
+  int64_t fz_uoff;  // row of its update in the fused-subtree buffer, -1 if not a fused root
+};
+
+struct FusedDev {
+  int root, lo, hi, rsT, rptrT, lv_begin, lv_end, pad;
+  int64_t uoff;
 };
 
 template <typename T>
@@ -260,8 +267,9 @@ struct Mma<float> {
 constexpr int kMaxTiles = 8;  // 16x16 register tiles a wave keeps across a barrier
 
 // C tile = A(rows r0.., K) * B(K, cols c0..), A read through A(r, k) (global),
-// B from LDS rows brow(k) with row stride TC.  Operands are fetched eight
+// B from LDS rows brow(k) with row stride TC.  Operands are fetched kPrefetch
 // k-steps at a time so that the global loads of a chunk are in flight together.
+constexpr int kPrefetch = 8;
 template <typename T, int TC, typename AFn, typename BRow>
 __device__ inline typename Mma<T>::v4 mma_tile(AFn A, int M, int K, const T* Bs, BRow brow,
                                                int r0, int c0, int lane) {
@@ -270,17 +278,17 @@ __device__ inline typename Mma<T>::v4 mma_tile(AFn A, int M, int K, const T* Bs,
   const int kq = lane >> 4;
   const int cc = c0 + (lane & 15);
   const bool rok = rr < M;
-  for (int k0 = 0; k0 < K; k0 += 32) {
-    T av[8], bv[8];
+  for (int k0 = 0; k0 < K; k0 += 4 * kPrefetch) {
+    T av[kPrefetch], bv[kPrefetch];
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
+    for (int j = 0; j < kPrefetch; ++j) {
       const int kk = k0 + 4 * j + kq;
       const bool kok = kk < K;
       av[j] = (rok && kok) ? A(rr, kk) : T(0);
       bv[j] = kok ? Bs[brow(kk) * TC + cc] : T(0);
     }
 #pragma unroll
-    for (int j = 0; j < 8; ++j)
+    for (int j = 0; j < kPrefetch; ++j)
       if (k0 + 4 * j < K) acc = Mma<T>::op(av[j], bv[j], acc);
   }
   return acc;
@@ -300,7 +308,7 @@ __global__ __launch_bounds__(kThreads) void k_fwd_level(
     int64_t panel_bs, const double* __restrict__ Bsrc, int64_t b_bs,
     const int* __restrict__ brow, T* __restrict__ W, int64_t w_bs, int k,
     T* __restrict__ ucur, int64_t ucur_bs, const T* __restrict__ uprev,
-    int64_t uprev_bs) {
+    int64_t uprev_bs, const T* __restrict__ ufz, int64_t ufz_bs) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T* f = reinterpret_cast<T*>(smem_raw);
   const int s = lvl[blockIdx.x];
@@ -325,7 +333,8 @@ __global__ __launch_bounds__(kThreads) void k_fwd_level(
     const int c = cidx[cc];
     if (active && !active[c]) continue;
     const SupDev C = sup[c];
-    const T* u = uprev + b * uprev_bs + C.upd_off * k;
+    const T* u = C.fz_uoff >= 0 ? ufz + b * ufz_bs + C.fz_uoff * k
+                                : uprev + b * uprev_bs + C.upd_off * k;
     const int* rp = relpos + C.rptr;
     for (int idx = tid; idx < C.rs * TC; idx += kThreads) {
       const int r = idx / TC, cl = idx % TC;
@@ -451,6 +460,214 @@ __global__ __launch_bounds__(kThreads) void k_bwd_level(
         }
       }
     }
+  }
+}
+
+struct LRow {
+  const int* l;
+  __device__ inline int operator()(int k) const { return l[k]; }
+};
+struct OffRow {
+  int o;
+  __device__ inline int operator()(int k) const { return o + k; }
+};
+
+// Fused bottom subtree, backward sweep.  The LDS block X holds the subtree's
+// rows [lo, hi) and, below them, the root's off-diagonal rows R_T (ancestors:
+// final x).  Sublevels root first: t = y_J - L_RJ^T x_R goes to the scratch
+// Tb, then x_J = L_JJ^{-T} t back into X; one HBM read and write per row.
+template <typename T, int TC>
+__global__ __launch_bounds__(kThreads) void k_bwd_fused(
+    const SupDev* __restrict__ sup, const FusedDev* __restrict__ fz,
+    const int* __restrict__ flv, const int* __restrict__ fmem,
+    const int* __restrict__ toff, const int* __restrict__ lridx,
+    const int* __restrict__ ridx, const int* __restrict__ active,
+    const T* __restrict__ panels, int64_t panel_bs, T* __restrict__ W, int64_t w_bs,
+    int k, double* __restrict__ X64, const int* __restrict__ xrow, int xmode) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* X = reinterpret_cast<T*>(smem_raw);
+  const FusedDev F = fz[blockIdx.x];
+  const int b = blockIdx.y;
+  const int col0 = blockIdx.z * TC;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int nloc = F.hi - F.lo, nrows = nloc + F.rsT;
+  T* Tb = X + nrows * TC;
+  const bool act = !active || active[F.root];
+  T* Wb = W + b * w_bs;
+  for (int idx = tid; idx < nrows * TC; idx += kThreads) {
+    const int r = idx / TC, c = idx % TC;
+    T v = T(0);
+    if (col0 + c < k) {
+      if (r < nloc)
+        v = act ? Wb[static_cast<int64_t>(F.lo + r) * k + col0 + c] : T(0);
+      else
+        v = Wb[static_cast<int64_t>(ridx[F.rptrT + r - nloc]) * k + col0 + c];
+    }
+    X[idx] = v;
+  }
+  __syncthreads();
+  constexpr int NCT = TC / 16;
+  for (int L = F.lv_begin; L < F.lv_end; ++L) {
+    const int m0 = flv[L], m1 = flv[L + 1];
+    int base = 0;
+    for (int mi = m0; mi < m1; ++mi) {
+      const int s = fmem[mi];
+      const SupDev S = sup[s];
+      const int ns = S.ns, nf = S.ns + S.rs;
+      const T* P = panels + b * panel_bs + S.panel_off;
+      auto LrjT = [=](int i, int kk) { return P[(ns + kk) + static_cast<int64_t>(i) * nf]; };
+      const int nt = ((ns + 15) / 16) * NCT;
+      const int to = toff[s], jl = S.first - F.lo;
+      for (int t = ((wave - base) % 4 + 4) % 4; t < nt; t += 4) {
+        const int tr = (t / NCT) * 16, tc = (t % NCT) * 16;
+        typename Mma<T>::v4 acc = mma_tile<T, TC>(LrjT, ns, S.rs, X, LRow{lridx + S.rptr}, tr, tc, lane);
+        const int c = tc + (lane & 15);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int r = tr + Mma<T>::row(lane, i);
+          if (r < ns) Tb[(to + r) * TC + c] = X[(jl + r) * TC + c] - acc[i];
+        }
+      }
+      base += nt;
+    }
+    __syncthreads();
+    base = 0;
+    for (int mi = m0; mi < m1; ++mi) {
+      const int s = fmem[mi];
+      const SupDev S = sup[s];
+      const int ns = S.ns, nf = S.ns + S.rs;
+      const T* P = panels + b * panel_bs + S.panel_off;
+      auto LinvT = [=](int i, int kk) { return P[kk + static_cast<int64_t>(i) * nf]; };
+      const int nt = ((ns + 15) / 16) * NCT;
+      const int to = toff[s], jl = S.first - F.lo;
+      for (int t = ((wave - base) % 4 + 4) % 4; t < nt; t += 4) {
+        const int tr = (t / NCT) * 16, tc = (t % NCT) * 16;
+        typename Mma<T>::v4 acc = mma_tile<T, TC>(LinvT, ns, ns, Tb, OffRow{to}, tr, tc, lane);
+        const int c = tc + (lane & 15);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int r = tr + Mma<T>::row(lane, i);
+          if (r < ns) X[(jl + r) * TC + c] = acc[i];
+        }
+      }
+      base += nt;
+    }
+    __syncthreads();
+  }
+  for (int idx = tid; idx < nloc * TC; idx += kThreads) {
+    const int r = idx / TC, c = idx % TC;
+    if (col0 + c >= k) continue;
+    const T v = X[idx];
+    Wb[static_cast<int64_t>(F.lo + r) * k + col0 + c] = v;
+    if (xmode) {
+      const int gr = xrow ? xrow[F.lo + r] : F.lo + r;
+      double* xp = X64 + b * w_bs + static_cast<int64_t>(gr) * k + col0 + c;
+      *xp = (xmode == 2 ? *xp : 0.0) + static_cast<double>(v);
+    }
+  }
+}
+
+// Fused bottom subtree, forward sweep.  X holds b on the subtree rows and
+// accumulates the updates of R_T below them.  Sublevels deepest first:
+// y_J = L_JJ^{-1} f_J (to Tb, then into X), then X[R rows] -= L_RJ y_J, one
+// colour group at a time so that no two members add into the same row at
+// once.  The root's accumulated R_T rows are its update block.
+template <typename T, int TC>
+__global__ __launch_bounds__(kThreads) void k_fwd_fused(
+    const SupDev* __restrict__ sup, const FusedDev* __restrict__ fz,
+    const int* __restrict__ flv, const int* __restrict__ fmem,
+    const int* __restrict__ fgptr, const int* __restrict__ fgb,
+    const int* __restrict__ toff, const int* __restrict__ lridx,
+    const int* __restrict__ active, const T* __restrict__ panels, int64_t panel_bs,
+    const double* __restrict__ Bsrc, int64_t b_bs, const int* __restrict__ brow,
+    T* __restrict__ W, int64_t w_bs, int k, T* __restrict__ ufz, int64_t ufz_bs) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* X = reinterpret_cast<T*>(smem_raw);
+  const FusedDev F = fz[blockIdx.x];
+  if (active && !active[F.root]) return;  // zero right-hand side below this root
+  const int b = blockIdx.y;
+  const int col0 = blockIdx.z * TC;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int nloc = F.hi - F.lo, nrows = nloc + F.rsT;
+  T* Tb = X + nrows * TC;
+  const double* Bb = Bsrc + b * b_bs;
+  for (int idx = tid; idx < nrows * TC; idx += kThreads) {
+    const int r = idx / TC, c = idx % TC;
+    T v = T(0);
+    if (r < nloc && col0 + c < k) {
+      const int gr = brow ? brow[F.lo + r] : F.lo + r;
+      v = static_cast<T>(Bb[static_cast<int64_t>(gr) * k + col0 + c]);
+    }
+    X[idx] = v;
+  }
+  __syncthreads();
+  constexpr int NCT = TC / 16;
+  for (int L = F.lv_end - 1; L >= F.lv_begin; --L) {
+    const int m0 = flv[L], m1 = flv[L + 1];
+    int base = 0;
+    for (int mi = m0; mi < m1; ++mi) {
+      const int s = fmem[mi];
+      const SupDev S = sup[s];
+      const int ns = S.ns, nf = S.ns + S.rs;
+      const T* P = panels + b * panel_bs + S.panel_off;
+      auto Linv = [=](int r, int kk) { return P[r + static_cast<int64_t>(kk) * nf]; };
+      const int nt = ((ns + 15) / 16) * NCT;
+      const int to = toff[s], jl = S.first - F.lo;
+      for (int t = ((wave - base) % 4 + 4) % 4; t < nt; t += 4) {
+        const int tr = (t / NCT) * 16, tc = (t % NCT) * 16;
+        typename Mma<T>::v4 acc = mma_tile<T, TC>(Linv, ns, ns, X, OffRow{jl}, tr, tc, lane);
+        const int c = tc + (lane & 15);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int r = tr + Mma<T>::row(lane, i);
+          if (r < ns) Tb[(to + r) * TC + c] = acc[i];
+        }
+      }
+      base += nt;
+    }
+    __syncthreads();
+    for (int mi = m0; mi < m1; ++mi) {  // y into the subtree rows
+      const int s = fmem[mi];
+      const SupDev S = sup[s];
+      const int to = toff[s], jl = S.first - F.lo;
+      for (int idx = tid; idx < S.ns * TC; idx += kThreads)
+        X[jl * TC + idx] = Tb[to * TC + idx];
+    }
+    for (int g = fgptr[L]; g < fgptr[L + 1]; ++g) {
+      base = 0;
+      for (int mi = fgb[g]; mi < fgb[g + 1]; ++mi) {
+        const int s = fmem[mi];
+        const SupDev S = sup[s];
+        const int ns = S.ns, rs = S.rs, nf = S.ns + S.rs;
+        const T* P = panels + b * panel_bs + S.panel_off;
+        auto Lrj = [=](int r, int kk) { return P[(ns + r) + static_cast<int64_t>(kk) * nf]; };
+        const int nt = ((rs + 15) / 16) * NCT;
+        const int to = toff[s];
+        const int* lr = lridx + S.rptr;
+        for (int t = ((wave - base) % 4 + 4) % 4; t < nt; t += 4) {
+          const int tr = (t / NCT) * 16, tc = (t % NCT) * 16;
+          typename Mma<T>::v4 acc = mma_tile<T, TC>(Lrj, rs, ns, Tb, OffRow{to}, tr, tc, lane);
+          const int c = tc + (lane & 15);
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const int r = tr + Mma<T>::row(lane, i);
+            if (r < rs) X[lr[r] * TC + c] -= acc[i];
+          }
+        }
+        base += nt;
+      }
+      __syncthreads();
+    }
+  }
+  T* Wb = W + b * w_bs;
+  T* uo = ufz + b * ufz_bs + F.uoff * k;
+  for (int idx = tid; idx < nrows * TC; idx += kThreads) {
+    const int r = idx / TC, c = idx % TC;
+    if (col0 + c >= k) continue;
+    if (r < nloc)
+      Wb[static_cast<int64_t>(F.lo + r) * k + col0 + c] = X[idx];
+    else
+      uo[static_cast<int64_t>(r - nloc) * k + col0 + c] = X[idx];
   }
 }
 
@@ -705,6 +922,13 @@ struct dcx_plan {
   void* d_front[2] = {nullptr, nullptr};
   double* d_vals = nullptr;
   bool factored = false;
+  // fused bottom subtrees
+  int fuse_rows = 0;
+  FusedDev* d_fz = nullptr;
+  int *d_flv = nullptr, *d_fmem = nullptr, *d_fgptr = nullptr, *d_fgb = nullptr,
+      *d_toff = nullptr, *d_lridx = nullptr, *d_ulvl = nullptr;
+  void* d_ufz = nullptr;
+  size_t ufz_bytes = 0;
   // sparse right-hand sides
   int* d_active = nullptr;
   std::vector<int> act_lvl_ptr;
@@ -724,13 +948,21 @@ struct dcx_plan {
     void* ptrs[] = {d_sup, d_lvl, d_relpos, d_cidx, d_ridx, d_a_e, d_a_row, d_a_col,
                     d_perm, d_iperm, d_pcrow, d_pcol, d_pvidx, d_crow, d_col, d_info, d_panels,
                     d_front[0], d_front[1], d_vals, d_active, d_act_lvl, d_upd[0],
-                    d_upd[1], d_w, d_r, d_b, d_x};
+                    d_upd[1], d_w, d_r, d_b, d_x, d_fz, d_flv, d_fmem, d_fgptr, d_fgb,
+                    d_toff, d_lridx, d_ulvl, d_ufz};
     for (void* p : ptrs)
       if (p) (void)hipFree(p);
   }
 };
 
 namespace {
+
+int default_fuse_rows(int precision) {
+  const char* e = std::getenv("DCX_FUSE_ROWS");
+  if (e) return std::atoi(e);
+  (void)precision;
+  return 0;  // fused subtrees off by default (slower than the level kernels at this occupancy)
+}
 
 int ensure(void** p, size_t* cap, size_t bytes, hipStream_t st) {
   if (*cap >= bytes && *p) return DCX_OK;
@@ -743,7 +975,43 @@ int ensure(void** p, size_t* cap, size_t bytes, hipStream_t st) {
   return DCX_OK;
 }
 
+// (re)upload the fused-subtree structure and the supernode table
+int upload_fused(dcx_plan* P) {
+  auto& S = P->S;
+  void* olds[] = {P->d_fz, P->d_flv, P->d_fmem, P->d_fgptr, P->d_fgb, P->d_toff,
+                  P->d_lridx, P->d_ulvl, P->d_sup};
+  for (void* q : olds)
+    if (q) HIP_TRY(hipFree(q));
+  P->d_fz = nullptr;
+  P->d_flv = P->d_fmem = P->d_fgptr = P->d_fgb = P->d_toff = P->d_lridx = P->d_ulvl = nullptr;
+  P->d_sup = nullptr;
+  std::vector<SupDev> sd(S.nsup);
+  for (int s = 0; s < S.nsup; ++s) {
+    const int t = S.fused_of_root[s];
+    sd[s] = SupDev{S.first[s], S.ns[s], S.rs[s], S.rptr[s], S.cptr[s], S.cptr[s + 1],
+                   S.aptr[s], S.aptr[s + 1], S.panel_off[s], S.front_off[s], S.upd_off[s],
+                   t >= 0 ? S.f_uoff[t] : int64_t(-1)};
+  }
+  HIP_TRY(upload(&P->d_sup, sd));
+  std::vector<FusedDev> fd(S.nfused);
+  for (int t = 0; t < S.nfused; ++t) {
+    const int r = S.f_root[t];
+    fd[t] = FusedDev{r, S.f_lo[t], S.f_hi[t], S.rs[r], S.rptr[r], S.f_lv_begin[t],
+                     S.f_lv_end[t], S.f_tb_rows[t], S.f_uoff[t]};
+  }
+  HIP_TRY(upload(&P->d_fz, fd));
+  HIP_TRY(upload(&P->d_flv, S.f_lv));
+  HIP_TRY(upload(&P->d_fmem, S.f_mem));
+  HIP_TRY(upload(&P->d_fgptr, S.f_gptr));
+  HIP_TRY(upload(&P->d_fgb, S.f_gb));
+  HIP_TRY(upload(&P->d_toff, S.toff));
+  HIP_TRY(upload(&P->d_lridx, S.lridx));
+  HIP_TRY(upload(&P->d_ulvl, S.ulvl_sup));
+  return DCX_OK;
+}
+
 constexpr size_t kLdsBudget = 64 * 1024;
+constexpr size_t kFusedLdsBudget = 80 * 1024;  // two workgroups per CU
 
 template <typename T>
 int launch_factor(dcx_plan* P, hipStream_t st) {
@@ -800,7 +1068,8 @@ void sweep_tc(dcx_plan* P, T* W, int k, int maxnf, bool fwd, int d, const int* l
     hipLaunchKernelGGL((k_fwd_level<T, TC>), grid, dim3(kThreads), lds, st, P->d_sup, lvl,
                        P->d_relpos, P->d_cidx, active, panels, S.panel_size, io.B, io.b_bs,
                        io.brow, W, w_bs, k, u[d & 1], ubs[d & 1], u[(d + 1) & 1],
-                       ubs[(d + 1) & 1]);
+                       ubs[(d + 1) & 1], static_cast<const T*>(P->d_ufz),
+                       P->S.f_upd_rows * k);
   } else {
     hipLaunchKernelGGL((k_bwd_level<T, TC>), grid, dim3(kThreads), lds, st, P->d_sup, lvl,
                        P->d_ridx, active, panels, S.panel_size, W, w_bs, k, io.X64, io.xrow,
@@ -819,23 +1088,66 @@ int pick_tc(int maxnf, int maxns, size_t tsize) {
 }
 
 // full solve on the working block W (elimination order, factor precision)
+template <typename T, int TC>
+void fused_tc(dcx_plan* P, T* W, int k, bool fwd, const int* active, const SweepIO& io,
+              hipStream_t st) {
+  const auto& S = P->S;
+  dim3 grid(S.nfused, P->nbatch, (k + TC - 1) / TC);
+  const size_t lds = static_cast<size_t>(S.f_max_lds_rows) * TC * sizeof(T);
+  T* panels = static_cast<T*>(P->d_panels);
+  const int64_t w_bs = static_cast<int64_t>(S.n) * k;
+  T* ufz = static_cast<T*>(P->d_ufz);
+  const int64_t ufz_bs = S.f_upd_rows * k;
+  if (fwd)
+    hipLaunchKernelGGL((k_fwd_fused<T, TC>), grid, dim3(kThreads), lds, st, P->d_sup, P->d_fz,
+                       P->d_flv, P->d_fmem, P->d_fgptr, P->d_fgb, P->d_toff, P->d_lridx, active,
+                       panels, S.panel_size, io.B, io.b_bs, io.brow, W, w_bs, k, ufz, ufz_bs);
+  else
+    hipLaunchKernelGGL((k_bwd_fused<T, TC>), grid, dim3(kThreads), lds, st, P->d_sup, P->d_fz,
+                       P->d_flv, P->d_fmem, P->d_toff, P->d_lridx, P->d_ridx, active, panels,
+                       S.panel_size, W, w_bs, k, io.X64, io.xrow, io.xmode);
+}
+
+template <typename T>
+int launch_fused(dcx_plan* P, T* W, int k, bool fwd, const int* active, const SweepIO& io,
+                 hipStream_t st) {
+  const size_t rows = static_cast<size_t>(P->S.f_max_lds_rows);
+  if (rows * 64 * sizeof(T) <= kFusedLdsBudget)
+    fused_tc<T, 64>(P, W, k, fwd, active, io, st);
+  else if (rows * 32 * sizeof(T) <= kFusedLdsBudget)
+    fused_tc<T, 32>(P, W, k, fwd, active, io, st);
+  else if (rows * 16 * sizeof(T) <= 160 * 1024)
+    fused_tc<T, 16>(P, W, k, fwd, active, io, st);
+  else
+    return fail(DCX_ERR_ARG, "fused subtree too large for LDS");
+  return DCX_OK;
+}
+
 template <typename T>
 int sweeps(dcx_plan* P, T* W, int k, bool sparse_rhs, const SweepIO& io, hipStream_t st) {
   const auto& S = P->S;
   if (P->upd_k < k) {
     const size_t ub0 = static_cast<size_t>(P->nbatch) * S.upd_rows[0] * k * sizeof(T);
     const size_t ub1 = static_cast<size_t>(P->nbatch) * S.upd_rows[1] * k * sizeof(T);
+    const size_t ubf = static_cast<size_t>(P->nbatch) * S.f_upd_rows * k * sizeof(T);
     HIP_TRY(hipStreamSynchronize(st));
-    if (P->d_upd[0]) HIP_TRY(hipFree(P->d_upd[0]));
-    if (P->d_upd[1]) HIP_TRY(hipFree(P->d_upd[1]));
-    P->d_upd[0] = P->d_upd[1] = nullptr;
+    void* olds[] = {P->d_upd[0], P->d_upd[1], P->d_ufz};
+    for (void* q : olds)
+      if (q) HIP_TRY(hipFree(q));
+    P->d_upd[0] = P->d_upd[1] = P->d_ufz = nullptr;
     HIP_TRY(hipMalloc(&P->d_upd[0], std::max<size_t>(ub0, 256)));
     HIP_TRY(hipMalloc(&P->d_upd[1], std::max<size_t>(ub1, 256)));
+    HIP_TRY(hipMalloc(&P->d_ufz, std::max<size_t>(ubf, 256)));
     P->upd_k = k;
   }
   const bool use_act = sparse_rhs && P->d_active;
+  const int* act = use_act ? P->d_active : nullptr;
   for (int pass = 0; pass < 2; ++pass) {
     const bool fwd = pass == 0;
+    if (fwd && S.nfused) {
+      int rc = launch_fused<T>(P, W, k, true, act, io, st);
+      if (rc) return rc;
+    }
     for (int dd = 0; dd < S.ndepth; ++dd) {
       const int d = fwd ? S.ndepth - 1 - dd : dd;
       const int* lvl;
@@ -844,24 +1156,27 @@ int sweeps(dcx_plan* P, T* W, int k, bool sparse_rhs, const SweepIO& io, hipStre
         lvl = P->d_act_lvl + P->act_lvl_ptr[d];
         cnt = P->act_lvl_ptr[d + 1] - P->act_lvl_ptr[d];
       } else {
-        lvl = P->d_lvl + S.lvl_ptr[d];
-        cnt = S.lvl_ptr[d + 1] - S.lvl_ptr[d];
+        lvl = P->d_ulvl + S.ulvl_ptr[d];
+        cnt = S.ulvl_ptr[d + 1] - S.ulvl_ptr[d];
       }
       if (!cnt) continue;
       int maxnf = 0, maxns = 0;
-      for (int q = S.lvl_ptr[d]; q < S.lvl_ptr[d + 1]; ++q) {
-        int s = S.lvl_sup[q];
+      for (int q = S.ulvl_ptr[d]; q < S.ulvl_ptr[d + 1]; ++q) {
+        int s = S.ulvl_sup[q];
         maxnf = std::max(maxnf, S.ns[s] + S.rs[s]);
         maxns = std::max(maxns, S.ns[s]);
       }
       const int tc = pick_tc(maxnf, maxns, sizeof(T));
       if (!tc) return fail(DCX_ERR_ARG, "front too large for the in-LDS sweep");
-      const int* act = use_act ? P->d_active : nullptr;
       switch (tc) {
         case 64: sweep_tc<T, 64>(P, W, k, maxnf, fwd, d, lvl, cnt, act, io, st); break;
         case 32: sweep_tc<T, 32>(P, W, k, maxnf, fwd, d, lvl, cnt, act, io, st); break;
         default: sweep_tc<T, 16>(P, W, k, maxnf, fwd, d, lvl, cnt, act, io, st); break;
       }
+    }
+    if (!fwd && S.nfused) {
+      int rc = launch_fused<T>(P, W, k, false, act, io, st);
+      if (rc) return rc;
     }
   }
   HIP_TRY(hipGetLastError());
@@ -906,19 +1221,18 @@ int dcx_plan_create(int n, const int32_t* crow, const int32_t* col, int nbatch,
       return fail(DCX_ERR_ARG, "precision must be DCX_FP64 or DCX_FP32");
     if (coords && (dim < 1 || dim > 3)) return fail(DCX_ERR_ARG, "dim must be 1..3");
     std::unique_ptr<dcx_plan> P(new dcx_plan());
-    std::string err = dcx::analyse(n, crow, col, coords, dim, leaf_max, P->S);
+    P->fuse_rows = default_fuse_rows(precision);
+    std::string err = dcx::analyse(n, crow, col, coords, dim, leaf_max, P->fuse_rows, P->S);
     if (!err.empty()) return fail(DCX_ERR_ARG, err);
     auto& S = P->S;
     HIP_TRY(hipGetDevice(&P->device));
     P->nbatch = nbatch;
     P->precision = precision;
     P->tsize = precision == DCX_FP32 ? 4 : 8;
-    std::vector<SupDev> sd(S.nsup);
-    for (int s = 0; s < S.nsup; ++s) {
-      sd[s] = SupDev{S.first[s], S.ns[s], S.rs[s], S.rptr[s], S.cptr[s], S.cptr[s + 1],
-                     S.aptr[s], S.aptr[s + 1], S.panel_off[s], S.front_off[s], S.upd_off[s]};
+    {
+      int rc = upload_fused(P.get());
+      if (rc) return rc;
     }
-    HIP_TRY(upload(&P->d_sup, sd));
     HIP_TRY(upload(&P->d_lvl, S.lvl_sup));
     HIP_TRY(upload(&P->d_relpos, S.relpos));
     HIP_TRY(upload(&P->d_cidx, S.cidx));
@@ -986,6 +1300,33 @@ int dcx_plan_set_refinement(dcx_plan* P, int passes) {
   return DCX_OK;
 }
 
+int dcx_plan_set_fusion(dcx_plan* P, int fuse_rows) {
+  if (!P || fuse_rows < 0) return fail(DCX_ERR_ARG, "bad arguments");
+  HIP_TRY(hipSetDevice(P->device));
+  HIP_TRY(hipDeviceSynchronize());
+  P->fuse_rows = fuse_rows;
+  dcx::build_fused(P->S, fuse_rows);
+  int rc = upload_fused(P);
+  if (rc) return rc;
+  P->upd_k = 0;  // the fused update buffer is resized on the next solve
+  if (P->d_active) {  // active lists follow the supernodes above the subtrees
+    std::vector<int> act(P->S.nsup);
+    HIP_TRY(hipMemcpy(act.data(), P->d_active, sizeof(int) * act.size(), hipMemcpyDeviceToHost));
+    std::vector<int> ptr(P->S.ndepth + 1, 0), lst;
+    for (int d = 0; d < P->S.ndepth; ++d) {
+      for (int q = P->S.ulvl_ptr[d]; q < P->S.ulvl_ptr[d + 1]; ++q)
+        if (act[P->S.ulvl_sup[q]]) lst.push_back(P->S.ulvl_sup[q]);
+      ptr[d + 1] = static_cast<int>(lst.size());
+    }
+    if (P->d_act_lvl) HIP_TRY(hipFree(P->d_act_lvl));
+    P->d_act_lvl = nullptr;
+    if (lst.empty()) lst.push_back(0);
+    HIP_TRY(upload(&P->d_act_lvl, lst));
+    P->act_lvl_ptr = ptr;
+  }
+  return DCX_OK;
+}
+
 int dcx_set_rhs_support(dcx_plan* P, const int32_t* rows, int nrows) {
   if (!P || (nrows > 0 && !rows)) return fail(DCX_ERR_ARG, "null argument");
   const auto& S = P->S;
@@ -996,8 +1337,8 @@ int dcx_set_rhs_support(dcx_plan* P, const int32_t* rows, int nrows) {
   }
   std::vector<int> ptr(S.ndepth + 1, 0), lst;
   for (int d = 0; d < S.ndepth; ++d) {
-    for (int q = S.lvl_ptr[d]; q < S.lvl_ptr[d + 1]; ++q)
-      if (act[S.lvl_sup[q]]) lst.push_back(S.lvl_sup[q]);
+    for (int q = S.ulvl_ptr[d]; q < S.ulvl_ptr[d + 1]; ++q)
+      if (act[S.ulvl_sup[q]]) lst.push_back(S.ulvl_sup[q]);
     ptr[d + 1] = static_cast<int>(lst.size());
   }
   if (P->d_active) HIP_TRY(hipFree(P->d_active));
@@ -1414,13 +1755,13 @@ struct dcx_symbolic {
 extern "C" {
 
 int dcx_symbolic_analyse(int n, const int32_t* crow, const int32_t* col, const double* coords,
-                         int dim, int leaf_max, dcx_symbolic** out) {
+                         int dim, int leaf_max, int fuse_rows, dcx_symbolic** out) {
   try {
     if (!out || !crow || !col) return fail(DCX_ERR_ARG, "null argument");
     *out = nullptr;
     if (coords && (dim < 1 || dim > 3)) return fail(DCX_ERR_ARG, "dim must be 1..3");
     std::unique_ptr<dcx_symbolic> Y(new dcx_symbolic());
-    std::string err = dcx::analyse(n, crow, col, coords, dim, leaf_max, Y->S);
+    std::string err = dcx::analyse(n, crow, col, coords, dim, leaf_max, fuse_rows, Y->S);
     if (!err.empty()) return fail(DCX_ERR_ARG, err);
     *out = Y.release();
     return DCX_OK;
@@ -1458,6 +1799,25 @@ int dcx_symbolic_view_get(const dcx_symbolic* Y, dcx_symbolic_view* o) {
   o->nnz_L = S.nnz_L;
   o->factor_flops = S.factor_flops;
   o->solve_flops_per_rhs = S.solve_flops_per_rhs;
+  o->nfused = S.nfused;
+  o->nfused_lv = static_cast<int>(S.f_lv.size()) - 1;
+  o->nfused_mem = static_cast<int>(S.f_mem.size());
+  o->nfused_grp = static_cast<int>(S.f_gb.size()) - 1;
+  o->f_max_lds_rows = S.f_max_lds_rows;
+  o->f_root = S.f_root.data();
+  o->f_lo = S.f_lo.data();
+  o->f_hi = S.f_hi.data();
+  o->f_lv_begin = S.f_lv_begin.data();
+  o->f_lv_end = S.f_lv_end.data();
+  o->f_tb_rows = S.f_tb_rows.data();
+  o->f_lv = S.f_lv.data();
+  o->f_mem = S.f_mem.data();
+  o->f_gptr = S.f_gptr.data();
+  o->f_gb = S.f_gb.data();
+  o->toff = S.toff.data();
+  o->lridx = S.lridx.data();
+  o->ulvl_ptr = S.ulvl_ptr.data();
+  o->ulvl_sup = S.ulvl_sup.data();
   return DCX_OK;
 }
 

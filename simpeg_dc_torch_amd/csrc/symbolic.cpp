@@ -253,8 +253,130 @@ class NestedDissection {
 
 }  // namespace
 
+// Fused bottom subtrees of the sweeps (see symbolic.h).
+void build_fused(Symbolic& S, int fuse_rows) {
+  const int ns_ = S.nsup;
+  S.in_fused.assign(ns_, -1);
+  S.fused_of_root.assign(ns_, -1);
+  S.toff.assign(ns_, 0);
+  S.lridx.assign(S.ridx.size(), 0);
+  S.f_root.clear(); S.f_lo.clear(); S.f_hi.clear(); S.f_lv_begin.clear();
+  S.f_lv_end.clear(); S.f_tb_rows.clear(); S.f_uoff.clear();
+  S.f_lv.clear(); S.f_mem.clear(); S.f_gptr.clear(); S.f_gb.clear();
+  S.f_max_lds_rows = 0;
+  std::vector<int> rows(ns_, 0), lo(ns_, 0);
+  for (int s = 0; s < ns_; ++s) {  // postorder: children first
+    rows[s] = S.ns[s];
+    lo[s] = S.first[s];
+    for (int q = S.cptr[s]; q < S.cptr[s + 1]; ++q) {
+      rows[s] += rows[S.cidx[q]];
+      lo[s] = std::min(lo[s], lo[S.cidx[q]]);
+    }
+  }
+  auto sublevels = [&](int root) {
+    std::vector<std::vector<int>> lv{{root}};
+    for (size_t d = 0; d < lv.size(); ++d) {
+      std::vector<int> nxt;
+      for (int s : lv[d])
+        for (int q = S.cptr[s]; q < S.cptr[s + 1]; ++q) nxt.push_back(S.cidx[q]);
+      if (!nxt.empty()) lv.push_back(nxt);
+    }
+    return lv;
+  };
+  auto tb_rows = [&](const std::vector<std::vector<int>>& lv) {
+    int tb = 0;
+    for (auto& l : lv) {
+      int t = 0;
+      for (int s : l) t += S.ns[s];
+      tb = std::max(tb, t);
+    }
+    return tb;
+  };
+  std::vector<char> fits(ns_, 0);
+  for (int s = 0; s < ns_; ++s)
+    fits[s] = rows[s] + S.rs[s] + tb_rows(sublevels(s)) <= fuse_rows;
+  std::vector<int> roots;
+  for (int s = 0; s < ns_; ++s)
+    if (fits[s] && (S.parent[s] < 0 || !fits[S.parent[s]]) && S.cptr[s + 1] > S.cptr[s])
+      roots.push_back(s);
+  S.nfused = static_cast<int>(roots.size());
+  std::vector<std::vector<int>> marks;  // per colour: sublevel stamp of each row
+  int stamp = 0;
+  int64_t uoff = 0;
+  for (int t = 0; t < S.nfused; ++t) {
+    const int root = roots[t];
+    const auto lv = sublevels(root);
+    const int flo = lo[root], fhi = S.first[root] + S.ns[root];
+    const int rsT = S.rs[root];
+    const int tb = tb_rows(lv);
+    S.f_root.push_back(root);
+    S.f_lo.push_back(flo);
+    S.f_hi.push_back(fhi);
+    S.f_tb_rows.push_back(tb);
+    S.f_uoff.push_back(uoff);
+    uoff += rsT;
+    S.f_max_lds_rows = std::max(S.f_max_lds_rows, (fhi - flo) + rsT + tb);
+    S.fused_of_root[root] = t;
+    S.f_lv_begin.push_back(static_cast<int>(S.f_lv.size()));
+    const int* Rt = S.ridx.data() + S.rptr[root];
+    for (const auto& l : lv) {
+      ++stamp;
+      std::vector<std::vector<int>> groups;
+      for (int s : l) {
+        S.in_fused[s] = t;
+        size_t c = 0;
+        for (; c < groups.size(); ++c) {
+          bool clash = false;
+          for (int q = S.rptr[s]; q < S.rptr[s + 1] && !clash; ++q) clash = marks[c][S.ridx[q]] == stamp;
+          if (!clash) break;
+        }
+        if (c == groups.size()) {
+          groups.emplace_back();
+          if (marks.size() <= c) marks.emplace_back(S.n, 0);
+        }
+        groups[c].push_back(s);
+        for (int q = S.rptr[s]; q < S.rptr[s + 1]; ++q) marks[c][S.ridx[q]] = stamp;
+      }
+      S.f_lv.push_back(static_cast<int>(S.f_mem.size()));
+      S.f_gptr.push_back(static_cast<int>(S.f_gb.size()));
+      int off = 0;
+      for (const auto& g : groups) {
+        S.f_gb.push_back(static_cast<int>(S.f_mem.size()));
+        for (int s : g) {
+          S.f_mem.push_back(s);
+          S.toff[s] = off;
+          off += S.ns[s];
+          for (int q = S.rptr[s]; q < S.rptr[s + 1]; ++q) {
+            const int r = S.ridx[q];
+            if (r >= flo && r < fhi) {
+              S.lridx[q] = r - flo;
+            } else {
+              const int pos = static_cast<int>(std::lower_bound(Rt, Rt + rsT, r) - Rt);
+              S.lridx[q] = (fhi - flo) + pos;  // rows outside lie in R_root
+            }
+          }
+        }
+      }
+    }
+    S.f_lv_end.push_back(static_cast<int>(S.f_lv.size()));
+  }
+  S.f_lv.push_back(static_cast<int>(S.f_mem.size()));
+  S.f_gptr.push_back(static_cast<int>(S.f_gb.size()));
+  S.f_gb.push_back(static_cast<int>(S.f_mem.size()));
+  S.f_upd_rows = uoff;
+  // levels of the supernodes above the subtrees
+  S.ulvl_ptr.assign(S.ndepth + 1, 0);
+  for (int s = 0; s < ns_; ++s)
+    if (S.in_fused[s] < 0) S.ulvl_ptr[S.depth[s] + 1]++;
+  for (int d = 0; d < S.ndepth; ++d) S.ulvl_ptr[d + 1] += S.ulvl_ptr[d];
+  S.ulvl_sup.assign(S.ulvl_ptr[S.ndepth], 0);
+  std::vector<int> fill(S.ulvl_ptr.begin(), S.ulvl_ptr.end() - 1);
+  for (int s = 0; s < ns_; ++s)
+    if (S.in_fused[s] < 0) S.ulvl_sup[fill[S.depth[s]]++] = s;
+}
+
 std::string analyse(int n, const int* crow, const int* col, const double* coords,
-                    int dim, int leaf_max, Symbolic& S) {
+                    int dim, int leaf_max, int fuse_rows, Symbolic& S) {
   if (n <= 0) return "matrix order must be positive";
   if (crow[0] != 0) return "crow[0] must be 0";
   const int nnz = crow[n];
@@ -439,6 +561,7 @@ std::string analyse(int n, const int* crow, const int* col, const double* coords
     S.solve_flops_per_rhs += 4.0 * (a + r) * a;
   }
   S.panel_size = po;
+  build_fused(S, fuse_rows);
   for (int d = 0; d < S.ndepth; ++d) {
     int64_t fo = 0, uo = 0;
     for (int q = S.lvl_ptr[d]; q < S.lvl_ptr[d + 1]; ++q) {

@@ -51,6 +51,24 @@ struct Symbolic {
   std::vector<int> pcrow, pcol, pval_idx;
   std::vector<int> e_prow, e_pcol;
 
+  // fused bottom subtrees (sweeps): maximal subtrees whose LDS block
+  // (subtree rows + root off-diagonal rows + widest sublevel) fits fuse_rows.
+  // Sublevels are root first; each sublevel's members are ordered by colour
+  // so that members of one colour have disjoint off-diagonal row sets.
+  int nfused = 0;
+  std::vector<int> f_root, f_lo, f_hi, f_lv_begin, f_lv_end, f_tb_rows;
+  std::vector<int64_t> f_uoff;      // rows in the fused update buffer
+  int64_t f_upd_rows = 0;
+  int f_max_lds_rows = 0;
+  std::vector<int> f_lv;            // sublevel -> first member (+ end sentinel per subtree)
+  std::vector<int> f_mem;           // members
+  std::vector<int> f_gptr, f_gb;    // colour groups: sublevel -> [f_gptr, f_gptr+1) into f_gb
+  std::vector<int> in_fused;        // supernode -> subtree index or -1
+  std::vector<int> fused_of_root;   // supernode -> subtree index if it is a fused root, else -1
+  std::vector<int> toff;            // member's offset in the sublevel scratch
+  std::vector<int> lridx;           // like ridx: row inside the subtree's LDS block
+  std::vector<int> ulvl_ptr, ulvl_sup;  // levels of the supernodes above the subtrees
+
   // statistics
   int64_t nnz_L = 0;                // dense supernode blocks, lower incl. diag
   double factor_flops = 0.0;        // per matrix
@@ -61,7 +79,9 @@ struct Symbolic {
 
 // Build the analysis.  coords (n x dim, row-major) may be null: the ordering
 // then falls back to level-structure bisection.  Returns "" or an error.
+// fuse_rows: LDS row budget of a fused subtree (0: no fusion).
 std::string analyse(int n, const int* crow, const int* col, const double* coords,
-                    int dim, int leaf_max, Symbolic& out);
+                    int dim, int leaf_max, int fuse_rows, Symbolic& out);
+void build_fused(Symbolic& S, int fuse_rows);
 
 }  // namespace dcx

@@ -31,7 +31,7 @@ from . import _lib
 #: elements per gather chunk of the CPU backward (dctorch/solver.py:72-73)
 _GRAD_CHUNK_ELEMS = int(os.environ.get("DCTORCH_GRAD_CHUNK", 8_000_000))
 #: leaf size of the nested dissection (dense leaf supernodes)
-DEFAULT_LEAF = int(os.environ.get("DCX_LEAF", 16))
+DEFAULT_LEAF = int(os.environ.get("DCX_LEAF", 48))
 
 
 def _rows_of(crow, n):
@@ -108,6 +108,12 @@ class HipBatchSolver:
     def set_refinement(self, passes):
         _lib.check(self._lib.dcx_plan_set_refinement(self._h, int(passes)), "set_refinement")
         self.refine = int(passes)
+
+    def set_fusion(self, rows):
+        """LDS row budget of the fused bottom subtrees of the sweeps (0: none)."""
+        with torch.cuda.device(self.device):
+            _lib.check(self._lib.dcx_plan_set_fusion(self._h, int(rows)), "set_fusion")
+        self.fuse_rows = int(rows)
 
     def set_rhs_support(self, rows):
         r, rp = _lib.i32(np.asarray(rows))

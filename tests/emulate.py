@@ -60,3 +60,80 @@ def solve(sym, Q, B):
             t = W[first[s]:first[s] + ns] - Q[s][ns:].T @ W[R]
             W[first[s]:first[s] + ns] = Q[s][:ns].T @ t
     return W
+
+
+def solve_fused(sym, Q, B):
+    """The sweeps with fused bottom subtrees, as the fused kernels run them:
+    each subtree works in a local block X (its rows [lo, hi) then the root's
+    off-diagonal rows), a sublevel scratch Tb and colour groups; the other
+    supernodes run level by level; fused roots pass their update through a
+    separate buffer."""
+    ns_, rs_, first = sym["ns"], sym["rs"], sym["first"]
+    rptr, ridx, lridx, toff = sym["rptr"], sym["ridx"], sym["lridx"], sym["toff"]
+    W = np.zeros_like(B, dtype=float)
+    upd = {}
+
+    def members(L):
+        return sym["f_mem"][sym["f_lv"][L]:sym["f_lv"][L + 1]]
+
+    # forward: fused subtrees first
+    for t in range(sym["nfused"]):
+        root, lo, hi = sym["f_root"][t], sym["f_lo"][t], sym["f_hi"][t]
+        nloc, rsT = hi - lo, rs_[root]
+        X = np.zeros((nloc + rsT, B.shape[1]))
+        X[:nloc] = B[lo:hi]
+        Tb = np.zeros((sym["f_tb_rows"][t], B.shape[1]))
+        for L in range(sym["f_lv_end"][t] - 1, sym["f_lv_begin"][t] - 1, -1):
+            for s in members(L):
+                jl = first[s] - lo
+                Tb[toff[s]:toff[s] + ns_[s]] = Q[s][:ns_[s]] @ X[jl:jl + ns_[s]]
+            for s in members(L):
+                jl = first[s] - lo
+                X[jl:jl + ns_[s]] = Tb[toff[s]:toff[s] + ns_[s]]
+            for g in range(sym["f_gptr"][L], sym["f_gptr"][L + 1]):
+                grp = sym["f_mem"][sym["f_gb"][g]:sym["f_gb"][g + 1]]
+                rows = [lridx[rptr[s]:rptr[s + 1]] for s in grp]
+                flat = np.concatenate(rows) if rows else np.zeros(0, int)
+                assert flat.size == np.unique(flat).size, "colour group rows overlap"
+                for s in grp:
+                    lr = lridx[rptr[s]:rptr[s + 1]]
+                    X[lr] -= Q[s][ns_[s]:] @ Tb[toff[s]:toff[s] + ns_[s]]
+        W[lo:hi] = X[:nloc]
+        upd[root] = X[nloc:].copy()
+    # forward: the levels above
+    up, usup = sym["ulvl_ptr"], sym["ulvl_sup"]
+    for d in range(sym["ndepth"] - 1, -1, -1):
+        for s in usup[up[d]:up[d + 1]]:
+            ns, rs = ns_[s], rs_[s]
+            f = np.zeros((ns + rs, B.shape[1]))
+            f[:ns] = B[first[s]:first[s] + ns]
+            for c in sym["cidx"][sym["cptr"][s]:sym["cptr"][s + 1]]:
+                f[sym["relpos"][rptr[c]:rptr[c + 1]]] += upd[c]
+            y = Q[s][:ns] @ f[:ns]
+            W[first[s]:first[s] + ns] = y
+            upd[s] = f[ns:] - Q[s][ns:] @ y
+    # backward: levels above, root first
+    for d in range(sym["ndepth"]):
+        for s in usup[up[d]:up[d + 1]]:
+            ns = ns_[s]
+            R = ridx[rptr[s]:rptr[s + 1]]
+            t_ = W[first[s]:first[s] + ns] - Q[s][ns:].T @ W[R]
+            W[first[s]:first[s] + ns] = Q[s][:ns].T @ t_
+    # backward: fused subtrees
+    for t in range(sym["nfused"]):
+        root, lo, hi = sym["f_root"][t], sym["f_lo"][t], sym["f_hi"][t]
+        nloc, rsT = hi - lo, rs_[root]
+        X = np.zeros((nloc + rsT, B.shape[1]))
+        X[:nloc] = W[lo:hi]
+        X[nloc:] = W[ridx[rptr[root]:rptr[root + 1]]]
+        Tb = np.zeros((sym["f_tb_rows"][t], B.shape[1]))
+        for L in range(sym["f_lv_begin"][t], sym["f_lv_end"][t]):
+            for s in members(L):
+                jl, ns = first[s] - lo, ns_[s]
+                lr = lridx[rptr[s]:rptr[s + 1]]
+                Tb[toff[s]:toff[s] + ns] = X[jl:jl + ns] - Q[s][ns:].T @ X[lr]
+            for s in members(L):
+                jl, ns = first[s] - lo, ns_[s]
+                X[jl:jl + ns] = Q[s][:ns].T @ Tb[toff[s]:toff[s] + ns]
+        W[lo:hi] = X[:nloc]
+    return W

@@ -104,6 +104,40 @@ def test_symbolic_structure_and_emulated_solve(coords):
     assert rel(x, spla.splu(A.tocsc()).solve(B)) < 1e-11
 
 
+@pytest.mark.parametrize("fuse_rows", [120, 320, 600])
+def test_fused_subtree_structure_and_emulated_solve(fuse_rows):
+    """Fused bottom subtrees: structure invariants and a numpy run of the
+    fused kernels' algorithm == SciPy."""
+    mesh, survey, m = case("small_padded")
+    kys, wq = quad(survey, 3)
+    P = Problem2DMaps(mesh, survey, kys)
+    sym = _lib.symbolic(P.crow, P.col, P.node_coords(), leaf_max=16, fuse_rows=fuse_rows)
+    assert sym["nfused"] > 0
+    assert sym["f_max_lds_rows"] <= fuse_rows
+    nf = sym["ns"] + sym["rs"]
+    covered = np.zeros(P.nN, bool)
+    for t in range(sym["nfused"]):
+        lo, hi = sym["f_lo"][t], sym["f_hi"][t]
+        assert not covered[lo:hi].any()
+        covered[lo:hi] = True
+    upper = np.zeros(sym["nsup"], bool)
+    upper[sym["ulvl_sup"]] = True
+    mem = np.zeros(sym["nsup"], bool)
+    mem[sym["f_mem"]] = True
+    assert np.all(upper ^ mem)                   # every supernode exactly once
+    V = P.values(np.exp(-m))
+    rng = np.random.default_rng(0)
+    B = rng.standard_normal((P.nN, 3))
+    A = sp.csr_matrix((V[1], P.col, P.crow), shape=(P.nN, P.nN))
+    Q = emulate.factor(sym, V[1])
+    perm = sym["perm"]
+    X = emulate.solve_fused(sym, Q, B[perm])
+    x = np.empty_like(X)
+    x[perm] = X
+    assert rel(x, spla.splu(A.tocsc()).solve(B)) < 1e-11
+    del nf
+
+
 def test_symbolic_laplacian_fixture():
     A, b = laplacian_fixture()
     sym = _lib.symbolic(A.indptr, A.indices, None, leaf_max=16)
