@@ -65,16 +65,16 @@ constexpr int kThreads = 256;
 // ---------------------------------------------------------------------------
 // numeric factorisation, one launch per depth level (deepest first)
 // grid (supernodes at level, batch), 256 threads.
-// F (nf x nf, column-major, lower triangle used) is assembled in the depth
-// buffer, factored by panels of kNB columns staged in LDS with 4x4
-// register-tiled trailing updates, then the panel
-//     P_s = [ L_JJ^{-1} ; L_RJ ]   (nf x ns, column-major)
-// is written: the diagonal blocks of the inverse (kNBI x kNBI) in LDS, the
-// off-diagonal blocks by blocked substitution.  F's (R,R) block is left
-// holding the Schur complement (this front's update block).
+// The front F (column-major, leading dimension nf + ns) holds the lower
+// triangle of [J; R] in its first nf rows and, below them, ns extra rows E
+// that start as the identity in the J columns.  Eliminating the ns J columns
+// (panels of kNB columns staged in LDS, one barrier per column, 4x4
+// register-tiled trailing updates) leaves L_JJ, L_RJ, the Schur complement in
+// the (R,R) block -- this front's update block -- and L_EJ = L_JJ^{-T} in the
+// E rows, i.e. the inverse the sweeps need, without a separate inversion.
+// The panel written for the sweeps is P_s = [ L_JJ^{-1} ; L_RJ ] (nf x ns).
 // ---------------------------------------------------------------------------
-constexpr int kNB = 16;   // panel width
-constexpr int kNBI = 64;  // diagonal block of the inverse
+constexpr int kNB = 32;  // panel width
 
 template <typename T>
 __global__ __launch_bounds__(kThreads) void k_factor_level(
@@ -83,74 +83,113 @@ __global__ __launch_bounds__(kThreads) void k_factor_level(
     const int* __restrict__ a_e, const int* __restrict__ a_row,
     const int* __restrict__ a_col, const double* __restrict__ vals, int64_t nnz,
     T* __restrict__ fcur, int64_t fcur_bs, const T* __restrict__ fprev,
-    int64_t fprev_bs, T* __restrict__ panels, int64_t panel_bs, int ldi,
+    int64_t fprev_bs, T* __restrict__ panels, int64_t panel_bs,
     int* __restrict__ info) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  T* sm = reinterpret_cast<T*>(smem_raw);
+  T* pan = reinterpret_cast<T*>(smem_raw);
   const int s = lvl[blockIdx.x];
   const int b = blockIdx.y;
   const SupDev S = sup[s];
-  const int ns = S.ns, nf = S.ns + S.rs;
+  const int ns = S.ns, rs = S.rs, nf = S.ns + S.rs, ld = nf + S.ns;
   const int tid = threadIdx.x;
   T* F = fcur + b * fcur_bs + S.front_off;
-  const int64_t nf2 = static_cast<int64_t>(nf) * nf;
-  T* Xs = F + nf2;  // scratch of the blocked inverse (ns x ns)
-  for (int64_t i = tid; i < nf2; i += kThreads) F[i] = T(0);
+  const int64_t fsz = static_cast<int64_t>(ld) * nf;
+  for (int64_t i = tid; i < fsz; i += kThreads) F[i] = T(0);
   __syncthreads();
+  for (int j = tid; j < ns; j += kThreads) F[(nf + j) + static_cast<int64_t>(j) * ld] = T(1);
   const double* v = vals + b * nnz;
   for (int q = S.a_begin + tid; q < S.a_end; q += kThreads)
-    F[a_row[q] + static_cast<int64_t>(a_col[q]) * nf] = static_cast<T>(v[a_e[q]]);
+    F[a_row[q] + static_cast<int64_t>(a_col[q]) * ld] = static_cast<T>(v[a_e[q]]);
   __syncthreads();
   for (int cc = S.c_begin; cc < S.c_end; ++cc) {
     const SupDev C = sup[cidx[cc]];
-    const int rc = C.rs, nfc = C.ns + C.rs;
+    const int rc = C.rs, ldc = C.ns + C.rs + C.ns;
     const T* Fc = fprev + b * fprev_bs + C.front_off;
     const int* rp = relpos + C.rptr;
     const int64_t rc2 = static_cast<int64_t>(rc) * rc;
     for (int64_t t = tid; t < rc2; t += kThreads) {
       const int i = static_cast<int>(t % rc), j = static_cast<int>(t / rc);
       if (i < j) continue;
-      F[rp[i] + static_cast<int64_t>(rp[j]) * nf] +=
-          Fc[(C.ns + i) + static_cast<int64_t>(C.ns + j) * nfc];
+      F[rp[i] + static_cast<int64_t>(rp[j]) * ld] +=
+          Fc[(C.ns + i) + static_cast<int64_t>(C.ns + j) * ldc];
     }
     __syncthreads();
   }
-  // ---- partial Cholesky by panels --------------------------------------
   constexpr int LDP = kNB + 1;
   for (int j0 = 0; j0 < ns; j0 += kNB) {
     const int jb = min(kNB, ns - j0);
-    const int m = nf - j0;
+    const int m = ld - j0;  // panel rows: J, R and E rows from j0 on
     for (int idx = tid; idx < m * jb; idx += kThreads) {
       const int r = idx % m, t = idx / m;
-      sm[r * LDP + t] = F[(j0 + r) + static_cast<int64_t>(j0 + t) * nf];
+      pan[r * LDP + t] = F[(j0 + r) + static_cast<int64_t>(j0 + t) * ld];
     }
-    __syncthreads();
+    // one barrier per column: each thread owns rows tid, tid + 256, ...; the
+    // scaled column t is kept in registers and stored at the next step, so
+    // column t is only read during step t
+    T keep0 = T(0), keep1 = T(0);
     for (int t = 0; t < jb; ++t) {
-      T d = sm[t * LDP + t];
+      __syncthreads();
+      if (t > 0) {
+        if (tid >= t - 1 && tid < m) pan[tid * LDP + t - 1] = keep0;
+        if (tid + kThreads < m) pan[(tid + kThreads) * LDP + t - 1] = keep1;
+      }
+      T d = pan[t * LDP + t];
       if (!(d > T(0))) {
         if (tid == 0) atomicMax(info, 1);
         d = T(1);
       }
-      d = sqrt(d);
-      __syncthreads();
-      for (int r = t + tid; r < m; r += kThreads) sm[r * LDP + t] = (r == t) ? d : sm[r * LDP + t] / d;
-      __syncthreads();
-      const int w = jb - t - 1;
-      for (int idx = tid; idx < m * w; idx += kThreads) {
-        const int r = idx % m, l = t + 1 + idx / m;
-        if (r >= l) sm[r * LDP + l] -= sm[r * LDP + t] * sm[l * LDP + t];
+      const T sq = sqrt(d);
+      const T dinv = T(1) / sq;
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        const int r = tid + q * kThreads;
+        if (r < t || r >= m) continue;
+        T vr;
+        if (r == t) {
+          vr = sq;
+        } else {
+          vr = pan[r * LDP + t] * dinv;
+          const int lmax = min(r, jb - 1);
+          for (int l = t + 1; l <= lmax; ++l) pan[r * LDP + l] -= vr * (pan[l * LDP + t] * dinv);
+        }
+        if (q == 0) keep0 = vr; else keep1 = vr;
       }
-      __syncthreads();
     }
+    __syncthreads();
+    if (tid >= jb - 1 && tid < m) pan[tid * LDP + jb - 1] = keep0;
+    if (tid + kThreads < m) pan[(tid + kThreads) * LDP + jb - 1] = keep1;
+    __syncthreads();
     for (int idx = tid; idx < m * jb; idx += kThreads) {
       const int r = idx % m, t = idx / m;
-      if (r >= t) F[(j0 + r) + static_cast<int64_t>(j0 + t) * nf] = sm[r * LDP + t];
+      if (r >= t) F[(j0 + r) + static_cast<int64_t>(j0 + t) * ld] = pan[r * LDP + t];
     }
-    const int mt = m - jb;
+    // trailing update: (a) lower triangle of rows/cols [j0+jb, nf);
+    // (b) E rows x remaining J columns [j0+jb, ns)
+    const int mt = nf - j0 - jb;
     const int nt = (mt + 3) >> 2;
-    for (int tt = tid; tt < nt * nt; tt += kThreads) {
-      const int ti = tt % nt, tj = tt / nt;
-      if (ti < tj) continue;
+    const int ntri = nt * (nt + 1) / 2;
+    const int ncj = ns - j0 - jb;
+    const int ntej = ((ns + 3) >> 2) * ((ncj + 3) >> 2);
+    for (int tt = tid; tt < ntri + ntej; tt += kThreads) {
+      int ri0, ci0, rmax, cmax;  // panel-local row / column origins and bounds
+      bool tri;
+      if (tt < ntri) {  // lower-triangle tile (ti >= tj), column-major walk
+        int tj = 0, rem = tt;
+        while (rem >= nt - tj) { rem -= nt - tj; ++tj; }
+        const int ti = tj + rem;
+        ri0 = jb + ti * 4;
+        ci0 = jb + tj * 4;
+        rmax = jb + mt;
+        cmax = jb + mt;
+        tri = true;
+      } else {
+        const int u = tt - ntri, ncb = (ncj + 3) >> 2;
+        ri0 = (nf - j0) + (u / ncb) * 4;
+        ci0 = jb + (u % ncb) * 4;
+        rmax = m;
+        cmax = jb + ncj;
+        tri = false;
+      }
       T acc[4][4];
 #pragma unroll
       for (int q = 0; q < 4; ++q)
@@ -160,9 +199,8 @@ __global__ __launch_bounds__(kThreads) void k_factor_level(
         T av[4], bv[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-          const int ri = ti * 4 + q, rj = tj * 4 + q;
-          av[q] = ri < mt ? sm[(jb + ri) * LDP + t] : T(0);
-          bv[q] = rj < mt ? sm[(jb + rj) * LDP + t] : T(0);
+          av[q] = ri0 + q < rmax ? pan[(ri0 + q) * LDP + t] : T(0);
+          bv[q] = ci0 + q < cmax ? pan[(ci0 + q) * LDP + t] : T(0);
         }
 #pragma unroll
         for (int q = 0; q < 4; ++q)
@@ -173,66 +211,23 @@ __global__ __launch_bounds__(kThreads) void k_factor_level(
       for (int q = 0; q < 4; ++q)
 #pragma unroll
         for (int p = 0; p < 4; ++p) {
-          const int i = ti * 4 + q, l = tj * 4 + p;
-          if (i < mt && l < mt && i >= l)
-            F[(j0 + jb + i) + static_cast<int64_t>(j0 + jb + l) * nf] -= acc[q][p];
+          const int i = ri0 + q, l = ci0 + p;
+          if (i < rmax && l < cmax && (!tri || i >= l))
+            F[(j0 + i) + static_cast<int64_t>(j0 + l) * ld] -= acc[q][p];
         }
     }
     __syncthreads();
   }
-  // ---- panel: L_RJ copy, then L_JJ^{-1} -----------------------------------
+  // panel for the sweeps: rows [0, ns) = L_JJ^{-1} (from the E rows), then L_RJ
   T* P = panels + b * panel_bs + S.panel_off;
-  const int rs = S.rs;
-  for (int64_t idx = tid; idx < static_cast<int64_t>(rs) * ns; idx += kThreads) {
-    const int r = static_cast<int>(idx % rs), c = static_cast<int>(idx / rs);
-    P[(ns + r) + static_cast<int64_t>(c) * nf] = F[(ns + r) + static_cast<int64_t>(c) * nf];
-  }
-  T* Lb = sm;  // ldi = min(kNBI, widest supernode of the level)
-  T* Xb = sm + ldi * ldi;
-  const int nblk = (ns + kNBI - 1) / kNBI;
-  for (int ib = 0; ib < nblk; ++ib) {
-    const int o = ib * kNBI, nb = min(kNBI, ns - o);
-    __syncthreads();
-    for (int idx = tid; idx < nb * nb; idx += kThreads) {
-      const int r = idx % nb, c = idx / nb;
-      Lb[r * ldi + c] = (r >= c) ? F[(o + r) + static_cast<int64_t>(o + c) * nf] : T(0);
-    }
-    __syncthreads();
-    if (tid < nb) {  // one column per thread, rows in lockstep
-      const int c = tid;
-      for (int i = 0; i < c; ++i) Xb[i * ldi + c] = T(0);
-      Xb[c * ldi + c] = T(1) / Lb[c * ldi + c];
-      for (int i = c + 1; i < nb; ++i) {
-        T acc = T(0);
-        for (int k = c; k < i; ++k) acc += Lb[i * ldi + k] * Xb[k * ldi + c];
-        Xb[i * ldi + c] = -acc / Lb[i * ldi + i];
-      }
-    }
-    __syncthreads();
-    for (int idx = tid; idx < nb * (ns - o); idx += kThreads) {
-      const int r = idx % nb, c = o + idx / nb;
-      P[(o + r) + static_cast<int64_t>(c) * nf] = (c < o + nb) ? Xb[r * ldi + (c - o)] : T(0);
-    }
-  }
-  // off-diagonal blocks: X_ib,<o = -X_ib,ib * (L_ib,<o X_<o,<o)
-  for (int ib = 1; ib < nblk; ++ib) {
-    const int o = ib * kNBI, nb = min(kNBI, ns - o);
-    __syncthreads();
-    for (int idx = tid; idx < nb * o; idx += kThreads) {
-      const int r = idx % nb, c = idx / nb;
-      T acc = T(0);
-      for (int k = c; k < o; ++k)
-        acc += F[(o + r) + static_cast<int64_t>(k) * nf] * P[k + static_cast<int64_t>(c) * nf];
-      Xs[r + static_cast<int64_t>(c) * nb] = acc;
-    }
-    __syncthreads();
-    for (int idx = tid; idx < nb * o; idx += kThreads) {
-      const int r = idx % nb, c = idx / nb;
-      T acc = T(0);
-      for (int q = 0; q <= r; ++q)
-        acc += P[(o + r) + static_cast<int64_t>(o + q) * nf] * Xs[q + static_cast<int64_t>(c) * nb];
-      P[(o + r) + static_cast<int64_t>(c) * nf] = -acc;
-    }
+  for (int64_t idx = tid; idx < static_cast<int64_t>(nf) * ns; idx += kThreads) {
+    const int r = static_cast<int>(idx % nf), c = static_cast<int>(idx / nf);
+    T val;
+    if (r < ns)
+      val = r >= c ? F[(nf + c) + static_cast<int64_t>(r) * ld] : T(0);
+    else
+      val = F[r + static_cast<int64_t>(c) * ld];
+    P[r + static_cast<int64_t>(c) * nf] = val;
   }
 }
 
@@ -1022,22 +1017,19 @@ int launch_factor(dcx_plan* P, hipStream_t st) {
   for (int d = S.ndepth - 1; d >= 0; --d) {
     const int cnt = S.lvl_ptr[d + 1] - S.lvl_ptr[d];
     if (!cnt) continue;
-    int maxnf = 0, maxns = 0;
+    int maxld = 0;
     for (int q = S.lvl_ptr[d]; q < S.lvl_ptr[d + 1]; ++q) {
       const int s = S.lvl_sup[q];
-      maxnf = std::max(maxnf, S.ns[s] + S.rs[s]);
-      maxns = std::max(maxns, S.ns[s]);
+      maxld = std::max(maxld, 2 * S.ns[s] + S.rs[s]);
     }
-    const int ldi = std::min(kNBI, maxns);
-    const size_t lds = std::max<size_t>(static_cast<size_t>(maxnf) * (kNB + 1),
-                                        2 * static_cast<size_t>(ldi) * ldi) * sizeof(T);
-    if (lds > kLdsBudget) return fail(DCX_ERR_ARG, "front too large for the factor kernel");
+    const size_t lds = static_cast<size_t>(maxld) * (kNB + 1) * sizeof(T);
+    if (lds > 160 * 1024) return fail(DCX_ERR_ARG, "front too large for the factor kernel");
     dim3 grid(cnt, P->nbatch);
     hipLaunchKernelGGL(k_factor_level<T>, grid, dim3(kThreads), lds, st, P->d_sup,
                        P->d_lvl + S.lvl_ptr[d], P->d_relpos, P->d_cidx, P->d_a_e,
                        P->d_a_row, P->d_a_col, P->d_vals, static_cast<int64_t>(S.nnz),
                        fr[d & 1], fbs[d & 1], fr[(d + 1) & 1], fbs[(d + 1) & 1], panels,
-                       S.panel_size, ldi, P->d_info);
+                       S.panel_size, P->d_info);
   }
   HIP_TRY(hipGetLastError());
   return DCX_OK;
@@ -1079,7 +1071,12 @@ void sweep_tc(dcx_plan* P, T* W, int k, int maxnf, bool fwd, int d, const int* l
 
 // widest column tile whose LDS block fits and whose register tiles fit
 int pick_tc(int maxnf, int maxns, size_t tsize) {
+  static const int cap = [] {
+    const char* e = std::getenv("DCX_SWEEP_TC");
+    return e ? std::atoi(e) : 32;
+  }();
   for (int tc : {64, 32, 16}) {
+    if (tc > cap) continue;
     const int tiles = ((maxns + 15) / 16) * (tc / 16);
     if (static_cast<size_t>(maxnf) * tc * tsize <= kLdsBudget && (tiles + 3) / 4 <= kMaxTiles)
       return tc;

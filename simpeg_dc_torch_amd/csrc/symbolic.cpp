@@ -568,7 +568,7 @@ std::string analyse(int n, const int* crow, const int* col, const double* coords
       int s = S.lvl_sup[q];
       int64_t nf = S.ns[s] + S.rs[s];
       S.front_off[s] = fo;
-      fo += nf * nf + static_cast<int64_t>(S.ns[s]) * S.ns[s];  // front + inverse scratch
+      fo += (nf + S.ns[s]) * nf;  // front with the identity rows below it
       S.upd_off[s] = uo;
       uo += S.rs[s];
     }
