@@ -294,8 +294,10 @@ struct RowId {
 };
 
 // forward sweep (deepest level first): f = [b_J; sum children updates] in LDS,
-// y_J = L_JJ^{-1} f_J (to W), u_s = f_R - L_RJ y_J (to the update buffer)
-template <typename T, int TC>
+// y_J = L_JJ^{-1} f_J (to W), u_s = f_R - L_RJ y_J (to the update buffer).
+// PL: the supernode's panel is staged in LDS once and the workgroup walks
+// `ct` column tiles with it; otherwise the panel is read from global memory.
+template <typename T, int TC, bool PL>
 __global__ __launch_bounds__(kThreads) void k_fwd_level(
     const SupDev* __restrict__ sup, const int* __restrict__ lvl,
     const int* __restrict__ relpos, const int* __restrict__ cidx,
@@ -303,155 +305,166 @@ __global__ __launch_bounds__(kThreads) void k_fwd_level(
     int64_t panel_bs, const double* __restrict__ Bsrc, int64_t b_bs,
     const int* __restrict__ brow, T* __restrict__ W, int64_t w_bs, int k,
     T* __restrict__ ucur, int64_t ucur_bs, const T* __restrict__ uprev,
-    int64_t uprev_bs, const T* __restrict__ ufz, int64_t ufz_bs) {
+    int64_t uprev_bs, const T* __restrict__ ufz, int64_t ufz_bs, int ct, int pmax, int nfmax) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  T* f = reinterpret_cast<T*>(smem_raw);
   const int s = lvl[blockIdx.x];
   const int b = blockIdx.y;
-  const int col0 = blockIdx.z * TC;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const SupDev S = sup[s];
   const int ns = S.ns, rs = S.rs, nf = S.ns + S.rs;
+  const T* Pg = panels + b * panel_bs + S.panel_off;
+  T* Ps = reinterpret_cast<T*>(smem_raw);
+  T* f = PL ? Ps + pmax : Ps;
+  T* Y = f + nfmax * TC;  // phase-1 scratch (ns x TC)
+  if (PL) {
+    const int np = nf * ns;
+    for (int idx = tid; idx < np; idx += kThreads) Ps[idx] = Pg[idx];
+  }
+  const T* P = PL ? Ps : Pg;
+  auto Linv = [=](int r, int kk) { return P[r + kk * nf]; };
+  auto Lrj = [=](int r, int kk) { return P[(ns + r) + kk * nf]; };
   T* Wb = W + b * w_bs;
   const double* Bb = Bsrc + b * b_bs;
-  for (int idx = tid; idx < nf * TC; idx += kThreads) {
-    const int r = idx / TC, c = idx % TC;
-    T v = T(0);
-    if (r < ns && col0 + c < k) {
-      const int gr = brow ? brow[S.first + r] : S.first + r;
-      v = static_cast<T>(Bb[static_cast<int64_t>(gr) * k + col0 + c]);
-    }
-    f[idx] = v;
-  }
-  __syncthreads();
-  for (int cc = S.c_begin; cc < S.c_end; ++cc) {
-    const int c = cidx[cc];
-    if (active && !active[c]) continue;
-    const SupDev C = sup[c];
-    const T* u = C.fz_uoff >= 0 ? ufz + b * ufz_bs + C.fz_uoff * k
-                                : uprev + b * uprev_bs + C.upd_off * k;
-    const int* rp = relpos + C.rptr;
-    for (int idx = tid; idx < C.rs * TC; idx += kThreads) {
-      const int r = idx / TC, cl = idx % TC;
-      if (col0 + cl < k) f[rp[r] * TC + cl] += u[static_cast<int64_t>(r) * k + col0 + cl];
-    }
-    __syncthreads();
-  }
-  const T* P = panels + b * panel_bs + S.panel_off;
-  auto Linv = [=](int r, int kk) { return P[r + static_cast<int64_t>(kk) * nf]; };
-  auto Lrj = [=](int r, int kk) { return P[(ns + r) + static_cast<int64_t>(kk) * nf]; };
+  T* uo = ucur + b * ucur_bs + S.upd_off * k;
   constexpr int NCT = TC / 16;
   const int nt1 = ((ns + 15) / 16) * NCT;
-  typename Mma<T>::v4 acc[kMaxTiles];
-#pragma unroll
-  for (int q = 0; q < kMaxTiles; ++q) {
-    const int t = wave + 4 * q;
-    if (t < nt1) acc[q] = mma_tile<T, TC>(Linv, ns, ns, f, RowId(), (t / NCT) * 16, (t % NCT) * 16, lane);
-  }
-  __syncthreads();
-#pragma unroll
-  for (int q = 0; q < kMaxTiles; ++q) {
-    const int t = wave + 4 * q;
-    if (t < nt1) {
-      const int r0 = (t / NCT) * 16, c = (t % NCT) * 16 + (lane & 15);
+  const int nt2 = ((rs + 15) / 16) * NCT;
+  for (int tile = 0; tile < ct; ++tile) {
+    const int col0 = (blockIdx.z * ct + tile) * TC;
+    if (col0 >= k) break;
+    __syncthreads();  // previous tile done with f (and the panel staged)
+    for (int idx = tid; idx < nf * TC; idx += kThreads) {
+      const int r = idx / TC, c = idx % TC;
+      T v = T(0);
+      if (r < ns && col0 + c < k) {
+        const int gr = brow ? brow[S.first + r] : S.first + r;
+        v = static_cast<T>(Bb[static_cast<int64_t>(gr) * k + col0 + c]);
+      }
+      f[idx] = v;
+    }
+    __syncthreads();
+    for (int cc = S.c_begin; cc < S.c_end; ++cc) {
+      const int c = cidx[cc];
+      if (active && !active[c]) continue;
+      const SupDev C = sup[c];
+      const T* u = C.fz_uoff >= 0 ? ufz + b * ufz_bs + C.fz_uoff * k
+                                  : uprev + b * uprev_bs + C.upd_off * k;
+      const int* rp = relpos + C.rptr;
+      for (int idx = tid; idx < C.rs * TC; idx += kThreads) {
+        const int r = idx / TC, cl = idx % TC;
+        if (col0 + cl < k) f[rp[r] * TC + cl] += u[static_cast<int64_t>(r) * k + col0 + cl];
+      }
+      __syncthreads();
+    }
+    for (int t = wave; t < nt1; t += 4) {  // y_J = L_JJ^{-1} f_J into the scratch Y
+      const int tr = (t / NCT) * 16, tc = (t % NCT) * 16;
+      typename Mma<T>::v4 a1 = mma_tile<T, TC>(Linv, ns, ns, f, RowId(), tr, tc, lane);
+      const int c = tc + (lane & 15);
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
-        const int r = r0 + Mma<T>::row(lane, i);
-        if (r < ns) {
-          f[r * TC + c] = acc[q][i];
-          if (col0 + c < k) Wb[static_cast<int64_t>(S.first + r) * k + col0 + c] = acc[q][i];
-        }
+        const int r = tr + Mma<T>::row(lane, i);
+        if (r < ns) Y[r * TC + c] = a1[i];
       }
     }
-  }
-  if (rs == 0) return;
-  __syncthreads();
-  T* uo = ucur + b * ucur_bs + S.upd_off * k;
-  const int nt2 = ((rs + 15) / 16) * NCT;
-  for (int t = wave; t < nt2; t += 4) {
-    const int tr = (t / NCT) * 16, tc = (t % NCT) * 16;
-    typename Mma<T>::v4 a2 = mma_tile<T, TC>(Lrj, rs, ns, f, RowId(), tr, tc, lane);
-    const int c = tc + (lane & 15);
+    __syncthreads();
+    for (int idx = tid; idx < ns * TC; idx += kThreads) {
+      const int r = idx / TC, c = idx % TC;
+      const T y = Y[idx];
+      f[idx] = y;
+      if (col0 + c < k) Wb[static_cast<int64_t>(S.first + r) * k + col0 + c] = y;
+    }
+    if (rs == 0) continue;
+    __syncthreads();
+    for (int t = wave; t < nt2; t += 4) {
+      const int tr = (t / NCT) * 16, tc = (t % NCT) * 16;
+      typename Mma<T>::v4 a2 = mma_tile<T, TC>(Lrj, rs, ns, f, RowId(), tr, tc, lane);
+      const int c = tc + (lane & 15);
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int r = tr + Mma<T>::row(lane, i);
-      if (r < rs && col0 + c < k)
-        uo[static_cast<int64_t>(r) * k + col0 + c] = f[(ns + r) * TC + c] - a2[i];
+      for (int i = 0; i < 4; ++i) {
+        const int r = tr + Mma<T>::row(lane, i);
+        if (r < rs && col0 + c < k)
+          uo[static_cast<int64_t>(r) * k + col0 + c] = f[(ns + r) * TC + c] - a2[i];
+      }
     }
   }
 }
 
 // backward sweep (root first): g = [y_J; x_R] in LDS,
-// t = y_J - L_RJ^T x_R, x_J = L_JJ^{-T} t (to W)
-template <typename T, int TC>
+// t = y_J - L_RJ^T x_R, x_J = L_JJ^{-T} t (to W).  PL / ct as in the forward.
+template <typename T, int TC, bool PL>
 __global__ __launch_bounds__(kThreads) void k_bwd_level(
     const SupDev* __restrict__ sup, const int* __restrict__ lvl,
     const int* __restrict__ ridx, const int* __restrict__ active,
     const T* __restrict__ panels, int64_t panel_bs, T* __restrict__ W,
     int64_t w_bs, int k, double* __restrict__ X64, const int* __restrict__ xrow,
-    int xmode) {
+    int xmode, int ct, int pmax, int nfmax) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  T* g = reinterpret_cast<T*>(smem_raw);
   const int s = lvl[blockIdx.x];
   const int b = blockIdx.y;
-  const int col0 = blockIdx.z * TC;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const SupDev S = sup[s];
   const int ns = S.ns, rs = S.rs, nf = S.ns + S.rs;
   const bool act = !active || active[s];
-  T* Wb = W + b * w_bs;
-  for (int idx = tid; idx < nf * TC; idx += kThreads) {
-    const int r = idx / TC, c = idx % TC;
-    T val = T(0);
-    if (col0 + c < k) {
-      if (r < ns)
-        val = act ? Wb[static_cast<int64_t>(S.first + r) * k + col0 + c] : T(0);
-      else
-        val = Wb[static_cast<int64_t>(ridx[S.rptr + r - ns]) * k + col0 + c];
-    }
-    g[idx] = val;
+  const T* Pg = panels + b * panel_bs + S.panel_off;
+  T* Ps = reinterpret_cast<T*>(smem_raw);
+  T* g = PL ? Ps + pmax : Ps;
+  T* Y = g + nfmax * TC;  // phase-1 scratch (ns x TC)
+  if (PL) {
+    const int np = nf * ns;
+    for (int idx = tid; idx < np; idx += kThreads) Ps[idx] = Pg[idx];
   }
-  __syncthreads();
-  const T* P = panels + b * panel_bs + S.panel_off;
-  auto LrjT = [=](int i, int kk) { return P[(ns + kk) + static_cast<int64_t>(i) * nf]; };
-  auto LinvT = [=](int i, int kk) { return P[kk + static_cast<int64_t>(i) * nf]; };
+  const T* P = PL ? Ps : Pg;
+  auto LrjT = [=](int i, int kk) { return P[(ns + kk) + i * nf]; };
+  auto LinvT = [=](int i, int kk) { return P[kk + i * nf]; };
+  T* Wb = W + b * w_bs;
   constexpr int NCT = TC / 16;
   const int nt1 = ((ns + 15) / 16) * NCT;
-  if (rs > 0) {
-    typename Mma<T>::v4 acc[kMaxTiles];
-#pragma unroll
-    for (int q = 0; q < kMaxTiles; ++q) {
-      const int t = wave + 4 * q;
-      if (t < nt1) acc[q] = mma_tile<T, TC>(LrjT, ns, rs, g + ns * TC, RowId(), (t / NCT) * 16, (t % NCT) * 16, lane);
+  for (int tile = 0; tile < ct; ++tile) {
+    const int col0 = (blockIdx.z * ct + tile) * TC;
+    if (col0 >= k) break;
+    __syncthreads();
+    for (int idx = tid; idx < nf * TC; idx += kThreads) {
+      const int r = idx / TC, c = idx % TC;
+      T val = T(0);
+      if (col0 + c < k) {
+        if (r < ns)
+          val = act ? Wb[static_cast<int64_t>(S.first + r) * k + col0 + c] : T(0);
+        else
+          val = Wb[static_cast<int64_t>(ridx[S.rptr + r - ns]) * k + col0 + c];
+      }
+      g[idx] = val;
     }
     __syncthreads();
-#pragma unroll
-    for (int q = 0; q < kMaxTiles; ++q) {
-      const int t = wave + 4 * q;
-      if (t < nt1) {
-        const int r0 = (t / NCT) * 16, c = (t % NCT) * 16 + (lane & 15);
+    if (rs > 0) {  // t = y_J - L_RJ^T x_R into the scratch Y, then into g
+      for (int t = wave; t < nt1; t += 4) {
+        const int tr = (t / NCT) * 16, tc = (t % NCT) * 16;
+        typename Mma<T>::v4 a1 = mma_tile<T, TC>(LrjT, ns, rs, g + ns * TC, RowId(), tr, tc, lane);
+        const int c = tc + (lane & 15);
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-          const int r = r0 + Mma<T>::row(lane, i);
-          if (r < ns) g[r * TC + c] -= acc[q][i];
+          const int r = tr + Mma<T>::row(lane, i);
+          if (r < ns) Y[r * TC + c] = g[r * TC + c] - a1[i];
         }
       }
+      __syncthreads();
+    } else {
+      for (int idx = tid; idx < ns * TC; idx += kThreads) Y[idx] = g[idx];
+      __syncthreads();
     }
-    __syncthreads();
-  }
-  for (int t = wave; t < nt1; t += 4) {
-    const int tr = (t / NCT) * 16, tc = (t % NCT) * 16;
-    typename Mma<T>::v4 a2 = mma_tile<T, TC>(LinvT, ns, ns, g, RowId(), tr, tc, lane);
-    const int c = tc + (lane & 15);
+    for (int t = wave; t < nt1; t += 4) {
+      const int tr = (t / NCT) * 16, tc = (t % NCT) * 16;
+      typename Mma<T>::v4 a2 = mma_tile<T, TC>(LinvT, ns, ns, Y, RowId(), tr, tc, lane);
+      const int c = tc + (lane & 15);
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int r = tr + Mma<T>::row(lane, i);
-      if (r < ns && col0 + c < k) {
-        Wb[static_cast<int64_t>(S.first + r) * k + col0 + c] = a2[i];
-        if (xmode) {  // 1: store, 2: accumulate the FP64 solution
-          const int gr = xrow ? xrow[S.first + r] : S.first + r;
-          double* xp = X64 + b * w_bs + static_cast<int64_t>(gr) * k + col0 + c;
-          *xp = (xmode == 2 ? *xp : 0.0) + static_cast<double>(a2[i]);
+      for (int i = 0; i < 4; ++i) {
+        const int r = tr + Mma<T>::row(lane, i);
+        if (r < ns && col0 + c < k) {
+          Wb[static_cast<int64_t>(S.first + r) * k + col0 + c] = a2[i];
+          if (xmode) {  // 1: store, 2: accumulate the FP64 solution
+            const int gr = xrow ? xrow[S.first + r] : S.first + r;
+            double* xp = X64 + b * w_bs + static_cast<int64_t>(gr) * k + col0 + c;
+            *xp = (xmode == 2 ? *xp : 0.0) + static_cast<double>(a2[i]);
+          }
         }
       }
     }
@@ -1046,26 +1059,58 @@ struct SweepIO {
   int xmode = 0;              // 0 none, 1 store, 2 accumulate
 };
 
-template <typename T, int TC>
-void sweep_tc(dcx_plan* P, T* W, int k, int maxnf, bool fwd, int d, const int* lvl,
-              int cnt, const int* active, const SweepIO& io, hipStream_t st) {
+template <typename T, int TC, bool PL>
+void sweep_launch(dcx_plan* P, T* W, int k, int maxnf, int maxns, int pmax, int ct, bool fwd, int d,
+                  const int* lvl, int cnt, const int* active, const SweepIO& io,
+                  hipStream_t st) {
   const auto& S = P->S;
-  dim3 grid(cnt, P->nbatch, (k + TC - 1) / TC);
-  const size_t lds = static_cast<size_t>(maxnf) * TC * sizeof(T);
+  const int nz = (k + TC - 1) / TC;
+  dim3 grid(cnt, P->nbatch, (nz + ct - 1) / ct);
+  const size_t lds = (static_cast<size_t>(PL ? pmax : 0) + static_cast<size_t>(maxnf + maxns) * TC) * sizeof(T);
   T* panels = static_cast<T*>(P->d_panels);
   const int64_t w_bs = static_cast<int64_t>(S.n) * k;
   if (fwd) {
     T* u[2] = {static_cast<T*>(P->d_upd[0]), static_cast<T*>(P->d_upd[1])};
     const int64_t ubs[2] = {S.upd_rows[0] * k, S.upd_rows[1] * k};
-    hipLaunchKernelGGL((k_fwd_level<T, TC>), grid, dim3(kThreads), lds, st, P->d_sup, lvl,
+    hipLaunchKernelGGL((k_fwd_level<T, TC, PL>), grid, dim3(kThreads), lds, st, P->d_sup, lvl,
                        P->d_relpos, P->d_cidx, active, panels, S.panel_size, io.B, io.b_bs,
                        io.brow, W, w_bs, k, u[d & 1], ubs[d & 1], u[(d + 1) & 1],
-                       ubs[(d + 1) & 1], static_cast<const T*>(P->d_ufz),
-                       P->S.f_upd_rows * k);
+                       ubs[(d + 1) & 1], static_cast<const T*>(P->d_ufz), P->S.f_upd_rows * k,
+                       ct, pmax, maxnf);
   } else {
-    hipLaunchKernelGGL((k_bwd_level<T, TC>), grid, dim3(kThreads), lds, st, P->d_sup, lvl,
+    hipLaunchKernelGGL((k_bwd_level<T, TC, PL>), grid, dim3(kThreads), lds, st, P->d_sup, lvl,
                        P->d_ridx, active, panels, S.panel_size, W, w_bs, k, io.X64, io.xrow,
-                       io.xmode);
+                       io.xmode, ct, pmax, maxnf);
+  }
+}
+
+// one level: panel staged in LDS when it fits; column tiles per workgroup so
+// that the level still has about kTargetBlocks workgroups
+constexpr size_t kSweepLds = 160 * 1024;
+
+int env_int(const char* name, int dflt) {
+  const char* e = std::getenv(name);
+  return e ? std::atoi(e) : dflt;
+}
+
+template <typename T, int TC>
+void sweep_tc(dcx_plan* P, T* W, int k, int maxnf, int maxns, int maxpanel, bool fwd, int d,
+              const int* lvl, int cnt, const int* active, const SweepIO& io, hipStream_t st) {
+  static const int use_pl = env_int("DCX_PANEL_LDS", 0);
+  static const int ct_max = env_int("DCX_CT_MAX", 1);
+  static const int target = env_int("DCX_TARGET_BLOCKS", 8192);
+  const int pmax = (maxpanel + 3) & ~3;
+  const bool pl = use_pl &&
+      (static_cast<size_t>(pmax) + static_cast<size_t>(maxnf + maxns) * TC) * sizeof(T) <= kSweepLds;
+  const int nz = (k + TC - 1) / TC;
+  int ct = 1;
+  if (pl) {
+    const int64_t blocks = static_cast<int64_t>(cnt) * P->nbatch * nz;
+    ct = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>({int64_t(nz), int64_t(ct_max),
+                                                                    blocks / target})));
+    sweep_launch<T, TC, true>(P, W, k, maxnf, maxns, pmax, ct, fwd, d, lvl, cnt, active, io, st);
+  } else {
+    sweep_launch<T, TC, false>(P, W, k, maxnf, maxns, 0, 1, fwd, d, lvl, cnt, active, io, st);
   }
 }
 
@@ -1077,9 +1122,7 @@ int pick_tc(int maxnf, int maxns, size_t tsize) {
   }();
   for (int tc : {64, 32, 16}) {
     if (tc > cap) continue;
-    const int tiles = ((maxns + 15) / 16) * (tc / 16);
-    if (static_cast<size_t>(maxnf) * tc * tsize <= kLdsBudget && (tiles + 3) / 4 <= kMaxTiles)
-      return tc;
+    if (static_cast<size_t>(maxnf + maxns) * tc * tsize <= kLdsBudget) return tc;
   }
   return 0;
 }
@@ -1157,18 +1200,19 @@ int sweeps(dcx_plan* P, T* W, int k, bool sparse_rhs, const SweepIO& io, hipStre
         cnt = S.ulvl_ptr[d + 1] - S.ulvl_ptr[d];
       }
       if (!cnt) continue;
-      int maxnf = 0, maxns = 0;
+      int maxnf = 0, maxns = 0, maxpanel = 0;
       for (int q = S.ulvl_ptr[d]; q < S.ulvl_ptr[d + 1]; ++q) {
         int s = S.ulvl_sup[q];
         maxnf = std::max(maxnf, S.ns[s] + S.rs[s]);
         maxns = std::max(maxns, S.ns[s]);
+        maxpanel = std::max(maxpanel, (S.ns[s] + S.rs[s]) * S.ns[s]);
       }
       const int tc = pick_tc(maxnf, maxns, sizeof(T));
       if (!tc) return fail(DCX_ERR_ARG, "front too large for the in-LDS sweep");
       switch (tc) {
-        case 64: sweep_tc<T, 64>(P, W, k, maxnf, fwd, d, lvl, cnt, act, io, st); break;
-        case 32: sweep_tc<T, 32>(P, W, k, maxnf, fwd, d, lvl, cnt, act, io, st); break;
-        default: sweep_tc<T, 16>(P, W, k, maxnf, fwd, d, lvl, cnt, act, io, st); break;
+        case 64: sweep_tc<T, 64>(P, W, k, maxnf, maxns, maxpanel, fwd, d, lvl, cnt, act, io, st); break;
+        case 32: sweep_tc<T, 32>(P, W, k, maxnf, maxns, maxpanel, fwd, d, lvl, cnt, act, io, st); break;
+        default: sweep_tc<T, 16>(P, W, k, maxnf, maxns, maxpanel, fwd, d, lvl, cnt, act, io, st); break;
       }
     }
     if (!fwd && S.nfused) {
