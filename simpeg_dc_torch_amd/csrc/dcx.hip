@@ -164,11 +164,12 @@ __global__ __launch_bounds__(kThreads) void k_factor_level(
       if (r >= t) F[(j0 + r) + static_cast<int64_t>(j0 + t) * ld] = pan[r * LDP + t];
     }
     // trailing update: (a) lower triangle of rows/cols [j0+jb, nf);
-    // (b) E rows x remaining J columns [j0+jb, ns)
+    // (b) E rows x all remaining columns [j0+jb, nf): the J part completes
+    // L_JJ^{-T}, the R part accumulates -L_EJ L_RJ^T = -Q_R^T
     const int mt = nf - j0 - jb;
     const int nt = (mt + 3) >> 2;
     const int ntri = nt * (nt + 1) / 2;
-    const int ncj = ns - j0 - jb;
+    const int ncj = nf - j0 - jb;
     const int ntej = ((ns + 3) >> 2) * ((ncj + 3) >> 2);
     for (int tt = tid; tt < ntri + ntej; tt += kThreads) {
       int ri0, ci0, rmax, cmax;  // panel-local row / column origins and bounds
@@ -218,7 +219,8 @@ __global__ __launch_bounds__(kThreads) void k_factor_level(
     }
     __syncthreads();
   }
-  // panel for the sweeps: rows [0, ns) = L_JJ^{-1} (from the E rows), then L_RJ
+  // panel for the sweeps: rows [0, ns) = L_JJ^{-1}, rows [ns, nf) = Q_R,
+  // both read from the E rows
   T* P = panels + b * panel_bs + S.panel_off;
   for (int64_t idx = tid; idx < static_cast<int64_t>(nf) * ns; idx += kThreads) {
     const int r = static_cast<int>(idx % nf), c = static_cast<int>(idx / nf);
@@ -226,7 +228,7 @@ __global__ __launch_bounds__(kThreads) void k_factor_level(
     if (r < ns)
       val = r >= c ? F[(nf + c) + static_cast<int64_t>(r) * ld] : T(0);
     else
-      val = F[r + static_cast<int64_t>(c) * ld];
+      val = -F[(nf + c) + static_cast<int64_t>(r) * ld];
     P[r + static_cast<int64_t>(c) * nf] = val;
   }
 }
@@ -293,10 +295,10 @@ struct RowId {
   __device__ inline int operator()(int k) const { return k; }
 };
 
-// forward sweep (deepest level first): f = [b_J; sum children updates] in LDS,
-// y_J = L_JJ^{-1} f_J (to W), u_s = f_R - L_RJ y_J (to the update buffer).
-// PL: the supernode's panel is staged in LDS once and the workgroup walks
-// `ct` column tiles with it; otherwise the panel is read from global memory.
+// forward sweep (deepest level first): f = [b_J; sum of the children's
+// updates] in LDS, then [y_J; w] = P_s f_J: y_J to W, u_s = f_R - w to the
+// update buffer of this depth.  PL: panel staged in LDS, `ct` column tiles
+// per workgroup.
 template <typename T, int TC, bool PL>
 __global__ __launch_bounds__(kThreads) void k_fwd_level(
     const SupDev* __restrict__ sup, const int* __restrict__ lvl,
@@ -305,34 +307,31 @@ __global__ __launch_bounds__(kThreads) void k_fwd_level(
     int64_t panel_bs, const double* __restrict__ Bsrc, int64_t b_bs,
     const int* __restrict__ brow, T* __restrict__ W, int64_t w_bs, int k,
     T* __restrict__ ucur, int64_t ucur_bs, const T* __restrict__ uprev,
-    int64_t uprev_bs, const T* __restrict__ ufz, int64_t ufz_bs, int ct, int pmax, int nfmax) {
+    int64_t uprev_bs, const T* __restrict__ ufz, int64_t ufz_bs, int ct, int pmax) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int s = lvl[blockIdx.x];
   const int b = blockIdx.y;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const SupDev S = sup[s];
-  const int ns = S.ns, rs = S.rs, nf = S.ns + S.rs;
+  const int ns = S.ns, nf = S.ns + S.rs;
   const T* Pg = panels + b * panel_bs + S.panel_off;
   T* Ps = reinterpret_cast<T*>(smem_raw);
   T* f = PL ? Ps + pmax : Ps;
-  T* Y = f + nfmax * TC;  // phase-1 scratch (ns x TC)
   if (PL) {
     const int np = nf * ns;
     for (int idx = tid; idx < np; idx += kThreads) Ps[idx] = Pg[idx];
   }
   const T* P = PL ? Ps : Pg;
-  auto Linv = [=](int r, int kk) { return P[r + kk * nf]; };
-  auto Lrj = [=](int r, int kk) { return P[(ns + r) + kk * nf]; };
+  auto Pa = [=](int r, int kk) { return P[r + kk * nf]; };
   T* Wb = W + b * w_bs;
   const double* Bb = Bsrc + b * b_bs;
   T* uo = ucur + b * ucur_bs + S.upd_off * k;
   constexpr int NCT = TC / 16;
-  const int nt1 = ((ns + 15) / 16) * NCT;
-  const int nt2 = ((rs + 15) / 16) * NCT;
+  const int nt = ((nf + 15) / 16) * NCT;
   for (int tile = 0; tile < ct; ++tile) {
     const int col0 = (blockIdx.z * ct + tile) * TC;
     if (col0 >= k) break;
-    __syncthreads();  // previous tile done with f (and the panel staged)
+    __syncthreads();
     for (int idx = tid; idx < nf * TC; idx += kThreads) {
       const int r = idx / TC, c = idx % TC;
       T v = T(0);
@@ -356,69 +355,51 @@ __global__ __launch_bounds__(kThreads) void k_fwd_level(
       }
       __syncthreads();
     }
-    for (int t = wave; t < nt1; t += 4) {  // y_J = L_JJ^{-1} f_J into the scratch Y
+    for (int t = wave; t < nt; t += 4) {
       const int tr = (t / NCT) * 16, tc = (t % NCT) * 16;
-      typename Mma<T>::v4 a1 = mma_tile<T, TC>(Linv, ns, ns, f, RowId(), tr, tc, lane);
+      typename Mma<T>::v4 a = mma_tile<T, TC>(Pa, nf, ns, f, RowId(), tr, tc, lane);
       const int c = tc + (lane & 15);
+      if (col0 + c >= k) continue;
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         const int r = tr + Mma<T>::row(lane, i);
-        if (r < ns) Y[r * TC + c] = a1[i];
-      }
-    }
-    __syncthreads();
-    for (int idx = tid; idx < ns * TC; idx += kThreads) {
-      const int r = idx / TC, c = idx % TC;
-      const T y = Y[idx];
-      f[idx] = y;
-      if (col0 + c < k) Wb[static_cast<int64_t>(S.first + r) * k + col0 + c] = y;
-    }
-    if (rs == 0) continue;
-    __syncthreads();
-    for (int t = wave; t < nt2; t += 4) {
-      const int tr = (t / NCT) * 16, tc = (t % NCT) * 16;
-      typename Mma<T>::v4 a2 = mma_tile<T, TC>(Lrj, rs, ns, f, RowId(), tr, tc, lane);
-      const int c = tc + (lane & 15);
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const int r = tr + Mma<T>::row(lane, i);
-        if (r < rs && col0 + c < k)
-          uo[static_cast<int64_t>(r) * k + col0 + c] = f[(ns + r) * TC + c] - a2[i];
+        if (r < ns)
+          Wb[static_cast<int64_t>(S.first + r) * k + col0 + c] = a[i];
+        else if (r < nf)
+          uo[static_cast<int64_t>(r - ns) * k + col0 + c] = f[r * TC + c] - a[i];
       }
     }
   }
 }
 
-// backward sweep (root first): g = [y_J; x_R] in LDS,
-// t = y_J - L_RJ^T x_R, x_J = L_JJ^{-T} t (to W).  PL / ct as in the forward.
+// backward sweep (root first): g = [y_J; -x_R] in LDS, x_J = P_s^T g to W
+// (and the FP64 solution, xmode 1 store / 2 accumulate).  PL / ct as above.
 template <typename T, int TC, bool PL>
 __global__ __launch_bounds__(kThreads) void k_bwd_level(
     const SupDev* __restrict__ sup, const int* __restrict__ lvl,
     const int* __restrict__ ridx, const int* __restrict__ active,
     const T* __restrict__ panels, int64_t panel_bs, T* __restrict__ W,
     int64_t w_bs, int k, double* __restrict__ X64, const int* __restrict__ xrow,
-    int xmode, int ct, int pmax, int nfmax) {
+    int xmode, int ct, int pmax) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int s = lvl[blockIdx.x];
   const int b = blockIdx.y;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const SupDev S = sup[s];
-  const int ns = S.ns, rs = S.rs, nf = S.ns + S.rs;
+  const int ns = S.ns, nf = S.ns + S.rs;
   const bool act = !active || active[s];
   const T* Pg = panels + b * panel_bs + S.panel_off;
   T* Ps = reinterpret_cast<T*>(smem_raw);
   T* g = PL ? Ps + pmax : Ps;
-  T* Y = g + nfmax * TC;  // phase-1 scratch (ns x TC)
   if (PL) {
     const int np = nf * ns;
     for (int idx = tid; idx < np; idx += kThreads) Ps[idx] = Pg[idx];
   }
   const T* P = PL ? Ps : Pg;
-  auto LrjT = [=](int i, int kk) { return P[(ns + kk) + i * nf]; };
-  auto LinvT = [=](int i, int kk) { return P[kk + i * nf]; };
+  auto PaT = [=](int i, int kk) { return P[kk + i * nf]; };
   T* Wb = W + b * w_bs;
   constexpr int NCT = TC / 16;
-  const int nt1 = ((ns + 15) / 16) * NCT;
+  const int nt = ((ns + 15) / 16) * NCT;
   for (int tile = 0; tile < ct; ++tile) {
     const int col0 = (blockIdx.z * ct + tile) * TC;
     if (col0 >= k) break;
@@ -430,40 +411,25 @@ __global__ __launch_bounds__(kThreads) void k_bwd_level(
         if (r < ns)
           val = act ? Wb[static_cast<int64_t>(S.first + r) * k + col0 + c] : T(0);
         else
-          val = Wb[static_cast<int64_t>(ridx[S.rptr + r - ns]) * k + col0 + c];
+          val = -Wb[static_cast<int64_t>(ridx[S.rptr + r - ns]) * k + col0 + c];
       }
       g[idx] = val;
     }
     __syncthreads();
-    if (rs > 0) {  // t = y_J - L_RJ^T x_R into the scratch Y, then into g
-      for (int t = wave; t < nt1; t += 4) {
-        const int tr = (t / NCT) * 16, tc = (t % NCT) * 16;
-        typename Mma<T>::v4 a1 = mma_tile<T, TC>(LrjT, ns, rs, g + ns * TC, RowId(), tr, tc, lane);
-        const int c = tc + (lane & 15);
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          const int r = tr + Mma<T>::row(lane, i);
-          if (r < ns) Y[r * TC + c] = g[r * TC + c] - a1[i];
-        }
-      }
-      __syncthreads();
-    } else {
-      for (int idx = tid; idx < ns * TC; idx += kThreads) Y[idx] = g[idx];
-      __syncthreads();
-    }
-    for (int t = wave; t < nt1; t += 4) {
+    for (int t = wave; t < nt; t += 4) {
       const int tr = (t / NCT) * 16, tc = (t % NCT) * 16;
-      typename Mma<T>::v4 a2 = mma_tile<T, TC>(LinvT, ns, ns, Y, RowId(), tr, tc, lane);
+      typename Mma<T>::v4 a = mma_tile<T, TC>(PaT, ns, nf, g, RowId(), tr, tc, lane);
       const int c = tc + (lane & 15);
+      if (col0 + c >= k) continue;
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         const int r = tr + Mma<T>::row(lane, i);
-        if (r < ns && col0 + c < k) {
-          Wb[static_cast<int64_t>(S.first + r) * k + col0 + c] = a2[i];
-          if (xmode) {  // 1: store, 2: accumulate the FP64 solution
+        if (r < ns) {
+          Wb[static_cast<int64_t>(S.first + r) * k + col0 + c] = a[i];
+          if (xmode) {
             const int gr = xrow ? xrow[S.first + r] : S.first + r;
             double* xp = X64 + b * w_bs + static_cast<int64_t>(gr) * k + col0 + c;
-            *xp = (xmode == 2 ? *xp : 0.0) + static_cast<double>(a2[i]);
+            *xp = (xmode == 2 ? *xp : 0.0) + static_cast<double>(a[i]);
           }
         }
       }
@@ -478,6 +444,12 @@ struct LRow {
 struct OffRow {
   int o;
   __device__ inline int operator()(int k) const { return o + k; }
+};
+// rows of [y_J; x_R] inside a fused subtree block: J rows from jl, R rows via l
+struct SplitRow {
+  int jl, ns;
+  const int* l;
+  __device__ inline int operator()(int k) const { return k < ns ? jl + k : l[k - ns]; }
 };
 
 // Fused bottom subtree, backward sweep.  The LDS block X holds the subtree's
@@ -518,47 +490,36 @@ __global__ __launch_bounds__(kThreads) void k_bwd_fused(
   for (int L = F.lv_begin; L < F.lv_end; ++L) {
     const int m0 = flv[L], m1 = flv[L + 1];
     int base = 0;
-    for (int mi = m0; mi < m1; ++mi) {
+    for (int mi = m0; mi < m1; ++mi) {  // x_J = P^T [y_J; -x_R] into the scratch
       const int s = fmem[mi];
       const SupDev S = sup[s];
       const int ns = S.ns, nf = S.ns + S.rs;
       const T* P = panels + b * panel_bs + S.panel_off;
-      auto LrjT = [=](int i, int kk) { return P[(ns + kk) + static_cast<int64_t>(i) * nf]; };
+      auto PaT = [=](int i, int kk) {
+        const T v = P[kk + static_cast<int64_t>(i) * nf];
+        return kk < ns ? v : -v;
+      };
       const int nt = ((ns + 15) / 16) * NCT;
       const int to = toff[s], jl = S.first - F.lo;
+      const SplitRow br{jl, ns, lridx + S.rptr};
       for (int t = ((wave - base) % 4 + 4) % 4; t < nt; t += 4) {
         const int tr = (t / NCT) * 16, tc = (t % NCT) * 16;
-        typename Mma<T>::v4 acc = mma_tile<T, TC>(LrjT, ns, S.rs, X, LRow{lridx + S.rptr}, tr, tc, lane);
+        typename Mma<T>::v4 acc = mma_tile<T, TC>(PaT, ns, nf, X, br, tr, tc, lane);
         const int c = tc + (lane & 15);
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
           const int r = tr + Mma<T>::row(lane, i);
-          if (r < ns) Tb[(to + r) * TC + c] = X[(jl + r) * TC + c] - acc[i];
+          if (r < ns) Tb[(to + r) * TC + c] = acc[i];
         }
       }
       base += nt;
     }
     __syncthreads();
-    base = 0;
     for (int mi = m0; mi < m1; ++mi) {
       const int s = fmem[mi];
       const SupDev S = sup[s];
-      const int ns = S.ns, nf = S.ns + S.rs;
-      const T* P = panels + b * panel_bs + S.panel_off;
-      auto LinvT = [=](int i, int kk) { return P[kk + static_cast<int64_t>(i) * nf]; };
-      const int nt = ((ns + 15) / 16) * NCT;
       const int to = toff[s], jl = S.first - F.lo;
-      for (int t = ((wave - base) % 4 + 4) % 4; t < nt; t += 4) {
-        const int tr = (t / NCT) * 16, tc = (t % NCT) * 16;
-        typename Mma<T>::v4 acc = mma_tile<T, TC>(LinvT, ns, ns, Tb, OffRow{to}, tr, tc, lane);
-        const int c = tc + (lane & 15);
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          const int r = tr + Mma<T>::row(lane, i);
-          if (r < ns) X[(jl + r) * TC + c] = acc[i];
-        }
-      }
-      base += nt;
+      for (int idx = tid; idx < S.ns * TC; idx += kThreads) X[jl * TC + idx] = Tb[to * TC + idx];
     }
     __syncthreads();
   }
@@ -613,7 +574,7 @@ __global__ __launch_bounds__(kThreads) void k_fwd_fused(
   for (int L = F.lv_end - 1; L >= F.lv_begin; --L) {
     const int m0 = flv[L], m1 = flv[L + 1];
     int base = 0;
-    for (int mi = m0; mi < m1; ++mi) {
+    for (int mi = m0; mi < m1; ++mi) {  // y_J = L_JJ^{-1} f_J into the scratch
       const int s = fmem[mi];
       const SupDev S = sup[s];
       const int ns = S.ns, nf = S.ns + S.rs;
@@ -633,14 +594,7 @@ __global__ __launch_bounds__(kThreads) void k_fwd_fused(
       }
       base += nt;
     }
-    __syncthreads();
-    for (int mi = m0; mi < m1; ++mi) {  // y into the subtree rows
-      const int s = fmem[mi];
-      const SupDev S = sup[s];
-      const int to = toff[s], jl = S.first - F.lo;
-      for (int idx = tid; idx < S.ns * TC; idx += kThreads)
-        X[jl * TC + idx] = Tb[to * TC + idx];
-    }
+    // X[R rows] -= Q_R f_J, one colour group at a time (f_J still in X)
     for (int g = fgptr[L]; g < fgptr[L + 1]; ++g) {
       base = 0;
       for (int mi = fgb[g]; mi < fgb[g + 1]; ++mi) {
@@ -648,13 +602,13 @@ __global__ __launch_bounds__(kThreads) void k_fwd_fused(
         const SupDev S = sup[s];
         const int ns = S.ns, rs = S.rs, nf = S.ns + S.rs;
         const T* P = panels + b * panel_bs + S.panel_off;
-        auto Lrj = [=](int r, int kk) { return P[(ns + r) + static_cast<int64_t>(kk) * nf]; };
+        auto Qr = [=](int r, int kk) { return P[(ns + r) + static_cast<int64_t>(kk) * nf]; };
         const int nt = ((rs + 15) / 16) * NCT;
-        const int to = toff[s];
+        const int jl = S.first - F.lo;
         const int* lr = lridx + S.rptr;
         for (int t = ((wave - base) % 4 + 4) % 4; t < nt; t += 4) {
           const int tr = (t / NCT) * 16, tc = (t % NCT) * 16;
-          typename Mma<T>::v4 acc = mma_tile<T, TC>(Lrj, rs, ns, Tb, OffRow{to}, tr, tc, lane);
+          typename Mma<T>::v4 acc = mma_tile<T, TC>(Qr, rs, ns, X, OffRow{jl}, tr, tc, lane);
           const int c = tc + (lane & 15);
 #pragma unroll
           for (int i = 0; i < 4; ++i) {
@@ -666,6 +620,13 @@ __global__ __launch_bounds__(kThreads) void k_fwd_fused(
       }
       __syncthreads();
     }
+    for (int mi = m0; mi < m1; ++mi) {  // y into the subtree rows
+      const int s = fmem[mi];
+      const SupDev S = sup[s];
+      const int to = toff[s], jl = S.first - F.lo;
+      for (int idx = tid; idx < S.ns * TC; idx += kThreads) X[jl * TC + idx] = Tb[to * TC + idx];
+    }
+    __syncthreads();
   }
   T* Wb = W + b * w_bs;
   T* uo = ufz + b * ufz_bs + F.uoff * k;
@@ -1066,7 +1027,7 @@ void sweep_launch(dcx_plan* P, T* W, int k, int maxnf, int maxns, int pmax, int 
   const auto& S = P->S;
   const int nz = (k + TC - 1) / TC;
   dim3 grid(cnt, P->nbatch, (nz + ct - 1) / ct);
-  const size_t lds = (static_cast<size_t>(PL ? pmax : 0) + static_cast<size_t>(maxnf + maxns) * TC) * sizeof(T);
+  const size_t lds = (static_cast<size_t>(PL ? pmax : 0) + static_cast<size_t>(maxnf) * TC) * sizeof(T);
   T* panels = static_cast<T*>(P->d_panels);
   const int64_t w_bs = static_cast<int64_t>(S.n) * k;
   if (fwd) {
@@ -1076,11 +1037,11 @@ void sweep_launch(dcx_plan* P, T* W, int k, int maxnf, int maxns, int pmax, int 
                        P->d_relpos, P->d_cidx, active, panels, S.panel_size, io.B, io.b_bs,
                        io.brow, W, w_bs, k, u[d & 1], ubs[d & 1], u[(d + 1) & 1],
                        ubs[(d + 1) & 1], static_cast<const T*>(P->d_ufz), P->S.f_upd_rows * k,
-                       ct, pmax, maxnf);
+                       ct, pmax);
   } else {
     hipLaunchKernelGGL((k_bwd_level<T, TC, PL>), grid, dim3(kThreads), lds, st, P->d_sup, lvl,
                        P->d_ridx, active, panels, S.panel_size, W, w_bs, k, io.X64, io.xrow,
-                       io.xmode, ct, pmax, maxnf);
+                       io.xmode, ct, pmax);
   }
 }
 
@@ -1101,7 +1062,7 @@ void sweep_tc(dcx_plan* P, T* W, int k, int maxnf, int maxns, int maxpanel, bool
   static const int target = env_int("DCX_TARGET_BLOCKS", 8192);
   const int pmax = (maxpanel + 3) & ~3;
   const bool pl = use_pl &&
-      (static_cast<size_t>(pmax) + static_cast<size_t>(maxnf + maxns) * TC) * sizeof(T) <= kSweepLds;
+      (static_cast<size_t>(pmax) + static_cast<size_t>(maxnf) * TC) * sizeof(T) <= kSweepLds;
   const int nz = (k + TC - 1) / TC;
   int ct = 1;
   if (pl) {
@@ -1122,7 +1083,7 @@ int pick_tc(int maxnf, int maxns, size_t tsize) {
   }();
   for (int tc : {64, 32, 16}) {
     if (tc > cap) continue;
-    if (static_cast<size_t>(maxnf + maxns) * tc * tsize <= kLdsBudget) return tc;
+    if (static_cast<size_t>(maxnf) * tc * tsize <= kLdsBudget) return tc;
   }
   return 0;
 }

@@ -1,7 +1,8 @@
 """Numpy emulation of the GPU factor and sweep kernels, step for step.
 
 Test helper: it runs the multifrontal factorisation (extend-add, partial
-Cholesky, panel P = [L_JJ^-1; L_RJ]) and the two sweeps exactly as the
+Cholesky, panel P = [L_JJ^-1; Q_R], Q_R = L_RJ L_JJ^-1) and the two sweeps
+exactly as the
 kernels in csrc/dcx.hip do, level by level, on the structure that the host
 symbolic analysis returns.  A mistake in the structure (fronts, relative
 positions, entry maps) or in the sweep algebra shows up here, on the CPU.
@@ -32,7 +33,7 @@ def factor(sym, vals):
                 F[j + 1:, j + 1:] -= np.tril(np.outer(F[j + 1:, j], F[j + 1:, j]))
             L_JJ = np.tril(F[:ns, :ns])
             Linv = np.linalg.inv(L_JJ)
-            Q[s] = np.vstack([np.tril(Linv), F[ns:, :ns]])
+            Q[s] = np.vstack([np.tril(Linv), F[ns:, :ns] @ np.tril(Linv)])
             fronts[s] = F
     return Q
 
@@ -50,15 +51,14 @@ def solve(sym, Q, B):
             for c in sym["cidx"][sym["cptr"][s]:sym["cptr"][s + 1]]:
                 rp = sym["relpos"][sym["rptr"][c]:sym["rptr"][c + 1]]
                 f[rp] += upd[c]
-            y = Q[s][:ns] @ f[:ns]
-            W[first[s]:first[s] + ns] = y
-            upd[s] = f[ns:] - Q[s][ns:] @ y
+            out = Q[s] @ f[:ns]
+            W[first[s]:first[s] + ns] = out[:ns]
+            upd[s] = f[ns:] - out[ns:]
     for d in range(sym["ndepth"]):
         for s in sym["lvl_sup"][sym["lvl_ptr"][d]:sym["lvl_ptr"][d + 1]]:
             ns, rs = ns_[s], rs_[s]
             R = sym["ridx"][sym["rptr"][s]:sym["rptr"][s + 1]]
-            t = W[first[s]:first[s] + ns] - Q[s][ns:].T @ W[R]
-            W[first[s]:first[s] + ns] = Q[s][:ns].T @ t
+            W[first[s]:first[s] + ns] = Q[s].T @ np.vstack([W[first[s]:first[s] + ns], -W[R]])
     return W
 
 
@@ -87,17 +87,18 @@ def solve_fused(sym, Q, B):
             for s in members(L):
                 jl = first[s] - lo
                 Tb[toff[s]:toff[s] + ns_[s]] = Q[s][:ns_[s]] @ X[jl:jl + ns_[s]]
-            for s in members(L):
-                jl = first[s] - lo
-                X[jl:jl + ns_[s]] = Tb[toff[s]:toff[s] + ns_[s]]
             for g in range(sym["f_gptr"][L], sym["f_gptr"][L + 1]):
                 grp = sym["f_mem"][sym["f_gb"][g]:sym["f_gb"][g + 1]]
                 rows = [lridx[rptr[s]:rptr[s + 1]] for s in grp]
                 flat = np.concatenate(rows) if rows else np.zeros(0, int)
                 assert flat.size == np.unique(flat).size, "colour group rows overlap"
                 for s in grp:
+                    jl = first[s] - lo
                     lr = lridx[rptr[s]:rptr[s + 1]]
-                    X[lr] -= Q[s][ns_[s]:] @ Tb[toff[s]:toff[s] + ns_[s]]
+                    X[lr] -= Q[s][ns_[s]:] @ X[jl:jl + ns_[s]]
+            for s in members(L):
+                jl = first[s] - lo
+                X[jl:jl + ns_[s]] = Tb[toff[s]:toff[s] + ns_[s]]
         W[lo:hi] = X[:nloc]
         upd[root] = X[nloc:].copy()
     # forward: the levels above
@@ -109,16 +110,15 @@ def solve_fused(sym, Q, B):
             f[:ns] = B[first[s]:first[s] + ns]
             for c in sym["cidx"][sym["cptr"][s]:sym["cptr"][s + 1]]:
                 f[sym["relpos"][rptr[c]:rptr[c + 1]]] += upd[c]
-            y = Q[s][:ns] @ f[:ns]
-            W[first[s]:first[s] + ns] = y
-            upd[s] = f[ns:] - Q[s][ns:] @ y
+            out = Q[s] @ f[:ns]
+            W[first[s]:first[s] + ns] = out[:ns]
+            upd[s] = f[ns:] - out[ns:]
     # backward: levels above, root first
     for d in range(sym["ndepth"]):
         for s in usup[up[d]:up[d + 1]]:
             ns = ns_[s]
             R = ridx[rptr[s]:rptr[s + 1]]
-            t_ = W[first[s]:first[s] + ns] - Q[s][ns:].T @ W[R]
-            W[first[s]:first[s] + ns] = Q[s][:ns].T @ t_
+            W[first[s]:first[s] + ns] = Q[s].T @ np.vstack([W[first[s]:first[s] + ns], -W[R]])
     # backward: fused subtrees
     for t in range(sym["nfused"]):
         root, lo, hi = sym["f_root"][t], sym["f_lo"][t], sym["f_hi"][t]
@@ -131,9 +131,9 @@ def solve_fused(sym, Q, B):
             for s in members(L):
                 jl, ns = first[s] - lo, ns_[s]
                 lr = lridx[rptr[s]:rptr[s + 1]]
-                Tb[toff[s]:toff[s] + ns] = X[jl:jl + ns] - Q[s][ns:].T @ X[lr]
+                Tb[toff[s]:toff[s] + ns] = Q[s].T @ np.vstack([X[jl:jl + ns], -X[lr]])
             for s in members(L):
                 jl, ns = first[s] - lo, ns_[s]
-                X[jl:jl + ns] = Q[s][:ns].T @ Tb[toff[s]:toff[s] + ns]
+                X[jl:jl + ns] = Tb[toff[s]:toff[s] + ns]
         W[lo:hi] = X[:nloc]
     return W
