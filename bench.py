@@ -3,16 +3,19 @@
 Workload: the 25,088-cell line (200 x 100 core of 1 m cells plus 12 padding
 cells growing 1.3x; 61 electrodes at 2 m; dipole-dipole a = 1..3, n = 1..6
 plus Wenner-Schlumberger a = 1..2, n = 1..8: 849 sources, 1,636 data;
-12 electrode-fitted wavenumbers requested; m = log(rho) a Gaussian random
-field, seed 1).  One step = misfit(m).backward(): assembly, factorisation of
-every wavenumber, forward solve of all sources, projection, misfit, adjoint
-solve, gradient gather and transposed map -- everything resident on the GPU.
+12 electrode-fitted wavenumbers requested, 10 carry weight; m = log(rho) a
+Gaussian random field, seed 1).  One step = misfit(m).backward(): assembly,
+factorisation of every wavenumber, forward solve of all sources, projection,
+misfit, adjoint solve, gradient gather and transposed map -- all resident on
+the GPU, inputs in HBM before the timed region.
 
-Prints one JSON line.  `value` is the mixed-precision (FP32 factor + FP64
-refinement) eval+grad time in ms; the FP64 time, the errors of the mixed
-result against FP64, the roofline of the dominant kernel and the CPU
-baseline (the oracle's SuperLU restatement of the reference CPU path on a
-bounded sample) are reported beside it.
+Prints one JSON line.  `value` is the mixed-precision eval+grad time in ms
+(FP32 factorisation and sweeps, FP64 assembly, projection, residual and
+gather; `--refine N` adds N FP64-residual refinement passes per solve).  The
+FP64 time, the errors of the mixed result against the FP64 one, the roofline
+of the dominant kernel (the backward sweep) from HIP events on the solve
+stream, and the CPU baseline (the oracle's SuperLU restatement of the
+reference CPU path on a bounded sample) are reported beside it.
 """
 import argparse
 import json
@@ -28,6 +31,7 @@ import torch  # noqa: E402
 
 METRIC = "eval+grad ms (2.5-D 25k cells, 12 ky): mixed vs FP64 GPU at 1e-6 rel-err"
 HBM_PEAK_GBS = 8000.0
+MFMA_PEAK = {"fp64": 78.6, "fp32": 157.3}  # dense TFLOP/s
 
 
 def log(msg):
@@ -68,29 +72,37 @@ def time_steps(eng, mt, dobs, w, steps, warmup):
     return 1e3 * (time.perf_counter() - t0) / steps
 
 
-def kernel_roofline(eng, mt, dobs, w, reps=5):
-    """Average duration of the gradient gather-dot (one launch per eval+grad,
-    HBM-bound) on the stream it runs on, and its algorithmic bytes."""
-    g = eng._gpu
-    with torch.no_grad():
-        sig = torch.exp(-mt)
-        vals = g.assemble(sig)
-        g.solver.factorize(vals)
-        U = g.solve_perm(g.q_perm, eng.nS)
-        Lam = g.solve_perm(g.q_perm, eng.nS)
-        torch.cuda.synchronize()
-        st = torch.cuda.current_stream()
-        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
-        g.solver.grad_values(Lam, U, scale=g.wq, permuted=True)
-        ev[0].record(st)
-        for _ in range(reps):
-            g.solver.grad_values(Lam, U, scale=g.wq, permuted=True)
-        ev[1].record(st)
-        torch.cuda.synchronize()
-        ms = ev[0].elapsed_time(ev[1]) / reps
-    nky, n, k, nnz = eng.nky, eng.nN, eng.nS, eng.nnzA
-    algo = nky * (2 * 8 * n * k + 8 * nnz)
-    return ms, algo
+def sweep_roofline(eng, mt, dobs, w, reps=3):
+    """Backward-sweep kernel (k_bwd_level) of the engine's solves: average
+    launch duration from HIP events recorded on the solve stream, and its
+    algorithmic bytes and flops per launch."""
+    sv = eng.solver
+    info = sv.info()
+    sv.set_timing(True)
+    for _ in range(reps):
+        eval_grad(eng, mt, dobs, w)
+    torch.cuda.synchronize()
+    f_ms, b_ms, nsolve = sv.sweep_times()
+    sv.set_timing(False)
+    launches = nsolve * info["level_launches"]
+    ts = 4 if info["precision"] == 1 else 8
+    nb, k = info["nbatch"], eng.nS
+    # per backward sweep: read y_J and x_R rows, write x_J rows (and the FP64
+    # solution in mixed mode), read every panel once
+    per_sweep = nb * k * (ts * (info["sweep_rows_nf"] + info["sweep_rows_ns"])
+                          + (8 * info["sweep_rows_ns"] if ts == 4 else 0)) \
+        + nb * info["panel_size"] * ts
+    flops = 2.0 * nb * k * info["solve_flops_per_rhs"] / 4.0
+    avg_ms = b_ms / launches
+    achieved = (per_sweep / info["level_launches"]) / (avg_ms * 1e-3) / 1e9
+    return {"kernel": "k_bwd_level", "bound": "hbm", "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None, "avg_ms": round(avg_ms, 4),
+            "bytes_per_launch": int(per_sweep / info["level_launches"]),
+            "tflops": round(flops / (b_ms / nsolve * 1e-3) / 1e12, 2),
+            "tflops_peak": MFMA_PEAK["fp32" if ts == 4 else "fp64"],
+            "launches_per_sweep": info["level_launches"],
+            "fwd_sweep_ms": round(f_ms / nsolve, 3), "bwd_sweep_ms": round(b_ms / nsolve, 3)}
 
 
 def cpu_baseline(mesh, survey, m, kys, wq, sample_ky=1):
@@ -113,29 +125,47 @@ def cpu_baseline(mesh, survey, m, kys, wq, sample_ky=1):
     return 1e3 * t * len(kys) / sample_ky, t
 
 
+def pmc_pass(args):
+    """Short run for hardware-counter passes: a 1 GiB device copy (the
+    counters' calibration) and two mixed eval+grad steps."""
+    log("pmc pass: calibration copy of 1 GiB")
+    a = torch.empty(1 << 27, dtype=torch.float64, device="cuda:0").fill_(1.0)
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    mesh, survey, m, e64, emx, mt, dobs, w, _ = build_engines(args.refine)
+    for _ in range(2):
+        eval_grad(emx, mt, dobs, w)
+    torch.cuda.synchronize()
+    info = emx.solver.info()
+    log(f"pmc pass done: nbatch={info['nbatch']} k={emx.nS} rows_nf={info['sweep_rows_nf']} "
+        f"rows_ns={info['sweep_rows_ns']} panel={info['panel_size']}")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--refine", type=int, default=1)
+    ap.add_argument("--refine", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--pmc-pass", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(see DESIGN.md for how the path would shard)", file=sys.stderr)
         sys.exit(2)
+    if args.pmc_pass:
+        pmc_pass(args)
+        return
     pr = torch.cuda.get_device_properties(0)
     log(f"device: {pr.name} CUs={pr.multi_processor_count} "
-        f"L2={getattr(pr, 'L2_cache_size', 0) >> 20} MiB "
-        f"lds/block={getattr(pr, 'shared_memory_per_block', 0) >> 10} KiB "
-        f"lds/CU={getattr(pr, 'shared_memory_per_multiprocessor', 0) >> 10} KiB")
-    log("building engines (config: 25k-cell line, 12 ky)")
+        f"L2={getattr(pr, 'L2_cache_size', 0) >> 20} MiB")
+    log("building engines (config: 25k-cell line, 12 ky requested)")
     mesh, survey, m, e64, emx, mt, dobs, w, t_build = build_engines(args.refine)
     info = e64.solver.info()
     log(f"plan: n={info['n']} nsup={info['nsup']} depth={info['ndepth']} "
         f"max_front={info['max_front']} nnz_L={info['nnz_L']} nky={e64.nky} nS={e64.nS}")
-    # accuracy of mixed against fp64
     phi64, g64 = eval_grad(e64, mt, dobs, w)
     phimx, gmx = eval_grad(emx, mt, dobs, w)
     with torch.no_grad():
@@ -150,23 +180,23 @@ def main():
     log("timing mixed")
     msmx = time_steps(emx, mt, dobs, w, args.steps, args.warmup)
     log(f"mixed eval+grad {msmx:.2f} ms")
-    kms, kbytes = kernel_roofline(e64, mt, dobs, w)
-    ach = kbytes / (kms * 1e-3) / 1e9
+    roof = sweep_roofline(emx, mt, dobs, w)
+    roof64 = sweep_roofline(e64, mt, dobs, w)
     cpu = None
     if not args.no_cpu:
         log("cpu baseline (oracle SuperLU, 1 wavenumber sample)")
         cms, ct = cpu_baseline(mesh, survey, m, e64.kys, e64.wq_np)
         cpu = {"value": round(cms, 1), "unit": "ms", "cores": 1, "kind": "port",
-               "sample": f"1 of {e64.nky} wavenumbers (factor + forward and adjoint solves "
-                         f"of all {e64.nS} sources + gradient gather), {ct:.1f} s, "
+               "sample": f"1 of {e64.nky} wavenumbers (SuperLU factor + forward and adjoint "
+                         f"solves of all {e64.nS} sources + gradient gather), {ct:.1f} s, "
                          f"scaled x{e64.nky}"}
     out = {
         "metric": METRIC, "value": round(msmx, 3), "unit": "ms", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(msmx, 3),
         "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32 factor+sweeps, f64 refinement/residual/gather",
+        "dtype": "f32 factor+sweeps; f64 assembly/projection/gather",
         "data": "synthetic",
-        "config": {"workload": "2.5-D line, 25,088 cells, 61 electrodes DD+WS, "
+        "config": {"workload": "2.5-D line: 25,088 cells, 61 electrodes DD+WS, "
                                f"{e64.nS} sources, {e64.nD} data, nky={e64.nky} "
                                "(12 requested, electrode-fitted)",
                    "cells": e64.nC, "nodes": e64.nN, "sources": e64.nS, "data": e64.nD,
@@ -174,9 +204,7 @@ def main():
         "fp64_ms": round(ms64, 3), "mixed_speedup_vs_fp64": round(ms64 / msmx, 3),
         "mixed_dpred_rel_err": err_d, "mixed_grad_rel_err": err_g,
         "build_s": round(t_build, 2),
-        "roofline": {"kernel": "k_grad_values", "bound": "hbm", "achieved": round(ach, 1),
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
-                     "traffic": None, "avg_ms": round(kms, 4)},
+        "roofline": roof, "roofline_fp64": roof64,
         "cpu_baseline": cpu,
     }
     if cpu:

@@ -52,6 +52,11 @@ typedef struct {
   double factor_flops;        /* per matrix */
   double solve_flops_per_rhs; /* forward + backward sweep, per column */
   int64_t device_bytes;       /* factor + front storage held by the plan */
+  int64_t sweep_rows_nf;      /* sum of front rows over the level-scheduled supernodes */
+  int64_t sweep_rows_ns;      /* sum of their own rows */
+  int64_t panel_size;         /* factor panel elements per batch member */
+  int level_launches;         /* level launches per sweep */
+  int nfused;                 /* fused bottom subtrees */
 } dcx_plan_info;
 
 typedef struct dcx_symbolic dcx_symbolic;
@@ -96,6 +101,12 @@ int dcx_plan_destroy(dcx_plan* plan);
 int dcx_plan_get_info(const dcx_plan* plan, dcx_plan_info* out);
 /* host (n): perm[new] = old */
 int dcx_plan_get_perm(const dcx_plan* plan, int32_t* perm);
+/* Record HIP events around the forward and backward sweeps of every solve
+ * (on the solve's stream); dcx_plan_sweep_times sums them (synchronising on
+ * the last event) over the solves since timing was enabled. */
+int dcx_plan_set_timing(dcx_plan* plan, int enable);
+int dcx_plan_sweep_times(dcx_plan* plan, double* fwd_ms, double* bwd_ms, int* nsweeps);
+
 /* LDS row budget of the fused bottom subtrees of the sweeps (0: none);
  * rebuilds the fused structure (host) and its device tables. */
 int dcx_plan_set_fusion(dcx_plan* plan, int fuse_rows);

@@ -24,7 +24,9 @@ class PlanInfo(C.Structure):
     _fields_ = [("n", C.c_int), ("nnz", C.c_int), ("nbatch", C.c_int), ("nsup", C.c_int),
                 ("ndepth", C.c_int), ("max_front", C.c_int), ("max_ns", C.c_int),
                 ("precision", C.c_int), ("nnz_L", C.c_int64), ("factor_flops", C.c_double),
-                ("solve_flops_per_rhs", C.c_double), ("device_bytes", C.c_int64)]
+                ("solve_flops_per_rhs", C.c_double), ("device_bytes", C.c_int64),
+                ("sweep_rows_nf", C.c_int64), ("sweep_rows_ns", C.c_int64),
+                ("panel_size", C.c_int64), ("level_launches", C.c_int), ("nfused", C.c_int)]
 
 
 class SymbolicView(C.Structure):
@@ -57,6 +59,9 @@ PROTOTYPES = {
     "dcx_plan_get_perm": (C.c_int, [_vp, _i32p]),
     "dcx_plan_set_refinement": (C.c_int, [_vp, C.c_int]),
     "dcx_plan_set_fusion": (C.c_int, [_vp, C.c_int]),
+    "dcx_plan_set_timing": (C.c_int, [_vp, C.c_int]),
+    "dcx_plan_sweep_times": (C.c_int, [_vp, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                       C.POINTER(C.c_int)]),
     "dcx_factorize": (C.c_int, [_vp, _vp, _vp]),
     "dcx_solve": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, _vp]),
     "dcx_set_rhs_support": (C.c_int, [_vp, _i32p, C.c_int]),

@@ -912,8 +912,12 @@ struct dcx_plan {
   double* d_x = nullptr;     // permuted FP64 iterate (mixed)
   size_t r_bytes = 0, b_bytes = 0, x_bytes = 0;
   int64_t device_bytes = 0;
+  // optional event timing of the sweeps (forward, backward) of every solve
+  bool timing = false;
+  std::vector<hipEvent_t> ev;  // triples per sweep pair: start, mid, end
 
   ~dcx_plan() {
+    for (hipEvent_t e : ev) (void)hipEventDestroy(e);
     void* ptrs[] = {d_sup, d_lvl, d_relpos, d_cidx, d_ridx, d_a_e, d_a_row, d_a_col,
                     d_perm, d_iperm, d_pcrow, d_pcol, d_pvidx, d_crow, d_col, d_info, d_panels,
                     d_front[0], d_front[1], d_vals, d_active, d_act_lvl, d_upd[0],
@@ -1143,8 +1147,17 @@ int sweeps(dcx_plan* P, T* W, int k, bool sparse_rhs, const SweepIO& io, hipStre
   }
   const bool use_act = sparse_rhs && P->d_active;
   const int* act = use_act ? P->d_active : nullptr;
+  hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+  if (P->timing) {
+    for (auto& e : ev) {
+      HIP_TRY(hipEventCreate(&e));
+      P->ev.push_back(e);
+    }
+    HIP_TRY(hipEventRecord(ev[0], st));
+  }
   for (int pass = 0; pass < 2; ++pass) {
     const bool fwd = pass == 0;
+    if (!fwd && P->timing) HIP_TRY(hipEventRecord(ev[1], st));
     if (fwd && S.nfused) {
       int rc = launch_fused<T>(P, W, k, true, act, io, st);
       if (rc) return rc;
@@ -1181,6 +1194,7 @@ int sweeps(dcx_plan* P, T* W, int k, bool sparse_rhs, const SweepIO& io, hipStre
       if (rc) return rc;
     }
   }
+  if (P->timing) HIP_TRY(hipEventRecord(ev[2], st));
   HIP_TRY(hipGetLastError());
   return DCX_OK;
 }
@@ -1287,6 +1301,20 @@ int dcx_plan_get_info(const dcx_plan* P, dcx_plan_info* o) {
   o->factor_flops = S.factor_flops;
   o->solve_flops_per_rhs = S.solve_flops_per_rhs;
   o->device_bytes = P->device_bytes;
+  int64_t snf = 0, sns = 0;
+  int nl = 0;
+  for (int d = 0; d < S.ndepth; ++d) {
+    if (S.ulvl_ptr[d + 1] > S.ulvl_ptr[d]) ++nl;
+    for (int q = S.ulvl_ptr[d]; q < S.ulvl_ptr[d + 1]; ++q) {
+      snf += S.ns[S.ulvl_sup[q]] + S.rs[S.ulvl_sup[q]];
+      sns += S.ns[S.ulvl_sup[q]];
+    }
+  }
+  o->sweep_rows_nf = snf;
+  o->sweep_rows_ns = sns;
+  o->panel_size = S.panel_size;
+  o->level_launches = nl;
+  o->nfused = S.nfused;
   return DCX_OK;
 }
 
@@ -1299,6 +1327,34 @@ int dcx_plan_get_perm(const dcx_plan* P, int32_t* perm) {
 int dcx_plan_set_refinement(dcx_plan* P, int passes) {
   if (!P || passes < 0 || passes > 16) return fail(DCX_ERR_ARG, "passes must be 0..16");
   P->refine = passes;
+  return DCX_OK;
+}
+
+int dcx_plan_set_timing(dcx_plan* P, int enable) {
+  if (!P) return fail(DCX_ERR_ARG, "null argument");
+  HIP_TRY(hipSetDevice(P->device));
+  for (hipEvent_t e : P->ev) HIP_TRY(hipEventDestroy(e));
+  P->ev.clear();
+  P->timing = enable != 0;
+  return DCX_OK;
+}
+
+int dcx_plan_sweep_times(dcx_plan* P, double* fwd_ms, double* bwd_ms, int* nsweeps) {
+  if (!P || !fwd_ms || !bwd_ms || !nsweeps) return fail(DCX_ERR_ARG, "null argument");
+  HIP_TRY(hipSetDevice(P->device));
+  double f = 0.0, b = 0.0;
+  const int n = static_cast<int>(P->ev.size() / 3);
+  for (int i = 0; i < n; ++i) {
+    float a = 0.f, c = 0.f;
+    HIP_TRY(hipEventSynchronize(P->ev[3 * i + 2]));
+    HIP_TRY(hipEventElapsedTime(&a, P->ev[3 * i], P->ev[3 * i + 1]));
+    HIP_TRY(hipEventElapsedTime(&c, P->ev[3 * i + 1], P->ev[3 * i + 2]));
+    f += a;
+    b += c;
+  }
+  *fwd_ms = f;
+  *bwd_ms = b;
+  *nsweeps = n;
   return DCX_OK;
 }
 

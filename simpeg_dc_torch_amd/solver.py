@@ -115,6 +115,17 @@ class HipBatchSolver:
             _lib.check(self._lib.dcx_plan_set_fusion(self._h, int(rows)), "set_fusion")
         self.fuse_rows = int(rows)
 
+    def set_timing(self, enable=True):
+        """Record HIP events around the sweeps of every solve (on its stream)."""
+        _lib.check(self._lib.dcx_plan_set_timing(self._h, int(bool(enable))), "set_timing")
+
+    def sweep_times(self):
+        """(forward ms, backward ms, number of solves) since timing was enabled."""
+        f, b, n = C.c_double(), C.c_double(), C.c_int()
+        _lib.check(self._lib.dcx_plan_sweep_times(self._h, C.byref(f), C.byref(b), C.byref(n)),
+                   "sweep_times")
+        return f.value, b.value, n.value
+
     def set_rhs_support(self, rows):
         r, rp = _lib.i32(np.asarray(rows))
         _lib.check(self._lib.dcx_set_rhs_support(self._h, rp, int(r.size)), "set_rhs_support")
