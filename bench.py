@@ -105,6 +105,19 @@ def sweep_roofline(eng, mt, dobs, w, reps=3):
             "fwd_sweep_ms": round(f_ms / nsolve, 3), "bwd_sweep_ms": round(b_ms / nsolve, 3)}
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_line25k_eval_grad.json")
+
+
+def measured_traffic(kernel):
+    """HBM bytes per dispatch of `kernel` from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE + WRITE_SIZE, calibrated on a 1 GiB copy), or None."""
+    try:
+        with open(TRAFFIC_FILE) as fh:
+            return int(json.load(fh)["kernels"][kernel]["traffic_bytes"])
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def cpu_baseline(mesh, survey, m, kys, wq, sample_ky=1):
     """The oracle's SuperLU path (reference CPU solver semantics) on a sample
     of `sample_ky` wavenumbers, scaled to all of them."""
@@ -152,8 +165,8 @@ def main():
     ap.add_argument("--pmc-pass", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(see DESIGN.md for how the path would shard)", file=sys.stderr)
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported "
+              "(the path runs on one GPU; there is no multi-GPU code path)", file=sys.stderr)
         sys.exit(2)
     if args.pmc_pass:
         pmc_pass(args)
@@ -182,6 +195,8 @@ def main():
     log(f"mixed eval+grad {msmx:.2f} ms")
     roof = sweep_roofline(emx, mt, dobs, w)
     roof64 = sweep_roofline(e64, mt, dobs, w)
+    roof["traffic"] = measured_traffic("void k_bwd_level<float, 32, false>")
+    roof64["traffic"] = measured_traffic("void k_bwd_level<double, 32, false>")
     cpu = None
     if not args.no_cpu:
         log("cpu baseline (oracle SuperLU, 1 wavenumber sample)")
