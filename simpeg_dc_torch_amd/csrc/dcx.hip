@@ -83,7 +83,7 @@ __global__ __launch_bounds__(kThreads) void k_factor_level(
     const int* __restrict__ a_e, const int* __restrict__ a_row,
     const int* __restrict__ a_col, const double* __restrict__ vals, int64_t nnz,
     T* __restrict__ fcur, int64_t fcur_bs, const T* __restrict__ fprev,
-    int64_t fprev_bs, T* __restrict__ panels, int64_t panel_bs,
+    int64_t fprev_bs, T* __restrict__ panels, T* __restrict__ panelsT, int64_t panel_bs,
     int* __restrict__ info) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T* pan = reinterpret_cast<T*>(smem_raw);
@@ -221,7 +221,10 @@ __global__ __launch_bounds__(kThreads) void k_factor_level(
   }
   // panel for the sweeps: rows [0, ns) = L_JJ^{-1}, rows [ns, nf) = Q_R,
   // both read from the E rows
+  // also stored transposed (PT = P^T, ns x nf column-major) for the backward
+  // sweep, whose operand runs along the rows of P
   T* P = panels + b * panel_bs + S.panel_off;
+  T* PT = panelsT + b * panel_bs + S.panel_off;
   for (int64_t idx = tid; idx < static_cast<int64_t>(nf) * ns; idx += kThreads) {
     const int r = static_cast<int>(idx % nf), c = static_cast<int>(idx / nf);
     T val;
@@ -230,6 +233,11 @@ __global__ __launch_bounds__(kThreads) void k_factor_level(
     else
       val = -F[(nf + c) + static_cast<int64_t>(r) * ld];
     P[r + static_cast<int64_t>(c) * nf] = val;
+  }
+  for (int64_t idx = tid; idx < static_cast<int64_t>(nf) * ns; idx += kThreads) {
+    const int c = static_cast<int>(idx % ns), r = static_cast<int>(idx / ns);
+    const T v = F[(nf + c) + static_cast<int64_t>(r) * ld];
+    PT[c + static_cast<int64_t>(r) * ns] = r < ns ? (r >= c ? v : T(0)) : -v;
   }
 }
 
@@ -388,7 +396,7 @@ __global__ __launch_bounds__(kThreads) void k_bwd_level(
   const SupDev S = sup[s];
   const int ns = S.ns, nf = S.ns + S.rs;
   const bool act = !active || active[s];
-  const T* Pg = panels + b * panel_bs + S.panel_off;
+  const T* Pg = panels + b * panel_bs + S.panel_off;  // the transposed panel PT
   T* Ps = reinterpret_cast<T*>(smem_raw);
   T* g = PL ? Ps + pmax : Ps;
   if (PL) {
@@ -396,7 +404,7 @@ __global__ __launch_bounds__(kThreads) void k_bwd_level(
     for (int idx = tid; idx < np; idx += kThreads) Ps[idx] = Pg[idx];
   }
   const T* P = PL ? Ps : Pg;
-  auto PaT = [=](int i, int kk) { return P[kk + i * nf]; };
+  auto PaT = [=](int i, int kk) { return P[i + kk * ns]; };
   T* Wb = W + b * w_bs;
   constexpr int NCT = TC / 16;
   const int nt = ((ns + 15) / 16) * NCT;
@@ -494,9 +502,9 @@ __global__ __launch_bounds__(kThreads) void k_bwd_fused(
       const int s = fmem[mi];
       const SupDev S = sup[s];
       const int ns = S.ns, nf = S.ns + S.rs;
-      const T* P = panels + b * panel_bs + S.panel_off;
+      const T* P = panels + b * panel_bs + S.panel_off;  // the transposed panel PT
       auto PaT = [=](int i, int kk) {
-        const T v = P[kk + static_cast<int64_t>(i) * nf];
+        const T v = P[i + static_cast<int64_t>(kk) * ns];
         return kk < ns ? v : -v;
       };
       const int nt = ((ns + 15) / 16) * NCT;
@@ -888,6 +896,7 @@ struct dcx_plan {
   int* d_info = nullptr;
   // factor
   void* d_panels = nullptr;
+  void* d_panelsT = nullptr;
   void* d_front[2] = {nullptr, nullptr};
   double* d_vals = nullptr;
   bool factored = false;
@@ -922,7 +931,7 @@ struct dcx_plan {
                     d_perm, d_iperm, d_pcrow, d_pcol, d_pvidx, d_crow, d_col, d_info, d_panels,
                     d_front[0], d_front[1], d_vals, d_active, d_act_lvl, d_upd[0],
                     d_upd[1], d_w, d_r, d_b, d_x, d_fz, d_flv, d_fmem, d_fgptr, d_fgb,
-                    d_toff, d_lridx, d_ulvl, d_ufz};
+                    d_toff, d_lridx, d_ulvl, d_ufz, d_panelsT};
     for (void* p : ptrs)
       if (p) (void)hipFree(p);
   }
@@ -1007,7 +1016,7 @@ int launch_factor(dcx_plan* P, hipStream_t st) {
                        P->d_lvl + S.lvl_ptr[d], P->d_relpos, P->d_cidx, P->d_a_e,
                        P->d_a_row, P->d_a_col, P->d_vals, static_cast<int64_t>(S.nnz),
                        fr[d & 1], fbs[d & 1], fr[(d + 1) & 1], fbs[(d + 1) & 1], panels,
-                       S.panel_size, P->d_info);
+                       static_cast<T*>(P->d_panelsT), S.panel_size, P->d_info);
   }
   HIP_TRY(hipGetLastError());
   return DCX_OK;
@@ -1044,8 +1053,8 @@ void sweep_launch(dcx_plan* P, T* W, int k, int maxnf, int maxns, int pmax, int 
                        ct, pmax);
   } else {
     hipLaunchKernelGGL((k_bwd_level<T, TC, PL>), grid, dim3(kThreads), lds, st, P->d_sup, lvl,
-                       P->d_ridx, active, panels, S.panel_size, W, w_bs, k, io.X64, io.xrow,
-                       io.xmode, ct, pmax);
+                       P->d_ridx, active, static_cast<const T*>(P->d_panelsT), S.panel_size, W,
+                       w_bs, k, io.X64, io.xrow, io.xmode, ct, pmax);
   }
 }
 
@@ -1109,8 +1118,9 @@ void fused_tc(dcx_plan* P, T* W, int k, bool fwd, const int* active, const Sweep
                        panels, S.panel_size, io.B, io.b_bs, io.brow, W, w_bs, k, ufz, ufz_bs);
   else
     hipLaunchKernelGGL((k_bwd_fused<T, TC>), grid, dim3(kThreads), lds, st, P->d_sup, P->d_fz,
-                       P->d_flv, P->d_fmem, P->d_toff, P->d_lridx, P->d_ridx, active, panels,
-                       S.panel_size, W, w_bs, k, io.X64, io.xrow, io.xmode);
+                       P->d_flv, P->d_fmem, P->d_toff, P->d_lridx, P->d_ridx, active,
+                       static_cast<const T*>(P->d_panelsT), S.panel_size, W, w_bs, k, io.X64,
+                       io.xrow, io.xmode);
 }
 
 template <typename T>
@@ -1270,10 +1280,11 @@ int dcx_plan_create(int n, const int32_t* crow, const int32_t* col, int nbatch,
     const size_t f1 = static_cast<size_t>(nbatch) * S.front_size[1] * P->tsize;
     const size_t vb = static_cast<size_t>(nbatch) * S.nnz * sizeof(double);
     HIP_TRY(hipMalloc(&P->d_panels, std::max<size_t>(pb, 256)));
+    HIP_TRY(hipMalloc(&P->d_panelsT, std::max<size_t>(pb, 256)));
     HIP_TRY(hipMalloc(&P->d_front[0], std::max<size_t>(f0, 256)));
     HIP_TRY(hipMalloc(&P->d_front[1], std::max<size_t>(f1, 256)));
     HIP_TRY(hipMalloc(&P->d_vals, std::max<size_t>(vb, 256)));
-    P->device_bytes = static_cast<int64_t>(pb + f0 + f1 + vb);
+    P->device_bytes = static_cast<int64_t>(2 * pb + f0 + f1 + vb);
     *out = P.release();
     return DCX_OK;
   } catch (const std::exception& e) {
