@@ -303,6 +303,29 @@ struct RowId {
   __device__ inline int operator()(int k) const { return k; }
 };
 
+// Sweep grids are 1-D over (column tile, supernode, batch), column tile
+// fastest, so that the workgroups that read one supernode's panel run
+// together.  The hardware deals workgroups round-robin to the 8 XCDs; the
+// remap gives each XCD a contiguous range of logical indices, so a panel is
+// fetched into one L2 and re-read from there.  gridDim.x is a multiple of 8;
+// indices at or past `total` are padding.
+struct SweepBlock {
+  int s, b, z;
+  bool live;
+};
+__device__ inline SweepBlock sweep_block(const int* lvl, int cnt, int nzb, int total) {
+  const int per = gridDim.x >> 3;
+  const int L = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+  SweepBlock o;
+  o.live = L < total;
+  const int Lc = o.live ? L : 0;
+  o.z = Lc % nzb;
+  const int rest = Lc / nzb;
+  o.s = lvl[rest % cnt];
+  o.b = rest / cnt;
+  return o;
+}
+
 // forward sweep (deepest level first): f = [b_J; sum of the children's
 // updates] in LDS, then [y_J; w] = P_s f_J: y_J to W, u_s = f_R - w to the
 // update buffer of this depth.  PL: panel staged in LDS, `ct` column tiles
@@ -315,10 +338,13 @@ __global__ __launch_bounds__(kThreads) void k_fwd_level(
     int64_t panel_bs, const double* __restrict__ Bsrc, int64_t b_bs,
     const int* __restrict__ brow, T* __restrict__ W, int64_t w_bs, int k,
     T* __restrict__ ucur, int64_t ucur_bs, const T* __restrict__ uprev,
-    int64_t uprev_bs, const T* __restrict__ ufz, int64_t ufz_bs, int ct, int pmax) {
+    int64_t uprev_bs, const T* __restrict__ ufz, int64_t ufz_bs, int ct, int pmax, int cnt,
+    int nzb, int total) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  const int s = lvl[blockIdx.x];
-  const int b = blockIdx.y;
+  const SweepBlock sb = sweep_block(lvl, cnt, nzb, total);
+  if (!sb.live) return;
+  const int s = sb.s;
+  const int b = sb.b;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const SupDev S = sup[s];
   const int ns = S.ns, nf = S.ns + S.rs;
@@ -337,7 +363,7 @@ __global__ __launch_bounds__(kThreads) void k_fwd_level(
   constexpr int NCT = TC / 16;
   const int nt = ((nf + 15) / 16) * NCT;
   for (int tile = 0; tile < ct; ++tile) {
-    const int col0 = (blockIdx.z * ct + tile) * TC;
+    const int col0 = (sb.z * ct + tile) * TC;
     if (col0 >= k) break;
     __syncthreads();
     for (int idx = tid; idx < nf * TC; idx += kThreads) {
@@ -388,10 +414,12 @@ __global__ __launch_bounds__(kThreads) void k_bwd_level(
     const int* __restrict__ ridx, const int* __restrict__ active,
     const T* __restrict__ panels, int64_t panel_bs, T* __restrict__ W,
     int64_t w_bs, int k, double* __restrict__ X64, const int* __restrict__ xrow,
-    int xmode, int ct, int pmax) {
+    int xmode, int ct, int pmax, int cnt, int nzb, int total) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  const int s = lvl[blockIdx.x];
-  const int b = blockIdx.y;
+  const SweepBlock sb = sweep_block(lvl, cnt, nzb, total);
+  if (!sb.live) return;
+  const int s = sb.s;
+  const int b = sb.b;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const SupDev S = sup[s];
   const int ns = S.ns, nf = S.ns + S.rs;
@@ -409,7 +437,7 @@ __global__ __launch_bounds__(kThreads) void k_bwd_level(
   constexpr int NCT = TC / 16;
   const int nt = ((ns + 15) / 16) * NCT;
   for (int tile = 0; tile < ct; ++tile) {
-    const int col0 = (blockIdx.z * ct + tile) * TC;
+    const int col0 = (sb.z * ct + tile) * TC;
     if (col0 >= k) break;
     __syncthreads();
     for (int idx = tid; idx < nf * TC; idx += kThreads) {
@@ -1039,7 +1067,9 @@ void sweep_launch(dcx_plan* P, T* W, int k, int maxnf, int maxns, int pmax, int 
                   hipStream_t st) {
   const auto& S = P->S;
   const int nz = (k + TC - 1) / TC;
-  dim3 grid(cnt, P->nbatch, (nz + ct - 1) / ct);
+  const int nzb = (nz + ct - 1) / ct;
+  const int total = cnt * P->nbatch * nzb;
+  dim3 grid(((total + 7) / 8) * 8);
   const size_t lds = (static_cast<size_t>(PL ? pmax : 0) + static_cast<size_t>(maxnf) * TC) * sizeof(T);
   T* panels = static_cast<T*>(P->d_panels);
   const int64_t w_bs = static_cast<int64_t>(S.n) * k;
@@ -1050,11 +1080,11 @@ void sweep_launch(dcx_plan* P, T* W, int k, int maxnf, int maxns, int pmax, int 
                        P->d_relpos, P->d_cidx, active, panels, S.panel_size, io.B, io.b_bs,
                        io.brow, W, w_bs, k, u[d & 1], ubs[d & 1], u[(d + 1) & 1],
                        ubs[(d + 1) & 1], static_cast<const T*>(P->d_ufz), P->S.f_upd_rows * k,
-                       ct, pmax);
+                       ct, pmax, cnt, nzb, total);
   } else {
     hipLaunchKernelGGL((k_bwd_level<T, TC, PL>), grid, dim3(kThreads), lds, st, P->d_sup, lvl,
                        P->d_ridx, active, static_cast<const T*>(P->d_panelsT), S.panel_size, W,
-                       w_bs, k, io.X64, io.xrow, io.xmode, ct, pmax);
+                       w_bs, k, io.X64, io.xrow, io.xmode, ct, pmax, cnt, nzb, total);
   }
 }
 
