@@ -72,12 +72,14 @@ def time_steps(eng, mt, dobs, w, steps, warmup):
     return 1e3 * (time.perf_counter() - t0) / steps
 
 
-def sweep_roofline(eng, mt, dobs, w, reps=3):
+def sweep_roofline(eng, mt, dobs, w, refine, reps=3):
     """Backward-sweep kernel (k_bwd_level) of the engine's solves: average
     launch duration from HIP events recorded on the solve stream, and its
     algorithmic bytes and flops per launch."""
     sv = eng.solver
     info = sv.info()
+    # a mixed plan writes an FP64 copy of x_J unless the fields stay FP32
+    x64_copy = info["precision"] == 1 and (not eng._gpu.fields32 or refine > 0)
     sv.set_timing(True)
     for _ in range(reps):
         eval_grad(eng, mt, dobs, w)
@@ -87,10 +89,10 @@ def sweep_roofline(eng, mt, dobs, w, reps=3):
     launches = nsolve * info["level_launches"]
     ts = 4 if info["precision"] == 1 else 8
     nb, k = info["nbatch"], eng.nS
-    # per backward sweep: read y_J and x_R rows, write x_J rows (and the FP64
-    # solution in mixed mode), read every panel once
+    # per backward sweep: read y_J and x_R rows, write x_J rows (and an FP64
+    # copy when x64_copy), read every panel once
     per_sweep = nb * k * (ts * (info["sweep_rows_nf"] + info["sweep_rows_ns"])
-                          + (8 * info["sweep_rows_ns"] if ts == 4 else 0)) \
+                          + (8 * info["sweep_rows_ns"] if x64_copy else 0)) \
         + nb * info["panel_size"] * ts
     flops = 2.0 * nb * k * info["solve_flops_per_rhs"] / 4.0
     avg_ms = b_ms / launches
@@ -193,8 +195,8 @@ def main():
     log("timing mixed")
     msmx = time_steps(emx, mt, dobs, w, args.steps, args.warmup)
     log(f"mixed eval+grad {msmx:.2f} ms")
-    roof = sweep_roofline(emx, mt, dobs, w)
-    roof64 = sweep_roofline(e64, mt, dobs, w)
+    roof = sweep_roofline(emx, mt, dobs, w, args.refine)
+    roof64 = sweep_roofline(e64, mt, dobs, w, 0)
     roof["traffic"] = measured_traffic("void k_bwd_level<float, 32, false>")
     roof64["traffic"] = measured_traffic("void k_bwd_level<double, 32, false>")
     cpu = None
@@ -209,7 +211,7 @@ def main():
         "metric": METRIC, "value": round(msmx, 3), "unit": "ms", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(msmx, 3),
         "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32 factor+sweeps; f64 assembly/projection/gather",
+        "dtype": "f32 factor, sweeps and fields; f64 assembly, projection, ky sum, gather sums",
         "data": "synthetic",
         "config": {"workload": "2.5-D line: 25,088 cells, 61 electrodes DD+WS, "
                                f"{e64.nS} sources, {e64.nD} data, nky={e64.nky} "

@@ -43,7 +43,9 @@ enum { DCX_FP64 = 0, DCX_FP32 = 1 };
 enum {
   DCX_SHARED_RHS = 1,  /* b is one (n x k) block shared by every batch member */
   DCX_PERMUTED = 2,    /* b and x are in the plan's elimination order */
-  DCX_SPARSE_RHS = 4   /* b is zero outside the rows set by dcx_set_rhs_support */
+  DCX_SPARSE_RHS = 4,  /* b is zero outside the rows set by dcx_set_rhs_support */
+  DCX_F32_FIELDS = 8   /* the field blocks (dcx_solve's x, dcx_grad_values' y and
+                          x, dcx_survey_dpred's U) are float, not double */
 };
 
 typedef struct {
@@ -121,9 +123,12 @@ int dcx_factorize(dcx_plan* plan, const double* values, void* stream);
 
 /* Multi-RHS solve of every batch member, FP64 in and out.  Replaces
  * CuDSSBatchSolver.solve (dctorch/solver.py:223-230).  b: device, (n x k)
- * with DCX_SHARED_RHS, else (nbatch x n x k); x: device (nbatch x n x k).
- * b and x may alias when not shared. */
-int dcx_solve(dcx_plan* plan, const double* b, double* x, int k, int flags,
+ * with DCX_SHARED_RHS, else (nbatch x n x k); x: device (nbatch x n x k),
+ * double, or float with DCX_F32_FIELDS (FP32 plans and DCX_PERMUTED only:
+ * x is then the sweeps' own working block, the FP64 refinement passes, if
+ * any, accumulate in a plan buffer and are rounded into x at the end).
+ * b and x may alias when not shared (double x only). */
+int dcx_solve(dcx_plan* plan, const double* b, void* x, int k, int flags,
               void* stream);
 
 /* Rows (host, original numbering) outside of which right-hand sides passed
@@ -134,9 +139,10 @@ int dcx_set_rhs_support(dcx_plan* plan, const int32_t* rows, int nrows);
 /* Backward gather of the solve: gv[b][e] = -scale[b] * sum_c y[b][row_e][c] *
  * x[b][col_e][c] over the pattern's entries e.  Replaces the chunked gather of
  * _SparseSolveBatchFn.backward (dctorch/solver.py:268-276).  y, x: device
- * (nbatch x n x k); scale: device (nbatch) or NULL (= 1); gv: device
+ * (nbatch x n x k), double, or float with DCX_F32_FIELDS (the sums are FP64
+ * either way); scale: device (nbatch) or NULL (= 1); gv: device
  * (nbatch x nnz).  With DCX_PERMUTED the blocks are in elimination order. */
-int dcx_grad_values(dcx_plan* plan, const double* y, const double* x, int k,
+int dcx_grad_values(dcx_plan* plan, const void* y, const void* x, int k,
                     const double* scale, double* gv, int flags, void* stream);
 
 /* Lane-layout probe of the 16x16x4 matrix instruction (fp32 = 0: FP64):
@@ -179,8 +185,9 @@ int dcx_survey_create(dcx_plan* plan, int nD, int nS, const int32_t* pd_ptr,
 int dcx_survey_destroy(dcx_survey* sv);
 /* d[i] = sum_k w[k] sum_j Pd[i,j] U[k][j][sidx_i - s0] for the data whose
  * source lies in [s0, s1) (the others are left untouched); U device
- * (nky x n x k), rows in elimination order (DCX_PERMUTED) or original order. */
-int dcx_survey_dpred(dcx_survey* sv, const double* U, int k, int nky,
+ * (nky x n x k), double or float (DCX_F32_FIELDS), rows in elimination order
+ * (DCX_PERMUTED) or original order. */
+int dcx_survey_dpred(dcx_survey* sv, const void* U, int k, int nky,
                      const double* w, int s0, int s1, double* d, int flags,
                      void* stream);
 /* G[j][s - s0] = sum over data i of source s in [s0, s1) of Pd[i,j] v[i]:

@@ -13,7 +13,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libdcx.so")
 
 DCX_FP64, DCX_FP32 = 0, 1
-DCX_SHARED_RHS, DCX_PERMUTED, DCX_SPARSE_RHS = 1, 2, 4
+DCX_SHARED_RHS, DCX_PERMUTED, DCX_SPARSE_RHS, DCX_F32_FIELDS = 1, 2, 4, 8
 
 _i32p = C.POINTER(C.c_int32)
 _f64p = C.POINTER(C.c_double)

@@ -753,9 +753,10 @@ __global__ void k_residual(const int* __restrict__ crow, const int* __restrict__
 // that each XCD walks one contiguous range of rows (its own L2).  gridDim.x
 // must be a multiple of 8.  iperm (may be null) maps rows to the storage
 // order of Y and X.
+template <typename TI>
 __global__ void k_grad_values(const int* __restrict__ crow, const int* __restrict__ ccol,
-                              const int* __restrict__ iperm, const double* __restrict__ Y,
-                              const double* __restrict__ X, int n, int k,
+                              const int* __restrict__ iperm, const TI* __restrict__ Y,
+                              const TI* __restrict__ X, int n, int k,
                               const double* __restrict__ scale, double* __restrict__ gv,
                               int64_t nnz) {
   constexpr int MAXE = 8;
@@ -767,11 +768,11 @@ __global__ void k_grad_values(const int* __restrict__ crow, const int* __restric
   if (row >= n) return;
   const int64_t xb = static_cast<int64_t>(b) * n * k;
   const int prow = iperm ? iperm[row] : row;
-  const double* y = Y + xb + static_cast<int64_t>(prow) * k;
+  const TI* y = Y + xb + static_cast<int64_t>(prow) * k;
   const double sc = scale ? scale[b] : 1.0;
   for (int p0 = crow[row]; p0 < crow[row + 1]; p0 += MAXE) {
     const int ne = min(MAXE, crow[row + 1] - p0);
-    const double* xr[MAXE];
+    const TI* xr[MAXE];
 #pragma unroll
     for (int q = 0; q < MAXE; ++q) {
       const int cq = q < ne ? ccol[p0 + q] : ccol[p0];
@@ -781,10 +782,10 @@ __global__ void k_grad_values(const int* __restrict__ crow, const int* __restric
 #pragma unroll
     for (int q = 0; q < MAXE; ++q) acc[q] = 0.0;
     for (int c = lane; c < k; c += 64) {
-      const double yv = y[c];
+      const double yv = static_cast<double>(y[c]);
 #pragma unroll
       for (int q = 0; q < MAXE; ++q)
-        if (q < ne) acc[q] += yv * xr[q][c];
+        if (q < ne) acc[q] += yv * static_cast<double>(xr[q][c]);
     }
 #pragma unroll
     for (int q = 0; q < MAXE; ++q) {
@@ -864,9 +865,10 @@ __global__ void k_lmap_t(int ncell, int nnz, int nky, const int* __restrict__ tp
 // ---------------------------------------------------------------------------
 // survey: projection and adjoint sources
 // ---------------------------------------------------------------------------
+template <typename TU>
 __global__ void k_dpred(int nD, const int* __restrict__ ptr, const int* __restrict__ rows,
                         const double* __restrict__ val, const int* __restrict__ sidx,
-                        const double* __restrict__ U, int64_t bs, int k, int nky,
+                        const TU* __restrict__ U, int64_t bs, int k, int nky,
                         const double* __restrict__ w, int s0, int s1, double* __restrict__ d) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nD) return;
@@ -876,7 +878,7 @@ __global__ void k_dpred(int nD, const int* __restrict__ ptr, const int* __restri
   for (int kk = 0; kk < nky; ++kk) {
     double a = 0.0;
     for (int p = ptr[i]; p < ptr[i + 1]; ++p)
-      a += val[p] * U[kk * bs + static_cast<int64_t>(rows[p]) * k + s];
+      a += val[p] * static_cast<double>(U[kk * bs + static_cast<int64_t>(rows[p]) * k + s]);
     acc += w[kk] * a;
   }
   d[i] = acc;
@@ -1463,8 +1465,8 @@ int dcx_factorize(dcx_plan* P, const double* values, void* stream) {
   return DCX_OK;
 }
 
-int dcx_solve(dcx_plan* P, const double* b, double* x, int k, int flags, void* stream) {
-  if (!P || !b || !x) return fail(DCX_ERR_ARG, "null argument");
+int dcx_solve(dcx_plan* P, const double* b, void* xv, int k, int flags, void* stream) {
+  if (!P || !b || !xv) return fail(DCX_ERR_ARG, "null argument");
   if (!P->factored) return fail(DCX_ERR_STATE, "dcx_solve before dcx_factorize");
   if (k < 1) return fail(DCX_ERR_ARG, "k must be >= 1");
   hipStream_t st = static_cast<hipStream_t>(stream);
@@ -1474,6 +1476,12 @@ int dcx_solve(dcx_plan* P, const double* b, double* x, int k, int flags, void* s
   const bool shared = flags & DCX_SHARED_RHS;
   const bool permuted = flags & DCX_PERMUTED;
   const bool sparse = (flags & DCX_SPARSE_RHS) != 0;
+  const bool f32x = (flags & DCX_F32_FIELDS) != 0;
+  if (f32x && (P->precision != DCX_FP32 || !permuted))
+    return fail(DCX_ERR_ARG, "DCX_F32_FIELDS needs an FP32 plan and DCX_PERMUTED");
+  if (f32x && static_cast<const void*>(b) == xv)
+    return fail(DCX_ERR_ARG, "b and a float x must not alias");
+  double* x = f32x ? nullptr : static_cast<double*>(xv);
   const int64_t blk = static_cast<int64_t>(S.n) * k;
   const int n = S.n;
   SweepIO io;
@@ -1492,10 +1500,22 @@ int dcx_solve(dcx_plan* P, const double* b, double* x, int k, int flags, void* s
     }
     return sweeps<double>(P, W, k, sparse, io, st);
   }
-  // mixed: FP32 sweeps, FP64 residual refinement; the solution accumulates in x
-  int rc = ensure(&P->d_w, &P->w_bytes, static_cast<size_t>(nb) * blk * 4, st);
-  if (rc) return rc;
-  float* W = static_cast<float*>(P->d_w);
+  // mixed: FP32 sweeps, FP64 residual refinement; the solution accumulates in
+  // x (double) or, for a float x, in a plan buffer rounded into x at the end
+  int rc;
+  float* W;
+  if (f32x) {
+    W = static_cast<float*>(xv);
+    if (P->refine == 0) return sweeps<float>(P, W, k, sparse, io, st);
+    rc = ensure(reinterpret_cast<void**>(&P->d_x), &P->x_bytes,
+                static_cast<size_t>(nb) * blk * 8, st);
+    if (rc) return rc;
+    x = P->d_x;
+  } else {
+    rc = ensure(&P->d_w, &P->w_bytes, static_cast<size_t>(nb) * blk * 4, st);
+    if (rc) return rc;
+    W = static_cast<float*>(P->d_w);
+  }
   if (P->refine > 0 && b == x) {  // the residual needs the right-hand side intact
     rc = ensure(reinterpret_cast<void**>(&P->d_b), &P->b_bytes,
                 static_cast<size_t>(shared ? 1 : nb) * blk * 8, st);
@@ -1530,11 +1550,14 @@ int dcx_solve(dcx_plan* P, const double* b, double* x, int k, int flags, void* s
     rc = sweeps<float>(P, W, k, false, ir, st);
     if (rc) return rc;
   }
+  if (f32x)
+    hipLaunchKernelGGL((k_convert<double, float>), dim3(grid_stride(nb * blk)), dim3(256), 0, st,
+                       x, W, nb * blk);
   HIP_TRY(hipGetLastError());
   return DCX_OK;
 }
 
-int dcx_grad_values(dcx_plan* P, const double* y, const double* x, int k,
+int dcx_grad_values(dcx_plan* P, const void* y, const void* x, int k,
                     const double* scale, double* gv, int flags, void* stream) {
   if (!P || !y || !x || !gv) return fail(DCX_ERR_ARG, "null argument");
   if (k < 1) return fail(DCX_ERR_ARG, "k must be >= 1");
@@ -1543,10 +1566,15 @@ int dcx_grad_values(dcx_plan* P, const double* y, const double* x, int k,
   const auto& S = P->S;
   const int blocks = ((grid1(S.n, 4) + 7) / 8) * 8;  // XCD remap needs a multiple of 8
   dim3 grid(blocks, P->nbatch);
-  hipLaunchKernelGGL(k_grad_values, grid, dim3(256), 0, st, P->d_crow, P->d_col,
-                     (flags & DCX_PERMUTED) ? static_cast<const int*>(P->d_iperm)
-                                            : static_cast<const int*>(nullptr),
-                     y, x, S.n, k, scale, gv, static_cast<int64_t>(S.nnz));
+  const int* ip = (flags & DCX_PERMUTED) ? static_cast<const int*>(P->d_iperm) : nullptr;
+  if (flags & DCX_F32_FIELDS)
+    hipLaunchKernelGGL(k_grad_values<float>, grid, dim3(256), 0, st, P->d_crow, P->d_col, ip,
+                       static_cast<const float*>(y), static_cast<const float*>(x), S.n, k,
+                       scale, gv, static_cast<int64_t>(S.nnz));
+  else
+    hipLaunchKernelGGL(k_grad_values<double>, grid, dim3(256), 0, st, P->d_crow, P->d_col, ip,
+                       static_cast<const double*>(y), static_cast<const double*>(x), S.n, k,
+                       scale, gv, static_cast<int64_t>(S.nnz));
   HIP_TRY(hipGetLastError());
   return DCX_OK;
 }
@@ -1811,7 +1839,7 @@ int dcx_survey_destroy(dcx_survey* v) {
   return DCX_OK;
 }
 
-int dcx_survey_dpred(dcx_survey* V, const double* U, int k, int nky, const double* w, int s0,
+int dcx_survey_dpred(dcx_survey* V, const void* U, int k, int nky, const double* w, int s0,
                      int s1, double* d, int flags, void* stream) {
   if (!V || !U || !w || !d) return fail(DCX_ERR_ARG, "null argument");
   if (s0 < 0 || s1 > V->nS || s0 > s1 || s1 - s0 > k)
@@ -1819,9 +1847,15 @@ int dcx_survey_dpred(dcx_survey* V, const double* U, int k, int nky, const doubl
   hipStream_t st = static_cast<hipStream_t>(stream);
   HIP_TRY(hipSetDevice(V->device));
   const int64_t bs = static_cast<int64_t>(V->n) * k;
-  hipLaunchKernelGGL(k_dpred, dim3(grid1(V->nD, 256)), dim3(256), 0, st, V->nD, V->d_ptr,
-                     (flags & DCX_PERMUTED) ? V->d_rows_p : V->d_rows, V->d_val, V->d_sidx, U,
-                     bs, k, nky, w, s0, s1, d);
+  const int* rows = (flags & DCX_PERMUTED) ? V->d_rows_p : V->d_rows;
+  if (flags & DCX_F32_FIELDS)
+    hipLaunchKernelGGL(k_dpred<float>, dim3(grid1(V->nD, 256)), dim3(256), 0, st, V->nD,
+                       V->d_ptr, rows, V->d_val, V->d_sidx, static_cast<const float*>(U), bs, k,
+                       nky, w, s0, s1, d);
+  else
+    hipLaunchKernelGGL(k_dpred<double>, dim3(grid1(V->nD, 256)), dim3(256), 0, st, V->nD,
+                       V->d_ptr, rows, V->d_val, V->d_sidx, static_cast<const double*>(U), bs,
+                       k, nky, w, s0, s1, d);
   HIP_TRY(hipGetLastError());
   return DCX_OK;
 }

@@ -94,6 +94,9 @@ class _GpuState:
         q = maps.q.tocsr()[self.solver.perm()].toarray()
         self.q_perm = torch.tensor(q, dtype=torch.float64, device=dev)
         self.wq = torch.tensor(eng.wq_np, dtype=torch.float64, device=dev)
+        # mixed precision keeps the eval+grad fields in FP32 (the factor's own
+        # precision): sums over data, wavenumbers and sources stay FP64
+        self.fields32 = precision == "mixed"
 
     def stream(self):
         return self.solver._stream()
@@ -111,19 +114,22 @@ class _GpuState:
                                                _lib.ptr(out), self.stream()), "lmap_transpose")
         return out
 
-    def solve_perm(self, b, k, sparse=True, shared=True):
+    def solve_perm(self, b, k, sparse=True, shared=True, f32=False):
+        """Solve in elimination order; f32: FP32 solution block (mixed plans)."""
         sv = self.solver
-        x = torch.empty(sv.nbatch, sv.n, k, dtype=torch.float64, device=b.device)
+        x = torch.empty(sv.nbatch, sv.n, k, dtype=torch.float32 if f32 else torch.float64,
+                        device=b.device)
         flags = _lib.DCX_PERMUTED | (_lib.DCX_SHARED_RHS if shared else 0) | \
-            (_lib.DCX_SPARSE_RHS if sparse else 0)
+            (_lib.DCX_SPARSE_RHS if sparse else 0) | (_lib.DCX_F32_FIELDS if f32 else 0)
         _lib.check(self.lib.dcx_solve(sv._h, _lib.ptr(b), _lib.ptr(x), int(k), flags,
                                       self.stream()), "solve")
         return x
 
     def dpred(self, U, k, s0, s1, d):
+        flags = _lib.DCX_PERMUTED | (_lib.DCX_F32_FIELDS if U.dtype == torch.float32 else 0)
         _lib.check(self.lib.dcx_survey_dpred(self.survey, _lib.ptr(U), int(k),
                                              self.solver.nbatch, _lib.ptr(self.wq), int(s0),
-                                             int(s1), _lib.ptr(d), _lib.DCX_PERMUTED,
+                                             int(s1), _lib.ptr(d), flags,
                                              self.stream()), "survey_dpred")
 
     def adjoint_rhs(self, v, k, s0, s1):
@@ -160,7 +166,7 @@ class _Forward2D(torch.autograd.Function):
             sig = torch.exp(-m_full.detach())
             vals = g.assemble(sig)
             g.solver.factorize(vals)
-            U = g.solve_perm(g.q_perm, eng.nS)
+            U = g.solve_perm(g.q_perm, eng.nS, f32=g.fields32)
             d = torch.empty(eng.nD, dtype=torch.float64, device=eng.device)
             g.dpred(U, eng.nS, 0, eng.nS, d)
         ctx.eng = eng
@@ -178,7 +184,7 @@ class _Forward2D(torch.autograd.Function):
                 g.solver.factorize(vals)
             v = gd.detach().to(torch.float64).contiguous()
             G = g.adjoint_rhs(v, eng.nS, 0, eng.nS)
-            Lam = g.solve_perm(G, eng.nS)
+            Lam = g.solve_perm(G, eng.nS, f32=g.fields32)
             gv = g.solver.grad_values(Lam, U, scale=g.wq, permuted=True)
             grad_m = g.lmap_t(gv, sig)
         return grad_m, None
@@ -380,13 +386,13 @@ class TorchDC2D:
             for s0, s1 in self._source_chunks(width):
                 k = s1 - s0
                 qb = g.q_perm[:, s0:s1].contiguous() if k != self.nS else g.q_perm
-                U = g.solve_perm(qb, k)
+                U = g.solve_perm(qb, k, f32=g.fields32)
                 g.dpred(U, k, s0, s1, d)
                 rows, _, _ = self._rows_of(s0, s1)
                 v, f = weights_of(d, rows)
                 total += f
                 G = g.adjoint_rhs(v, k, s0, s1)
-                Lam = g.solve_perm(G, k)
+                Lam = g.solve_perm(G, k, f32=g.fields32)
                 gv += g.solver.grad_values(Lam, U, scale=g.wq, permuted=True)
                 del U, Lam, G
             grad = self._model_from_full(g.lmap_t(gv, sig))

@@ -163,16 +163,20 @@ class HipBatchSolver:
         return x.to(self.dtype)
 
     def grad_values(self, adj, x, scale=None, permuted=False):
-        """-scale[b] sum_s adj[b, row_e, s] x[b, col_e, s] -> (nbatch, nnz) FP64."""
-        a = adj.detach().to(device=self.device, dtype=torch.float64).contiguous()
-        xx = x.detach().to(device=self.device, dtype=torch.float64).contiguous()
+        """-scale[b] sum_s adj[b, row_e, s] x[b, col_e, s] -> (nbatch, nnz) FP64.
+        Float32 adj and x are read as they are (FP64 sums)."""
+        f32 = adj.dtype == torch.float32 and x.dtype == torch.float32
+        dt = torch.float32 if f32 else torch.float64
+        a = adj.detach().to(device=self.device, dtype=dt).contiguous()
+        xx = x.detach().to(device=self.device, dtype=dt).contiguous()
         k = a.shape[-1]
         gv = torch.empty(self.nbatch, self.col.numel(), dtype=torch.float64, device=self.device)
         sc = None if scale is None else scale.to(self.device, torch.float64).contiguous()
         with self._lock, torch.cuda.device(self.device):
             _lib.check(self._lib.dcx_grad_values(
                 self._h, _lib.ptr(a), _lib.ptr(xx), int(k), _lib.ptr(sc), _lib.ptr(gv),
-                _lib.DCX_PERMUTED if permuted else 0, self._stream()), "grad_values")
+                (_lib.DCX_PERMUTED if permuted else 0) | (_lib.DCX_F32_FIELDS if f32 else 0),
+                self._stream()), "grad_values")
         return gv
 
     def free(self):

@@ -121,6 +121,44 @@ def test_batch_solver_shared_and_batched(precision, tol):
         assert rel(xb[i], lu.solve(bb[i])) < tol
 
 
+@pytest.mark.parametrize("refine", [0, 1])
+def test_f32_fields_match_f64_output(refine):
+    """DCX_F32_FIELDS: the FP32 solution block of a mixed plan is the FP64
+    output rounded to float (bit for bit), and the gather and projection read
+    it exactly as they read its FP64 copy."""
+    from simpeg_dc_torch_amd import _lib
+    _say(f"f32 fields refine={refine}")
+    A, _ = laplacian_fixture(nx=30, n_rhs=1)
+    n, k, nb = A.shape[0], 70, 2
+    rng = np.random.default_rng(11)
+    slv = _solver(A, k, nbatch=nb, precision="mixed", refine=refine)
+    slv.factorize(torch.tensor(np.stack([A.data, 1.7 * A.data]), device=DEV))
+    b = torch.tensor(rng.standard_normal((n, k)), device=DEV)
+    lib, st = slv._lib, slv._stream()
+    base = _lib.DCX_PERMUTED | _lib.DCX_SHARED_RHS
+    x64 = torch.empty(nb, n, k, dtype=torch.float64, device=DEV)
+    x32 = torch.empty(nb, n, k, dtype=torch.float32, device=DEV)
+    _lib.check(lib.dcx_solve(slv._h, _lib.ptr(b), _lib.ptr(x64), k, base, st), "solve")
+    _lib.check(lib.dcx_solve(slv._h, _lib.ptr(b), _lib.ptr(x32), k,
+                             base | _lib.DCX_F32_FIELDS, st), "solve f32")
+    torch.cuda.synchronize()
+    assert torch.equal(x32, x64.float())
+    y32 = torch.tensor(rng.standard_normal((nb, n, k)), dtype=torch.float32, device=DEV)
+    sc = torch.tensor([0.5, 2.0], dtype=torch.float64, device=DEV)
+    g32 = slv.grad_values(y32, x32, scale=sc, permuted=True)
+    g64 = slv.grad_values(y32.double(), x32.double(), scale=sc, permuted=True)
+    assert torch.equal(g32, g64)
+    # the FP32 result against SciPy at FP32-factor accuracy
+    perm = slv.perm()  # elimination row -> original row
+    xo, bo = np.empty((n, k)), np.empty((n, k))
+    bo[perm] = b.cpu().numpy()
+    for i, s in enumerate([1.0, 1.7]):
+        Ai = sp.csr_matrix((s * A.data, A.indices, A.indptr), shape=A.shape)
+        xo[perm] = x32[i].double().cpu().numpy()
+        ref = spla.splu(Ai.tocsc()).solve(bo)
+        assert rel(xo, ref) < (1e-5 if refine == 0 else 1e-6)
+
+
 def test_sparse_rhs_support_matches_dense():
     from simpeg_dc_torch_amd import _lib
     A, _ = laplacian_fixture(nx=40, n_rhs=1)
@@ -169,9 +207,10 @@ def test_engine_fp64_dpred_and_gradient(name):
     assert rel(g.cpu().numpy(), g_o) < 1e-9
 
 
-def test_engine_mixed_precision():
-    _say("engine mixed small_padded")
-    eng, O, m = _engine("small_padded", precision="mixed", refine=2)
+@pytest.mark.parametrize("refine", [0, 2])
+def test_engine_mixed_precision(refine):
+    _say(f"engine mixed small_padded refine={refine}")
+    eng, O, m = _engine("small_padded", precision="mixed", refine=refine)
     d_o = O.dpred(m)
     mt = torch.tensor(m, device=DEV, requires_grad=True)
     d = eng.dpred(mt)
@@ -179,7 +218,8 @@ def test_engine_mixed_precision():
     dobs, w = noisy_data(d_o)
     eng.misfit(mt, torch.tensor(dobs, device=DEV), torch.tensor(w, device=DEV)).backward()
     _, g_o, _ = O.misfit_and_grad(m, dobs, w)
-    assert rel(mt.grad.cpu().numpy(), g_o) < 1e-5
+    # gradient: FP32-factor accuracy without refinement, 1e-5 with it
+    assert rel(mt.grad.cpu().numpy(), g_o) < (1e-5 if refine else 1e-4)
 
 
 def test_engine_jtvec_jmatrix_chunks():
