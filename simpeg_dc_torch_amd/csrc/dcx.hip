@@ -63,187 +63,7 @@ hipError_t upload(T** dst, const std::vector<T>& v) {
 constexpr int kThreads = 256;
 
 // ---------------------------------------------------------------------------
-// numeric factorisation, one launch per depth level (deepest first)
-// grid (supernodes at level, batch), 256 threads.
-// The front F (column-major, leading dimension nf + ns) holds the lower
-// triangle of [J; R] in its first nf rows and, below them, ns extra rows E
-// that start as the identity in the J columns.  Eliminating the ns J columns
-// (panels of kNB columns staged in LDS, one barrier per column, 4x4
-// register-tiled trailing updates) leaves L_JJ, L_RJ, the Schur complement in
-// the (R,R) block -- this front's update block -- and L_EJ = L_JJ^{-T} in the
-// E rows, i.e. the inverse the sweeps need, without a separate inversion.
-// The panel written for the sweeps is P_s = [ L_JJ^{-1} ; L_RJ ] (nf x ns).
-// ---------------------------------------------------------------------------
-constexpr int kNB = 32;  // panel width
-
-template <typename T>
-__global__ __launch_bounds__(kThreads) void k_factor_level(
-    const SupDev* __restrict__ sup, const int* __restrict__ lvl,
-    const int* __restrict__ relpos, const int* __restrict__ cidx,
-    const int* __restrict__ a_e, const int* __restrict__ a_row,
-    const int* __restrict__ a_col, const double* __restrict__ vals, int64_t nnz,
-    T* __restrict__ fcur, int64_t fcur_bs, const T* __restrict__ fprev,
-    int64_t fprev_bs, T* __restrict__ panels, T* __restrict__ panelsT, int64_t panel_bs,
-    int* __restrict__ info) {
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  T* pan = reinterpret_cast<T*>(smem_raw);
-  const int s = lvl[blockIdx.x];
-  const int b = blockIdx.y;
-  const SupDev S = sup[s];
-  const int ns = S.ns, rs = S.rs, nf = S.ns + S.rs, ld = nf + S.ns;
-  const int tid = threadIdx.x;
-  T* F = fcur + b * fcur_bs + S.front_off;
-  const int64_t fsz = static_cast<int64_t>(ld) * nf;
-  for (int64_t i = tid; i < fsz; i += kThreads) F[i] = T(0);
-  __syncthreads();
-  for (int j = tid; j < ns; j += kThreads) F[(nf + j) + static_cast<int64_t>(j) * ld] = T(1);
-  const double* v = vals + b * nnz;
-  for (int q = S.a_begin + tid; q < S.a_end; q += kThreads)
-    F[a_row[q] + static_cast<int64_t>(a_col[q]) * ld] = static_cast<T>(v[a_e[q]]);
-  __syncthreads();
-  for (int cc = S.c_begin; cc < S.c_end; ++cc) {
-    const SupDev C = sup[cidx[cc]];
-    const int rc = C.rs, ldc = C.ns + C.rs + C.ns;
-    const T* Fc = fprev + b * fprev_bs + C.front_off;
-    const int* rp = relpos + C.rptr;
-    const int64_t rc2 = static_cast<int64_t>(rc) * rc;
-    for (int64_t t = tid; t < rc2; t += kThreads) {
-      const int i = static_cast<int>(t % rc), j = static_cast<int>(t / rc);
-      if (i < j) continue;
-      F[rp[i] + static_cast<int64_t>(rp[j]) * ld] +=
-          Fc[(C.ns + i) + static_cast<int64_t>(C.ns + j) * ldc];
-    }
-    __syncthreads();
-  }
-  constexpr int LDP = kNB + 1;
-  for (int j0 = 0; j0 < ns; j0 += kNB) {
-    const int jb = min(kNB, ns - j0);
-    const int m = ld - j0;  // panel rows: J, R and E rows from j0 on
-    for (int idx = tid; idx < m * jb; idx += kThreads) {
-      const int r = idx % m, t = idx / m;
-      pan[r * LDP + t] = F[(j0 + r) + static_cast<int64_t>(j0 + t) * ld];
-    }
-    // one barrier per column: each thread owns rows tid, tid + 256, ...; the
-    // scaled column t is kept in registers and stored at the next step, so
-    // column t is only read during step t
-    T keep0 = T(0), keep1 = T(0);
-    for (int t = 0; t < jb; ++t) {
-      __syncthreads();
-      if (t > 0) {
-        if (tid >= t - 1 && tid < m) pan[tid * LDP + t - 1] = keep0;
-        if (tid + kThreads < m) pan[(tid + kThreads) * LDP + t - 1] = keep1;
-      }
-      T d = pan[t * LDP + t];
-      if (!(d > T(0))) {
-        if (tid == 0) atomicMax(info, 1);
-        d = T(1);
-      }
-      const T sq = sqrt(d);
-      const T dinv = T(1) / sq;
-#pragma unroll
-      for (int q = 0; q < 2; ++q) {
-        const int r = tid + q * kThreads;
-        if (r < t || r >= m) continue;
-        T vr;
-        if (r == t) {
-          vr = sq;
-        } else {
-          vr = pan[r * LDP + t] * dinv;
-          const int lmax = min(r, jb - 1);
-          for (int l = t + 1; l <= lmax; ++l) pan[r * LDP + l] -= vr * (pan[l * LDP + t] * dinv);
-        }
-        if (q == 0) keep0 = vr; else keep1 = vr;
-      }
-    }
-    __syncthreads();
-    if (tid >= jb - 1 && tid < m) pan[tid * LDP + jb - 1] = keep0;
-    if (tid + kThreads < m) pan[(tid + kThreads) * LDP + jb - 1] = keep1;
-    __syncthreads();
-    for (int idx = tid; idx < m * jb; idx += kThreads) {
-      const int r = idx % m, t = idx / m;
-      if (r >= t) F[(j0 + r) + static_cast<int64_t>(j0 + t) * ld] = pan[r * LDP + t];
-    }
-    // trailing update: (a) lower triangle of rows/cols [j0+jb, nf);
-    // (b) E rows x all remaining columns [j0+jb, nf): the J part completes
-    // L_JJ^{-T}, the R part accumulates -L_EJ L_RJ^T = -Q_R^T
-    const int mt = nf - j0 - jb;
-    const int nt = (mt + 3) >> 2;
-    const int ntri = nt * (nt + 1) / 2;
-    const int ncj = nf - j0 - jb;
-    const int ntej = ((ns + 3) >> 2) * ((ncj + 3) >> 2);
-    for (int tt = tid; tt < ntri + ntej; tt += kThreads) {
-      int ri0, ci0, rmax, cmax;  // panel-local row / column origins and bounds
-      bool tri;
-      if (tt < ntri) {  // lower-triangle tile (ti >= tj), column-major walk
-        int tj = 0, rem = tt;
-        while (rem >= nt - tj) { rem -= nt - tj; ++tj; }
-        const int ti = tj + rem;
-        ri0 = jb + ti * 4;
-        ci0 = jb + tj * 4;
-        rmax = jb + mt;
-        cmax = jb + mt;
-        tri = true;
-      } else {
-        const int u = tt - ntri, ncb = (ncj + 3) >> 2;
-        ri0 = (nf - j0) + (u / ncb) * 4;
-        ci0 = jb + (u % ncb) * 4;
-        rmax = m;
-        cmax = jb + ncj;
-        tri = false;
-      }
-      T acc[4][4];
-#pragma unroll
-      for (int q = 0; q < 4; ++q)
-#pragma unroll
-        for (int p = 0; p < 4; ++p) acc[q][p] = T(0);
-      for (int t = 0; t < jb; ++t) {
-        T av[4], bv[4];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          av[q] = ri0 + q < rmax ? pan[(ri0 + q) * LDP + t] : T(0);
-          bv[q] = ci0 + q < cmax ? pan[(ci0 + q) * LDP + t] : T(0);
-        }
-#pragma unroll
-        for (int q = 0; q < 4; ++q)
-#pragma unroll
-          for (int p = 0; p < 4; ++p) acc[q][p] += av[q] * bv[p];
-      }
-#pragma unroll
-      for (int q = 0; q < 4; ++q)
-#pragma unroll
-        for (int p = 0; p < 4; ++p) {
-          const int i = ri0 + q, l = ci0 + p;
-          if (i < rmax && l < cmax && (!tri || i >= l))
-            F[(j0 + i) + static_cast<int64_t>(j0 + l) * ld] -= acc[q][p];
-        }
-    }
-    __syncthreads();
-  }
-  // panel for the sweeps: rows [0, ns) = L_JJ^{-1}, rows [ns, nf) = Q_R,
-  // both read from the E rows
-  // also stored transposed (PT = P^T, ns x nf column-major) for the backward
-  // sweep, whose operand runs along the rows of P
-  T* P = panels + b * panel_bs + S.panel_off;
-  T* PT = panelsT + b * panel_bs + S.panel_off;
-  for (int64_t idx = tid; idx < static_cast<int64_t>(nf) * ns; idx += kThreads) {
-    const int r = static_cast<int>(idx % nf), c = static_cast<int>(idx / nf);
-    T val;
-    if (r < ns)
-      val = r >= c ? F[(nf + c) + static_cast<int64_t>(r) * ld] : T(0);
-    else
-      val = -F[(nf + c) + static_cast<int64_t>(r) * ld];
-    P[r + static_cast<int64_t>(c) * nf] = val;
-  }
-  for (int64_t idx = tid; idx < static_cast<int64_t>(nf) * ns; idx += kThreads) {
-    const int c = static_cast<int>(idx % ns), r = static_cast<int>(idx / ns);
-    const T v = F[(nf + c) + static_cast<int64_t>(r) * ld];
-    PT[c + static_cast<int64_t>(r) * ns] = r < ns ? (r >= c ? v : T(0)) : -v;
-  }
-}
-
-// ---------------------------------------------------------------------------
-// multi-RHS sweeps on MFMA.  One wavefront computes 16x16 output tiles with
-// the 16x16x4 matrix instruction of the element type:
+// 16x16x4 matrix instruction of the element type, one wavefront:
 //   A operand: lane l holds A[r0 + l%16][k0 + l/16]
 //   B operand: lane l holds B[k0 + l/16][c0 + l%16]
 //   result   : lane l, register i holds C[r0 + Mma<T>::row(l, i)][c0 + l%16]
@@ -268,6 +88,200 @@ struct Mma<float> {
   }
   __device__ static inline int row(int lane, int i) { return 4 * (lane >> 4) + i; }
 };
+
+// ---------------------------------------------------------------------------
+// numeric factorisation, one launch per depth level (deepest first)
+// grid (supernodes at level, batch), NT threads.
+// The front F (column-major, leading dimension nf + ns) holds the lower
+// triangle of [J; R] in its first nf rows and, below them, ns extra rows E
+// that start as the identity in the J columns.  Eliminating the ns J columns
+// leaves L_JJ, L_RJ, the Schur complement in the (R,R) block -- this front's
+// update block -- and L_EJ = L_JJ^{-T} in the E rows, i.e. the inverse the
+// sweeps need, without a separate inversion.
+// Per panel of kNB columns: every thread holds RPT panel rows in registers
+// and eliminates the columns one after the other (one barrier per column:
+// the pivot rows' column is broadcast through LDS); the panel then goes to
+// LDS and the trailing update -- lower triangle of the J/R rows and the E
+// rows times the J/R columns -- runs as 16x16 tiles on the matrix unit, one
+// tile per wavefront at a time.
+// The panel written for the sweeps is P_s = [ L_JJ^{-1} ; L_RJ L_JJ^{-1} ]
+// (nf x ns) and its transpose.
+// ---------------------------------------------------------------------------
+constexpr int kNB = 32;        // panel width
+constexpr int kLDP = kNB + 4;  // LDS row stride of the panel
+
+template <typename T, int NT, int RPT>
+__global__ __launch_bounds__(NT) void k_factor_level(
+    const SupDev* __restrict__ sup, const int* __restrict__ lvl,
+    const int* __restrict__ relpos, const int* __restrict__ cidx,
+    const int* __restrict__ a_e, const int* __restrict__ a_row,
+    const int* __restrict__ a_col, const double* __restrict__ vals, int64_t nnz,
+    T* __restrict__ fcur, int64_t fcur_bs, const T* __restrict__ fprev,
+    int64_t fprev_bs, T* __restrict__ panels, T* __restrict__ panelsT, int64_t panel_bs,
+    int* __restrict__ info) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int s = lvl[blockIdx.x];
+  const int b = blockIdx.y;
+  const SupDev S = sup[s];
+  const int ns = S.ns, rs = S.rs, nf = S.ns + S.rs, ld = nf + S.ns;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid / 64;
+  constexpr int NW = NT / 64;
+  T* colb = reinterpret_cast<T*>(smem_raw);  // 2 x kNB pivot-column broadcast
+  T* pan = colb + 2 * kNB;                    // panel rows, stride kLDP
+  T* F = fcur + b * fcur_bs + S.front_off;
+  const int64_t fsz = static_cast<int64_t>(ld) * nf;
+  for (int64_t i = tid; i < fsz; i += NT) F[i] = T(0);
+  if (tid < 2 * kNB) colb[tid] = T(0);
+  __syncthreads();
+  for (int j = tid; j < ns; j += NT) F[(nf + j) + static_cast<int64_t>(j) * ld] = T(1);
+  const double* v = vals + b * nnz;
+  for (int q = S.a_begin + tid; q < S.a_end; q += NT)
+    F[a_row[q] + static_cast<int64_t>(a_col[q]) * ld] = static_cast<T>(v[a_e[q]]);
+  __syncthreads();
+  for (int cc = S.c_begin; cc < S.c_end; ++cc) {
+    const SupDev C = sup[cidx[cc]];
+    const int rc = C.rs, ldc = C.ns + C.rs + C.ns;
+    const T* Fc = fprev + b * fprev_bs + C.front_off;
+    const int* rp = relpos + C.rptr;
+    const int64_t rc2 = static_cast<int64_t>(rc) * rc;
+    for (int64_t t = tid; t < rc2; t += NT) {
+      const int i = static_cast<int>(t % rc), j = static_cast<int>(t / rc);
+      if (i < j) continue;
+      F[rp[i] + static_cast<int64_t>(rp[j]) * ld] +=
+          Fc[(C.ns + i) + static_cast<int64_t>(C.ns + j) * ldc];
+    }
+    __syncthreads();
+  }
+  for (int j0 = 0; j0 < ns; j0 += kNB) {
+    const int jb = min(kNB, ns - j0);
+    const int m = ld - j0;     // panel rows: J, R and E rows from j0 on
+    const int n1 = nf - j0;    // end of the J/R rows (panel-local)
+    T rv[RPT][kNB];
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) {
+      const int r = tid + q * NT;
+#pragma unroll
+      for (int c = 0; c < kNB; ++c)
+        rv[q][c] = (r < m && c < jb) ? F[(j0 + r) + static_cast<int64_t>(j0 + c) * ld] : T(0);
+    }
+    // column t: rows t..jb-1 (threads t..jb-1, first slot) publish their
+    // entry, every row scales its own entry and updates its later columns
+#pragma unroll
+    for (int t = 0; t < kNB; ++t) {
+      if (t < jb) {  // uniform
+        T* cb = colb + (t & 1) * kNB;
+        if (tid >= t && tid < jb) cb[tid] = rv[0][t];
+        __syncthreads();
+        T d = cb[t];
+        if (!(d > T(0))) {
+          if (tid == 0) atomicMax(info, 1);
+          d = T(1);
+        }
+        const T sq = sqrt(d);
+        const T dinv = T(1) / sq;
+#pragma unroll
+        for (int q = 0; q < RPT; ++q) {
+          const int r = tid + q * NT;
+          if (r == t) {
+            rv[q][t] = sq;
+          } else if (r > t) {
+            const T lr = rv[q][t] * dinv;
+            rv[q][t] = lr;
+#pragma unroll
+            for (int l = t + 1; l < kNB; ++l) rv[q][l] -= lr * (cb[l] * dinv);
+          }
+        }
+      }
+    }
+    // panel to F (its lower part) and to LDS (zero columns past jb; the 16
+    // rows past m are zeroed for the tile reads)
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) {
+      const int r = tid + q * NT;
+      if (r < m) {
+#pragma unroll
+        for (int c = 0; c < kNB; ++c) {
+          if (c < jb && r >= c) F[(j0 + r) + static_cast<int64_t>(j0 + c) * ld] = rv[q][c];
+          pan[r * kLDP + c] = c < jb ? rv[q][c] : T(0);
+        }
+      }
+    }
+    for (int idx = tid; idx < 16 * kLDP; idx += NT) pan[m * kLDP + idx] = T(0);
+    __syncthreads();
+    // trailing update, 16x16 tiles, computed transposed so that a lane's
+    // output runs down a column of F:
+    //   (a) rows [jb, n1) x cols [jb, n1), lower triangle
+    //   (b) E rows [n1, m) x cols [jb, n1)
+    const int w = n1 - jb;
+    const int nt = (w + 15) >> 4;
+    const int ntri = nt * (nt + 1) / 2;
+    const int nte = ((m - n1 + 15) >> 4) * nt;
+    for (int tt = wave; tt < ntri + nte; tt += NW) {
+      int i0, l0, imax;
+      bool diag;
+      if (tt < ntri) {
+        int tj = 0, rem = tt;
+        while (rem >= nt - tj) { rem -= nt - tj; ++tj; }
+        i0 = jb + (tj + rem) * 16;
+        l0 = jb + tj * 16;
+        imax = n1;
+        diag = rem == 0;
+      } else {
+        const int u = tt - ntri;
+        i0 = n1 + (u / nt) * 16;
+        l0 = jb + (u % nt) * 16;
+        imax = m;
+        diag = false;
+      }
+      const int i = i0 + (lane & 15);
+      T cold[4];
+      int64_t addr[4];
+      bool ok[4];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int l = l0 + Mma<T>::row(lane, e);
+        ok[e] = i < imax && l < n1 && (!diag || i >= l);
+        addr[e] = (j0 + i) + static_cast<int64_t>(j0 + l) * ld;
+        cold[e] = ok[e] ? F[addr[e]] : T(0);
+      }
+      typename Mma<T>::v4 acc = {T(0), T(0), T(0), T(0)};
+      const T* pa = pan + (l0 + (lane & 15)) * kLDP + (lane >> 4);
+      const T* pb = pan + (i0 + (lane & 15)) * kLDP + (lane >> 4);
+#pragma unroll
+      for (int k0 = 0; k0 < kNB; k0 += 4) acc = Mma<T>::op(pa[k0], pb[k0], acc);
+#pragma unroll
+      for (int e = 0; e < 4; ++e)
+        if (ok[e]) F[addr[e]] = cold[e] - acc[e];
+    }
+    __syncthreads();
+  }
+  // panel for the sweeps: rows [0, ns) = L_JJ^{-1}, rows [ns, nf) = Q_R,
+  // both read from the E rows; also stored transposed (PT = P^T, ns x nf
+  // column-major) for the backward sweep, whose operand runs along the rows
+  // of P
+  T* P = panels + b * panel_bs + S.panel_off;
+  T* PT = panelsT + b * panel_bs + S.panel_off;
+  for (int64_t idx = tid; idx < static_cast<int64_t>(nf) * ns; idx += NT) {
+    const int r = static_cast<int>(idx % nf), c = static_cast<int>(idx / nf);
+    T val;
+    if (r < ns)
+      val = r >= c ? F[(nf + c) + static_cast<int64_t>(r) * ld] : T(0);
+    else
+      val = -F[(nf + c) + static_cast<int64_t>(r) * ld];
+    P[r + static_cast<int64_t>(c) * nf] = val;
+  }
+  for (int64_t idx = tid; idx < static_cast<int64_t>(nf) * ns; idx += NT) {
+    const int c = static_cast<int>(idx % ns), r = static_cast<int>(idx / ns);
+    const T v = F[(nf + c) + static_cast<int64_t>(r) * ld];
+    PT[c + static_cast<int64_t>(r) * ns] = r < ns ? (r >= c ? v : T(0)) : -v;
+  }
+  (void)rs;
+}
+
+// ---------------------------------------------------------------------------
+// multi-RHS sweeps on MFMA (Mma<T> above), one wavefront per 16x16 output
+// tile at a time.
+// ---------------------------------------------------------------------------
 
 constexpr int kMaxTiles = 8;  // 16x16 register tiles a wave keeps across a barrier
 
@@ -1025,12 +1039,23 @@ int upload_fused(dcx_plan* P) {
 constexpr size_t kLdsBudget = 64 * 1024;
 constexpr size_t kFusedLdsBudget = 80 * 1024;  // two workgroups per CU
 
+template <typename T, int NT, int RPT>
+void factor_launch(dcx_plan* P, int d, int cnt, size_t lds, hipStream_t st) {
+  const auto& S = P->S;
+  T* fr[2] = {static_cast<T*>(P->d_front[0]), static_cast<T*>(P->d_front[1])};
+  const int64_t fbs[2] = {S.front_size[0], S.front_size[1]};
+  dim3 grid(cnt, P->nbatch);
+  hipLaunchKernelGGL((k_factor_level<T, NT, RPT>), grid, dim3(NT), lds, st, P->d_sup,
+                     P->d_lvl + S.lvl_ptr[d], P->d_relpos, P->d_cidx, P->d_a_e, P->d_a_row,
+                     P->d_a_col, P->d_vals, static_cast<int64_t>(S.nnz), fr[d & 1], fbs[d & 1],
+                     fr[(d + 1) & 1], fbs[(d + 1) & 1], static_cast<T*>(P->d_panels),
+                     static_cast<T*>(P->d_panelsT), S.panel_size, P->d_info);
+}
+
+// one launch per level; threads x rows per thread cover the tallest panel
 template <typename T>
 int launch_factor(dcx_plan* P, hipStream_t st) {
   const auto& S = P->S;
-  T* panels = static_cast<T*>(P->d_panels);
-  T* fr[2] = {static_cast<T*>(P->d_front[0]), static_cast<T*>(P->d_front[1])};
-  const int64_t fbs[2] = {S.front_size[0], S.front_size[1]};
   for (int d = S.ndepth - 1; d >= 0; --d) {
     const int cnt = S.lvl_ptr[d + 1] - S.lvl_ptr[d];
     if (!cnt) continue;
@@ -1039,14 +1064,15 @@ int launch_factor(dcx_plan* P, hipStream_t st) {
       const int s = S.lvl_sup[q];
       maxld = std::max(maxld, 2 * S.ns[s] + S.rs[s]);
     }
-    const size_t lds = static_cast<size_t>(maxld) * (kNB + 1) * sizeof(T);
-    if (lds > 160 * 1024) return fail(DCX_ERR_ARG, "front too large for the factor kernel");
-    dim3 grid(cnt, P->nbatch);
-    hipLaunchKernelGGL(k_factor_level<T>, grid, dim3(kThreads), lds, st, P->d_sup,
-                       P->d_lvl + S.lvl_ptr[d], P->d_relpos, P->d_cidx, P->d_a_e,
-                       P->d_a_row, P->d_a_col, P->d_vals, static_cast<int64_t>(S.nnz),
-                       fr[d & 1], fbs[d & 1], fr[(d + 1) & 1], fbs[(d + 1) & 1], panels,
-                       static_cast<T*>(P->d_panelsT), S.panel_size, P->d_info);
+    const size_t lds = (2 * kNB + static_cast<size_t>(maxld + 16) * kLDP) * sizeof(T);
+    if (lds > 160 * 1024 || maxld > 1024)
+      return fail(DCX_ERR_ARG, "front too large for the factor kernel");
+    if (maxld <= 256)
+      factor_launch<T, 256, 1>(P, d, cnt, lds, st);
+    else if (maxld <= 512)
+      factor_launch<T, 512, 1>(P, d, cnt, lds, st);
+    else
+      factor_launch<T, 512, 2>(P, d, cnt, lds, st);
   }
   HIP_TRY(hipGetLastError());
   return DCX_OK;
