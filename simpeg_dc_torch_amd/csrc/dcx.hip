@@ -18,6 +18,7 @@
 #include <memory>
 #include <string>
 #include <tuple>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/dcx.h"
@@ -487,6 +488,170 @@ __global__ __launch_bounds__(kThreads) void k_bwd_level(
   }
 }
 
+// backward sweep, wavefront tiles read straight from W into the matrix
+// operands (no LDS staging, no barrier except for split-K):
+//   x_J = P_s^T g,   g = [y_J; -x_R]  (rows of W: J rows, then R rows)
+// A wavefront owns CW = 16 V columns: lane l holds the V adjacent columns
+// c0 + V (l%16) + v of every row it touches, loaded and stored as one vector,
+// and the column set {c0 + V j + v : j < 16} is the v-th 16-column matrix
+// tile.  KS = 1: the four wavefronts of a workgroup take four adjacent column
+// blocks; KS = 4: they split the K range (the front rows) of one column block
+// and add their partial tiles through LDS (levels with few supernodes).
+// MT: 16-row output tiles (ns <= 16 MT); PF: k-steps whose loads are in
+// flight together.  Grid as sweep_block, nzb = column groups per workgroup.
+template <typename T, int V>
+struct Vec {
+  typedef T v __attribute__((ext_vector_type(V)));
+};
+template <typename T>
+struct Vec<T, 1> {
+  typedef T v;
+};
+
+template <typename T, int V>
+__device__ inline void load_cols(const T* p, int navail, T (&out)[V]) {
+  if (navail >= V) {
+    typename Vec<T, V>::v t;
+    __builtin_memcpy(&t, p, sizeof(t));
+    if constexpr (V == 1) {
+      out[0] = t;
+    } else {
+#pragma unroll
+      for (int v = 0; v < V; ++v) out[v] = t[v];
+    }
+  } else {
+#pragma unroll
+    for (int v = 0; v < V; ++v) out[v] = v < navail ? p[v] : T(0);
+  }
+}
+
+template <typename T, int MT, int V, int KS, int PF>
+__global__ __launch_bounds__(kThreads) void k_bwd_wave(
+    const SupDev* __restrict__ sup, const int* __restrict__ lvl,
+    const int* __restrict__ ridx, const int* __restrict__ active,
+    const T* __restrict__ panelsT, int64_t panel_bs, T* __restrict__ W, int64_t w_bs, int k,
+    double* __restrict__ X64, const int* __restrict__ xrow, int xmode, int cnt, int nzb,
+    int total) {
+  constexpr int CW = 16 * V;
+  const SweepBlock sb = sweep_block(lvl, cnt, nzb, total);
+  if (!sb.live) return;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int col0 = (KS == 1 ? sb.z * 4 + wave : sb.z) * CW;
+  if (col0 >= k) return;  // uniform over the workgroup when KS > 1
+  const SupDev S = sup[sb.s];
+  const int ns = S.ns, nf = S.ns + S.rs;
+  const bool act = !active || active[sb.s];
+  const T* PT = panelsT + sb.b * panel_bs + S.panel_off;
+  T* Wb = W + sb.b * w_bs;
+  const int cl = col0 + V * (lane & 15);  // first column of this lane
+  const int navail = k - cl;               // columns of the lane inside the block
+  const int kq = lane >> 4;
+  // K range of this wavefront
+  int kbeg = 0, kend = nf;
+  if (KS > 1) {
+    const int chunk = ((nf + 4 * KS - 1) / (4 * KS)) * 4;
+    kbeg = min(nf, wave * chunk);
+    kend = min(nf, kbeg + chunk);
+  }
+  typename Mma<T>::v4 acc[MT][V];
+#pragma unroll
+  for (int t = 0; t < MT; ++t)
+#pragma unroll
+    for (int v = 0; v < V; ++v) acc[t][v] = {T(0), T(0), T(0), T(0)};
+  for (int k0 = kbeg; k0 < kend; k0 += 4 * PF) {
+    T bv[PF][V], av[PF][MT];
+#pragma unroll
+    for (int j = 0; j < PF; ++j) {
+      const int kk = k0 + 4 * j + kq;
+      const bool kok = kk < kend;
+#pragma unroll
+      for (int v = 0; v < V; ++v) bv[j][v] = T(0);
+      if (kok && navail > 0) {
+        if (kk < ns) {
+          if (act) load_cols<T, V>(Wb + static_cast<int64_t>(S.first + kk) * k + cl, navail, bv[j]);
+        } else {
+          load_cols<T, V>(Wb + static_cast<int64_t>(ridx[S.rptr + kk - ns]) * k + cl, navail,
+                          bv[j]);
+#pragma unroll
+          for (int v = 0; v < V; ++v) bv[j][v] = -bv[j][v];
+        }
+      }
+#pragma unroll
+      for (int t = 0; t < MT; ++t) {
+        const int r = t * 16 + (lane & 15);
+        av[j][t] = (kok && r < ns) ? PT[r + static_cast<int64_t>(kk) * ns] : T(0);
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < PF; ++j)
+      if (k0 + 4 * j < kend) {
+#pragma unroll
+        for (int t = 0; t < MT; ++t)
+#pragma unroll
+          for (int v = 0; v < V; ++v) acc[t][v] = Mma<T>::op(av[j][t], bv[j][v], acc[t][v]);
+      }
+  }
+  if (KS > 1) {
+    // partial tiles through LDS: wave w adds up and stores the tiles t with
+    // t % KS == w
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T* red = reinterpret_cast<T*>(smem_raw);  // [KS][MT][V][4][64]
+#pragma unroll
+    for (int t = 0; t < MT; ++t)
+#pragma unroll
+      for (int v = 0; v < V; ++v)
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+          red[(((wave * MT + t) * V + v) * 4 + i) * 64 + lane] = acc[t][v][i];
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < MT; ++t) {
+      if (t % KS != wave) continue;
+#pragma unroll
+      for (int v = 0; v < V; ++v)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          T a = T(0);
+#pragma unroll
+          for (int w = 0; w < KS; ++w) a += red[(((w * MT + t) * V + v) * 4 + i) * 64 + lane];
+          acc[t][v][i] = a;
+        }
+    }
+  }
+  if (navail <= 0) return;
+#pragma unroll
+  for (int t = 0; t < MT; ++t) {
+    if (KS > 1 && t % KS != wave) continue;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int r = t * 16 + Mma<T>::row(lane, i);
+      if (r >= ns) continue;
+      T* dst = Wb + static_cast<int64_t>(S.first + r) * k + cl;
+      if (navail >= V) {
+        typename Vec<T, V>::v o;
+        if constexpr (V == 1) {
+          o = acc[t][0][i];
+        } else {
+#pragma unroll
+          for (int v = 0; v < V; ++v) o[v] = acc[t][v][i];
+        }
+        __builtin_memcpy(dst, &o, sizeof(o));
+      } else {
+#pragma unroll
+        for (int v = 0; v < V; ++v)
+          if (v < navail) dst[v] = acc[t][v][i];
+      }
+      if (xmode) {
+        const int gr = xrow ? xrow[S.first + r] : S.first + r;
+        double* xp = X64 + sb.b * w_bs + static_cast<int64_t>(gr) * k + cl;
+#pragma unroll
+        for (int v = 0; v < V; ++v)
+          if (v < navail) xp[v] = (xmode == 2 ? xp[v] : 0.0) + static_cast<double>(acc[t][v][i]);
+      }
+    }
+  }
+}
+
 struct LRow {
   const int* l;
   __device__ inline int operator()(int k) const { return l[k]; }
@@ -766,14 +931,16 @@ __global__ void k_residual(const int* __restrict__ crow, const int* __restrict__
 // deals workgroups round-robin to the 8 XCDs; the block index is remapped so
 // that each XCD walks one contiguous range of rows (its own L2).  gridDim.x
 // must be a multiple of 8.  iperm (may be null) maps rows to the storage
-// order of Y and X.
-template <typename TI>
+// order of Y and X.  Up to MAXE entries of a row at a time; each lane loads
+// 4 columns of the Y row and of every X row before it multiplies, so that
+// 4 (1 + MAXE) loads per lane are in flight; FP64 sums.
+template <typename TI, int MAXE>
 __global__ void k_grad_values(const int* __restrict__ crow, const int* __restrict__ ccol,
                               const int* __restrict__ iperm, const TI* __restrict__ Y,
                               const TI* __restrict__ X, int n, int k,
                               const double* __restrict__ scale, double* __restrict__ gv,
                               int64_t nnz) {
-  constexpr int MAXE = 8;
+  constexpr int U = 4;
   const int per = gridDim.x >> 3;
   const int lb = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
   const int row = lb * (blockDim.x / 64) + threadIdx.x / 64;
@@ -789,17 +956,31 @@ __global__ void k_grad_values(const int* __restrict__ crow, const int* __restric
     const TI* xr[MAXE];
 #pragma unroll
     for (int q = 0; q < MAXE; ++q) {
-      const int cq = q < ne ? ccol[p0 + q] : ccol[p0];
+      const int cq = ccol[p0 + min(q, ne - 1)];  // surplus slots repeat the last entry
       xr[q] = X + xb + static_cast<int64_t>(iperm ? iperm[cq] : cq) * k;
     }
     double acc[MAXE];
 #pragma unroll
     for (int q = 0; q < MAXE; ++q) acc[q] = 0.0;
-    for (int c = lane; c < k; c += 64) {
-      const double yv = static_cast<double>(y[c]);
+    int c = lane;
+    for (; c + 64 * (U - 1) < k; c += 64 * U) {
+      TI yv[U], xv[MAXE][U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) yv[u] = y[c + 64 * u];
 #pragma unroll
       for (int q = 0; q < MAXE; ++q)
-        if (q < ne) acc[q] += yv * static_cast<double>(xr[q][c]);
+#pragma unroll
+        for (int u = 0; u < U; ++u) xv[q][u] = xr[q][c + 64 * u];
+#pragma unroll
+      for (int q = 0; q < MAXE; ++q)
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+          acc[q] += static_cast<double>(yv[u]) * static_cast<double>(xv[q][u]);
+    }
+    for (; c < k; c += 64) {
+      const double yv = static_cast<double>(y[c]);
+#pragma unroll
+      for (int q = 0; q < MAXE; ++q) acc[q] += yv * static_cast<double>(xr[q][c]);
     }
 #pragma unroll
     for (int q = 0; q < MAXE; ++q) {
@@ -965,6 +1146,7 @@ struct dcx_plan {
   double* d_x = nullptr;     // permuted FP64 iterate (mixed)
   size_t r_bytes = 0, b_bytes = 0, x_bytes = 0;
   int64_t device_bytes = 0;
+  int max_row_nnz = 0;  // longest row of the caller's pattern
   // optional event timing of the sweeps (forward, backward) of every solve
   bool timing = false;
   std::vector<hipEvent_t> ev;  // triples per sweep pair: start, mid, end
@@ -1125,9 +1307,54 @@ int env_int(const char* name, int dflt) {
   return e ? std::atoi(e) : dflt;
 }
 
+template <typename T, int MT, int V, int KS>
+void bwd_wave_launch(dcx_plan* P, T* W, int k, const int* lvl, int cnt, const int* active,
+                     const SweepIO& io, hipStream_t st) {
+  constexpr int CW = 16 * V;
+  const int ncb = (k + CW - 1) / CW;              // column blocks
+  const int nzb = KS == 1 ? (ncb + 3) / 4 : ncb;  // workgroups per supernode and batch member
+  const int total = cnt * P->nbatch * nzb;
+  dim3 grid(((total + 7) / 8) * 8);
+  const size_t lds = KS > 1 ? static_cast<size_t>(KS) * MT * V * 4 * 64 * sizeof(T) : 0;
+  const int64_t w_bs = static_cast<int64_t>(P->S.n) * k;
+  hipLaunchKernelGGL((k_bwd_wave<T, MT, V, KS, 4>), grid, dim3(kThreads), lds, st, P->d_sup,
+                     lvl, P->d_ridx, active, static_cast<const T*>(P->d_panelsT),
+                     P->S.panel_size, W, w_bs, k, io.X64, io.xrow, io.xmode, cnt, nzb, total);
+}
+
+template <typename T, int V, int KS>
+bool bwd_wave_mt(dcx_plan* P, T* W, int k, int maxns, const int* lvl, int cnt,
+                 const int* active, const SweepIO& io, hipStream_t st) {
+  const int mt = (maxns + 15) / 16;
+  if (mt <= 1) bwd_wave_launch<T, 1, V, KS>(P, W, k, lvl, cnt, active, io, st);
+  else if (mt <= 2) bwd_wave_launch<T, 2, V, KS>(P, W, k, lvl, cnt, active, io, st);
+  else if (mt <= 3) bwd_wave_launch<T, 3, V, KS>(P, W, k, lvl, cnt, active, io, st);
+  else if (mt <= 4) bwd_wave_launch<T, 4, V, KS>(P, W, k, lvl, cnt, active, io, st);
+  else if (mt <= 6) bwd_wave_launch<T, 6, V, KS>(P, W, k, lvl, cnt, active, io, st);
+  else if (mt <= 8) bwd_wave_launch<T, 8, V, KS>(P, W, k, lvl, cnt, active, io, st);
+  else return false;
+  return true;
+}
+
+// backward level through the wavefront kernel when its output tiles fit:
+// wide vector columns where the level has enough wavefronts to fill the
+// chip, split-K otherwise
+template <typename T>
+bool bwd_wave(dcx_plan* P, T* W, int k, int maxns, const int* lvl, int cnt, const int* active,
+              const SweepIO& io, hipStream_t st) {
+  static const int use = env_int("DCX_BWD_WAVE", 1);
+  static const int min_waves = env_int("DCX_BWD_SPLITK_BELOW", 16384);
+  if (!use) return false;
+  constexpr int VW = 16 / sizeof(T);  // 16-byte vectors
+  const int64_t waves = static_cast<int64_t>(cnt) * P->nbatch * ((k + 16 * VW - 1) / (16 * VW));
+  if (waves < min_waves) return bwd_wave_mt<T, 1, 4>(P, W, k, maxns, lvl, cnt, active, io, st);
+  return bwd_wave_mt<T, VW, 1>(P, W, k, maxns, lvl, cnt, active, io, st);
+}
+
 template <typename T, int TC>
 void sweep_tc(dcx_plan* P, T* W, int k, int maxnf, int maxns, int maxpanel, bool fwd, int d,
               const int* lvl, int cnt, const int* active, const SweepIO& io, hipStream_t st) {
+  if (!fwd && bwd_wave<T>(P, W, k, maxns, lvl, cnt, active, io, st)) return;
   static const int use_pl = env_int("DCX_PANEL_LDS", 0);
   static const int ct_max = env_int("DCX_CT_MAX", 1);
   static const int target = env_int("DCX_TARGET_BLOCKS", 8192);
@@ -1330,6 +1557,7 @@ int dcx_plan_create(int n, const int32_t* crow, const int32_t* col, int nbatch,
     HIP_TRY(upload(&P->d_pcol, S.pcol));
     HIP_TRY(upload(&P->d_pvidx, S.pval_idx));
     std::vector<int> vcrow(crow, crow + n + 1), vcol(col, col + crow[n]);
+    for (int i = 0; i < n; ++i) P->max_row_nnz = std::max(P->max_row_nnz, crow[i + 1] - crow[i]);
     HIP_TRY(upload(&P->d_crow, vcrow));
     HIP_TRY(upload(&P->d_col, vcol));
     HIP_TRY(hipMalloc(&P->d_info, sizeof(int)));
@@ -1593,14 +1821,22 @@ int dcx_grad_values(dcx_plan* P, const void* y, const void* x, int k,
   const int blocks = ((grid1(S.n, 4) + 7) / 8) * 8;  // XCD remap needs a multiple of 8
   dim3 grid(blocks, P->nbatch);
   const int* ip = (flags & DCX_PERMUTED) ? static_cast<const int*>(P->d_iperm) : nullptr;
-  if (flags & DCX_F32_FIELDS)
-    hipLaunchKernelGGL(k_grad_values<float>, grid, dim3(256), 0, st, P->d_crow, P->d_col, ip,
-                       static_cast<const float*>(y), static_cast<const float*>(x), S.n, k,
-                       scale, gv, static_cast<int64_t>(S.nnz));
-  else
-    hipLaunchKernelGGL(k_grad_values<double>, grid, dim3(256), 0, st, P->d_crow, P->d_col, ip,
-                       static_cast<const double*>(y), static_cast<const double*>(x), S.n, k,
-                       scale, gv, static_cast<int64_t>(S.nnz));
+  // entries per pass: the longest row up to 5 (2-D stencils), else 8
+  const bool wide = P->max_row_nnz > 5;
+  auto go = [&](auto tag, auto me) {
+    using TI = decltype(tag);
+    hipLaunchKernelGGL((k_grad_values<TI, decltype(me)::value>), grid, dim3(256), 0, st,
+                       P->d_crow, P->d_col, ip, static_cast<const TI*>(y),
+                       static_cast<const TI*>(x), S.n, k, scale, gv,
+                       static_cast<int64_t>(S.nnz));
+  };
+  using E5 = std::integral_constant<int, 5>;
+  using E8 = std::integral_constant<int, 8>;
+  if (flags & DCX_F32_FIELDS) {
+    if (wide) go(0.0f, E8()); else go(0.0f, E5());
+  } else {
+    if (wide) go(0.0, E8()); else go(0.0, E5());
+  }
   HIP_TRY(hipGetLastError());
   return DCX_OK;
 }
