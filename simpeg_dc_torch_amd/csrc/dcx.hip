@@ -128,7 +128,9 @@ __global__ __launch_bounds__(NT) void k_factor_level(
   const int tid = threadIdx.x, lane = tid & 63, wave = tid / 64;
   constexpr int NW = NT / 64;
   T* colb = reinterpret_cast<T*>(smem_raw);  // 2 x kNB pivot-column broadcast
-  T* pan = colb + 2 * kNB;                    // panel rows, stride kLDP
+  T* ldg = colb + 2 * kNB;                    // diagonal block rows, stride kNB + 1
+  T* dgi = ldg + kNB * (kNB + 1);             // reciprocal diagonal
+  T* pan = dgi + kNB;                         // panel rows, stride kLDP
   T* F = fcur + b * fcur_bs + S.front_off;
   const int64_t fsz = static_cast<int64_t>(ld) * nf;
   for (int64_t i = tid; i < fsz; i += NT) F[i] = T(0);
@@ -136,23 +138,56 @@ __global__ __launch_bounds__(NT) void k_factor_level(
   __syncthreads();
   for (int j = tid; j < ns; j += NT) F[(nf + j) + static_cast<int64_t>(j) * ld] = T(1);
   const double* v = vals + b * nnz;
-  for (int q = S.a_begin + tid; q < S.a_end; q += NT)
-    F[a_row[q] + static_cast<int64_t>(a_col[q]) * ld] = static_cast<T>(v[a_e[q]]);
-  __syncthreads();
+  // loops over global gathers take kU entries per thread at a time, loads
+  // first, so that their latencies overlap
+  constexpr int kU = 16;
+  for (int q0 = S.a_begin + tid; q0 < S.a_end; q0 += NT * kU) {
+    T av[kU];
+    int64_t ad[kU];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      const int q = q0 + u * NT;
+      ad[u] = -1;
+      if (q < S.a_end) {
+        ad[u] = a_row[q] + static_cast<int64_t>(a_col[q]) * ld;
+        av[u] = static_cast<T>(v[a_e[q]]);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kU; ++u)
+      if (ad[u] >= 0) F[ad[u]] = av[u];
+  }
+  // extend-add of the children's update blocks (lower triangles), their
+  // relative positions staged in LDS
+  int* rpl = reinterpret_cast<int*>(pan);
   for (int cc = S.c_begin; cc < S.c_end; ++cc) {
     const SupDev C = sup[cidx[cc]];
     const int rc = C.rs, ldc = C.ns + C.rs + C.ns;
     const T* Fc = fprev + b * fprev_bs + C.front_off;
-    const int* rp = relpos + C.rptr;
-    const int64_t rc2 = static_cast<int64_t>(rc) * rc;
-    for (int64_t t = tid; t < rc2; t += NT) {
-      const int i = static_cast<int>(t % rc), j = static_cast<int>(t / rc);
-      if (i < j) continue;
-      F[rp[i] + static_cast<int64_t>(rp[j]) * ld] +=
-          Fc[(C.ns + i) + static_cast<int64_t>(C.ns + j) * ldc];
-    }
+    __syncthreads();  // F writes before, rpl reads of the previous child
+    for (int i = tid; i < rc; i += NT) rpl[i] = relpos[C.rptr + i];
     __syncthreads();
+    const int64_t rc2 = static_cast<int64_t>(rc) * rc;
+    for (int64_t t0 = tid; t0 < rc2; t0 += NT * kU) {
+      T cv[kU], fv[kU];
+      int fa[kU];  // offsets inside the front (fronts hold < 2^31 entries)
+#pragma unroll
+      for (int u = 0; u < kU; ++u) {
+        const int64_t t = t0 + u * NT;
+        const int i = static_cast<int>(t % rc), j = static_cast<int>(t / rc);
+        fa[u] = -1;
+        if (t < rc2 && i >= j) {
+          fa[u] = rpl[i] + rpl[j] * ld;
+          cv[u] = Fc[(C.ns + i) + static_cast<int64_t>(C.ns + j) * ldc];
+          fv[u] = F[fa[u]];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < kU; ++u)
+        if (fa[u] >= 0) F[fa[u]] = fv[u] + cv[u];
+    }
   }
+  __syncthreads();
   for (int j0 = 0; j0 < ns; j0 += kNB) {
     const int jb = min(kNB, ns - j0);
     const int m = ld - j0;     // panel rows: J, R and E rows from j0 on
@@ -165,31 +200,56 @@ __global__ __launch_bounds__(NT) void k_factor_level(
       for (int c = 0; c < kNB; ++c)
         rv[q][c] = (r < m && c < jb) ? F[(j0 + r) + static_cast<int64_t>(j0 + c) * ld] : T(0);
     }
-    // column t: rows t..jb-1 (threads t..jb-1, first slot) publish their
-    // entry, every row scales its own entry and updates its later columns
+    // (1) the jb x jb diagonal block, by wavefront 0 alone (lanes < jb hold
+    // its rows; the pivot column is broadcast through LDS, ordered within
+    // the wavefront by fences): its L rows and reciprocal diagonal go to LDS
+    if (wave == 0) {
 #pragma unroll
-    for (int t = 0; t < kNB; ++t) {
-      if (t < jb) {  // uniform
-        T* cb = colb + (t & 1) * kNB;
-        if (tid >= t && tid < jb) cb[tid] = rv[0][t];
-        __syncthreads();
-        T d = cb[t];
-        if (!(d > T(0))) {
-          if (tid == 0) atomicMax(info, 1);
-          d = T(1);
+      for (int t = 0; t < kNB; ++t) {
+        if (t < jb) {  // uniform
+          if (lane >= t && lane < jb) colb[lane] = rv[0][t];
+          __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+          T d = colb[t];
+          if (!(d > T(0))) {
+            if (lane == 0) atomicMax(info, 1);
+            d = T(1);
+          }
+          const T sq = sqrt(d);
+          const T dinv = T(1) / sq;
+          if (lane == t) {
+            rv[0][t] = sq;
+            dgi[t] = dinv;
+          } else if (lane > t && lane < jb) {
+            const T lr = rv[0][t] * dinv;
+            rv[0][t] = lr;
+#pragma unroll
+            for (int l = t + 1; l < kNB; ++l) rv[0][l] -= lr * (colb[l] * dinv);
+          }
+          __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         }
-        const T sq = sqrt(d);
-        const T dinv = T(1) / sq;
+      }
+      if (lane < jb) {
 #pragma unroll
-        for (int q = 0; q < RPT; ++q) {
-          const int r = tid + q * NT;
-          if (r == t) {
-            rv[q][t] = sq;
-          } else if (r > t) {
-            const T lr = rv[q][t] * dinv;
-            rv[q][t] = lr;
+        for (int c = 0; c < kNB; ++c) ldg[lane * (kNB + 1) + c] = rv[0][c];
+      }
+    }
+    __syncthreads();
+    // (2) every row below the block: L_r = A_r L_diag^{-T}, a forward
+    // substitution on the thread's own registers with the block from LDS
 #pragma unroll
-            for (int l = t + 1; l < kNB; ++l) rv[q][l] -= lr * (cb[l] * dinv);
+    for (int q = 0; q < RPT; ++q) {
+      const int r = tid + q * NT;
+      if (r >= jb && r < m) {
+#pragma unroll
+        for (int t = 0; t < kNB; ++t) {
+          if (t < jb) {
+            T s0 = rv[q][t], s1 = T(0);
+#pragma unroll
+            for (int l = 0; l < t; l += 2) {
+              s0 -= rv[q][l] * ldg[t * (kNB + 1) + l];
+              if (l + 1 < t) s1 -= rv[q][l + 1] * ldg[t * (kNB + 1) + l + 1];
+            }
+            rv[q][t] = (s0 + s1) * dgi[t];
           }
         }
       }
@@ -217,42 +277,51 @@ __global__ __launch_bounds__(NT) void k_factor_level(
     const int nt = (w + 15) >> 4;
     const int ntri = nt * (nt + 1) / 2;
     const int nte = ((m - n1 + 15) >> 4) * nt;
-    for (int tt = wave; tt < ntri + nte; tt += NW) {
-      int i0, l0, imax;
-      bool diag;
-      if (tt < ntri) {
-        int tj = 0, rem = tt;
-        while (rem >= nt - tj) { rem -= nt - tj; ++tj; }
-        i0 = jb + (tj + rem) * 16;
-        l0 = jb + tj * 16;
-        imax = n1;
-        diag = rem == 0;
-      } else {
-        const int u = tt - ntri;
-        i0 = n1 + (u / nt) * 16;
-        l0 = jb + (u % nt) * 16;
-        imax = m;
-        diag = false;
+    constexpr int TB = 2;  // tiles per wavefront whose loads are in flight together
+    for (int tt0 = wave; tt0 < ntri + nte; tt0 += NW * TB) {
+      T cold[TB][4];
+      int64_t addr[TB][4];
+      bool ok[TB][4];
+      int i0[TB], l0[TB];
+#pragma unroll
+      for (int u = 0; u < TB; ++u) {
+        const int tt = tt0 + u * NW;
+        int imax = 0;
+        bool diag = false;
+        i0[u] = l0[u] = jb;
+        if (tt < ntri) {
+          int tj = 0, rem = tt;
+          while (rem >= nt - tj) { rem -= nt - tj; ++tj; }
+          i0[u] = jb + (tj + rem) * 16;
+          l0[u] = jb + tj * 16;
+          imax = n1;
+          diag = rem == 0;
+        } else if (tt < ntri + nte) {
+          const int w2 = tt - ntri;
+          i0[u] = n1 + (w2 / nt) * 16;
+          l0[u] = jb + (w2 % nt) * 16;
+          imax = m;
+        }  // else: no tile, imax = 0 masks every entry
+        const int i = i0[u] + (lane & 15);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int l = l0[u] + Mma<T>::row(lane, e);
+          ok[u][e] = i < imax && l < n1 && (!diag || i >= l);
+          addr[u][e] = (j0 + i) + static_cast<int64_t>(j0 + l) * ld;
+          cold[u][e] = ok[u][e] ? F[addr[u][e]] : T(0);
+        }
       }
-      const int i = i0 + (lane & 15);
-      T cold[4];
-      int64_t addr[4];
-      bool ok[4];
 #pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const int l = l0 + Mma<T>::row(lane, e);
-        ok[e] = i < imax && l < n1 && (!diag || i >= l);
-        addr[e] = (j0 + i) + static_cast<int64_t>(j0 + l) * ld;
-        cold[e] = ok[e] ? F[addr[e]] : T(0);
+      for (int u = 0; u < TB; ++u) {
+        typename Mma<T>::v4 acc = {T(0), T(0), T(0), T(0)};
+        const T* pa = pan + (l0[u] + (lane & 15)) * kLDP + (lane >> 4);
+        const T* pb = pan + (i0[u] + (lane & 15)) * kLDP + (lane >> 4);
+#pragma unroll
+        for (int k0 = 0; k0 < kNB; k0 += 4) acc = Mma<T>::op(pa[k0], pb[k0], acc);
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+          if (ok[u][e]) F[addr[u][e]] = cold[u][e] - acc[e];
       }
-      typename Mma<T>::v4 acc = {T(0), T(0), T(0), T(0)};
-      const T* pa = pan + (l0 + (lane & 15)) * kLDP + (lane >> 4);
-      const T* pb = pan + (i0 + (lane & 15)) * kLDP + (lane >> 4);
-#pragma unroll
-      for (int k0 = 0; k0 < kNB; k0 += 4) acc = Mma<T>::op(pa[k0], pb[k0], acc);
-#pragma unroll
-      for (int e = 0; e < 4; ++e)
-        if (ok[e]) F[addr[e]] = cold[e] - acc[e];
     }
     __syncthreads();
   }
@@ -262,19 +331,26 @@ __global__ __launch_bounds__(NT) void k_factor_level(
   // of P
   T* P = panels + b * panel_bs + S.panel_off;
   T* PT = panelsT + b * panel_bs + S.panel_off;
-  for (int64_t idx = tid; idx < static_cast<int64_t>(nf) * ns; idx += NT) {
-    const int r = static_cast<int>(idx % nf), c = static_cast<int>(idx / nf);
-    T val;
-    if (r < ns)
-      val = r >= c ? F[(nf + c) + static_cast<int64_t>(r) * ld] : T(0);
-    else
-      val = -F[(nf + c) + static_cast<int64_t>(r) * ld];
-    P[r + static_cast<int64_t>(c) * nf] = val;
-  }
-  for (int64_t idx = tid; idx < static_cast<int64_t>(nf) * ns; idx += NT) {
-    const int c = static_cast<int>(idx % ns), r = static_cast<int>(idx / ns);
-    const T v = F[(nf + c) + static_cast<int64_t>(r) * ld];
-    PT[c + static_cast<int64_t>(r) * ns] = r < ns ? (r >= c ? v : T(0)) : -v;
+  const int64_t np = static_cast<int64_t>(nf) * ns;
+  for (int64_t i0 = tid; i0 < np; i0 += NT * kU) {
+    T pv[kU];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      const int64_t idx = i0 + u * NT;
+      const int c = static_cast<int>(idx % ns), r = static_cast<int>(idx / ns);
+      pv[u] = idx < np ? F[(nf + c) + static_cast<int64_t>(r) * ld] : T(0);
+      if (r < ns && r < c) pv[u] = T(0);
+      if (r >= ns) pv[u] = -pv[u];
+    }
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      const int64_t idx = i0 + u * NT;
+      if (idx < np) {
+        const int c = static_cast<int>(idx % ns), r = static_cast<int>(idx / ns);
+        PT[idx] = pv[u];
+        P[r + static_cast<int64_t>(c) * nf] = pv[u];
+      }
+    }
   }
   (void)rs;
 }
@@ -1165,6 +1241,11 @@ struct dcx_plan {
 
 namespace {
 
+int env_int(const char* name, int dflt) {
+  const char* e = std::getenv(name);
+  return e ? std::atoi(e) : dflt;
+}
+
 int default_fuse_rows(int precision) {
   const char* e = std::getenv("DCX_FUSE_ROWS");
   if (e) return std::atoi(e);
@@ -1246,15 +1327,19 @@ int launch_factor(dcx_plan* P, hipStream_t st) {
       const int s = S.lvl_sup[q];
       maxld = std::max(maxld, 2 * S.ns[s] + S.rs[s]);
     }
-    const size_t lds = (2 * kNB + static_cast<size_t>(maxld + 16) * kLDP) * sizeof(T);
+    const size_t lds =
+        (2 * kNB + kNB * (kNB + 1) + kNB + static_cast<size_t>(maxld + 16) * kLDP) * sizeof(T);
     if (lds > 160 * 1024 || maxld > 1024)
       return fail(DCX_ERR_ARG, "front too large for the factor kernel");
+    static const int big = env_int("DCX_FACTOR_BIG", 1024);
     if (maxld <= 256)
       factor_launch<T, 256, 1>(P, d, cnt, lds, st);
-    else if (maxld <= 512)
+    else if (maxld <= 512 && big == 512)
       factor_launch<T, 512, 1>(P, d, cnt, lds, st);
+    else if (maxld <= 1024)
+      factor_launch<T, 1024, 1>(P, d, cnt, lds, st);
     else
-      factor_launch<T, 512, 2>(P, d, cnt, lds, st);
+      return fail(DCX_ERR_ARG, "front too large for the factor kernel");
   }
   HIP_TRY(hipGetLastError());
   return DCX_OK;
@@ -1302,10 +1387,6 @@ void sweep_launch(dcx_plan* P, T* W, int k, int maxnf, int maxns, int pmax, int 
 // that the level still has about kTargetBlocks workgroups
 constexpr size_t kSweepLds = 160 * 1024;
 
-int env_int(const char* name, int dflt) {
-  const char* e = std::getenv(name);
-  return e ? std::atoi(e) : dflt;
-}
 
 template <typename T, int MT, int V, int KS>
 void bwd_wave_launch(dcx_plan* P, T* W, int k, const int* lvl, int cnt, const int* active,
