@@ -73,9 +73,9 @@ def time_steps(eng, mt, dobs, w, steps, warmup):
 
 
 def sweep_roofline(eng, mt, dobs, w, refine, reps=3):
-    """Backward-sweep kernel (k_bwd_level) of the engine's solves: average
-    launch duration from HIP events recorded on the solve stream, and its
-    algorithmic bytes and flops per launch."""
+    """Backward-sweep kernel (k_bwd_wave, one launch per tree level) of the
+    engine's solves: average launch duration from HIP events recorded on the
+    solve stream, and its algorithmic bytes and flops per launch."""
     sv = eng.solver
     info = sv.info()
     # a mixed plan writes an FP64 copy of x_J unless the fields stay FP32
@@ -97,7 +97,7 @@ def sweep_roofline(eng, mt, dobs, w, refine, reps=3):
     flops = 2.0 * nb * k * info["solve_flops_per_rhs"] / 4.0
     avg_ms = b_ms / launches
     achieved = (per_sweep / info["level_launches"]) / (avg_ms * 1e-3) / 1e9
-    return {"kernel": "k_bwd_level", "bound": "hbm", "achieved": round(achieved, 1),
+    return {"kernel": "k_bwd_wave", "bound": "hbm", "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": None, "avg_ms": round(avg_ms, 4),
             "bytes_per_launch": int(per_sweep / info["level_launches"]),
@@ -111,11 +111,12 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_line25k_eval_grad.jso
 
 
 def measured_traffic(kernel):
-    """HBM bytes per dispatch of `kernel` from the committed rocprofv3 --pmc
-    summary (FETCH_SIZE + WRITE_SIZE, calibrated on a 1 GiB copy), or None."""
+    """HBM bytes per dispatch of the kernel family `kernel` (name up to the
+    first template argument) from the committed rocprofv3 --pmc summary
+    (FETCH_SIZE + WRITE_SIZE, calibrated on a 1 GiB copy), or None."""
     try:
         with open(TRAFFIC_FILE) as fh:
-            return int(json.load(fh)["kernels"][kernel]["traffic_bytes"])
+            return int(json.load(fh)["families"][kernel]["traffic_bytes"])
     except (OSError, KeyError, ValueError):
         return None
 
@@ -197,8 +198,8 @@ def main():
     log(f"mixed eval+grad {msmx:.2f} ms")
     roof = sweep_roofline(emx, mt, dobs, w, args.refine)
     roof64 = sweep_roofline(e64, mt, dobs, w, 0)
-    roof["traffic"] = measured_traffic("void k_bwd_level<float, 32, false>")
-    roof64["traffic"] = measured_traffic("void k_bwd_level<double, 32, false>")
+    roof["traffic"] = measured_traffic("k_bwd_wave<float>")
+    roof64["traffic"] = measured_traffic("k_bwd_wave<double>")
     cpu = None
     if not args.no_cpu:
         log("cpu baseline (oracle SuperLU, 1 wavenumber sample)")

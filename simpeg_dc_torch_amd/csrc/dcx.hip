@@ -99,17 +99,32 @@ struct Mma<float> {
 // leaves L_JJ, L_RJ, the Schur complement in the (R,R) block -- this front's
 // update block -- and L_EJ = L_JJ^{-T} in the E rows, i.e. the inverse the
 // sweeps need, without a separate inversion.
-// Per panel of kNB columns: every thread holds RPT panel rows in registers
-// and eliminates the columns one after the other (one barrier per column:
-// the pivot rows' column is broadcast through LDS); the panel then goes to
-// LDS and the trailing update -- lower triangle of the J/R rows and the E
-// rows times the J/R columns -- runs as 16x16 tiles on the matrix unit, one
-// tile per wavefront at a time.
+// Per panel of kNB columns: every thread holds RPT panel rows in registers;
+// wavefront 0 factors the kNB x kNB diagonal block (pivot columns read across
+// lanes, no barrier), then every other row solves against it on its own
+// registers (one barrier); the panel goes to LDS and the trailing update --
+// lower triangle of the J/R rows and the E rows times the J/R columns --
+// runs as 16x16 tiles on the matrix unit, two tiles per wavefront at a time.
+// Global gathers (assembly, extend-add, outputs) take 16 entries per thread
+// at a time, loads first.
 // The panel written for the sweeps is P_s = [ L_JJ^{-1} ; L_RJ L_JJ^{-1} ]
 // (nf x ns) and its transpose.
 // ---------------------------------------------------------------------------
 constexpr int kNB = 32;        // panel width
 constexpr int kLDP = kNB + 4;  // LDS row stride of the panel
+constexpr int kLDG = kNB + 4;  // LDS row stride of the diagonal block
+
+// value of v in lane l (l uniform) for every lane of the wavefront
+__device__ inline float lane_value(float v, int l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+__device__ inline double lane_value(double v, int l) {
+  const long long x = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane(static_cast<int>(x), l);
+  const int hi = __builtin_amdgcn_readlane(static_cast<int>(x >> 32), l);
+  return __longlong_as_double((static_cast<long long>(hi) << 32) |
+                              static_cast<unsigned int>(lo));
+}
 
 template <typename T, int NT, int RPT>
 __global__ __launch_bounds__(NT) void k_factor_level(
@@ -127,14 +142,12 @@ __global__ __launch_bounds__(NT) void k_factor_level(
   const int ns = S.ns, rs = S.rs, nf = S.ns + S.rs, ld = nf + S.ns;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid / 64;
   constexpr int NW = NT / 64;
-  T* colb = reinterpret_cast<T*>(smem_raw);  // 2 x kNB pivot-column broadcast
-  T* ldg = colb + 2 * kNB;                    // diagonal block rows, stride kNB + 1
-  T* dgi = ldg + kNB * (kNB + 1);             // reciprocal diagonal
+  T* ldg = reinterpret_cast<T*>(smem_raw);   // diagonal block rows, stride kLDG
+  T* dgi = ldg + kNB * kLDG;                  // reciprocal diagonal
   T* pan = dgi + kNB;                         // panel rows, stride kLDP
   T* F = fcur + b * fcur_bs + S.front_off;
   const int64_t fsz = static_cast<int64_t>(ld) * nf;
   for (int64_t i = tid; i < fsz; i += NT) F[i] = T(0);
-  if (tid < 2 * kNB) colb[tid] = T(0);
   __syncthreads();
   for (int j = tid; j < ns; j += NT) F[(nf + j) + static_cast<int64_t>(j) * ld] = T(1);
   const double* v = vals + b * nnz;
@@ -200,37 +213,37 @@ __global__ __launch_bounds__(NT) void k_factor_level(
       for (int c = 0; c < kNB; ++c)
         rv[q][c] = (r < m && c < jb) ? F[(j0 + r) + static_cast<int64_t>(j0 + c) * ld] : T(0);
     }
-    // (1) the jb x jb diagonal block, by wavefront 0 alone (lanes < jb hold
-    // its rows; the pivot column is broadcast through LDS, ordered within
-    // the wavefront by fences): its L rows and reciprocal diagonal go to LDS
+    // (1) the jb x jb diagonal block, by wavefront 0 alone: lanes < jb hold
+    // its rows, the pivot column is read across lanes (readlane, no LDS);
+    // its L rows and reciprocal diagonal go to LDS
     if (wave == 0) {
 #pragma unroll
       for (int t = 0; t < kNB; ++t) {
         if (t < jb) {  // uniform
-          if (lane >= t && lane < jb) colb[lane] = rv[0][t];
-          __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-          T d = colb[t];
-          if (!(d > T(0))) {
+          T d = lane_value(rv[0][t], t);
+          if (!(d > T(0))) {  // uniform
             if (lane == 0) atomicMax(info, 1);
             d = T(1);
           }
           const T sq = sqrt(d);
           const T dinv = T(1) / sq;
+          T cl[kNB];  // column t of rows l > t, unscaled
+#pragma unroll
+          for (int l = t + 1; l < kNB; ++l) cl[l] = lane_value(rv[0][t], l);
           if (lane == t) {
             rv[0][t] = sq;
             dgi[t] = dinv;
           } else if (lane > t && lane < jb) {
-            const T lr = rv[0][t] * dinv;
-            rv[0][t] = lr;
+            const T g = rv[0][t] * (dinv * dinv);
+            rv[0][t] *= dinv;
 #pragma unroll
-            for (int l = t + 1; l < kNB; ++l) rv[0][l] -= lr * (colb[l] * dinv);
+            for (int l = t + 1; l < kNB; ++l) rv[0][l] -= g * cl[l];
           }
-          __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         }
       }
       if (lane < jb) {
 #pragma unroll
-        for (int c = 0; c < kNB; ++c) ldg[lane * (kNB + 1) + c] = rv[0][c];
+        for (int c = 0; c < kNB; ++c) ldg[lane * kLDG + c] = rv[0][c];
       }
     }
     __syncthreads();
@@ -246,8 +259,8 @@ __global__ __launch_bounds__(NT) void k_factor_level(
             T s0 = rv[q][t], s1 = T(0);
 #pragma unroll
             for (int l = 0; l < t; l += 2) {
-              s0 -= rv[q][l] * ldg[t * (kNB + 1) + l];
-              if (l + 1 < t) s1 -= rv[q][l + 1] * ldg[t * (kNB + 1) + l + 1];
+              s0 -= rv[q][l] * ldg[t * kLDG + l];
+              if (l + 1 < t) s1 -= rv[q][l + 1] * ldg[t * kLDG + l + 1];
             }
             rv[q][t] = (s0 + s1) * dgi[t];
           }
@@ -1315,7 +1328,8 @@ void factor_launch(dcx_plan* P, int d, int cnt, size_t lds, hipStream_t st) {
                      static_cast<T*>(P->d_panelsT), S.panel_size, P->d_info);
 }
 
-// one launch per level; threads x rows per thread cover the tallest panel
+// one launch per level; the threads (x rows per thread) cover the tallest
+// panel of the level
 template <typename T>
 int launch_factor(dcx_plan* P, hipStream_t st) {
   const auto& S = P->S;
@@ -1328,14 +1342,11 @@ int launch_factor(dcx_plan* P, hipStream_t st) {
       maxld = std::max(maxld, 2 * S.ns[s] + S.rs[s]);
     }
     const size_t lds =
-        (2 * kNB + kNB * (kNB + 1) + kNB + static_cast<size_t>(maxld + 16) * kLDP) * sizeof(T);
+        (kNB * kLDG + kNB + static_cast<size_t>(maxld + 16) * kLDP) * sizeof(T);
     if (lds > 160 * 1024 || maxld > 1024)
       return fail(DCX_ERR_ARG, "front too large for the factor kernel");
-    static const int big = env_int("DCX_FACTOR_BIG", 1024);
     if (maxld <= 256)
       factor_launch<T, 256, 1>(P, d, cnt, lds, st);
-    else if (maxld <= 512 && big == 512)
-      factor_launch<T, 512, 1>(P, d, cnt, lds, st);
     else if (maxld <= 1024)
       factor_launch<T, 1024, 1>(P, d, cnt, lds, st);
     else
@@ -1424,7 +1435,7 @@ template <typename T>
 bool bwd_wave(dcx_plan* P, T* W, int k, int maxns, const int* lvl, int cnt, const int* active,
               const SweepIO& io, hipStream_t st) {
   static const int use = env_int("DCX_BWD_WAVE", 1);
-  static const int min_waves = env_int("DCX_BWD_SPLITK_BELOW", 16384);
+  static const int min_waves = env_int("DCX_BWD_SPLITK_BELOW", 4096);
   if (!use) return false;
   constexpr int VW = 16 / sizeof(T);  // 16-byte vectors
   const int64_t waves = static_cast<int64_t>(cnt) * P->nbatch * ((k + 16 * VW - 1) / (16 * VW));
