@@ -1399,7 +1399,7 @@ void sweep_launch(dcx_plan* P, T* W, int k, int maxnf, int maxns, int pmax, int 
 constexpr size_t kSweepLds = 160 * 1024;
 
 
-template <typename T, int MT, int V, int KS>
+template <typename T, int MT, int V, int KS, int PF>
 void bwd_wave_launch(dcx_plan* P, T* W, int k, const int* lvl, int cnt, const int* active,
                      const SweepIO& io, hipStream_t st) {
   constexpr int CW = 16 * V;
@@ -1409,21 +1409,21 @@ void bwd_wave_launch(dcx_plan* P, T* W, int k, const int* lvl, int cnt, const in
   dim3 grid(((total + 7) / 8) * 8);
   const size_t lds = KS > 1 ? static_cast<size_t>(KS) * MT * V * 4 * 64 * sizeof(T) : 0;
   const int64_t w_bs = static_cast<int64_t>(P->S.n) * k;
-  hipLaunchKernelGGL((k_bwd_wave<T, MT, V, KS, 4>), grid, dim3(kThreads), lds, st, P->d_sup,
+  hipLaunchKernelGGL((k_bwd_wave<T, MT, V, KS, PF>), grid, dim3(kThreads), lds, st, P->d_sup,
                      lvl, P->d_ridx, active, static_cast<const T*>(P->d_panelsT),
                      P->S.panel_size, W, w_bs, k, io.X64, io.xrow, io.xmode, cnt, nzb, total);
 }
 
-template <typename T, int V, int KS>
+template <typename T, int V, int KS, int PF>
 bool bwd_wave_mt(dcx_plan* P, T* W, int k, int maxns, const int* lvl, int cnt,
                  const int* active, const SweepIO& io, hipStream_t st) {
   const int mt = (maxns + 15) / 16;
-  if (mt <= 1) bwd_wave_launch<T, 1, V, KS>(P, W, k, lvl, cnt, active, io, st);
-  else if (mt <= 2) bwd_wave_launch<T, 2, V, KS>(P, W, k, lvl, cnt, active, io, st);
-  else if (mt <= 3) bwd_wave_launch<T, 3, V, KS>(P, W, k, lvl, cnt, active, io, st);
-  else if (mt <= 4) bwd_wave_launch<T, 4, V, KS>(P, W, k, lvl, cnt, active, io, st);
-  else if (mt <= 6) bwd_wave_launch<T, 6, V, KS>(P, W, k, lvl, cnt, active, io, st);
-  else if (mt <= 8) bwd_wave_launch<T, 8, V, KS>(P, W, k, lvl, cnt, active, io, st);
+  if (mt <= 1) bwd_wave_launch<T, 1, V, KS, PF>(P, W, k, lvl, cnt, active, io, st);
+  else if (mt <= 2) bwd_wave_launch<T, 2, V, KS, PF>(P, W, k, lvl, cnt, active, io, st);
+  else if (mt <= 3) bwd_wave_launch<T, 3, V, KS, PF>(P, W, k, lvl, cnt, active, io, st);
+  else if (mt <= 4) bwd_wave_launch<T, 4, V, KS, PF>(P, W, k, lvl, cnt, active, io, st);
+  else if (mt <= 6) bwd_wave_launch<T, 6, V, KS, PF>(P, W, k, lvl, cnt, active, io, st);
+  else if (mt <= 8) bwd_wave_launch<T, 8, V, KS, PF>(P, W, k, lvl, cnt, active, io, st);
   else return false;
   return true;
 }
@@ -1439,8 +1439,8 @@ bool bwd_wave(dcx_plan* P, T* W, int k, int maxns, const int* lvl, int cnt, cons
   if (!use) return false;
   constexpr int VW = 16 / sizeof(T);  // 16-byte vectors
   const int64_t waves = static_cast<int64_t>(cnt) * P->nbatch * ((k + 16 * VW - 1) / (16 * VW));
-  if (waves < min_waves) return bwd_wave_mt<T, 1, 4>(P, W, k, maxns, lvl, cnt, active, io, st);
-  return bwd_wave_mt<T, VW, 1>(P, W, k, maxns, lvl, cnt, active, io, st);
+  if (waves < min_waves) return bwd_wave_mt<T, 1, 4, 4>(P, W, k, maxns, lvl, cnt, active, io, st);
+  return bwd_wave_mt<T, VW, 1, 4>(P, W, k, maxns, lvl, cnt, active, io, st);
 }
 
 template <typename T, int TC>
