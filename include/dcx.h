@@ -131,6 +131,16 @@ int dcx_factorize(dcx_plan* plan, const double* values, void* stream);
 int dcx_solve(dcx_plan* plan, const double* b, void* x, int k, int flags,
               void* stream);
 
+/* dcx_factorize followed by dcx_solve, with the factorisation and the
+ * solve's forward sweep overlapped: the forward sweep of each tree level runs
+ * on a second stream of the plan (default priority) as soon as that level is
+ * factored; the backward sweep and everything after it are ordered on
+ * `stream`.  Same arguments and results as the two calls.  Replaces the
+ * factorize + solve pair of _SparseSolveBatchFn.forward
+ * (dctorch/solver.py:251-259). */
+int dcx_factor_solve(dcx_plan* plan, const double* values, const double* b, void* x, int k,
+                     int flags, void* stream);
+
 /* Rows (host, original numbering) outside of which right-hand sides passed
  * with DCX_SPARSE_RHS are zero; lets the forward sweep skip the supernodes
  * whose subtree holds none of them. */

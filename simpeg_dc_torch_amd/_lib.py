@@ -10,7 +10,8 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libdcx.so")
+# the in-tree build; DCX_LIBRARY names another build of the same library
+LIB_PATH = os.environ.get("DCX_LIBRARY") or os.path.join(_HERE, "libdcx.so")
 
 DCX_FP64, DCX_FP32 = 0, 1
 DCX_SHARED_RHS, DCX_PERMUTED, DCX_SPARSE_RHS, DCX_F32_FIELDS = 1, 2, 4, 8
@@ -64,6 +65,7 @@ PROTOTYPES = {
                                        C.POINTER(C.c_int)]),
     "dcx_factorize": (C.c_int, [_vp, _vp, _vp]),
     "dcx_solve": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, _vp]),
+    "dcx_factor_solve": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, C.c_int, _vp]),
     "dcx_set_rhs_support": (C.c_int, [_vp, _i32p, C.c_int]),
     "dcx_grad_values": (C.c_int, [_vp, _vp, _vp, C.c_int, _vp, _vp, C.c_int, _vp]),
     "dcx_mma_probe": (C.c_int, [C.c_int, _vp, _vp, _vp, _vp]),

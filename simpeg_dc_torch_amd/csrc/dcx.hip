@@ -267,15 +267,17 @@ __global__ __launch_bounds__(NT) void k_factor_level(
         }
       }
     }
-    // panel to F (its lower part) and to LDS (zero columns past jb; the 16
-    // rows past m are zeroed for the tile reads)
+    // panel to F (its E rows) and to LDS (zero columns past jb; the 16 rows
+    // past m are zeroed for the tile reads)
 #pragma unroll
     for (int q = 0; q < RPT; ++q) {
       const int r = tid + q * NT;
       if (r < m) {
 #pragma unroll
         for (int c = 0; c < kNB; ++c) {
-          if (c < jb && r >= c) F[(j0 + r) + static_cast<int64_t>(j0 + c) * ld] = rv[q][c];
+          // only the E rows of a finished panel are read again (the sweep
+          // panel comes from them); its J/R rows live on in LDS only
+          if (c < jb && r >= n1) F[(j0 + r) + static_cast<int64_t>(j0 + c) * ld] = rv[q][c];
           pan[r * kLDP + c] = c < jb ? rv[q][c] : T(0);
         }
       }
@@ -1236,12 +1238,17 @@ struct dcx_plan {
   size_t r_bytes = 0, b_bytes = 0, x_bytes = 0;
   int64_t device_bytes = 0;
   int max_row_nnz = 0;  // longest row of the caller's pattern
+  // second stream and level events of dcx_factor_solve
+  hipStream_t side = nullptr;
+  std::vector<hipEvent_t> lev_ev;
   // optional event timing of the sweeps (forward, backward) of every solve
   bool timing = false;
   std::vector<hipEvent_t> ev;  // triples per sweep pair: start, mid, end
 
   ~dcx_plan() {
     for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+    for (hipEvent_t e : lev_ev) (void)hipEventDestroy(e);
+    if (side) (void)hipStreamDestroy(side);
     void* ptrs[] = {d_sup, d_lvl, d_relpos, d_cidx, d_ridx, d_a_e, d_a_row, d_a_col,
                     d_perm, d_iperm, d_pcrow, d_pcol, d_pvidx, d_crow, d_col, d_info, d_panels,
                     d_front[0], d_front[1], d_vals, d_active, d_act_lvl, d_upd[0],
@@ -1331,26 +1338,30 @@ void factor_launch(dcx_plan* P, int d, int cnt, size_t lds, hipStream_t st) {
 // one launch per level; the threads (x rows per thread) cover the tallest
 // panel of the level
 template <typename T>
-int launch_factor(dcx_plan* P, hipStream_t st) {
+int factor_level(dcx_plan* P, int d, hipStream_t st) {
   const auto& S = P->S;
-  for (int d = S.ndepth - 1; d >= 0; --d) {
-    const int cnt = S.lvl_ptr[d + 1] - S.lvl_ptr[d];
-    if (!cnt) continue;
-    int maxld = 0;
-    for (int q = S.lvl_ptr[d]; q < S.lvl_ptr[d + 1]; ++q) {
-      const int s = S.lvl_sup[q];
-      maxld = std::max(maxld, 2 * S.ns[s] + S.rs[s]);
-    }
-    const size_t lds =
-        (kNB * kLDG + kNB + static_cast<size_t>(maxld + 16) * kLDP) * sizeof(T);
-    if (lds > 160 * 1024 || maxld > 1024)
-      return fail(DCX_ERR_ARG, "front too large for the factor kernel");
-    if (maxld <= 256)
-      factor_launch<T, 256, 1>(P, d, cnt, lds, st);
-    else if (maxld <= 1024)
-      factor_launch<T, 1024, 1>(P, d, cnt, lds, st);
-    else
-      return fail(DCX_ERR_ARG, "front too large for the factor kernel");
+  const int cnt = S.lvl_ptr[d + 1] - S.lvl_ptr[d];
+  if (!cnt) return DCX_OK;
+  int maxld = 0;
+  for (int q = S.lvl_ptr[d]; q < S.lvl_ptr[d + 1]; ++q) {
+    const int s = S.lvl_sup[q];
+    maxld = std::max(maxld, 2 * S.ns[s] + S.rs[s]);
+  }
+  const size_t lds = (kNB * kLDG + kNB + static_cast<size_t>(maxld + 16) * kLDP) * sizeof(T);
+  if (lds > 160 * 1024 || maxld > 1024)
+    return fail(DCX_ERR_ARG, "front too large for the factor kernel");
+  if (maxld <= 256)
+    factor_launch<T, 256, 1>(P, d, cnt, lds, st);
+  else
+    factor_launch<T, 1024, 1>(P, d, cnt, lds, st);
+  return DCX_OK;
+}
+
+template <typename T>
+int launch_factor(dcx_plan* P, hipStream_t st) {
+  for (int d = P->S.ndepth - 1; d >= 0; --d) {
+    const int rc = factor_level<T>(P, d, st);
+    if (rc) return rc;
   }
   HIP_TRY(hipGetLastError());
   return DCX_OK;
@@ -1515,75 +1526,143 @@ int launch_fused(dcx_plan* P, T* W, int k, bool fwd, const int* active, const Sw
   return DCX_OK;
 }
 
+// update buffers of the forward sweep for k right-hand sides (grown on
+// demand; growing synchronises the stream once)
 template <typename T>
-int sweeps(dcx_plan* P, T* W, int k, bool sparse_rhs, const SweepIO& io, hipStream_t st) {
+int ensure_upd(dcx_plan* P, int k, hipStream_t st) {
   const auto& S = P->S;
-  if (P->upd_k < k) {
-    const size_t ub0 = static_cast<size_t>(P->nbatch) * S.upd_rows[0] * k * sizeof(T);
-    const size_t ub1 = static_cast<size_t>(P->nbatch) * S.upd_rows[1] * k * sizeof(T);
-    const size_t ubf = static_cast<size_t>(P->nbatch) * S.f_upd_rows * k * sizeof(T);
-    HIP_TRY(hipStreamSynchronize(st));
-    void* olds[] = {P->d_upd[0], P->d_upd[1], P->d_ufz};
-    for (void* q : olds)
-      if (q) HIP_TRY(hipFree(q));
-    P->d_upd[0] = P->d_upd[1] = P->d_ufz = nullptr;
-    HIP_TRY(hipMalloc(&P->d_upd[0], std::max<size_t>(ub0, 256)));
-    HIP_TRY(hipMalloc(&P->d_upd[1], std::max<size_t>(ub1, 256)));
-    HIP_TRY(hipMalloc(&P->d_ufz, std::max<size_t>(ubf, 256)));
-    P->upd_k = k;
+  if (P->upd_k >= k) return DCX_OK;
+  const size_t ub0 = static_cast<size_t>(P->nbatch) * S.upd_rows[0] * k * sizeof(T);
+  const size_t ub1 = static_cast<size_t>(P->nbatch) * S.upd_rows[1] * k * sizeof(T);
+  const size_t ubf = static_cast<size_t>(P->nbatch) * S.f_upd_rows * k * sizeof(T);
+  HIP_TRY(hipStreamSynchronize(st));
+  void* olds[] = {P->d_upd[0], P->d_upd[1], P->d_ufz};
+  for (void* q : olds)
+    if (q) HIP_TRY(hipFree(q));
+  P->d_upd[0] = P->d_upd[1] = P->d_ufz = nullptr;
+  HIP_TRY(hipMalloc(&P->d_upd[0], std::max<size_t>(ub0, 256)));
+  HIP_TRY(hipMalloc(&P->d_upd[1], std::max<size_t>(ub1, 256)));
+  HIP_TRY(hipMalloc(&P->d_ufz, std::max<size_t>(ubf, 256)));
+  P->upd_k = k;
+  return DCX_OK;
+}
+
+// one level of one sweep direction (sparse forward: active supernodes only)
+template <typename T>
+int sweep_level(dcx_plan* P, T* W, int k, bool fwd, int d, const int* act, const SweepIO& io,
+                hipStream_t st) {
+  const auto& S = P->S;
+  const int* lvl;
+  int cnt;
+  if (fwd && act) {
+    lvl = P->d_act_lvl + P->act_lvl_ptr[d];
+    cnt = P->act_lvl_ptr[d + 1] - P->act_lvl_ptr[d];
+  } else {
+    lvl = P->d_ulvl + S.ulvl_ptr[d];
+    cnt = S.ulvl_ptr[d + 1] - S.ulvl_ptr[d];
   }
+  if (!cnt) return DCX_OK;
+  int maxnf = 0, maxns = 0, maxpanel = 0;
+  for (int q = S.ulvl_ptr[d]; q < S.ulvl_ptr[d + 1]; ++q) {
+    int s = S.ulvl_sup[q];
+    maxnf = std::max(maxnf, S.ns[s] + S.rs[s]);
+    maxns = std::max(maxns, S.ns[s]);
+    maxpanel = std::max(maxpanel, (S.ns[s] + S.rs[s]) * S.ns[s]);
+  }
+  const int tc = pick_tc(maxnf, maxns, sizeof(T));
+  if (!tc) return fail(DCX_ERR_ARG, "front too large for the in-LDS sweep");
+  switch (tc) {
+    case 64: sweep_tc<T, 64>(P, W, k, maxnf, maxns, maxpanel, fwd, d, lvl, cnt, act, io, st); break;
+    case 32: sweep_tc<T, 32>(P, W, k, maxnf, maxns, maxpanel, fwd, d, lvl, cnt, act, io, st); break;
+    default: sweep_tc<T, 16>(P, W, k, maxnf, maxns, maxpanel, fwd, d, lvl, cnt, act, io, st); break;
+  }
+  return DCX_OK;
+}
+
+// forward then backward sweep on W; skip_fwd: the forward sweep was already
+// run (overlapped with the factorisation), only the backward one remains
+template <typename T>
+int sweeps(dcx_plan* P, T* W, int k, bool sparse_rhs, const SweepIO& io, hipStream_t st,
+           bool skip_fwd = false) {
+  const auto& S = P->S;
+  int rc = ensure_upd<T>(P, k, st);
+  if (rc) return rc;
   const bool use_act = sparse_rhs && P->d_active;
   const int* act = use_act ? P->d_active : nullptr;
+  const bool timed = P->timing && !skip_fwd;
   hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
-  if (P->timing) {
+  if (timed) {
     for (auto& e : ev) {
       HIP_TRY(hipEventCreate(&e));
       P->ev.push_back(e);
     }
     HIP_TRY(hipEventRecord(ev[0], st));
   }
-  for (int pass = 0; pass < 2; ++pass) {
+  for (int pass = skip_fwd ? 1 : 0; pass < 2; ++pass) {
     const bool fwd = pass == 0;
-    if (!fwd && P->timing) HIP_TRY(hipEventRecord(ev[1], st));
+    if (!fwd && timed) HIP_TRY(hipEventRecord(ev[1], st));
     if (fwd && S.nfused) {
-      int rc = launch_fused<T>(P, W, k, true, act, io, st);
+      rc = launch_fused<T>(P, W, k, true, act, io, st);
       if (rc) return rc;
     }
     for (int dd = 0; dd < S.ndepth; ++dd) {
       const int d = fwd ? S.ndepth - 1 - dd : dd;
-      const int* lvl;
-      int cnt;
-      if (fwd && use_act) {
-        lvl = P->d_act_lvl + P->act_lvl_ptr[d];
-        cnt = P->act_lvl_ptr[d + 1] - P->act_lvl_ptr[d];
-      } else {
-        lvl = P->d_ulvl + S.ulvl_ptr[d];
-        cnt = S.ulvl_ptr[d + 1] - S.ulvl_ptr[d];
-      }
-      if (!cnt) continue;
-      int maxnf = 0, maxns = 0, maxpanel = 0;
-      for (int q = S.ulvl_ptr[d]; q < S.ulvl_ptr[d + 1]; ++q) {
-        int s = S.ulvl_sup[q];
-        maxnf = std::max(maxnf, S.ns[s] + S.rs[s]);
-        maxns = std::max(maxns, S.ns[s]);
-        maxpanel = std::max(maxpanel, (S.ns[s] + S.rs[s]) * S.ns[s]);
-      }
-      const int tc = pick_tc(maxnf, maxns, sizeof(T));
-      if (!tc) return fail(DCX_ERR_ARG, "front too large for the in-LDS sweep");
-      switch (tc) {
-        case 64: sweep_tc<T, 64>(P, W, k, maxnf, maxns, maxpanel, fwd, d, lvl, cnt, act, io, st); break;
-        case 32: sweep_tc<T, 32>(P, W, k, maxnf, maxns, maxpanel, fwd, d, lvl, cnt, act, io, st); break;
-        default: sweep_tc<T, 16>(P, W, k, maxnf, maxns, maxpanel, fwd, d, lvl, cnt, act, io, st); break;
-      }
+      rc = sweep_level<T>(P, W, k, fwd, d, act, io, st);
+      if (rc) return rc;
     }
     if (!fwd && S.nfused) {
-      int rc = launch_fused<T>(P, W, k, false, act, io, st);
+      rc = launch_fused<T>(P, W, k, false, act, io, st);
       if (rc) return rc;
     }
   }
-  if (P->timing) HIP_TRY(hipEventRecord(ev[2], st));
+  if (timed) HIP_TRY(hipEventRecord(ev[2], st));
   HIP_TRY(hipGetLastError());
   return DCX_OK;
+}
+
+// the plan's second stream (default priority) and its level events, made once
+int ensure_side(dcx_plan* P) {
+  if (P->side) return DCX_OK;
+  HIP_TRY(hipStreamCreateWithFlags(&P->side, hipStreamNonBlocking));
+  P->lev_ev.assign(P->S.ndepth + 2, nullptr);
+  for (auto& e : P->lev_ev) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  return DCX_OK;
+}
+
+// Factorisation and the first solve's forward sweep, overlapped: the factor
+// levels run on `st`; the forward sweep of level d (deepest first) runs on the
+// plan's second stream as soon as level d is factored, using the compute
+// units the narrow upper factor levels leave idle; the backward sweep then
+// runs on `st` after both.  Without the overlap (fused subtrees) it is the
+// plain sequence.
+template <typename T>
+int factor_and_sweeps(dcx_plan* P, T* W, int k, bool sparse_rhs, const SweepIO& io,
+                      hipStream_t st) {
+  const auto& S = P->S;
+  if (S.nfused) {
+    int rc = launch_factor<T>(P, st);
+    if (rc) return rc;
+    return sweeps<T>(P, W, k, sparse_rhs, io, st);
+  }
+  int rc = ensure_upd<T>(P, k, st);
+  if (rc) return rc;
+  rc = ensure_side(P);
+  if (rc) return rc;
+  const int* act = sparse_rhs && P->d_active ? P->d_active : nullptr;
+  hipEvent_t start = P->lev_ev[S.ndepth], join = P->lev_ev[S.ndepth + 1];
+  HIP_TRY(hipEventRecord(start, st));
+  HIP_TRY(hipStreamWaitEvent(P->side, start, 0));
+  for (int d = S.ndepth - 1; d >= 0; --d) {
+    rc = factor_level<T>(P, d, st);
+    if (rc) return rc;
+    HIP_TRY(hipEventRecord(P->lev_ev[d], st));
+    HIP_TRY(hipStreamWaitEvent(P->side, P->lev_ev[d], 0));
+    rc = sweep_level<T>(P, W, k, true, d, act, io, P->side);
+    if (rc) return rc;
+  }
+  HIP_TRY(hipEventRecord(join, P->side));
+  HIP_TRY(hipStreamWaitEvent(st, join, 0));
+  return sweeps<T>(P, W, k, sparse_rhs, io, st, true);
 }
 
 void launch_permute_d2T(dcx_plan* P, const double* in, void* out, int k, int nblocks,
@@ -1811,12 +1890,10 @@ int dcx_factorize(dcx_plan* P, const double* values, void* stream) {
   return DCX_OK;
 }
 
-int dcx_solve(dcx_plan* P, const double* b, void* xv, int k, int flags, void* stream) {
-  if (!P || !b || !xv) return fail(DCX_ERR_ARG, "null argument");
-  if (!P->factored) return fail(DCX_ERR_STATE, "dcx_solve before dcx_factorize");
-  if (k < 1) return fail(DCX_ERR_ARG, "k must be >= 1");
-  hipStream_t st = static_cast<hipStream_t>(stream);
-  HIP_TRY(hipSetDevice(P->device));
+// dcx_solve and dcx_factor_solve (with_factor: the factorisation runs here,
+// overlapped with the first forward sweep)
+static int solve_impl(dcx_plan* P, const double* b, void* xv, int k, int flags, hipStream_t st,
+               bool with_factor) {
   const auto& S = P->S;
   const int nb = P->nbatch;
   const bool shared = flags & DCX_SHARED_RHS;
@@ -1844,7 +1921,8 @@ int dcx_solve(dcx_plan* P, const double* b, void* xv, int k, int flags, void* st
       io.X64 = x;
       io.xmode = 1;
     }
-    return sweeps<double>(P, W, k, sparse, io, st);
+    return with_factor ? factor_and_sweeps<double>(P, W, k, sparse, io, st)
+                       : sweeps<double>(P, W, k, sparse, io, st);
   }
   // mixed: FP32 sweeps, FP64 residual refinement; the solution accumulates in
   // x (double) or, for a float x, in a plan buffer rounded into x at the end
@@ -1852,7 +1930,9 @@ int dcx_solve(dcx_plan* P, const double* b, void* xv, int k, int flags, void* st
   float* W;
   if (f32x) {
     W = static_cast<float*>(xv);
-    if (P->refine == 0) return sweeps<float>(P, W, k, sparse, io, st);
+    if (P->refine == 0)
+      return with_factor ? factor_and_sweeps<float>(P, W, k, sparse, io, st)
+                         : sweeps<float>(P, W, k, sparse, io, st);
     rc = ensure(reinterpret_cast<void**>(&P->d_x), &P->x_bytes,
                 static_cast<size_t>(nb) * blk * 8, st);
     if (rc) return rc;
@@ -1873,7 +1953,8 @@ int dcx_solve(dcx_plan* P, const double* b, void* xv, int k, int flags, void* st
   const int64_t Bbs = io.b_bs;
   io.X64 = x;
   io.xmode = 1;
-  rc = sweeps<float>(P, W, k, sparse, io, st);
+  rc = with_factor ? factor_and_sweeps<float>(P, W, k, sparse, io, st)
+                   : sweeps<float>(P, W, k, sparse, io, st);
   if (rc) return rc;
   if (P->refine > 0) {
     rc = ensure(reinterpret_cast<void**>(&P->d_r), &P->r_bytes, static_cast<size_t>(nb) * blk * 8, st);
@@ -1901,6 +1982,28 @@ int dcx_solve(dcx_plan* P, const double* b, void* xv, int k, int flags, void* st
                        x, W, nb * blk);
   HIP_TRY(hipGetLastError());
   return DCX_OK;
+}
+
+int dcx_solve(dcx_plan* P, const double* b, void* xv, int k, int flags, void* stream) {
+  if (!P || !b || !xv) return fail(DCX_ERR_ARG, "null argument");
+  if (!P->factored) return fail(DCX_ERR_STATE, "dcx_solve before dcx_factorize");
+  if (k < 1) return fail(DCX_ERR_ARG, "k must be >= 1");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  HIP_TRY(hipSetDevice(P->device));
+  return solve_impl(P, b, xv, k, flags, st, false);
+}
+
+int dcx_factor_solve(dcx_plan* P, const double* values, const double* b, void* xv, int k,
+                     int flags, void* stream) {
+  if (!P || !values || !b || !xv) return fail(DCX_ERR_ARG, "null argument");
+  if (k < 1) return fail(DCX_ERR_ARG, "k must be >= 1");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  HIP_TRY(hipSetDevice(P->device));
+  const size_t vb = static_cast<size_t>(P->nbatch) * P->S.nnz * sizeof(double);
+  HIP_TRY(hipMemcpyAsync(P->d_vals, values, vb, hipMemcpyDeviceToDevice, st));
+  hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, st, P->d_info);
+  P->factored = true;
+  return solve_impl(P, b, xv, k, flags, st, true);
 }
 
 int dcx_grad_values(dcx_plan* P, const void* y, const void* x, int k,

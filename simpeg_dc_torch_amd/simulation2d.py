@@ -125,6 +125,22 @@ class _GpuState:
                                       self.stream()), "solve")
         return x
 
+    def factor_solve_perm(self, vals, b, k, f32=False):
+        """Factor every wavenumber and solve the shared, sparse right-hand
+        sides b in elimination order, the two overlapped (dcx_factor_solve)."""
+        sv = self.solver
+        x = torch.empty(sv.nbatch, sv.n, k, dtype=torch.float32 if f32 else torch.float64,
+                        device=b.device)
+        flags = _lib.DCX_PERMUTED | _lib.DCX_SHARED_RHS | _lib.DCX_SPARSE_RHS | \
+            (_lib.DCX_F32_FIELDS if f32 else 0)
+        v = vals.contiguous()
+        with sv._lock:
+            _lib.check(self.lib.dcx_factor_solve(sv._h, _lib.ptr(v), _lib.ptr(b), _lib.ptr(x),
+                                                 int(k), flags, self.stream()), "factor_solve")
+            sv._vals_keep = v
+        sv.stamp += 1
+        return x
+
     def dpred(self, U, k, s0, s1, d):
         flags = _lib.DCX_PERMUTED | (_lib.DCX_F32_FIELDS if U.dtype == torch.float32 else 0)
         _lib.check(self.lib.dcx_survey_dpred(self.survey, _lib.ptr(U), int(k),
@@ -165,8 +181,7 @@ class _Forward2D(torch.autograd.Function):
         with torch.cuda.device(eng.device):
             sig = torch.exp(-m_full.detach())
             vals = g.assemble(sig)
-            g.solver.factorize(vals)
-            U = g.solve_perm(g.q_perm, eng.nS, f32=g.fields32)
+            U = g.factor_solve_perm(vals, g.q_perm, eng.nS, f32=g.fields32)
             d = torch.empty(eng.nD, dtype=torch.float64, device=eng.device)
             g.dpred(U, eng.nS, 0, eng.nS, d)
         ctx.eng = eng

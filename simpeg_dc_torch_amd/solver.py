@@ -162,6 +162,29 @@ class HipBatchSolver:
                                            int(flags), self._stream()), "solve")
         return x.to(self.dtype)
 
+    def factor_solve(self, values, b, flags=0):
+        """factorize(values) then solve(b), with the factorisation and the
+        solve's forward sweep overlapped on the device (dcx_factor_solve)."""
+        v = values.detach().to(device=self.device, dtype=torch.float64).contiguous()
+        if v.shape != (self.nbatch, self.col.numel()):
+            v = v.reshape(self.nbatch, self.col.numel())
+        shared = b.dim() == 2
+        bb = b.detach().to(device=self.device, dtype=torch.float64).contiguous()
+        k = bb.shape[-1] if bb.dim() >= 2 else 1
+        if shared:
+            bb = bb.reshape(self.n, k)
+            flags |= _lib.DCX_SHARED_RHS
+        else:
+            bb = bb.reshape(self.nbatch, self.n, k)
+        x = torch.empty(self.nbatch, self.n, k, dtype=torch.float64, device=self.device)
+        with self._lock, torch.cuda.device(self.device):
+            _lib.check(self._lib.dcx_factor_solve(self._h, _lib.ptr(v), _lib.ptr(bb),
+                                                  _lib.ptr(x), int(k), int(flags),
+                                                  self._stream()), "factor_solve")
+            self._vals_keep = v
+        self.stamp += 1
+        return x.to(self.dtype)
+
     def grad_values(self, adj, x, scale=None, permuted=False):
         """-scale[b] sum_s adj[b, row_e, s] x[b, col_e, s] -> (nbatch, nnz) FP64.
         Float32 adj and x are read as they are (FP64 sums)."""
@@ -206,6 +229,11 @@ class HipSolver(HipBatchSolver):
     def solve(self, b):
         shape = b.shape
         x = super().solve(b.reshape(self.n, -1))
+        return x[0].reshape(shape)
+
+    def factor_solve(self, values, b, flags=0):
+        shape = b.shape
+        x = super().factor_solve(values.reshape(1, -1), b.reshape(self.n, -1), flags)
         return x[0].reshape(shape)
 
 
@@ -292,8 +320,11 @@ def _gather_grad(solver, adj, x):
 class _SparseSolveFn(torch.autograd.Function):
     @staticmethod
     def forward(ctx, values, b, solver):
-        solver.factorize(values.detach())
-        x = solver.solve(b.detach())
+        if hasattr(solver, "factor_solve"):
+            x = solver.factor_solve(values.detach(), b.detach())
+        else:
+            solver.factorize(values.detach())
+            x = solver.solve(b.detach())
         ctx.solver = solver
         ctx.dt_vals, ctx.dt_b = values.dtype, b.dtype
         ctx.save_for_backward(x)
@@ -320,8 +351,11 @@ def sparse_solve(values, b, solver):
 class _SparseSolveBatchFn(torch.autograd.Function):
     @staticmethod
     def forward(ctx, values, b, solver):
-        solver.factorize(values.detach())
-        x = solver.solve(b.detach())
+        if hasattr(solver, "factor_solve"):
+            x = solver.factor_solve(values.detach(), b.detach())
+        else:
+            solver.factorize(values.detach())
+            x = solver.solve(b.detach())
         ctx.solver = solver
         ctx.b_shared = b.dim() == 2
         ctx.dt_vals, ctx.dt_b = values.dtype, b.dtype

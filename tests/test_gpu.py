@@ -159,6 +159,33 @@ def test_f32_fields_match_f64_output(refine):
         assert rel(xo, ref) < (1e-5 if refine == 0 else 1e-6)
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_factor_solve_matches_factorize_then_solve(precision):
+    """dcx_factor_solve (factorisation overlapped with the forward sweep on a
+    second stream) gives bit for bit the factorize + solve result, dense and
+    sparse right-hand sides."""
+    _say(f"factor_solve {precision}")
+    A, _ = laplacian_fixture(nx=40, n_rhs=1)
+    n = A.shape[0]
+    rng = np.random.default_rng(5)
+    vals = torch.tensor(np.stack([A.data, 1.3 * A.data, 0.7 * A.data]), device=DEV)
+    b = torch.tensor(rng.standard_normal((n, 33)), device=DEV)
+    rows = np.array([5, 77, 640, 1311])
+    bs = torch.zeros(n, 12, dtype=torch.float64, device=DEV)
+    bs[torch.tensor(rows, device=DEV)] = torch.tensor(rng.standard_normal((4, 12)), device=DEV)
+    slv = _solver(A, 33, nbatch=3, precision=precision, refine=1)
+    slv.set_rhs_support(rows)
+    slv.factorize(vals)
+    x_ref = slv.solve(b)
+    from simpeg_dc_torch_amd import _lib
+    xs_ref = slv.solve(bs, flags=_lib.DCX_SPARSE_RHS)
+    x = slv.factor_solve(vals, b)
+    xs = slv.factor_solve(vals, bs, flags=_lib.DCX_SPARSE_RHS)
+    torch.cuda.synchronize()
+    assert torch.equal(x, x_ref)
+    assert torch.equal(xs, xs_ref)
+
+
 def test_sparse_rhs_support_matches_dense():
     from simpeg_dc_torch_amd import _lib
     A, _ = laplacian_fixture(nx=40, n_rhs=1)
