@@ -155,6 +155,14 @@ int dcx_set_rhs_support(dcx_plan* plan, const int32_t* rows, int nrows);
 int dcx_grad_values(dcx_plan* plan, const void* y, const void* x, int k,
                     const double* scale, double* gv, int flags, void* stream);
 
+/* y[b] = alpha * A_b x[b] for every batch member, A_b the plan's pattern with
+ * the values vals[b] (device, nbatch x nnz, the caller's CSR order); x, y:
+ * device (nbatch x n x k), rows in elimination order with DCX_PERMUTED.
+ * Replaces the per-wavenumber sparse products -A_k(dsigma) U_k of
+ * Linearization2D.Jvec (dctorch/simulation2d.py:689-710). */
+int dcx_spmm(dcx_plan* plan, const double* vals, const double* x, double* y, int k,
+             double alpha, int flags, void* stream);
+
 /* Lane-layout probe of the 16x16x4 matrix instruction (fp32 = 0: FP64):
  * C (16x16, row-major) = A (16x4, row-major) * B (4x16, row-major), one
  * wavefront.  Device pointers.  A test of the instruction layout the sweep
@@ -184,6 +192,20 @@ int dcx_assemble(dcx_lmap* map, const double* sigma, double* values, void* strea
  * sparse-mm backward + exp backward (dctorch/simulation2d.py:392, autograd). */
 int dcx_lmap_transpose(dcx_lmap* map, const double* gv, const double* sigma,
                        double* grad_m, void* stream);
+
+/* Rows of the Jacobian of dpred for r data at once, transposed:
+ * J[c][j] = d dpred_j / d m_c (m = log rho on all cells).  Y: device
+ * (nbatch x n x r), A_b^{-1} applied to the data's projection rows (Pd^T
+ * columns); U: device (nbatch x n x nS), the forward fields; src: device (r)
+ * source of each datum; wq: device (nbatch) quadrature weights; sigma: device
+ * (ncell); J: device (ncell x r).  With DCX_PERMUTED, Y and U rows are in the
+ * plan's elimination order.  The map and the plan must describe the same
+ * systems (nky = nbatch).  Replaces _rows_of_J's pairwise gather and L^T
+ * (dctorch/simulation2d.py:734-776, :808-818) and TorchDC2D.Jmatrix's block
+ * loop (:439-518). */
+int dcx_jacobian_rows(dcx_lmap* map, dcx_plan* plan, const double* Y, int r, const double* U,
+                      int nS, const int32_t* src, const double* wq, const double* sigma,
+                      double* J, int flags, void* stream);
 
 /* ---- receivers and adjoint sources ----------------------------------------
  * Pd (nD x nN, CSR, host) and the source index of each datum.  Replaces the

@@ -13,5 +13,6 @@ from .solver import (CuDSSBatchSolver, CuDSSSolver, HipBatchSolver,  # noqa: F40
                      make_solver, sparse_solve, sparse_solve_batch,
                      sparse_solve_batch_prefactored)
 from .simulation2d import TorchDC2D  # noqa: F401
+from .linearization2d import Linearization2D  # noqa: F401
 from .mesh import TensorMesh  # noqa: F401
 from . import survey  # noqa: F401

@@ -66,6 +66,9 @@ PROTOTYPES = {
     "dcx_factorize": (C.c_int, [_vp, _vp, _vp]),
     "dcx_solve": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, _vp]),
     "dcx_factor_solve": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, C.c_int, _vp]),
+    "dcx_spmm": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, C.c_double, C.c_int, _vp]),
+    "dcx_jacobian_rows": (C.c_int, [_vp, _vp, _vp, C.c_int, _vp, C.c_int, _vp, _vp, _vp, _vp,
+                                    C.c_int, _vp]),
     "dcx_set_rhs_support": (C.c_int, [_vp, _i32p, C.c_int]),
     "dcx_grad_values": (C.c_int, [_vp, _vp, _vp, C.c_int, _vp, _vp, C.c_int, _vp]),
     "dcx_mma_probe": (C.c_int, [C.c_int, _vp, _vp, _vp, _vp]),

@@ -1016,6 +1016,92 @@ __global__ void k_residual(const int* __restrict__ crow, const int* __restrict__
   }
 }
 
+// Y[b] = alpha A_b X[b] (FP64; A_b from vals[b], CSR rows in the storage
+// order of X and Y when vidx maps their entries to the caller's order)
+__global__ void k_spmm(const int* __restrict__ crow, const int* __restrict__ ccol,
+                       const int* __restrict__ vidx, const double* __restrict__ vals,
+                       int64_t nnz, double alpha, const double* __restrict__ X,
+                       double* __restrict__ Y, int n, int k) {
+  const int row = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+  const int b = blockIdx.y;
+  if (row >= n) return;
+  const int64_t xb = static_cast<int64_t>(b) * n * k;
+  const double* vb = vals + b * nnz;
+  const int p0 = crow[row], p1 = crow[row + 1];
+  for (int c = threadIdx.x & 63; c < k; c += 64) {
+    double acc = 0.0;
+    for (int p = p0; p < p1; ++p)
+      acc += vb[vidx ? vidx[p] : p] * X[xb + static_cast<int64_t>(ccol[p]) * k + c];
+    Y[xb + static_cast<int64_t>(row) * k + c] = alpha * acc;
+  }
+}
+
+// Rows of the Jacobian, r data at once (datum j: adjoint field Y[.][.][j],
+// forward field of its source U[.][.][src[j]]):
+//   gv[b][e][j] = -wq[b] Y[b][row_e][j] U[b][col_e][src[j]]
+// reduced over the wavenumbers straight away: ge = sum_b gv, gn = sum_b ky_b^2 gv,
+// and for each boundary entry t of e, gb[t] = sum_b bcoef[b][t] gv.  One
+// wavefront per row of the caller's pattern, lanes over the data.  iperm
+// (may be null) maps rows to the storage order of Y and U.
+constexpr int kMaxBcPerEntry = 4;
+__global__ void k_jpair(int n, const int* __restrict__ crow, const int* __restrict__ ccol,
+                        const int* __restrict__ iperm, const double* __restrict__ Y, int r,
+                        const double* __restrict__ U, int nS, const int* __restrict__ src,
+                        const double* __restrict__ wq, int nb, const double* __restrict__ ky2,
+                        const int* __restrict__ bptr, const double* __restrict__ bcoef, int nbc,
+                        double* __restrict__ ge, double* __restrict__ gn,
+                        double* __restrict__ gb) {
+  const int row = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+  const int lane = threadIdx.x & 63;
+  if (row >= n) return;
+  const int prow = iperm ? iperm[row] : row;
+  for (int j = lane; j < r; j += 64) {
+    const int sj = src[j];
+    for (int e = crow[row]; e < crow[row + 1]; ++e) {
+      const int pc = iperm ? iperm[ccol[e]] : ccol[e];
+      const int t0 = bptr[e], nt = min(bptr[e + 1] - t0, kMaxBcPerEntry);
+      double a = 0.0, c = 0.0, bq[kMaxBcPerEntry] = {0.0, 0.0, 0.0, 0.0};
+      for (int b = 0; b < nb; ++b) {
+        const double p = -wq[b] * Y[(static_cast<int64_t>(b) * n + prow) * r + j] *
+                         U[(static_cast<int64_t>(b) * n + pc) * nS + sj];
+        a += p;
+        c += ky2[b] * p;
+#pragma unroll
+        for (int q = 0; q < kMaxBcPerEntry; ++q)
+          if (q < nt) bq[q] += bcoef[static_cast<int64_t>(b) * nbc + t0 + q] * p;
+      }
+      ge[static_cast<int64_t>(e) * r + j] = a;
+      gn[static_cast<int64_t>(e) * r + j] = c;
+#pragma unroll
+      for (int q = 0; q < kMaxBcPerEntry; ++q)
+        if (q < nt) gb[static_cast<int64_t>(t0 + q) * r + j] = bq[q];
+    }
+  }
+}
+
+// J[c][j] = -sigma[c] (sum over the cell's map entries of le ge + ln gn, plus
+// its boundary entries' gb); one wavefront per cell, lanes over the data
+__global__ void k_jmap(int ncell, int r, const int* __restrict__ tptr,
+                       const int* __restrict__ tent, const int* __restrict__ te,
+                       const double* __restrict__ le, const double* __restrict__ ln,
+                       const int* __restrict__ tbptr, const int* __restrict__ tbq,
+                       const double* __restrict__ ge, const double* __restrict__ gn,
+                       const double* __restrict__ gb, const double* __restrict__ sigma,
+                       double* __restrict__ J) {
+  const int c = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+  const int lane = threadIdx.x & 63;
+  if (c >= ncell) return;
+  for (int j = lane; j < r; j += 64) {
+    double acc = 0.0;
+    for (int t = tptr[c]; t < tptr[c + 1]; ++t) {
+      const int64_t e = te[t];
+      acc += le[tent[t]] * ge[e * r + j] + ln[tent[t]] * gn[e * r + j];
+    }
+    for (int t = tbptr[c]; t < tbptr[c + 1]; ++t) acc += gb[static_cast<int64_t>(tbq[t]) * r + j];
+    J[static_cast<int64_t>(c) * r + j] = -sigma[c] * acc;
+  }
+}
+
 // gv[b][e] = -scale[b] sum_c Y[b][row][c] X[b][col_e][c]; one wavefront per
 // row of the caller's pattern, rows taken in the pattern's own (geometric)
 // order so that neighbouring rows share their X rows in L2.  The hardware
@@ -2036,6 +2122,26 @@ int dcx_grad_values(dcx_plan* P, const void* y, const void* x, int k,
   return DCX_OK;
 }
 
+int dcx_spmm(dcx_plan* P, const double* vals, const double* x, double* y, int k, double alpha,
+             int flags, void* stream) {
+  if (!P || !vals || !x || !y) return fail(DCX_ERR_ARG, "null argument");
+  if (k < 1) return fail(DCX_ERR_ARG, "k must be >= 1");
+  if (x == y) return fail(DCX_ERR_ARG, "x and y must not alias");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  HIP_TRY(hipSetDevice(P->device));
+  const auto& S = P->S;
+  dim3 grid(grid1(S.n, 4), P->nbatch);
+  if (flags & DCX_PERMUTED)
+    hipLaunchKernelGGL(k_spmm, grid, dim3(256), 0, st, P->d_pcrow, P->d_pcol, P->d_pvidx, vals,
+                       static_cast<int64_t>(S.nnz), alpha, x, y, S.n, k);
+  else
+    hipLaunchKernelGGL(k_spmm, grid, dim3(256), 0, st, P->d_crow, P->d_col,
+                       static_cast<const int*>(nullptr), vals, static_cast<int64_t>(S.nnz), alpha,
+                       x, y, S.n, k);
+  HIP_TRY(hipGetLastError());
+  return DCX_OK;
+}
+
 int dcx_mma_probe(int fp32, const void* A, const void* B, void* C, void* stream) {
   if (!A || !B || !C) return fail(DCX_ERR_ARG, "null argument");
   hipStream_t st = static_cast<hipStream_t>(stream);
@@ -2078,9 +2184,12 @@ struct dcx_lmap {
   double* d_bcoef = nullptr;
   int *d_tptr = nullptr, *d_tent = nullptr, *d_te = nullptr, *d_tbptr = nullptr, *d_tbq = nullptr;
   double *d_ge = nullptr, *d_gn = nullptr;
+  int max_bc_per_entry = 0;
+  double* d_jws = nullptr;  // Jacobian-rows workspace, grown on demand
+  size_t jws_bytes = 0;
   ~dcx_lmap() {
     void* ptrs[] = {d_ky2, d_ptr, d_cell, d_le, d_ln, d_bptr, d_bcell, d_bc_e, d_bcoef,
-                    d_tptr, d_tent, d_te, d_tbptr, d_tbq, d_ge, d_gn};
+                    d_tptr, d_tent, d_te, d_tbptr, d_tbq, d_ge, d_gn, d_jws};
     for (void* p : ptrs)
       if (p) (void)hipFree(p);
   }
@@ -2112,6 +2221,7 @@ int dcx_lmap_create(int nnz, int ncell, int nky, const double* ky2, const int32_
         return fail(DCX_ERR_ARG, "bc entry out of range");
       bptr[bc_e[q] + 1]++;
     }
+    for (int e = 0; e < nnz; ++e) M->max_bc_per_entry = std::max(M->max_bc_per_entry, bptr[e + 1]);
     for (int e = 0; e < nnz; ++e) bptr[e + 1] += bptr[e];
     {
       std::vector<int> fill(bptr.begin(), bptr.end() - 1);
@@ -2199,6 +2309,42 @@ int dcx_lmap_transpose(dcx_lmap* M, const double* gv, const double* sigma, doubl
                      M->nky, M->d_tptr, M->d_tent, M->d_te, M->d_le, M->d_ln, M->d_tbptr,
                      M->d_tbq, M->d_bc_e, M->d_bcoef, M->nbc, M->d_ge, M->d_gn, gv, sigma,
                      grad_m);
+  HIP_TRY(hipGetLastError());
+  return DCX_OK;
+}
+
+int dcx_jacobian_rows(dcx_lmap* M, dcx_plan* P, const double* Y, int r, const double* U, int nS,
+                      const int32_t* src, const double* wq, const double* sigma, double* J,
+                      int flags, void* stream) {
+  if (!M || !P || !Y || !U || !src || !wq || !sigma || !J)
+    return fail(DCX_ERR_ARG, "null argument");
+  if (r < 1 || nS < 1) return fail(DCX_ERR_ARG, "r and nS must be >= 1");
+  if (P->nbatch != M->nky || P->S.nnz != M->nnz)
+    return fail(DCX_ERR_ARG, "map and plan do not describe the same systems");
+  if (M->max_bc_per_entry > kMaxBcPerEntry)
+    return fail(DCX_ERR_ARG, "more boundary terms per entry than k_jpair holds");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  HIP_TRY(hipSetDevice(M->device));
+  const size_t need = (2 * static_cast<size_t>(M->nnz) + std::max(M->nbc, 1)) * r * sizeof(double);
+  if (M->jws_bytes < need) {
+    HIP_TRY(hipStreamSynchronize(st));
+    if (M->d_jws) HIP_TRY(hipFree(M->d_jws));
+    M->d_jws = nullptr;
+    M->jws_bytes = 0;
+    HIP_TRY(hipMalloc(&M->d_jws, need));
+    M->jws_bytes = need;
+  }
+  double* ge = M->d_jws;
+  double* gn = ge + static_cast<size_t>(M->nnz) * r;
+  double* gb = gn + static_cast<size_t>(M->nnz) * r;
+  const int n = P->S.n;
+  hipLaunchKernelGGL(k_jpair, dim3(grid1(n, 4)), dim3(256), 0, st, n, P->d_crow, P->d_col,
+                     (flags & DCX_PERMUTED) ? static_cast<const int*>(P->d_iperm) : nullptr, Y,
+                     r, U, nS, src, wq, M->nky, M->d_ky2, M->d_bptr, M->d_bcoef, M->nbc, ge, gn,
+                     gb);
+  hipLaunchKernelGGL(k_jmap, dim3(grid1(M->ncell, 4)), dim3(256), 0, st, M->ncell, r, M->d_tptr,
+                     M->d_tent, M->d_te, M->d_le, M->d_ln, M->d_tbptr, M->d_tbq, ge, gn, gb,
+                     sigma, J);
   HIP_TRY(hipGetLastError());
   return DCX_OK;
 }

@@ -40,6 +40,7 @@ from . import _lib
 from .operators2d import Problem2DMaps
 from .quadrature import (amplification, electrode_distance_range, k0_quadrature,
                          simpeg_quadrature, transform_error)
+from .linearization2d import Linearization2D
 from .solver import (HipBatchSolver, make_batch_solver, sparse_solve_batch,
                      sparse_solve_batch_prefactored)
 
@@ -491,37 +492,24 @@ class TorchDC2D:
         adjoint: the adjoint right-hand sides are the columns of Pd^T)."""
         block = max(1, min(int(block), self.nD))
         dev0, dt0 = m.device, m.dtype
+        if self._gpu is not None:
+            J = Linearization2D(self, m, store_fields=True).Jmatrix(sub=block)
+            return J.to(device=dev0, dtype=dt0)
         Lt = self.Lstack.t().coalesce()
         with torch.no_grad():
             m_full = self._model_to_full(m).to(torch.float64)
             sig = torch.exp(-m_full)
-            if self._gpu is not None:
-                g = self._gpu
-                with torch.cuda.device(self.device):
-                    vals = g.assemble(sig)
-                    g.solver.factorize(vals)
-                    U = g.solve_perm(g.q_perm, self.nS)
-                    PdT = torch.tensor(self._maps.Pd.T.tocsr()[self.solver.perm()].toarray(),
-                                       dtype=torch.float64, device=self.device)
-                    # pattern entries in elimination numbering
-                    perm_inv = torch.empty_like(g.perm)
-                    perm_inv[g.perm] = torch.arange(self.nN, device=self.device)
-                    row = perm_inv[self.solver.row]
-                    col = perm_inv[self.solver.col.long()]
-                    solve = lambda rhs: g.solve_perm(rhs, rhs.shape[1], sparse=True)
-            else:
-                solver = self._solver_for(self.nS)
-                VALS = torch.sparse.mm(self.Lstack, sig.unsqueeze(1)).view(self.nky, self.nnzA)
-                solver.factorize(VALS.to(self.dtype))
-                U = solver.solve(self.q.to(self.dtype)).to(torch.float64)
-                PdT = self.Pd.t().coalesce().to_dense().to(torch.float64)
-                row, col = solver.row.long(), solver.col.long()
-                solve = lambda rhs: solver.solve(rhs.to(self.dtype)).to(torch.float64)
+            solver = self._solver_for(self.nS)
+            VALS = torch.sparse.mm(self.Lstack, sig.unsqueeze(1)).view(self.nky, self.nnzA)
+            solver.factorize(VALS.to(self.dtype))
+            U = solver.solve(self.q.to(self.dtype)).to(torch.float64)
+            PdT = self.Pd.t().coalesce().to_dense().to(torch.float64)
+            row, col = solver.row.long(), solver.col.long()
             wq = self.wq.to(torch.float64)
             Jm = torch.empty(self.nC, self.nD, dtype=torch.float64, device=self.device)
             for a in range(0, self.nD, block):
                 b = min(a + block, self.nD)
-                Y = solve(PdT[:, a:b].contiguous())
+                Y = solver.solve(PdT[:, a:b].contiguous().to(self.dtype)).to(torch.float64)
                 cols = self.sidx[a:b]
                 per_ky = [(-wq[k]) * Y[k][row] * U[k][:, cols][col] for k in range(self.nky)]
                 Jm[:, a:b] = torch.sparse.mm(Lt.to(torch.float64), torch.cat(per_ky, 0))
@@ -529,47 +517,13 @@ class TorchDC2D:
             Jm = self._model_from_full(Jm)
             return Jm.T.contiguous().to(device=dev0, dtype=dt0)
 
+    def linearize(self, m, store_fields=None, fields_max_gib=2.0):
+        """The forward frozen at m: one factorisation, the fields kept, and
+        J v / J^T v as one batched solve each (Linearization2D)."""
+        return Linearization2D(self, m, store_fields=store_fields,
+                               fields_max_gib=fields_max_gib)
+
     def Jvec(self, m, v):
-        """J v (nD,) by finite-free linearisation: one forward + one extra solve."""
-        with torch.no_grad():
-            m_full = self._model_to_full(m).to(torch.float64)
-            sig = torch.exp(-m_full)
-            dm = self._model_to_full(v).to(torch.float64)
-            if self._P_inj is not None:
-                dm = dm - self._v_inj.to(torch.float64)
-            dsig = -sig * dm
-            if self._gpu is not None:
-                g = self._gpu
-                with torch.cuda.device(self.device):
-                    vals = g.assemble(sig)
-                    dvals = g.assemble(dsig)
-                    g.solver.factorize(vals)
-                    U = g.solve_perm(g.q_perm, self.nS)
-                    rhs = torch.empty_like(U)
-                    pinv = torch.empty_like(g.perm)
-                    pinv[g.perm] = torch.arange(self.nN, device=self.device)
-                    crow = self._crow.to(self.device, torch.int64)
-                    colp = pinv[self._ccol.to(self.device).long()]
-                    rowp = pinv[self.solver.row]
-                    for k in range(self.nky):
-                        A = torch.sparse_coo_tensor(torch.stack([rowp, colp]), -dvals[k],
-                                                    (self.nN, self.nN))
-                        rhs[k] = torch.sparse.mm(A, U[k])
-                    del crow
-                    dU = g.solve_perm(rhs, self.nS, sparse=False, shared=False)
-                    d = torch.empty(self.nD, dtype=torch.float64, device=self.device)
-                    g.dpred(dU, self.nS, 0, self.nS, d)
-                    return d.to(m.device, m.dtype)
-            VALS = torch.sparse.mm(self.Lstack, sig.unsqueeze(1)).view(self.nky, self.nnzA)
-            dV = torch.sparse.mm(self.Lstack, dsig.unsqueeze(1)).view(self.nky, self.nnzA)
-            solver = self._solver_for(self.nS)
-            solver.factorize(VALS)
-            U = solver.solve(self.q)
-            rhs = torch.empty_like(U)
-            for k in range(self.nky):
-                A = torch.sparse_csr_tensor(self._crow.long(), self._ccol.long(), -dV[k],
-                                            (self.nN, self.nN))
-                rhs[k] = A @ U[k]
-            dU = solver.solve(rhs)
-            Ubar = (self.wq.view(-1, 1, 1) * dU).sum(0)
-            return torch.sparse.mm(self.Pd, Ubar)[self.iD, self.sidx].to(m.device, m.dtype)
+        """J v for J = d(dpred)/dm without forming J: one forward and one extra
+        solve.  For many products at one model, `linearize` once."""
+        return self.linearize(m).Jvec(v)

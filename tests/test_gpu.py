@@ -265,6 +265,33 @@ def test_engine_jtvec_jmatrix_chunks():
     assert rel(eng.Jvec(mt, torch.tensor(u, device=DEV)).cpu().numpy(), J @ u) < 1e-9
 
 
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-9), ("mixed", 1e-5)])
+def test_linearization_products(precision, tol):
+    """Linearization2D on the GPU (dcx_spmm, dcx_jacobian_rows): J v, J^T v,
+    diag(J^T D J) and the explicit J against the oracle's Jacobian; the
+    adjoint identity u.(J^T v) = (J u).v."""
+    from simpeg_dc_torch_amd import Linearization2D
+    _say(f"linearization suite {precision}")
+    eng, O, m = _engine("suite", nky=4, precision=precision, refine=0)
+    mt = torch.tensor(m, device=DEV)
+    lin = Linearization2D(eng, mt)
+    J = O.Jmatrix(m)
+    assert rel(lin.d.cpu().numpy(), O.dpred(m)) < tol
+    rng = np.random.default_rng(3)
+    u = rng.standard_normal(m.size)
+    v = rng.standard_normal(eng.nD)
+    Ju = lin.Jvec(torch.tensor(u, device=DEV)).cpu().numpy()
+    Jtv = lin.Jtvec(torch.tensor(v, device=DEV)).cpu().numpy()
+    assert rel(Ju, J @ u) < tol
+    assert rel(Jtv, J.T @ v) < tol
+    assert abs(Ju @ v - u @ Jtv) <= 10 * tol * abs(Ju @ v)
+    dw = rng.random(eng.nD)
+    dd = lin.diag_JtDJ(torch.tensor(dw, device=DEV), sub=33).cpu().numpy()
+    assert rel(dd, (J ** 2 * dw[:, None]).sum(0)) < tol
+    assert rel(lin.Jmatrix(sub=64).cpu().numpy(), J) < tol
+    assert rel(eng.Jmatrix(mt, block=7).cpu().numpy(), J) < tol
+
+
 def test_line25k_subset_against_oracle_and_fd():
     """Full 25k-cell line: dpred of a subset of sources against the oracle
     (same mesh and quadrature), and the gradient against central differences

@@ -193,3 +193,26 @@ def test_cpu_engine_jtvec_jmatrix_jvec():
     assert rel(eng.Jtvec(mt, torch.tensor(v), chunk=4).numpy(), J.T @ v) < 1e-10
     u = rng.standard_normal(m.size)
     assert rel(eng.Jvec(mt, torch.tensor(u)).numpy(), J @ u) < 1e-9
+
+
+def test_cpu_linearization_products():
+    """Linearization2D on the cpu engine: J v, J^T v, diag(J^T D J) and J
+    against the oracle's operator-form Jacobian; the adjoint identity
+    (dctorch/tests/test_linearization.py:48-64)."""
+    from simpeg_dc_torch_amd import Linearization2D
+    eng, O, m = _engine_and_oracle("suite", nky=4)
+    mt = torch.tensor(m)
+    lin = Linearization2D(eng, mt)
+    J = O.Jmatrix(m)
+    assert rel(lin.d.numpy(), O.dpred(m)) < 1e-10
+    rng = np.random.default_rng(3)
+    u = rng.standard_normal(m.size)
+    v = rng.standard_normal(eng.nD)
+    Ju = lin.Jvec(torch.tensor(u)).numpy()
+    Jtv = lin.Jtvec(torch.tensor(v)).numpy()
+    assert rel(Ju, J @ u) < 1e-9
+    assert rel(Jtv, J.T @ v) < 1e-9
+    assert abs(Ju @ v - u @ Jtv) <= 1e-11 * abs(Ju @ v)
+    dw = rng.random(eng.nD)
+    assert rel(lin.diag_JtDJ(torch.tensor(dw), sub=5).numpy(), (J ** 2 * dw[:, None]).sum(0)) < 1e-9
+    assert rel(lin.Jmatrix(sub=7).numpy(), J) < 1e-9
