@@ -121,6 +121,36 @@ def measured_traffic(kernel):
         return None
 
 
+def linearization_times(eng, mt, reps=3):
+    """Gauss-Newton building blocks on the FP64 engine at m: the linearisation
+    (factorisation + forward fields), J v, J^T v, the explicit J and
+    diag(J^T D J), each the median of `reps` runs."""
+    from simpeg_dc_torch_amd import Linearization2D
+
+    def med(fn):
+        ts = []
+        for _ in range(reps):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            out = fn()
+            torch.cuda.synchronize()
+            ts.append(1e3 * (time.perf_counter() - t0))
+        return sorted(ts)[len(ts) // 2], out
+
+    t_lin, lin = med(lambda: Linearization2D(eng, mt))
+    rng = np.random.default_rng(5)
+    u = torch.tensor(rng.standard_normal(eng.nC), device="cuda:0")
+    v = torch.tensor(rng.standard_normal(eng.nD), device="cuda:0")
+    t_jv, _ = med(lambda: lin.Jvec(u))
+    t_jtv, _ = med(lambda: lin.Jtvec(v))
+    t_j, J = med(lambda: lin.Jmatrix())
+    t_dg, _ = med(lambda: lin.diag_JtDJ(torch.ones_like(v)))
+    return {"linearize_ms": round(t_lin, 2), "jvec_ms": round(t_jv, 2),
+            "jtvec_ms": round(t_jtv, 2), "jmatrix_ms": round(t_j, 2),
+            "diag_jtdj_ms": round(t_dg, 2), "J_shape": list(J.shape),
+            "precision": "fp64"}
+
+
 def cpu_baseline(mesh, survey, m, kys, wq, sample_ky=1):
     """The oracle's SuperLU path (reference CPU solver semantics) on a sample
     of `sample_ky` wavenumbers, scaled to all of them."""
@@ -200,6 +230,9 @@ def main():
     roof64 = sweep_roofline(e64, mt, dobs, w, 0)
     roof["traffic"] = measured_traffic("k_bwd_wave<float>")
     roof64["traffic"] = measured_traffic("k_bwd_wave<double>")
+    log("linearisation (FP64): J v, J^T v, J, diag(J^T D J)")
+    lin = linearization_times(e64, mt)
+    log(f"linearisation {lin}")
     cpu = None
     if not args.no_cpu:
         log("cpu baseline (oracle SuperLU, 1 wavenumber sample)")
@@ -223,6 +256,7 @@ def main():
         "mixed_dpred_rel_err": err_d, "mixed_grad_rel_err": err_g,
         "build_s": round(t_build, 2),
         "roofline": roof, "roofline_fp64": roof64,
+        "linearization": lin,
         "cpu_baseline": cpu,
     }
     if cpu:
