@@ -14,5 +14,6 @@ from .solver import (CuDSSBatchSolver, CuDSSSolver, HipBatchSolver,  # noqa: F40
                      sparse_solve_batch_prefactored)
 from .simulation2d import TorchDC2D  # noqa: F401
 from .linearization2d import Linearization2D  # noqa: F401
+from .simulation3d import TorchDC3D  # noqa: F401
 from .mesh import TensorMesh  # noqa: F401
 from . import survey  # noqa: F401

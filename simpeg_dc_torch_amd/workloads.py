@@ -155,5 +155,75 @@ def line_25k_case(seed=1):
     return mesh, survey, m, rng
 
 
+# ---------------------------------------------------------------------------
+# 3-D surveys (pole sources, dipole receivers on a surface electrode grid)
+# ---------------------------------------------------------------------------
+def surface_grid(nx_e, ny_e, spacing, x0=None, y0=None, z=0.0):
+    """(n, 3) electrode positions on a surface grid, x fastest, centred."""
+    xs = (np.arange(nx_e) - (nx_e - 1) / 2.0) * spacing if x0 is None else x0 + spacing * np.arange(nx_e)
+    ys = (np.arange(ny_e) - (ny_e - 1) / 2.0) * spacing if y0 is None else y0 + spacing * np.arange(ny_e)
+    X, Y = np.meshgrid(xs, ys)
+    return np.c_[X.ravel(), Y.ravel(), np.full(X.size, z)]
+
+
+def pole_dipole_survey(sources, electrodes, dipoles, n_rx):
+    """One pole source per row of `sources`; its receivers are the `n_rx`
+    dipoles (index pairs into `electrodes`) whose midpoints lie nearest to
+    it, skipping dipoles that touch the source position."""
+    from .survey import DipoleReceiver, PoleSource, Survey
+    e = np.asarray(electrodes, float)
+    dp = np.asarray(dipoles, int)
+    mid = 0.5 * (e[dp[:, 0]] + e[dp[:, 1]])
+    out = []
+    for a in np.asarray(sources, float):
+        ok = (np.linalg.norm(e[dp[:, 0]] - a, axis=1) > 1e-9) & \
+             (np.linalg.norm(e[dp[:, 1]] - a, axis=1) > 1e-9)
+        cand = np.where(ok)[0]
+        cand = cand[np.argsort(np.linalg.norm(mid[cand] - a, axis=1), kind="stable")][:n_rx]
+        rx = DipoleReceiver(e[dp[cand, 0]], e[dp[cand, 1]])
+        out.append(PoleSource([rx], a))
+    return Survey(out)
+
+
+def x_dipoles(nx_e, ny_e):
+    """Adjacent electrode pairs along x of a surface_grid."""
+    return np.array([(j * nx_e + i, j * nx_e + i + 1)
+                     for j in range(ny_e) for i in range(nx_e - 1)])
+
+
+def small_3d_case(seed=0):
+    """Small 3-D problem for parity tests: 10 x 10 x 5 core of 1 m cells, 3
+    padding cells growing 1.3x; a 5 x 3 electrode grid at 2 m, every
+    electrode a pole source read by its 6 nearest x-dipoles; m = log 100 plus
+    a seeded perturbation."""
+    mesh = TensorMesh([[(1.0, 3, -1.3), (1.0, 10), (1.0, 3, 1.3)],
+                       [(1.0, 3, -1.3), (1.0, 10), (1.0, 3, 1.3)],
+                       [(1.0, 3, -1.3), (1.0, 5)]], origin="CCN")
+    el = surface_grid(5, 3, 2.0)
+    survey = pole_dipole_survey(el, el, x_dipoles(5, 3), 6)
+    rng = np.random.default_rng(seed)
+    m = np.log(100.0) + 0.3 * rng.standard_normal(mesh.nC)
+    return mesh, survey, m
+
+
+def grid_3d_case(seed=3):
+    """The 3-D L-BFGS workload: 40 x 40 x 30 core of 5 m cells plus 8
+    padding cells per side growing 1.3x (56 x 56 x 38 = 119,168 cells); 64
+    surface electrodes on an 8 x 8 grid at 10 m; pole-dipole with 200 source
+    positions (a 20 x 10 grid at 5 m over the core) each read by its 30
+    nearest x-dipoles of the electrode grid: 6,000 data; m = log(rho) a 3-D
+    Gaussian random field (mean log 100, std 0.8, correlation length 25 m)."""
+    pad = (5.0, 8, 1.3)
+    mesh = TensorMesh([[(pad[0], pad[1], -pad[2]), (5.0, 40), pad],
+                       [(pad[0], pad[1], -pad[2]), (5.0, 40), pad],
+                       [(pad[0], pad[1], -pad[2]), (5.0, 30)]], origin="CCN")
+    el = surface_grid(8, 8, 10.0)
+    src = surface_grid(20, 10, 5.0)
+    survey = pole_dipole_survey(src, el, x_dipoles(8, 8), 30)
+    rng = np.random.default_rng(seed)
+    m = gaussian_random_field(mesh, rng, corr_len=25.0, std=0.8, dx=2.5)
+    return mesh, survey, m
+
+
 CASES = {"regression": regression_case, "suite": suite_case,
-         "small_padded": small_padded_case}
+         "small_padded": small_padded_case, "small_3d": small_3d_case}

@@ -292,6 +292,27 @@ def test_linearization_products(precision, tol):
     assert rel(eng.Jmatrix(mt, block=7).cpu().numpy(), J) < tol
 
 
+@pytest.mark.parametrize("precision,tol_d,tol_g", [("fp64", 1e-10, 1e-9), ("mixed", 1e-6, 1e-4)])
+def test_engine3d_against_oracle(precision, tol_d, tol_g):
+    """TorchDC3D on the GPU (the 2.5-D kernels with one zero wavenumber)
+    against the 3-D oracle: dpred, misfit and gradient."""
+    from oracle.dc3d import Problem3D
+    from simpeg_dc_torch_amd import TorchDC3D
+    _say(f"engine 3-D small {precision}")
+    mesh, survey, m = case("small_3d")
+    eng = TorchDC3D(mesh, survey, device=DEV, precision=precision, refine=0)
+    O = Problem3D(mesh.h[0], mesh.h[1], mesh.h[2], mesh.origin, survey)
+    mt = torch.tensor(m, device=DEV)
+    d_o = O.dpred(m)
+    assert rel(eng.dpred(mt).cpu().numpy(), d_o) < tol_d
+    dobs, w = noisy_data(d_o)
+    phi, g = eng.misfit_and_grad(mt, torch.tensor(dobs, device=DEV),
+                                 torch.tensor(w, device=DEV))
+    phi_o, g_o, _ = O.misfit_and_grad(m, dobs, w)
+    assert float(phi) == pytest.approx(phi_o, rel=max(tol_d, 1e-12) * 100)
+    assert rel(g.cpu().numpy(), g_o) < tol_g
+
+
 def test_line25k_subset_against_oracle_and_fd():
     """Full 25k-cell line: dpred of a subset of sources against the oracle
     (same mesh and quadrature), and the gradient against central differences

@@ -216,3 +216,89 @@ def test_cpu_linearization_products():
     dw = rng.random(eng.nD)
     assert rel(lin.diag_JtDJ(torch.tensor(dw), sub=5).numpy(), (J ** 2 * dw[:, None]).sum(0)) < 1e-9
     assert rel(lin.Jmatrix(sub=7).numpy(), J) < 1e-9
+
+
+# ---------------------------------------------------------------------------
+# 3-D
+# ---------------------------------------------------------------------------
+def _oracle3d(mesh, survey):
+    from oracle.dc3d import Problem3D
+    return Problem3D(mesh.h[0], mesh.h[1], mesh.h[2], mesh.origin, survey)
+
+
+def test_3d_assembly_map_matches_oracle():
+    """Factored 3-D map == operator-form assembly at two models
+    (dctorch/simulation3d.py:132-140 validates its map the same way)."""
+    from simpeg_dc_torch_amd.operators3d import Problem3DMaps
+    mesh, survey, m = case("small_3d")
+    P = Problem3DMaps(mesh, survey)
+    O = _oracle3d(mesh, survey)
+    rng = np.random.default_rng(7)
+    for mm in (m, m + 0.5 * rng.standard_normal(m.size)):
+        sig = np.exp(-mm)
+        A = O.A(sig)
+        assert np.array_equal(A.indptr, P.crow) and np.array_equal(A.indices, P.col)
+        assert np.abs(A.data - P.values(sig)[0]).max() <= 1e-12 * np.abs(A.data).max()
+    assert abs(P.q - sp.csr_matrix(O.q)).max() == 0.0
+    assert abs(P.Pd - O.Pd).max() == 0.0
+    assert np.array_equal(P.sidx, O.sidx)
+
+
+def test_3d_oracle_closed_form_stencil():
+    """Uniform unit cells, sigma = 1: interior node 6 / -1; free-surface node
+    3, -1/2 along the surface, -1 downwards; Grad-term rows sum to zero."""
+    from simpeg_dc_torch_amd.mesh import TensorMesh
+    from simpeg_dc_torch_amd import workloads
+    mesh = TensorMesh([[(1.0, 4)], [(1.0, 4)], [(1.0, 4)]], origin="000")
+    el = workloads.surface_grid(3, 1, 1.0, x0=1.0, y0=1.0, z=4.0)
+    survey = workloads.pole_dipole_survey(el[:1], el, np.array([[1, 2]]), 1)
+    O = _oracle3d(mesh, survey)
+    K = (O.Grad.T @ sp.diags(O.Qe_ @ np.ones(mesh.nC)) @ O.Grad).tocsr()
+    g = O.g
+    c = g.node(2, 2, 2)
+    assert K[c, c] == pytest.approx(6.0) and K[c, c + 1] == pytest.approx(-1.0)
+    t = g.node(2, 2, 4)
+    assert K[t, t] == pytest.approx(3.0)
+    assert K[t, g.node(3, 2, 4)] == pytest.approx(-0.5)
+    assert K[t, g.node(2, 2, 3)] == pytest.approx(-1.0)
+    assert np.abs(np.asarray(K.sum(1))).max() < 1e-12
+    assert abs(K - K.T).max() == 0.0
+
+
+def test_3d_oracle_halfspace_pole_converges():
+    """Pole source on a half space: the surface potential approaches
+    rho/(2 pi r) as the cells halve (dctorch/tests/test_singularity.py:49-51
+    pins the 2-D analogue)."""
+    from simpeg_dc_torch_amd.mesh import TensorMesh
+    from simpeg_dc_torch_amd.survey import PoleReceiver, PoleSource, Survey
+    rs = np.array([4.0, 6.0])
+    errs = []
+    for h in (2.0, 1.0):
+        ncore = (int(24 / h), int(24 / h), int(12 / h))
+        hx = [(h, 10, -1.5), (h, ncore[0]), (h, 10, 1.5)]
+        hz = [(h, 10, -1.5), (h, ncore[2])]
+        mesh = TensorMesh([hx, hx, hz], origin="CCN")
+        rx = PoleReceiver(np.c_[rs, np.zeros(2), np.zeros(2)])
+        survey = Survey([PoleSource([rx], np.zeros(3))])
+        O = _oracle3d(mesh, survey)
+        d = O.dpred(np.full(mesh.nC, np.log(100.0)))
+        errs.append(np.abs(d / (100.0 / (2 * np.pi * rs)) - 1))
+    assert errs[1][0] < errs[0][0] / 3
+    assert errs[1].max() < 0.02
+
+
+def test_3d_cpu_engine_vs_oracle():
+    from simpeg_dc_torch_amd import TorchDC3D
+    mesh, survey, m = case("small_3d")
+    eng = TorchDC3D(mesh, survey, device="cpu")
+    O = _oracle3d(mesh, survey)
+    mt = torch.tensor(m)
+    d_o = O.dpred(m)
+    assert rel(eng.dpred(mt).numpy(), d_o) < 1e-10
+    dobs, w = noisy_data(d_o)
+    phi, g = eng.misfit_and_grad(mt, torch.tensor(dobs), torch.tensor(w))
+    phi_o, g_o, _ = O.misfit_and_grad(m, dobs, w)
+    assert float(phi) == pytest.approx(phi_o, rel=1e-10)
+    assert rel(g.numpy(), g_o) < 1e-9
+    v = np.random.default_rng(1).standard_normal(eng.nD)
+    assert rel(eng.Jtvec(mt, torch.tensor(v)).numpy(), O.Jtvec(m, v)) < 1e-9
