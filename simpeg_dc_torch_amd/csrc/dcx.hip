@@ -126,6 +126,157 @@ __device__ inline double lane_value(double v, int l) {
                               static_cast<unsigned int>(lo));
 }
 
+// zero the front, seed the E rows with the identity, scatter the A entries
+// and extend-add the children's update blocks (lower triangles; relative
+// positions staged in the LDS array rpl).  Global gathers take kGather
+// entries per thread at a time, loads first, so that their latencies overlap.
+constexpr int kGather = 16;
+template <typename T, int NT>
+__device__ void front_init(const SupDev& S, int b, const SupDev* __restrict__ sup,
+                           const int* __restrict__ relpos, const int* __restrict__ cidx,
+                           const int* __restrict__ a_e, const int* __restrict__ a_row,
+                           const int* __restrict__ a_col, const double* __restrict__ vals,
+                           int64_t nnz, T* __restrict__ F, const T* __restrict__ fprev,
+                           int64_t fprev_bs, int* rpl) {
+  constexpr int kU = kGather;
+  const int tid = threadIdx.x;
+  const int ns = S.ns, nf = S.ns + S.rs, ld = nf + S.ns;
+  const int64_t fsz = static_cast<int64_t>(ld) * nf;
+  for (int64_t i = tid; i < fsz; i += NT) F[i] = T(0);
+  __syncthreads();
+  for (int j = tid; j < ns; j += NT) F[(nf + j) + static_cast<int64_t>(j) * ld] = T(1);
+  const double* v = vals + b * nnz;
+  for (int q0 = S.a_begin + tid; q0 < S.a_end; q0 += NT * kU) {
+    T av[kU];
+    int64_t ad[kU];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      const int q = q0 + u * NT;
+      ad[u] = -1;
+      if (q < S.a_end) {
+        ad[u] = a_row[q] + static_cast<int64_t>(a_col[q]) * ld;
+        av[u] = static_cast<T>(v[a_e[q]]);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kU; ++u)
+      if (ad[u] >= 0) F[ad[u]] = av[u];
+  }
+  for (int cc = S.c_begin; cc < S.c_end; ++cc) {
+    const SupDev C = sup[cidx[cc]];
+    const int rc = C.rs, ldc = C.ns + C.rs + C.ns;
+    const T* Fc = fprev + b * fprev_bs + C.front_off;
+    __syncthreads();  // F writes before, rpl reads of the previous child
+    for (int i = tid; i < rc; i += NT) rpl[i] = relpos[C.rptr + i];
+    __syncthreads();
+    const int64_t rc2 = static_cast<int64_t>(rc) * rc;
+    for (int64_t t0 = tid; t0 < rc2; t0 += NT * kU) {
+      T cv[kU], fv[kU];
+      int64_t fa[kU];
+#pragma unroll
+      for (int u = 0; u < kU; ++u) {
+        const int64_t t = t0 + u * NT;
+        const int i = static_cast<int>(t % rc), j = static_cast<int>(t / rc);
+        fa[u] = -1;
+        if (t < rc2 && i >= j) {
+          fa[u] = rpl[i] + static_cast<int64_t>(rpl[j]) * ld;
+          cv[u] = Fc[(C.ns + i) + static_cast<int64_t>(C.ns + j) * ldc];
+          fv[u] = F[fa[u]];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < kU; ++u)
+        if (fa[u] >= 0) F[fa[u]] = fv[u] + cv[u];
+    }
+  }
+  __syncthreads();
+}
+
+// the jb x jb diagonal block of a panel, by one wavefront: lanes < jb hold its
+// rows in rv, pivot columns are read across lanes (no LDS, no barrier); the
+// L rows go to ldg (stride kLDG) and the reciprocal diagonal to dgi
+template <typename T>
+__device__ inline void diag_block(T (&rv)[kNB], int jb, int lane, T* ldg, T* dgi,
+                                  int* __restrict__ info) {
+#pragma unroll
+  for (int t = 0; t < kNB; ++t) {
+    if (t < jb) {  // uniform
+      T d = lane_value(rv[t], t);
+      if (!(d > T(0))) {  // uniform
+        if (lane == 0) atomicMax(info, 1);
+        d = T(1);
+      }
+      const T sq = sqrt(d);
+      const T dinv = T(1) / sq;
+      T cl[kNB];  // column t of rows l > t, unscaled
+#pragma unroll
+      for (int l = t + 1; l < kNB; ++l) cl[l] = lane_value(rv[t], l);
+      if (lane == t) {
+        rv[t] = sq;
+        dgi[t] = dinv;
+      } else if (lane > t && lane < jb) {
+        const T g = rv[t] * (dinv * dinv);
+        rv[t] *= dinv;
+#pragma unroll
+        for (int l = t + 1; l < kNB; ++l) rv[l] -= g * cl[l];
+      }
+    }
+  }
+  if (lane < jb) {
+#pragma unroll
+    for (int c = 0; c < kNB; ++c) ldg[lane * kLDG + c] = rv[c];
+  }
+}
+
+// a row below the diagonal block: L_r = A_r L_diag^{-T} (forward substitution
+// on the row's registers with the block from LDS)
+template <typename T>
+__device__ inline void row_solve(T (&rv)[kNB], int jb, const T* ldg, const T* dgi) {
+#pragma unroll
+  for (int t = 0; t < kNB; ++t) {
+    if (t < jb) {
+      T s0 = rv[t], s1 = T(0);
+#pragma unroll
+      for (int l = 0; l < t; l += 2) {
+        s0 -= rv[l] * ldg[t * kLDG + l];
+        if (l + 1 < t) s1 -= rv[l + 1] * ldg[t * kLDG + l + 1];
+      }
+      rv[t] = (s0 + s1) * dgi[t];
+    }
+  }
+}
+
+// the sweep panel P (nf x ns, rows [0, ns) = L_JJ^{-1}, rows [ns, nf) = Q_R)
+// and its transpose PT (ns x nf) from the eliminated front's E rows
+template <typename T, int NT>
+__device__ void panel_out(const SupDev& S, const T* __restrict__ F, T* __restrict__ P,
+                          T* __restrict__ PT) {
+  constexpr int kU = kGather;
+  const int tid = threadIdx.x;
+  const int ns = S.ns, nf = S.ns + S.rs, ld = nf + S.ns;
+  const int64_t np = static_cast<int64_t>(nf) * ns;
+  for (int64_t i0 = tid; i0 < np; i0 += NT * kU) {
+    T pv[kU];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      const int64_t idx = i0 + u * NT;
+      const int c = static_cast<int>(idx % ns), r = static_cast<int>(idx / ns);
+      pv[u] = idx < np ? F[(nf + c) + static_cast<int64_t>(r) * ld] : T(0);
+      if (r < ns && r < c) pv[u] = T(0);
+      if (r >= ns) pv[u] = -pv[u];
+    }
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      const int64_t idx = i0 + u * NT;
+      if (idx < np) {
+        const int c = static_cast<int>(idx % ns), r = static_cast<int>(idx / ns);
+        PT[idx] = pv[u];
+        P[r + static_cast<int64_t>(c) * nf] = pv[u];
+      }
+    }
+  }
+}
+
 template <typename T, int NT, int RPT>
 __global__ __launch_bounds__(NT) void k_factor_level(
     const SupDev* __restrict__ sup, const int* __restrict__ lvl,
@@ -146,61 +297,8 @@ __global__ __launch_bounds__(NT) void k_factor_level(
   T* dgi = ldg + kNB * kLDG;                  // reciprocal diagonal
   T* pan = dgi + kNB;                         // panel rows, stride kLDP
   T* F = fcur + b * fcur_bs + S.front_off;
-  const int64_t fsz = static_cast<int64_t>(ld) * nf;
-  for (int64_t i = tid; i < fsz; i += NT) F[i] = T(0);
-  __syncthreads();
-  for (int j = tid; j < ns; j += NT) F[(nf + j) + static_cast<int64_t>(j) * ld] = T(1);
-  const double* v = vals + b * nnz;
-  // loops over global gathers take kU entries per thread at a time, loads
-  // first, so that their latencies overlap
-  constexpr int kU = 16;
-  for (int q0 = S.a_begin + tid; q0 < S.a_end; q0 += NT * kU) {
-    T av[kU];
-    int64_t ad[kU];
-#pragma unroll
-    for (int u = 0; u < kU; ++u) {
-      const int q = q0 + u * NT;
-      ad[u] = -1;
-      if (q < S.a_end) {
-        ad[u] = a_row[q] + static_cast<int64_t>(a_col[q]) * ld;
-        av[u] = static_cast<T>(v[a_e[q]]);
-      }
-    }
-#pragma unroll
-    for (int u = 0; u < kU; ++u)
-      if (ad[u] >= 0) F[ad[u]] = av[u];
-  }
-  // extend-add of the children's update blocks (lower triangles), their
-  // relative positions staged in LDS
-  int* rpl = reinterpret_cast<int*>(pan);
-  for (int cc = S.c_begin; cc < S.c_end; ++cc) {
-    const SupDev C = sup[cidx[cc]];
-    const int rc = C.rs, ldc = C.ns + C.rs + C.ns;
-    const T* Fc = fprev + b * fprev_bs + C.front_off;
-    __syncthreads();  // F writes before, rpl reads of the previous child
-    for (int i = tid; i < rc; i += NT) rpl[i] = relpos[C.rptr + i];
-    __syncthreads();
-    const int64_t rc2 = static_cast<int64_t>(rc) * rc;
-    for (int64_t t0 = tid; t0 < rc2; t0 += NT * kU) {
-      T cv[kU], fv[kU];
-      int fa[kU];  // offsets inside the front (fronts hold < 2^31 entries)
-#pragma unroll
-      for (int u = 0; u < kU; ++u) {
-        const int64_t t = t0 + u * NT;
-        const int i = static_cast<int>(t % rc), j = static_cast<int>(t / rc);
-        fa[u] = -1;
-        if (t < rc2 && i >= j) {
-          fa[u] = rpl[i] + rpl[j] * ld;
-          cv[u] = Fc[(C.ns + i) + static_cast<int64_t>(C.ns + j) * ldc];
-          fv[u] = F[fa[u]];
-        }
-      }
-#pragma unroll
-      for (int u = 0; u < kU; ++u)
-        if (fa[u] >= 0) F[fa[u]] = fv[u] + cv[u];
-    }
-  }
-  __syncthreads();
+  front_init<T, NT>(S, b, sup, relpos, cidx, a_e, a_row, a_col, vals, nnz, F, fprev, fprev_bs,
+                    reinterpret_cast<int*>(pan));
   for (int j0 = 0; j0 < ns; j0 += kNB) {
     const int jb = min(kNB, ns - j0);
     const int m = ld - j0;     // panel rows: J, R and E rows from j0 on
@@ -213,59 +311,15 @@ __global__ __launch_bounds__(NT) void k_factor_level(
       for (int c = 0; c < kNB; ++c)
         rv[q][c] = (r < m && c < jb) ? F[(j0 + r) + static_cast<int64_t>(j0 + c) * ld] : T(0);
     }
-    // (1) the jb x jb diagonal block, by wavefront 0 alone: lanes < jb hold
-    // its rows, the pivot column is read across lanes (readlane, no LDS);
-    // its L rows and reciprocal diagonal go to LDS
-    if (wave == 0) {
-#pragma unroll
-      for (int t = 0; t < kNB; ++t) {
-        if (t < jb) {  // uniform
-          T d = lane_value(rv[0][t], t);
-          if (!(d > T(0))) {  // uniform
-            if (lane == 0) atomicMax(info, 1);
-            d = T(1);
-          }
-          const T sq = sqrt(d);
-          const T dinv = T(1) / sq;
-          T cl[kNB];  // column t of rows l > t, unscaled
-#pragma unroll
-          for (int l = t + 1; l < kNB; ++l) cl[l] = lane_value(rv[0][t], l);
-          if (lane == t) {
-            rv[0][t] = sq;
-            dgi[t] = dinv;
-          } else if (lane > t && lane < jb) {
-            const T g = rv[0][t] * (dinv * dinv);
-            rv[0][t] *= dinv;
-#pragma unroll
-            for (int l = t + 1; l < kNB; ++l) rv[0][l] -= g * cl[l];
-          }
-        }
-      }
-      if (lane < jb) {
-#pragma unroll
-        for (int c = 0; c < kNB; ++c) ldg[lane * kLDG + c] = rv[0][c];
-      }
-    }
+    // (1) the jb x jb diagonal block, by wavefront 0 alone (lanes < jb hold
+    // its rows)
+    if (wave == 0) diag_block<T>(rv[0], jb, lane, ldg, dgi, info);
     __syncthreads();
-    // (2) every row below the block: L_r = A_r L_diag^{-T}, a forward
-    // substitution on the thread's own registers with the block from LDS
+    // (2) every row below the block, on the thread's own registers
 #pragma unroll
     for (int q = 0; q < RPT; ++q) {
       const int r = tid + q * NT;
-      if (r >= jb && r < m) {
-#pragma unroll
-        for (int t = 0; t < kNB; ++t) {
-          if (t < jb) {
-            T s0 = rv[q][t], s1 = T(0);
-#pragma unroll
-            for (int l = 0; l < t; l += 2) {
-              s0 -= rv[q][l] * ldg[t * kLDG + l];
-              if (l + 1 < t) s1 -= rv[q][l + 1] * ldg[t * kLDG + l + 1];
-            }
-            rv[q][t] = (s0 + s1) * dgi[t];
-          }
-        }
-      }
+      if (r >= jb && r < m) row_solve<T>(rv[q], jb, ldg, dgi);
     }
     // panel to F (its E rows) and to LDS (zero columns past jb; the 16 rows
     // past m are zeroed for the tile reads)
@@ -340,34 +394,135 @@ __global__ __launch_bounds__(NT) void k_factor_level(
     }
     __syncthreads();
   }
-  // panel for the sweeps: rows [0, ns) = L_JJ^{-1}, rows [ns, nf) = Q_R,
-  // both read from the E rows; also stored transposed (PT = P^T, ns x nf
-  // column-major) for the backward sweep, whose operand runs along the rows
-  // of P
-  T* P = panels + b * panel_bs + S.panel_off;
-  T* PT = panelsT + b * panel_bs + S.panel_off;
-  const int64_t np = static_cast<int64_t>(nf) * ns;
-  for (int64_t i0 = tid; i0 < np; i0 += NT * kU) {
-    T pv[kU];
+  // the sweep panel and its transpose (the backward sweep's operand runs
+  // along the rows of P)
+  panel_out<T, NT>(S, F, panels + b * panel_bs + S.panel_off,
+                   panelsT + b * panel_bs + S.panel_off);
+  (void)rs;
+}
+
+// ---------------------------------------------------------------------------
+// Fronts too tall for the in-LDS panel (3-D separators of thousands of rows):
+// the same elimination as separate launches per 32-column panel --
+// k_front_init_big (assembly, one workgroup per front), per panel
+// k_panel_big (the diagonal block by wavefront 0, then every row below it
+// streamed through registers and solved, in place in F) and k_trail_big (the
+// trailing update as 64x64 MFMA tiles over a grid of workgroups, operands
+// read from F), then k_front_out_big (the sweep panels).
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(1024) void k_front_init_big(
+    const SupDev* __restrict__ sup, const int* __restrict__ lvl,
+    const int* __restrict__ relpos, const int* __restrict__ cidx,
+    const int* __restrict__ a_e, const int* __restrict__ a_row,
+    const int* __restrict__ a_col, const double* __restrict__ vals, int64_t nnz,
+    T* __restrict__ fcur, int64_t fcur_bs, const T* __restrict__ fprev, int64_t fprev_bs) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const SupDev S = sup[lvl[blockIdx.x]];
+  const int b = blockIdx.y;
+  front_init<T, 1024>(S, b, sup, relpos, cidx, a_e, a_row, a_col, vals, nnz,
+                      fcur + b * fcur_bs + S.front_off, fprev, fprev_bs,
+                      reinterpret_cast<int*>(smem_raw));
+}
+
+template <typename T>
+__global__ __launch_bounds__(1024) void k_panel_big(const SupDev* __restrict__ sup,
+                                                   const int* __restrict__ lvl,
+                                                   T* __restrict__ fcur, int64_t fcur_bs,
+                                                   int j0, int* __restrict__ info) {
+  __shared__ T ldg[kNB * kLDG];
+  __shared__ T dgi[kNB];
+  const SupDev S = sup[lvl[blockIdx.x]];
+  const int b = blockIdx.y;
+  const int ns = S.ns, nf = S.ns + S.rs, ld = nf + S.ns;
+  if (j0 >= ns) return;  // this front has no panel j0 (uniform)
+  const int jb = min(kNB, ns - j0), m = ld - j0;
+  const int tid = threadIdx.x, lane = tid & 63;
+  T* Pn = fcur + b * fcur_bs + S.front_off + j0 + static_cast<int64_t>(j0) * ld;
+  if (tid < 64) {
+    T rv[kNB];
 #pragma unroll
-    for (int u = 0; u < kU; ++u) {
-      const int64_t idx = i0 + u * NT;
-      const int c = static_cast<int>(idx % ns), r = static_cast<int>(idx / ns);
-      pv[u] = idx < np ? F[(nf + c) + static_cast<int64_t>(r) * ld] : T(0);
-      if (r < ns && r < c) pv[u] = T(0);
-      if (r >= ns) pv[u] = -pv[u];
-    }
+    for (int c = 0; c < kNB; ++c)
+      rv[c] = (lane < jb && c < jb && lane >= c) ? Pn[lane + static_cast<int64_t>(c) * ld] : T(0);
+    diag_block<T>(rv, jb, lane, ldg, dgi, info);
+    if (lane < jb) {
 #pragma unroll
-    for (int u = 0; u < kU; ++u) {
-      const int64_t idx = i0 + u * NT;
-      if (idx < np) {
-        const int c = static_cast<int>(idx % ns), r = static_cast<int>(idx / ns);
-        PT[idx] = pv[u];
-        P[r + static_cast<int64_t>(c) * nf] = pv[u];
-      }
+      for (int c = 0; c < kNB; ++c)
+        if (c <= lane) Pn[lane + static_cast<int64_t>(c) * ld] = rv[c];
     }
   }
-  (void)rs;
+  __syncthreads();
+  for (int r = jb + tid; r < m; r += 1024) {
+    T rv[kNB];
+#pragma unroll
+    for (int c = 0; c < kNB; ++c) rv[c] = c < jb ? Pn[r + static_cast<int64_t>(c) * ld] : T(0);
+    row_solve<T>(rv, jb, ldg, dgi);
+#pragma unroll
+    for (int c = 0; c < kNB; ++c)
+      if (c < jb) Pn[r + static_cast<int64_t>(c) * ld] = rv[c];
+  }
+}
+
+// C[i][l] -= sum_k Pan[i][k] Pan[l][k] over the trailing region of panel j0:
+// rows i in [jb, m) (J/R rows, then E rows), columns l in [jb, n1), lower
+// triangle only for the J/R rows.  Workgroup (x, y) covers rows jb + 64x..,
+// columns jb + 64y..; z = front x batch member.  Each wavefront: 16 rows x 64
+// columns (four 16x16 MFMA tiles), computed transposed so a lane's results
+// run down a column of F.
+template <typename T>
+__global__ __launch_bounds__(256) void k_trail_big(const SupDev* __restrict__ sup,
+                                                  const int* __restrict__ lvl,
+                                                  T* __restrict__ fcur, int64_t fcur_bs,
+                                                  int j0, int nbatch) {
+  const int f = blockIdx.z / nbatch, b = blockIdx.z % nbatch;
+  const SupDev S = sup[lvl[f]];
+  const int ns = S.ns, nf = S.ns + S.rs, ld = nf + S.ns;
+  if (j0 >= ns) return;
+  const int jb = min(kNB, ns - j0), m = ld - j0, n1 = nf - j0;
+  const int rb = jb + blockIdx.x * 64, cb = jb + blockIdx.y * 64;
+  if (rb >= m || cb >= n1 || (rb + 63 < n1 && rb + 63 < cb)) return;  // empty or upper
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  T* Pn = fcur + b * fcur_bs + S.front_off + j0 + static_cast<int64_t>(j0) * ld;
+  const int i = rb + wave * 16 + (lane & 15);
+  const int kq = lane >> 4;
+  typename Mma<T>::v4 acc[4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) acc[t] = {T(0), T(0), T(0), T(0)};
+#pragma unroll
+  for (int k0 = 0; k0 < kNB; k0 += 4) {
+    const int kk = k0 + kq;
+    const bool kok = kk < jb;
+    const T bv = (kok && i < m) ? Pn[i + static_cast<int64_t>(kk) * ld] : T(0);
+    T av[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int l = cb + 16 * t + (lane & 15);
+      av[t] = (kok && l < n1) ? Pn[l + static_cast<int64_t>(kk) * ld] : T(0);
+    }
+#pragma unroll
+    for (int t = 0; t < 4; ++t) acc[t] = Mma<T>::op(av[t], bv, acc[t]);
+  }
+  if (i >= m) return;
+#pragma unroll
+  for (int t = 0; t < 4; ++t)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int l = cb + 16 * t + Mma<T>::row(lane, e);
+      if (l < n1 && (i >= n1 || i >= l)) Pn[i + static_cast<int64_t>(l) * ld] -= acc[t][e];
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(1024) void k_front_out_big(const SupDev* __restrict__ sup,
+                                                       const int* __restrict__ lvl,
+                                                       const T* __restrict__ fcur,
+                                                       int64_t fcur_bs, T* __restrict__ panels,
+                                                       T* __restrict__ panelsT,
+                                                       int64_t panel_bs) {
+  const SupDev S = sup[lvl[blockIdx.x]];
+  const int b = blockIdx.y;
+  panel_out<T, 1024>(S, fcur + b * fcur_bs + S.front_off, panels + b * panel_bs + S.panel_off,
+                     panelsT + b * panel_bs + S.panel_off);
 }
 
 // ---------------------------------------------------------------------------
@@ -622,15 +777,18 @@ __global__ __launch_bounds__(kThreads) void k_bwd_wave(
     const int* __restrict__ ridx, const int* __restrict__ active,
     const T* __restrict__ panelsT, int64_t panel_bs, T* __restrict__ W, int64_t w_bs, int k,
     double* __restrict__ X64, const int* __restrict__ xrow, int xmode, int cnt, int nzb,
-    int total) {
+    int total, int nzc) {
   constexpr int CW = 16 * V;
   const SweepBlock sb = sweep_block(lvl, cnt, nzb, total);
   if (!sb.live) return;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int col0 = (KS == 1 ? sb.z * 4 + wave : sb.z) * CW;
+  // z = column group + nzc * row group; a row group is 16 MT output rows
+  const int zc = sb.z % nzc, r0 = (sb.z / nzc) * 16 * MT;
+  const int col0 = (KS == 1 ? zc * 4 + wave : zc) * CW;
   if (col0 >= k) return;  // uniform over the workgroup when KS > 1
   const SupDev S = sup[sb.s];
   const int ns = S.ns, nf = S.ns + S.rs;
+  if (r0 >= ns) return;  // uniform
   const bool act = !active || active[sb.s];
   const T* PT = panelsT + sb.b * panel_bs + S.panel_off;
   T* Wb = W + sb.b * w_bs;
@@ -669,7 +827,7 @@ __global__ __launch_bounds__(kThreads) void k_bwd_wave(
       }
 #pragma unroll
       for (int t = 0; t < MT; ++t) {
-        const int r = t * 16 + (lane & 15);
+        const int r = r0 + t * 16 + (lane & 15);
         av[j][t] = (kok && r < ns) ? PT[r + static_cast<int64_t>(kk) * ns] : T(0);
       }
     }
@@ -715,7 +873,7 @@ __global__ __launch_bounds__(kThreads) void k_bwd_wave(
     if (KS > 1 && t % KS != wave) continue;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      const int r = t * 16 + Mma<T>::row(lane, i);
+      const int r = r0 + t * 16 + Mma<T>::row(lane, i);
       if (r >= ns) continue;
       T* dst = Wb + static_cast<int64_t>(S.first + r) * k + cl;
       if (navail >= V) {
@@ -741,6 +899,131 @@ __global__ __launch_bounds__(kThreads) void k_bwd_wave(
       }
     }
   }
+}
+
+// Forward sweep for fronts too tall for the in-LDS kernel: (1) f = [b_J; 0] +
+// the children's updates, assembled in a global block per supernode (rows
+// of the front, k columns; children one after the other); (2) the GEMM
+// [y_J; w] = P_s f_J by wavefronts reading f and P_s straight into MFMA
+// registers, 16 MT rows x 16 columns per wavefront: y_J to the solution
+// rows, u_s = f_R - w to the update buffer.
+// Grid of both: 1-D over (z, supernode of the level, batch member), z
+// fastest, XCD-remapped as sweep_block.
+struct LevelBlock {
+  int q, b, z;
+  bool live;
+};
+__device__ inline LevelBlock level_block(int cnt, int nzb, int total) {
+  const int per = gridDim.x >> 3;
+  const int L = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+  LevelBlock o;
+  o.live = L < total;
+  const int Lc = o.live ? L : 0;
+  o.z = Lc % nzb;
+  o.q = (Lc / nzb) % cnt;
+  o.b = Lc / nzb / cnt;
+  return o;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_fwd_gather_big(
+    const SupDev* __restrict__ sup, const int* __restrict__ lvl, int cnt,
+    const int* __restrict__ relpos, const int* __restrict__ cidx,
+    const int* __restrict__ active, const double* __restrict__ Bsrc, int64_t b_bs,
+    const int* __restrict__ brow, int k, const T* __restrict__ uprev, int64_t uprev_bs,
+    const T* __restrict__ ufz, int64_t ufz_bs, T* __restrict__ fs,
+    const int64_t* __restrict__ fs_off, int64_t fs_bs, int nzb, int total) {
+  const LevelBlock lb = level_block(cnt, nzb, total);
+  if (!lb.live) return;
+  const int s = lvl[lb.q];
+  const SupDev S = sup[s];
+  const int ns = S.ns, nf = S.ns + S.rs;
+  const int tid = threadIdx.x;
+  const int col0 = lb.z * 64;
+  T* f = fs + lb.b * fs_bs + fs_off[s] * k;
+  const double* Bb = Bsrc + lb.b * b_bs;
+  for (int idx = tid; idx < nf * 64; idx += 256) {
+    const int r = idx >> 6, c = col0 + (idx & 63);
+    if (c >= k) continue;
+    T v = T(0);
+    if (r < ns) {
+      const int gr = brow ? brow[S.first + r] : S.first + r;
+      v = static_cast<T>(Bb[static_cast<int64_t>(gr) * k + c]);
+    }
+    f[static_cast<int64_t>(r) * k + c] = v;
+  }
+  for (int cc = S.c_begin; cc < S.c_end; ++cc) {
+    const int ch = cidx[cc];
+    if (active && !active[ch]) continue;
+    __syncthreads();
+    const SupDev C = sup[ch];
+    const T* u = C.fz_uoff >= 0 ? ufz + lb.b * ufz_bs + C.fz_uoff * k
+                                : uprev + lb.b * uprev_bs + C.upd_off * k;
+    const int* rp = relpos + C.rptr;
+    for (int idx = tid; idx < C.rs * 64; idx += 256) {
+      const int r = idx >> 6, c = col0 + (idx & 63);
+      if (c < k) f[static_cast<int64_t>(rp[r]) * k + c] += u[static_cast<int64_t>(r) * k + c];
+    }
+  }
+}
+
+template <typename T, int MT>
+__global__ __launch_bounds__(256) void k_fwd_wave_big(
+    const SupDev* __restrict__ sup, const int* __restrict__ lvl, int cnt,
+    const T* __restrict__ panels, int64_t panel_bs, const T* __restrict__ fs,
+    const int64_t* __restrict__ fs_off, int64_t fs_bs, T* __restrict__ W, int64_t w_bs, int k,
+    T* __restrict__ ucur, int64_t ucur_bs, int nzc, int nzb, int total) {
+  constexpr int PF = 4;
+  const LevelBlock lb = level_block(cnt, nzb, total);
+  if (!lb.live) return;
+  const int s = lvl[lb.q];
+  const SupDev S = sup[s];
+  const int ns = S.ns, nf = S.ns + S.rs;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int zc = lb.z % nzc, r0 = (lb.z / nzc) * 16 * MT;
+  const int col0 = (zc * 4 + wave) * 16;
+  if (r0 >= nf || col0 >= k) return;
+  const int c = col0 + (lane & 15);
+  const bool cok = c < k;
+  const int kq = lane >> 4;
+  const T* P = panels + lb.b * panel_bs + S.panel_off;  // nf x ns, column-major
+  const T* f = fs + lb.b * fs_bs + fs_off[s] * k;
+  typename Mma<T>::v4 acc[MT];
+#pragma unroll
+  for (int t = 0; t < MT; ++t) acc[t] = {T(0), T(0), T(0), T(0)};
+  for (int k0 = 0; k0 < ns; k0 += 4 * PF) {
+    T bv[PF], av[PF][MT];
+#pragma unroll
+    for (int j = 0; j < PF; ++j) {
+      const int kk = k0 + 4 * j + kq;
+      const bool kok = kk < ns;
+      bv[j] = (kok && cok) ? f[static_cast<int64_t>(kk) * k + c] : T(0);
+#pragma unroll
+      for (int t = 0; t < MT; ++t) {
+        const int r = r0 + t * 16 + (lane & 15);
+        av[j][t] = (kok && r < nf) ? P[r + static_cast<int64_t>(kk) * nf] : T(0);
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < PF; ++j)
+      if (k0 + 4 * j < ns) {
+#pragma unroll
+        for (int t = 0; t < MT; ++t) acc[t] = Mma<T>::op(av[j][t], bv[j], acc[t]);
+      }
+  }
+  if (!cok) return;
+  T* Wb = W + lb.b * w_bs;
+  T* uo = ucur + lb.b * ucur_bs + S.upd_off * k;
+#pragma unroll
+  for (int t = 0; t < MT; ++t)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int r = r0 + t * 16 + Mma<T>::row(lane, i);
+      if (r < ns)
+        Wb[static_cast<int64_t>(S.first + r) * k + c] = acc[t][i];
+      else if (r < nf)
+        uo[static_cast<int64_t>(r - ns) * k + c] = f[static_cast<int64_t>(r) * k + c] - acc[t][i];
+    }
 }
 
 struct LRow {
@@ -1324,6 +1607,10 @@ struct dcx_plan {
   size_t r_bytes = 0, b_bytes = 0, x_bytes = 0;
   int64_t device_bytes = 0;
   int max_row_nnz = 0;  // longest row of the caller's pattern
+  // forward-sweep scratch of the levels whose fronts exceed the in-LDS kernel
+  int64_t* d_fs_off = nullptr;  // row offset of each supernode within its level
+  void* d_fs = nullptr;
+  size_t fs_bytes = 0;
   // second stream and level events of dcx_factor_solve
   hipStream_t side = nullptr;
   std::vector<hipEvent_t> lev_ev;
@@ -1339,7 +1626,7 @@ struct dcx_plan {
                     d_perm, d_iperm, d_pcrow, d_pcol, d_pvidx, d_crow, d_col, d_info, d_panels,
                     d_front[0], d_front[1], d_vals, d_active, d_act_lvl, d_upd[0],
                     d_upd[1], d_w, d_r, d_b, d_x, d_fz, d_flv, d_fmem, d_fgptr, d_fgb,
-                    d_toff, d_lridx, d_ulvl, d_ufz, d_panelsT};
+                    d_toff, d_lridx, d_ulvl, d_ufz, d_panelsT, d_fs_off, d_fs};
     for (void* p : ptrs)
       if (p) (void)hipFree(p);
   }
@@ -1421,6 +1708,41 @@ void factor_launch(dcx_plan* P, int d, int cnt, size_t lds, hipStream_t st) {
                      static_cast<T*>(P->d_panelsT), S.panel_size, P->d_info);
 }
 
+// a level with fronts too tall for k_factor_level: launches per panel
+template <typename T>
+int factor_level_big(dcx_plan* P, int d, int cnt, hipStream_t st) {
+  const auto& S = P->S;
+  T* fr[2] = {static_cast<T*>(P->d_front[0]), static_cast<T*>(P->d_front[1])};
+  const int64_t fbs[2] = {S.front_size[0], S.front_size[1]};
+  const int* lvl = P->d_lvl + S.lvl_ptr[d];
+  int maxns = 0, maxnf = 0, maxrc = 0;
+  for (int q = S.lvl_ptr[d]; q < S.lvl_ptr[d + 1]; ++q) {
+    const int s = S.lvl_sup[q];
+    maxns = std::max(maxns, S.ns[s]);
+    maxnf = std::max(maxnf, S.ns[s] + S.rs[s]);
+    for (int c = S.cptr[s]; c < S.cptr[s + 1]; ++c) maxrc = std::max(maxrc, S.rs[S.cidx[c]]);
+  }
+  if (static_cast<size_t>(maxrc) * sizeof(int) > 160 * 1024)
+    return fail(DCX_ERR_ARG, "child update block too large for the factor kernel");
+  dim3 grid(cnt, P->nbatch);
+  hipLaunchKernelGGL(k_front_init_big<T>, grid, dim3(1024), std::max<size_t>(maxrc, 1) * sizeof(int),
+                     st, P->d_sup, lvl, P->d_relpos, P->d_cidx, P->d_a_e, P->d_a_row, P->d_a_col,
+                     P->d_vals, static_cast<int64_t>(S.nnz), fr[d & 1], fbs[d & 1],
+                     fr[(d + 1) & 1], fbs[(d + 1) & 1]);
+  for (int j0 = 0; j0 < maxns; j0 += kNB) {
+    hipLaunchKernelGGL(k_panel_big<T>, grid, dim3(1024), 0, st, P->d_sup, lvl, fr[d & 1],
+                       fbs[d & 1], j0, P->d_info);
+    const int rows = maxnf + maxns - j0, cols = maxnf - j0;  // tallest m and widest n1
+    dim3 tg((rows + 63) / 64, (cols + 63) / 64, cnt * P->nbatch);
+    hipLaunchKernelGGL(k_trail_big<T>, tg, dim3(256), 0, st, P->d_sup, lvl, fr[d & 1],
+                       fbs[d & 1], j0, P->nbatch);
+  }
+  hipLaunchKernelGGL(k_front_out_big<T>, grid, dim3(1024), 0, st, P->d_sup, lvl, fr[d & 1],
+                     fbs[d & 1], static_cast<T*>(P->d_panels), static_cast<T*>(P->d_panelsT),
+                     S.panel_size);
+  return DCX_OK;
+}
+
 // one launch per level; the threads (x rows per thread) cover the tallest
 // panel of the level
 template <typename T>
@@ -1434,8 +1756,7 @@ int factor_level(dcx_plan* P, int d, hipStream_t st) {
     maxld = std::max(maxld, 2 * S.ns[s] + S.rs[s]);
   }
   const size_t lds = (kNB * kLDG + kNB + static_cast<size_t>(maxld + 16) * kLDP) * sizeof(T);
-  if (lds > 160 * 1024 || maxld > 1024)
-    return fail(DCX_ERR_ARG, "front too large for the factor kernel");
+  if (lds > 160 * 1024 || maxld > 1024) return factor_level_big<T>(P, d, cnt, st);
   if (maxld <= 256)
     factor_launch<T, 256, 1>(P, d, cnt, lds, st);
   else
@@ -1498,30 +1819,32 @@ constexpr size_t kSweepLds = 160 * 1024;
 
 template <typename T, int MT, int V, int KS, int PF>
 void bwd_wave_launch(dcx_plan* P, T* W, int k, const int* lvl, int cnt, const int* active,
-                     const SweepIO& io, hipStream_t st) {
+                     const SweepIO& io, hipStream_t st, int nrg) {
   constexpr int CW = 16 * V;
   const int ncb = (k + CW - 1) / CW;              // column blocks
-  const int nzb = KS == 1 ? (ncb + 3) / 4 : ncb;  // workgroups per supernode and batch member
+  const int nzc = KS == 1 ? (ncb + 3) / 4 : ncb;  // column groups per supernode and batch member
+  const int nzb = nzc * nrg;                      // x row groups of 16 MT rows
   const int total = cnt * P->nbatch * nzb;
   dim3 grid(((total + 7) / 8) * 8);
   const size_t lds = KS > 1 ? static_cast<size_t>(KS) * MT * V * 4 * 64 * sizeof(T) : 0;
   const int64_t w_bs = static_cast<int64_t>(P->S.n) * k;
   hipLaunchKernelGGL((k_bwd_wave<T, MT, V, KS, PF>), grid, dim3(kThreads), lds, st, P->d_sup,
                      lvl, P->d_ridx, active, static_cast<const T*>(P->d_panelsT),
-                     P->S.panel_size, W, w_bs, k, io.X64, io.xrow, io.xmode, cnt, nzb, total);
+                     P->S.panel_size, W, w_bs, k, io.X64, io.xrow, io.xmode, cnt, nzb, total,
+                     nzc);
 }
 
 template <typename T, int V, int KS, int PF>
 bool bwd_wave_mt(dcx_plan* P, T* W, int k, int maxns, const int* lvl, int cnt,
                  const int* active, const SweepIO& io, hipStream_t st) {
   const int mt = (maxns + 15) / 16;
-  if (mt <= 1) bwd_wave_launch<T, 1, V, KS, PF>(P, W, k, lvl, cnt, active, io, st);
-  else if (mt <= 2) bwd_wave_launch<T, 2, V, KS, PF>(P, W, k, lvl, cnt, active, io, st);
-  else if (mt <= 3) bwd_wave_launch<T, 3, V, KS, PF>(P, W, k, lvl, cnt, active, io, st);
-  else if (mt <= 4) bwd_wave_launch<T, 4, V, KS, PF>(P, W, k, lvl, cnt, active, io, st);
-  else if (mt <= 6) bwd_wave_launch<T, 6, V, KS, PF>(P, W, k, lvl, cnt, active, io, st);
-  else if (mt <= 8) bwd_wave_launch<T, 8, V, KS, PF>(P, W, k, lvl, cnt, active, io, st);
-  else return false;
+  if (mt <= 1) bwd_wave_launch<T, 1, V, KS, PF>(P, W, k, lvl, cnt, active, io, st, 1);
+  else if (mt <= 2) bwd_wave_launch<T, 2, V, KS, PF>(P, W, k, lvl, cnt, active, io, st, 1);
+  else if (mt <= 3) bwd_wave_launch<T, 3, V, KS, PF>(P, W, k, lvl, cnt, active, io, st, 1);
+  else if (mt <= 4) bwd_wave_launch<T, 4, V, KS, PF>(P, W, k, lvl, cnt, active, io, st, 1);
+  else if (mt <= 6) bwd_wave_launch<T, 6, V, KS, PF>(P, W, k, lvl, cnt, active, io, st, 1);
+  else  // taller fronts: row groups of 128 rows
+    bwd_wave_launch<T, 8, V, KS, PF>(P, W, k, lvl, cnt, active, io, st, (mt + 7) / 8);
   return true;
 }
 
@@ -1543,7 +1866,6 @@ bool bwd_wave(dcx_plan* P, T* W, int k, int maxns, const int* lvl, int cnt, cons
 template <typename T, int TC>
 void sweep_tc(dcx_plan* P, T* W, int k, int maxnf, int maxns, int maxpanel, bool fwd, int d,
               const int* lvl, int cnt, const int* active, const SweepIO& io, hipStream_t st) {
-  if (!fwd && bwd_wave<T>(P, W, k, maxns, lvl, cnt, active, io, st)) return;
   static const int use_pl = env_int("DCX_PANEL_LDS", 0);
   static const int ct_max = env_int("DCX_CT_MAX", 1);
   static const int target = env_int("DCX_TARGET_BLOCKS", 8192);
@@ -1614,9 +1936,38 @@ int launch_fused(dcx_plan* P, T* W, int k, bool fwd, const int* active, const Sw
 
 // update buffers of the forward sweep for k right-hand sides (grown on
 // demand; growing synchronises the stream once)
+int pick_tc(int maxnf, int maxns, size_t tsize);
+
+// rows of the widest level whose forward sweep takes the large-front path
+inline int64_t big_fwd_rows(const dcx_plan* P, size_t tsize) {
+  const auto& S = P->S;
+  int64_t best = 0;
+  for (int d = 0; d < S.ndepth; ++d) {
+    int maxnf = 0, maxns = 0;
+    int64_t rows = 0;
+    for (int q = S.lvl_ptr[d]; q < S.lvl_ptr[d + 1]; ++q) {
+      const int s = S.lvl_sup[q];
+      maxnf = std::max(maxnf, S.ns[s] + S.rs[s]);
+      maxns = std::max(maxns, S.ns[s]);
+      rows += S.ns[s] + S.rs[s];
+    }
+    if (pick_tc(maxnf, maxns, tsize) == 0) best = std::max(best, rows);
+  }
+  return best;
+}
+
 template <typename T>
 int ensure_upd(dcx_plan* P, int k, hipStream_t st) {
   const auto& S = P->S;
+  const size_t fsb = static_cast<size_t>(P->nbatch) * big_fwd_rows(P, sizeof(T)) * k * sizeof(T);
+  if (fsb > P->fs_bytes) {
+    HIP_TRY(hipStreamSynchronize(st));
+    if (P->d_fs) HIP_TRY(hipFree(P->d_fs));
+    P->d_fs = nullptr;
+    P->fs_bytes = 0;
+    HIP_TRY(hipMalloc(&P->d_fs, fsb));
+    P->fs_bytes = fsb;
+  }
   if (P->upd_k >= k) return DCX_OK;
   const size_t ub0 = static_cast<size_t>(P->nbatch) * S.upd_rows[0] * k * sizeof(T);
   const size_t ub1 = static_cast<size_t>(P->nbatch) * S.upd_rows[1] * k * sizeof(T);
@@ -1630,6 +1981,33 @@ int ensure_upd(dcx_plan* P, int k, hipStream_t st) {
   HIP_TRY(hipMalloc(&P->d_upd[1], std::max<size_t>(ub1, 256)));
   HIP_TRY(hipMalloc(&P->d_ufz, std::max<size_t>(ubf, 256)));
   P->upd_k = k;
+  return DCX_OK;
+}
+
+// forward sweep of a level with fronts too tall for the in-LDS kernel
+template <typename T>
+int fwd_big(dcx_plan* P, T* W, int k, int d, const int* lvl, int cnt, int maxnf,
+            const int* act, const SweepIO& io, hipStream_t st) {
+  const auto& S = P->S;
+  const int64_t rows = big_fwd_rows(P, sizeof(T));
+  const int64_t fs_bs = rows * k;
+  T* fs = static_cast<T*>(P->d_fs);
+  T* u[2] = {static_cast<T*>(P->d_upd[0]), static_cast<T*>(P->d_upd[1])};
+  const int64_t ubs[2] = {S.upd_rows[0] * k, S.upd_rows[1] * k};
+  const int nzg = (k + 63) / 64;
+  const int tg = cnt * P->nbatch * nzg;
+  hipLaunchKernelGGL(k_fwd_gather_big<T>, dim3(((tg + 7) / 8) * 8), dim3(256), 0, st, P->d_sup,
+                     lvl, cnt, P->d_relpos, P->d_cidx, act, io.B, io.b_bs, io.brow, k,
+                     u[(d + 1) & 1], ubs[(d + 1) & 1], static_cast<const T*>(P->d_ufz),
+                     P->S.f_upd_rows * k, fs, P->d_fs_off, fs_bs, nzg, tg);
+  constexpr int MT = 8;
+  const int nzc = ((k + 15) / 16 + 3) / 4;
+  const int nrg = (maxnf + 16 * MT - 1) / (16 * MT);
+  const int tw = cnt * P->nbatch * nzc * nrg;
+  hipLaunchKernelGGL((k_fwd_wave_big<T, MT>), dim3(((tw + 7) / 8) * 8), dim3(256), 0, st,
+                     P->d_sup, lvl, cnt, static_cast<const T*>(P->d_panels), S.panel_size, fs,
+                     P->d_fs_off, fs_bs, W, static_cast<int64_t>(S.n) * k, k, u[d & 1],
+                     ubs[d & 1], nzc, nzc * nrg, tw);
   return DCX_OK;
 }
 
@@ -1655,8 +2033,12 @@ int sweep_level(dcx_plan* P, T* W, int k, bool fwd, int d, const int* act, const
     maxns = std::max(maxns, S.ns[s]);
     maxpanel = std::max(maxpanel, (S.ns[s] + S.rs[s]) * S.ns[s]);
   }
+  if (!fwd && bwd_wave<T>(P, W, k, maxns, lvl, cnt, act, io, st)) return DCX_OK;
   const int tc = pick_tc(maxnf, maxns, sizeof(T));
-  if (!tc) return fail(DCX_ERR_ARG, "front too large for the in-LDS sweep");
+  if (!tc) {
+    if (fwd) return fwd_big<T>(P, W, k, d, lvl, cnt, maxnf, act, io, st);
+    return fail(DCX_ERR_ARG, "front too large for the in-LDS sweep");
+  }
   switch (tc) {
     case 64: sweep_tc<T, 64>(P, W, k, maxnf, maxns, maxpanel, fwd, d, lvl, cnt, act, io, st); break;
     case 32: sweep_tc<T, 32>(P, W, k, maxnf, maxns, maxpanel, fwd, d, lvl, cnt, act, io, st); break;
@@ -1813,6 +2195,18 @@ int dcx_plan_create(int n, const int32_t* crow, const int32_t* col, int nbatch,
     HIP_TRY(upload(&P->d_pcrow, S.pcrow));
     HIP_TRY(upload(&P->d_pcol, S.pcol));
     HIP_TRY(upload(&P->d_pvidx, S.pval_idx));
+    {
+      std::vector<int64_t> off(S.nsup, 0);
+      for (int d = 0; d < S.ndepth; ++d) {
+        int64_t rows = 0;
+        for (int q = S.lvl_ptr[d]; q < S.lvl_ptr[d + 1]; ++q) {
+          const int sp = S.lvl_sup[q];
+          off[sp] = rows;
+          rows += S.ns[sp] + S.rs[sp];
+        }
+      }
+      HIP_TRY(upload(&P->d_fs_off, off));
+    }
     std::vector<int> vcrow(crow, crow + n + 1), vcol(col, col + crow[n]);
     for (int i = 0; i < n; ++i) P->max_row_nnz = std::max(P->max_row_nnz, crow[i + 1] - crow[i]);
     HIP_TRY(upload(&P->d_crow, vcrow));

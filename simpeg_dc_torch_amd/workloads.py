@@ -206,6 +206,22 @@ def small_3d_case(seed=0):
     return mesh, survey, m
 
 
+def medium_3d_case(seed=1):
+    """3-D problem whose upper fronts exceed the in-LDS kernels (FP64 factor
+    and forward sweep take the large-front path): 16 x 16 x 8 core of 1 m
+    cells, 6 padding cells growing 1.3x (28 x 28 x 14 cells); a 5 x 5
+    electrode grid at 2 m, every electrode a pole source read by its 8
+    nearest x-dipoles."""
+    mesh = TensorMesh([[(1.0, 6, -1.3), (1.0, 16), (1.0, 6, 1.3)],
+                       [(1.0, 6, -1.3), (1.0, 16), (1.0, 6, 1.3)],
+                       [(1.0, 6, -1.3), (1.0, 8)]], origin="CCN")
+    el = surface_grid(5, 5, 2.0)
+    survey = pole_dipole_survey(el, el, x_dipoles(5, 5), 8)
+    rng = np.random.default_rng(seed)
+    m = gaussian_random_field(mesh, rng, corr_len=4.0, std=0.5, dx=0.5)
+    return mesh, survey, m
+
+
 def grid_3d_case(seed=3):
     """The 3-D L-BFGS workload: 40 x 40 x 30 core of 5 m cells plus 8
     padding cells per side growing 1.3x (56 x 56 x 38 = 119,168 cells); 64
@@ -226,4 +242,5 @@ def grid_3d_case(seed=3):
 
 
 CASES = {"regression": regression_case, "suite": suite_case,
-         "small_padded": small_padded_case, "small_3d": small_3d_case}
+         "small_padded": small_padded_case, "small_3d": small_3d_case,
+         "medium_3d": medium_3d_case}

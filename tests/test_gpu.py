@@ -313,6 +313,26 @@ def test_engine3d_against_oracle(precision, tol_d, tol_g):
     assert rel(g.cpu().numpy(), g_o) < tol_g
 
 
+@pytest.mark.parametrize("precision,tol_d,tol_g", [("fp64", 1e-10, 1e-9), ("mixed", 1e-6, 1e-3)])
+def test_engine3d_large_fronts(precision, tol_d, tol_g):
+    """3-D fronts too tall for the in-LDS kernels (largest front 645 rows,
+    870 with the inverse rows): the per-panel factor launches, the row-group
+    backward sweep and the two-step forward sweep, against the oracle."""
+    from oracle.dc3d import Problem3D
+    from simpeg_dc_torch_amd import TorchDC3D
+    _say(f"engine 3-D medium {precision}")
+    mesh, survey, m = case("medium_3d")
+    eng = TorchDC3D(mesh, survey, device=DEV, precision=precision, refine=0)
+    O = Problem3D(mesh.h[0], mesh.h[1], mesh.h[2], mesh.origin, survey)
+    mt = torch.tensor(m, device=DEV)
+    d_o = O.dpred(m)
+    assert rel(eng.dpred(mt).cpu().numpy(), d_o) < tol_d
+    dobs, w = noisy_data(d_o)
+    _, g = eng.misfit_and_grad(mt, torch.tensor(dobs, device=DEV), torch.tensor(w, device=DEV))
+    _, g_o, _ = O.misfit_and_grad(m, dobs, w)
+    assert rel(g.cpu().numpy(), g_o) < tol_g
+
+
 def test_line25k_subset_against_oracle_and_fd():
     """Full 25k-cell line: dpred of a subset of sources against the oracle
     (same mesh and quadrature), and the gradient against central differences
