@@ -777,7 +777,8 @@ __global__ __launch_bounds__(kThreads) void k_bwd_wave(
     const int* __restrict__ ridx, const int* __restrict__ active,
     const T* __restrict__ panelsT, int64_t panel_bs, T* __restrict__ W, int64_t w_bs, int k,
     double* __restrict__ X64, const int* __restrict__ xrow, int xmode, int cnt, int nzb,
-    int total, int nzc) {
+    int total, int nzc, T* __restrict__ xs, const int64_t* __restrict__ xs_off,
+    int64_t xs_bs) {
   constexpr int CW = 16 * V;
   const SweepBlock sb = sweep_block(lvl, cnt, nzb, total);
   if (!sb.live) return;
@@ -875,7 +876,10 @@ __global__ __launch_bounds__(kThreads) void k_bwd_wave(
     for (int i = 0; i < 4; ++i) {
       const int r = r0 + t * 16 + Mma<T>::row(lane, i);
       if (r >= ns) continue;
-      T* dst = Wb + static_cast<int64_t>(S.first + r) * k + cl;
+      // with row groups the other groups still read y_J from W: the result
+      // goes to the scratch block xs, copied into W by k_bwd_copy_big
+      T* dst = xs ? xs + sb.b * xs_bs + (xs_off[sb.s] + r) * k + cl
+                  : Wb + static_cast<int64_t>(S.first + r) * k + cl;
       if (navail >= V) {
         typename Vec<T, V>::v o;
         if constexpr (V == 1) {
@@ -890,7 +894,7 @@ __global__ __launch_bounds__(kThreads) void k_bwd_wave(
         for (int v = 0; v < V; ++v)
           if (v < navail) dst[v] = acc[t][v][i];
       }
-      if (xmode) {
+      if (xmode && !xs) {
         const int gr = xrow ? xrow[S.first + r] : S.first + r;
         double* xp = X64 + sb.b * w_bs + static_cast<int64_t>(gr) * k + cl;
 #pragma unroll
@@ -923,6 +927,35 @@ __device__ inline LevelBlock level_block(int cnt, int nzb, int total) {
   o.q = (Lc / nzb) % cnt;
   o.b = Lc / nzb / cnt;
   return o;
+}
+
+// x_J of the row-grouped supernodes of a level: scratch -> W (and the FP64
+// solution, xmode 1 store / 2 accumulate); one workgroup per (64 columns,
+// supernode, batch member), grid as level_block
+template <typename T>
+__global__ __launch_bounds__(256) void k_bwd_copy_big(
+    const SupDev* __restrict__ sup, const int* __restrict__ lvl, int cnt,
+    const T* __restrict__ xs, const int64_t* __restrict__ xs_off, int64_t xs_bs,
+    T* __restrict__ W, int64_t w_bs, int k, double* __restrict__ X64,
+    const int* __restrict__ xrow, int xmode, int nzb, int total) {
+  const LevelBlock lb = level_block(cnt, nzb, total);
+  if (!lb.live) return;
+  const int s = lvl[lb.q];
+  const SupDev S = sup[s];
+  const int col0 = lb.z * 64;
+  const T* x = xs + lb.b * xs_bs + xs_off[s] * k;
+  T* Wb = W + lb.b * w_bs;
+  for (int idx = threadIdx.x; idx < S.ns * 64; idx += 256) {
+    const int r = idx >> 6, c = col0 + (idx & 63);
+    if (c >= k) continue;
+    const T v = x[static_cast<int64_t>(r) * k + c];
+    Wb[static_cast<int64_t>(S.first + r) * k + c] = v;
+    if (xmode) {
+      const int gr = xrow ? xrow[S.first + r] : S.first + r;
+      double* xp = X64 + lb.b * w_bs + static_cast<int64_t>(gr) * k + c;
+      *xp = (xmode == 2 ? *xp : 0.0) + static_cast<double>(v);
+    }
+  }
 }
 
 template <typename T>
@@ -1817,6 +1850,27 @@ void sweep_launch(dcx_plan* P, T* W, int k, int maxnf, int maxns, int pmax, int 
 constexpr size_t kSweepLds = 160 * 1024;
 
 
+int pick_tc(int maxnf, int maxns, size_t tsize);
+
+// rows of the widest level whose forward sweep takes the large-front path or
+// whose backward sweep is row-grouped (both use the scratch block d_fs)
+inline int64_t big_fwd_rows(const dcx_plan* P, size_t tsize) {
+  const auto& S = P->S;
+  int64_t best = 0;
+  for (int d = 0; d < S.ndepth; ++d) {
+    int maxnf = 0, maxns = 0;
+    int64_t rows = 0;
+    for (int q = S.lvl_ptr[d]; q < S.lvl_ptr[d + 1]; ++q) {
+      const int s = S.lvl_sup[q];
+      maxnf = std::max(maxnf, S.ns[s] + S.rs[s]);
+      maxns = std::max(maxns, S.ns[s]);
+      rows += S.ns[s] + S.rs[s];
+    }
+    if (pick_tc(maxnf, maxns, tsize) == 0 || maxns > 128) best = std::max(best, rows);
+  }
+  return best;
+}
+
 template <typename T, int MT, int V, int KS, int PF>
 void bwd_wave_launch(dcx_plan* P, T* W, int k, const int* lvl, int cnt, const int* active,
                      const SweepIO& io, hipStream_t st, int nrg) {
@@ -1828,10 +1882,19 @@ void bwd_wave_launch(dcx_plan* P, T* W, int k, const int* lvl, int cnt, const in
   dim3 grid(((total + 7) / 8) * 8);
   const size_t lds = KS > 1 ? static_cast<size_t>(KS) * MT * V * 4 * 64 * sizeof(T) : 0;
   const int64_t w_bs = static_cast<int64_t>(P->S.n) * k;
+  T* xs = nrg > 1 ? static_cast<T*>(P->d_fs) : nullptr;
+  const int64_t xs_bs = big_fwd_rows(P, sizeof(T)) * k;
   hipLaunchKernelGGL((k_bwd_wave<T, MT, V, KS, PF>), grid, dim3(kThreads), lds, st, P->d_sup,
                      lvl, P->d_ridx, active, static_cast<const T*>(P->d_panelsT),
                      P->S.panel_size, W, w_bs, k, io.X64, io.xrow, io.xmode, cnt, nzb, total,
-                     nzc);
+                     nzc, xs, P->d_fs_off, xs_bs);
+  if (xs) {
+    const int nzg = (k + 63) / 64;
+    const int tg = cnt * P->nbatch * nzg;
+    hipLaunchKernelGGL(k_bwd_copy_big<T>, dim3(((tg + 7) / 8) * 8), dim3(256), 0, st, P->d_sup,
+                       lvl, cnt, xs, P->d_fs_off, xs_bs, W, w_bs, k, io.X64, io.xrow, io.xmode,
+                       nzg, tg);
+  }
 }
 
 template <typename T, int V, int KS, int PF>
@@ -1936,26 +1999,6 @@ int launch_fused(dcx_plan* P, T* W, int k, bool fwd, const int* active, const Sw
 
 // update buffers of the forward sweep for k right-hand sides (grown on
 // demand; growing synchronises the stream once)
-int pick_tc(int maxnf, int maxns, size_t tsize);
-
-// rows of the widest level whose forward sweep takes the large-front path
-inline int64_t big_fwd_rows(const dcx_plan* P, size_t tsize) {
-  const auto& S = P->S;
-  int64_t best = 0;
-  for (int d = 0; d < S.ndepth; ++d) {
-    int maxnf = 0, maxns = 0;
-    int64_t rows = 0;
-    for (int q = S.lvl_ptr[d]; q < S.lvl_ptr[d + 1]; ++q) {
-      const int s = S.lvl_sup[q];
-      maxnf = std::max(maxnf, S.ns[s] + S.rs[s]);
-      maxns = std::max(maxns, S.ns[s]);
-      rows += S.ns[s] + S.rs[s];
-    }
-    if (pick_tc(maxnf, maxns, tsize) == 0) best = std::max(best, rows);
-  }
-  return best;
-}
-
 template <typename T>
 int ensure_upd(dcx_plan* P, int k, hipStream_t st) {
   const auto& S = P->S;
