@@ -403,40 +403,113 @@ __global__ __launch_bounds__(NT) void k_factor_level(
 
 // ---------------------------------------------------------------------------
 // Fronts too tall for the in-LDS panel (3-D separators of thousands of rows):
-// the same elimination as separate launches per 32-column panel --
-// k_front_init_big (assembly, one workgroup per front), per panel
-// k_panel_big (the diagonal block by wavefront 0, then every row below it
-// streamed through registers and solved, in place in F) and k_trail_big (the
-// trailing update as 64x64 MFMA tiles over a grid of workgroups, operands
-// read from F), then k_front_out_big (the sweep panels).
+// the same elimination as grid-wide launches, so that one front spreads over
+// the whole chip --
+//   k_front_zero_big   workgroup per 16 front columns: zero, E identity, and
+//                      the A entries of those columns (sorted by column);
+//   k_extend_big       per child index k, the k-th child's update block added
+//                      in 64x64 tiles (one child's map is injective, so the
+//                      tiles never collide and the sum order is fixed);
+//   per 32-column panel:
+//   k_panel_rows_big   workgroup per 256 panel rows: wavefront 0 factors the
+//                      diagonal block (read only, so every workgroup can), then
+//                      each thread solves one row in place in F;
+//   k_trail_big        the trailing update as 64x64 MFMA tiles;
+//   k_front_out_big    the sweep panels as 64x64 tiles, transposed via LDS.
 // ---------------------------------------------------------------------------
+constexpr int kZeroCols = 16;
 template <typename T>
-__global__ __launch_bounds__(1024) void k_front_init_big(
-    const SupDev* __restrict__ sup, const int* __restrict__ lvl,
-    const int* __restrict__ relpos, const int* __restrict__ cidx,
-    const int* __restrict__ a_e, const int* __restrict__ a_row,
-    const int* __restrict__ a_col, const double* __restrict__ vals, int64_t nnz,
-    T* __restrict__ fcur, int64_t fcur_bs, const T* __restrict__ fprev, int64_t fprev_bs) {
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  const SupDev S = sup[lvl[blockIdx.x]];
-  const int b = blockIdx.y;
-  front_init<T, 1024>(S, b, sup, relpos, cidx, a_e, a_row, a_col, vals, nnz,
-                      fcur + b * fcur_bs + S.front_off, fprev, fprev_bs,
-                      reinterpret_cast<int*>(smem_raw));
+__global__ __launch_bounds__(256) void k_front_zero_big(
+    const SupDev* __restrict__ sup, const int* __restrict__ lvl, const int* __restrict__ a_e,
+    const int* __restrict__ a_row, const int* __restrict__ a_col,
+    const double* __restrict__ vals, int64_t nnz, T* __restrict__ fcur, int64_t fcur_bs,
+    int nbatch) {
+  const int f = blockIdx.y / nbatch, b = blockIdx.y % nbatch;
+  const SupDev S = sup[lvl[f]];
+  const int nf = S.ns + S.rs, ld = nf + S.ns;
+  const int c0 = blockIdx.x * kZeroCols;
+  if (c0 >= nf) return;
+  const int c1 = min(nf, c0 + kZeroCols);
+  T* F = fcur + b * fcur_bs + S.front_off;
+  const int64_t base = static_cast<int64_t>(c0) * ld, n = static_cast<int64_t>(c1 - c0) * ld;
+  for (int64_t i = threadIdx.x; i < n; i += 256) {
+    const int64_t idx = base + i;
+    const int col = static_cast<int>(idx / ld), row = static_cast<int>(idx - int64_t(col) * ld);
+    F[idx] = row == nf + col ? T(1) : T(0);  // E row nf + j holds the identity column j
+  }
+  // the entries of columns [c0, c1): a_col ascends within a supernode
+  int lo = S.a_begin, hi = S.a_end;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (a_col[mid] < c0) lo = mid + 1; else hi = mid;
+  }
+  int end = lo, hi2 = S.a_end;
+  while (end < hi2) {
+    const int mid = (end + hi2) >> 1;
+    if (a_col[mid] < c1) end = mid + 1; else hi2 = mid;
+  }
+  __syncthreads();
+  const double* v = vals + b * nnz;
+  for (int q = lo + threadIdx.x; q < end; q += 256)
+    F[a_row[q] + static_cast<int64_t>(a_col[q]) * ld] = static_cast<T>(v[a_e[q]]);
 }
 
 template <typename T>
-__global__ __launch_bounds__(1024) void k_panel_big(const SupDev* __restrict__ sup,
-                                                   const int* __restrict__ lvl,
-                                                   T* __restrict__ fcur, int64_t fcur_bs,
-                                                   int j0, int* __restrict__ info) {
+__global__ __launch_bounds__(256) void k_extend_big(
+    const SupDev* __restrict__ sup, const int* __restrict__ lvl, const int* __restrict__ relpos,
+    const int* __restrict__ cidx, T* __restrict__ fcur, int64_t fcur_bs,
+    const T* __restrict__ fprev, int64_t fprev_bs, int kchild, int nbatch) {
+  __shared__ int rp[2][64];
+  const int f = blockIdx.z / nbatch, b = blockIdx.z % nbatch;
+  const SupDev S = sup[lvl[f]];
+  if (S.c_begin + kchild >= S.c_end) return;
+  const SupDev C = sup[cidx[S.c_begin + kchild]];
+  const int rc = C.rs, ldc = C.ns + C.rs + C.ns, ld = S.ns + S.rs + S.ns;
+  const int i0 = blockIdx.x * 64, j0 = blockIdx.y * 64;
+  if (i0 >= rc || j0 >= rc || i0 + 63 < j0) return;  // outside, or above the diagonal
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  if (tid < 128) {
+    const int q = (tid < 64 ? i0 : j0) + lane;
+    rp[tid >> 6][lane] = q < rc ? relpos[C.rptr + q] : 0;
+  }
+  __syncthreads();
+  const T* Fc = fprev + b * fprev_bs + C.front_off + C.ns + static_cast<int64_t>(C.ns) * ldc;
+  T* F = fcur + b * fcur_bs + S.front_off;
+  const int i = i0 + lane;
+  constexpr int kCols = 16;  // columns per wavefront, loads first
+  T cv[kCols], fv[kCols];
+  int64_t fa[kCols];
+#pragma unroll
+  for (int u = 0; u < kCols; ++u) {
+    const int jj = w * kCols + u, j = j0 + jj;
+    fa[u] = -1;
+    if (i < rc && j < rc && i >= j) {
+      fa[u] = rp[0][lane] + static_cast<int64_t>(rp[1][jj]) * ld;
+      cv[u] = Fc[i + static_cast<int64_t>(j) * ldc];
+      fv[u] = F[fa[u]];
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < kCols; ++u)
+    if (fa[u] >= 0) F[fa[u]] = fv[u] + cv[u];
+}
+
+constexpr int kRowsBig = 256;
+template <typename T>
+__global__ __launch_bounds__(kRowsBig) void k_panel_rows_big(const SupDev* __restrict__ sup,
+                                                            const int* __restrict__ lvl,
+                                                            T* __restrict__ fcur, int64_t fcur_bs,
+                                                            int j0, int nbatch,
+                                                            int* __restrict__ info) {
   __shared__ T ldg[kNB * kLDG];
   __shared__ T dgi[kNB];
-  const SupDev S = sup[lvl[blockIdx.x]];
-  const int b = blockIdx.y;
+  const int f = blockIdx.y / nbatch, b = blockIdx.y % nbatch;
+  const SupDev S = sup[lvl[f]];
   const int ns = S.ns, nf = S.ns + S.rs, ld = nf + S.ns;
   if (j0 >= ns) return;  // this front has no panel j0 (uniform)
   const int jb = min(kNB, ns - j0), m = ld - j0;
+  const int rbase = jb + blockIdx.x * kRowsBig;
+  if (rbase >= m) return;
   const int tid = threadIdx.x, lane = tid & 63;
   T* Pn = fcur + b * fcur_bs + S.front_off + j0 + static_cast<int64_t>(j0) * ld;
   if (tid < 64) {
@@ -445,14 +518,10 @@ __global__ __launch_bounds__(1024) void k_panel_big(const SupDev* __restrict__ s
     for (int c = 0; c < kNB; ++c)
       rv[c] = (lane < jb && c < jb && lane >= c) ? Pn[lane + static_cast<int64_t>(c) * ld] : T(0);
     diag_block<T>(rv, jb, lane, ldg, dgi, info);
-    if (lane < jb) {
-#pragma unroll
-      for (int c = 0; c < kNB; ++c)
-        if (c <= lane) Pn[lane + static_cast<int64_t>(c) * ld] = rv[c];
-    }
   }
   __syncthreads();
-  for (int r = jb + tid; r < m; r += 1024) {
+  const int r = rbase + tid;
+  if (r < m) {
     T rv[kNB];
 #pragma unroll
     for (int c = 0; c < kNB; ++c) rv[c] = c < jb ? Pn[r + static_cast<int64_t>(c) * ld] : T(0);
@@ -512,17 +581,43 @@ __global__ __launch_bounds__(256) void k_trail_big(const SupDev* __restrict__ su
     }
 }
 
+// the sweep panels of a big front: tile (x, y) holds panel rows r in
+// [64x, 64x+64) and columns c in [64y, 64y+64), read from the E rows (F row
+// nf + c, column r; contiguous in c) straight into PT and, transposed through
+// LDS, into P (contiguous in r)
 template <typename T>
-__global__ __launch_bounds__(1024) void k_front_out_big(const SupDev* __restrict__ sup,
-                                                       const int* __restrict__ lvl,
-                                                       const T* __restrict__ fcur,
-                                                       int64_t fcur_bs, T* __restrict__ panels,
-                                                       T* __restrict__ panelsT,
-                                                       int64_t panel_bs) {
-  const SupDev S = sup[lvl[blockIdx.x]];
-  const int b = blockIdx.y;
-  panel_out<T, 1024>(S, fcur + b * fcur_bs + S.front_off, panels + b * panel_bs + S.panel_off,
-                     panelsT + b * panel_bs + S.panel_off);
+__global__ __launch_bounds__(256) void k_front_out_big(const SupDev* __restrict__ sup,
+                                                      const int* __restrict__ lvl,
+                                                      const T* __restrict__ fcur,
+                                                      int64_t fcur_bs, T* __restrict__ panels,
+                                                      T* __restrict__ panelsT, int64_t panel_bs,
+                                                      int nbatch) {
+  __shared__ T tile[64][65];
+  const int f = blockIdx.z / nbatch, b = blockIdx.z % nbatch;
+  const SupDev S = sup[lvl[f]];
+  const int ns = S.ns, nf = S.ns + S.rs, ld = nf + S.ns;
+  const int r0 = blockIdx.x * 64, c0 = blockIdx.y * 64;
+  if (r0 >= nf || c0 >= ns) return;
+  const T* F = fcur + b * fcur_bs + S.front_off;
+  T* P = panels + b * panel_bs + S.panel_off;
+  T* PT = panelsT + b * panel_bs + S.panel_off;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int rr = w; rr < 64; rr += 4) {
+    const int r = r0 + rr, c = c0 + lane;
+    T v = T(0);
+    if (r < nf && c < ns) {
+      v = F[(nf + c) + static_cast<int64_t>(r) * ld];
+      if (r < ns && r < c) v = T(0);
+      if (r >= ns) v = -v;
+      PT[static_cast<int64_t>(r) * ns + c] = v;
+    }
+    tile[rr][lane] = v;
+  }
+  __syncthreads();
+  for (int cc = w; cc < 64; cc += 4) {
+    const int c = c0 + cc, r = r0 + lane;
+    if (r < nf && c < ns) P[r + static_cast<int64_t>(c) * nf] = tile[lane][cc];
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1741,38 +1836,50 @@ void factor_launch(dcx_plan* P, int d, int cnt, size_t lds, hipStream_t st) {
                      static_cast<T*>(P->d_panelsT), S.panel_size, P->d_info);
 }
 
-// a level with fronts too tall for k_factor_level: launches per panel
+// a level with fronts too tall for k_factor_level: grid-wide launches
 template <typename T>
 int factor_level_big(dcx_plan* P, int d, int cnt, hipStream_t st) {
   const auto& S = P->S;
   T* fr[2] = {static_cast<T*>(P->d_front[0]), static_cast<T*>(P->d_front[1])};
   const int64_t fbs[2] = {S.front_size[0], S.front_size[1]};
   const int* lvl = P->d_lvl + S.lvl_ptr[d];
-  int maxns = 0, maxnf = 0, maxrc = 0;
+  const int nb = P->nbatch;
+  int maxns = 0, maxnf = 0, maxkids = 0;
+  std::vector<int> maxrc;
   for (int q = S.lvl_ptr[d]; q < S.lvl_ptr[d + 1]; ++q) {
     const int s = S.lvl_sup[q];
     maxns = std::max(maxns, S.ns[s]);
     maxnf = std::max(maxnf, S.ns[s] + S.rs[s]);
-    for (int c = S.cptr[s]; c < S.cptr[s + 1]; ++c) maxrc = std::max(maxrc, S.rs[S.cidx[c]]);
+    const int nk = S.cptr[s + 1] - S.cptr[s];
+    if (nk > maxkids) maxrc.resize(nk, 0);
+    maxkids = std::max(maxkids, nk);
+    for (int c = 0; c < nk; ++c) maxrc[c] = std::max(maxrc[c], S.rs[S.cidx[S.cptr[s] + c]]);
   }
-  if (static_cast<size_t>(maxrc) * sizeof(int) > 160 * 1024)
-    return fail(DCX_ERR_ARG, "child update block too large for the factor kernel");
-  dim3 grid(cnt, P->nbatch);
-  hipLaunchKernelGGL(k_front_init_big<T>, grid, dim3(1024), std::max<size_t>(maxrc, 1) * sizeof(int),
-                     st, P->d_sup, lvl, P->d_relpos, P->d_cidx, P->d_a_e, P->d_a_row, P->d_a_col,
-                     P->d_vals, static_cast<int64_t>(S.nnz), fr[d & 1], fbs[d & 1],
-                     fr[(d + 1) & 1], fbs[(d + 1) & 1]);
+  if (static_cast<int64_t>(cnt) * nb > 65535)
+    return fail(DCX_ERR_ARG, "too many large fronts in one level");
+  hipLaunchKernelGGL(k_front_zero_big<T>, dim3((maxnf + kZeroCols - 1) / kZeroCols, cnt * nb),
+                     dim3(256), 0, st, P->d_sup, lvl, P->d_a_e, P->d_a_row, P->d_a_col, P->d_vals,
+                     static_cast<int64_t>(S.nnz), fr[d & 1], fbs[d & 1], nb);
+  for (int c = 0; c < maxkids; ++c) {
+    if (!maxrc[c]) continue;
+    const int nt = (maxrc[c] + 63) / 64;
+    hipLaunchKernelGGL(k_extend_big<T>, dim3(nt, nt, cnt * nb), dim3(256), 0, st, P->d_sup, lvl,
+                       P->d_relpos, P->d_cidx, fr[d & 1], fbs[d & 1], fr[(d + 1) & 1],
+                       fbs[(d + 1) & 1], c, nb);
+  }
   for (int j0 = 0; j0 < maxns; j0 += kNB) {
-    hipLaunchKernelGGL(k_panel_big<T>, grid, dim3(1024), 0, st, P->d_sup, lvl, fr[d & 1],
-                       fbs[d & 1], j0, P->d_info);
     const int rows = maxnf + maxns - j0, cols = maxnf - j0;  // tallest m and widest n1
-    dim3 tg((rows + 63) / 64, (cols + 63) / 64, cnt * P->nbatch);
+    hipLaunchKernelGGL(k_panel_rows_big<T>, dim3((rows + kRowsBig - 1) / kRowsBig, cnt * nb),
+                       dim3(kRowsBig), 0, st, P->d_sup, lvl, fr[d & 1], fbs[d & 1], j0, nb,
+                       P->d_info);
+    dim3 tg((rows + 63) / 64, (cols + 63) / 64, cnt * nb);
     hipLaunchKernelGGL(k_trail_big<T>, tg, dim3(256), 0, st, P->d_sup, lvl, fr[d & 1],
-                       fbs[d & 1], j0, P->nbatch);
+                       fbs[d & 1], j0, nb);
   }
-  hipLaunchKernelGGL(k_front_out_big<T>, grid, dim3(1024), 0, st, P->d_sup, lvl, fr[d & 1],
-                     fbs[d & 1], static_cast<T*>(P->d_panels), static_cast<T*>(P->d_panelsT),
-                     S.panel_size);
+  hipLaunchKernelGGL(k_front_out_big<T>, dim3((maxnf + 63) / 64, (maxns + 63) / 64, cnt * nb),
+                     dim3(256), 0, st, P->d_sup, lvl, fr[d & 1], fbs[d & 1],
+                     static_cast<T*>(P->d_panels), static_cast<T*>(P->d_panelsT), S.panel_size,
+                     nb);
   return DCX_OK;
 }
 

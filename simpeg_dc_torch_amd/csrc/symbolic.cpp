@@ -516,6 +516,26 @@ std::string analyse(int n, const int* crow, const int* col, const double* coords
         S.a_row[k] = lr;
         S.a_col[k] = pj - S.first[s];
       }
+    // column order within a supernode: the large-front assembly finds the
+    // entries of a column range by binary search
+    std::vector<int> ord;
+    for (int s = 0; s < ns_; ++s) {
+      const int a0 = S.aptr[s], a1 = S.aptr[s + 1];
+      ord.resize(a1 - a0);
+      for (int k = a0; k < a1; ++k) ord[k - a0] = k;
+      std::sort(ord.begin(), ord.end(), [&](int x, int y) {
+        return S.a_col[x] != S.a_col[y] ? S.a_col[x] < S.a_col[y] : S.a_row[x] < S.a_row[y];
+      });
+      std::vector<int> e(a1 - a0), r(a1 - a0), c(a1 - a0);
+      for (int k = 0; k < a1 - a0; ++k) {
+        e[k] = S.a_e[ord[k]];
+        r[k] = S.a_row[ord[k]];
+        c[k] = S.a_col[ord[k]];
+      }
+      std::copy(e.begin(), e.end(), S.a_e.begin() + a0);
+      std::copy(r.begin(), r.end(), S.a_row.begin() + a0);
+      std::copy(c.begin(), c.end(), S.a_col.begin() + a0);
+    }
   }
   S.pcrow.assign(n + 1, 0);
   for (int i = 0; i < n; ++i) S.pcrow[S.iperm[i] + 1] = crow[i + 1] - crow[i];
