@@ -192,39 +192,63 @@ __device__ void front_init(const SupDev& S, int b, const SupDev* __restrict__ su
   __syncthreads();
 }
 
+// pivot d -> (sqrt d, 1 / sqrt d) from the hardware reciprocal square root
+// (about 23 bits in FP64) refined by two Newton steps
+// y <- y + y (1 - d y^2) / 2, each doubling the correct bits; an FP32 pivot
+// goes through the FP64 steps and is rounded once
+__device__ inline void pivot(double d, double& sq, double& dinv) {
+  double y = __builtin_amdgcn_rsq(d);
+#pragma unroll
+  for (int it = 0; it < 2; ++it) {
+    const double e = fma(-(d * y), y, 1.0);
+    y = fma(0.5 * y, e, y);
+  }
+  dinv = y;
+  sq = d * y;
+}
+__device__ inline void pivot(float d, float& sq, float& dinv) {
+  double s, y;
+  pivot(static_cast<double>(d), s, y);
+  sq = static_cast<float>(s);
+  dinv = static_cast<float>(y);
+}
+
 // the jb x jb diagonal block of a panel, by one wavefront: lanes < jb hold its
-// rows in rv, pivot columns are read across lanes (no LDS, no barrier); the
-// L rows go to ldg (stride kLDG) and the reciprocal diagonal to dgi
+// rows in rv; each pivot column is broadcast through ldg's first row (free
+// until the end) and read back as vectors; the L rows go to ldg (stride
+// kLDG) and the reciprocal diagonal to dgi
 template <typename T>
 __device__ inline void diag_block(T (&rv)[kNB], int jb, int lane, T* ldg, T* dgi,
                                   int* __restrict__ info) {
+  T mydinv = T(0);
 #pragma unroll
   for (int t = 0; t < kNB; ++t) {
     if (t < jb) {  // uniform
+      if (lane < kNB) ldg[lane] = rv[t];
+      __builtin_amdgcn_wave_barrier();
       T d = lane_value(rv[t], t);
       if (!(d > T(0))) {  // uniform
         if (lane == 0) atomicMax(info, 1);
         d = T(1);
       }
-      const T sq = sqrt(d);
-      const T dinv = T(1) / sq;
-      T cl[kNB];  // column t of rows l > t, unscaled
+      T sq, dinv;
+      pivot(d, sq, dinv);
+      // rank-1 update by column t of the rows l > t (unscaled); lanes
+      // outside the update take g = 0, an exact no-op, so that the update
+      // needs no per-entry select
+      const bool below = lane > t && lane < jb;
+      const T g = below ? rv[t] * (dinv * dinv) : T(0);
 #pragma unroll
-      for (int l = t + 1; l < kNB; ++l) cl[l] = lane_value(rv[t], l);
-      if (lane == t) {
-        rv[t] = sq;
-        dgi[t] = dinv;
-      } else if (lane > t && lane < jb) {
-        const T g = rv[t] * (dinv * dinv);
-        rv[t] *= dinv;
-#pragma unroll
-        for (int l = t + 1; l < kNB; ++l) rv[l] -= g * cl[l];
-      }
+      for (int l = t + 1; l < kNB; ++l) rv[l] -= g * ldg[l];
+      __builtin_amdgcn_wave_barrier();
+      rv[t] = lane == t ? sq : (below ? rv[t] * dinv : rv[t]);
+      mydinv = lane == t ? dinv : mydinv;
     }
   }
   if (lane < jb) {
 #pragma unroll
     for (int c = 0; c < kNB; ++c) ldg[lane * kLDG + c] = rv[c];
+    dgi[lane] = mydinv;
   }
 }
 
@@ -410,7 +434,7 @@ __global__ __launch_bounds__(NT) void k_factor_level(
 //   k_extend_big       per child index k, the k-th child's update block added
 //                      in 64x64 tiles (one child's map is injective, so the
 //                      tiles never collide and the sum order is fixed);
-//   per 32-column panel:
+//   per 64-column panel:
 //   k_panel_rows_big   workgroup per 256 panel rows: wavefront 0 factors the
 //                      diagonal block (read only, so every workgroup can), then
 //                      each thread solves one row in place in F;
@@ -494,41 +518,163 @@ __global__ __launch_bounds__(256) void k_extend_big(
     if (fa[u] >= 0) F[fa[u]] = fv[u] + cv[u];
 }
 
+// Big-front panels are kNB2 = 2 kNB columns wide.  Their diagonal block is
+// factored as two kNB halves, D1 = L11 L11^T, L21 = A21 L11^{-T},
+// D2 = A22 - L21 L21^T = L22 L22^T, by wavefront 0 of one workgroup
+// (panel_diag), into the image [L11 | 1/diag L11 | L21 | L22 | 1/diag L22]
+// (rows at stride kLDG) kept per front in global memory.  Panel 0's comes
+// from k_diag_big; panel j0 + kNB2's from workgroup (0, 0) of panel j0's
+// trailing update, which finishes that block (a one-panel lookahead), so the
+// row solve of a panel never waits on the sequential diagonal factor.
 constexpr int kRowsBig = 256;
+constexpr int kNB2 = 2 * kNB;
+constexpr int kDg = 3 * kNB * kLDG + 2 * kNB;
+
+// called by every thread of the workgroup (barriers inside); Pn points at
+// the block's top-left entry in F (column-major, leading dimension ld)
 template <typename T>
-__global__ __launch_bounds__(kRowsBig) void k_panel_rows_big(const SupDev* __restrict__ sup,
-                                                            const int* __restrict__ lvl,
-                                                            T* __restrict__ fcur, int64_t fcur_bs,
-                                                            int j0, int nbatch,
-                                                            int* __restrict__ info) {
-  __shared__ T ldg[kNB * kLDG];
-  __shared__ T dgi[kNB];
-  const int f = blockIdx.y / nbatch, b = blockIdx.y % nbatch;
-  const SupDev S = sup[lvl[f]];
-  const int ns = S.ns, nf = S.ns + S.rs, ld = nf + S.ns;
-  if (j0 >= ns) return;  // this front has no panel j0 (uniform)
-  const int jb = min(kNB, ns - j0), m = ld - j0;
-  const int rbase = jb + blockIdx.x * kRowsBig;
-  if (rbase >= m) return;
+__device__ __forceinline__ void panel_diag(const T* Pn, int ld, int jb, T* sm, int* __restrict__ info) {
+  T* ldg0 = sm;
+  T* dgi0 = ldg0 + kNB * kLDG;
+  T* l21 = dgi0 + kNB;
+  T* ldg1 = l21 + kNB * kLDG;
+  T* dgi1 = ldg1 + kNB * kLDG;
   const int tid = threadIdx.x, lane = tid & 63;
-  T* Pn = fcur + b * fcur_bs + S.front_off + j0 + static_cast<int64_t>(j0) * ld;
+  const int jb1 = min(kNB, jb), jb2 = jb - jb1;
   if (tid < 64) {
     T rv[kNB];
 #pragma unroll
     for (int c = 0; c < kNB; ++c)
-      rv[c] = (lane < jb && c < jb && lane >= c) ? Pn[lane + static_cast<int64_t>(c) * ld] : T(0);
-    diag_block<T>(rv, jb, lane, ldg, dgi, info);
+      rv[c] = (lane < jb1 && c < jb1 && lane >= c) ? Pn[lane + static_cast<int64_t>(c) * ld]
+                                                  : T(0);
+    diag_block<T>(rv, jb1, lane, ldg0, dgi0, info);
   }
   __syncthreads();
-  const int r = rbase + tid;
-  if (r < m) {
-    T rv[kNB];
+  if (jb2 > 0) {
+    if (tid < 64) {
+      T rv[kNB];
 #pragma unroll
-    for (int c = 0; c < kNB; ++c) rv[c] = c < jb ? Pn[r + static_cast<int64_t>(c) * ld] : T(0);
-    row_solve<T>(rv, jb, ldg, dgi);
+      for (int c = 0; c < kNB; ++c)
+        rv[c] = (lane < jb2 && c < jb1) ? Pn[(jb1 + lane) + static_cast<int64_t>(c) * ld] : T(0);
+      row_solve<T>(rv, jb1, ldg0, dgi0);
+      if (lane < jb2) {
+#pragma unroll
+        for (int c = 0; c < kNB; ++c) l21[lane * kLDG + c] = rv[c];
+      }
+    }
+    __syncthreads();
+    if (tid < 64) {
+      T rv[kNB];
+#pragma unroll
+      for (int c = 0; c < kNB; ++c)
+        rv[c] = (lane < jb2 && c < jb2 && lane >= c)
+                    ? Pn[(jb1 + lane) + static_cast<int64_t>(jb1 + c) * ld]
+                    : T(0);
+      // L21 L21^T as 2 x 2 16x16 tiles on the matrix unit, staged in ldg1
+      // (overwritten by the factor below)
+      typename Mma<T>::v4 acc[2][2];
+#pragma unroll
+      for (int tr = 0; tr < 2; ++tr)
+#pragma unroll
+        for (int tc = 0; tc < 2; ++tc) acc[tr][tc] = {T(0), T(0), T(0), T(0)};
+#pragma unroll
+      for (int q0 = 0; q0 < kNB; q0 += 4) {
+        const int q = q0 + (lane >> 4);
+        T op[2];
+#pragma unroll
+        for (int tr = 0; tr < 2; ++tr) {
+          const int rr = 16 * tr + (lane & 15);
+          op[tr] = rr < jb2 ? l21[rr * kLDG + q] : T(0);
+        }
+#pragma unroll
+        for (int tr = 0; tr < 2; ++tr)
+#pragma unroll
+          for (int tc = 0; tc < 2; ++tc) acc[tr][tc] = Mma<T>::op(op[tr], op[tc], acc[tr][tc]);
+      }
+#pragma unroll
+      for (int tr = 0; tr < 2; ++tr)
+#pragma unroll
+        for (int tc = 0; tc < 2; ++tc)
+#pragma unroll
+          for (int e = 0; e < 4; ++e)
+            ldg1[(16 * tr + Mma<T>::row(lane, e)) * kLDG + 16 * tc + (lane & 15)] = acc[tr][tc][e];
+#pragma unroll
+      for (int c = 0; c < kNB; ++c)
+        if (c <= lane && lane < jb2 && c < jb2) rv[c] -= ldg1[lane * kLDG + c];
+      diag_block<T>(rv, jb2, lane, ldg1, dgi1, info);
+    }
+  }
+  __syncthreads();
+}
+
+// panel 0 of every big front: one wavefront per (front, batch member)
+template <typename T>
+__global__ __launch_bounds__(64) void k_diag_big(const SupDev* __restrict__ sup,
+                                                const int* __restrict__ lvl,
+                                                const T* __restrict__ fcur, int64_t fcur_bs,
+                                                T* __restrict__ dg, int nbatch,
+                                                int* __restrict__ info) {
+  __shared__ T sm[kDg];
+  const int f = blockIdx.x / nbatch, b = blockIdx.x % nbatch;
+  const SupDev S = sup[lvl[f]];
+  const int ld = 2 * S.ns + S.rs;
+  panel_diag<T>(fcur + b * fcur_bs + S.front_off, ld, min(kNB2, S.ns), sm, info);
+  T* o = dg + static_cast<int64_t>(blockIdx.x) * kDg;
+  for (int i = threadIdx.x; i < kDg; i += 64) o[i] = sm[i];
+}
+
+// the rows below a panel's diagonal block: workgroup x takes rows
+// [jb + 256 x, jb + 256 x + 256) and solves each against the factored block
+template <typename T>
+__global__ __launch_bounds__(kRowsBig) void k_panel_rows_big(const SupDev* __restrict__ sup,
+                                                            const int* __restrict__ lvl,
+                                                            T* __restrict__ fcur, int64_t fcur_bs,
+                                                            const T* __restrict__ dg, int j0,
+                                                            int nbatch) {
+  __shared__ T sm[kDg];
+  const int f = blockIdx.y / nbatch, b = blockIdx.y % nbatch;
+  const SupDev S = sup[lvl[f]];
+  const int ns = S.ns, nf = S.ns + S.rs, ld = nf + S.ns;
+  if (j0 >= ns) return;  // this front has no panel j0 (uniform)
+  const int jb = min(kNB2, ns - j0), jb1 = min(kNB, jb), jb2 = jb - jb1, m = ld - j0;
+  const int rbase = jb + blockIdx.x * kRowsBig;
+  if (rbase >= m) return;
+  const int tid = threadIdx.x;
+  T* Pn = fcur + b * fcur_bs + S.front_off + j0 + static_cast<int64_t>(j0) * ld;
+  const int r = rbase + tid;
+  T ra[kNB], rb[kNB];
+#pragma unroll
+  for (int c = 0; c < kNB; ++c) {
+    ra[c] = (r < m && c < jb1) ? Pn[r + static_cast<int64_t>(c) * ld] : T(0);
+    rb[c] = (r < m && c < jb2) ? Pn[r + static_cast<int64_t>(jb1 + c) * ld] : T(0);
+  }
+  const T* dgb = dg + static_cast<int64_t>(blockIdx.y) * kDg;
+  for (int i = tid; i < kDg; i += kRowsBig) sm[i] = dgb[i];
+  __syncthreads();
+  if (r >= m) return;
+  const T* ldg0 = sm;
+  const T* dgi0 = ldg0 + kNB * kLDG;
+  const T* l21 = dgi0 + kNB;
+  const T* ldg1 = l21 + kNB * kLDG;
+  const T* dgi1 = ldg1 + kNB * kLDG;
+  row_solve<T>(ra, jb1, ldg0, dgi0);
+#pragma unroll
+  for (int c = 0; c < kNB; ++c)
+    if (c < jb1) Pn[r + static_cast<int64_t>(c) * ld] = ra[c];
+  if (jb2 > 0) {
+#pragma unroll
+    for (int c = 0; c < kNB; ++c) {
+      if (c < jb2) {
+        T s = T(0);
+#pragma unroll
+        for (int q = 0; q < kNB; ++q) s += ra[q] * l21[c * kLDG + q];
+        rb[c] -= s;
+      }
+    }
+    row_solve<T>(rb, jb2, ldg1, dgi1);
 #pragma unroll
     for (int c = 0; c < kNB; ++c)
-      if (c < jb) Pn[r + static_cast<int64_t>(c) * ld] = rv[c];
+      if (c < jb2) Pn[r + static_cast<int64_t>(jb1 + c) * ld] = rb[c];
   }
 }
 
@@ -542,12 +688,13 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_trail_big(const SupDev* __restrict__ sup,
                                                   const int* __restrict__ lvl,
                                                   T* __restrict__ fcur, int64_t fcur_bs,
-                                                  int j0, int nbatch) {
+                                                  int j0, int nbatch, T* __restrict__ dg,
+                                                  int* __restrict__ info) {
   const int f = blockIdx.z / nbatch, b = blockIdx.z % nbatch;
   const SupDev S = sup[lvl[f]];
   const int ns = S.ns, nf = S.ns + S.rs, ld = nf + S.ns;
   if (j0 >= ns) return;
-  const int jb = min(kNB, ns - j0), m = ld - j0, n1 = nf - j0;
+  const int jb = min(kNB2, ns - j0), m = ld - j0, n1 = nf - j0;
   const int rb = jb + blockIdx.x * 64, cb = jb + blockIdx.y * 64;
   if (rb >= m || cb >= n1 || (rb + 63 < n1 && rb + 63 < cb)) return;  // empty or upper
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -557,28 +704,44 @@ __global__ __launch_bounds__(256) void k_trail_big(const SupDev* __restrict__ su
   typename Mma<T>::v4 acc[4];
 #pragma unroll
   for (int t = 0; t < 4; ++t) acc[t] = {T(0), T(0), T(0), T(0)};
+  constexpr int kStage = 8;  // k-steps whose operand loads are issued together
 #pragma unroll
-  for (int k0 = 0; k0 < kNB; k0 += 4) {
-    const int kk = k0 + kq;
-    const bool kok = kk < jb;
-    const T bv = (kok && i < m) ? Pn[i + static_cast<int64_t>(kk) * ld] : T(0);
-    T av[4];
+  for (int k0 = 0; k0 < kNB2; k0 += 4 * kStage) {
+    T bv[kStage], av[kStage][4];
 #pragma unroll
-    for (int t = 0; t < 4; ++t) {
-      const int l = cb + 16 * t + (lane & 15);
-      av[t] = (kok && l < n1) ? Pn[l + static_cast<int64_t>(kk) * ld] : T(0);
+    for (int j = 0; j < kStage; ++j) {
+      const int kk = k0 + 4 * j + kq;
+      const bool kok = kk < jb;
+      bv[j] = (kok && i < m) ? Pn[i + static_cast<int64_t>(kk) * ld] : T(0);
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const int l = cb + 16 * t + (lane & 15);
+        av[j][t] = (kok && l < n1) ? Pn[l + static_cast<int64_t>(kk) * ld] : T(0);
+      }
     }
 #pragma unroll
-    for (int t = 0; t < 4; ++t) acc[t] = Mma<T>::op(av[t], bv, acc[t]);
+    for (int j = 0; j < kStage; ++j)
+#pragma unroll
+      for (int t = 0; t < 4; ++t) acc[t] = Mma<T>::op(av[j][t], bv[j], acc[t]);
   }
-  if (i >= m) return;
+  if (i < m) {
 #pragma unroll
-  for (int t = 0; t < 4; ++t)
+    for (int t = 0; t < 4; ++t)
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      const int l = cb + 16 * t + Mma<T>::row(lane, e);
-      if (l < n1 && (i >= n1 || i >= l)) Pn[i + static_cast<int64_t>(l) * ld] -= acc[t][e];
-    }
+      for (int e = 0; e < 4; ++e) {
+        const int l = cb + 16 * t + Mma<T>::row(lane, e);
+        if (l < n1 && (i >= n1 || i >= l)) Pn[i + static_cast<int64_t>(l) * ld] -= acc[t][e];
+      }
+  }
+  // lookahead: tile (0, 0) is the next panel's diagonal block, now final
+  if (blockIdx.x == 0 && blockIdx.y == 0 && j0 + kNB2 < ns) {
+    __shared__ T sm[kDg];
+    __syncthreads();  // the tile's stores by every wavefront before wavefront 0 reads them
+    panel_diag<T>(Pn + jb + static_cast<int64_t>(jb) * ld, ld, min(kNB2, ns - j0 - kNB2), sm,
+                  info);
+    T* o = dg + static_cast<int64_t>(blockIdx.z) * kDg;
+    for (int q = threadIdx.x; q < kDg; q += 256) o[q] = sm[q];
+  }
 }
 
 // the sweep panels of a big front: tile (x, y) holds panel rows r in
@@ -1025,74 +1188,119 @@ __device__ inline LevelBlock level_block(int cnt, int nzb, int total) {
 }
 
 // x_J of the row-grouped supernodes of a level: scratch -> W (and the FP64
-// solution, xmode 1 store / 2 accumulate); one workgroup per (64 columns,
-// supernode, batch member), grid as level_block
+// solution, xmode 1 store / 2 accumulate); workgroup (x, front x batch, z)
+// copies rows [32x, 32x + 32) and columns [64z, 64z + 64)
+constexpr int kCopyRows = 32;
 template <typename T>
 __global__ __launch_bounds__(256) void k_bwd_copy_big(
-    const SupDev* __restrict__ sup, const int* __restrict__ lvl, int cnt,
-    const T* __restrict__ xs, const int64_t* __restrict__ xs_off, int64_t xs_bs,
+    const SupDev* __restrict__ sup, const int* __restrict__ lvl, const T* __restrict__ xs,
+    const int64_t* __restrict__ xs_off, int64_t xs_bs,
     T* __restrict__ W, int64_t w_bs, int k, double* __restrict__ X64,
-    const int* __restrict__ xrow, int xmode, int nzb, int total) {
-  const LevelBlock lb = level_block(cnt, nzb, total);
-  if (!lb.live) return;
-  const int s = lvl[lb.q];
+    const int* __restrict__ xrow, int xmode, int nbatch) {
+  const int f = blockIdx.y / nbatch, b = blockIdx.y % nbatch;
+  const int s = lvl[f];
   const SupDev S = sup[s];
-  const int col0 = lb.z * 64;
-  const T* x = xs + lb.b * xs_bs + xs_off[s] * k;
-  T* Wb = W + lb.b * w_bs;
-  for (int idx = threadIdx.x; idx < S.ns * 64; idx += 256) {
-    const int r = idx >> 6, c = col0 + (idx & 63);
-    if (c >= k) continue;
-    const T v = x[static_cast<int64_t>(r) * k + c];
-    Wb[static_cast<int64_t>(S.first + r) * k + c] = v;
+  const int r0 = blockIdx.x * kCopyRows, c = blockIdx.z * 64 + (threadIdx.x & 63);
+  if (r0 >= S.ns || c >= k) return;
+  const T* x = xs + b * xs_bs + xs_off[s] * k;
+  T* Wb = W + b * w_bs;
+  constexpr int kU = kCopyRows / 4;
+  T v[kU];
+#pragma unroll
+  for (int u = 0; u < kU; ++u) {
+    const int r = r0 + (threadIdx.x >> 6) + 4 * u;
+    v[u] = r < S.ns ? x[static_cast<int64_t>(r) * k + c] : T(0);
+  }
+#pragma unroll
+  for (int u = 0; u < kU; ++u) {
+    const int r = r0 + (threadIdx.x >> 6) + 4 * u;
+    if (r >= S.ns) continue;
+    Wb[static_cast<int64_t>(S.first + r) * k + c] = v[u];
     if (xmode) {
       const int gr = xrow ? xrow[S.first + r] : S.first + r;
-      double* xp = X64 + lb.b * w_bs + static_cast<int64_t>(gr) * k + c;
-      *xp = (xmode == 2 ? *xp : 0.0) + static_cast<double>(v);
+      double* xp = X64 + b * w_bs + static_cast<int64_t>(gr) * k + c;
+      *xp = (xmode == 2 ? *xp : 0.0) + static_cast<double>(v[u]);
     }
   }
 }
 
+// forward-sweep right-hand side of a big front, f = [B_J; 0] plus the
+// children's updates: k_fwd_init_big fills workgroup x's kFwdRows rows and
+// 64 columns; k_fwd_extend_big adds the kchild-th child's update rows (one
+// child's rows land on distinct front rows, so the workgroups never collide
+// and the sum order is fixed)
+constexpr int kFwdRows = 32;
 template <typename T>
-__global__ __launch_bounds__(256) void k_fwd_gather_big(
-    const SupDev* __restrict__ sup, const int* __restrict__ lvl, int cnt,
-    const int* __restrict__ relpos, const int* __restrict__ cidx,
-    const int* __restrict__ active, const double* __restrict__ Bsrc, int64_t b_bs,
-    const int* __restrict__ brow, int k, const T* __restrict__ uprev, int64_t uprev_bs,
-    const T* __restrict__ ufz, int64_t ufz_bs, T* __restrict__ fs,
-    const int64_t* __restrict__ fs_off, int64_t fs_bs, int nzb, int total) {
-  const LevelBlock lb = level_block(cnt, nzb, total);
-  if (!lb.live) return;
-  const int s = lvl[lb.q];
+__global__ __launch_bounds__(256) void k_fwd_init_big(
+    const SupDev* __restrict__ sup, const int* __restrict__ lvl, const double* __restrict__ Bsrc,
+    int64_t b_bs, const int* __restrict__ brow, int k, T* __restrict__ fs,
+    const int64_t* __restrict__ fs_off, int64_t fs_bs, int nbatch) {
+  const int f = blockIdx.y / nbatch, b = blockIdx.y % nbatch;
+  const int s = lvl[f];
   const SupDev S = sup[s];
   const int ns = S.ns, nf = S.ns + S.rs;
-  const int tid = threadIdx.x;
-  const int col0 = lb.z * 64;
-  T* f = fs + lb.b * fs_bs + fs_off[s] * k;
-  const double* Bb = Bsrc + lb.b * b_bs;
-  for (int idx = tid; idx < nf * 64; idx += 256) {
-    const int r = idx >> 6, c = col0 + (idx & 63);
-    if (c >= k) continue;
-    T v = T(0);
+  const int r0 = blockIdx.x * kFwdRows, col0 = blockIdx.z * 64;
+  if (r0 >= nf) return;
+  T* fo = fs + b * fs_bs + fs_off[s] * k;
+  const double* Bb = Bsrc + b * b_bs;
+  const int c = col0 + (threadIdx.x & 63);
+  if (c >= k) return;
+  constexpr int kU = kFwdRows / 4;
+  T v[kU];
+#pragma unroll
+  for (int u = 0; u < kU; ++u) {
+    const int r = r0 + (threadIdx.x >> 6) + 4 * u;
+    v[u] = T(0);
     if (r < ns) {
       const int gr = brow ? brow[S.first + r] : S.first + r;
-      v = static_cast<T>(Bb[static_cast<int64_t>(gr) * k + c]);
-    }
-    f[static_cast<int64_t>(r) * k + c] = v;
-  }
-  for (int cc = S.c_begin; cc < S.c_end; ++cc) {
-    const int ch = cidx[cc];
-    if (active && !active[ch]) continue;
-    __syncthreads();
-    const SupDev C = sup[ch];
-    const T* u = C.fz_uoff >= 0 ? ufz + lb.b * ufz_bs + C.fz_uoff * k
-                                : uprev + lb.b * uprev_bs + C.upd_off * k;
-    const int* rp = relpos + C.rptr;
-    for (int idx = tid; idx < C.rs * 64; idx += 256) {
-      const int r = idx >> 6, c = col0 + (idx & 63);
-      if (c < k) f[static_cast<int64_t>(rp[r]) * k + c] += u[static_cast<int64_t>(r) * k + c];
+      v[u] = static_cast<T>(Bb[static_cast<int64_t>(gr) * k + c]);
     }
   }
+#pragma unroll
+  for (int u = 0; u < kU; ++u) {
+    const int r = r0 + (threadIdx.x >> 6) + 4 * u;
+    if (r < nf) fo[static_cast<int64_t>(r) * k + c] = v[u];
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_fwd_extend_big(
+    const SupDev* __restrict__ sup, const int* __restrict__ lvl, const int* __restrict__ relpos,
+    const int* __restrict__ cidx, const int* __restrict__ active, int k,
+    const T* __restrict__ uprev, int64_t uprev_bs, const T* __restrict__ ufz, int64_t ufz_bs,
+    T* __restrict__ fs, const int64_t* __restrict__ fs_off, int64_t fs_bs, int kchild,
+    int nbatch) {
+  const int f = blockIdx.y / nbatch, b = blockIdx.y % nbatch;
+  const int s = lvl[f];
+  const SupDev S = sup[s];
+  if (S.c_begin + kchild >= S.c_end) return;
+  const int ch = cidx[S.c_begin + kchild];
+  if (active && !active[ch]) return;
+  const SupDev C = sup[ch];
+  const int r0 = blockIdx.x * kFwdRows, col0 = blockIdx.z * 64;
+  if (r0 >= C.rs) return;
+  const T* u = C.fz_uoff >= 0 ? ufz + b * ufz_bs + C.fz_uoff * k
+                              : uprev + b * uprev_bs + C.upd_off * k;
+  const int* rp = relpos + C.rptr;
+  T* fo = fs + b * fs_bs + fs_off[s] * k;
+  const int c = col0 + (threadIdx.x & 63);
+  if (c >= k) return;
+  constexpr int kU = kFwdRows / 4;
+  T uv[kU], fv[kU];
+  int64_t fa[kU];
+#pragma unroll
+  for (int q = 0; q < kU; ++q) {
+    const int r = r0 + (threadIdx.x >> 6) + 4 * q;
+    fa[q] = -1;
+    if (r < C.rs) {
+      fa[q] = static_cast<int64_t>(rp[r]) * k + c;
+      uv[q] = u[static_cast<int64_t>(r) * k + c];
+      fv[q] = fo[fa[q]];
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < kU; ++q)
+    if (fa[q] >= 0) fo[fa[q]] = fv[q] + uv[q];
 }
 
 template <typename T, int MT>
@@ -1739,6 +1947,7 @@ struct dcx_plan {
   int64_t* d_fs_off = nullptr;  // row offset of each supernode within its level
   void* d_fs = nullptr;
   size_t fs_bytes = 0;
+  void* d_dg = nullptr;  // factored diagonal blocks of the big fronts' panels, kDg per front
   // second stream and level events of dcx_factor_solve
   hipStream_t side = nullptr;
   std::vector<hipEvent_t> lev_ev;
@@ -1754,7 +1963,7 @@ struct dcx_plan {
                     d_perm, d_iperm, d_pcrow, d_pcol, d_pvidx, d_crow, d_col, d_info, d_panels,
                     d_front[0], d_front[1], d_vals, d_active, d_act_lvl, d_upd[0],
                     d_upd[1], d_w, d_r, d_b, d_x, d_fz, d_flv, d_fmem, d_fgptr, d_fgb,
-                    d_toff, d_lridx, d_ulvl, d_ufz, d_panelsT, d_fs_off, d_fs};
+                    d_toff, d_lridx, d_ulvl, d_ufz, d_panelsT, d_fs_off, d_fs, d_dg};
     for (void* p : ptrs)
       if (p) (void)hipFree(p);
   }
@@ -1823,6 +2032,24 @@ int upload_fused(dcx_plan* P) {
 constexpr size_t kLdsBudget = 64 * 1024;
 constexpr size_t kFusedLdsBudget = 80 * 1024;  // two workgroups per CU
 
+// levels whose tallest panel does not fit k_factor_level's LDS or threads
+// take the grid-wide large-front kernels
+inline int level_max_ld(const dcx::Symbolic& S, int d) {
+  int maxld = 0;
+  for (int q = S.lvl_ptr[d]; q < S.lvl_ptr[d + 1]; ++q) {
+    const int s = S.lvl_sup[q];
+    maxld = std::max(maxld, 2 * S.ns[s] + S.rs[s]);
+  }
+  return maxld;
+}
+inline size_t factor_lds(int maxld, size_t tsize) {
+  return (kNB * kLDG + kNB + static_cast<size_t>(maxld + 16) * kLDP) * tsize;
+}
+inline bool level_is_big(const dcx::Symbolic& S, int d, size_t tsize) {
+  const int maxld = level_max_ld(S, d);
+  return factor_lds(maxld, tsize) > 160 * 1024 || maxld > 1024;
+}
+
 template <typename T, int NT, int RPT>
 void factor_launch(dcx_plan* P, int d, int cnt, size_t lds, hipStream_t st) {
   const auto& S = P->S;
@@ -1867,14 +2094,16 @@ int factor_level_big(dcx_plan* P, int d, int cnt, hipStream_t st) {
                        P->d_relpos, P->d_cidx, fr[d & 1], fbs[d & 1], fr[(d + 1) & 1],
                        fbs[(d + 1) & 1], c, nb);
   }
-  for (int j0 = 0; j0 < maxns; j0 += kNB) {
+  T* dg = static_cast<T*>(P->d_dg);
+  hipLaunchKernelGGL(k_diag_big<T>, dim3(cnt * nb), dim3(64), 0, st, P->d_sup, lvl, fr[d & 1],
+                     fbs[d & 1], dg, nb, P->d_info);
+  for (int j0 = 0; j0 < maxns; j0 += kNB2) {
     const int rows = maxnf + maxns - j0, cols = maxnf - j0;  // tallest m and widest n1
     hipLaunchKernelGGL(k_panel_rows_big<T>, dim3((rows + kRowsBig - 1) / kRowsBig, cnt * nb),
-                       dim3(kRowsBig), 0, st, P->d_sup, lvl, fr[d & 1], fbs[d & 1], j0, nb,
-                       P->d_info);
+                       dim3(kRowsBig), 0, st, P->d_sup, lvl, fr[d & 1], fbs[d & 1], dg, j0, nb);
     dim3 tg((rows + 63) / 64, (cols + 63) / 64, cnt * nb);
     hipLaunchKernelGGL(k_trail_big<T>, tg, dim3(256), 0, st, P->d_sup, lvl, fr[d & 1],
-                       fbs[d & 1], j0, nb);
+                       fbs[d & 1], j0, nb, dg, P->d_info);
   }
   hipLaunchKernelGGL(k_front_out_big<T>, dim3((maxnf + 63) / 64, (maxns + 63) / 64, cnt * nb),
                      dim3(256), 0, st, P->d_sup, lvl, fr[d & 1], fbs[d & 1],
@@ -1890,13 +2119,9 @@ int factor_level(dcx_plan* P, int d, hipStream_t st) {
   const auto& S = P->S;
   const int cnt = S.lvl_ptr[d + 1] - S.lvl_ptr[d];
   if (!cnt) return DCX_OK;
-  int maxld = 0;
-  for (int q = S.lvl_ptr[d]; q < S.lvl_ptr[d + 1]; ++q) {
-    const int s = S.lvl_sup[q];
-    maxld = std::max(maxld, 2 * S.ns[s] + S.rs[s]);
-  }
-  const size_t lds = (kNB * kLDG + kNB + static_cast<size_t>(maxld + 16) * kLDP) * sizeof(T);
-  if (lds > 160 * 1024 || maxld > 1024) return factor_level_big<T>(P, d, cnt, st);
+  const int maxld = level_max_ld(S, d);
+  const size_t lds = factor_lds(maxld, sizeof(T));
+  if (level_is_big(S, d, sizeof(T))) return factor_level_big<T>(P, d, cnt, st);
   if (maxld <= 256)
     factor_launch<T, 256, 1>(P, d, cnt, lds, st);
   else
@@ -1996,11 +2221,11 @@ void bwd_wave_launch(dcx_plan* P, T* W, int k, const int* lvl, int cnt, const in
                      P->S.panel_size, W, w_bs, k, io.X64, io.xrow, io.xmode, cnt, nzb, total,
                      nzc, xs, P->d_fs_off, xs_bs);
   if (xs) {
-    const int nzg = (k + 63) / 64;
-    const int tg = cnt * P->nbatch * nzg;
-    hipLaunchKernelGGL(k_bwd_copy_big<T>, dim3(((tg + 7) / 8) * 8), dim3(256), 0, st, P->d_sup,
-                       lvl, cnt, xs, P->d_fs_off, xs_bs, W, w_bs, k, io.X64, io.xrow, io.xmode,
-                       nzg, tg);
+    const int maxns = nrg * 16 * MT;  // bounds ns over the level
+    hipLaunchKernelGGL(k_bwd_copy_big<T>,
+                       dim3((maxns + kCopyRows - 1) / kCopyRows, cnt * P->nbatch, (k + 63) / 64),
+                       dim3(256), 0, st, P->d_sup, lvl, xs, P->d_fs_off, xs_bs, W, w_bs,
+                       k, io.X64, io.xrow, io.xmode, P->nbatch);
   }
 }
 
@@ -2144,12 +2369,29 @@ int fwd_big(dcx_plan* P, T* W, int k, int d, const int* lvl, int cnt, int maxnf,
   T* fs = static_cast<T*>(P->d_fs);
   T* u[2] = {static_cast<T*>(P->d_upd[0]), static_cast<T*>(P->d_upd[1])};
   const int64_t ubs[2] = {S.upd_rows[0] * k, S.upd_rows[1] * k};
-  const int nzg = (k + 63) / 64;
-  const int tg = cnt * P->nbatch * nzg;
-  hipLaunchKernelGGL(k_fwd_gather_big<T>, dim3(((tg + 7) / 8) * 8), dim3(256), 0, st, P->d_sup,
-                     lvl, cnt, P->d_relpos, P->d_cidx, act, io.B, io.b_bs, io.brow, k,
-                     u[(d + 1) & 1], ubs[(d + 1) & 1], static_cast<const T*>(P->d_ufz),
-                     P->S.f_upd_rows * k, fs, P->d_fs_off, fs_bs, nzg, tg);
+  const int nzg = (k + 63) / 64, nb = P->nbatch;
+  if (static_cast<int64_t>(cnt) * nb > 65535 || nzg > 65535)
+    return fail(DCX_ERR_ARG, "too many large fronts or right-hand sides in one level");
+  int maxkids = 0;
+  std::vector<int> maxrc;
+  for (int q = S.ulvl_ptr[d]; q < S.ulvl_ptr[d + 1]; ++q) {  // bounds the active subset too
+    const int s = S.ulvl_sup[q];
+    const int nk = S.cptr[s + 1] - S.cptr[s];
+    if (nk > maxkids) maxrc.resize(nk, 0);
+    maxkids = std::max(maxkids, nk);
+    for (int c = 0; c < nk; ++c) maxrc[c] = std::max(maxrc[c], S.rs[S.cidx[S.cptr[s] + c]]);
+  }
+  hipLaunchKernelGGL(k_fwd_init_big<T>, dim3((maxnf + kFwdRows - 1) / kFwdRows, cnt * nb, nzg),
+                     dim3(256), 0, st, P->d_sup, lvl, io.B, io.b_bs, io.brow, k, fs, P->d_fs_off,
+                     fs_bs, nb);
+  for (int c = 0; c < maxkids; ++c) {
+    if (!maxrc[c]) continue;
+    hipLaunchKernelGGL(k_fwd_extend_big<T>,
+                       dim3((maxrc[c] + kFwdRows - 1) / kFwdRows, cnt * nb, nzg), dim3(256), 0,
+                       st, P->d_sup, lvl, P->d_relpos, P->d_cidx, act, k, u[(d + 1) & 1],
+                       ubs[(d + 1) & 1], static_cast<const T*>(P->d_ufz), P->S.f_upd_rows * k,
+                       fs, P->d_fs_off, fs_bs, c, nb);
+  }
   constexpr int MT = 8;
   const int nzc = ((k + 15) / 16 + 3) / 4;
   const int nrg = (maxnf + 16 * MT - 1) / (16 * MT);
@@ -2371,7 +2613,12 @@ int dcx_plan_create(int n, const int32_t* crow, const int32_t* col, int nbatch,
     HIP_TRY(hipMalloc(&P->d_front[0], std::max<size_t>(f0, 256)));
     HIP_TRY(hipMalloc(&P->d_front[1], std::max<size_t>(f1, 256)));
     HIP_TRY(hipMalloc(&P->d_vals, std::max<size_t>(vb, 256)));
-    P->device_bytes = static_cast<int64_t>(2 * pb + f0 + f1 + vb);
+    int maxbig = 0;
+    for (int d = 0; d < S.ndepth; ++d)
+      if (level_is_big(S, d, P->tsize)) maxbig = std::max(maxbig, S.lvl_ptr[d + 1] - S.lvl_ptr[d]);
+    const size_t dgb = static_cast<size_t>(nbatch) * maxbig * kDg * P->tsize;
+    HIP_TRY(hipMalloc(&P->d_dg, std::max<size_t>(dgb, 256)));
+    P->device_bytes = static_cast<int64_t>(2 * pb + f0 + f1 + vb + dgb);
     *out = P.release();
     return DCX_OK;
   } catch (const std::exception& e) {

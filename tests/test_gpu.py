@@ -313,16 +313,20 @@ def test_engine3d_against_oracle(precision, tol_d, tol_g):
     assert rel(g.cpu().numpy(), g_o) < tol_g
 
 
-@pytest.mark.parametrize("precision,tol_d,tol_g", [("fp64", 1e-10, 1e-9), ("mixed", 1e-6, 1e-3)])
-def test_engine3d_large_fronts(precision, tol_d, tol_g):
+# mixed without refinement is the bare FP32 factor and sweeps (relative
+# error a few 1e-6 on this grid); one FP64 refinement step brings dpred to 1e-6
+@pytest.mark.parametrize("precision,refine,tol_d,tol_g", [("fp64", 0, 1e-10, 1e-9),
+                                                          ("mixed", 0, 5e-6, 1e-3),
+                                                          ("mixed", 1, 1e-6, 1e-3)])
+def test_engine3d_large_fronts(precision, refine, tol_d, tol_g):
     """3-D fronts too tall for the in-LDS kernels (largest front 645 rows,
-    870 with the inverse rows): the per-panel factor launches, the row-group
+    870 with the inverse rows): the grid-wide factor launches, the row-group
     backward sweep and the two-step forward sweep, against the oracle."""
     from oracle.dc3d import Problem3D
     from simpeg_dc_torch_amd import TorchDC3D
-    _say(f"engine 3-D medium {precision}")
+    _say(f"engine 3-D medium {precision} refine {refine}")
     mesh, survey, m = case("medium_3d")
-    eng = TorchDC3D(mesh, survey, device=DEV, precision=precision, refine=0)
+    eng = TorchDC3D(mesh, survey, device=DEV, precision=precision, refine=refine)
     O = Problem3D(mesh.h[0], mesh.h[1], mesh.h[2], mesh.origin, survey)
     mt = torch.tensor(m, device=DEV)
     d_o = O.dpred(m)
