@@ -15,7 +15,9 @@ gather; `--refine N` adds N FP64-residual refinement passes per solve).  The
 FP64 time, the errors of the mixed result against the FP64 one, the roofline
 of the dominant kernel (the backward sweep) from HIP events on the solve
 stream, and the CPU baseline (the oracle's SuperLU restatement of the
-reference CPU path on a bounded sample) are reported beside it.
+reference CPU path on a bounded sample) are reported beside it, with the
+Gauss-Newton building blocks (`linearization`) and the 3-D engine on a
+56 x 56 x 38-cell grid (`grid3d`).
 """
 import argparse
 import json
@@ -151,6 +153,48 @@ def linearization_times(eng, mt, reps=3):
             "precision": "fp64"}
 
 
+def grid3d_times(steps, warmup):
+    """The 3-D nodal engine (TorchDC3D) on a 56 x 56 x 38-cell grid with 200
+    pole sources and 6,000 pole-dipole data: FP64 and mixed eval+grad ms and
+    the mixed result's error against FP64 (one system, fronts up to 3,315
+    rows: the grid-wide large-front kernels).  Data uncertainty 5% of |dobs|
+    plus a floor of 1e-4 (0.1% of the median datum; the data span 2.5e-5 to
+    18), the rel * |d| + floor form of the reference's own scripts: without a
+    floor the smallest data get weights near 1e6 and the misfit gradient
+    measures FP32's absolute dpred error on them rather than the solver.  The
+    J^T v error (random v) isolates the adjoint path."""
+    from simpeg_dc_torch_amd import TorchDC3D, workloads
+    mesh, survey, m = workloads.grid_3d_case()
+    mt = torch.tensor(m, device="cuda:0")
+    out, ref = {}, None
+    for prec, refine in (("fp64", 0), ("mixed", 0), ("mixed", 1)):
+        eng = TorchDC3D(mesh, survey, device="cuda:0", precision=prec, refine=refine)
+        if ref is None:
+            with torch.no_grad():
+                d = eng.dpred(mt)
+            rng = np.random.default_rng(3)
+            dobs = d * (1 + 0.05 * torch.tensor(rng.standard_normal(eng.nD), device="cuda:0"))
+            w = 1.0 / (0.05 * dobs.abs() + 1e-4)
+            v = torch.tensor(rng.standard_normal(eng.nD), device="cuda:0")
+            info = eng.solver.info()
+            out.update(cells=eng.nC, nodes=eng.nN, sources=eng.nS, data=eng.nD,
+                       max_front=info["max_front"], nnz_L=info["nnz_L"])
+        ms = time_steps(eng, mt, dobs, w, steps, warmup)
+        with torch.no_grad():
+            d = eng.dpred(mt)
+        _, g = eval_grad(eng, mt, dobs, w)
+        jt = eng.Jtvec(mt, v)
+        key = prec if prec == "fp64" else f"mixed_refine{refine}"
+        out[f"{key}_ms"] = round(ms, 2)
+        if ref is None:
+            ref = (d, g, jt)
+        else:
+            for nm, a, b in (("dpred", d, ref[0]), ("grad", g, ref[1]), ("jtvec", jt, ref[2])):
+                out[f"{key}_{nm}_rel_err"] = float(torch.linalg.norm(a - b) / torch.linalg.norm(b))
+        del eng
+    return out
+
+
 def cpu_baseline(mesh, survey, m, kys, wq, sample_ky=1):
     """The oracle's SuperLU path (reference CPU solver semantics) on a sample
     of `sample_ky` wavenumbers, scaled to all of them."""
@@ -196,6 +240,9 @@ def main():
     ap.add_argument("--refine", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--pmc-pass", action="store_true")
+    ap.add_argument("--workload", choices=("all", "line25k", "grid3d"), default="all",
+                    help="all (default): the 2.5-D line with the 3-D grid beside it; "
+                         "line25k or grid3d alone (profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported "
@@ -203,6 +250,11 @@ def main():
         sys.exit(2)
     if args.pmc_pass:
         pmc_pass(args)
+        return
+    if args.workload == "grid3d":
+        log("3-D grid only (TorchDC3D, 56 x 56 x 38 cells, 200 sources)")
+        print(json.dumps({"workload": "grid3d", **grid3d_times(args.steps, args.warmup)}),
+              flush=True)
         return
     pr = torch.cuda.get_device_properties(0)
     log(f"device: {pr.name} CUs={pr.multi_processor_count} "
@@ -233,6 +285,11 @@ def main():
     log("linearisation (FP64): J v, J^T v, J, diag(J^T D J)")
     lin = linearization_times(e64, mt)
     log(f"linearisation {lin}")
+    g3 = None
+    if args.workload == "all":
+        log("3-D grid (TorchDC3D, 56 x 56 x 38 cells, 200 sources)")
+        g3 = grid3d_times(args.steps, args.warmup)
+        log(f"3-D {g3}")
     cpu = None
     if not args.no_cpu:
         log("cpu baseline (oracle SuperLU, 1 wavenumber sample)")
@@ -257,6 +314,7 @@ def main():
         "build_s": round(t_build, 2),
         "roofline": roof, "roofline_fp64": roof64,
         "linearization": lin,
+        "grid3d": g3,
         "cpu_baseline": cpu,
     }
     if cpu:

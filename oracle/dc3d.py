@@ -28,13 +28,14 @@ What it restates
 
 Pinning
 -------
-* operators: the closed-form 7-point stencil on a uniform mesh (6 sigma h /
-  -sigma h for unit cells, 3 sigma h on a free-surface... checked in tests),
-  symmetry, zero row sums of the Grad term, linearity in sigma;
-* physics: the analytic half-space pole-dipole potential
-  V = rho/(2 pi) (1/AM - 1/AN) with the error falling as the mesh refines;
-* gradient / Jtvec / Jmatrix: central finite differences and the adjoint
-  identity.
+* operators: the closed-form 7-point stencil on a uniform mesh of unit
+  cells with sigma = 1 (interior node 6 with -1 to each neighbour; a
+  free-surface node 3, -1/2 along the surface, -1 downwards), symmetry, zero
+  row sums of the Grad term;
+* physics: the analytic half-space pole potential rho / (2 pi r), the error
+  falling as the cells halve;
+* gradient and Jtvec: central finite differences of the misfit and of
+  dpred (tests/test_host.py).
 * The Robin coefficient and the source point are recalled from SimPEG 0.25
   (not importable here): parity with stock SimPEG is unpinned; the
   half-space check bounds their effect.

@@ -302,3 +302,30 @@ def test_3d_cpu_engine_vs_oracle():
     assert rel(g.numpy(), g_o) < 1e-9
     v = np.random.default_rng(1).standard_normal(eng.nD)
     assert rel(eng.Jtvec(mt, torch.tensor(v)).numpy(), O.Jtvec(m, v)) < 1e-9
+
+
+def test_3d_oracle_gradient_and_jtvec_vs_fd():
+    """The oracle's adjoint gradient and J^T v against central differences of
+    its own misfit and dpred (independent of the adjoint formula)."""
+    from simpeg_dc_torch_amd.mesh import TensorMesh
+    from simpeg_dc_torch_amd import workloads
+    mesh = TensorMesh([[(1.0, 6)], [(1.0, 5)], [(1.0, 4)]], origin="000")
+    el = workloads.surface_grid(3, 2, 1.5, x0=1.2, y0=1.3, z=4.0)
+    survey = workloads.pole_dipole_survey(el[:2], el, np.array([[2, 3], [3, 4], [4, 5]]), 2)
+    O = _oracle3d(mesh, survey)
+    rng = np.random.default_rng(5)
+    m = np.log(50.0) + 0.3 * rng.standard_normal(mesh.nC)
+    d0 = O.dpred(m)
+    dobs, w = noisy_data(d0)
+    _, g, _ = O.misfit_and_grad(m, dobs, w)
+    v = rng.standard_normal(d0.size)
+    jtv = O.Jtvec(m, v)
+    eps = 1e-5
+    for c in rng.choice(mesh.nC, 8, replace=False):
+        e = np.zeros(mesh.nC)
+        e[c] = eps
+        phi = [float(np.sum((w * (O.dpred(m + s * e) - dobs)) ** 2)) for s in (1, -1)]
+        fd = (phi[0] - phi[1]) / (2 * eps)
+        assert abs(fd - g[c]) <= 1e-6 * max(abs(fd), 1e-3 * np.abs(g).max())
+        fdj = float(v @ (O.dpred(m + e) - O.dpred(m - e))) / (2 * eps)
+        assert abs(fdj - jtv[c]) <= 1e-6 * max(abs(fdj), 1e-3 * np.abs(jtv).max())
