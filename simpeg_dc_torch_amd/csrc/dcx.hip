@@ -1084,10 +1084,15 @@ __global__ __launch_bounds__(kThreads) void k_bwd_wave(
           for (int v = 0; v < V; ++v) bv[j][v] = -bv[j][v];
         }
       }
+      // L_JJ^{-1} (the J rows of P_s) is lower triangular: k-steps wholly
+      // above a tile's first row multiply zeros and are skipped (not in the
+      // split-K variants, where the branches cost more registers than the
+      // skipped work saves)
 #pragma unroll
       for (int t = 0; t < MT; ++t) {
         const int r = r0 + t * 16 + (lane & 15);
-        av[j][t] = (kok && r < ns) ? PT[r + static_cast<int64_t>(kk) * ns] : T(0);
+        const bool live = KS > 1 || k0 + 4 * j + 4 > r0 + 16 * t;
+        av[j][t] = (kok && live && r < ns) ? PT[r + static_cast<int64_t>(kk) * ns] : T(0);
       }
     }
 #pragma unroll
@@ -1095,8 +1100,10 @@ __global__ __launch_bounds__(kThreads) void k_bwd_wave(
       if (k0 + 4 * j < kend) {
 #pragma unroll
         for (int t = 0; t < MT; ++t)
+          if (KS > 1 || k0 + 4 * j + 4 > r0 + 16 * t) {
 #pragma unroll
-          for (int v = 0; v < V; ++v) acc[t][v] = Mma<T>::op(av[j][t], bv[j][v], acc[t][v]);
+            for (int v = 0; v < V; ++v) acc[t][v] = Mma<T>::op(av[j][t], bv[j][v], acc[t][v]);
+          }
       }
   }
   if (KS > 1) {
@@ -2122,8 +2129,15 @@ int factor_level(dcx_plan* P, int d, hipStream_t st) {
   const int maxld = level_max_ld(S, d);
   const size_t lds = factor_lds(maxld, sizeof(T));
   if (level_is_big(S, d, sizeof(T))) return factor_level_big<T>(P, d, cnt, st);
-  if (maxld <= 256)
+  // levels with few workgroups (the top of the tree) take more wavefronts per
+  // front for the trailing update
+  static const int wide_below = env_int("DCX_FACTOR_WIDE_BELOW", 321);
+  static const int wide_nt = env_int("DCX_FACTOR_WIDE_NT", 1024);
+  const bool wide = static_cast<int64_t>(cnt) * P->nbatch < wide_below;
+  if (maxld <= 256 && !wide)
     factor_launch<T, 256, 1>(P, d, cnt, lds, st);
+  else if (maxld <= 512 && wide && wide_nt == 512)
+    factor_launch<T, 512, 1>(P, d, cnt, lds, st);
   else
     factor_launch<T, 1024, 1>(P, d, cnt, lds, st);
   return DCX_OK;
