@@ -325,7 +325,10 @@ __global__ __launch_bounds__(NT) void k_factor_level(
                     reinterpret_cast<int*>(pan));
   for (int j0 = 0; j0 < ns; j0 += kNB) {
     const int jb = min(kNB, ns - j0);
-    const int m = ld - j0;     // panel rows: J, R and E rows from j0 on
+    // panel rows: J, R and E rows from j0 on.  E row e (front row nf + e)
+    // has no entry left of column e until column e is eliminated, so the E
+    // rows e >= j0 + jb are zero in this panel and take no part in it
+    const int m = min(ld - j0, nf + jb);
     const int n1 = nf - j0;    // end of the J/R rows (panel-local)
     T rv[RPT][kNB];
 #pragma unroll
@@ -636,7 +639,9 @@ __global__ __launch_bounds__(kRowsBig) void k_panel_rows_big(const SupDev* __res
   const SupDev S = sup[lvl[f]];
   const int ns = S.ns, nf = S.ns + S.rs, ld = nf + S.ns;
   if (j0 >= ns) return;  // this front has no panel j0 (uniform)
-  const int jb = min(kNB2, ns - j0), jb1 = min(kNB, jb), jb2 = jb - jb1, m = ld - j0;
+  // E rows e >= j0 + jb are zero in this panel's columns (see k_factor_level)
+  const int jb = min(kNB2, ns - j0), jb1 = min(kNB, jb), jb2 = jb - jb1;
+  const int m = min(ld - j0, nf + jb);
   const int rbase = jb + blockIdx.x * kRowsBig;
   if (rbase >= m) return;
   const int tid = threadIdx.x;
@@ -694,7 +699,8 @@ __global__ __launch_bounds__(256) void k_trail_big(const SupDev* __restrict__ su
   const SupDev S = sup[lvl[f]];
   const int ns = S.ns, nf = S.ns + S.rs, ld = nf + S.ns;
   if (j0 >= ns) return;
-  const int jb = min(kNB2, ns - j0), m = ld - j0, n1 = nf - j0;
+  // E rows e >= j0 + jb are zero in this panel (see k_factor_level)
+  const int jb = min(kNB2, ns - j0), m = min(ld - j0, nf + jb), n1 = nf - j0;
   const int rb = jb + blockIdx.x * 64, cb = jb + blockIdx.y * 64;
   if (rb >= m || cb >= n1 || (rb + 63 < n1 && rb + 63 < cb)) return;  // empty or upper
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -910,7 +916,10 @@ __global__ __launch_bounds__(kThreads) void k_fwd_level(
     }
     for (int t = wave; t < nt; t += 4) {
       const int tr = (t / NCT) * 16, tc = (t % NCT) * 16;
-      typename Mma<T>::v4 a = mma_tile<T, TC>(Pa, nf, ns, f, RowId(), tr, tc, lane);
+      // the J rows of P_s hold L_JJ^{-1}, lower triangular: row tile tr needs
+      // columns up to its last row only
+      const int kt = tr < ns ? min(ns, tr + 16) : ns;
+      typename Mma<T>::v4 a = mma_tile<T, TC>(Pa, nf, kt, f, RowId(), tr, tc, lane);
       const int c = tc + (lane & 15);
       if (col0 + c >= k) continue;
 #pragma unroll
@@ -1334,12 +1343,15 @@ __global__ __launch_bounds__(256) void k_fwd_wave_big(
   typename Mma<T>::v4 acc[MT];
 #pragma unroll
   for (int t = 0; t < MT; ++t) acc[t] = {T(0), T(0), T(0), T(0)};
-  for (int k0 = 0; k0 < ns; k0 += 4 * PF) {
+  // the J rows of P_s hold L_JJ^{-1}, lower triangular: rows below ns need
+  // columns up to their own index only
+  const int kend = r0 + 16 * MT <= ns ? r0 + 16 * MT : ns;
+  for (int k0 = 0; k0 < kend; k0 += 4 * PF) {
     T bv[PF], av[PF][MT];
 #pragma unroll
     for (int j = 0; j < PF; ++j) {
       const int kk = k0 + 4 * j + kq;
-      const bool kok = kk < ns;
+      const bool kok = kk < kend;
       bv[j] = (kok && cok) ? f[static_cast<int64_t>(kk) * k + c] : T(0);
 #pragma unroll
       for (int t = 0; t < MT; ++t) {
@@ -1349,7 +1361,7 @@ __global__ __launch_bounds__(256) void k_fwd_wave_big(
     }
 #pragma unroll
     for (int j = 0; j < PF; ++j)
-      if (k0 + 4 * j < ns) {
+      if (k0 + 4 * j < kend) {
 #pragma unroll
         for (int t = 0; t < MT; ++t) acc[t] = Mma<T>::op(av[j][t], bv[j], acc[t]);
       }
@@ -2105,7 +2117,8 @@ int factor_level_big(dcx_plan* P, int d, int cnt, hipStream_t st) {
   hipLaunchKernelGGL(k_diag_big<T>, dim3(cnt * nb), dim3(64), 0, st, P->d_sup, lvl, fr[d & 1],
                      fbs[d & 1], dg, nb, P->d_info);
   for (int j0 = 0; j0 < maxns; j0 += kNB2) {
-    const int rows = maxnf + maxns - j0, cols = maxnf - j0;  // tallest m and widest n1
+    // tallest live row range (J, R and the E rows e < j0 + kNB2) and widest n1
+    const int rows = std::min(maxnf + maxns - j0, maxnf + kNB2), cols = maxnf - j0;
     hipLaunchKernelGGL(k_panel_rows_big<T>, dim3((rows + kRowsBig - 1) / kRowsBig, cnt * nb),
                        dim3(kRowsBig), 0, st, P->d_sup, lvl, fr[d & 1], fbs[d & 1], dg, j0, nb);
     dim3 tg((rows + 63) / 64, (cols + 63) / 64, cnt * nb);
