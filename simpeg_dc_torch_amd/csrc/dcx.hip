@@ -2064,9 +2064,13 @@ inline int level_max_ld(const dcx::Symbolic& S, int d) {
 inline size_t factor_lds(int maxld, size_t tsize) {
   return (kNB * kLDG + kNB + static_cast<size_t>(maxld + 16) * kLDP) * tsize;
 }
-inline bool level_is_big(const dcx::Symbolic& S, int d, size_t tsize) {
+// also levels with fewer than DCX_FACTOR_GRID_BELOW (front x batch)
+// workgroups, where one workgroup per front leaves the chip idle
+inline bool level_is_big(const dcx::Symbolic& S, int d, size_t tsize, int nbatch) {
+  static const int grid_below = env_int("DCX_FACTOR_GRID_BELOW", 0);
   const int maxld = level_max_ld(S, d);
-  return factor_lds(maxld, tsize) > 160 * 1024 || maxld > 1024;
+  const int64_t wgs = static_cast<int64_t>(S.lvl_ptr[d + 1] - S.lvl_ptr[d]) * nbatch;
+  return factor_lds(maxld, tsize) > 160 * 1024 || maxld > 1024 || wgs < grid_below;
 }
 
 template <typename T, int NT, int RPT>
@@ -2141,7 +2145,7 @@ int factor_level(dcx_plan* P, int d, hipStream_t st) {
   if (!cnt) return DCX_OK;
   const int maxld = level_max_ld(S, d);
   const size_t lds = factor_lds(maxld, sizeof(T));
-  if (level_is_big(S, d, sizeof(T))) return factor_level_big<T>(P, d, cnt, st);
+  if (level_is_big(S, d, sizeof(T), P->nbatch)) return factor_level_big<T>(P, d, cnt, st);
   // levels with few workgroups (the top of the tree) take more wavefronts per
   // front for the trailing update
   static const int wide_below = env_int("DCX_FACTOR_WIDE_BELOW", 321);
@@ -2642,7 +2646,7 @@ int dcx_plan_create(int n, const int32_t* crow, const int32_t* col, int nbatch,
     HIP_TRY(hipMalloc(&P->d_vals, std::max<size_t>(vb, 256)));
     int maxbig = 0;
     for (int d = 0; d < S.ndepth; ++d)
-      if (level_is_big(S, d, P->tsize)) maxbig = std::max(maxbig, S.lvl_ptr[d + 1] - S.lvl_ptr[d]);
+      if (level_is_big(S, d, P->tsize, nbatch)) maxbig = std::max(maxbig, S.lvl_ptr[d + 1] - S.lvl_ptr[d]);
     const size_t dgb = static_cast<size_t>(nbatch) * maxbig * kDg * P->tsize;
     HIP_TRY(hipMalloc(&P->d_dg, std::max<size_t>(dgb, 256)));
     P->device_bytes = static_cast<int64_t>(2 * pb + f0 + f1 + vb + dgb);
