@@ -17,3 +17,5 @@ from .linearization2d import Linearization2D  # noqa: F401
 from .simulation3d import TorchDC3D  # noqa: F401
 from .mesh import TensorMesh  # noqa: F401
 from . import survey  # noqa: F401
+from . import gauss_newton  # noqa: F401
+from .lbfgs import lbfgs_minimize  # noqa: F401

@@ -373,3 +373,57 @@ def test_line25k_subset_against_oracle_and_fd():
     with torch.no_grad():
         dm = engm.dpred(mt).cpu().numpy()
     assert rel(dm, d) < 1e-6
+
+
+def test_grid3d_gradient_vs_fd():
+    """The 56 x 56 x 38-cell 3-D workload (126,711 nodes, 200 pole sources,
+    6,000 data, fronts up to 3,315 rows), FP64: the misfit gradient against
+    central differences of the GPU misfit itself on the cells with the
+    largest gradient and a few random ones."""
+    from simpeg_dc_torch_amd import TorchDC3D, workloads
+    _say("grid3d fp64 gradient vs central differences")
+    mesh, survey, m = workloads.grid_3d_case()
+    eng = TorchDC3D(mesh, survey, device=DEV)
+    mt = torch.tensor(m, device=DEV)
+    with torch.no_grad():
+        d = eng.dpred(mt)
+    rng = np.random.default_rng(3)
+    dobs = d * (1 + 0.05 * torch.tensor(rng.standard_normal(eng.nD), device=DEV))
+    w = 1.0 / (0.05 * dobs.abs() + 1e-4)
+    _, g = eng.misfit_and_grad(mt, dobs, w)
+    g = g.cpu().numpy()
+    cells = np.r_[np.argsort(-np.abs(g))[:5], rng.choice(m.size, 3, replace=False)]
+    def central(c, eps):
+        e = torch.zeros_like(mt)
+        e[c] = eps
+        with torch.no_grad():
+            fp = float(eng.misfit(mt + e, dobs, w))
+            fm = float(eng.misfit(mt - e, dobs, w))
+        return (fp - fm) / (2 * eps)
+
+    # Richardson extrapolation of two central differences (truncation error
+    # O(eps^4)): cells next to an electrode make the misfit strongly
+    # nonlinear in their log-resistivity
+    eps = 2e-3
+    for c in cells:
+        f1, f2 = central(c, eps), central(c, eps / 2)
+        fd = (4 * f2 - f1) / 3
+        assert abs(fd - g[c]) <= 1e-5 * max(abs(fd), 1e-3 * np.abs(g).max()), (c, f1, f2, fd, g[c])
+    del eng
+    torch.cuda.empty_cache()
+
+
+def test_lbfgs_3d_gpu_decreases_objective():
+    from simpeg_dc_torch_amd import TorchDC3D, lbfgs_minimize
+    from simpeg_dc_torch_amd.gauss_newton import tensor_regularization
+    _say("lbfgs 3-D small")
+    mesh, survey, m = case("small_3d")
+    eng = TorchDC3D(mesh, survey, device=DEV)
+    with torch.no_grad():
+        d = eng.dpred(torch.tensor(m, device=DEV)).cpu().numpy()
+    dobs, w = noisy_data(d, seed=7)
+    m0 = np.full(m.size, float(np.mean(m)))
+    Hm = tensor_regularization(mesh, alpha_s=1e-3)
+    m1, hist = lbfgs_minimize(eng, m0, dobs, w, Hm=Hm, beta=0.1, max_iter=6)
+    assert hist[-1][0] + 0.1 * hist[-1][1] < hist[0][0] + 0.1 * hist[0][1]
+    assert m1.device.type == "cuda"
