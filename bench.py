@@ -16,8 +16,9 @@ FP64 time, the errors of the mixed result against the FP64 one, the roofline
 of the dominant kernel (the backward sweep) from HIP events on the solve
 stream, and the CPU baseline (the oracle's SuperLU restatement of the
 reference CPU path on a bounded sample) are reported beside it, with the
-Gauss-Newton building blocks (`linearization`) and the 3-D engine on a
-56 x 56 x 38-cell grid (`grid3d`).
+Gauss-Newton building blocks (`linearization`), the 3-D engine on a
+56 x 56 x 38-cell grid (`grid3d`) and an uneven batch of 16 independent lines
+(`uneven_lines`).
 """
 import argparse
 import json
@@ -195,6 +196,57 @@ def grid3d_times(steps, warmup):
     return out
 
 
+def lines_times(steps, warmup, autotune=True):
+    """The uneven batch: 16 independent 2.5-D lines of 3k-40k cells
+    (log-uniform, seed 5; 21-96 electrodes, dipole-dipole, 12 wavenumbers
+    requested), mixed precision.  One step = eval+grad of every line.  Timed
+    twice: the lines one after another on one stream, and concurrently on 4
+    default-priority streams with the leaf size autotuned per size class
+    (tuning outside the timed region)."""
+    from simpeg_dc_torch_amd import workloads
+    from simpeg_dc_torch_amd.lines import LineBatch
+    L = workloads.uneven_lines_case()
+    lines = [(mesh, survey) for mesh, survey, _ in L]
+    ms = [torch.tensor(m, device="cuda:0") for _, _, m in L]
+    probe = LineBatch(lines, precision="fp64", streams=1)
+    data = []
+    for eng, m in zip(probe.engines, ms):
+        with torch.no_grad():
+            d = eng.dpred(m)
+        rng = np.random.default_rng(7)
+        dobs = d * (1 + 0.05 * torch.tensor(rng.standard_normal(d.numel()), device="cuda:0"))
+        data.append((dobs, 1.0 / (0.05 * dobs.abs())))
+    del probe
+    tune = [(m.cpu().numpy(), d.cpu().numpy(), w.cpu().numpy()) for m, (d, w) in zip(ms, data)]
+    t0 = time.perf_counter()
+    lb = LineBatch(lines, precision="mixed", streams=4, leaf="auto" if autotune else None,
+                   tune_data=tune if autotune else None)
+    t_tune = time.perf_counter() - t0
+    lb.set_data([d for d, _ in data], [w for _, w in data])
+
+    def timed(nstreams):
+        saved = lb.streams
+        lb.streams = saved[:nstreams]
+        for _ in range(warmup):
+            lb.eval_grad(ms)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(steps):
+            lb.eval_grad(ms)
+        torch.cuda.synchronize()
+        lb.streams = saved
+        return 1e3 * (time.perf_counter() - t1) / steps
+
+    ms_seq, ms_par = timed(1), timed(4)
+    cells = sum(mesh.nC for mesh, _ in lines)
+    return {"lines": len(lines), "cells_total": cells,
+            "cells_range": [min(mesh.nC for mesh, _ in lines), max(mesh.nC for mesh, _ in lines)],
+            "precision": "mixed", "sequential_ms": round(ms_seq, 2),
+            "streams4_ms": round(ms_par, 2), "lines_per_s": round(1e3 * len(lines) / ms_par, 1),
+            "leaf_by_size_class": {str(k): v for k, v in (lb.leaf or {}).items()},
+            "build_and_tune_s": round(t_tune, 1), "graph_capture": "off"}
+
+
 def cpu_baseline(mesh, survey, m, kys, wq, sample_ky=1):
     """The oracle's SuperLU path (reference CPU solver semantics) on a sample
     of `sample_ky` wavenumbers, scaled to all of them."""
@@ -240,9 +292,9 @@ def main():
     ap.add_argument("--refine", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--pmc-pass", action="store_true")
-    ap.add_argument("--workload", choices=("all", "line25k", "grid3d"), default="all",
-                    help="all (default): the 2.5-D line with the 3-D grid beside it; "
-                         "line25k or grid3d alone (profiling runs)")
+    ap.add_argument("--workload", choices=("all", "line25k", "grid3d", "lines"), default="all",
+                    help="all (default): the 2.5-D line with the 3-D grid and the uneven "
+                         "batch of lines beside it; line25k, grid3d or lines alone")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported "
@@ -254,6 +306,11 @@ def main():
     if args.workload == "grid3d":
         log("3-D grid only (TorchDC3D, 56 x 56 x 38 cells, 200 sources)")
         print(json.dumps({"workload": "grid3d", **grid3d_times(args.steps, args.warmup)}),
+              flush=True)
+        return
+    if args.workload == "lines":
+        log("uneven batch only (16 lines, 3k-40k cells)")
+        print(json.dumps({"workload": "lines", **lines_times(args.steps, args.warmup)}),
               flush=True)
         return
     pr = torch.cuda.get_device_properties(0)
@@ -285,11 +342,14 @@ def main():
     log("linearisation (FP64): J v, J^T v, J, diag(J^T D J)")
     lin = linearization_times(e64, mt)
     log(f"linearisation {lin}")
-    g3 = None
+    g3 = lb = None
     if args.workload == "all":
         log("3-D grid (TorchDC3D, 56 x 56 x 38 cells, 200 sources)")
         g3 = grid3d_times(args.steps, args.warmup)
         log(f"3-D {g3}")
+        log("uneven batch (16 lines, 3k-40k cells, leaf autotuned per size class)")
+        lb = lines_times(args.steps, args.warmup)
+        log(f"lines {lb}")
     cpu = None
     if not args.no_cpu:
         log("cpu baseline (oracle SuperLU, 1 wavenumber sample)")
@@ -315,6 +375,7 @@ def main():
         "roofline": roof, "roofline_fp64": roof64,
         "linearization": lin,
         "grid3d": g3,
+        "uneven_lines": lb,
         "cpu_baseline": cpu,
     }
     if cpu:

@@ -14,6 +14,8 @@
   spacings, n = 1..6) plus Wenner-Schlumberger (a = 1, 2, n = 1..8), sources
   merged by electrode pair: 849 sources, 1,636 data; m = log(rho) a Gaussian
   random field (mean log 100, std 1, correlation length 10 m, seed 1).
+* `uneven_lines_case` -- 16 independent lines of 3k-40k cells (log-uniform),
+  21-96 electrodes each, dipole-dipole, independent random fields.
 """
 import numpy as np
 
@@ -153,6 +155,31 @@ def line_25k_case(seed=1):
     rng = np.random.default_rng(seed)
     m = gaussian_random_field(mesh, rng, corr_len=10.0, std=1.0)
     return mesh, survey, m, rng
+
+
+def uneven_lines_case(seed=5, nlines=16, cells=(3000, 40000)):
+    """An uneven batch of independent 2.5-D lines: cell counts log-uniform in
+    `cells`; each line a 2:1 core of 1 m cells with 12 padding cells growing
+    1.3x on the sides and below; 21-96 electrodes on nodes across the core, a
+    dipole-dipole array (n = 1..6); m = log(rho) an independent Gaussian random
+    field (mean log 100, std 1, correlation length a tenth of the core width).
+    Returns a list of (mesh, survey, m)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(nlines):
+        N = float(np.exp(rng.uniform(np.log(cells[0]), np.log(cells[1]))))
+        nx = max(20, int(round(np.sqrt(2.0 * N) - 24.0)))   # (nx + 24)(nx / 2 + 12) = N
+        nz = max(10, nx // 2)
+        mesh = TensorMesh([[(1.0, 12, -1.3), (1.0, nx), (1.0, 12, 1.3)],
+                           [(1.0, 12, -1.3), (1.0, nz)]], origin="CN")
+        ne = int(np.clip(round(nx / 2.7), 21, 96))
+        step = max(1, (nx - 4) // (ne - 1))
+        start = 12 + (nx - (ne - 1) * step) // 2          # core nodes, centred array
+        xe = mesh.nodes_x[start + step * np.arange(ne)]
+        survey = build_survey(xe, dipole_dipole(xe, 6))
+        m = gaussian_random_field(mesh, rng, corr_len=max(5.0, nx / 10.0), std=1.0)
+        out.append((mesh, survey, m))
+    return out
 
 
 # ---------------------------------------------------------------------------
