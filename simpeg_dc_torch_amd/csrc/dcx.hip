@@ -1741,17 +1741,19 @@ __global__ void k_jmap(int ncell, int r, const int* __restrict__ tptr,
 }
 
 // gv[b][e] = -scale[b] sum_c Y[b][row][c] X[b][col_e][c]; one wavefront per
-// row of the caller's pattern, rows taken in the pattern's own (geometric)
-// order so that neighbouring rows share their X rows in L2.  The hardware
+// row of the given pattern: the caller's, or, when the fields are stored in
+// elimination order, the permuted one with oidx mapping its entries back to
+// the caller's order (rows then run in elimination order: consecutive
+// wavefronts read consecutive rows of Y, and a row's neighbours mostly lie in
+// its own supernode).  The hardware
 // deals workgroups round-robin to the 8 XCDs; the block index is remapped so
 // that each XCD walks one contiguous range of rows (its own L2).  gridDim.x
-// must be a multiple of 8.  iperm (may be null) maps rows to the storage
-// order of Y and X.  Up to MAXE entries of a row at a time; each lane loads
+// must be a multiple of 8.  Up to MAXE entries of a row at a time; each lane loads
 // 4 columns of the Y row and of every X row before it multiplies, so that
 // 4 (1 + MAXE) loads per lane are in flight; FP64 sums.
 template <typename TI, int MAXE>
 __global__ void k_grad_values(const int* __restrict__ crow, const int* __restrict__ ccol,
-                              const int* __restrict__ iperm, const TI* __restrict__ Y,
+                              const int* __restrict__ oidx, const TI* __restrict__ Y,
                               const TI* __restrict__ X, int n, int k,
                               const double* __restrict__ scale, double* __restrict__ gv,
                               int64_t nnz) {
@@ -1763,8 +1765,7 @@ __global__ void k_grad_values(const int* __restrict__ crow, const int* __restric
   const int lane = threadIdx.x & 63;
   if (row >= n) return;
   const int64_t xb = static_cast<int64_t>(b) * n * k;
-  const int prow = iperm ? iperm[row] : row;
-  const TI* y = Y + xb + static_cast<int64_t>(prow) * k;
+  const TI* y = Y + xb + static_cast<int64_t>(row) * k;
   const double sc = scale ? scale[b] : 1.0;
   for (int p0 = crow[row]; p0 < crow[row + 1]; p0 += MAXE) {
     const int ne = min(MAXE, crow[row + 1] - p0);
@@ -1772,7 +1773,7 @@ __global__ void k_grad_values(const int* __restrict__ crow, const int* __restric
 #pragma unroll
     for (int q = 0; q < MAXE; ++q) {
       const int cq = ccol[p0 + min(q, ne - 1)];  // surplus slots repeat the last entry
-      xr[q] = X + xb + static_cast<int64_t>(iperm ? iperm[cq] : cq) * k;
+      xr[q] = X + xb + static_cast<int64_t>(cq) * k;
     }
     double acc[MAXE];
 #pragma unroll
@@ -1801,7 +1802,7 @@ __global__ void k_grad_values(const int* __restrict__ crow, const int* __restric
     for (int q = 0; q < MAXE; ++q) {
       double a = acc[q];
       for (int off = 32; off > 0; off >>= 1) a += __shfl_xor(a, off, 64);
-      if (lane == 0 && q < ne) gv[b * nnz + p0 + q] = -sc * a;
+      if (lane == 0 && q < ne) gv[b * nnz + (oidx ? oidx[p0 + q] : p0 + q)] = -sc * a;
     }
   }
 }
@@ -2923,13 +2924,20 @@ int dcx_grad_values(dcx_plan* P, const void* y, const void* x, int k,
   const auto& S = P->S;
   const int blocks = ((grid1(S.n, 4) + 7) / 8) * 8;  // XCD remap needs a multiple of 8
   dim3 grid(blocks, P->nbatch);
-  const int* ip = (flags & DCX_PERMUTED) ? static_cast<const int*>(P->d_iperm) : nullptr;
+  // permuted fields: rows walked in elimination order through the permuted
+  // pattern (consecutive waves read consecutive rows of y, and the neighbours
+  // mostly lie in the same supernode), results scattered to the caller's
+  // entry order
+  const bool perm = (flags & DCX_PERMUTED) != 0;
+  const int* rp = perm ? P->d_pcrow : P->d_crow;
+  const int* cl = perm ? P->d_pcol : P->d_col;
+  const int* oi = perm ? P->d_pvidx : nullptr;
   // entries per pass: the longest row up to 5 (2-D stencils), else 8
   const bool wide = P->max_row_nnz > 5;
   auto go = [&](auto tag, auto me) {
     using TI = decltype(tag);
-    hipLaunchKernelGGL((k_grad_values<TI, decltype(me)::value>), grid, dim3(256), 0, st,
-                       P->d_crow, P->d_col, ip, static_cast<const TI*>(y),
+    hipLaunchKernelGGL((k_grad_values<TI, decltype(me)::value>), grid, dim3(256), 0, st, rp,
+                       cl, oi, static_cast<const TI*>(y),
                        static_cast<const TI*>(x), S.n, k, scale, gv,
                        static_cast<int64_t>(S.nnz));
   };
