@@ -1094,13 +1094,14 @@ __global__ __launch_bounds__(kThreads) void k_bwd_wave(
         }
       }
       // L_JJ^{-1} (the J rows of P_s) is lower triangular: k-steps wholly
-      // above a tile's first row multiply zeros and are skipped (not in the
-      // split-K variants, where the branches cost more registers than the
-      // skipped work saves)
+      // above a tile's first row multiply zeros and are skipped, and so are
+      // tiles wholly past this supernode's ns rows (the level's tile count
+      // fits its tallest supernode); not in the split-K variants, where the
+      // branches cost more registers than the skipped work saves
 #pragma unroll
       for (int t = 0; t < MT; ++t) {
         const int r = r0 + t * 16 + (lane & 15);
-        const bool live = KS > 1 || k0 + 4 * j + 4 > r0 + 16 * t;
+        const bool live = KS > 1 || (k0 + 4 * j + 4 > r0 + 16 * t && r0 + 16 * t < ns);
         av[j][t] = (kok && live && r < ns) ? PT[r + static_cast<int64_t>(kk) * ns] : T(0);
       }
     }
@@ -1109,7 +1110,7 @@ __global__ __launch_bounds__(kThreads) void k_bwd_wave(
       if (k0 + 4 * j < kend) {
 #pragma unroll
         for (int t = 0; t < MT; ++t)
-          if (KS > 1 || k0 + 4 * j + 4 > r0 + 16 * t) {
+          if (KS > 1 || (k0 + 4 * j + 4 > r0 + 16 * t && r0 + 16 * t < ns)) {
 #pragma unroll
             for (int v = 0; v < V; ++v) acc[t][v] = Mma<T>::op(av[j][t], bv[j][v], acc[t][v]);
           }
