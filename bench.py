@@ -196,6 +196,29 @@ def grid3d_times(steps, warmup):
     return out
 
 
+def readme_case_times(steps, warmup):
+    """The reference's published eval+grad case (README.md:33-39: 18.5 ms,
+    FP64, RTX 4070 Laptop): its 2.5-D regression problem, 100 x 50 cells of
+    1 m, 21 electrodes, dipole-dipole n = 1..6 (93 data), nky = 11 with the
+    SimPEG quadrature, a 10 ohm-m block in 100 ohm-m."""
+    from simpeg_dc_torch_amd import TorchDC2D, workloads
+    mesh, survey, m = workloads.regression_case()
+    mt = torch.tensor(m, device="cuda:0")
+    out = {"cells": mesh.nC, "reference_published_ms": 18.5,
+           "reference_hardware": "NVIDIA RTX 4070 Laptop (README.md:39)"}
+    for prec in ("fp64", "mixed"):
+        eng = TorchDC2D(mesh, survey, nky=11, device="cuda:0", precision=prec,
+                        quadrature="simpeg")
+        with torch.no_grad():
+            d = eng.dpred(mt)
+        rng = np.random.default_rng(0)
+        dobs = d * (1 + 0.05 * torch.tensor(rng.standard_normal(eng.nD), device="cuda:0"))
+        w = 1.0 / (0.05 * dobs.abs())
+        out[f"{prec}_ms"] = round(time_steps(eng, mt, dobs, w, steps, warmup), 3)
+        out["data"] = eng.nD
+    return out
+
+
 def lines_times(steps, warmup, autotune=True):
     """The uneven batch: 16 independent 2.5-D lines of 3k-40k cells
     (log-uniform, seed 5; 21-96 electrodes, dipole-dipole, 12 wavenumbers
@@ -342,7 +365,7 @@ def main():
     log("linearisation (FP64): J v, J^T v, J, diag(J^T D J)")
     lin = linearization_times(e64, mt)
     log(f"linearisation {lin}")
-    g3 = lb = None
+    g3 = lb = rc = None
     if args.workload == "all":
         log("3-D grid (TorchDC3D, 56 x 56 x 38 cells, 200 sources)")
         g3 = grid3d_times(args.steps, args.warmup)
@@ -350,6 +373,9 @@ def main():
         log("uneven batch (16 lines, 3k-40k cells, leaf autotuned per size class)")
         lb = lines_times(args.steps, args.warmup)
         log(f"lines {lb}")
+        log("the reference's published eval+grad case (5k cells, nky 11)")
+        rc = readme_case_times(args.steps, args.warmup)
+        log(f"readme case {rc}")
     cpu = None
     if not args.no_cpu:
         log("cpu baseline (oracle SuperLU, 1 wavenumber sample)")
@@ -376,6 +402,7 @@ def main():
         "linearization": lin,
         "grid3d": g3,
         "uneven_lines": lb,
+        "reference_readme_case": rc,
         "cpu_baseline": cpu,
     }
     if cpu:

@@ -531,6 +531,7 @@ __global__ __launch_bounds__(256) void k_extend_big(
 // row solve of a panel never waits on the sequential diagonal factor.
 constexpr int kRowsBig = 256;
 constexpr int kNB2 = 2 * kNB;
+constexpr int kOuter = 4 * kNB2;  // outer panel width of the large-front factor
 constexpr int kDg = 3 * kNB * kLDG + 2 * kNB;
 
 // called by every thread of the workgroup (barriers inside); Pn points at
@@ -683,26 +684,129 @@ __global__ __launch_bounds__(kRowsBig) void k_panel_rows_big(const SupDev* __res
   }
 }
 
-// C[i][l] -= sum_k Pan[i][k] Pan[l][k] over the trailing region of panel j0:
-// rows i in [jb, m) (J/R rows, then E rows), columns l in [jb, n1), lower
-// triangle only for the J/R rows.  Workgroup (x, y) covers rows jb + 64x..,
-// columns jb + 64y..; z = front x batch member.  Each wavefront: 16 rows x 64
-// columns (four 16x16 MFMA tiles), computed transposed so a lane's results
-// run down a column of F.
+// The update after an outer panel (K up to kOuter): an LDS-tiled GEMM.
+// Workgroup tile kTR rows (i) x kTC columns (l); K in chunks of kKC staged in
+// LDS k-major (a chunk's rows are contiguous in the column-major front, so
+// both the loads and the per-lane LDS reads are unit-stride), the next chunk
+// held in registers while the current one is multiplied.  Each wavefront:
+// kTR / 4 rows x kTC columns as (kTR / 64) x (kTC / 16) MFMA tiles, results
+// transposed so a lane's values run down a column of F.  Same row, column
+// and lower-triangle rules and the same lookahead tail as k_trail_big.
+constexpr int kTR = 128, kTC = 64, kKC = 16;
 template <typename T>
-__global__ __launch_bounds__(256) void k_trail_big(const SupDev* __restrict__ sup,
-                                                  const int* __restrict__ lvl,
-                                                  T* __restrict__ fcur, int64_t fcur_bs,
-                                                  int j0, int nbatch, T* __restrict__ dg,
-                                                  int* __restrict__ info) {
+__global__ __launch_bounds__(256) void k_trail_outer(const SupDev* __restrict__ sup,
+                                                    const int* __restrict__ lvl,
+                                                    T* __restrict__ fcur, int64_t fcur_bs,
+                                                    int j0, int nbatch, T* __restrict__ dg,
+                                                    int* __restrict__ info) {
+  __shared__ T As[kKC][kTR + 4];  // rows i of the chunk
+  __shared__ T Bs[kKC][kTC + 4];  // rows l of the chunk
   const int f = blockIdx.z / nbatch, b = blockIdx.z % nbatch;
   const SupDev S = sup[lvl[f]];
   const int ns = S.ns, nf = S.ns + S.rs, ld = nf + S.ns;
   if (j0 >= ns) return;
-  // E rows e >= j0 + jb are zero in this panel (see k_factor_level)
-  const int jb = min(kNB2, ns - j0), m = min(ld - j0, nf + jb), n1 = nf - j0;
+  const int jb = min(kOuter, ns - j0), m = min(ld - j0, nf + jb), n1 = nf - j0;
+  const int rb = jb + blockIdx.x * kTR, cb = jb + blockIdx.y * kTC;
+  if (rb >= m || cb >= n1 || (rb + kTR - 1 < n1 && rb + kTR - 1 < cb)) return;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  T* Pn = fcur + b * fcur_bs + S.front_off + j0 + static_cast<int64_t>(j0) * ld;
+  // chunk loads: A: kTR x kKC (16 per thread), B: kTC x kKC (8 per thread);
+  // thread t takes row (t % kTR) [A] / (t % kTC) [B] of columns t / kTR + ...
+  constexpr int kLA = kTR * kKC / 256, kLB = kTC * kKC / 256;
+  T ra[kLA], rbv[kLB];
+  auto load = [&](int k0) {
+#pragma unroll
+    for (int u = 0; u < kLA; ++u) {
+      const int idx = tid + 256 * u, r = idx % kTR, kk = k0 + idx / kTR;
+      ra[u] = (rb + r < m && kk < jb) ? Pn[(rb + r) + static_cast<int64_t>(kk) * ld] : T(0);
+    }
+#pragma unroll
+    for (int u = 0; u < kLB; ++u) {
+      const int idx = tid + 256 * u, r = idx % kTC, kk = k0 + idx / kTC;
+      rbv[u] = (cb + r < n1 && kk < jb) ? Pn[(cb + r) + static_cast<int64_t>(kk) * ld] : T(0);
+    }
+  };
+  constexpr int kMR = kTR / 64, kMC = kTC / 16;  // MFMA tiles per wavefront
+  typename Mma<T>::v4 acc[kMR][kMC];
+#pragma unroll
+  for (int x = 0; x < kMR; ++x)
+#pragma unroll
+    for (int y = 0; y < kMC; ++y) acc[x][y] = {T(0), T(0), T(0), T(0)};
+  const int kq = lane >> 4, l16 = lane & 15;
+  load(0);
+  for (int k0 = 0; k0 < jb; k0 += kKC) {
+    __syncthreads();  // the previous chunk's reads are done
+#pragma unroll
+    for (int u = 0; u < kLA; ++u) {
+      const int idx = tid + 256 * u;
+      As[idx / kTR][idx % kTR] = ra[u];
+    }
+#pragma unroll
+    for (int u = 0; u < kLB; ++u) {
+      const int idx = tid + 256 * u;
+      Bs[idx / kTC][idx % kTC] = rbv[u];
+    }
+    __syncthreads();
+    if (k0 + kKC < jb) load(k0 + kKC);
+#pragma unroll
+    for (int kk = 0; kk < kKC; kk += 4) {
+      T bi[kMR], al[kMC];
+#pragma unroll
+      for (int x = 0; x < kMR; ++x) bi[x] = As[kk + kq][wave * (kTR / 4) + 16 * x + l16];
+#pragma unroll
+      for (int y = 0; y < kMC; ++y) al[y] = Bs[kk + kq][16 * y + l16];
+#pragma unroll
+      for (int x = 0; x < kMR; ++x)
+#pragma unroll
+        for (int y = 0; y < kMC; ++y) acc[x][y] = Mma<T>::op(al[y], bi[x], acc[x][y]);
+    }
+  }
+#pragma unroll
+  for (int x = 0; x < kMR; ++x) {
+    const int i = rb + wave * (kTR / 4) + 16 * x + l16;
+    if (i >= m) continue;
+#pragma unroll
+    for (int y = 0; y < kMC; ++y)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int l = cb + 16 * y + Mma<T>::row(lane, e);
+        if (l < n1 && (i >= n1 || i >= l)) Pn[i + static_cast<int64_t>(l) * ld] -= acc[x][y][e];
+      }
+  }
+  if (blockIdx.x == 0 && blockIdx.y == 0 && j0 + jb < ns) {
+    __shared__ T sm[kDg];
+    __syncthreads();  // the tile's stores by every wavefront before wavefront 0 reads them
+    panel_diag<T>(Pn + jb + static_cast<int64_t>(jb) * ld, ld, min(kNB2, ns - j0 - jb), sm, info);
+    T* o = dg + static_cast<int64_t>(blockIdx.z) * kDg;
+    for (int q = threadIdx.x; q < kDg; q += 256) o[q] = sm[q];
+  }
+}
+
+// C[i][l] -= sum_k Pan[i][k] Pan[l][k] over the trailing region of the
+// solved panel columns [j0, j0 + jb), jb = min(KW, ns - j0): rows i in
+// [jb, m) (J/R rows, then the E rows still live), columns l in [jb, ce),
+// lower triangle only for the J/R rows.  ce = n1 (cend < 0: the whole
+// trailing front) or, inside an outer panel, min(cend, ns) - j0 (the outer
+// panel's own columns; the rest of the front takes one update of width KW =
+// kOuter after the outer panel).  Workgroup (x, y) covers rows jb + 64x..,
+// columns jb + 64y..; z = front x batch member.  Each wavefront: 16 rows x 64
+// columns (four 16x16 MFMA tiles), computed transposed so a lane's results
+// run down a column of F.
+template <typename T, int KW>
+__global__ __launch_bounds__(256) void k_trail_big(const SupDev* __restrict__ sup,
+                                                  const int* __restrict__ lvl,
+                                                  T* __restrict__ fcur, int64_t fcur_bs,
+                                                  int j0, int cend, int nbatch,
+                                                  T* __restrict__ dg, int* __restrict__ info) {
+  const int f = blockIdx.z / nbatch, b = blockIdx.z % nbatch;
+  const SupDev S = sup[lvl[f]];
+  const int ns = S.ns, nf = S.ns + S.rs, ld = nf + S.ns;
+  if (j0 >= ns) return;
+  // E rows e >= j0 + jb are zero in these panel columns (see k_factor_level)
+  const int jb = min(KW, ns - j0), m = min(ld - j0, nf + jb), n1 = nf - j0;
+  const int ce = cend < 0 ? n1 : min(n1, min(cend, ns) - j0);
   const int rb = jb + blockIdx.x * 64, cb = jb + blockIdx.y * 64;
-  if (rb >= m || cb >= n1 || (rb + 63 < n1 && rb + 63 < cb)) return;  // empty or upper
+  if (rb >= m || cb >= ce || (rb + 63 < n1 && rb + 63 < cb)) return;  // empty or upper
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   T* Pn = fcur + b * fcur_bs + S.front_off + j0 + static_cast<int64_t>(j0) * ld;
   const int i = rb + wave * 16 + (lane & 15);
@@ -711,8 +815,8 @@ __global__ __launch_bounds__(256) void k_trail_big(const SupDev* __restrict__ su
 #pragma unroll
   for (int t = 0; t < 4; ++t) acc[t] = {T(0), T(0), T(0), T(0)};
   constexpr int kStage = 8;  // k-steps whose operand loads are issued together
-#pragma unroll
-  for (int k0 = 0; k0 < kNB2; k0 += 4 * kStage) {
+#pragma unroll(KW <= kNB2 ? KW / (4 * kStage) : 1)
+  for (int k0 = 0; k0 < KW; k0 += 4 * kStage) {
     T bv[kStage], av[kStage][4];
 #pragma unroll
     for (int j = 0; j < kStage; ++j) {
@@ -722,7 +826,7 @@ __global__ __launch_bounds__(256) void k_trail_big(const SupDev* __restrict__ su
 #pragma unroll
       for (int t = 0; t < 4; ++t) {
         const int l = cb + 16 * t + (lane & 15);
-        av[j][t] = (kok && l < n1) ? Pn[l + static_cast<int64_t>(kk) * ld] : T(0);
+        av[j][t] = (kok && l < ce) ? Pn[l + static_cast<int64_t>(kk) * ld] : T(0);
       }
     }
 #pragma unroll
@@ -736,15 +840,15 @@ __global__ __launch_bounds__(256) void k_trail_big(const SupDev* __restrict__ su
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
         const int l = cb + 16 * t + Mma<T>::row(lane, e);
-        if (l < n1 && (i >= n1 || i >= l)) Pn[i + static_cast<int64_t>(l) * ld] -= acc[t][e];
+        if (l < ce && (i >= n1 || i >= l)) Pn[i + static_cast<int64_t>(l) * ld] -= acc[t][e];
       }
   }
-  // lookahead: tile (0, 0) is the next panel's diagonal block, now final
-  if (blockIdx.x == 0 && blockIdx.y == 0 && j0 + kNB2 < ns) {
+  // lookahead: tile (0, 0) is the next panel's (columns j0 + jb on) diagonal
+  // block, now final
+  if (blockIdx.x == 0 && blockIdx.y == 0 && j0 + jb < ns) {
     __shared__ T sm[kDg];
     __syncthreads();  // the tile's stores by every wavefront before wavefront 0 reads them
-    panel_diag<T>(Pn + jb + static_cast<int64_t>(jb) * ld, ld, min(kNB2, ns - j0 - kNB2), sm,
-                  info);
+    panel_diag<T>(Pn + jb + static_cast<int64_t>(jb) * ld, ld, min(kNB2, ns - j0 - jb), sm, info);
     T* o = dg + static_cast<int64_t>(blockIdx.z) * kDg;
     for (int q = threadIdx.x; q < kDg; q += 256) o[q] = sm[q];
   }
@@ -2122,14 +2226,26 @@ int factor_level_big(dcx_plan* P, int d, int cnt, hipStream_t st) {
   T* dg = static_cast<T*>(P->d_dg);
   hipLaunchKernelGGL(k_diag_big<T>, dim3(cnt * nb), dim3(64), 0, st, P->d_sup, lvl, fr[d & 1],
                      fbs[d & 1], dg, nb, P->d_info);
-  for (int j0 = 0; j0 < maxns; j0 += kNB2) {
-    // tallest live row range (J, R and the E rows e < j0 + kNB2) and widest n1
-    const int rows = std::min(maxnf + maxns - j0, maxnf + kNB2), cols = maxnf - j0;
-    hipLaunchKernelGGL(k_panel_rows_big<T>, dim3((rows + kRowsBig - 1) / kRowsBig, cnt * nb),
-                       dim3(kRowsBig), 0, st, P->d_sup, lvl, fr[d & 1], fbs[d & 1], dg, j0, nb);
-    dim3 tg((rows + 63) / 64, (cols + 63) / 64, cnt * nb);
-    hipLaunchKernelGGL(k_trail_big<T>, tg, dim3(256), 0, st, P->d_sup, lvl, fr[d & 1],
-                       fbs[d & 1], j0, nb, dg, P->d_info);
+  // outer panels of kOuter columns: inside one, 64-column panels whose
+  // trailing updates stay within its columns; then one update of width kOuter
+  // for the rest of the front
+  for (int J0 = 0; J0 < maxns; J0 += kOuter) {
+    const int Jend = J0 + kOuter;
+    for (int j0 = J0; j0 < std::min(Jend, maxns); j0 += kNB2) {
+      // tallest live row range (J, R and the E rows e < j0 + kNB2)
+      const int rows = std::min(maxnf + maxns - j0, maxnf + kNB2);
+      hipLaunchKernelGGL(k_panel_rows_big<T>, dim3((rows + kRowsBig - 1) / kRowsBig, cnt * nb),
+                         dim3(kRowsBig), 0, st, P->d_sup, lvl, fr[d & 1], fbs[d & 1], dg, j0, nb);
+      if (j0 + kNB2 < std::min(Jend, maxns)) {
+        dim3 tg((rows + 63) / 64, (Jend - j0 - kNB2 + 63) / 64, cnt * nb);
+        hipLaunchKernelGGL((k_trail_big<T, kNB2>), tg, dim3(256), 0, st, P->d_sup, lvl,
+                           fr[d & 1], fbs[d & 1], j0, Jend, nb, dg, P->d_info);
+      }
+    }
+    const int rows = std::min(maxnf + maxns - J0, maxnf + kOuter), cols = maxnf - J0;
+    dim3 tg((rows + kTR - 1) / kTR, (cols + kTC - 1) / kTC, cnt * nb);
+    hipLaunchKernelGGL(k_trail_outer<T>, tg, dim3(256), 0, st, P->d_sup, lvl, fr[d & 1],
+                       fbs[d & 1], J0, nb, dg, P->d_info);
   }
   hipLaunchKernelGGL(k_front_out_big<T>, dim3((maxnf + 63) / 64, (maxns + 63) / 64, cnt * nb),
                      dim3(256), 0, st, P->d_sup, lvl, fr[d & 1], fbs[d & 1],
