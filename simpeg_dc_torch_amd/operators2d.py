@@ -156,8 +156,10 @@ class Problem2DMaps:
         self.nS = len(src)
         q_rows, q_cols, q_vals = [], [], []
         P_rows, P_cols, P_vals, sidx = [], [], [], []
+        self._src_at, self._rx_at = [], []  # (sign, (x, z)) per source / per datum
         nd = 0
         for s, so in enumerate(src):
+            at = []
             for sign, loc in ((1.0, so.location_a), (-1.0, getattr(so, "location_b", None))):
                 if loc is None:
                     continue
@@ -166,6 +168,8 @@ class Problem2DMaps:
                     continue
                 n_, w_ = self.interp(loc[0], loc[-1])
                 q_rows += n_; q_cols += [s] * len(n_); q_vals += [sign * v for v in w_]
+                at.append((sign, (float(loc[0]), float(loc[-1]))))
+            self._src_at.append(at)
             for rx in so.receiver_list:
                 M = np.atleast_2d(np.asarray(rx.locations_m, float))
                 Nn = getattr(rx, "locations_n", None)
@@ -173,9 +177,12 @@ class Problem2DMaps:
                 for i in range(M.shape[0]):
                     n_, w_ = self.interp(M[i, 0], M[i, -1])
                     P_rows += [nd] * len(n_); P_cols += n_; P_vals += w_
+                    at = [(1.0, (float(M[i, 0]), float(M[i, -1])))]
                     if Nn is not None:
                         n_, w_ = self.interp(Nn[i, 0], Nn[i, -1])
                         P_rows += [nd] * len(n_); P_cols += n_; P_vals += [-v for v in w_]
+                        at.append((-1.0, (float(Nn[i, 0]), float(Nn[i, -1]))))
+                    self._rx_at.append(at)
                     sidx.append(s)
                     nd += 1
         self.nD = nd
@@ -186,6 +193,33 @@ class Problem2DMaps:
         Pd.sort_indices()
         self.Pd = Pd
         self.sidx = np.asarray(sidx, np.int64)
+
+    def electrode_basis(self):
+        """(E, S, R): E (nN x ne) the interpolation columns of the ne distinct
+        electrode positions (sources and receivers), S (ne x nS) and R
+        (ne x nD) their signed combinations, so that q = E S and
+        Pd = (E R)^T exactly (the same interpolation weights)."""
+        import scipy.sparse as sp
+        pos = {}
+        for at in self._src_at + self._rx_at:
+            for _, loc in at:
+                pos.setdefault(loc, len(pos))
+        ne = len(pos)
+        er, ec, ev = [], [], []
+        for loc, e in pos.items():
+            n_, w_ = self.interp(loc[0], loc[1])
+            er += n_; ec += [e] * len(n_); ev += list(w_)
+        E = sp.csr_matrix((ev, (er, ec)), shape=(self.nN, ne))
+
+        def combo(ats, ncol):
+            r, c, v = [], [], []
+            for j, at in enumerate(ats):
+                for sign, loc in at:
+                    r.append(pos[loc]); c.append(j); v.append(sign)
+            M = sp.csr_matrix((v, (r, c)), shape=(ne, ncol))
+            M.sum_duplicates()
+            return M
+        return E, combo(self._src_at, self.nS), combo(self._rx_at, self.nD)
 
     # -- helpers ----------------------------------------------------------------
     def values(self, sigma):

@@ -98,6 +98,23 @@ class _GpuState:
         # mixed precision keeps the eval+grad fields in FP32 (the factor's own
         # precision): sums over data, wavenumbers and sources stay FP64
         self.fields32 = precision == "mixed"
+        if getattr(eng, "rhs_basis", "sources") == "electrodes":
+            self._electrode_basis(maps, dev)
+
+    def _electrode_basis(self, maps, dev):
+        """Electrode-basis data: E (elimination order, dense n x ne) as the
+        right-hand side, its support rows and weights for E^T G, and the
+        source / receiver combinations S (ne x nS) and R (ne x nD)."""
+        E, S, R = maps.electrode_basis()
+        Ep = E.tocsr()[self.solver.perm()]
+        self.ne = E.shape[1]
+        self.E_perm = torch.tensor(Ep.toarray(), dtype=torch.float64, device=dev)
+        sup = np.unique(Ep.tocoo().row)
+        self.e_sup = torch.tensor(sup, dtype=torch.int64, device=dev)
+        self.E_supT = torch.tensor(Ep[sup].toarray().T.copy(), dtype=torch.float64, device=dev)
+        self.S = torch.tensor(S.toarray(), dtype=torch.float64, device=dev)
+        self.R = torch.tensor(R.toarray(), dtype=torch.float64, device=dev)
+        self.sidx = torch.tensor(maps.sidx, dtype=torch.int64, device=dev)
 
     def stream(self):
         return self.solver._stream()
@@ -206,13 +223,60 @@ class _Forward2D(torch.autograd.Function):
         return grad_m, None
 
 
+class _ForwardElectrodes2D(torch.autograd.Function):
+    """m_full -> dpred on the GPU through the electrode basis.
+
+    Every source and receiver is a signed sum of electrode positions
+    (q = E S, Pd = (E R)^T), so one solve per wavenumber with the ne electrode
+    columns, G_k = A_k^{-1} E, gives every field: U_k = G_k S and, for the
+    adjoint, Lambda_k = G_k T with T[:, s] = sum over the data j of source s of
+    v_j R[:, j].  Forward: d_j = R_j^T (sum_k w_k E^T G_k) S_src(j).  Reverse:
+    dphi/dA_k = -w_k G_k M G_k^T with M = T S^T (ne x ne), gathered entry-wise
+    as -w_k sum_c G_k[row, c] (G_k M^T)[col, c]; no second solve.  The same
+    results as the source basis (the reference's formulation), with ne
+    right-hand sides instead of nS, and nS instead of 2 nS solved columns."""
+
+    @staticmethod
+    def forward(ctx, m_full, eng):
+        g = eng._gpu
+        with torch.cuda.device(eng.device):
+            sig = torch.exp(-m_full.detach())
+            vals = g.assemble(sig)
+            G = g.factor_solve_perm(vals, g.E_perm, g.ne, f32=g.fields32)
+            Ke = torch.matmul(g.E_supT, G[:, g.e_sup, :].to(torch.float64))  # (nky, ne, ne)
+            K = (g.wq.view(-1, 1, 1) * Ke).sum(0)
+            d = (g.R * (K @ g.S)[:, g.sidx]).sum(0)
+        ctx.eng = eng
+        ctx.stamp = g.solver.stamp
+        ctx.save_for_backward(sig, vals, G)
+        return d
+
+    @staticmethod
+    def backward(ctx, gd):
+        sig, vals, G = ctx.saved_tensors
+        eng = ctx.eng
+        g = eng._gpu
+        with torch.cuda.device(eng.device):
+            if g.solver.stamp != ctx.stamp:      # refactorised since: solve again
+                g.solver.factorize(vals)
+                G = g.solve_perm(g.E_perm, g.ne, f32=g.fields32)
+            v = gd.detach().to(torch.float64)
+            T = torch.zeros(g.ne, eng.nS, dtype=torch.float64, device=eng.device)
+            T.index_add_(1, g.sidx, g.R * v)
+            M = T @ g.S.T
+            H = torch.matmul(G, M.T.to(G.dtype).contiguous())
+            gv = g.solver.grad_values(G, H, scale=g.wq, permuted=True)
+            grad_m = g.lmap_t(gv, sig)
+        return grad_m, None
+
+
 class TorchDC2D:
     """Differentiable dpred/misfit for 2.5-D nodal DC on a fixed mesh+survey."""
 
     def __init__(self, mesh, survey, nky=12, device="cuda", dtype=torch.float64,
                  active_cells=None, val_inactive=None, chunk=None,
                  singularity_removal=False, rho_primary=None, quadrature=None,
-                 precision="fp64", refine=1, leaf=None):
+                 precision="fp64", refine=1, leaf=None, rhs_basis="sources"):
         if singularity_removal:
             raise NotImplementedError(
                 "singularity removal is not part of this engine (off by default in the "
@@ -228,6 +292,9 @@ class TorchDC2D:
         self.chunk = None if chunk is None else int(chunk)
         self._sr = False
         self.precision = precision
+        if rhs_basis not in ("sources", "electrodes"):
+            raise ValueError(f"rhs_basis must be sources|electrodes, got {rhs_basis!r}")
+        self.rhs_basis = rhs_basis
 
         rmin, rmax = electrode_distance_range(survey)
         sp_pts, sp_wts, fell_back = simpeg_quadrature(mesh, nky)
@@ -355,7 +422,8 @@ class TorchDC2D:
         dev0, dt0 = m.device, m.dtype
         m_full = self._model_to_full(m)
         if self._gpu is not None:
-            d = _Forward2D.apply(m_full.to(torch.float64), self)
+            fn = _ForwardElectrodes2D if self.rhs_basis == "electrodes" else _Forward2D
+            d = fn.apply(m_full.to(torch.float64), self)
             return d.to(device=dev0, dtype=dt0)
         if self.chunk and self.chunk < self.nS:
             d = torch.empty(self.nD, device=self.device, dtype=self.dtype)

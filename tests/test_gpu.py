@@ -427,3 +427,32 @@ def test_lbfgs_3d_gpu_decreases_objective():
     m1, hist = lbfgs_minimize(eng, m0, dobs, w, Hm=Hm, beta=0.1, max_iter=6)
     assert hist[-1][0] + 0.1 * hist[-1][1] < hist[0][0] + 0.1 * hist[0][1]
     assert m1.device.type == "cuda"
+
+
+@pytest.mark.parametrize("name,precision", [("suite", "fp64"), ("regression", "fp64"),
+                                            ("suite", "mixed"), ("line25k", "fp64")])
+def test_electrode_basis_matches_source_basis(name, precision):
+    """rhs_basis="electrodes" (one solve with the electrode columns, adjoint
+    by the electrode-space algebra) against the source basis (the
+    reference's formulation): dpred, misfit gradient and J^T v."""
+    from simpeg_dc_torch_amd import TorchDC2D
+    _say(f"electrode basis {name} {precision}")
+    mesh, survey, m = case(name)
+    nky = 12 if name == "line25k" else 8
+    e_s = TorchDC2D(mesh, survey, nky=nky, device=DEV, precision=precision, refine=0)
+    e_e = TorchDC2D(mesh, survey, nky=nky, device=DEV, precision=precision, refine=0,
+                    rhs_basis="electrodes")
+    mt = torch.tensor(m, device=DEV)
+    with torch.no_grad():
+        d_s, d_e = e_s.dpred(mt), e_e.dpred(mt)
+    tol = 1e-12 if precision == "fp64" else 1e-6
+    assert float(torch.linalg.norm(d_e - d_s) / torch.linalg.norm(d_s)) < tol
+    dobs, w = noisy_data(d_s.cpu().numpy())
+    dobs, w = torch.tensor(dobs, device=DEV), torch.tensor(w, device=DEV)
+    _, g_s = e_s.misfit_and_grad(mt, dobs, w)
+    _, g_e = e_e.misfit_and_grad(mt, dobs, w)
+    tolg = 1e-10 if precision == "fp64" else 1e-4
+    assert float(torch.linalg.norm(g_e - g_s) / torch.linalg.norm(g_s)) < tolg
+    v = torch.tensor(np.random.default_rng(4).standard_normal(e_s.nD), device=DEV)
+    j_s, j_e = e_s.Jtvec(mt, v), e_e.Jtvec(mt, v)
+    assert float(torch.linalg.norm(j_e - j_s) / torch.linalg.norm(j_s)) < tolg
