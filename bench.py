@@ -193,6 +193,12 @@ def grid3d_times(steps, warmup):
             for nm, a, b in (("dpred", d, ref[0]), ("grad", g, ref[1]), ("jtvec", jt, ref[2])):
                 out[f"{key}_{nm}_rel_err"] = float(torch.linalg.norm(a - b) / torch.linalg.norm(b))
         del eng
+    for prec in ("fp64", "mixed"):
+        eng = TorchDC3D(mesh, survey, device="cuda:0", precision=prec, refine=0,
+                        rhs_basis="electrodes")
+        out[f"electrode_basis_{prec}_ms"] = round(time_steps(eng, mt, dobs, w, steps, warmup), 2)
+        out["electrodes"] = eng._gpu.ne
+        del eng
     return out
 
 

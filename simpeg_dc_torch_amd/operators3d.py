@@ -178,8 +178,10 @@ class Problem3DMaps:
         self.nS = len(src)
         q_rows, q_cols, q_vals = [], [], []
         P_rows, P_cols, P_vals, sidx = [], [], [], []
+        self._src_at, self._rx_at = [], []  # (sign, position) per source / per datum
         nd = 0
         for s, so in enumerate(src):
+            at = []
             for sign, loc in ((1.0, so.location_a), (-1.0, getattr(so, "location_b", None))):
                 if loc is None:
                     continue
@@ -188,6 +190,8 @@ class Problem3DMaps:
                     continue
                 n_, w_ = self.interp(loc)
                 q_rows += n_; q_cols += [s] * len(n_); q_vals += [sign * v for v in w_]
+                at.append((sign, tuple(float(x) for x in loc)))
+            self._src_at.append(at)
             for rx in so.receiver_list:
                 M = np.atleast_2d(np.asarray(rx.locations_m, float))
                 Nn = getattr(rx, "locations_n", None)
@@ -195,9 +199,12 @@ class Problem3DMaps:
                 for i in range(M.shape[0]):
                     n_, w_ = self.interp(M[i])
                     P_rows += [nd] * len(n_); P_cols += n_; P_vals += w_
+                    at = [(1.0, tuple(float(x) for x in M[i]))]
                     if Nn is not None:
                         n_, w_ = self.interp(Nn[i])
                         P_rows += [nd] * len(n_); P_cols += n_; P_vals += [-v for v in w_]
+                        at.append((-1.0, tuple(float(x) for x in Nn[i])))
+                    self._rx_at.append(at)
                     sidx.append(s)
                     nd += 1
         self.nD = nd
@@ -207,6 +214,31 @@ class Problem3DMaps:
         Pd.sort_indices()
         self.Pd = Pd
         self.sidx = np.asarray(sidx, np.int64)
+
+    def electrode_basis(self):
+        """(E, S, R) with q = E S and Pd = (E R)^T: E (nN x ne) the
+        interpolation columns of the distinct electrode positions, S and R
+        their signed combinations per source and per datum."""
+        pos = {}
+        for at in self._src_at + self._rx_at:
+            for _, loc in at:
+                pos.setdefault(loc, len(pos))
+        ne = len(pos)
+        er, ec, ev = [], [], []
+        for loc, e in pos.items():
+            n_, w_ = self.interp(np.asarray(loc))
+            er += n_; ec += [e] * len(n_); ev += list(w_)
+        E = sp.csr_matrix((ev, (er, ec)), shape=(self.nN, ne))
+
+        def combo(ats, ncol):
+            r, c, v = [], [], []
+            for j, at in enumerate(ats):
+                for sign, loc in at:
+                    r.append(pos[loc]); c.append(j); v.append(sign)
+            M = sp.csr_matrix((v, (r, c)), shape=(ne, ncol))
+            M.sum_duplicates()
+            return M
+        return E, combo(self._src_at, self.nS), combo(self._rx_at, self.nD)
 
     # -- helpers ----------------------------------------------------------------
     def values(self, sigma):

@@ -5,7 +5,9 @@ device="cuda", dtype=torch.float64, active_cells=None, val_inactive=None,
 bc_type="Robin")`` with ``dpred``, ``Jtvec``, ``misfit`` and the attributes
 ``nC, nN, nS, nD, nnzA, L3, L3t, q, Pd, sidx, iD, solver``.  The model is
 m = log(rho).  Extra keywords as in TorchDC2D: ``precision`` ("fp64" or
-"mixed"), ``refine``, ``leaf``.
+"mixed"), ``refine``, ``leaf``, ``rhs_basis`` ("sources" or "electrodes":
+one solve with the distinct electrode positions as right-hand sides, the
+adjoint by electrode-space algebra; the same results).
 
 The system is the 2.5-D one with a single "wavenumber" of zero and weight
 one, so on a GPU device the same libdcx.so path runs: the factored
@@ -20,7 +22,7 @@ import scipy.sparse as sp
 import torch
 
 from .operators3d import Problem3DMaps
-from .simulation2d import _Forward2D, _GpuState
+from .simulation2d import _Forward2D, _ForwardElectrodes2D, _GpuState
 from .solver import make_solver, sparse_solve
 
 
@@ -51,7 +53,8 @@ class TorchDC3D:
     """Differentiable dpred/misfit for 3-D nodal DC on a fixed mesh+survey."""
 
     def __init__(self, mesh, survey, device="cuda", dtype=torch.float64, active_cells=None,
-                 val_inactive=None, bc_type="Robin", precision="fp64", refine=1, leaf=None):
+                 val_inactive=None, bc_type="Robin", precision="fp64", refine=1, leaf=None,
+                 rhs_basis="sources"):
         if bc_type != "Robin":
             raise NotImplementedError("only the Robin boundary condition (the default) is built")
         self.device = torch.device(device)
@@ -59,6 +62,9 @@ class TorchDC3D:
             self.device = torch.device("cuda", torch.cuda.current_device())
         self.dtype = dtype
         self.precision = precision
+        if rhs_basis not in ("sources", "electrodes"):
+            raise ValueError(f"rhs_basis must be sources|electrodes, got {rhs_basis!r}")
+        self.rhs_basis = rhs_basis
         maps = Problem3DMaps(mesh, survey)
         self._maps = maps
         self.kys = np.zeros(1)
@@ -136,7 +142,8 @@ class TorchDC3D:
         dev0, dt0 = m.device, m.dtype
         m_full = self._model_to_full(m)
         if self._gpu is not None:
-            d = _Forward2D.apply(m_full.to(torch.float64), self)
+            fn = _ForwardElectrodes2D if self.rhs_basis == "electrodes" else _Forward2D
+            d = fn.apply(m_full.to(torch.float64), self)
             return d.to(device=dev0, dtype=dt0)
         sig = torch.exp(-m_full)
         VALS = _SpMV.apply(sig, self.L3, self.L3t)

@@ -456,3 +456,21 @@ def test_electrode_basis_matches_source_basis(name, precision):
     v = torch.tensor(np.random.default_rng(4).standard_normal(e_s.nD), device=DEV)
     j_s, j_e = e_s.Jtvec(mt, v), e_e.Jtvec(mt, v)
     assert float(torch.linalg.norm(j_e - j_s) / torch.linalg.norm(j_s)) < tolg
+
+
+@pytest.mark.parametrize("name", ["small_3d", "medium_3d"])
+def test_electrode_basis_3d_matches_source_basis(name):
+    from simpeg_dc_torch_amd import TorchDC3D
+    _say(f"electrode basis 3-D {name}")
+    mesh, survey, m = case(name)
+    e_s = TorchDC3D(mesh, survey, device=DEV)
+    e_e = TorchDC3D(mesh, survey, device=DEV, rhs_basis="electrodes")
+    mt = torch.tensor(m, device=DEV)
+    with torch.no_grad():
+        d_s, d_e = e_s.dpred(mt), e_e.dpred(mt)
+    assert float(torch.linalg.norm(d_e - d_s) / torch.linalg.norm(d_s)) < 1e-12
+    dobs, w = noisy_data(d_s.cpu().numpy())
+    dobs, w = torch.tensor(dobs, device=DEV), torch.tensor(w, device=DEV)
+    _, g_s = e_s.misfit_and_grad(mt, dobs, w)
+    _, g_e = e_e.misfit_and_grad(mt, dobs, w)
+    assert float(torch.linalg.norm(g_e - g_s) / torch.linalg.norm(g_s)) < 1e-10
