@@ -225,6 +225,71 @@ def electrode_basis_times(mesh, survey, mt, dobs, w, d64, g64, steps, warmup):
     return out
 
 
+def gn_step_times(reps=3):
+    """One Gauss-Newton step on the 62,016-cell, 2,430-datum line (FP64): data
+    from the true model with 5% + 1e-4 V noise (seed 3), start at a 200 ohm-m
+    half-space, SimPEG-style tensor regularisation (alpha_s 1e-4), beta 10.
+    Times (median of `reps`) of the linearisation, the explicit J (batched
+    adjoint), the Woodbury direct step (H_m solve of J^T through the HIP
+    solver, S = J K, dense Cholesky) and the Occam beta search (one eigh), and
+    the misfit before and after the Occam step (one forward)."""
+    from simpeg_dc_torch_amd import TorchDC2D, workloads
+    from simpeg_dc_torch_amd.gauss_newton import (HmSolver, direct_step, occam_step,
+                                                  tensor_regularization)
+    mesh, survey, m_true = workloads.gn_line_case()
+    dev = torch.device("cuda:0")
+    eng = TorchDC2D(mesh, survey, nky=12, device=dev, precision="fp64")
+    with torch.no_grad():
+        d_true = eng.dpred(torch.tensor(m_true, device=dev))
+    rng = np.random.default_rng(3)
+    std = 0.05 * d_true.abs() + 1e-4
+    dobs = d_true + std * torch.tensor(rng.standard_normal(eng.nD), device=dev)
+    w = 1.0 / (0.05 * dobs.abs() + 1e-4)
+    m0 = torch.full((mesh.nC,), float(np.log(200.0)), dtype=torch.float64, device=dev)
+    Hm = tensor_regularization(mesh, alpha_s=1e-4)
+    hs = HmSolver(dev)
+    beta = 10.0
+
+    def med(fn):
+        ts, out = [], None
+        for _ in range(reps):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            out = fn()
+            torch.cuda.synchronize()
+            ts.append(1e3 * (time.perf_counter() - t0))
+        return sorted(ts)[len(ts) // 2], out
+
+    t_lin, lin = med(lambda: eng.linearize(m0))
+    t_j, J = med(lambda: lin.Jmatrix())
+    with torch.no_grad():
+        d0 = eng.dpred(m0)
+    r = w * (d0 - dobs)
+    D, wd = 2.0 * w ** 2, w * (2.0 * r)
+    g_d = J.T @ wd
+    import scipy.sparse as sp_
+    H = sp_.csr_matrix(Hm)
+    Ht = torch.sparse_csr_tensor(torch.as_tensor(H.indptr, dtype=torch.int64, device=dev),
+                                 torch.as_tensor(H.indices, dtype=torch.int64, device=dev),
+                                 torch.as_tensor(H.data, dtype=torch.float64, device=dev),
+                                 H.shape)
+    g_m = Ht @ m0
+    t_dir, p_dir = med(lambda: direct_step(J, D, g_d + beta * g_m, H, beta, hs))
+    t_occ, occ = med(lambda: occam_step(J, D, g_d, g_m, r, beta, H, hs,
+                                        lambda rr: (rr ** 2).sum(), w))
+    b_occ, p_occ, pred = occ
+    with torch.no_grad():
+        d1 = eng.dpred(m0 + p_occ)
+    return {"cells": mesh.nC, "data": eng.nD, "sources": eng.nS, "nky": eng.nky,
+            "precision": "fp64", "linearize_ms": round(t_lin, 2),
+            "jmatrix_ms": round(t_j, 2), "direct_step_ms": round(t_dir, 2),
+            "occam_step_ms": round(t_occ, 2), "hm_solve": hs.last,
+            "phi_d_start": float((r ** 2).sum()),
+            "occam_beta": b_occ, "occam_predicted_phi_d": pred,
+            "phi_d_after_occam_step": float(((w * (d1 - dobs)) ** 2).sum()),
+            "direct_step_finite": bool(p_dir is not None and torch.isfinite(p_dir).all())}
+
+
 def readme_case_times(steps, warmup):
     """The reference's published eval+grad case (README.md:33-39: 18.5 ms,
     FP64, RTX 4070 Laptop): its 2.5-D regression problem, 100 x 50 cells of
@@ -344,7 +409,8 @@ def main():
     ap.add_argument("--refine", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--pmc-pass", action="store_true")
-    ap.add_argument("--workload", choices=("all", "line25k", "grid3d", "lines"), default="all",
+    ap.add_argument("--workload", choices=("all", "line25k", "grid3d", "lines", "gn"),
+                    default="all",
                     help="all (default): the 2.5-D line with the 3-D grid and the uneven "
                          "batch of lines beside it; line25k, grid3d or lines alone")
     args = ap.parse_args()
@@ -359,6 +425,10 @@ def main():
         log("3-D grid only (TorchDC3D, 56 x 56 x 38 cells, 200 sources)")
         print(json.dumps({"workload": "grid3d", **grid3d_times(args.steps, args.warmup)}),
               flush=True)
+        return
+    if args.workload == "gn":
+        log("Gauss-Newton step only (62k cells, 2,430 data)")
+        print(json.dumps({"workload": "gn", **gn_step_times()}), flush=True)
         return
     if args.workload == "lines":
         log("uneven batch only (16 lines, 3k-40k cells)")
@@ -397,7 +467,7 @@ def main():
     log("linearisation (FP64): J v, J^T v, J, diag(J^T D J)")
     lin = linearization_times(e64, mt)
     log(f"linearisation {lin}")
-    g3 = lb = rc = None
+    g3 = lb = rc = gn = None
     if args.workload == "all":
         log("3-D grid (TorchDC3D, 56 x 56 x 38 cells, 200 sources)")
         g3 = grid3d_times(args.steps, args.warmup)
@@ -405,6 +475,9 @@ def main():
         log("uneven batch (16 lines, 3k-40k cells, leaf autotuned per size class)")
         lb = lines_times(args.steps, args.warmup)
         log(f"lines {lb}")
+        log("Gauss-Newton step (62k cells, 2,430 data, FP64)")
+        gn = gn_step_times()
+        log(f"gn step {gn}")
         log("the reference's published eval+grad case (5k cells, nky 11)")
         rc = readme_case_times(args.steps, args.warmup)
         log(f"readme case {rc}")
@@ -436,6 +509,7 @@ def main():
         "grid3d": g3,
         "uneven_lines": lb,
         "reference_readme_case": rc,
+        "gn_step": gn,
         "cpu_baseline": cpu,
     }
     if cpu:

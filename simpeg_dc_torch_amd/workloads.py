@@ -14,6 +14,9 @@
   spacings, n = 1..6) plus Wenner-Schlumberger (a = 1, 2, n = 1..8), sources
   merged by electrode pair: 849 sources, 1,636 data; m = log(rho) a Gaussian
   random field (mean log 100, std 1, correlation length 10 m, seed 1).
+* `gn_line_case`     -- the Gauss-Newton line: 62,016 cells, 96 electrodes at
+  5 m, dipole-dipole a = 1..3, n = 1..10 (270 sources, 2,430 data), blocks of
+  5, 50 and 2000 ohm-m in 200 ohm-m.
 * `uneven_lines_case` -- 16 independent lines of 3k-40k cells (log-uniform),
   21-96 electrodes each, dipole-dipole, independent random fields.
 """
@@ -155,6 +158,26 @@ def line_25k_case(seed=1):
     rng = np.random.default_rng(seed)
     m = gaussian_random_field(mesh, rng, corr_len=10.0, std=1.0)
     return mesh, survey, m, rng
+
+
+def gn_line_case():
+    """The Gauss-Newton line: 62,016 cells (384 x 140 core of 1.25 m cells plus
+    12 padding cells growing 1.3x), 96 electrodes at 5 m, dipole-dipole with
+    a = 1, 2, 3 electrode spacings and n = 1..10 (a = 1 alone: 93 sources and
+    885 data; all three: 270 sources, 2,430 data); true model 200 ohm-m with
+    blocks of 5, 50 and 2000 ohm-m.  Returns (mesh, survey, m_true)."""
+    mesh = TensorMesh([[(1.25, 12, -1.3), (1.25, 384), (1.25, 12, 1.3)],
+                       [(1.25, 12, -1.3), (1.25, 140)]], origin="CN")
+    xe = np.arange(96) * 5.0 - 237.5
+    survey = build_survey(xe, dipole_dipole(xe, 10, spacings=(1, 2, 3)))
+    cc = mesh.cell_centers
+    m = np.full(mesh.nC, np.log(200.0))
+    for (x0, x1, z0, z1), rho in (((-150.0, -90.0, -40.0, -15.0), 5.0),
+                                  ((-30.0, 30.0, -60.0, -20.0), 50.0),
+                                  ((80.0, 150.0, -35.0, -10.0), 2000.0)):
+        inside = (cc[:, 0] > x0) & (cc[:, 0] < x1) & (cc[:, 1] > z0) & (cc[:, 1] < z1)
+        m[inside] = np.log(rho)
+    return mesh, survey, m
 
 
 def uneven_lines_case(seed=5, nlines=16, cells=(3000, 40000)):
