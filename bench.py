@@ -280,10 +280,19 @@ def gn_step_times(reps=3):
     b_occ, p_occ, pred = occ
     with torch.no_grad():
         d1 = eng.dpred(m0 + p_occ)
+    del lin
+    engE = TorchDC2D(mesh, survey, nky=12, device=dev, precision="fp64",
+                     rhs_basis="electrodes")
+    t_linE, linE = med(lambda: engE.linearize(m0))
+    t_jE, JE = med(lambda: linE.Jmatrix())
+    j_err = float(torch.linalg.norm(JE - J) / torch.linalg.norm(J))
+    del linE, JE, engE
     return {"cells": mesh.nC, "data": eng.nD, "sources": eng.nS, "nky": eng.nky,
             "precision": "fp64", "linearize_ms": round(t_lin, 2),
             "jmatrix_ms": round(t_j, 2), "direct_step_ms": round(t_dir, 2),
             "occam_step_ms": round(t_occ, 2), "hm_solve": hs.last,
+            "electrode_basis_linearize_ms": round(t_linE, 2),
+            "electrode_basis_jmatrix_ms": round(t_jE, 2), "electrode_basis_J_rel_err": j_err,
             "phi_d_start": float((r ** 2).sum()),
             "occam_beta": b_occ, "occam_predicted_phi_d": pred,
             "phi_d_after_occam_step": float(((w * (d1 - dobs)) ** 2).sum()),

@@ -78,9 +78,16 @@ class Linearization2D:
         eng, g = self.eng, self.eng._gpu
         self.g = g
         dev = eng.device
+        # engines built with rhs_basis="electrodes": every field from one solve
+        # with the electrode columns, G = A^{-1} E (U = G S, the adjoint field
+        # of datum j is G R_j), so J, J v and J^T v need no further solve
+        self._elec = getattr(eng, "rhs_basis", "sources") == "electrodes"
+        self._G = None
         with torch.no_grad(), torch.cuda.device(dev):
             self.vals = g.assemble(self.sig)
             self._factorize()
+            if self._elec:
+                self._G = g.solve_perm(g.E_perm, g.ne)
             U = self._solve_fields_gpu()
             self._U = U if self._store else None
             d = torch.empty(eng.nD, dtype=torch.float64, device=dev)
@@ -104,7 +111,15 @@ class Linearization2D:
         if self.g.solver.stamp != self._stamp:
             self._factorize()
 
+    def _electrode_fields(self):
+        if self.g.solver.stamp != self._stamp:   # refactorised since: solve again
+            self._factorize()
+            self._G = self.g.solve_perm(self.g.E_perm, self.g.ne)
+        return self._G
+
     def _solve_fields_gpu(self):
+        if self._elec:
+            return torch.matmul(self._electrode_fields(), self.g.S)
         self._ensure()
         return self.g.solve_perm(self.g.q_perm, self.eng.nS)
 
@@ -119,6 +134,13 @@ class Linearization2D:
             rhs = torch.empty_like(U)
             _lib.check(g.lib.dcx_spmm(g.solver._h, _lib.ptr(dvals), _lib.ptr(U), _lib.ptr(rhs),
                                       eng.nS, -1.0, _lib.DCX_PERMUTED, g.stream()), "spmm")
+            if self._elec:
+                # P_j A^{-1} rhs[:, s] = R_j^T (G^T rhs)[:, s], summed over ky
+                G = self._electrode_fields()
+                Z = (g.wq.view(-1, 1, 1) * torch.matmul(G.transpose(1, 2).contiguous(),
+                                                        rhs)).sum(0)
+                out = (g.R * Z[:, g.sidx]).sum(0)
+                return out.to(self.m_device, self.m_dtype)
             self._ensure()
             dU = g.solve_perm(rhs, eng.nS, sparse=False, shared=False)
             out = torch.empty(eng.nD, dtype=torch.float64, device=eng.device)
@@ -129,6 +151,13 @@ class Linearization2D:
         eng, g = self.eng, self.g
         with torch.no_grad(), torch.cuda.device(eng.device):
             vv = v.to(eng.device, torch.float64).reshape(-1).contiguous()
+            if self._elec:
+                Ge = self._electrode_fields()
+                T = torch.zeros(g.ne, eng.nS, dtype=torch.float64, device=eng.device)
+                T.index_add_(1, g.sidx, g.R * vv)
+                H = torch.matmul(Ge, (T @ g.S.T).T.contiguous())
+                gv = g.solver.grad_values(Ge, H, scale=g.wq, permuted=True)
+                return self._to_model(g.lmap_t(gv, self.sig))
             G = g.adjoint_rhs(vv, eng.nS, 0, eng.nS)
             U = self._fields_gpu()
             self._ensure()
@@ -144,12 +173,15 @@ class Linearization2D:
             U = self._fields_gpu()
             for a, b in self._row_blocks(sub):
                 r = b - a
-                sel = (self._pd_datum >= a) & (self._pd_datum < b)
-                E = torch.zeros(eng.nN, r, dtype=torch.float64, device=eng.device)
-                E.index_put_((self._pd_row[sel], self._pd_datum[sel] - a), self._pd_val[sel],
-                             accumulate=True)
-                self._ensure()
-                Y = g.solve_perm(E, r, sparse=True, shared=True)
+                if self._elec:  # adjoint fields of data a..b: G R[:, a:b]
+                    Y = torch.matmul(self._electrode_fields(), g.R[:, a:b].contiguous())
+                else:
+                    sel = (self._pd_datum >= a) & (self._pd_datum < b)
+                    E = torch.zeros(eng.nN, r, dtype=torch.float64, device=eng.device)
+                    E.index_put_((self._pd_row[sel], self._pd_datum[sel] - a),
+                                 self._pd_val[sel], accumulate=True)
+                    self._ensure()
+                    Y = g.solve_perm(E, r, sparse=True, shared=True)
                 src = self._src32[a:b].contiguous()
                 Jt = torch.empty(eng.nC, r, dtype=torch.float64, device=eng.device)
                 _lib.check(g.lib.dcx_jacobian_rows(

@@ -265,14 +265,18 @@ def test_engine_jtvec_jmatrix_chunks():
     assert rel(eng.Jvec(mt, torch.tensor(u, device=DEV)).cpu().numpy(), J @ u) < 1e-9
 
 
-@pytest.mark.parametrize("precision,tol", [("fp64", 1e-9), ("mixed", 1e-5)])
-def test_linearization_products(precision, tol):
+@pytest.mark.parametrize("precision,tol,basis", [("fp64", 1e-9, "sources"),
+                                                ("mixed", 1e-5, "sources"),
+                                                ("fp64", 1e-9, "electrodes"),
+                                                ("mixed", 1e-5, "electrodes")])
+def test_linearization_products(precision, tol, basis):
     """Linearization2D on the GPU (dcx_spmm, dcx_jacobian_rows): J v, J^T v,
     diag(J^T D J) and the explicit J against the oracle's Jacobian; the
-    adjoint identity u.(J^T v) = (J u).v."""
+    adjoint identity u.(J^T v) = (J u).v.  basis "electrodes": every field
+    from one solve with the electrode columns (no adjoint solves)."""
     from simpeg_dc_torch_amd import Linearization2D
-    _say(f"linearization suite {precision}")
-    eng, O, m = _engine("suite", nky=4, precision=precision, refine=0)
+    _say(f"linearization suite {precision} {basis}")
+    eng, O, m = _engine("suite", nky=4, precision=precision, refine=0, rhs_basis=basis)
     mt = torch.tensor(m, device=DEV)
     lin = Linearization2D(eng, mt)
     J = O.Jmatrix(m)
