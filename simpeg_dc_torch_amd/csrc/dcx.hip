@@ -2267,12 +2267,9 @@ int factor_level(dcx_plan* P, int d, hipStream_t st) {
   // levels with few workgroups (the top of the tree) take more wavefronts per
   // front for the trailing update
   static const int wide_below = env_int("DCX_FACTOR_WIDE_BELOW", 321);
-  static const int wide_nt = env_int("DCX_FACTOR_WIDE_NT", 1024);
   const bool wide = static_cast<int64_t>(cnt) * P->nbatch < wide_below;
   if (maxld <= 256 && !wide)
     factor_launch<T, 256, 1>(P, d, cnt, lds, st);
-  else if (maxld <= 512 && wide && wide_nt == 512)
-    factor_launch<T, 512, 1>(P, d, cnt, lds, st);
   else
     factor_launch<T, 1024, 1>(P, d, cnt, lds, st);
   return DCX_OK;

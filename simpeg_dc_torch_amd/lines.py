@@ -112,15 +112,19 @@ class LineBatch:
         return self.streams[i % len(self.streams)]
 
     def capture(self, ms, warmup=2):
-        """Record each line's eval+grad at models `ms` as a graph (after
-        `warmup` eager runs on its stream, which also sizes every buffer the
-        library keeps, so nothing is allocated during capture).
-
-        Switched off: raises NotImplementedError (a replay of mixed-precision
-        graphs faulted on the host inside hipGraphLaunch; cause not found)."""
+        """Switched off: raises NotImplementedError.  A replay of the
+        mixed-precision graphs that `_capture_graphs` records ended the process
+        with a host segmentation fault inside hipGraphLaunch (FP64 replays
+        matched the eager results bit for bit); the cause is not found."""
         raise NotImplementedError(
             "graph capture of the eval+grad is switched off: a replay of mixed-precision "
             "graphs ended in a host segmentation fault inside hipGraphLaunch (cause not found)")
+
+    def _capture_graphs(self, ms, warmup=2):
+        """Record each line's eval+grad at models `ms` as a graph (after
+        `warmup` eager runs on its stream, which also sizes every buffer the
+        library keeps, so nothing is allocated during capture).  Not called:
+        see `capture`."""
         if self._data is None:
             raise RuntimeError("set_data first")
         graphs = []
