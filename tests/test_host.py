@@ -329,3 +329,29 @@ def test_3d_oracle_gradient_and_jtvec_vs_fd():
         assert abs(fd - g[c]) <= 1e-6 * max(abs(fd), 1e-3 * np.abs(g).max())
         fdj = float(v @ (O.dpred(m + e) - O.dpred(m - e))) / (2 * eps)
         assert abs(fdj - jtv[c]) <= 1e-6 * max(abs(fdj), 1e-3 * np.abs(jtv).max())
+
+
+def test_active_cells_restrict_the_full_model_cpu():
+    """Inactive cells held at val_inactive: dpred, gradient and J of the
+    active-cell engine equal the full-mesh engine's on the active columns
+    (the reference's test_jmatrix.py:80-97 and test_linearization.py:83-88)."""
+    from simpeg_dc_torch_amd import TorchDC2D
+    mesh, survey, m = case("suite")
+    act = mesh.cell_centers[:, 1] < -1.0
+    val = float(np.log(100.0))
+    m_full = m.copy()
+    m_full[~act] = val
+    eng_f = TorchDC2D(mesh, survey, nky=6, device="cpu")
+    eng_a = TorchDC2D(mesh, survey, nky=6, device="cpu", active_cells=act, val_inactive=val)
+    mf, ma = torch.tensor(m_full), torch.tensor(m_full[act])
+    with torch.no_grad():
+        d_f, d_a = eng_f.dpred(mf), eng_a.dpred(ma)
+    assert rel(d_a.numpy(), d_f.numpy()) < 1e-13
+    dobs, w = noisy_data(d_f.numpy())
+    _, g_f = eng_f.misfit_and_grad(mf, torch.tensor(dobs), torch.tensor(w))
+    _, g_a = eng_a.misfit_and_grad(ma, torch.tensor(dobs), torch.tensor(w))
+    assert g_a.shape == (int(act.sum()),)
+    assert rel(g_a.numpy(), g_f.numpy()[act]) < 1e-12
+    Jf, Ja = eng_f.Jmatrix(mf).numpy(), eng_a.Jmatrix(ma).numpy()
+    assert Ja.shape == (eng_a.nD, int(act.sum()))
+    assert rel(Ja, Jf[:, act]) < 1e-13
