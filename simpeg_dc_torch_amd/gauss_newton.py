@@ -18,6 +18,8 @@ directly.  Each mirrors the reference routine named beside it:
 * `cg_step`        CG on (J^T D J + beta H_m) p = -g with the Jacobi
                    preconditioner from the exact diagonal (optimization.py:
                    405-439 and the CG branch of `findSearchDirection`);
+* `beta_by_eig`    the initial beta from the two Hessians' largest
+                   eigenvalues (SimPEG's BetaEstimate_ByEig);
 * `gauss_newton_step`  one step of the L2 misfit at a given beta through an
                    engine's `linearize` (or `Jmatrix`), `Linearization2D`
                    supplying J, J v, J^T v and diag(J^T D J).
@@ -232,6 +234,36 @@ def cg_step(Jv, Jtv, D, g, Hm_apply, Hm_diag, beta, jtdj_diag, maxiter=40, rtol=
     else:
         Minv = torch.ones_like(g)
     return pcg(H, -g, Minv, maxiter, rtol)
+
+
+def beta_by_eig(Jv, Jtv, w, Hm, n_model, ratio=100.0, n_iter=4, seed=0, device="cpu"):
+    """Initial trade-off beta0 = ratio * lam_max(J^T D J) / lam_max(H_m),
+    D = 2 W^2, both largest eigenvalues by `n_iter` power iterations from a
+    seeded random vector (SimPEG's BetaEstimate_ByEig, which the reference's
+    inversions use with beta0_ratio 10-100)."""
+    dev = torch.device(device)
+    H = sp.csr_matrix(Hm)
+    Ht = torch.sparse_csr_tensor(torch.as_tensor(H.indptr, dtype=torch.int64, device=dev),
+                                 torch.as_tensor(H.indices, dtype=torch.int64, device=dev),
+                                 torch.as_tensor(H.data, dtype=torch.float64, device=dev),
+                                 H.shape)
+    w = torch.as_tensor(w, dtype=torch.float64, device=dev)
+    D = 2.0 * w ** 2
+    g = torch.Generator(device="cpu").manual_seed(seed)
+
+    def lam(apply):
+        x = torch.randn(n_model, generator=g, dtype=torch.float64).to(dev)
+        x /= torch.linalg.norm(x)
+        val = 0.0
+        for _ in range(n_iter):
+            y = apply(x)
+            val = float(x @ y)
+            x = y / torch.linalg.norm(y)
+        return val
+
+    lam_d = lam(lambda x: Jtv(D * Jv(x)))
+    lam_m = lam(lambda x: Ht @ x)
+    return ratio * lam_d / lam_m
 
 
 # ---------------------------------------------------------------------------

@@ -56,7 +56,7 @@ def lbfgs_minimize(eng, m0, dobs, w, Hm=None, beta=0.0, m_ref=None, max_iter=30,
             phi_m = 0.5 * (dm @ (Ht @ dm))
             loss = loss + beta * phi_m
         loss.backward()
-        history.append((float(phi_d), float(phi_m)))
+        history.append((float(phi_d.detach()), float(phi_m.detach())))
         return loss
 
     opt.step(closure)

@@ -212,3 +212,22 @@ def test_lbfgs_3d_cpu_decreases_objective():
     with torch.no_grad():
         d1 = eng.dpred(m1).numpy()
     assert float(((w * (d1 - dobs)) ** 2).sum()) < hist[0][0]
+
+
+def test_beta_by_eig_matches_dense_eigenvalues():
+    """With enough power iterations the estimate is ratio * lam_max(J^T D J)
+    / lam_max(H_m) from numpy's dense eigensolver (D = 2 W^2)."""
+    from simpeg_dc_torch_amd.gauss_newton import beta_by_eig
+    J, D, _, _ = _problem()
+    # an H_m with a well separated largest eigenvalue, so the power iteration
+    # converges to 1e-6 in a few hundred steps
+    hd = np.random.default_rng(3).uniform(1.0, 2.0, J.shape[1])
+    hd[7] = 4.0
+    H = sp.diags(hd, format="csr")
+    w = np.sqrt(D / 2.0)
+    Jt = torch.tensor(J)
+    beta = beta_by_eig(lambda v: Jt @ v, lambda u: Jt.T @ u, w, H, J.shape[1], ratio=100.0,
+                       n_iter=400)
+    lam_d = np.linalg.eigvalsh(J.T @ np.diag(D) @ J).max()
+    lam_m = np.linalg.eigvalsh(H.toarray()).max()
+    assert beta == pytest.approx(100.0 * lam_d / lam_m, rel=1e-6)
