@@ -326,9 +326,12 @@ def lines_times(steps, warmup, autotune=True):
     """The uneven batch: 16 independent 2.5-D lines of 3k-40k cells
     (log-uniform, seed 5; 21-96 electrodes, dipole-dipole, 12 wavenumbers
     requested), mixed precision.  One step = eval+grad of every line.  Timed
-    twice: the lines one after another on one stream, and concurrently on 4
-    default-priority streams with the leaf size autotuned per size class
-    (tuning outside the timed region)."""
+    four times: the lines one after another on one stream; concurrently on 4
+    default-priority streams, issued from one host thread; on the 4 streams
+    issued from 4 host threads (leaf size autotuned per size class, tuning
+    outside the timed region); and all lines in one plan (`LineForest`: one
+    launch sequence per step).  The one-plan results are compared with the
+    per-line engines' (both mixed precision)."""
     from simpeg_dc_torch_amd import workloads
     from simpeg_dc_torch_amd.lines import LineBatch
     L = workloads.uneven_lines_case()
@@ -346,13 +349,15 @@ def lines_times(steps, warmup, autotune=True):
     tune = [(m.cpu().numpy(), d.cpu().numpy(), w.cpu().numpy()) for m, (d, w) in zip(ms, data)]
     t0 = time.perf_counter()
     lb = LineBatch(lines, precision="mixed", streams=4, leaf="auto" if autotune else None,
-                   tune_data=tune if autotune else None)
+                   tune_data=tune if autotune else None, threads=True)
+    pool = lb._pool
     t_tune = time.perf_counter() - t0
     lb.set_data([d for d, _ in data], [w for _, w in data])
 
-    def timed(nstreams):
+    def timed(nstreams, threads=False):
         saved = lb.streams
         lb.streams = saved[:nstreams]
+        lb._pool = pool if threads else None
         for _ in range(warmup):
             lb.eval_grad(ms)
         torch.cuda.synchronize()
@@ -363,12 +368,41 @@ def lines_times(steps, warmup, autotune=True):
         lb.streams = saved
         return 1e3 * (time.perf_counter() - t1) / steps
 
-    ms_seq, ms_par = timed(1), timed(4)
+    ms_seq, ms_par, ms_thr = timed(1), timed(4), timed(4, threads=True)
+
+    # every line in one plan (a forest of the lines' separator trees)
+    from simpeg_dc_torch_amd.lines import LineForest
+    t0 = time.perf_counter()
+    lf = LineForest(lines, precision="mixed")
+    t_forest = time.perf_counter() - t0
+    lf.set_data([d for d, _ in data], [w for _, w in data])
+    for _ in range(warmup):
+        lf.eval_grad(ms)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(steps):
+        lf.eval_grad(ms)
+    torch.cuda.synchronize()
+    ms_forest = 1e3 * (time.perf_counter() - t1) / steps
+    out_f, out_b = lf.eval_grad(ms), lb.eval_grad(ms)
+    torch.cuda.synchronize()
+    dphi = max(abs(float(a[0]) / float(b[0]) - 1.0) for a, b in zip(out_f, out_b))
+    dgrad = max(float(torch.linalg.norm(a[1] - b[1]) / torch.linalg.norm(b[1]))
+                for a, b in zip(out_f, out_b))
+    info = lf.solver.info()
+    lf.free()
     cells = sum(mesh.nC for mesh, _ in lines)
     return {"lines": len(lines), "cells_total": cells,
             "cells_range": [min(mesh.nC for mesh, _ in lines), max(mesh.nC for mesh, _ in lines)],
             "precision": "mixed", "sequential_ms": round(ms_seq, 2),
-            "streams4_ms": round(ms_par, 2), "lines_per_s": round(1e3 * len(lines) / ms_par, 1),
+            "streams4_ms": round(ms_par, 2), "streams4_threads4_ms": round(ms_thr, 2),
+            "one_plan_ms": round(ms_forest, 2),
+            "lines_per_s": round(1e3 * len(lines) / min(ms_par, ms_thr, ms_forest), 1),
+            "one_plan": {"source_columns": lf.K, "systems": lf.nbatch,
+                         "level_launches": info.get("level_launches"),
+                         "build_s": round(t_forest, 1),
+                         "phi_max_rel_diff_vs_per_line": float(f"{dphi:.2e}"),
+                         "grad_max_rel_diff_vs_per_line": float(f"{dgrad:.2e}")},
             "leaf_by_size_class": {str(k): v for k, v in (lb.leaf or {}).items()},
             "build_and_tune_s": round(t_tune, 1), "graph_capture": "off"}
 

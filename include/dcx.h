@@ -185,6 +185,15 @@ int dcx_lmap_create(int nnz, int ncell, int nky, const double* ky2,
                     const int32_t* bc_cell, const double* bc_coef,
                     dcx_lmap** out);
 int dcx_lmap_destroy(dcx_lmap* map);
+/* Wavenumber groups: entry e of the values takes ky^2 = ky2[k * ngroup +
+ * group[e]] in system k (host arrays: group has nnz entries, ky2 nky x
+ * ngroup), for a map that stacks independent problems with their own
+ * wavenumbers (the uneven batch of lines in one plan).  No counterpart in the
+ * reference, whose engines hold one problem each (dctorch/simulation2d.py:
+ * 249-260 builds one ky list per engine).  Maps with groups are refused by
+ * dcx_jacobian_rows. */
+int dcx_lmap_set_groups(dcx_lmap* map, int ngroup, const int32_t* group,
+                        const double* ky2);
 /* values (device, nky x nnz) from sigma (device, ncell) */
 int dcx_assemble(dcx_lmap* map, const double* sigma, double* values, void* stream);
 /* grad_m[c] = -sigma[c] * (L^T gv)[c]: the transposed map applied to the value

@@ -76,6 +76,7 @@ PROTOTYPES = {
     "dcx_lmap_create": (C.c_int, [C.c_int, C.c_int, C.c_int, _f64p, _i32p, _i32p, _f64p,
                                   _f64p, C.c_int, _i32p, _i32p, _f64p, C.POINTER(_vp)]),
     "dcx_lmap_destroy": (C.c_int, [_vp]),
+    "dcx_lmap_set_groups": (C.c_int, [_vp, C.c_int, _i32p, _f64p]),
     "dcx_assemble": (C.c_int, [_vp, _vp, _vp, _vp]),
     "dcx_lmap_transpose": (C.c_int, [_vp, _vp, _vp, _vp, _vp]),
     "dcx_survey_create": (C.c_int, [_vp, C.c_int, C.c_int, _i32p, _i32p, _f64p, _i32p,

@@ -1845,29 +1845,36 @@ __global__ void k_jmap(int ncell, int r, const int* __restrict__ tptr,
   }
 }
 
-// gv[b][e] = -scale[b] sum_c Y[b][row][c] X[b][col_e][c]; one wavefront per
-// row of the given pattern: the caller's, or, when the fields are stored in
-// elimination order, the permuted one with oidx mapping its entries back to
-// the caller's order (rows then run in elimination order: consecutive
-// wavefronts read consecutive rows of Y, and a row's neighbours mostly lie in
-// its own supernode).  The hardware
-// deals workgroups round-robin to the 8 XCDs; the block index is remapped so
-// that each XCD walks one contiguous range of rows (its own L2).  gridDim.x
-// must be a multiple of 8.  Up to MAXE entries of a row at a time; each lane loads
-// 4 columns of the Y row and of every X row before it multiplies, so that
-// 4 (1 + MAXE) loads per lane are in flight; FP64 sums.
-template <typename TI, int MAXE>
+// gv[b][e] = -scale[b] sum_c Y[b][row][c] X[b][col_e][c]; LPR lanes per row
+// (64: one wavefront per row; 32 or 16 for rows of at most 128 or 64
+// columns, so that one wavefront serves 64 / LPR rows through the same chain
+// of dependent loads: row pointers, columns, fields, output index) of the given
+// pattern: the caller's, or, when the fields are stored in elimination order,
+// the permuted one with oidx mapping its entries back to the caller's order
+// (rows then run in elimination order: consecutive wavefronts read
+// consecutive rows of Y, and a row's neighbours mostly lie in its own
+// supernode).  The hardware deals workgroups round-robin to the 8 XCDs; the
+// block index is remapped so that each XCD walks one contiguous range of rows
+// (its own L2).  gridDim.x must be a multiple of 8.  Up to MAXE entries of a
+// row at a time; each lane loads 4 columns of the Y row and of every X row
+// before it multiplies, so that 4 (1 + MAXE) loads per lane are in flight,
+// the last partial block of columns included; FP64 sums.
+template <typename TI, int MAXE, int LPR>
 __global__ void k_grad_values(const int* __restrict__ crow, const int* __restrict__ ccol,
                               const int* __restrict__ oidx, const TI* __restrict__ Y,
                               const TI* __restrict__ X, int n, int k,
                               const double* __restrict__ scale, double* __restrict__ gv,
                               int64_t nnz) {
   constexpr int U = 4;
+  constexpr int RPW = 64 / LPR;  // rows per wavefront
   const int per = gridDim.x >> 3;
   const int lb = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
-  const int row = lb * (blockDim.x / 64) + threadIdx.x / 64;
-  const int b = blockIdx.y;
   const int lane = threadIdx.x & 63;
+  const int sl = lane % LPR;  // lane within the row's segment
+  const int row = (lb * (blockDim.x / 64) + threadIdx.x / 64) * RPW + lane / LPR;
+  const int b = blockIdx.y;
+  // a row's segment is wholly active or wholly gone, and the reductions
+  // below stay inside one segment
   if (row >= n) return;
   const int64_t xb = static_cast<int64_t>(b) * n * k;
   const TI* y = Y + xb + static_cast<int64_t>(row) * k;
@@ -1883,31 +1890,49 @@ __global__ void k_grad_values(const int* __restrict__ crow, const int* __restric
     double acc[MAXE];
 #pragma unroll
     for (int q = 0; q < MAXE; ++q) acc[q] = 0.0;
-    int c = lane;
-    for (; c + 64 * (U - 1) < k; c += 64 * U) {
+    int c = sl;
+    for (; c + LPR * (U - 1) < k; c += LPR * U) {
       TI yv[U], xv[MAXE][U];
 #pragma unroll
-      for (int u = 0; u < U; ++u) yv[u] = y[c + 64 * u];
+      for (int u = 0; u < U; ++u) yv[u] = y[c + LPR * u];
 #pragma unroll
       for (int q = 0; q < MAXE; ++q)
 #pragma unroll
-        for (int u = 0; u < U; ++u) xv[q][u] = xr[q][c + 64 * u];
+        for (int u = 0; u < U; ++u) xv[q][u] = xr[q][c + LPR * u];
 #pragma unroll
       for (int q = 0; q < MAXE; ++q)
 #pragma unroll
         for (int u = 0; u < U; ++u)
           acc[q] += static_cast<double>(yv[u]) * static_cast<double>(xv[q][u]);
     }
-    for (; c < k; c += 64) {
-      const double yv = static_cast<double>(y[c]);
+    if (c < k) {  // the remaining < LPR U columns: one block, every load issued
+      // before the first use; columns past k read column k - 1 (in bounds,
+      // same cache line) and count zero, so no load sits behind a branch
+      TI yv[U], xv[MAXE][U];
+      int cu[U];
 #pragma unroll
-      for (int q = 0; q < MAXE; ++q) acc[q] += yv * static_cast<double>(xr[q][c]);
+      for (int u = 0; u < U; ++u) cu[u] = min(c + LPR * u, k - 1);
+#pragma unroll
+      for (int u = 0; u < U; ++u) yv[u] = y[cu[u]];
+#pragma unroll
+      for (int q = 0; q < MAXE; ++q)
+#pragma unroll
+        for (int u = 0; u < U; ++u) xv[q][u] = xr[q][cu[u]];
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (c + LPR * u >= k) yv[u] = TI(0);
+#pragma unroll
+      for (int q = 0; q < MAXE; ++q)
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+          acc[q] += static_cast<double>(yv[u]) * static_cast<double>(xv[q][u]);
     }
 #pragma unroll
     for (int q = 0; q < MAXE; ++q) {
       double a = acc[q];
-      for (int off = 32; off > 0; off >>= 1) a += __shfl_xor(a, off, 64);
-      if (lane == 0 && q < ne) gv[b * nnz + (oidx ? oidx[p0 + q] : p0 + q)] = -sc * a;
+#pragma unroll
+      for (int off = LPR / 2; off > 0; off >>= 1) a += __shfl_xor(a, off, 64);
+      if (sl == 0 && q < ne) gv[b * nnz + (oidx ? oidx[p0 + q] : p0 + q)] = -sc * a;
     }
   }
 }
@@ -1915,7 +1940,10 @@ __global__ void k_grad_values(const int* __restrict__ crow, const int* __restric
 // ---------------------------------------------------------------------------
 // assembly map
 // ---------------------------------------------------------------------------
+// ky2 holds nky x ngroup squared wavenumbers, entry e taking column grp[e]
+// (grp null: one group)
 __global__ void k_assemble(int nnz, int nky, const double* __restrict__ ky2,
+                           const int* __restrict__ grp, int ngroup,
                            const int* __restrict__ ptr, const int* __restrict__ cell,
                            const double* __restrict__ le, const double* __restrict__ ln,
                            const int* __restrict__ bptr, const int* __restrict__ bcell,
@@ -1930,8 +1958,9 @@ __global__ void k_assemble(int nnz, int nky, const double* __restrict__ ky2,
     an += ln[p] * s;
   }
   const int q0 = bptr[e], q1 = bptr[e + 1];
+  const int g = grp ? grp[e] : 0;
   for (int k = 0; k < nky; ++k) {
-    double v = ae + ky2[k] * an;
+    double v = ae + ky2[k * ngroup + g] * an;
     for (int q = q0; q < q1; ++q) v += bcoef[static_cast<int64_t>(k) * nbc + q] * sigma[bcell[q]];
     vals[static_cast<int64_t>(k) * nnz + e] = v;
   }
@@ -1939,15 +1968,17 @@ __global__ void k_assemble(int nnz, int nky, const double* __restrict__ ky2,
 
 // per entry: ge = sum_k gv[k][e], gn = sum_k ky_k^2 gv[k][e]
 __global__ void k_reduce_ky(int nnz, int nky, const double* __restrict__ ky2,
+                            const int* __restrict__ grp, int ngroup,
                             const double* __restrict__ gv, double* __restrict__ ge,
                             double* __restrict__ gn) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= nnz) return;
+  const int gr = grp ? grp[e] : 0;
   double a = 0.0, c = 0.0;
   for (int k = 0; k < nky; ++k) {
     const double g = gv[static_cast<int64_t>(k) * nnz + e];
     a += g;
-    c += ky2[k] * g;
+    c += ky2[k * ngroup + gr] * g;
   }
   ge[e] = a;
   gn[e] = c;
@@ -3036,7 +3067,11 @@ int dcx_grad_values(dcx_plan* P, const void* y, const void* x, int k,
   hipStream_t st = static_cast<hipStream_t>(stream);
   HIP_TRY(hipSetDevice(P->device));
   const auto& S = P->S;
-  const int blocks = ((grid1(S.n, 4) + 7) / 8) * 8;  // XCD remap needs a multiple of 8
+  // lanes per row: whole wavefronts for long rows, 32 or 16 lanes (2 or 4
+  // rows per wavefront) when 4 columns per lane cover the row
+  const int lpr = k <= 64 ? 16 : k <= 128 ? 32 : 64;
+  const int rows_per_block = 4 * (64 / lpr);  // 256 threads
+  const int blocks = ((grid1(S.n, rows_per_block) + 7) / 8) * 8;  // XCD remap: multiple of 8
   dim3 grid(blocks, P->nbatch);
   // permuted fields: rows walked in elimination order through the permuted
   // pattern (consecutive waves read consecutive rows of y, and the neighbours
@@ -3050,10 +3085,15 @@ int dcx_grad_values(dcx_plan* P, const void* y, const void* x, int k,
   const bool wide = P->max_row_nnz > 5;
   auto go = [&](auto tag, auto me) {
     using TI = decltype(tag);
-    hipLaunchKernelGGL((k_grad_values<TI, decltype(me)::value>), grid, dim3(256), 0, st, rp,
-                       cl, oi, static_cast<const TI*>(y),
-                       static_cast<const TI*>(x), S.n, k, scale, gv,
-                       static_cast<int64_t>(S.nnz));
+    constexpr int ME = decltype(me)::value;
+    auto launch = [&](auto kern) {
+      hipLaunchKernelGGL(kern, grid, dim3(256), 0, st, rp, cl, oi, static_cast<const TI*>(y),
+                         static_cast<const TI*>(x), S.n, k, scale, gv,
+                         static_cast<int64_t>(S.nnz));
+    };
+    if (lpr == 16) launch(k_grad_values<TI, ME, 16>);
+    else if (lpr == 32) launch(k_grad_values<TI, ME, 32>);
+    else launch(k_grad_values<TI, ME, 64>);
   };
   using E5 = std::integral_constant<int, 5>;
   using E8 = std::integral_constant<int, 8>;
@@ -3121,7 +3161,9 @@ int dcx_permute_rows(dcx_plan* P, const double* in, double* out, int k, int nblo
 // ===========================================================================
 struct dcx_lmap {
   int nnz = 0, ncell = 0, nky = 0, nbc = 0, device = 0;
-  double* d_ky2 = nullptr;
+  int ngroup = 1;           // wavenumber groups (dcx_lmap_set_groups)
+  double* d_ky2 = nullptr;  // nky x ngroup
+  int* d_grp = nullptr;     // group of each value entry, null for one group
   int *d_ptr = nullptr, *d_cell = nullptr;
   double *d_le = nullptr, *d_ln = nullptr;
   int *d_bptr = nullptr, *d_bcell = nullptr, *d_bc_e = nullptr;
@@ -3132,7 +3174,7 @@ struct dcx_lmap {
   double* d_jws = nullptr;  // Jacobian-rows workspace, grown on demand
   size_t jws_bytes = 0;
   ~dcx_lmap() {
-    void* ptrs[] = {d_ky2, d_ptr, d_cell, d_le, d_ln, d_bptr, d_bcell, d_bc_e, d_bcoef,
+    void* ptrs[] = {d_ky2, d_grp, d_ptr, d_cell, d_le, d_ln, d_bptr, d_bcell, d_bc_e, d_bcoef,
                     d_tptr, d_tent, d_te, d_tbptr, d_tbq, d_ge, d_gn, d_jws};
     for (void* p : ptrs)
       if (p) (void)hipFree(p);
@@ -3231,12 +3273,33 @@ int dcx_lmap_destroy(dcx_lmap* m) {
   return DCX_OK;
 }
 
+int dcx_lmap_set_groups(dcx_lmap* M, int ngroup, const int32_t* group, const double* ky2) {
+  try {
+    if (!M || !group || !ky2 || ngroup < 1) return fail(DCX_ERR_ARG, "bad arguments");
+    for (int e = 0; e < M->nnz; ++e)
+      if (group[e] < 0 || group[e] >= ngroup) return fail(DCX_ERR_ARG, "group out of range");
+    HIP_TRY(hipSetDevice(M->device));
+    HIP_TRY(hipDeviceSynchronize());  // earlier launches may still read the old table
+    if (M->d_ky2) HIP_TRY(hipFree(M->d_ky2));
+    if (M->d_grp) HIP_TRY(hipFree(M->d_grp));
+    M->d_ky2 = nullptr;
+    M->d_grp = nullptr;
+    HIP_TRY(upload(&M->d_ky2,
+                   std::vector<double>(ky2, ky2 + static_cast<size_t>(M->nky) * ngroup)));
+    HIP_TRY(upload(&M->d_grp, std::vector<int>(group, group + M->nnz)));
+    M->ngroup = ngroup;
+    return DCX_OK;
+  } catch (const std::exception& e) {
+    return fail(DCX_ERR_INTERNAL, e.what());
+  }
+}
+
 int dcx_assemble(dcx_lmap* M, const double* sigma, double* values, void* stream) {
   if (!M || !sigma || !values) return fail(DCX_ERR_ARG, "null argument");
   hipStream_t st = static_cast<hipStream_t>(stream);
   HIP_TRY(hipSetDevice(M->device));
   hipLaunchKernelGGL(k_assemble, dim3(grid1(M->nnz, 256)), dim3(256), 0, st, M->nnz, M->nky,
-                     M->d_ky2, M->d_ptr, M->d_cell, M->d_le, M->d_ln, M->d_bptr, M->d_bcell,
+                     M->d_ky2, M->d_grp, M->ngroup, M->d_ptr, M->d_cell, M->d_le, M->d_ln, M->d_bptr, M->d_bcell,
                      M->d_bcoef, M->nbc, sigma, values);
   HIP_TRY(hipGetLastError());
   return DCX_OK;
@@ -3248,7 +3311,7 @@ int dcx_lmap_transpose(dcx_lmap* M, const double* gv, const double* sigma, doubl
   hipStream_t st = static_cast<hipStream_t>(stream);
   HIP_TRY(hipSetDevice(M->device));
   hipLaunchKernelGGL(k_reduce_ky, dim3(grid1(M->nnz, 256)), dim3(256), 0, st, M->nnz, M->nky,
-                     M->d_ky2, gv, M->d_ge, M->d_gn);
+                     M->d_ky2, M->d_grp, M->ngroup, gv, M->d_ge, M->d_gn);
   hipLaunchKernelGGL(k_lmap_t, dim3(grid1(M->ncell, 256)), dim3(256), 0, st, M->ncell, M->nnz,
                      M->nky, M->d_tptr, M->d_tent, M->d_te, M->d_le, M->d_ln, M->d_tbptr,
                      M->d_tbq, M->d_bc_e, M->d_bcoef, M->nbc, M->d_ge, M->d_gn, gv, sigma,
@@ -3267,6 +3330,8 @@ int dcx_jacobian_rows(dcx_lmap* M, dcx_plan* P, const double* Y, int r, const do
     return fail(DCX_ERR_ARG, "map and plan do not describe the same systems");
   if (M->max_bc_per_entry > kMaxBcPerEntry)
     return fail(DCX_ERR_ARG, "more boundary terms per entry than k_jpair holds");
+  if (M->ngroup != 1)
+    return fail(DCX_ERR_ARG, "Jacobian rows of a map with wavenumber groups are not supported");
   hipStream_t st = static_cast<hipStream_t>(stream);
   HIP_TRY(hipSetDevice(M->device));
   const size_t need = (2 * static_cast<size_t>(M->nnz) + std::max(M->nbc, 1)) * r * sizeof(double);

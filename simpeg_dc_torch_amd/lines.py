@@ -8,6 +8,11 @@ launch economy:
 * streams: every line's eval+grad is issued on one of `streams`
   default-priority streams, so the small lines' level launches overlap and
   fill the chip together instead of running one line after another;
+* host threads: with `threads=True` one host thread per stream issues its
+  lines (`TorchDC2D.eval_grad_device`: no autograd graph, no host
+  synchronisation), so the launch work of several lines proceeds in
+  parallel on the host as well: the library's entry points are C calls,
+  which run without the interpreter lock;
 * graphs: `capture()` would record each line's whole eval+grad as a HIP
   graph through `torch.cuda.CUDAGraph` and replay it; it is switched off and
   raises: replays of FP64 graphs matched the eager results bit for bit, but
@@ -22,9 +27,12 @@ The nearest reference analogue is cuDSS's batched sequence (dctorch/
 solver.py:181-185), which needs one pattern; there is no uneven-batch code in
 the reference.
 """
+import ctypes as C
 import time
+from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
+import scipy.sparse as sp
 import torch
 
 from .simulation2d import TorchDC2D
@@ -75,7 +83,7 @@ class LineBatch:
     [(m, dobs, w) per line] to run `autotune_leaf` on one line per class."""
 
     def __init__(self, lines, device="cuda", nky=12, precision="mixed", refine=0, streams=4,
-                 leaf=None, tune_data=None):
+                 leaf=None, tune_data=None, threads=False):
         self.device = torch.device(device)
         if self.device.type == "cuda" and self.device.index is None:
             self.device = torch.device("cuda", torch.cuda.current_device())
@@ -98,6 +106,7 @@ class LineBatch:
             self.engines.append(TorchDC2D(mesh, survey, nky=nky, device=self.device,
                                           precision=precision, refine=refine, leaf=lf))
         self.streams = [torch.cuda.Stream(self.device) for _ in range(max(1, streams))]
+        self._pool = ThreadPoolExecutor(max_workers=len(self.streams)) if threads else None
         self._data = None
         self._graphs = None
 
@@ -147,35 +156,293 @@ class LineBatch:
         torch.cuda.synchronize(self.device)
         self._graphs = graphs
 
+    def _issue(self, t, ms, out):
+        """The lines of stream t, one after another, on that stream."""
+        s = self.streams[t]
+        with torch.cuda.device(self.device), torch.cuda.stream(s):
+            for i in range(t, len(self.engines), len(self.streams)):
+                self._line(i, ms, out)
+
+    def _line(self, i, ms, out):
+        x = torch.as_tensor(ms[i], dtype=torch.float64, device=self.device)
+        dobs, w = self._data[i]
+        out[i] = self.engines[i].eval_grad_device(x, dobs, w)
+
     def eval_grad(self, ms):
-        """[(phi, grad)] for the models `ms` (one per line), every line on its
-        stream; graph replays when captured.  The results are ordered after
-        the caller's stream: phi and grad of a captured line are its graph's
-        output buffers, overwritten by the next call."""
+        """[(phi, grad)] for the models `ms` (one per line), line i on stream
+        i mod streams, issued round-robin from this thread, or from one host
+        thread per stream with `threads=True`; graph replays when captured.
+        The results are ordered after the caller's stream: phi and grad of a
+        captured line are its graph's output buffers, overwritten by the next
+        call."""
         if self._data is None:
             raise RuntimeError("set_data first")
         cur = torch.cuda.current_stream(self.device)
         for s in self.streams:
             s.wait_stream(cur)
-        out = []
-        for i, eng in enumerate(self.engines):
-            s = self._stream(i)
-            with torch.cuda.stream(s):
-                if self._graphs is not None:
-                    g, x, phi, grad = self._graphs[i]
+        if self._graphs is not None:
+            out = []
+            for i, (g, x, phi, grad) in enumerate(self._graphs):
+                with torch.cuda.stream(self._stream(i)):
                     with torch.no_grad():
                         x.copy_(torch.as_tensor(ms[i], dtype=torch.float64, device=self.device))
                     g.replay()
-                    out.append((phi, grad))
-                else:
-                    x = torch.as_tensor(ms[i], dtype=torch.float64, device=self.device)
-                    x = x.detach().clone().requires_grad_(True)
-                    dobs, w = self._data[i]
-                    phi, grad = _eval_grad(self.engines[i], x, dobs, w)
-                    phi, grad = phi.detach(), grad
-                    phi.record_stream(cur)
-                    grad.record_stream(cur)
-                    out.append((phi, grad))
+                out.append((phi, grad))
+        else:
+            out = [None] * len(self.engines)
+            if self._pool is not None:
+                for f in [self._pool.submit(self._issue, t, ms, out)
+                          for t in range(len(self.streams))]:
+                    f.result()
+            else:
+                for i in range(len(self.engines)):
+                    with torch.cuda.stream(self._stream(i)):
+                        self._line(i, ms, out)
+            for phi, grad in out:
+                phi.record_stream(cur)
+                grad.record_stream(cur)
         for s in self.streams:
             cur.wait_stream(s)
         return out
+
+
+# ---------------------------------------------------------------------------
+# every line in one plan
+# ---------------------------------------------------------------------------
+def merge_line_maps(maps_list, kys_list, wq_list):
+    """Stack the operator maps of independent 2.5-D lines into one block-
+    diagonal problem (pure numpy, host).
+
+    maps_list: Problem2DMaps per line; kys_list / wq_list: each line's
+    wavenumbers and quadrature weights.  Lines with fewer wavenumbers than the
+    longest list are padded with their last wavenumber at weight 0 (a valid
+    SPD system whose fields are multiplied by 0).  Returns a dict with the
+    merged CSR pattern, assembly map (entry group = line), per-(system, line)
+    ky^2 and weights, stacked survey (Pd, sidx: line-local source column),
+    sources (q, columns 0..nS_i-1 of line i), node coordinates, electrode rows,
+    and the offsets of nodes, cells, entries and data per line."""
+    L = len(maps_list)
+    nb = max(len(k) for k in kys_list)
+    node_off = np.r_[0, np.cumsum([m.nN for m in maps_list])]
+    cell_off = np.r_[0, np.cumsum([m.nC for m in maps_list])]
+    ent_off = np.r_[0, np.cumsum([m.nnz for m in maps_list])]
+    term_off = np.r_[0, np.cumsum([m.map_ptr[-1] for m in maps_list])]
+    data_off = np.r_[0, np.cumsum([m.nD for m in maps_list])]
+    K = max(m.nS for m in maps_list)
+    ky2 = np.empty((nb, L))
+    wq = np.zeros((nb, L))
+    bc_coef = []
+    for i, (m, ky, w) in enumerate(zip(maps_list, kys_list, wq_list)):
+        ky = np.asarray(ky, float)
+        n = ky.size
+        ky2[:n, i] = ky ** 2
+        ky2[n:, i] = ky[-1] ** 2
+        wq[:n, i] = np.asarray(w, float)
+        c = np.asarray(m.bc_coef, float)
+        bc_coef.append(np.vstack([c, np.repeat(c[-1:], nb - n, axis=0)]))
+    cat = np.concatenate
+    out = dict(
+        nlines=L, nbatch=nb, K=K, n=int(node_off[-1]), nC=int(cell_off[-1]),
+        nnz=int(ent_off[-1]), nD=int(data_off[-1]),
+        node_off=node_off, cell_off=cell_off, ent_off=ent_off, data_off=data_off,
+        crow=cat([m.crow[:-1] + ent_off[i] for i, m in enumerate(maps_list)]
+                 + [[ent_off[-1]]]).astype(np.int32),
+        col=cat([m.col + node_off[i] for i, m in enumerate(maps_list)]).astype(np.int32),
+        map_ptr=cat([m.map_ptr[:-1] + term_off[i] for i, m in enumerate(maps_list)]
+                    + [[term_off[-1]]]).astype(np.int32),
+        map_cell=cat([m.map_cell + cell_off[i] for i, m in enumerate(maps_list)]).astype(np.int32),
+        map_le=cat([m.map_le for m in maps_list]),
+        map_ln=cat([m.map_ln for m in maps_list]),
+        bc_e=cat([m.bc_e + ent_off[i] for i, m in enumerate(maps_list)]).astype(np.int32),
+        bc_cell=cat([m.bc_cell + cell_off[i] for i, m in enumerate(maps_list)]).astype(np.int32),
+        bc_coef=np.hstack(bc_coef),
+        group=cat([np.full(m.nnz, i) for i, m in enumerate(maps_list)]).astype(np.int32),
+        ky2=ky2, wq=wq,
+        Pd=sp.block_diag([m.Pd for m in maps_list], format="csr"),
+        sidx=cat([m.sidx for m in maps_list]).astype(np.int64),
+        data_line=cat([np.full(m.nD, i) for i, m in enumerate(maps_list)]).astype(np.int64),
+        node_line=cat([np.full(m.nN, i) for i, m in enumerate(maps_list)]).astype(np.int64),
+        coords=np.vstack([m.node_coords() for m in maps_list]),
+        rows=cat([m.electrode_rows() + node_off[i] for i, m in enumerate(maps_list)]).astype(np.int32),
+    )
+    blocks = []
+    for i, m in enumerate(maps_list):
+        qc = sp.coo_matrix(m.q)
+        blocks.append((qc.row + node_off[i], qc.col, qc.data))
+    r = cat([b[0] for b in blocks])
+    c = cat([b[1] for b in blocks])
+    v = cat([b[2] for b in blocks])
+    out["q"] = sp.csr_matrix((v, (r, c)), shape=(out["n"], K))
+    return out
+
+
+def merged_values_host(M, sigma):
+    """Host evaluation of the merged assembly map (nbatch, nnz): entry e of
+    system k is sum_p (le + ky2[k, group e] ln) sigma + boundary terms."""
+    ent = np.repeat(np.arange(M["nnz"]), np.diff(M["map_ptr"]))
+    s = sigma[M["map_cell"]]
+    ae = np.bincount(ent, M["map_le"] * s, minlength=M["nnz"])
+    an = np.bincount(ent, M["map_ln"] * s, minlength=M["nnz"])
+    out = ae[None, :] + M["ky2"][:, M["group"]] * an[None, :]
+    for k in range(M["nbatch"]):
+        out[k] += np.bincount(M["bc_e"], M["bc_coef"][k] * sigma[M["bc_cell"]],
+                              minlength=M["nnz"])
+    return out
+
+
+class LineForest:
+    """eval+grad of many independent 2.5-D lines through ONE plan.
+
+    The lines' systems are stacked block-diagonally: the nested dissection
+    splits the disconnected pattern into its components first, so the plan's
+    elimination tree is a forest of the lines' own separator trees and every
+    level launch (factor and sweeps) serves all lines at once: one launch
+    sequence per step instead of one per line.  Each line keeps its own
+    wavenumbers (a grouped assembly map, `dcx_lmap_set_groups`) and
+    quadrature weights, which enter through per-system forward right-hand
+    sides B_k = w_(k, line) q, so the projected data and the gradient gathers
+    take unit weights:
+        U'_k = A_k^-1 B_k,  d = sum_k Pd U'_k,
+        dphi/dA_k = -Lambda_k U'_k^T with Lambda_k = A_k^-1 Pd^T v.
+    Every line is solved with K = max nS source columns (a line with fewer
+    sources carries zero columns).  Full-mesh models only (no active cells).
+
+    lines: list of (mesh, survey); nky: an int or one per line; quadrature as
+    TorchDC2D."""
+
+    def __init__(self, lines, device="cuda", nky=12, precision="mixed", refine=0, leaf=None,
+                 quadrature=None):
+        from .solver import HipBatchSolver
+        from . import _lib as lib_mod
+        self._libm = lib_mod
+        self.device = torch.device(device)
+        if self.device.type == "cuda" and self.device.index is None:
+            self.device = torch.device("cuda", torch.cuda.current_device())
+        dev = self.device
+        nkys = list(nky) if np.iterable(nky) else [int(nky)] * len(lines)
+        maps, kys, wqs = [], [], []
+        for (mesh, survey), nk in zip(lines, nkys):
+            e = TorchDC2D(mesh, survey, nky=nk, device="cpu", quadrature=quadrature)
+            maps.append(e._maps)
+            kys.append(e.kys)
+            wqs.append(e.wq_np)
+        M = self.M = merge_line_maps(maps, kys, wqs)
+        self.nlines, self.K, self.nbatch = M["nlines"], M["K"], M["nbatch"]
+        self.nS = [m.nS for m in maps]
+        self.nD = [m.nD for m in maps]
+        self.nC = [m.nC for m in maps]
+        self.precision = precision
+        self.solver = HipBatchSolver(
+            torch.tensor(M["crow"]), torch.tensor(M["col"]), M["n"], M["K"], M["nbatch"],
+            device=dev, dtype=torch.float64, precision=precision, refine=refine,
+            coords=M["coords"], leaf=leaf)
+        lib = self.lib = self.solver._lib
+        self.solver.set_rhs_support(M["rows"])
+        perm = self.solver.perm()
+        L = lib_mod
+        ky2_0, ky2_0p = L.f64(M["ky2"][:, 0])
+        ptr, ptrp = L.i32(M["map_ptr"])
+        cell, cellp = L.i32(M["map_cell"])
+        le, lep = L.f64(M["map_le"])
+        ln, lnp = L.f64(M["map_ln"])
+        be, bep = L.i32(M["bc_e"])
+        bcl, bclp = L.i32(M["bc_cell"])
+        bco, bcop = L.f64(M["bc_coef"])
+        grp, grpp = L.i32(M["group"])
+        ky2, ky2p = L.f64(M["ky2"])
+        h = C.c_void_p()
+        with torch.cuda.device(dev):
+            L.check(lib.dcx_lmap_create(M["nnz"], M["nC"], M["nbatch"], ky2_0p, ptrp, cellp, lep,
+                                        lnp, int(be.size), bep, bclp, bcop, C.byref(h)),
+                    "lmap_create")
+            self.lmap = h
+            L.check(lib.dcx_lmap_set_groups(h, M["nlines"], grpp, ky2p), "lmap_set_groups")
+        Pd = M["Pd"]
+        pp, ppp = L.i32(Pd.indptr)
+        pc, pcp = L.i32(Pd.indices)
+        pv, pvp = L.f64(Pd.data)
+        si, sip = L.i32(M["sidx"])
+        h2 = C.c_void_p()
+        with torch.cuda.device(dev):
+            L.check(lib.dcx_survey_create(self.solver._h, M["nD"], M["K"], ppp, pcp, pvp, sip,
+                                          C.byref(h2)), "survey_create")
+        self.survey = h2
+        # weighted forward right-hand sides, elimination order: (nbatch, n, K)
+        q = torch.tensor(M["q"][perm].toarray(), dtype=torch.float64, device=dev)
+        wrow = torch.tensor(M["wq"][:, M["node_line"][perm]], dtype=torch.float64, device=dev)
+        self.B = (wrow.unsqueeze(2) * q.unsqueeze(0)).contiguous()
+        del q, wrow
+        self.ones = torch.ones(M["nbatch"], dtype=torch.float64, device=dev)
+        self.data_line = torch.tensor(M["data_line"], device=dev)
+        self._data = None
+
+    def _stream(self):
+        return self._libm.stream_of(self.device)
+
+    def set_data(self, dobs, w):
+        """Observed data and weights per line (stacked on the device)."""
+        cat = lambda xs: torch.cat([torch.as_tensor(x, dtype=torch.float64).reshape(-1).to(
+            self.device) for x in xs])
+        self._data = (cat(dobs), cat(w))
+
+    def eval_grad(self, ms):
+        """[(phi_i, grad_i)] for the models `ms` (one per line, m = log rho):
+        phi_i = ||w_i (dpred_i - dobs_i)||^2 and its gradient, all lines in
+        one launch sequence on the current stream, no host synchronisation."""
+        if self._data is None:
+            raise RuntimeError("set_data first")
+        dobs, w = self._data
+        L, lib, sv, dev = self._libm, self.lib, self.solver, self.device
+        f32 = self.precision == "mixed"
+        fl32 = L.DCX_F32_FIELDS if f32 else 0
+        n, K, nb = sv.n, self.K, self.nbatch
+        st = self._stream()
+        with torch.no_grad(), torch.cuda.device(dev):
+            m = torch.cat([torch.as_tensor(x, dtype=torch.float64).reshape(-1).to(dev)
+                           for x in ms])
+            sig = torch.exp(-m)
+            vals = torch.empty(nb, self.M["nnz"], dtype=torch.float64, device=dev)
+            L.check(lib.dcx_assemble(self.lmap, L.ptr(sig), L.ptr(vals), st), "assemble")
+            U = torch.empty(nb, n, K, dtype=torch.float32 if f32 else torch.float64, device=dev)
+            with sv._lock:
+                L.check(lib.dcx_factor_solve(sv._h, L.ptr(vals), L.ptr(self.B), L.ptr(U), int(K),
+                                             L.DCX_PERMUTED | L.DCX_SPARSE_RHS | fl32, st),
+                        "factor_solve")
+                sv._vals_keep = vals
+            d = torch.empty(self.M["nD"], dtype=torch.float64, device=dev)
+            L.check(lib.dcx_survey_dpred(self.survey, L.ptr(U), int(K), nb, L.ptr(self.ones), 0,
+                                         int(K), L.ptr(d), L.DCX_PERMUTED | fl32, st),
+                    "survey_dpred")
+            r = w * (d - dobs)
+            phi = torch.zeros(self.nlines, dtype=torch.float64, device=dev)
+            phi.index_add_(0, self.data_line, r ** 2)
+            gd = ((2.0 * r) * w).contiguous()
+            G = torch.empty(n, K, dtype=torch.float64, device=dev)
+            L.check(lib.dcx_survey_adjoint_rhs(self.survey, L.ptr(gd), L.ptr(G), int(K), 0,
+                                               int(K), L.DCX_PERMUTED, st), "adjoint_rhs")
+            Lam = torch.empty_like(U)
+            with sv._lock:
+                L.check(lib.dcx_solve(sv._h, L.ptr(G), L.ptr(Lam), int(K),
+                                      L.DCX_PERMUTED | L.DCX_SHARED_RHS | L.DCX_SPARSE_RHS | fl32,
+                                      st), "solve")
+            gv = sv.grad_values(Lam, U, scale=self.ones, permuted=True)
+            grad = torch.empty_like(sig)
+            L.check(lib.dcx_lmap_transpose(self.lmap, L.ptr(gv), L.ptr(sig), L.ptr(grad), st),
+                    "lmap_transpose")
+        co = self.M["cell_off"]
+        return [(phi[i], grad[co[i]:co[i + 1]]) for i in range(self.nlines)]
+
+    def free(self):
+        if getattr(self, "lmap", None):
+            self.lib.dcx_lmap_destroy(self.lmap)
+            self.lmap = None
+        if getattr(self, "survey", None):
+            self.lib.dcx_survey_destroy(self.survey)
+            self.survey = None
+        self.solver.free()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass

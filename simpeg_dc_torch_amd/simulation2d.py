@@ -190,36 +190,92 @@ class _GpuState:
             pass
 
 
+def _sources_forward(eng, m_full):
+    """dpred through the source columns: one factor_solve per wavenumber batch
+    with the nS source right-hand sides.  Returns (d, saved)."""
+    g = eng._gpu
+    sig = torch.exp(-m_full)
+    vals = g.assemble(sig)
+    U = g.factor_solve_perm(vals, g.q_perm, eng.nS, f32=g.fields32)
+    d = torch.empty(eng.nD, dtype=torch.float64, device=eng.device)
+    g.dpred(U, eng.nS, 0, eng.nS, d)
+    return d, (sig, vals, U)
+
+
+def _sources_backward(eng, saved, stamp, gd):
+    """dphi/dm_full from dphi/dd: one adjoint solve and the entry gathers."""
+    sig, vals, U = saved
+    g = eng._gpu
+    if g.solver.stamp != stamp:      # someone refactorised since
+        g.solver.factorize(vals)
+    v = gd.to(torch.float64).contiguous()
+    G = g.adjoint_rhs(v, eng.nS, 0, eng.nS)
+    Lam = g.solve_perm(G, eng.nS, f32=g.fields32)
+    gv = g.solver.grad_values(Lam, U, scale=g.wq, permuted=True)
+    return g.lmap_t(gv, sig)
+
+
+def _electrodes_forward(eng, m_full):
+    """dpred through the electrode columns (see _ForwardElectrodes2D)."""
+    g = eng._gpu
+    sig = torch.exp(-m_full)
+    vals = g.assemble(sig)
+    G = g.factor_solve_perm(vals, g.E_perm, g.ne, f32=g.fields32)
+    Ke = torch.matmul(g.E_supT, G[:, g.e_sup, :].to(torch.float64))  # (nky, ne, ne)
+    K = (g.wq.view(-1, 1, 1) * Ke).sum(0)
+    d = (g.R * (K @ g.S)[:, g.sidx]).sum(0)
+    return d, (sig, vals, G)
+
+
+def _electrodes_backward(eng, saved, stamp, gd):
+    sig, vals, G = saved
+    g = eng._gpu
+    if g.solver.stamp != stamp:      # refactorised since: solve again
+        g.solver.factorize(vals)
+        G = g.solve_perm(g.E_perm, g.ne, f32=g.fields32)
+    v = gd.to(torch.float64)
+    T = torch.zeros(g.ne, eng.nS, dtype=torch.float64, device=eng.device)
+    T.index_add_(1, g.sidx, g.R * v)
+    M = T @ g.S.T
+    H = torch.matmul(G, M.T.to(G.dtype).contiguous())
+    gv = g.solver.grad_values(G, H, scale=g.wq, permuted=True)
+    return g.lmap_t(gv, sig)
+
+
+def _eval_grad_device(eng, m, dobs, w):
+    """Body of TorchDC2D/TorchDC3D.eval_grad_device: forward, the misfit's
+    derivative in autograd's order (d phi/d d = (2 r) w with r = w (d - dobs)),
+    backward."""
+    if eng._gpu is None:
+        raise RuntimeError("eval_grad_device needs a GPU engine")
+    fwd, bwd = ((_electrodes_forward, _electrodes_backward)
+                if eng.rhs_basis == "electrodes" else (_sources_forward, _sources_backward))
+    with torch.no_grad(), torch.cuda.device(eng.device):
+        m_full = eng._model_to_full(m.detach()).to(torch.float64)
+        d, saved = fwd(eng, m_full)
+        stamp = eng._gpu.solver.stamp
+        r = w * (d - dobs)
+        phi = (r ** 2).sum()
+        grad = eng._model_from_full(bwd(eng, saved, stamp, (2.0 * r) * w))
+    return phi, grad
+
+
 class _Forward2D(torch.autograd.Function):
     """m_full -> dpred on the GPU; backward = one adjoint solve + gathers."""
 
     @staticmethod
     def forward(ctx, m_full, eng):
-        g = eng._gpu
         with torch.cuda.device(eng.device):
-            sig = torch.exp(-m_full.detach())
-            vals = g.assemble(sig)
-            U = g.factor_solve_perm(vals, g.q_perm, eng.nS, f32=g.fields32)
-            d = torch.empty(eng.nD, dtype=torch.float64, device=eng.device)
-            g.dpred(U, eng.nS, 0, eng.nS, d)
+            d, saved = _sources_forward(eng, m_full.detach())
         ctx.eng = eng
-        ctx.stamp = g.solver.stamp
-        ctx.save_for_backward(sig, vals, U)
+        ctx.stamp = eng._gpu.solver.stamp
+        ctx.save_for_backward(*saved)
         return d
 
     @staticmethod
     def backward(ctx, gd):
-        sig, vals, U = ctx.saved_tensors
-        eng = ctx.eng
-        g = eng._gpu
-        with torch.cuda.device(eng.device):
-            if g.solver.stamp != ctx.stamp:      # someone refactorised since
-                g.solver.factorize(vals)
-            v = gd.detach().to(torch.float64).contiguous()
-            G = g.adjoint_rhs(v, eng.nS, 0, eng.nS)
-            Lam = g.solve_perm(G, eng.nS, f32=g.fields32)
-            gv = g.solver.grad_values(Lam, U, scale=g.wq, permuted=True)
-            grad_m = g.lmap_t(gv, sig)
+        with torch.cuda.device(ctx.eng.device):
+            grad_m = _sources_backward(ctx.eng, ctx.saved_tensors, ctx.stamp, gd.detach())
         return grad_m, None
 
 
@@ -238,35 +294,17 @@ class _ForwardElectrodes2D(torch.autograd.Function):
 
     @staticmethod
     def forward(ctx, m_full, eng):
-        g = eng._gpu
         with torch.cuda.device(eng.device):
-            sig = torch.exp(-m_full.detach())
-            vals = g.assemble(sig)
-            G = g.factor_solve_perm(vals, g.E_perm, g.ne, f32=g.fields32)
-            Ke = torch.matmul(g.E_supT, G[:, g.e_sup, :].to(torch.float64))  # (nky, ne, ne)
-            K = (g.wq.view(-1, 1, 1) * Ke).sum(0)
-            d = (g.R * (K @ g.S)[:, g.sidx]).sum(0)
+            d, saved = _electrodes_forward(eng, m_full.detach())
         ctx.eng = eng
-        ctx.stamp = g.solver.stamp
-        ctx.save_for_backward(sig, vals, G)
+        ctx.stamp = eng._gpu.solver.stamp
+        ctx.save_for_backward(*saved)
         return d
 
     @staticmethod
     def backward(ctx, gd):
-        sig, vals, G = ctx.saved_tensors
-        eng = ctx.eng
-        g = eng._gpu
-        with torch.cuda.device(eng.device):
-            if g.solver.stamp != ctx.stamp:      # refactorised since: solve again
-                g.solver.factorize(vals)
-                G = g.solve_perm(g.E_perm, g.ne, f32=g.fields32)
-            v = gd.detach().to(torch.float64)
-            T = torch.zeros(g.ne, eng.nS, dtype=torch.float64, device=eng.device)
-            T.index_add_(1, g.sidx, g.R * v)
-            M = T @ g.S.T
-            H = torch.matmul(G, M.T.to(G.dtype).contiguous())
-            gv = g.solver.grad_values(G, H, scale=g.wq, permuted=True)
-            grad_m = g.lmap_t(gv, sig)
+        with torch.cuda.device(ctx.eng.device):
+            grad_m = _electrodes_backward(ctx.eng, ctx.saved_tensors, ctx.stamp, gd.detach())
         return grad_m, None
 
 
@@ -450,6 +488,15 @@ class TorchDC2D:
         """phi_d = ||w (dpred - dobs)||^2 (no 1/2 factor)."""
         r = w * (self.dpred(m) - dobs)
         return (r ** 2).sum()
+
+    def eval_grad_device(self, m, dobs, w):
+        """(phi_d, dphi_d/dm) as device tensors, with no autograd graph and no
+        host synchronisation: the same kernels, in the same order, as
+        `misfit(m, dobs, w).backward()` (bit-identical results), issued on the
+        calling thread's current stream.  Meant for callers that drive many
+        engines from several host threads (torch's autograd engine runs every
+        CUDA backward of a device on one worker thread).  GPU engines only."""
+        return _eval_grad_device(self, m, dobs, w)
 
     # -- chunked reverse passes ---------------------------------------------------
     def _gpu_reverse(self, m, weights_of, chunk):

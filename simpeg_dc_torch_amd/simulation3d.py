@@ -22,7 +22,7 @@ import scipy.sparse as sp
 import torch
 
 from .operators3d import Problem3DMaps
-from .simulation2d import _Forward2D, _ForwardElectrodes2D, _GpuState
+from .simulation2d import _Forward2D, _ForwardElectrodes2D, _GpuState, _eval_grad_device
 from .solver import make_solver, sparse_solve
 
 
@@ -163,6 +163,12 @@ class TorchDC3D:
         """phi_d = ||w (dpred - dobs)||^2 (no 1/2 factor)."""
         r = w * (self.dpred(m) - dobs)
         return (r ** 2).sum()
+
+    def eval_grad_device(self, m, dobs, w):
+        """(phi_d, dphi_d/dm) as device tensors without an autograd graph or a
+        host synchronisation, bit-identical to `misfit(...).backward()`, on
+        the calling thread's current stream.  GPU engines only."""
+        return _eval_grad_device(self, m, dobs, w)
 
     def misfit_and_grad(self, m, dobs, w):
         """(phi_d, dphi_d/dm) by one reverse pass."""

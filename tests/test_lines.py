@@ -1,8 +1,8 @@
 """Uneven batch of independent lines (simpeg_dc_torch_amd/lines.py).
 
 CPU: the workload generator's sizes and electrode placement.  GPU: the
-stream-concurrent batch against each line's own eval+grad on the default
-stream, the captured graphs against eager runs at several models with eager
+stream-concurrent batch (one issuing host thread, or one per stream) against
+each line's own eval+grad on the default stream, the captured graphs against eager runs at several models with eager
 work and synchronisations between the replays (every replay checked), and
 the leaf autotuner's contract.
 """
@@ -34,7 +34,7 @@ def test_uneven_lines_workload():
         assert 21 <= ne <= 96 and m.size == mesh.nC
 
 
-def _setup(n=4, nky=6, precision="fp64"):
+def _setup(n=4, nky=6, precision="fp64", threads=False):
     from simpeg_dc_torch_amd import TorchDC2D
     from simpeg_dc_torch_amd.lines import LineBatch
     L = _lines(n)
@@ -45,7 +45,7 @@ def _setup(n=4, nky=6, precision="fp64"):
             d = eng.dpred(torch.tensor(m, device="cuda")).cpu().numpy()
         data.append(noisy_data(d, seed=9))
     lb = LineBatch([(mesh, survey) for mesh, survey, _ in L], nky=nky, precision=precision,
-                   streams=4)
+                   streams=4, threads=threads)
     lb.set_data([d for d, _ in data], [w for _, w in data])
     return L, data, lb
 
@@ -54,20 +54,27 @@ def _single(eng, m, dobs, w):
     x = torch.tensor(m, device="cuda", requires_grad=True)
     phi = eng.misfit(x, torch.tensor(dobs, device="cuda"), torch.tensor(w, device="cuda"))
     phi.backward()
-    return float(phi), x.grad.cpu().numpy()
+    return float(phi.detach()), x.grad.cpu().numpy()
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("threads", [False, True])
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_line_batch_streams_match_single(precision):
-    print(f"line batch streams {precision}", flush=True)
-    L, data, lb = _setup(precision=precision)
-    out = lb.eval_grad([m for _, _, m in L])
-    torch.cuda.synchronize()
-    for i, (mesh, survey, m) in enumerate(L):
-        phi, g = _single(lb.engines[i], m, *data[i])
-        assert float(out[i][0]) == pytest.approx(phi, rel=1e-13)
-        assert np.abs(out[i][1].cpu().numpy() - g).max() <= 1e-13 * np.abs(g).max()
+def test_line_batch_streams_match_single(precision, threads):
+    """Every line of the batch (4 streams, issued from one host thread or
+    from one per stream) equals its own autograd eval+grad on the default
+    stream; twice in a row, the second at other models."""
+    print(f"line batch streams {precision} threads={threads}", flush=True)
+    L, data, lb = _setup(precision=precision, threads=threads)
+    rng = np.random.default_rng(4)
+    for rep in range(2):
+        ms = [m + 0.1 * rep * rng.standard_normal(m.size) for _, _, m in L]
+        out = lb.eval_grad(ms)
+        torch.cuda.synchronize()
+        for i in range(len(L)):
+            phi, g = _single(lb.engines[i], ms[i], *data[i])
+            assert float(out[i][0]) == pytest.approx(phi, rel=1e-13), (rep, i)
+            assert np.abs(out[i][1].cpu().numpy() - g).max() <= 1e-13 * np.abs(g).max(), (rep, i)
 
 
 @pytest.mark.gpu
@@ -111,3 +118,83 @@ def test_autotune_leaf_contract():
     assert best in (32, 48) and set(times) == {32, 48} and times[best] == min(times.values())
     lb = LineBatch([(mesh, survey)], nky=6, leaf="auto", tune_data=[(m, dobs, w)])
     assert size_class(mesh.nC) in lb.leaf and lb.leaf[size_class(mesh.nC)] in (32, 48, 64)
+
+
+def _small_lines():
+    """Three small lines of 2.3k-5.2k cells with 18, 26 and 23 sources and
+    6, 4 and 5 wavenumbers, so the merge pads both source columns and
+    systems."""
+    L = _lines(3, (2000, 6000), seed=11)
+    return L, [6, 4, 5]
+
+
+def test_merge_line_maps_host():
+    """The merged problem, solved on the host (scipy, per system), gives each
+    line's data from its own CPU engine: the merged pattern, grouped
+    wavenumbers, boundary terms, padded systems, weighted right-hand sides,
+    survey rows and source columns all land where they belong.  The merged
+    values equal the lines' own values block by block."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+    from simpeg_dc_torch_amd import TorchDC2D
+    from simpeg_dc_torch_amd.lines import merge_line_maps, merged_values_host
+    L, nkys = _small_lines()
+    engs = [TorchDC2D(mesh, survey, nky=nk, device="cpu") for (mesh, survey, _), nk in zip(L, nkys)]
+    M = merge_line_maps([e._maps for e in engs], [e.kys for e in engs], [e.wq_np for e in engs])
+    assert M["nbatch"] == max(e.nky for e in engs) and M["K"] == max(e.nS for e in engs)
+    m = np.concatenate([mm for _, _, mm in L])
+    V = merged_values_host(M, np.exp(-m))
+    for i, e in enumerate(engs):
+        own = e._maps.values(np.exp(-L[i][2]))
+        blk = V[:, M["ent_off"][i]:M["ent_off"][i + 1]]
+        assert np.array_equal(blk[:own.shape[0]], own)
+        assert np.array_equal(blk[own.shape[0]:], np.repeat(own[-1:], M["nbatch"] - own.shape[0], 0))
+    rows = np.repeat(np.arange(M["n"]), np.diff(M["crow"]))
+    q = M["q"].toarray()
+    d = np.zeros(M["nD"])
+    for k in range(M["nbatch"]):
+        A = sp.csc_matrix((V[k], (rows, M["col"])), shape=(M["n"], M["n"]))
+        Bk = M["wq"][k, M["node_line"]][:, None] * q
+        Uk = spla.spsolve(A, Bk).reshape(M["n"], M["K"])
+        d += (M["Pd"] @ Uk)[np.arange(M["nD"]), M["sidx"]]
+    for i, e in enumerate(engs):
+        with torch.no_grad():
+            di = e.dpred(torch.tensor(L[i][2])).numpy()
+        got = d[M["data_off"][i]:M["data_off"][i + 1]]
+        assert np.abs(got - di).max() <= 1e-10 * np.abs(di).max(), i
+
+
+# mixed: with 3% noise the residual is ~0.03 |d|, so a relative data error e
+# moves phi by ~2 e / 0.03; the 1e-6 data bar allows phi to move by ~7e-5
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision,tol_phi,tol_g", [("fp64", 1e-11, 1e-10),
+                                                     ("mixed", 1e-4, 1e-3)])
+def test_line_forest_matches_lines(precision, tol_phi, tol_g):
+    """All lines in one plan against each line's own FP64 engine (autograd
+    eval+grad): three small lines with 6, 4 and 5 wavenumbers (two padded),
+    then the four-line batch of 3k-8k cells; twice in a row, the second at
+    other models."""
+    from simpeg_dc_torch_amd import TorchDC2D
+    from simpeg_dc_torch_amd.lines import LineForest
+    for name, (L, nkys) in (("small", _small_lines()), ("batch4", (_lines(4), [6] * 4))):
+        print(f"line forest {precision} {name}", flush=True)
+        engs = [TorchDC2D(mesh, survey, nky=nk, device="cuda")
+                for (mesh, survey, _), nk in zip(L, nkys)]
+        data = []
+        for e, (_, _, m) in zip(engs, L):
+            with torch.no_grad():
+                d = e.dpred(torch.tensor(m, device="cuda")).cpu().numpy()
+            data.append(noisy_data(d, seed=9))
+        lf = LineForest([(mesh, survey) for mesh, survey, _ in L], nky=nkys, precision=precision)
+        lf.set_data([d for d, _ in data], [w for _, w in data])
+        rng = np.random.default_rng(6)
+        for rep in range(2):
+            ms = [m + 0.1 * rep * rng.standard_normal(m.size) for _, _, m in L]
+            out = lf.eval_grad(ms)
+            torch.cuda.synchronize()
+            for i, e in enumerate(engs):
+                phi, g = _single(e, ms[i], *data[i])
+                assert float(out[i][0]) == pytest.approx(phi, rel=tol_phi), (name, rep, i)
+                gi = out[i][1].cpu().numpy()
+                assert np.linalg.norm(gi - g) <= tol_g * np.linalg.norm(g), (name, rep, i)
+        lf.free()
