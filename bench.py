@@ -322,7 +322,7 @@ def readme_case_times(steps, warmup):
     return out
 
 
-def lines_times(steps, warmup, autotune=True):
+def lines_times(steps, warmup, autotune=True, per_line=True):
     """The uneven batch: 16 independent 2.5-D lines of 3k-40k cells
     (log-uniform, seed 5; 21-96 electrodes, dipole-dipole, 12 wavenumbers
     requested), mixed precision.  One step = eval+grad of every line.  Timed
@@ -331,7 +331,8 @@ def lines_times(steps, warmup, autotune=True):
     issued from 4 host threads (leaf size autotuned per size class, tuning
     outside the timed region); and all lines in one plan (`LineForest`: one
     launch sequence per step).  The one-plan results are compared with the
-    per-line engines' (both mixed precision)."""
+    per-line engines' (both mixed precision).  per_line=False: the one-plan
+    leg alone (for profiling it)."""
     from simpeg_dc_torch_amd import workloads
     from simpeg_dc_torch_amd.lines import LineBatch
     L = workloads.uneven_lines_case()
@@ -346,6 +347,8 @@ def lines_times(steps, warmup, autotune=True):
         dobs = d * (1 + 0.05 * torch.tensor(rng.standard_normal(d.numel()), device="cuda:0"))
         data.append((dobs, 1.0 / (0.05 * dobs.abs())))
     del probe
+    if not per_line:
+        return one_plan_times(lines, ms, data, steps, warmup)
     tune = [(m.cpu().numpy(), d.cpu().numpy(), w.cpu().numpy()) for m, (d, w) in zip(ms, data)]
     t0 = time.perf_counter()
     lb = LineBatch(lines, precision="mixed", streams=4, leaf="auto" if autotune else None,
@@ -369,8 +372,24 @@ def lines_times(steps, warmup, autotune=True):
         return 1e3 * (time.perf_counter() - t1) / steps
 
     ms_seq, ms_par, ms_thr = timed(1), timed(4), timed(4, threads=True)
+    one = one_plan_times(lines, ms, data, steps, warmup, compare=lb)
+    ms_forest = one["one_plan_ms"]
+    cells = sum(mesh.nC for mesh, _ in lines)
+    return {"lines": len(lines), "cells_total": cells,
+            "cells_range": [min(mesh.nC for mesh, _ in lines), max(mesh.nC for mesh, _ in lines)],
+            "precision": "mixed", "sequential_ms": round(ms_seq, 2),
+            "streams4_ms": round(ms_par, 2), "streams4_threads4_ms": round(ms_thr, 2),
+            "one_plan_ms": ms_forest,
+            "lines_per_s": round(1e3 * len(lines) / min(ms_par, ms_thr, ms_forest), 1),
+            "one_plan": one["one_plan"],
+            "leaf_by_size_class": {str(k): v for k, v in (lb.leaf or {}).items()},
+            "build_and_tune_s": round(t_tune, 1), "graph_capture": "off"}
 
-    # every line in one plan (a forest of the lines' separator trees)
+
+def one_plan_times(lines, ms, data, steps, warmup, compare=None):
+    """Every line in one plan (a forest of the lines' separator trees), mixed
+    precision; with `compare` (a LineBatch) the largest relative differences
+    of phi and of the gradient against the per-line engines."""
     from simpeg_dc_torch_amd.lines import LineForest
     t0 = time.perf_counter()
     lf = LineForest(lines, precision="mixed")
@@ -384,27 +403,19 @@ def lines_times(steps, warmup, autotune=True):
         lf.eval_grad(ms)
     torch.cuda.synchronize()
     ms_forest = 1e3 * (time.perf_counter() - t1) / steps
-    out_f, out_b = lf.eval_grad(ms), lb.eval_grad(ms)
-    torch.cuda.synchronize()
-    dphi = max(abs(float(a[0]) / float(b[0]) - 1.0) for a, b in zip(out_f, out_b))
-    dgrad = max(float(torch.linalg.norm(a[1] - b[1]) / torch.linalg.norm(b[1]))
-                for a, b in zip(out_f, out_b))
-    info = lf.solver.info()
+    one = {"source_columns": lf.K, "systems": lf.nbatch,
+           "level_launches": lf.solver.info().get("level_launches"),
+           "build_s": round(t_forest, 1)}
+    if compare is not None:
+        out_f, out_b = lf.eval_grad(ms), compare.eval_grad(ms)
+        torch.cuda.synchronize()
+        one["phi_max_rel_diff_vs_per_line"] = float(
+            f"{max(abs(float(a[0]) / float(b[0]) - 1.0) for a, b in zip(out_f, out_b)):.2e}")
+        one["grad_max_rel_diff_vs_per_line"] = float(f"""{max(
+            float(torch.linalg.norm(a[1] - b[1]) / torch.linalg.norm(b[1]))
+            for a, b in zip(out_f, out_b)):.2e}""")
     lf.free()
-    cells = sum(mesh.nC for mesh, _ in lines)
-    return {"lines": len(lines), "cells_total": cells,
-            "cells_range": [min(mesh.nC for mesh, _ in lines), max(mesh.nC for mesh, _ in lines)],
-            "precision": "mixed", "sequential_ms": round(ms_seq, 2),
-            "streams4_ms": round(ms_par, 2), "streams4_threads4_ms": round(ms_thr, 2),
-            "one_plan_ms": round(ms_forest, 2),
-            "lines_per_s": round(1e3 * len(lines) / min(ms_par, ms_thr, ms_forest), 1),
-            "one_plan": {"source_columns": lf.K, "systems": lf.nbatch,
-                         "level_launches": info.get("level_launches"),
-                         "build_s": round(t_forest, 1),
-                         "phi_max_rel_diff_vs_per_line": float(f"{dphi:.2e}"),
-                         "grad_max_rel_diff_vs_per_line": float(f"{dgrad:.2e}")},
-            "leaf_by_size_class": {str(k): v for k, v in (lb.leaf or {}).items()},
-            "build_and_tune_s": round(t_tune, 1), "graph_capture": "off"}
+    return {"one_plan_ms": round(ms_forest, 2), "one_plan": one}
 
 
 def cpu_baseline(mesh, survey, m, kys, wq, sample_ky=1):
@@ -452,10 +463,12 @@ def main():
     ap.add_argument("--refine", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--pmc-pass", action="store_true")
-    ap.add_argument("--workload", choices=("all", "line25k", "grid3d", "lines", "gn"),
+    ap.add_argument("--workload", choices=("all", "line25k", "grid3d", "lines",
+                                           "lines_one_plan", "gn"),
                     default="all",
                     help="all (default): the 2.5-D line with the 3-D grid and the uneven "
-                         "batch of lines beside it; line25k, grid3d or lines alone")
+                         "batch of lines beside it; line25k, grid3d, lines or gn alone; "
+                         "lines_one_plan: the uneven batch in one plan only")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported "
@@ -472,6 +485,11 @@ def main():
     if args.workload == "gn":
         log("Gauss-Newton step only (62k cells, 2,430 data)")
         print(json.dumps({"workload": "gn", **gn_step_times()}), flush=True)
+        return
+    if args.workload == "lines_one_plan":
+        log("uneven batch in one plan only (16 lines, 3k-40k cells)")
+        print(json.dumps({"workload": "lines_one_plan",
+                          **lines_times(args.steps, args.warmup, per_line=False)}), flush=True)
         return
     if args.workload == "lines":
         log("uneven batch only (16 lines, 3k-40k cells)")

@@ -1905,9 +1905,17 @@ __global__ void k_grad_values(const int* __restrict__ crow, const int* __restric
         for (int u = 0; u < U; ++u)
           acc[q] += static_cast<double>(yv[u]) * static_cast<double>(xv[q][u]);
     }
-    if (c < k) {  // the remaining < LPR U columns: one block, every load issued
-      // before the first use; columns past k read column k - 1 (in bounds,
-      // same cache line) and count zero, so no load sits behind a branch
+    if constexpr (sizeof(TI) == 8 && LPR == 64) {
+      // FP64 whole-wavefront rows: the remainder chunk by chunk (the block
+      // form below holds 4 (1 + MAXE) doubles per lane and measured slower)
+      for (; c < k; c += 64) {
+        const double yv = static_cast<double>(y[c]);
+#pragma unroll
+        for (int q = 0; q < MAXE; ++q) acc[q] += yv * static_cast<double>(xr[q][c]);
+      }
+    } else if (c < k) {  // the remaining < LPR U columns: one block, every load
+      // issued before the first use; columns past k read column k - 1 (in
+      // bounds, same cache line) and count zero, so no load sits behind a branch
       TI yv[U], xv[MAXE][U];
       int cu[U];
 #pragma unroll
