@@ -2074,6 +2074,11 @@ struct dcx_plan {
   // device maps
   SupDev* d_sup = nullptr;
   int *d_lvl = nullptr, *d_relpos = nullptr, *d_cidx = nullptr, *d_ridx = nullptr;
+  // factor launch order: each level's supernodes by front height (ld)
+  // descending, cut into buckets launched with their own LDS size
+  int* d_flvl = nullptr;
+  std::vector<int> flvl_ld;                  // ld of each entry of that order
+  std::vector<int> fbkt_ptr, fbkt;           // level d: bucket starts fbkt[fbkt_ptr[d]..]
   int *d_a_e = nullptr, *d_a_row = nullptr, *d_a_col = nullptr;
   int *d_perm = nullptr, *d_iperm = nullptr, *d_pcrow = nullptr, *d_pcol = nullptr,
       *d_pvidx = nullptr;
@@ -2123,7 +2128,7 @@ struct dcx_plan {
     for (hipEvent_t e : ev) (void)hipEventDestroy(e);
     for (hipEvent_t e : lev_ev) (void)hipEventDestroy(e);
     if (side) (void)hipStreamDestroy(side);
-    void* ptrs[] = {d_sup, d_lvl, d_relpos, d_cidx, d_ridx, d_a_e, d_a_row, d_a_col,
+    void* ptrs[] = {d_sup, d_lvl, d_flvl, d_relpos, d_cidx, d_ridx, d_a_e, d_a_row, d_a_col,
                     d_perm, d_iperm, d_pcrow, d_pcol, d_pvidx, d_crow, d_col, d_info, d_panels,
                     d_front[0], d_front[1], d_vals, d_active, d_act_lvl, d_upd[0],
                     d_upd[1], d_w, d_r, d_b, d_x, d_fz, d_flv, d_fmem, d_fgptr, d_fgb,
@@ -2218,14 +2223,52 @@ inline bool level_is_big(const dcx::Symbolic& S, int d, size_t tsize, int nbatch
   return factor_lds(maxld, tsize) > 160 * 1024 || maxld > 1024 || wgs < grid_below;
 }
 
+// fronts of one LDS size a CU holds at once (8: the wavefront limit of
+// 256-thread workgroups)
+inline int factor_occupancy(int ld, size_t tsize) {
+  return static_cast<int>(std::min<size_t>(8, (160 * 1024) / factor_lds(ld, tsize)));
+}
+
+// the factor launch order (returned, for upload) and its buckets: within a
+// level, supernodes by ld descending; a new bucket starts where the
+// fronts-per-CU class rises by 2 or more and the rest of the level holds at
+// least 2048 workgroups (a smaller gain does not pay for the extra launch on
+// the critical path)
+std::vector<int> build_factor_order(dcx_plan* P) {
+  const auto& S = P->S;
+  std::vector<int> order(S.lvl_sup);
+  P->flvl_ld.assign(S.nsup, 0);
+  P->fbkt_ptr.assign(S.ndepth + 1, 0);
+  P->fbkt.clear();
+  auto ld_of = [&](int s) { return 2 * S.ns[s] + S.rs[s]; };
+  for (int d = 0; d < S.ndepth; ++d) {
+    const int q0 = S.lvl_ptr[d], q1 = S.lvl_ptr[d + 1];
+    std::stable_sort(order.begin() + q0, order.begin() + q1,
+                     [&](int a, int b) { return ld_of(a) > ld_of(b); });
+    for (int q = q0; q < q1; ++q) P->flvl_ld[q] = ld_of(order[q]);
+    if (q1 > q0) P->fbkt.push_back(q0);
+    int occ = q1 > q0 ? factor_occupancy(P->flvl_ld[q0], P->tsize) : 0;
+    for (int q = q0 + 1; q < q1; ++q) {
+      const int o = factor_occupancy(P->flvl_ld[q], P->tsize);
+      if (o >= occ + 2 && static_cast<int64_t>(q1 - q) * P->nbatch >= 2048) {
+        P->fbkt.push_back(q);
+        occ = o;
+      }
+    }
+    P->fbkt_ptr[d + 1] = static_cast<int>(P->fbkt.size());
+  }
+  P->fbkt.push_back(S.nsup);  // end sentinel
+  return order;
+}
+
 template <typename T, int NT, int RPT>
-void factor_launch(dcx_plan* P, int d, int cnt, size_t lds, hipStream_t st) {
+void factor_launch(dcx_plan* P, int d, const int* lvl, int cnt, size_t lds, hipStream_t st) {
   const auto& S = P->S;
   T* fr[2] = {static_cast<T*>(P->d_front[0]), static_cast<T*>(P->d_front[1])};
   const int64_t fbs[2] = {S.front_size[0], S.front_size[1]};
   dim3 grid(cnt, P->nbatch);
   hipLaunchKernelGGL((k_factor_level<T, NT, RPT>), grid, dim3(NT), lds, st, P->d_sup,
-                     P->d_lvl + S.lvl_ptr[d], P->d_relpos, P->d_cidx, P->d_a_e, P->d_a_row,
+                     lvl, P->d_relpos, P->d_cidx, P->d_a_e, P->d_a_row,
                      P->d_a_col, P->d_vals, static_cast<int64_t>(S.nnz), fr[d & 1], fbs[d & 1],
                      fr[(d + 1) & 1], fbs[(d + 1) & 1], static_cast<T*>(P->d_panels),
                      static_cast<T*>(P->d_panelsT), S.panel_size, P->d_info);
@@ -2306,11 +2349,29 @@ int factor_level(dcx_plan* P, int d, hipStream_t st) {
   // levels with few workgroups (the top of the tree) take more wavefronts per
   // front for the trailing update
   static const int wide_below = env_int("DCX_FACTOR_WIDE_BELOW", 321);
+  static const int buckets = env_int("DCX_FACTOR_BUCKETS", 1);
   const bool wide = static_cast<int64_t>(cnt) * P->nbatch < wide_below;
-  if (maxld <= 256 && !wide)
-    factor_launch<T, 256, 1>(P, d, cnt, lds, st);
-  else
-    factor_launch<T, 1024, 1>(P, d, cnt, lds, st);
+  if (!buckets) {
+    const int* lvl = P->d_lvl + S.lvl_ptr[d];
+    if (maxld <= 256 && !wide)
+      factor_launch<T, 256, 1>(P, d, lvl, cnt, lds, st);
+    else
+      factor_launch<T, 1024, 1>(P, d, lvl, cnt, lds, st);
+    return DCX_OK;
+  }
+  // one launch per bucket of the level (fronts of one occupancy class), each
+  // with the LDS of its tallest front (the first: descending order)
+  for (int t = P->fbkt_ptr[d]; t < P->fbkt_ptr[d + 1]; ++t) {
+    const int b0 = P->fbkt[t];
+    const int b1 = t + 1 < P->fbkt_ptr[d + 1] ? P->fbkt[t + 1] : S.lvl_ptr[d + 1];
+    const int bld = P->flvl_ld[b0];
+    const size_t blds = factor_lds(bld, sizeof(T));
+    const int* lvl = P->d_flvl + b0;
+    if (bld <= 256 && !wide)
+      factor_launch<T, 256, 1>(P, d, lvl, b1 - b0, blds, st);
+    else
+      factor_launch<T, 1024, 1>(P, d, lvl, b1 - b0, blds, st);
+  }
   return DCX_OK;
 }
 
@@ -2761,6 +2822,7 @@ int dcx_plan_create(int n, const int32_t* crow, const int32_t* col, int nbatch,
       if (rc) return rc;
     }
     HIP_TRY(upload(&P->d_lvl, S.lvl_sup));
+    HIP_TRY(upload(&P->d_flvl, build_factor_order(P.get())));
     HIP_TRY(upload(&P->d_relpos, S.relpos));
     HIP_TRY(upload(&P->d_cidx, S.cidx));
     HIP_TRY(upload(&P->d_ridx, S.ridx));
