@@ -1,0 +1,284 @@
+"""Induced polarisation on the DC engines.
+
+Linearised IP (Seigel), the convention of stock SimPEG's
+``induced_polarization.Simulation2DNodal`` / ``Simulation3DNodal`` that the
+reference restates (dctorch/ip2d.py:1-27, dctorch/ip3d.py:1-24):
+
+    d_IP = scale * J_sigma[sigma_bg] (-diag(sigma_bg)) eta,
+    scale = 1 / V_dc for receivers of data_type "apparent_chargeability",
+            1 otherwise.
+
+With the DC model m = log(rho) (sigma = exp(-m), d sigma / d m = -sigma)
+J_sigma (-diag sigma) is exactly the Jacobian J_m of the DC data in m, so
+
+    d_IP = scale * J_m(m_bg) eta,   grad_eta phi = J_m^T (scale * dphi/dd),
+
+one J v and one J^T v of the DC engine frozen at the background.  On a GPU
+engine those are `Linearization2D.Jvec` / `Jtvec` (libdcx.so: dcx_assemble,
+dcx_spmm, dcx_solve on the background factorisation, dcx_survey_dpred; and
+dcx_survey_adjoint_rhs, dcx_solve, dcx_grad_values, dcx_lmap_transpose);
+the background is factorised once at construction.  On the cpu the 2.5-D
+case runs the same linearisation's CPU path and the 3-D case the
+reference's own chain (fixed-factor SciPy solves, torch sparse products).
+
+`TorchDCIP2D` is the joint non-linear two-state forward of
+dctorch/dcip2d.py:1-60: V0 = F(sigma), V1 = F(sigma (1 - eta)),
+d_IP = ip_scale (V1 - V0) / V1, d_DC = V1 ("total") or V0
+("instantaneous"), eta = eta_max sigmoid(m_eta).  The two states run as two
+`TorchDC2D` engines (one plan each, m_eff = m_rho - log(1 - eta)), so both
+forward solves and both adjoint solves are the HIP eval+grad path and autograd
+differentiates both blocks of the model.
+"""
+import numpy as np
+import torch
+
+from .linearization2d import Linearization2D
+
+
+def chargeability_scale(survey, v_dc):
+    """Per-datum scale of stock SimPEG's BaseIPSimulation._scale: 1 / V_dc on
+    the data of "apparent_chargeability" receivers, 1 elsewhere (data in
+    survey order: sources, then their receivers).  Follows
+    dctorch/ip2d.py:105-116."""
+    scale = torch.ones_like(v_dc)
+    i0 = 0
+    for src in survey.source_list:
+        for rx in src.receiver_list:
+            i1 = i0 + int(rx.nD)
+            if getattr(rx, "data_type", "volt") == "apparent_chargeability":
+                scale[i0:i1] = 1.0 / v_dc[i0:i1]
+            i0 = i1
+    if i0 != v_dc.numel():
+        raise ValueError(f"the survey lists {i0} data, the engine {v_dc.numel()}")
+    return scale
+
+
+class _LinearIP(torch.autograd.Function):
+    """eta -> scale * J eta through a frozen linearisation (any device)."""
+
+    @staticmethod
+    def forward(ctx, e, ip):
+        ctx.ip = ip
+        return ip._scale * ip._jvec(e.detach())
+
+    @staticmethod
+    def backward(ctx, g):
+        ip = ctx.ip
+        return ip._jtvec(ip._scale * g.detach()), None
+
+
+class _FixedSolve(torch.autograd.Function):
+    """x = A^{-1} b with A fixed and factored (symmetric): differentiable in b
+    only, the backward one solve on the same factorisation
+    (dctorch/ip2d.py:32-48)."""
+
+    @staticmethod
+    def forward(ctx, b, solver):
+        ctx.solver = solver
+        return solver.solve(b.detach())
+
+    @staticmethod
+    def backward(ctx, g):
+        return ctx.solver.solve(g.contiguous()), None
+
+
+class _IPBase:
+    def _finish(self, survey, v_dc):
+        self.v_dc = v_dc
+        self._scale = chargeability_scale(survey, v_dc)
+
+    def misfit(self, eta, dobs, w):
+        """phi_d = ||w (dpred - dobs)||^2 (no 1/2 factor)."""
+        r = w * (self.dpred(eta) - dobs)
+        return (r ** 2).sum()
+
+
+class TorchIP2D(_IPBase):
+    """Differentiable linearised-IP dpred / misfit for the 2.5-D nodal
+    problem on a fixed mesh, survey and background (dctorch/ip2d.py:51-143).
+
+    m_dc: (nC,) log-rho background on the full mesh; active_cells: optional
+    mask of the eta model (inactive eta = 0).  Extra keywords as TorchDC2D
+    (`precision`, `refine`, `leaf`, `rhs_basis`).  The wrapped engine `dc`
+    holds the background factorisation: calling its dpred refactorises it
+    (the linearisation restores the background before its next product)."""
+
+    def __init__(self, mesh, survey, m_dc, nky=12, device="cuda", dtype=torch.float64,
+                 active_cells=None, quadrature=None, **engine_kw):
+        from .simulation2d import TorchDC2D
+        dc = TorchDC2D(mesh, survey, nky=nky, device=device, dtype=dtype,
+                       quadrature=quadrature, **engine_kw)
+        self.dc = dc
+        self.device, self.dtype = dc.device, dc.dtype
+        m_bg = torch.as_tensor(np.asarray(m_dc, dtype=float).reshape(-1), dtype=torch.float64,
+                               device=dc.device)
+        if m_bg.numel() != mesh.nC:
+            raise ValueError("m_dc must live on the full mesh")
+        self._sig_bg = torch.exp(-m_bg)
+        self._lin = Linearization2D(dc, m_bg)
+        self._act = None
+        if active_cells is not None:
+            self._act = torch.as_tensor(np.where(np.asarray(active_cells, bool))[0],
+                                        device=dc.device)
+        self._finish(survey, self._lin.d.to(torch.float64))
+
+    def _jvec(self, e):
+        return self._lin.Jvec(e).to(torch.float64)
+
+    def _jtvec(self, g):
+        return self._lin.Jtvec(g).to(torch.float64)
+
+    def dpred(self, eta):
+        """IP data (nD,) for eta on the full mesh, or on the active cells
+        when built with active_cells.  Autograd-traceable; linear in eta."""
+        dev0, dt0 = eta.device, eta.dtype
+        e = eta.to(device=self.device, dtype=torch.float64).reshape(-1)
+        if self._act is not None:
+            e = torch.zeros(self.dc.nC, dtype=torch.float64,
+                            device=self.device).index_put((self._act,), e)
+        return _LinearIP.apply(e, self).to(device=dev0, dtype=dt0)
+
+
+class TorchIP3D(_IPBase):
+    """Linearised IP for the 3-D nodal problem (dctorch/ip3d.py:68-151).
+
+    m_dc: log-rho background on the active cells when active_cells is given
+    (inactive cells at val_inactive), else on the full mesh; eta lives on the
+    same cells (inactive eta = 0)."""
+
+    def __init__(self, mesh, survey, m_dc, device="cuda", dtype=torch.float64,
+                 active_cells=None, val_inactive=None, bc_type="Robin", **engine_kw):
+        from .simulation3d import TorchDC3D
+        dc = TorchDC3D(mesh, survey, device=device, dtype=dtype, active_cells=active_cells,
+                       val_inactive=val_inactive, bc_type=bc_type, **engine_kw)
+        self.dc = dc
+        self.device, self.dtype = dc.device, dc.dtype
+        m_bg = torch.as_tensor(np.asarray(m_dc, dtype=float).reshape(-1), dtype=torch.float64,
+                               device=dc.device)
+        nP = mesh.nC if dc._P_inj is None else int(dc._act.numel())
+        if m_bg.numel() != nP:
+            raise ValueError(f"m_dc must have {nP} entries (the engine's model cells)")
+        with torch.no_grad():
+            self._sig_bg = torch.exp(-dc._model_to_full(m_bg).to(torch.float64))
+        if dc._gpu is not None:
+            self._lin = Linearization2D(dc, m_bg)
+            v_dc = self._lin.d.to(torch.float64)
+        else:
+            v_dc = self._init_cpu()
+        self._finish(survey, v_dc)
+
+    # the reference's CPU chain, with the background factorised once
+    def _init_cpu(self):
+        dc = self.dc
+        with torch.no_grad():
+            vals = torch.sparse.mm(dc.L3, self._sig_bg.unsqueeze(1)).squeeze(1)
+            dc.solver.factorize(vals)
+            self._U = dc.solver.solve(dc.q)
+            self._row = dc.solver.row.to(torch.int64)
+            self._col = dc.solver.col.to(torch.int64)
+            return (dc.Pd @ self._U)[dc.iD, dc.sidx]
+
+    def _dpred_cpu(self, e_full):
+        dc = self.dc
+        dvals = torch.sparse.mm(dc.L3, (-self._sig_bg * e_full).unsqueeze(1)).squeeze(1)
+        rhs = torch.zeros_like(self._U).index_add(0, self._row,
+                                                   dvals.unsqueeze(1) * self._U[self._col])
+        dU = _FixedSolve.apply(-rhs, dc.solver)
+        return (dc.Pd @ dU)[dc.iD, dc.sidx] * self._scale
+
+    def _jvec(self, e):
+        return self._lin.Jvec(e).to(torch.float64)
+
+    def _jtvec(self, g):
+        return self._lin.Jtvec(g).to(torch.float64)
+
+    def dpred(self, eta):
+        """IP data (nD,) for eta on the engine's model cells.  Autograd-
+        traceable; linear in eta."""
+        dev0, dt0 = eta.device, eta.dtype
+        e = eta.to(device=self.device, dtype=torch.float64).reshape(-1)
+        if self.dc._gpu is not None:
+            d = _LinearIP.apply(e, self)
+        else:
+            if self.dc._P_inj is not None:
+                e = torch.zeros(self.dc.nC, dtype=torch.float64,
+                                device=self.device).index_put((self.dc._act,), e)
+            d = self._dpred_cpu(e)
+        return d.to(device=dev0, dtype=dt0)
+
+
+class TorchDCIP2D:
+    """Joint non-linear (rho, eta) 2.5-D DC-IP forward (dctorch/dcip2d.py:
+    62-193): model m = cat(log rho, logit(eta / eta_max)) on the model cells;
+    `dpred(m)` -> (d_DC, d_IP); `misfit(m, dobs_dc, w_dc, dobs_ip, w_ip, lam)`.
+    Receivers are plain "volt" (the ratio is formed here).  Extra keywords
+    go to both TorchDC2D engines."""
+
+    def __init__(self, mesh, survey, nky=12, device="cuda", dtype=torch.float64,
+                 active_cells=None, val_inactive=None, idx_dc=None, idx_ip=None, eta_max=1.0,
+                 ip_scale=1.0, dc_state="total", quadrature=None, **engine_kw):
+        from .simulation2d import TorchDC2D
+        if dc_state not in ("total", "instantaneous"):
+            raise ValueError(f"dc_state must be total|instantaneous, got {dc_state!r}")
+        kw = dict(nky=nky, device=device, dtype=dtype, active_cells=active_cells,
+                  val_inactive=val_inactive, quadrature=quadrature, **engine_kw)
+        self.dc = TorchDC2D(mesh, survey, **kw)            # instantaneous state sigma
+        self.dc_polarized = TorchDC2D(mesh, survey, **kw)  # sigma (1 - eta)
+        dc = self.dc
+        self.device, self.dtype = dc.device, dc.dtype
+        self.nky, self.nD = dc.nky, dc.nD
+        self.nP = dc.nC if dc._P_inj is None else int(dc._act.numel())
+        self.eta_max = float(eta_max)
+        self.ip_scale = float(ip_scale)
+        self.dc_state = dc_state
+
+        def idx(a):
+            if a is None:
+                return torch.arange(dc.nD, dtype=torch.int64, device=self.device)
+            return torch.as_tensor(np.asarray(a, dtype=np.int64), device=self.device)
+
+        self.idx_dc, self.idx_ip = idx(idx_dc), idx(idx_ip)
+        self.nD_dc, self.nD_ip = int(self.idx_dc.numel()), int(self.idx_ip.numel())
+
+    def split(self, m):
+        """(m_rho, m_eta) blocks of the stacked model."""
+        m = m.reshape(-1)
+        if m.numel() != 2 * self.nP:
+            raise ValueError(f"model must be 2*{self.nP} long, got {m.numel()}")
+        return m[:self.nP], m[self.nP:]
+
+    def eta_of(self, m_eta):
+        """Physical eta from the logit block."""
+        return self.eta_max * torch.sigmoid(m_eta)
+
+    def m_eta_of(self, eta):
+        """Inverse of eta_of (numpy or tensor)."""
+        if not torch.is_tensor(eta):
+            e = np.clip(np.asarray(eta, dtype=float) / self.eta_max, 1e-12, 1 - 1e-12)
+            return np.log(e / (1 - e))
+        e = (eta / self.eta_max).clamp(1e-12, 1 - 1e-12)
+        return torch.log(e / (1 - e))
+
+    def dpred_from_fields(self, m_rho, eta):
+        """(d_DC, d_IP) from log-rho and physical eta on the model cells."""
+        dev0, dt0 = m_rho.device, m_rho.dtype
+        mr = m_rho.to(device=self.device, dtype=torch.float64).reshape(-1)
+        e = eta.to(device=self.device, dtype=torch.float64).reshape(-1)
+        V0 = self.dc.dpred(mr)
+        V1 = self.dc_polarized.dpred(mr - torch.log1p(-e))   # rho / (1 - eta)
+        d_dc = (V1 if self.dc_state == "total" else V0)[self.idx_dc]
+        v1 = V1[self.idx_ip]
+        d_ip = self.ip_scale * (v1 - V0[self.idx_ip]) / v1
+        return d_dc.to(device=dev0, dtype=dt0), d_ip.to(device=dev0, dtype=dt0)
+
+    def dpred(self, m):
+        """(d_DC, d_IP) for the stacked model m = cat(log rho, logit eta)."""
+        m_rho, m_eta = self.split(m)
+        return self.dpred_from_fields(m_rho, self.eta_of(m_eta))
+
+    def misfit(self, m, dobs_dc, w_dc, dobs_ip, w_ip, lam=1.0):
+        """phi_d^DC + lam phi_d^IP, each ||w (dpred - dobs)||^2 (no 1/2)."""
+        d_dc, d_ip = self.dpred(m)
+        r0 = w_dc * (d_dc - dobs_dc)
+        r1 = w_ip * (d_ip - dobs_ip)
+        return (r0 ** 2).sum() + lam * (r1 ** 2).sum()

@@ -5,13 +5,25 @@
 receiver's `locations_m` / `locations_n` (a pole has no N) and `nD`.  A stock
 `simpeg.electromagnetics.static.resistivity.survey.Survey` has exactly these
 attributes and works unchanged; the classes below supply them on hosts
-without SimPEG.  Receivers measure potential ("volt"), with unit current.
+without SimPEG.  Receivers measure potential ("volt"), with unit current;
+an induced-polarisation survey marks receivers "apparent_chargeability"
+(stock SimPEG's name), which the DC engines treat as "volt" and the IP
+engines divide by the background voltage.
 """
 import numpy as np
 
 
 def _locs(a):
     return np.atleast_2d(np.asarray(a, dtype=float))
+
+
+_DATA_TYPES = ("volt", "apparent_chargeability")
+
+
+def _check_type(data_type):
+    if data_type not in _DATA_TYPES:
+        raise ValueError(f"data_type must be one of {_DATA_TYPES}, got {data_type!r}")
+    return data_type
 
 
 class _Receiver:
@@ -26,8 +38,7 @@ class DipoleReceiver(_Receiver):
     """Potential difference u(M) - u(N) for each row of M and N."""
 
     def __init__(self, locations_m, locations_n, data_type="volt"):
-        if data_type != "volt":
-            raise ValueError("only data_type='volt' is supported")
+        self.data_type = _check_type(data_type)
         self.locations_m = _locs(locations_m)
         self.locations_n = _locs(locations_n)
         if self.locations_m.shape != self.locations_n.shape:
@@ -44,8 +55,7 @@ class PoleReceiver(_Receiver):
     locations_n = None
 
     def __init__(self, locations, data_type="volt"):
-        if data_type != "volt":
-            raise ValueError("only data_type='volt' is supported")
+        self.data_type = _check_type(data_type)
         self.locations_m = _locs(locations)
 
     @property
