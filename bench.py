@@ -322,6 +322,77 @@ def readme_case_times(steps, warmup):
     return out
 
 
+def ip_times(steps, warmup):
+    """Induced polarisation, FP64, eval+grad = misfit and its gradient:
+    linearised IP on the 25k-cell line (TorchIP2D: one J v and one J^T v of
+    the DC engine frozen at the background, apparent-chargeability data;
+    also with the electrode basis, where neither product solves again) and
+    on the 56 x 56 x 38 grid (TorchIP3D), and the joint non-linear DC-IP
+    forward on the 25k-cell line (TorchDCIP2D: two DC eval+grads, sigma and
+    sigma (1 - eta))."""
+    from simpeg_dc_torch_amd import TorchDCIP2D, TorchIP2D, TorchIP3D, workloads
+    from simpeg_dc_torch_amd import survey as S
+    dev = "cuda:0"
+
+    def charge(sv):
+        srcs = []
+        for src in sv.source_list:
+            rxs = [S.DipoleReceiver(r.locations_m, r.locations_n,
+                                    data_type="apparent_chargeability")
+                   if r.locations_n is not None else
+                   S.PoleReceiver(r.locations_m, data_type="apparent_chargeability")
+                   for r in src.receiver_list]
+            srcs.append(S.DipoleSource(rxs, src.location_a, src.location_b)
+                        if src.location_b is not None else S.PoleSource(rxs, src.location_a))
+        return S.Survey(srcs)
+
+    def timed(fn):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        torch.cuda.synchronize()
+        return round(1e3 * (time.perf_counter() - t0) / steps, 3)
+
+    out = {}
+    rng = np.random.default_rng(8)
+    line = workloads.line_25k_case()[:3]
+    for name, (mesh, survey, m), kw in (("line25k", line, {}),
+                                        ("line25k_electrodes", line, {"rhs_basis": "electrodes"}),
+                                        ("grid3d", workloads.grid_3d_case(), {})):
+        t0 = time.perf_counter()
+        ip = (TorchIP3D(mesh, charge(survey), m, device=dev) if name == "grid3d"
+              else TorchIP2D(mesh, charge(survey), m, nky=12, device=dev, **kw))
+        t_build = time.perf_counter() - t0
+        eta = torch.tensor(0.01 + 0.05 * rng.random(mesh.nC), device=dev)
+        with torch.no_grad():
+            d = ip.dpred(eta)
+        dobs = d * (1 + 0.05 * torch.tensor(rng.standard_normal(d.numel()), device=dev))
+        w = 1.0 / (0.05 * dobs.abs() + 1e-6)
+
+        def step():
+            x = eta.clone().requires_grad_(True)
+            ip.misfit(x, dobs, w).backward()
+        out[f"ip_{name}_ms"] = timed(step)
+        out[f"ip_{name}_build_s"] = round(t_build, 1)
+        del ip
+    mesh, survey, m = workloads.line_25k_case()[:3]
+    dcip = TorchDCIP2D(mesh, survey, nky=12, device=dev, ip_scale=1000.0)
+    x0 = torch.tensor(np.r_[m, dcip.m_eta_of(0.02 + 0.1 * rng.random(mesh.nC))], device=dev)
+    with torch.no_grad():
+        d_dc, d_ip = dcip.dpred(x0)
+    w_dc, w_ip = 1.0 / (0.05 * d_dc.abs()), torch.full_like(d_ip, 1.0)
+
+    def step_dcip():
+        x = x0.clone().requires_grad_(True)
+        dcip.misfit(x, d_dc * 1.01, w_dc, d_ip + 1.0, w_ip).backward()
+    out["dcip_line25k_ms"] = timed(step_dcip)
+    out["precision"] = "fp64"
+    return out
+
+
 def lines_times(steps, warmup, autotune=True, per_line=True):
     """The uneven batch: 16 independent 2.5-D lines of 3k-40k cells
     (log-uniform, seed 5; 21-96 electrodes, dipole-dipole, 12 wavenumbers
@@ -464,7 +535,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--pmc-pass", action="store_true")
     ap.add_argument("--workload", choices=("all", "line25k", "grid3d", "lines",
-                                           "lines_one_plan", "gn"),
+                                           "lines_one_plan", "gn", "ip"),
                     default="all",
                     help="all (default): the 2.5-D line with the 3-D grid and the uneven "
                          "batch of lines beside it; line25k, grid3d, lines or gn alone; "
@@ -485,6 +556,10 @@ def main():
     if args.workload == "gn":
         log("Gauss-Newton step only (62k cells, 2,430 data)")
         print(json.dumps({"workload": "gn", **gn_step_times()}), flush=True)
+        return
+    if args.workload == "ip":
+        log("induced polarisation only (IP2D, IP3D, DCIP2D)")
+        print(json.dumps({"workload": "ip", **ip_times(args.steps, args.warmup)}), flush=True)
         return
     if args.workload == "lines_one_plan":
         log("uneven batch in one plan only (16 lines, 3k-40k cells)")
@@ -528,7 +603,7 @@ def main():
     log("linearisation (FP64): J v, J^T v, J, diag(J^T D J)")
     lin = linearization_times(e64, mt)
     log(f"linearisation {lin}")
-    g3 = lb = rc = gn = None
+    g3 = lb = rc = gn = ipt = None
     if args.workload == "all":
         log("3-D grid (TorchDC3D, 56 x 56 x 38 cells, 200 sources)")
         g3 = grid3d_times(args.steps, args.warmup)
@@ -539,6 +614,9 @@ def main():
         log("Gauss-Newton step (62k cells, 2,430 data, FP64)")
         gn = gn_step_times()
         log(f"gn step {gn}")
+        log("induced polarisation (IP2D, IP3D, DCIP2D)")
+        ipt = ip_times(args.steps, args.warmup)
+        log(f"ip {ipt}")
         log("the reference's published eval+grad case (5k cells, nky 11)")
         rc = readme_case_times(args.steps, args.warmup)
         log(f"readme case {rc}")
@@ -571,6 +649,7 @@ def main():
         "uneven_lines": lb,
         "reference_readme_case": rc,
         "gn_step": gn,
+        "induced_polarisation": ipt,
         "cpu_baseline": cpu,
     }
     if cpu:
