@@ -41,6 +41,16 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def tune_tiles(engines, mt, dobs, w):
+    """Kernel-variant thresholds of each engine's plan by the tile autotuner
+    (outside every timed region); {label: chosen setting} and the seconds
+    spent."""
+    from simpeg_dc_torch_amd.lines import autotune_tiles
+    t0 = time.perf_counter()
+    out = {label: autotune_tiles(eng, mt, dobs, w, reps=5)[0] for label, eng in engines.items()}
+    return out, round(time.perf_counter() - t0, 2)
+
+
 def build_engines(refine):
     from simpeg_dc_torch_amd import TorchDC2D, workloads
     mesh, survey, m, rng = workloads.line_25k_case()
@@ -423,7 +433,8 @@ def lines_times(steps, warmup, autotune=True, per_line=True):
     tune = [(m.cpu().numpy(), d.cpu().numpy(), w.cpu().numpy()) for m, (d, w) in zip(ms, data)]
     t0 = time.perf_counter()
     lb = LineBatch(lines, precision="mixed", streams=4, leaf="auto" if autotune else None,
-                   tune_data=tune if autotune else None, threads=True)
+                   tune_data=tune if autotune else None, threads=True,
+                   tiles="auto" if autotune else None)
     pool = lb._pool
     t_tune = time.perf_counter() - t0
     lb.set_data([d for d, _ in data], [w for _, w in data])
@@ -454,6 +465,7 @@ def lines_times(steps, warmup, autotune=True, per_line=True):
             "lines_per_s": round(1e3 * len(lines) / min(ms_par, ms_thr, ms_forest), 1),
             "one_plan": one["one_plan"],
             "leaf_by_size_class": {str(k): v for k, v in (lb.leaf or {}).items()},
+            "tiles_by_size_class": {str(k): v for k, v in lb.tiles.items()},
             "build_and_tune_s": round(t_tune, 1), "graph_capture": "off"}
 
 
@@ -466,6 +478,9 @@ def one_plan_times(lines, ms, data, steps, warmup, compare=None):
     lf = LineForest(lines, precision="mixed")
     t_forest = time.perf_counter() - t0
     lf.set_data([d for d, _ in data], [w for _, w in data])
+    t0 = time.perf_counter()
+    tiles, _ = lf.autotune_tiles(ms)
+    t_tiles = time.perf_counter() - t0
     for _ in range(warmup):
         lf.eval_grad(ms)
     torch.cuda.synchronize()
@@ -476,7 +491,7 @@ def one_plan_times(lines, ms, data, steps, warmup, compare=None):
     ms_forest = 1e3 * (time.perf_counter() - t1) / steps
     one = {"source_columns": lf.K, "systems": lf.nbatch,
            "level_launches": lf.solver.info().get("level_launches"),
-           "build_s": round(t_forest, 1)}
+           "build_s": round(t_forest, 1), "tiles": tiles, "tile_tuning_s": round(t_tiles, 1)}
     if compare is not None:
         out_f, out_b = lf.eval_grad(ms), compare.eval_grad(ms)
         torch.cuda.synchronize()
@@ -518,6 +533,8 @@ def pmc_pass(args):
     b.copy_(a)
     torch.cuda.synchronize()
     mesh, survey, m, e64, emx, mt, dobs, w, _ = build_engines(args.refine)
+    # default kernel-variant thresholds: a tuning search here would add its
+    # trial runs to the per-dispatch counter averages
     for _ in range(2):
         eval_grad(emx, mt, dobs, w)
     torch.cuda.synchronize()
@@ -534,6 +551,8 @@ def main():
     ap.add_argument("--refine", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--pmc-pass", action="store_true")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="keep the default kernel-variant thresholds (no tile autotuning)")
     ap.add_argument("--workload", choices=("all", "line25k", "grid3d", "lines",
                                            "lines_one_plan", "gn", "ip"),
                     default="all",
@@ -587,6 +606,11 @@ def main():
     err_d = float(torch.linalg.norm(dmx - d64) / torch.linalg.norm(d64))
     err_g = float(torch.linalg.norm(gmx - g64) / torch.linalg.norm(g64))
     log(f"mixed vs fp64: dpred {err_d:.2e} grad {err_g:.2e}")
+    tiles = None
+    if not args.no_tune:
+        tiles, t_tune = tune_tiles({"fp64": e64, "mixed": emx}, mt, dobs, w)
+        tiles["tune_s"] = t_tune
+        log(f"tile tuning {tiles}")
     log("timing fp64")
     ms64 = time_steps(e64, mt, dobs, w, args.steps, args.warmup)
     log(f"fp64 eval+grad {ms64:.2f} ms")
@@ -642,6 +666,7 @@ def main():
         "fp64_ms": round(ms64, 3), "mixed_speedup_vs_fp64": round(ms64 / msmx, 3),
         "mixed_dpred_rel_err": err_d, "mixed_grad_rel_err": err_g,
         "build_s": round(t_build, 2),
+        "tile_tuning": tiles,
         "roofline": roof, "roofline_fp64": roof64,
         "electrode_basis": eb,
         "linearization": lin,

@@ -38,6 +38,8 @@ enum {
 
 /* factor precision */
 enum { DCX_FP64 = 0, DCX_FP32 = 1 };
+/* keys of dcx_plan_set_tuning / dcx_plan_get_tuning */
+enum { DCX_TUNE_SWEEP_TC = 1, DCX_TUNE_SPLITK_BELOW = 2, DCX_TUNE_WIDE_BELOW = 3 };
 
 /* dcx_solve flags */
 enum {
@@ -108,6 +110,16 @@ int dcx_plan_get_perm(const dcx_plan* plan, int32_t* perm);
  * the last event) over the solves since timing was enabled. */
 int dcx_plan_set_timing(dcx_plan* plan, int enable);
 int dcx_plan_sweep_times(dcx_plan* plan, double* fwd_ms, double* bwd_ms, int* nsweeps);
+/* Kernel-variant thresholds of one plan (defaults from the environment at
+ * dcx_plan_create): DCX_TUNE_SWEEP_TC, the widest column tile of the in-LDS
+ * sweeps (16, 32, 64); DCX_TUNE_SPLITK_BELOW, backward-sweep levels with fewer
+ * wavefronts take the split-K variant; DCX_TUNE_WIDE_BELOW, factor levels with
+ * fewer workgroups take 1024-thread fronts.  Every value selects among
+ * kernels that are correct for any level; only speed and summation order
+ * change.  No counterpart in the reference (cuDSS chooses internally behind
+ * dctorch/solver.py:187-213); a tile autotuner times them per size class. */
+int dcx_plan_set_tuning(dcx_plan* plan, int key, int value);
+int dcx_plan_get_tuning(const dcx_plan* plan, int key, int* value);
 
 /* LDS row budget of the fused bottom subtrees of the sweeps (0: none);
  * rebuilds the fused structure (host) and its device tables. */

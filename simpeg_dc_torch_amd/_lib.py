@@ -15,6 +15,7 @@ LIB_PATH = os.environ.get("DCX_LIBRARY") or os.path.join(_HERE, "libdcx.so")
 
 DCX_FP64, DCX_FP32 = 0, 1
 DCX_SHARED_RHS, DCX_PERMUTED, DCX_SPARSE_RHS, DCX_F32_FIELDS = 1, 2, 4, 8
+DCX_TUNE_SWEEP_TC, DCX_TUNE_SPLITK_BELOW, DCX_TUNE_WIDE_BELOW = 1, 2, 3
 
 _i32p = C.POINTER(C.c_int32)
 _f64p = C.POINTER(C.c_double)
@@ -61,6 +62,8 @@ PROTOTYPES = {
     "dcx_plan_set_refinement": (C.c_int, [_vp, C.c_int]),
     "dcx_plan_set_fusion": (C.c_int, [_vp, C.c_int]),
     "dcx_plan_set_timing": (C.c_int, [_vp, C.c_int]),
+    "dcx_plan_set_tuning": (C.c_int, [_vp, C.c_int, C.c_int]),
+    "dcx_plan_get_tuning": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_int)]),
     "dcx_plan_sweep_times": (C.c_int, [_vp, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                        C.POINTER(C.c_int)]),
     "dcx_factorize": (C.c_int, [_vp, _vp, _vp]),

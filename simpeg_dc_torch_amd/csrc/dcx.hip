@@ -2074,6 +2074,11 @@ struct dcx_plan {
   // device maps
   SupDev* d_sup = nullptr;
   int *d_lvl = nullptr, *d_relpos = nullptr, *d_cidx = nullptr, *d_ridx = nullptr;
+  // kernel-variant thresholds (dcx_plan_set_tuning; defaults from the
+  // environment at plan creation)
+  int tc_cap = 32;           // widest column tile of the in-LDS sweeps
+  int splitk_below = 4096;   // backward levels with fewer wavefronts take split-K
+  int wide_below = 321;      // factor levels with fewer workgroups take 1024 threads
   // factor launch order: each level's supernodes by front height (ld)
   // descending, cut into buckets launched with their own LDS size
   int* d_flvl = nullptr;
@@ -2348,9 +2353,8 @@ int factor_level(dcx_plan* P, int d, hipStream_t st) {
   if (level_is_big(S, d, sizeof(T), P->nbatch)) return factor_level_big<T>(P, d, cnt, st);
   // levels with few workgroups (the top of the tree) take more wavefronts per
   // front for the trailing update
-  static const int wide_below = env_int("DCX_FACTOR_WIDE_BELOW", 321);
   static const int buckets = env_int("DCX_FACTOR_BUCKETS", 1);
-  const bool wide = static_cast<int64_t>(cnt) * P->nbatch < wide_below;
+  const bool wide = static_cast<int64_t>(cnt) * P->nbatch < P->wide_below;
   if (!buckets) {
     const int* lvl = P->d_lvl + S.lvl_ptr[d];
     if (maxld <= 256 && !wide)
@@ -2428,7 +2432,7 @@ void sweep_launch(dcx_plan* P, T* W, int k, int maxnf, int maxns, int pmax, int 
 constexpr size_t kSweepLds = 160 * 1024;
 
 
-int pick_tc(int maxnf, int maxns, size_t tsize);
+int pick_tc(int maxnf, int maxns, size_t tsize, int cap);
 
 // rows of the widest level whose forward sweep takes the large-front path or
 // whose backward sweep is row-grouped (both use the scratch block d_fs)
@@ -2444,7 +2448,7 @@ inline int64_t big_fwd_rows(const dcx_plan* P, size_t tsize) {
       maxns = std::max(maxns, S.ns[s]);
       rows += S.ns[s] + S.rs[s];
     }
-    if (pick_tc(maxnf, maxns, tsize) == 0 || maxns > 128) best = std::max(best, rows);
+    if (pick_tc(maxnf, maxns, tsize, P->tc_cap) == 0 || maxns > 128) best = std::max(best, rows);
   }
   return best;
 }
@@ -2496,7 +2500,7 @@ template <typename T>
 bool bwd_wave(dcx_plan* P, T* W, int k, int maxns, const int* lvl, int cnt, const int* active,
               const SweepIO& io, hipStream_t st) {
   static const int use = env_int("DCX_BWD_WAVE", 1);
-  static const int min_waves = env_int("DCX_BWD_SPLITK_BELOW", 4096);
+  const int min_waves = P->splitk_below;
   if (!use) return false;
   constexpr int VW = 16 / sizeof(T);  // 16-byte vectors
   const int64_t waves = static_cast<int64_t>(cnt) * P->nbatch * ((k + 16 * VW - 1) / (16 * VW));
@@ -2526,11 +2530,7 @@ void sweep_tc(dcx_plan* P, T* W, int k, int maxnf, int maxns, int maxpanel, bool
 }
 
 // widest column tile whose LDS block fits and whose register tiles fit
-int pick_tc(int maxnf, int maxns, size_t tsize) {
-  static const int cap = [] {
-    const char* e = std::getenv("DCX_SWEEP_TC");
-    return e ? std::atoi(e) : 32;
-  }();
+int pick_tc(int maxnf, int maxns, size_t tsize, int cap) {
   for (int tc : {64, 32, 16}) {
     if (tc > cap) continue;
     if (static_cast<size_t>(maxnf) * tc * tsize <= kLdsBudget) return tc;
@@ -2672,7 +2672,7 @@ int sweep_level(dcx_plan* P, T* W, int k, bool fwd, int d, const int* act, const
     maxpanel = std::max(maxpanel, (S.ns[s] + S.rs[s]) * S.ns[s]);
   }
   if (!fwd && bwd_wave<T>(P, W, k, maxns, lvl, cnt, act, io, st)) return DCX_OK;
-  const int tc = pick_tc(maxnf, maxns, sizeof(T));
+  const int tc = pick_tc(maxnf, maxns, sizeof(T), P->tc_cap);
   if (!tc) {
     if (fwd) return fwd_big<T>(P, W, k, d, lvl, cnt, maxnf, act, io, st);
     return fail(DCX_ERR_ARG, "front too large for the in-LDS sweep");
@@ -2817,6 +2817,9 @@ int dcx_plan_create(int n, const int32_t* crow, const int32_t* col, int nbatch,
     P->nbatch = nbatch;
     P->precision = precision;
     P->tsize = precision == DCX_FP32 ? 4 : 8;
+    P->tc_cap = env_int("DCX_SWEEP_TC", 32);
+    P->splitk_below = env_int("DCX_BWD_SPLITK_BELOW", 4096);
+    P->wide_below = env_int("DCX_FACTOR_WIDE_BELOW", 321);
     {
       int rc = upload_fused(P.get());
       if (rc) return rc;
@@ -2919,6 +2922,39 @@ int dcx_plan_get_perm(const dcx_plan* P, int32_t* perm) {
 int dcx_plan_set_refinement(dcx_plan* P, int passes) {
   if (!P || passes < 0 || passes > 16) return fail(DCX_ERR_ARG, "passes must be 0..16");
   P->refine = passes;
+  return DCX_OK;
+}
+
+int dcx_plan_set_tuning(dcx_plan* P, int key, int value) {
+  if (!P) return fail(DCX_ERR_ARG, "null argument");
+  switch (key) {
+    case DCX_TUNE_SWEEP_TC:
+      if (value != 16 && value != 32 && value != 64)
+        return fail(DCX_ERR_ARG, "sweep column tile must be 16, 32 or 64");
+      P->tc_cap = value;
+      break;
+    case DCX_TUNE_SPLITK_BELOW:
+      if (value < 0) return fail(DCX_ERR_ARG, "split-K threshold must be >= 0");
+      P->splitk_below = value;
+      break;
+    case DCX_TUNE_WIDE_BELOW:
+      if (value < 0) return fail(DCX_ERR_ARG, "wide-factor threshold must be >= 0");
+      P->wide_below = value;
+      break;
+    default:
+      return fail(DCX_ERR_ARG, "unknown tuning key");
+  }
+  return DCX_OK;
+}
+
+int dcx_plan_get_tuning(const dcx_plan* P, int key, int* value) {
+  if (!P || !value) return fail(DCX_ERR_ARG, "null argument");
+  switch (key) {
+    case DCX_TUNE_SWEEP_TC: *value = P->tc_cap; break;
+    case DCX_TUNE_SPLITK_BELOW: *value = P->splitk_below; break;
+    case DCX_TUNE_WIDE_BELOW: *value = P->wide_below; break;
+    default: return fail(DCX_ERR_ARG, "unknown tuning key");
+  }
   return DCX_OK;
 }
 

@@ -20,8 +20,12 @@ launch economy:
   segmentation fault inside hipGraphLaunch whose cause is not found;
 * autotuning: `autotune_leaf` times the nested-dissection leaf size per size
   class (cells rounded to a power of two) on one line of that class and
-  `LineBatch(leaf="auto")` builds every line with its class's choice.
-  Tuning is not part of a timed step.
+  `LineBatch(leaf="auto")` builds every line with its class's choice;
+  `autotune_tiles` then times the kernel-variant thresholds of a plan (sweep
+  column tile, split-K threshold of the backward sweep, wide-factor
+  threshold; `dcx_plan_set_tuning`) by coordinate search, per size class in
+  `LineBatch(tiles="auto")` and for the one-plan batch
+  (`LineForest.autotune_tiles`).  Tuning is not part of a timed step.
 
 The nearest reference analogue is cuDSS's batched sequence (dctorch/
 solver.py:181-185), which needs one pattern; there is no uneven-batch code in
@@ -48,6 +52,63 @@ def _eval_grad(eng, x, dobs, w):
     phi = eng.misfit(x, dobs, w)
     phi.backward()
     return phi, x.grad
+
+
+#: candidate values of each kernel-variant threshold (dcx_plan_set_tuning)
+TILE_GRID = {"sweep_tc": (16, 32, 64), "splitk_below": (1024, 4096, 16384),
+             "wide_below": (0, 321, 4096)}
+
+
+def tile_key(setting):
+    """Canonical text of a tuning setting (the keys of the tuners' timings)."""
+    return ",".join(f"{k}={v}" for k, v in sorted(setting.items()))
+
+
+def tune_solver_tiles(solver, step, reps=3, grid=None, device=None):
+    """Coordinate search over the plan's kernel-variant thresholds: each knob
+    in turn takes every value of its grid with the others at the best so
+    far; `step()` issues one eval+grad, timed (median of `reps` after one
+    warm-up) with device synchronisation.  Leaves the best setting on the
+    solver.  Returns (best setting, {setting: ms})."""
+    grid = TILE_GRID if grid is None else grid
+    best = solver.tuning()
+    times = {}
+
+    def timed():
+        step()
+        torch.cuda.synchronize(device)
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            step()
+            torch.cuda.synchronize(device)
+            ts.append(1e3 * (time.perf_counter() - t0))
+        return sorted(ts)[len(ts) // 2]
+
+    key = tile_key
+    t_best = times[key(best)] = timed()
+    for name, values in grid.items():
+        for v in values:
+            trial = dict(best, **{name: v})
+            if key(trial) in times:
+                continue
+            solver.set_tuning(**trial)
+            times[key(trial)] = t = timed()
+            if t < t_best:
+                best, t_best = trial, t
+        solver.set_tuning(**best)
+    return best, times
+
+
+def autotune_tiles(eng, m, dobs, w, reps=3, grid=None):
+    """`tune_solver_tiles` on one GPU engine's eval+grad (TorchDC2D or
+    TorchDC3D); the engine keeps the best setting."""
+    dev = eng.device
+    x = torch.as_tensor(m, dtype=torch.float64, device=dev)
+    d_t = torch.as_tensor(dobs, dtype=torch.float64, device=dev)
+    w_t = torch.as_tensor(w, dtype=torch.float64, device=dev)
+    return tune_solver_tiles(eng._gpu.solver, lambda: eng.eval_grad_device(x, d_t, w_t),
+                             reps=reps, grid=grid, device=dev)
 
 
 def autotune_leaf(mesh, survey, m, dobs, w, candidates=(32, 48, 64), reps=3, nky=12,
@@ -80,10 +141,13 @@ class LineBatch:
 
     lines: list of (mesh, survey); leaf: None (the solver default), an int, a
     {size_class: leaf} dict, or "auto" together with `tune_data` =
-    [(m, dobs, w) per line] to run `autotune_leaf` on one line per class."""
+    [(m, dobs, w) per line] to run `autotune_leaf` on one line per class;
+    tiles: None, a {size_class: setting} dict, or "auto" (with `tune_data`:
+    `autotune_tiles` on one line per class, its setting given to every line
+    of the class)."""
 
     def __init__(self, lines, device="cuda", nky=12, precision="mixed", refine=0, streams=4,
-                 leaf=None, tune_data=None, threads=False):
+                 leaf=None, tune_data=None, threads=False, tiles=None):
         self.device = torch.device(device)
         if self.device.type == "cuda" and self.device.index is None:
             self.device = torch.device("cuda", torch.cuda.current_device())
@@ -105,6 +169,21 @@ class LineBatch:
             lf = leaf.get(size_class(mesh.nC)) if isinstance(leaf, dict) else leaf
             self.engines.append(TorchDC2D(mesh, survey, nky=nky, device=self.device,
                                           precision=precision, refine=refine, leaf=lf))
+        self.tiles = {}
+        if tiles == "auto":
+            if tune_data is None:
+                raise ValueError("tiles='auto' needs tune_data")
+            for i, (mesh, _) in enumerate(lines):
+                c = size_class(mesh.nC)
+                if c not in self.tiles:
+                    m, d, w = tune_data[i]
+                    self.tiles[c], _ = autotune_tiles(self.engines[i], m, d, w)
+        elif isinstance(tiles, dict):
+            self.tiles = dict(tiles)
+        for (mesh, _), eng in zip(lines, self.engines):
+            t = self.tiles.get(size_class(mesh.nC))
+            if t:
+                eng._gpu.solver.set_tuning(**t)
         self.streams = [torch.cuda.Stream(self.device) for _ in range(max(1, streams))]
         self._pool = ThreadPoolExecutor(max_workers=len(self.streams)) if threads else None
         self._data = None
@@ -378,6 +457,14 @@ class LineForest:
 
     def _stream(self):
         return self._libm.stream_of(self.device)
+
+    def autotune_tiles(self, ms, reps=3, grid=None):
+        """`tune_solver_tiles` on this plan's eval+grad at the models `ms`
+        (set_data first); the plan keeps the best setting."""
+        if self._data is None:
+            raise RuntimeError("set_data first")
+        return tune_solver_tiles(self.solver, lambda: self.eval_grad(ms), reps=reps, grid=grid,
+                                 device=self.device)
 
     def set_data(self, dobs, w):
         """Observed data and weights per line (stacked on the device)."""

@@ -115,6 +115,26 @@ class HipBatchSolver:
             _lib.check(self._lib.dcx_plan_set_fusion(self._h, int(rows)), "set_fusion")
         self.fuse_rows = int(rows)
 
+    _TUNE_KEYS = {"sweep_tc": 1, "splitk_below": 2, "wide_below": 3}
+
+    def set_tuning(self, **values):
+        """Kernel-variant thresholds of this plan: sweep_tc (16, 32, 64),
+        splitk_below, wide_below (see dcx_plan_set_tuning)."""
+        for name, v in values.items():
+            if name not in self._TUNE_KEYS:
+                raise ValueError(f"unknown tuning key {name!r}")
+            _lib.check(self._lib.dcx_plan_set_tuning(self._h, self._TUNE_KEYS[name], int(v)),
+                       "plan_set_tuning")
+
+    def tuning(self):
+        out = {}
+        for name, key in self._TUNE_KEYS.items():
+            v = C.c_int()
+            _lib.check(self._lib.dcx_plan_get_tuning(self._h, key, C.byref(v)),
+                       "plan_get_tuning")
+            out[name] = v.value
+        return out
+
     def set_timing(self, enable=True):
         """Record HIP events around the sweeps of every solve (on its stream)."""
         _lib.check(self._lib.dcx_plan_set_timing(self._h, int(bool(enable))), "set_timing")

@@ -198,3 +198,35 @@ def test_line_forest_matches_lines(precision, tol_phi, tol_g):
                 gi = out[i][1].cpu().numpy()
                 assert np.linalg.norm(gi - g) <= tol_g * np.linalg.norm(g), (name, rep, i)
         lf.free()
+
+
+@pytest.mark.gpu
+def test_tile_tuning_keeps_results_and_contract():
+    """Every kernel-variant setting of the tuning grid gives the default
+    setting's FP64 eval+grad (same kernels up to summation order: 1e-12);
+    the tuner returns a setting of its grid, leaves it on the plan, and
+    timed every setting it reports; bad values are refused."""
+    from simpeg_dc_torch_amd import TorchDC2D
+    from simpeg_dc_torch_amd.lines import TILE_GRID, autotune_tiles, tile_key
+    print("tile tuning", flush=True)
+    mesh, survey, m = _lines(1, (6000, 9000), seed=3)[0]
+    eng = TorchDC2D(mesh, survey, nky=6, device="cuda")
+    with torch.no_grad():
+        d = eng.dpred(torch.tensor(m, device="cuda")).cpu().numpy()
+    dobs, w = noisy_data(d, seed=4)
+    args = [torch.tensor(a, device="cuda") for a in (m, dobs, w)]
+    phi0, g0 = eng.eval_grad_device(*args)
+    sv = eng._gpu.solver
+    default = sv.tuning()
+    for name, values in TILE_GRID.items():
+        for v in values:
+            sv.set_tuning(**dict(default, **{name: v}))
+            phi, g = eng.eval_grad_device(*args)
+            assert float(phi) == pytest.approx(float(phi0), rel=1e-12), (name, v)
+            assert float((g - g0).norm() / g0.norm()) < 1e-12, (name, v)
+    sv.set_tuning(**default)
+    best, times = autotune_tiles(eng, m, dobs, w, reps=1)
+    assert sv.tuning() == best and tile_key(best) in times
+    assert all(best[k] in TILE_GRID[k] or best[k] == default[k] for k in best)
+    with pytest.raises(RuntimeError):
+        sv.set_tuning(sweep_tc=48)
