@@ -15,7 +15,7 @@ from .solver import (CuDSSBatchSolver, CuDSSSolver, HipBatchSolver,  # noqa: F40
 from .simulation2d import TorchDC2D  # noqa: F401
 from .linearization2d import Linearization2D  # noqa: F401
 from .simulation3d import TorchDC3D  # noqa: F401
-from .ip import TorchDCIP2D, TorchIP2D, TorchIP3D  # noqa: F401
+from .ip import TorchCR2D, TorchDCIP2D, TorchIP2D, TorchIP3D  # noqa: F401
 from .mesh import TensorMesh  # noqa: F401
 from . import survey  # noqa: F401
 from . import gauss_newton  # noqa: F401

@@ -28,6 +28,10 @@ d_IP = ip_scale (V1 - V0) / V1, d_DC = V1 ("total") or V0
 `TorchDC2D` engines (one plan each, m_eff = m_rho - log(1 - eta)), so both
 forward solves and both adjoint solves are the HIP eval+grad path and autograd
 differentiates both blocks of the model.
+
+`TorchCR2D` is the complex-resistivity forward of dctorch/cr2d.py
+(sigma* = sigma (1 - i m), complex-symmetric solves) on the cpu device only:
+libdcx.so factors real SPD systems, so a GPU device raises.
 """
 import numpy as np
 import torch
@@ -282,3 +286,81 @@ class TorchDCIP2D:
         r0 = w_dc * (d_dc - dobs_dc)
         r1 = w_ip * (d_ip - dobs_ip)
         return (r0 ** 2).sum() + lam * (r1 ** 2).sum()
+
+
+class TorchCR2D:
+    """Complex-resistivity 2.5-D nodal forward (dctorch/cr2d.py:13-108):
+    sigma* = sigma_DC (1 - i m), one complex-symmetric solve per wavenumber,
+    d_rho = |V*| ("magnitude") or Re V* ("real"), d_ip = ip_scale Im V* /
+    Re V*; model cat(log rho_DC, m) on the model cells.
+
+    The cpu device only: the complex-symmetric solves are SciPy SuperLU (as
+    the reference's cpu path), through the complex-safe autograd op
+    `sparse_solve_batch`.  libdcx.so has no complex-symmetric factorisation,
+    so a GPU device raises NotImplementedError (no fallback)."""
+
+    def __init__(self, mesh, survey, nky=12, device="cuda", dtype=torch.complex128,
+                 active_cells=None, val_inactive=None, ip_scale=1.0, dc_datum="magnitude",
+                 quadrature=None):
+        from .simulation2d import TorchDC2D
+        from .solver import make_batch_solver
+        if torch.device(device).type != "cpu":
+            raise NotImplementedError(
+                "TorchCR2D needs a complex-symmetric factorisation, which libdcx.so does "
+                "not have: use device='cpu'")
+        if dc_datum not in ("magnitude", "real"):
+            raise ValueError(f"dc_datum must be magnitude|real, got {dc_datum!r}")
+        rdt = torch.float64 if dtype == torch.complex128 else torch.float32
+        dc = TorchDC2D(mesh, survey, nky=nky, device=device, dtype=rdt,
+                       active_cells=active_cells, val_inactive=val_inactive,
+                       quadrature=quadrature)
+        self.dc = dc
+        self.device, self.dtype, self.rdtype = dc.device, dtype, rdt
+        self.nky, self.nD = dc.nky, dc.nD
+        self.nP = dc.nC if dc._P_inj is None else int(dc._act.numel())
+        self.ip_scale = float(ip_scale)
+        self.dc_datum = dc_datum
+        self.solver = make_batch_solver(dc._crow, dc._ccol, dc.nN, dc.nS, dc.nky,
+                                        device=self.device, dtype=dtype,
+                                        matrix_type="symmetric")
+        self.q = dc.q.to(dtype)
+
+    def _inject_ip(self, m_ip):
+        if self.dc._P_inj is None:
+            return m_ip
+        return torch.zeros(self.dc.nC, dtype=m_ip.dtype,
+                           device=self.device).index_put((self.dc._act,), m_ip)
+
+    def split(self, m):
+        m = m.reshape(-1)
+        if m.numel() != 2 * self.nP:
+            raise ValueError(f"model must be 2*{self.nP} long, got {m.numel()}")
+        return m[:self.nP], m[self.nP:]
+
+    def dpred_from_fields(self, m_rho, m_ip):
+        """(d_rho, d_ip) from log-rho_DC and chargeability m (model cells)."""
+        from .solver import sparse_solve_batch
+        dc = self.dc
+        dev0, dt0 = m_rho.device, m_rho.dtype
+        mr = m_rho.to(device=self.device, dtype=self.rdtype).reshape(-1)
+        mi = m_ip.to(device=self.device, dtype=self.rdtype).reshape(-1)
+        sig = torch.exp(-dc._model_to_full(mr))
+        chg = self._inject_ip(mi)
+        vre = torch.sparse.mm(dc.Lstack, sig.unsqueeze(1)).view(dc.nky, dc.nnzA)
+        vim = torch.sparse.mm(dc.Lstack, (-sig * chg).unsqueeze(1)).view(dc.nky, dc.nnzA)
+        U = sparse_solve_batch(torch.complex(vre, vim), self.q, self.solver)
+        Ubar = (dc.wq.to(self.dtype).view(-1, 1, 1) * U).sum(0)
+        Vr = torch.sparse.mm(dc.Pd, Ubar.real)[dc.iD, dc.sidx]
+        Vi = torch.sparse.mm(dc.Pd, Ubar.imag)[dc.iD, dc.sidx]
+        d_rho = torch.sqrt(Vr ** 2 + Vi ** 2) if self.dc_datum == "magnitude" else Vr
+        d_ip = self.ip_scale * Vi / Vr
+        return d_rho.to(device=dev0, dtype=dt0), d_ip.to(device=dev0, dtype=dt0)
+
+    def dpred(self, m):
+        """(d_rho, d_ip) for the stacked model m = cat(log rho_DC, m)."""
+        return self.dpred_from_fields(*self.split(m))
+
+    def misfit(self, m, dobs_rho, w_rho, dobs_ip, w_ip, lam=1.0):
+        """phi_d^rho + lam phi_d^IP (no 1/2)."""
+        d_r, d_i = self.dpred(m)
+        return ((w_rho * (d_r - dobs_rho)) ** 2).sum() + lam * ((w_ip * (d_i - dobs_ip)) ** 2).sum()

@@ -352,14 +352,18 @@ class _SparseSolveFn(torch.autograd.Function):
 
     @staticmethod
     def backward(ctx, grad_x):
+        # complex-symmetric A (A^T = A): torch passes dL/d conj(x), so
+        # adj = A^{-1} conj(grad_x), dL/d conj(b) = conj(adj) and
+        # dL/d conj(A_ij) = -conj(adj_i x_j) (solver.py:236-281); every conj
+        # is a no-op on real data
         (x,) = ctx.saved_tensors
         solver = ctx.solver
-        adj = solver.solve(grad_x.contiguous())
+        adj = solver.solve(grad_x.conj().resolve_conj().contiguous())
         grad_vals = None
         if ctx.needs_input_grad[0]:
             g = _gather_grad(solver, adj.reshape(solver.n, -1), x.reshape(solver.n, -1))
-            grad_vals = g.reshape(-1).to(ctx.dt_vals)
-        grad_b = adj.to(ctx.dt_b) if ctx.needs_input_grad[1] else None
+            grad_vals = g.reshape(-1).conj().resolve_conj().to(ctx.dt_vals)
+        grad_b = adj.conj().resolve_conj().to(ctx.dt_b) if ctx.needs_input_grad[1] else None
         return grad_vals, grad_b, None
 
 
@@ -384,15 +388,16 @@ class _SparseSolveBatchFn(torch.autograd.Function):
 
     @staticmethod
     def backward(ctx, grad_x):
+        # complex-symmetric A: see _SparseSolveFn.backward
         (x,) = ctx.saved_tensors
         solver = ctx.solver
-        adj = solver.solve(grad_x.contiguous())
+        adj = solver.solve(grad_x.conj().resolve_conj().contiguous())
         grad_vals = None
         if ctx.needs_input_grad[0]:
-            grad_vals = _gather_grad(solver, adj, x).to(ctx.dt_vals)
+            grad_vals = _gather_grad(solver, adj, x).conj().resolve_conj().to(ctx.dt_vals)
         grad_b = None
         if ctx.needs_input_grad[1]:
-            grad_b = (adj.sum(0) if ctx.b_shared else adj).to(ctx.dt_b)
+            grad_b = (adj.sum(0) if ctx.b_shared else adj).conj().resolve_conj().to(ctx.dt_b)
         return grad_vals, grad_b, None
 
 

@@ -12,7 +12,11 @@ What pins each piece:
   and the "total" DC datum is V0 / (1 - eta); its gradient against central
   differences of its own misfit; to first order in eta it equals the
   linearised IP (the difference is O(eta)).
-* GPU (marked gpu): every class on the HIP path against the same class on
+* TorchCR2D (cpu): m = 0 gives the DC data; a uniform m gives V* = V_dc /
+  (1 - i m) exactly (phase ratio m, magnitude |V_dc| / sqrt(1 + m^2)); the
+  complex-safe autograd solve against central differences; a GPU device is
+  refused.
+* GPU (marked gpu): every GPU class on the HIP path against the same class on
   the CPU path.
 """
 import numpy as np
@@ -195,3 +199,48 @@ def test_dcip_gpu_matches_cpu():
         outs.append((d_dc.detach(), d_ip.detach(), x.grad))
     for a, b in zip(outs[1], outs[0]):
         assert _rel(a, b) < 1e-9
+
+
+def test_cr2d_zero_and_uniform_chargeability_closed_forms():
+    """m = 0: the DC data (Re V* = V_dc, Im V* = 0).  Uniform m: sigma* =
+    sigma (1 - i m) everywhere, so V* = V_dc / (1 - i m): d_ip = ip_scale m
+    and |V*| = |V_dc| / sqrt(1 + m^2), exactly."""
+    from simpeg_dc_torch_amd import TorchCR2D, TorchDC2D
+    mesh, survey, m = case("suite")
+    v_dc = TorchDC2D(mesh, survey, nky=6, device="cpu").dpred(torch.tensor(m)).detach()
+    cr = TorchCR2D(mesh, survey, nky=6, device="cpu", ip_scale=1000.0, dc_datum="real")
+    d_r, d_i = cr.dpred(torch.tensor(np.r_[m, np.zeros(mesh.nC)]))
+    assert float((d_r - v_dc).norm() / v_dc.norm()) < 1e-12 and float(d_i.abs().max()) < 1e-12
+    crm = TorchCR2D(mesh, survey, nky=6, device="cpu", ip_scale=1000.0)
+    d_r, d_i = crm.dpred(torch.tensor(np.r_[m, np.full(mesh.nC, 0.05)]))
+    assert float((d_i - 50.0).abs().max()) < 1e-9
+    ref = v_dc.abs() / np.sqrt(1 + 0.05 ** 2)
+    assert float((d_r - ref).norm() / ref.norm()) < 1e-12
+
+
+def test_cr2d_gradient_vs_fd():
+    """The complex-safe autograd solve (Wirtinger adjoint on the same
+    factorisation) against central differences of the real misfit."""
+    from simpeg_dc_torch_amd import TorchCR2D
+    mesh, survey, m = case("suite")
+    cr = TorchCR2D(mesh, survey, nky=6, device="cpu", ip_scale=1000.0)
+    rng = np.random.default_rng(7)
+    x = torch.tensor(np.r_[m, 0.02 + 0.05 * rng.random(mesh.nC)])
+    d_r, d_i = cr.dpred(x)
+    args = [torch.tensor(a) for a in (*noisy_data(d_r.detach().numpy(), seed=1),
+                                      *noisy_data(d_i.detach().numpy(), seed=2))]
+    xg = x.clone().requires_grad_(True)
+    cr.misfit(xg, *args).backward()
+    for _ in range(3):
+        v = torch.tensor(rng.standard_normal(x.numel()))
+        h = 1e-6
+        with torch.no_grad():
+            fd = (cr.misfit(x + h * v, *args) - cr.misfit(x - h * v, *args)) / (2 * h)
+        assert float(xg.grad @ v) == pytest.approx(float(fd), rel=1e-5)
+
+
+def test_cr2d_refuses_gpu():
+    from simpeg_dc_torch_amd import TorchCR2D
+    mesh, survey, _ = case("suite")
+    with pytest.raises(NotImplementedError):
+        TorchCR2D(mesh, survey, nky=6, device="cuda")
