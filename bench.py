@@ -164,7 +164,7 @@ def linearization_times(eng, mt, reps=3):
             "precision": "fp64"}
 
 
-def grid3d_times(steps, warmup):
+def grid3d_times(steps, warmup, tune=True):
     """The 3-D nodal engine (TorchDC3D) on a 56 x 56 x 38-cell grid with 200
     pole sources and 6,000 pole-dipole data: FP64 and mixed eval+grad ms and
     the mixed result's error against FP64 (one system, fronts up to 3,315
@@ -190,12 +190,14 @@ def grid3d_times(steps, warmup):
             info = eng.solver.info()
             out.update(cells=eng.nC, nodes=eng.nN, sources=eng.nS, data=eng.nD,
                        max_front=info["max_front"], nnz_L=info["nnz_L"])
+        key = prec if prec == "fp64" else f"mixed_refine{refine}"
+        if tune:
+            out.setdefault("tiles", {})[key] = tune_tiles({key: eng}, mt, dobs, w)[0][key]
         ms = time_steps(eng, mt, dobs, w, steps, warmup)
         with torch.no_grad():
             d = eng.dpred(mt)
         _, g = eval_grad(eng, mt, dobs, w)
         jt = eng.Jtvec(mt, v)
-        key = prec if prec == "fp64" else f"mixed_refine{refine}"
         out[f"{key}_ms"] = round(ms, 2)
         if ref is None:
             ref = (d, g, jt)
@@ -569,7 +571,8 @@ def main():
         return
     if args.workload == "grid3d":
         log("3-D grid only (TorchDC3D, 56 x 56 x 38 cells, 200 sources)")
-        print(json.dumps({"workload": "grid3d", **grid3d_times(args.steps, args.warmup)}),
+        print(json.dumps({"workload": "grid3d",
+                          **grid3d_times(args.steps, args.warmup, tune=not args.no_tune)}),
               flush=True)
         return
     if args.workload == "gn":
@@ -630,7 +633,7 @@ def main():
     g3 = lb = rc = gn = ipt = None
     if args.workload == "all":
         log("3-D grid (TorchDC3D, 56 x 56 x 38 cells, 200 sources)")
-        g3 = grid3d_times(args.steps, args.warmup)
+        g3 = grid3d_times(args.steps, args.warmup, tune=not args.no_tune)
         log(f"3-D {g3}")
         log("uneven batch (16 lines, 3k-40k cells, leaf autotuned per size class)")
         lb = lines_times(args.steps, args.warmup)

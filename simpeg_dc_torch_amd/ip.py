@@ -86,6 +86,38 @@ class _FixedSolve(torch.autograd.Function):
         return ctx.solver.solve(g.contiguous()), None
 
 
+#: entries per gather chunk of the cpu dA U product (about 64 MB of FP64)
+_SPMV_CHUNK = 8_000_000
+
+
+class _SpmvFixedU(torch.autograd.Function):
+    """rhs = dA(vals) U with U fixed: rhs[r] = sum_p vals[p] U[col_p] over the
+    pattern rows; forward and backward in chunks, so nothing nnz x nS-sized
+    is kept (the role of dctorch/ip3d.py:36-65)."""
+
+    @staticmethod
+    def forward(ctx, vals, U, row, col):
+        k = U.shape[1]
+        rhs = torch.zeros_like(U)
+        step = max(_SPMV_CHUNK // max(k, 1), 1)
+        for p0 in range(0, vals.numel(), step):
+            p1 = min(p0 + step, vals.numel())
+            rhs.index_add_(0, row[p0:p1], vals[p0:p1].detach().unsqueeze(1) * U[col[p0:p1]])
+        ctx.save_for_backward(U, row, col)
+        return rhs
+
+    @staticmethod
+    def backward(ctx, grad_rhs):
+        U, row, col = ctx.saved_tensors
+        k = U.shape[1]
+        g = torch.empty(row.numel(), dtype=U.dtype, device=U.device)
+        step = max(_SPMV_CHUNK // max(k, 1), 1)
+        for p0 in range(0, row.numel(), step):
+            p1 = min(p0 + step, row.numel())
+            g[p0:p1] = (grad_rhs[row[p0:p1]] * U[col[p0:p1]]).sum(1)
+        return g, None, None, None
+
+
 class _IPBase:
     def _finish(self, survey, v_dc):
         self.v_dc = v_dc
@@ -185,8 +217,7 @@ class TorchIP3D(_IPBase):
     def _dpred_cpu(self, e_full):
         dc = self.dc
         dvals = torch.sparse.mm(dc.L3, (-self._sig_bg * e_full).unsqueeze(1)).squeeze(1)
-        rhs = torch.zeros_like(self._U).index_add(0, self._row,
-                                                   dvals.unsqueeze(1) * self._U[self._col])
+        rhs = _SpmvFixedU.apply(dvals, self._U, self._row, self._col)
         dU = _FixedSolve.apply(-rhs, dc.solver)
         return (dc.Pd @ dU)[dc.iD, dc.sidx] * self._scale
 
