@@ -457,6 +457,30 @@ def lines_times(steps, warmup, autotune=True, per_line=True):
 
     ms_seq, ms_par, ms_thr = timed(1), timed(4), timed(4, threads=True)
     one = one_plan_times(lines, ms, data, steps, warmup, compare=lb)
+
+    # FP64 engines: eager on 4 streams, then each line's eval+grad replayed
+    # as a captured HIP graph (capture is limited to FP64 engines)
+    lb64 = LineBatch(lines, precision="fp64", streams=4)
+    lb64.set_data([d for d, _ in data], [w for _, w in data])
+
+    def timed64():
+        for _ in range(warmup):
+            lb64.eval_grad(ms)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(steps):
+            lb64.eval_grad(ms)
+        torch.cuda.synchronize()
+        return 1e3 * (time.perf_counter() - t1) / steps
+
+    ms64_eager = timed64()
+    ref64 = [(float(p), g.clone()) for p, g in lb64.eval_grad(ms)]
+    lb64.capture([m.cpu().numpy() for m in ms])
+    ms64_graph = timed64()
+    out64 = lb64.eval_grad(ms)
+    torch.cuda.synchronize()
+    graph_diff = max(abs(float(p) / r[0] - 1.0) for (p, _), r in zip(out64, ref64))
+    del lb64
     ms_forest = one["one_plan_ms"]
     cells = sum(mesh.nC for mesh, _ in lines)
     return {"lines": len(lines), "cells_total": cells,
@@ -464,11 +488,14 @@ def lines_times(steps, warmup, autotune=True, per_line=True):
             "precision": "mixed", "sequential_ms": round(ms_seq, 2),
             "streams4_ms": round(ms_par, 2), "streams4_threads4_ms": round(ms_thr, 2),
             "one_plan_ms": ms_forest,
+            "fp64_streams4_ms": round(ms64_eager, 2), "fp64_graphs_ms": round(ms64_graph, 2),
+            "fp64_graph_phi_max_rel_diff": float(f"{graph_diff:.1e}"),
             "lines_per_s": round(1e3 * len(lines) / min(ms_par, ms_thr, ms_forest), 1),
             "one_plan": one["one_plan"],
             "leaf_by_size_class": {str(k): v for k, v in (lb.leaf or {}).items()},
             "tiles_by_size_class": {str(k): v for k, v in lb.tiles.items()},
-            "build_and_tune_s": round(t_tune, 1), "graph_capture": "off"}
+            "build_and_tune_s": round(t_tune, 1),
+            "graph_capture": "FP64 engines (mixed-precision capture refused)"}
 
 
 def one_plan_times(lines, ms, data, steps, warmup, compare=None):

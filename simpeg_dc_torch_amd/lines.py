@@ -13,11 +13,12 @@ launch economy:
   synchronisation), so the launch work of several lines proceeds in
   parallel on the host as well: the library's entry points are C calls,
   which run without the interpreter lock;
-* graphs: `capture()` would record each line's whole eval+grad as a HIP
-  graph through `torch.cuda.CUDAGraph` and replay it; it is switched off and
-  raises: replays of FP64 graphs matched the eager results bit for bit, but
-  a replay of mixed-precision graphs ended the process with a host
-  segmentation fault inside hipGraphLaunch whose cause is not found;
+* graphs: `capture()` records each line's whole eval+grad as a HIP graph
+  through `torch.cuda.CUDAGraph` and `eval_grad` then replays them, for FP64
+  engines only: replays of FP64 graphs match the eager results bit for bit,
+  while a replay of mixed-precision graphs ended the process with a host
+  segmentation fault inside hipGraphLaunch whose cause is not found, so
+  capture refuses mixed-precision engines;
 * autotuning: `autotune_leaf` times the nested-dissection leaf size per size
   class (cells rounded to a power of two) on one line of that class and
   `LineBatch(leaf="auto")` builds every line with its class's choice;
@@ -200,19 +201,21 @@ class LineBatch:
         return self.streams[i % len(self.streams)]
 
     def capture(self, ms, warmup=2):
-        """Switched off: raises NotImplementedError.  A replay of the
-        mixed-precision graphs that `_capture_graphs` records ended the process
-        with a host segmentation fault inside hipGraphLaunch (FP64 replays
-        matched the eager results bit for bit); the cause is not found."""
-        raise NotImplementedError(
-            "graph capture of the eval+grad is switched off: a replay of mixed-precision "
-            "graphs ended in a host segmentation fault inside hipGraphLaunch (cause not found)")
+        """Record each line's eval+grad at models `ms` as a HIP graph (FP64
+        engines only); later `eval_grad` calls replay them.  Mixed-precision
+        engines are refused: a replay of their graphs ended the process with
+        a host segmentation fault inside hipGraphLaunch, cause not found."""
+        if any(e.precision != "fp64" for e in self.engines):
+            raise NotImplementedError(
+                "graph capture is limited to FP64 engines: a replay of mixed-precision "
+                "graphs ended in a host segmentation fault inside hipGraphLaunch "
+                "(cause not found)")
+        self._capture_graphs(ms, warmup)
 
     def _capture_graphs(self, ms, warmup=2):
         """Record each line's eval+grad at models `ms` as a graph (after
         `warmup` eager runs on its stream, which also sizes every buffer the
-        library keeps, so nothing is allocated during capture).  Not called:
-        see `capture`."""
+        library keeps, so nothing is allocated during capture)."""
         if self._data is None:
             raise RuntimeError("set_data first")
         graphs = []

@@ -77,11 +77,13 @@ def test_line_batch_streams_match_single(precision, threads):
             assert np.abs(out[i][1].cpu().numpy() - g).max() <= 1e-13 * np.abs(g).max(), (rep, i)
 
 
+_MIXED_GRAPHS = pytest.mark.skip(
+    reason="capture refuses mixed-precision engines: a replay of their graphs ended the "
+           "process with a host segmentation fault inside hipGraphLaunch, cause not found")
+
+
 @pytest.mark.gpu
-@pytest.mark.skip(reason="graph capture is switched off: a replay of the mixed-precision "
-                         "graphs ended the process with a host segmentation fault inside "
-                         "hipGraphLaunch, cause not found")
-@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("precision", ["fp64", pytest.param("mixed", marks=_MIXED_GRAPHS)])
 def test_line_batch_graph_replays(precision):
     """Capture at one model, then replay at three models, each replay
     preceded by eager eval+grad work and a synchronisation; every replay
@@ -101,6 +103,16 @@ def test_line_batch_graph_replays(precision):
             assert float(out[i][0]) == pytest.approx(ref[i][0], rel=1e-13), (rep, i)
             g = out[i][1].cpu().numpy()
             assert np.abs(g - ref[i][1]).max() <= 1e-13 * np.abs(ref[i][1]).max(), (rep, i)
+
+
+@pytest.mark.gpu
+def test_capture_refuses_mixed_engines():
+    from simpeg_dc_torch_amd.lines import LineBatch
+    print("capture refuses mixed", flush=True)
+    L = _lines(1)
+    lb = LineBatch([(mesh, survey) for mesh, survey, _ in L], nky=6, precision="mixed")
+    with pytest.raises(NotImplementedError):
+        lb.capture([m for _, _, m in L])
 
 
 @pytest.mark.gpu
