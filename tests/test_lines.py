@@ -242,3 +242,34 @@ def test_tile_tuning_keeps_results_and_contract():
     assert all(best[k] in TILE_GRID[k] or best[k] == default[k] for k in best)
     with pytest.raises(RuntimeError):
         sv.set_tuning(sweep_tc=48)
+
+
+@pytest.mark.gpu
+def test_line_forest_full_batch_fp64():
+    """The full uneven workload (16 lines of 3k-31k cells, 251k cells, 18-81
+    sources) in one FP64 plan against each line's own FP64 engine (the
+    sync-free eval+grad), at 5% noise."""
+    from simpeg_dc_torch_amd import TorchDC2D, workloads
+    from simpeg_dc_torch_amd.lines import LineForest
+    print("line forest full batch fp64", flush=True)
+    L = workloads.uneven_lines_case()
+    lines = [(mesh, survey) for mesh, survey, _ in L]
+    lf = LineForest(lines, precision="fp64")
+    data, refs = [], []
+    for mesh, survey, m in L:
+        eng = TorchDC2D(mesh, survey, nky=12, device="cuda")
+        x = torch.tensor(m, device="cuda")
+        with torch.no_grad():
+            d = eng.dpred(x).cpu().numpy()
+        dobs, w = noisy_data(d, seed=11, rel_noise=0.05)
+        data.append((dobs, w))
+        refs.append(eng.eval_grad_device(x, torch.tensor(dobs, device="cuda"),
+                                         torch.tensor(w, device="cuda")))
+        del eng
+    lf.set_data([d for d, _ in data], [w for _, w in data])
+    out = lf.eval_grad([m for _, _, m in L])
+    torch.cuda.synchronize()
+    for i, ((phi, g), (phi_r, g_r)) in enumerate(zip(out, refs)):
+        assert float(phi) == pytest.approx(float(phi_r), rel=1e-11), i
+        assert float((g - g_r).norm() / g_r.norm()) < 1e-10, i
+    lf.free()
