@@ -251,7 +251,9 @@ def _eval_grad_device(eng, m, dobs, w):
     fwd, bwd = ((_electrodes_forward, _electrodes_backward)
                 if eng.rhs_basis == "electrodes" else (_sources_forward, _sources_backward))
     with torch.no_grad(), torch.cuda.device(eng.device):
-        m_full = eng._model_to_full(m.detach()).to(torch.float64)
+        m_full = eng._model_to_full(torch.as_tensor(m).detach()).to(torch.float64)
+        dobs = torch.as_tensor(dobs, dtype=torch.float64, device=eng.device)
+        w = torch.as_tensor(w, dtype=torch.float64, device=eng.device)
         d, saved = fwd(eng, m_full)
         stamp = eng._gpu.solver.stamp
         r = w * (d - dobs)
