@@ -304,10 +304,24 @@ def tensor_regularization(mesh, alpha_s=1e-4, alphas=None):
 # ---------------------------------------------------------------------------
 # one step through an engine
 # ---------------------------------------------------------------------------
+def student_t_weights(r, w, nu):
+    """(phi_d, w dphi/dr, D) of the weighted residual r = w (d - dobs):
+    L2 (nu = 0): ||r||^2, 2 w r, 2 w^2; Student-t (nu > 0):
+    (nu + 1) sum log1p(r^2 / nu), its gradient weight w 2 (nu + 1) r /
+    (nu + r^2) and the IRLS Gauss-Newton weight 2 (nu + 1) w^2 / (nu + r^2)
+    (dctorch/optimization.py:126-131, :621-630)."""
+    if nu and nu > 0:
+        phi = float((nu + 1.0) * torch.log1p(r ** 2 / nu).sum())
+        return phi, w * (2.0 * (nu + 1.0) * r / (nu + r ** 2)), \
+            2.0 * (nu + 1.0) / (nu + r ** 2) * w ** 2
+    return float((r ** 2).sum()), w * (2.0 * r), 2.0 * w ** 2
+
+
 def gauss_newton_step(eng, m, dobs, w, Hm, beta, m_ref=None, inner="direct", hm_solve=None,
-                      cg_maxiter=40, cg_rtol=1e-2, **occam_kw):
-    """One Gauss-Newton step of phi = ||w (dpred - dobs)||^2 + beta phi_m at m
-    (FP64), with J from the engine's linearisation.
+                      cg_maxiter=40, cg_rtol=1e-2, nu=0.0, **occam_kw):
+    """One Gauss-Newton step of phi = phi_d + beta phi_m at m (FP64), with J
+    from the engine's linearisation; phi_d = ||w (dpred - dobs)||^2, or the
+    Student-t misfit with IRLS weights when nu > 0 ("occam" is L2 only).
 
     inner: "direct" (Woodbury, explicit J), "occam" (beta searched inside the
     step, explicit J), "cg" (matrix-free).  Returns a dict with the step `p`,
@@ -324,8 +338,9 @@ def gauss_newton_step(eng, m, dobs, w, Hm, beta, m_ref=None, inner="direct", hm_
     with torch.no_grad():
         d = eng.dpred(m).to(torch.float64)
     r = w * (d - dobs)
-    phi_d = float((r ** 2).sum())
-    wd, D = w * (2.0 * r), 2.0 * w ** 2
+    phi_d, wd, D = student_t_weights(r, w, nu)
+    if nu and nu > 0 and inner == "occam":
+        raise ValueError("the Occam beta search is defined for the L2 misfit (nu = 0)")
     H = sp.csr_matrix(Hm)
     Ht = torch.sparse_csr_tensor(torch.as_tensor(H.indptr, dtype=torch.int64, device=dev),
                                  torch.as_tensor(H.indices, dtype=torch.int64, device=dev),

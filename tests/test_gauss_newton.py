@@ -231,3 +231,41 @@ def test_beta_by_eig_matches_dense_eigenvalues():
     lam_d = np.linalg.eigvalsh(J.T @ np.diag(D) @ J).max()
     lam_m = np.linalg.eigvalsh(H.toarray()).max()
     assert beta == pytest.approx(100.0 * lam_d / lam_m, rel=1e-6)
+
+
+@pytest.mark.parametrize("inner", ["direct", "cg"])
+def test_student_t_step_gradient_and_weights(inner):
+    """nu > 0: the step's gradient is the autograd gradient of the Student-t
+    misfit (nu + 1) sum log1p(r^2 / nu) plus beta H_m (m - m_ref), and the
+    IRLS weight is its Gauss-Newton curvature; a huge nu gives the L2 step
+    scaled as (nu + 1) / nu -> 1."""
+    from simpeg_dc_torch_amd.gauss_newton import gauss_newton_step, student_t_weights
+    eng, m0, dobs, w, Hm = _engine_problem("cpu")
+    nu, beta = 4.0, 1.0
+    out = gauss_newton_step(eng, m0, dobs, w, Hm, beta, inner=inner, nu=nu, cg_maxiter=60,
+                            cg_rtol=1e-3)
+    x = torch.tensor(m0, requires_grad=True)
+    r = torch.tensor(w) * (eng.dpred(x) - torch.tensor(dobs))
+    phi = (nu + 1.0) * torch.log1p(r ** 2 / nu).sum()
+    phi.backward()
+    assert out["phi_d"] == pytest.approx(float(phi), rel=1e-12)
+    g_ref = x.grad.numpy() + beta * (Hm @ m0)
+    assert rel(out["g"].numpy(), g_ref) < 1e-9
+    rr = torch.tensor([0.0, 0.5, 3.0])
+    _, wd, D = student_t_weights(rr, torch.ones(3), nu)
+    # D is the derivative of the gradient weight at r = 0 and the secant
+    # form 2 (nu + 1) / (nu + r^2) elsewhere
+    assert float(D[0]) == pytest.approx(2.0 * (nu + 1.0) / nu)
+    assert torch.allclose(wd, D * rr)
+
+
+def test_lbfgs_student_t_history_is_the_robust_misfit():
+    from simpeg_dc_torch_amd import lbfgs_minimize
+    eng, m0, dobs, w, Hm = _engine_problem("cpu")
+    nu = 3.0
+    m1, hist = lbfgs_minimize(eng, m0, dobs, w, Hm=Hm, beta=1.0, max_iter=4, nu=nu)
+    with torch.no_grad():
+        r = torch.tensor(w) * (eng.dpred(torch.tensor(m0)) - torch.tensor(dobs))
+    assert hist[0][0] == pytest.approx(float((nu + 1.0) * torch.log1p(r ** 2 / nu).sum()),
+                                       rel=1e-12)
+    assert hist[-1][0] + hist[-1][1] < hist[0][0] + hist[0][1]
