@@ -304,6 +304,19 @@ def tensor_regularization(mesh, alpha_s=1e-4, alphas=None):
 # ---------------------------------------------------------------------------
 # one step through an engine
 # ---------------------------------------------------------------------------
+def data_residual(d, dobs, w, log_data=False):
+    """(r, dr/dd): r = w (d - dobs), or for log data r = w |dobs| (ln(sign(dobs)
+    d) - ln|dobs|) with dr/dd = w |dobs| / d (a sign flip clamps to 1e-300,
+    a huge misfit) (dctorch/optimization.py:103-124)."""
+    if log_data:
+        if bool((dobs == 0).any()):
+            raise ValueError("log_data needs non-zero observed data")
+        wl = w * dobs.abs()
+        sd = torch.clamp(torch.sign(dobs) * d, min=1e-300)
+        return wl * (torch.log(sd) - torch.log(dobs.abs())), wl / d
+    return w * (d - dobs), w
+
+
 def student_t_weights(r, w, nu):
     """(phi_d, w dphi/dr, D) of the weighted residual r = w (d - dobs):
     L2 (nu = 0): ||r||^2, 2 w r, 2 w^2; Student-t (nu > 0):
@@ -318,10 +331,11 @@ def student_t_weights(r, w, nu):
 
 
 def gauss_newton_step(eng, m, dobs, w, Hm, beta, m_ref=None, inner="direct", hm_solve=None,
-                      cg_maxiter=40, cg_rtol=1e-2, nu=0.0, **occam_kw):
+                      cg_maxiter=40, cg_rtol=1e-2, nu=0.0, log_data=False, **occam_kw):
     """One Gauss-Newton step of phi = phi_d + beta phi_m at m (FP64), with J
-    from the engine's linearisation; phi_d = ||w (dpred - dobs)||^2, or the
-    Student-t misfit with IRLS weights when nu > 0 ("occam" is L2 only).
+    from the engine's linearisation; phi_d = ||r||^2 of the weighted residual
+    r (`data_residual`: linear or log data), or the Student-t misfit with
+    IRLS weights when nu > 0 ("occam" is L2 only).
 
     inner: "direct" (Woodbury, explicit J), "occam" (beta searched inside the
     step, explicit J), "cg" (matrix-free).  Returns a dict with the step `p`,
@@ -337,8 +351,8 @@ def gauss_newton_step(eng, m, dobs, w, Hm, beta, m_ref=None, inner="direct", hm_
     lin = eng.linearize(m)
     with torch.no_grad():
         d = eng.dpred(m).to(torch.float64)
-    r = w * (d - dobs)
-    phi_d, wd, D = student_t_weights(r, w, nu)
+    r, lw = data_residual(d, dobs, w, log_data)
+    phi_d, wd, D = student_t_weights(r, lw, nu)
     if nu and nu > 0 and inner == "occam":
         raise ValueError("the Occam beta search is defined for the L2 misfit (nu = 0)")
     H = sp.csr_matrix(Hm)
@@ -363,7 +377,7 @@ def gauss_newton_step(eng, m, dobs, w, Hm, beta, m_ref=None, inner="direct", hm_
         out.update(p=direct_step(J, D, g, H, beta, hm_solve), g=g)
     elif inner == "occam":
         res = occam_step(J, D, g_d, g_m, r, beta, H, hm_solve,
-                         lambda rr: (rr ** 2).sum(), w, **occam_kw)
+                         lambda rr: (rr ** 2).sum(), lw, **occam_kw)
         if res is None:
             out.update(p=None, g=g)
         else:

@@ -16,15 +16,18 @@ import numpy as np
 import scipy.sparse as sp
 import torch
 
+from .gauss_newton import data_residual
+
 
 def lbfgs_minimize(eng, m0, dobs, w, Hm=None, beta=0.0, m_ref=None, max_iter=30,
                    history_size=12, lr=1.0, line_search_fn="strong_wolfe", lower=None,
-                   upper=None, nu=0.0):
+                   upper=None, nu=0.0, log_data=False):
     """Minimise phi_d + beta phi_m from m0 with one torch L-BFGS run of at
     most `max_iter` iterations; phi_d = ||w (dpred - dobs)||^2, or the
     Student-t misfit (nu + 1) sum log1p(r^2 / nu) when nu > 0
-    (dctorch/optimization.py:126-131).  Returns (m, history) with history a
-    list of (phi_d, phi_m) per closure."""
+    (dctorch/optimization.py:126-131); log_data: the residual of the
+    log-amplitudes (`gauss_newton.data_residual`).  Returns (m, history) with
+    history a list of (phi_d, phi_m) per closure."""
     dev = torch.device(eng.device)
     dt = torch.float64
     m = torch.tensor(np.asarray(m0, float), dtype=dt, device=dev, requires_grad=True)
@@ -50,9 +53,10 @@ def lbfgs_minimize(eng, m0, dobs, w, Hm=None, beta=0.0, m_ref=None, max_iter=30,
     def closure():
         opt.zero_grad()
         mc = bind(m)
-        if nu and nu > 0:
-            r = w * (eng.dpred(mc) - dobs)
-            phi_d = (nu + 1.0) * torch.log1p(r ** 2 / nu).sum()
+        if (nu and nu > 0) or log_data:
+            r, _ = data_residual(eng.dpred(mc), dobs, w, log_data)
+            phi_d = (nu + 1.0) * torch.log1p(r ** 2 / nu).sum() if nu and nu > 0 \
+                else (r ** 2).sum()
         else:
             phi_d = eng.misfit(mc, dobs, w)
         loss = phi_d

@@ -269,3 +269,21 @@ def test_lbfgs_student_t_history_is_the_robust_misfit():
     assert hist[0][0] == pytest.approx(float((nu + 1.0) * torch.log1p(r ** 2 / nu).sum()),
                                        rel=1e-12)
     assert hist[-1][0] + hist[-1][1] < hist[0][0] + hist[0][1]
+
+
+def test_log_data_step_gradient():
+    """log_data: the step's gradient is the autograd gradient of
+    ||w |dobs| (ln d - ln dobs)||^2 + beta phi_m, and near the data the log
+    residual is the relative-error L2 residual to first order."""
+    from simpeg_dc_torch_amd.gauss_newton import data_residual, gauss_newton_step
+    eng, m0, dobs, w, Hm = _engine_problem("cpu")
+    out = gauss_newton_step(eng, m0, dobs, w, Hm, 1.0, inner="direct", log_data=True)
+    x = torch.tensor(m0, requires_grad=True)
+    r, _ = data_residual(eng.dpred(x), torch.tensor(dobs), torch.tensor(w), True)
+    (r ** 2).sum().backward()
+    assert out["phi_d"] == pytest.approx(float((r.detach() ** 2).sum()), rel=1e-12)
+    assert rel(out["g"].numpy(), x.grad.numpy() + Hm @ m0) < 1e-9
+    d0 = torch.tensor(dobs)
+    r_small, lw = data_residual(d0 * (1 + 1e-6), d0, torch.tensor(w), True)
+    assert torch.allclose(r_small, torch.tensor(w) * d0.abs() * 1e-6, rtol=1e-5)
+    assert torch.allclose(lw, torch.tensor(w) * d0.abs() / (d0 * (1 + 1e-6)))
