@@ -1992,10 +1992,12 @@ __global__ void k_reduce_ky(int nnz, int nky, const double* __restrict__ ky2,
   gn[e] = c;
 }
 
-// grad_m[c] = -sigma[c] * sum over the cell's map entries (transposed CSR)
+// grad_m[c] = -sigma[c] * sum over the cell's map entries (transposed CSR;
+// tle / tln: the map coefficients in the same cell-major order, read
+// contiguously)
 __global__ void k_lmap_t(int ncell, int nnz, int nky, const int* __restrict__ tptr,
-                         const int* __restrict__ tent, const int* __restrict__ te,
-                         const double* __restrict__ le, const double* __restrict__ ln,
+                         const int* __restrict__ te,
+                         const double* __restrict__ tle, const double* __restrict__ tln,
                          const int* __restrict__ tbptr, const int* __restrict__ tbq,
                          const int* __restrict__ bc_e, const double* __restrict__ bcoef,
                          int nbc, const double* __restrict__ ge, const double* __restrict__ gn,
@@ -2005,8 +2007,8 @@ __global__ void k_lmap_t(int ncell, int nnz, int nky, const int* __restrict__ tp
   if (c >= ncell) return;
   double acc = 0.0;
   for (int t = tptr[c]; t < tptr[c + 1]; ++t) {
-    const int p = tent[t], e = te[t];
-    acc += le[p] * ge[e] + ln[p] * gn[e];
+    const int e = te[t];
+    acc += tle[t] * ge[e] + tln[t] * gn[e];
   }
   for (int t = tbptr[c]; t < tbptr[c + 1]; ++t) {
     const int q = tbq[t];
@@ -3275,13 +3277,14 @@ struct dcx_lmap {
   int *d_bptr = nullptr, *d_bcell = nullptr, *d_bc_e = nullptr;
   double* d_bcoef = nullptr;
   int *d_tptr = nullptr, *d_tent = nullptr, *d_te = nullptr, *d_tbptr = nullptr, *d_tbq = nullptr;
+  double *d_tle = nullptr, *d_tln = nullptr;  // le, ln in transposed (cell-major) order
   double *d_ge = nullptr, *d_gn = nullptr;
   int max_bc_per_entry = 0;
   double* d_jws = nullptr;  // Jacobian-rows workspace, grown on demand
   size_t jws_bytes = 0;
   ~dcx_lmap() {
     void* ptrs[] = {d_ky2, d_grp, d_ptr, d_cell, d_le, d_ln, d_bptr, d_bcell, d_bc_e, d_bcoef,
-                    d_tptr, d_tent, d_te, d_tbptr, d_tbq, d_ge, d_gn, d_jws};
+                    d_tptr, d_tent, d_te, d_tbptr, d_tbq, d_ge, d_gn, d_jws, d_tle, d_tln};
     for (void* p : ptrs)
       if (p) (void)hipFree(p);
   }
@@ -3362,6 +3365,15 @@ int dcx_lmap_create(int nnz, int ncell, int nky, const double* ky2, const int32_
       HIP_TRY(upload(&M->d_tbq, tbq));
     }
     HIP_TRY(upload(&M->d_tptr, tptr));
+    {
+      std::vector<double> tle(np), tln(np);
+      for (int t = 0; t < np; ++t) {
+        tle[t] = le[tent[t]];
+        tln[t] = ln[tent[t]];
+      }
+      HIP_TRY(upload(&M->d_tle, tle));
+      HIP_TRY(upload(&M->d_tln, tln));
+    }
     HIP_TRY(upload(&M->d_tent, tent));
     HIP_TRY(upload(&M->d_te, te));
     HIP_TRY(upload(&M->d_tbptr, tbptr));
@@ -3419,7 +3431,7 @@ int dcx_lmap_transpose(dcx_lmap* M, const double* gv, const double* sigma, doubl
   hipLaunchKernelGGL(k_reduce_ky, dim3(grid1(M->nnz, 256)), dim3(256), 0, st, M->nnz, M->nky,
                      M->d_ky2, M->d_grp, M->ngroup, gv, M->d_ge, M->d_gn);
   hipLaunchKernelGGL(k_lmap_t, dim3(grid1(M->ncell, 256)), dim3(256), 0, st, M->ncell, M->nnz,
-                     M->nky, M->d_tptr, M->d_tent, M->d_te, M->d_le, M->d_ln, M->d_tbptr,
+                     M->nky, M->d_tptr, M->d_te, M->d_tle, M->d_tln, M->d_tbptr,
                      M->d_tbq, M->d_bc_e, M->d_bcoef, M->nbc, M->d_ge, M->d_gn, gv, sigma,
                      grad_m);
   HIP_TRY(hipGetLastError());
