@@ -39,7 +39,8 @@ enum {
 /* factor precision */
 enum { DCX_FP64 = 0, DCX_FP32 = 1 };
 /* keys of dcx_plan_set_tuning / dcx_plan_get_tuning */
-enum { DCX_TUNE_SWEEP_TC = 1, DCX_TUNE_SPLITK_BELOW = 2, DCX_TUNE_WIDE_BELOW = 3 };
+enum { DCX_TUNE_SWEEP_TC = 1, DCX_TUNE_SPLITK_BELOW = 2, DCX_TUNE_WIDE_BELOW = 3,
+       DCX_TUNE_PANEL_WIDTH = 4 };
 
 /* dcx_solve flags */
 enum {
@@ -114,7 +115,8 @@ int dcx_plan_sweep_times(dcx_plan* plan, double* fwd_ms, double* bwd_ms, int* ns
  * dcx_plan_create): DCX_TUNE_SWEEP_TC, the widest column tile of the in-LDS
  * sweeps (16, 32, 64); DCX_TUNE_SPLITK_BELOW, backward-sweep levels with fewer
  * wavefronts take the split-K variant; DCX_TUNE_WIDE_BELOW, factor levels with
- * fewer workgroups take 1024-thread fronts.  Every value selects among
+ * fewer workgroups take 1024-thread fronts; DCX_TUNE_PANEL_WIDTH (16 or 32),
+ * columns per panel of the in-LDS factor kernel.  Every value selects among
  * kernels that are correct for any level; only speed and summation order
  * change.  No counterpart in the reference (cuDSS chooses internally behind
  * dctorch/solver.py:187-213); a tile autotuner times them per size class. */

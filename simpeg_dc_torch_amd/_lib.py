@@ -15,7 +15,7 @@ LIB_PATH = os.environ.get("DCX_LIBRARY") or os.path.join(_HERE, "libdcx.so")
 
 DCX_FP64, DCX_FP32 = 0, 1
 DCX_SHARED_RHS, DCX_PERMUTED, DCX_SPARSE_RHS, DCX_F32_FIELDS = 1, 2, 4, 8
-DCX_TUNE_SWEEP_TC, DCX_TUNE_SPLITK_BELOW, DCX_TUNE_WIDE_BELOW = 1, 2, 3
+DCX_TUNE_SWEEP_TC, DCX_TUNE_SPLITK_BELOW, DCX_TUNE_WIDE_BELOW, DCX_TUNE_PANEL_WIDTH = 1, 2, 3, 4
 
 _i32p = C.POINTER(C.c_int32)
 _f64p = C.POINTER(C.c_double)

@@ -216,15 +216,17 @@ __device__ inline void pivot(float d, float& sq, float& dinv) {
 // the jb x jb diagonal block of a panel, by one wavefront: lanes < jb hold its
 // rows in rv; each pivot column is broadcast through ldg's first row (free
 // until the end) and read back as vectors; the L rows go to ldg (stride
-// kLDG) and the reciprocal diagonal to dgi
-template <typename T>
-__device__ inline void diag_block(T (&rv)[kNB], int jb, int lane, T* ldg, T* dgi,
+// NB + 4) and the reciprocal diagonal to dgi.  NB: the panel width (32, or 16
+// in the narrow-panel factor variant)
+template <typename T, int NB = kNB>
+__device__ inline void diag_block(T (&rv)[NB], int jb, int lane, T* ldg, T* dgi,
                                   int* __restrict__ info) {
+  constexpr int LDG = NB + 4;
   T mydinv = T(0);
 #pragma unroll
-  for (int t = 0; t < kNB; ++t) {
+  for (int t = 0; t < NB; ++t) {
     if (t < jb) {  // uniform
-      if (lane < kNB) ldg[lane] = rv[t];
+      if (lane < NB) ldg[lane] = rv[t];
       __builtin_amdgcn_wave_barrier();
       T d = lane_value(rv[t], t);
       if (!(d > T(0))) {  // uniform
@@ -239,7 +241,7 @@ __device__ inline void diag_block(T (&rv)[kNB], int jb, int lane, T* ldg, T* dgi
       const bool below = lane > t && lane < jb;
       const T g = below ? rv[t] * (dinv * dinv) : T(0);
 #pragma unroll
-      for (int l = t + 1; l < kNB; ++l) rv[l] -= g * ldg[l];
+      for (int l = t + 1; l < NB; ++l) rv[l] -= g * ldg[l];
       __builtin_amdgcn_wave_barrier();
       rv[t] = lane == t ? sq : (below ? rv[t] * dinv : rv[t]);
       mydinv = lane == t ? dinv : mydinv;
@@ -247,23 +249,24 @@ __device__ inline void diag_block(T (&rv)[kNB], int jb, int lane, T* ldg, T* dgi
   }
   if (lane < jb) {
 #pragma unroll
-    for (int c = 0; c < kNB; ++c) ldg[lane * kLDG + c] = rv[c];
+    for (int c = 0; c < NB; ++c) ldg[lane * LDG + c] = rv[c];
     dgi[lane] = mydinv;
   }
 }
 
 // a row below the diagonal block: L_r = A_r L_diag^{-T} (forward substitution
 // on the row's registers with the block from LDS)
-template <typename T>
-__device__ inline void row_solve(T (&rv)[kNB], int jb, const T* ldg, const T* dgi) {
+template <typename T, int NB = kNB>
+__device__ inline void row_solve(T (&rv)[NB], int jb, const T* ldg, const T* dgi) {
+  constexpr int LDG = NB + 4;
 #pragma unroll
-  for (int t = 0; t < kNB; ++t) {
+  for (int t = 0; t < NB; ++t) {
     if (t < jb) {
       T s0 = rv[t], s1 = T(0);
 #pragma unroll
       for (int l = 0; l < t; l += 2) {
-        s0 -= rv[l] * ldg[t * kLDG + l];
-        if (l + 1 < t) s1 -= rv[l + 1] * ldg[t * kLDG + l + 1];
+        s0 -= rv[l] * ldg[t * LDG + l];
+        if (l + 1 < t) s1 -= rv[l + 1] * ldg[t * LDG + l + 1];
       }
       rv[t] = (s0 + s1) * dgi[t];
     }
@@ -301,7 +304,9 @@ __device__ void panel_out(const SupDev& S, const T* __restrict__ F, T* __restric
   }
 }
 
-template <typename T, int NT, int RPT>
+// NB: panel width (32; 16 in the narrow-panel variant: shorter register
+// rows and diagonal-block steps, one more trailing pass per 16 columns)
+template <typename T, int NT, int RPT, int NB>
 __global__ __launch_bounds__(NT) void k_factor_level(
     const SupDev* __restrict__ sup, const int* __restrict__ lvl,
     const int* __restrict__ relpos, const int* __restrict__ cidx,
@@ -311,42 +316,43 @@ __global__ __launch_bounds__(NT) void k_factor_level(
     int64_t fprev_bs, T* __restrict__ panels, T* __restrict__ panelsT, int64_t panel_bs,
     int* __restrict__ info) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
+  constexpr int LDP = NB + 4, LDG = NB + 4;
   const int s = lvl[blockIdx.x];
   const int b = blockIdx.y;
   const SupDev S = sup[s];
   const int ns = S.ns, rs = S.rs, nf = S.ns + S.rs, ld = nf + S.ns;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid / 64;
   constexpr int NW = NT / 64;
-  T* ldg = reinterpret_cast<T*>(smem_raw);   // diagonal block rows, stride kLDG
-  T* dgi = ldg + kNB * kLDG;                  // reciprocal diagonal
-  T* pan = dgi + kNB;                         // panel rows, stride kLDP
+  T* ldg = reinterpret_cast<T*>(smem_raw);   // diagonal block rows, stride LDG
+  T* dgi = ldg + NB * LDG;                  // reciprocal diagonal
+  T* pan = dgi + NB;                         // panel rows, stride LDP
   T* F = fcur + b * fcur_bs + S.front_off;
   front_init<T, NT>(S, b, sup, relpos, cidx, a_e, a_row, a_col, vals, nnz, F, fprev, fprev_bs,
                     reinterpret_cast<int*>(pan));
-  for (int j0 = 0; j0 < ns; j0 += kNB) {
-    const int jb = min(kNB, ns - j0);
+  for (int j0 = 0; j0 < ns; j0 += NB) {
+    const int jb = min(NB, ns - j0);
     // panel rows: J, R and E rows from j0 on.  E row e (front row nf + e)
     // has no entry left of column e until column e is eliminated, so the E
     // rows e >= j0 + jb are zero in this panel and take no part in it
     const int m = min(ld - j0, nf + jb);
     const int n1 = nf - j0;    // end of the J/R rows (panel-local)
-    T rv[RPT][kNB];
+    T rv[RPT][NB];
 #pragma unroll
     for (int q = 0; q < RPT; ++q) {
       const int r = tid + q * NT;
 #pragma unroll
-      for (int c = 0; c < kNB; ++c)
+      for (int c = 0; c < NB; ++c)
         rv[q][c] = (r < m && c < jb) ? F[(j0 + r) + static_cast<int64_t>(j0 + c) * ld] : T(0);
     }
     // (1) the jb x jb diagonal block, by wavefront 0 alone (lanes < jb hold
     // its rows)
-    if (wave == 0) diag_block<T>(rv[0], jb, lane, ldg, dgi, info);
+    if (wave == 0) diag_block<T, NB>(rv[0], jb, lane, ldg, dgi, info);
     __syncthreads();
     // (2) every row below the block, on the thread's own registers
 #pragma unroll
     for (int q = 0; q < RPT; ++q) {
       const int r = tid + q * NT;
-      if (r >= jb && r < m) row_solve<T>(rv[q], jb, ldg, dgi);
+      if (r >= jb && r < m) row_solve<T, NB>(rv[q], jb, ldg, dgi);
     }
     // panel to F (its E rows) and to LDS (zero columns past jb; the 16 rows
     // past m are zeroed for the tile reads)
@@ -355,15 +361,15 @@ __global__ __launch_bounds__(NT) void k_factor_level(
       const int r = tid + q * NT;
       if (r < m) {
 #pragma unroll
-        for (int c = 0; c < kNB; ++c) {
+        for (int c = 0; c < NB; ++c) {
           // only the E rows of a finished panel are read again (the sweep
           // panel comes from them); its J/R rows live on in LDS only
           if (c < jb && r >= n1) F[(j0 + r) + static_cast<int64_t>(j0 + c) * ld] = rv[q][c];
-          pan[r * kLDP + c] = c < jb ? rv[q][c] : T(0);
+          pan[r * LDP + c] = c < jb ? rv[q][c] : T(0);
         }
       }
     }
-    for (int idx = tid; idx < 16 * kLDP; idx += NT) pan[m * kLDP + idx] = T(0);
+    for (int idx = tid; idx < 16 * LDP; idx += NT) pan[m * LDP + idx] = T(0);
     __syncthreads();
     // trailing update, 16x16 tiles, computed transposed so that a lane's
     // output runs down a column of F:
@@ -410,10 +416,10 @@ __global__ __launch_bounds__(NT) void k_factor_level(
 #pragma unroll
       for (int u = 0; u < TB; ++u) {
         typename Mma<T>::v4 acc = {T(0), T(0), T(0), T(0)};
-        const T* pa = pan + (l0[u] + (lane & 15)) * kLDP + (lane >> 4);
-        const T* pb = pan + (i0[u] + (lane & 15)) * kLDP + (lane >> 4);
+        const T* pa = pan + (l0[u] + (lane & 15)) * LDP + (lane >> 4);
+        const T* pb = pan + (i0[u] + (lane & 15)) * LDP + (lane >> 4);
 #pragma unroll
-        for (int k0 = 0; k0 < kNB; k0 += 4) acc = Mma<T>::op(pa[k0], pb[k0], acc);
+        for (int k0 = 0; k0 < NB; k0 += 4) acc = Mma<T>::op(pa[k0], pb[k0], acc);
 #pragma unroll
         for (int e = 0; e < 4; ++e)
           if (ok[u][e]) F[addr[u][e]] = cold[u][e] - acc[e];
@@ -2081,6 +2087,7 @@ struct dcx_plan {
   int tc_cap = 32;           // widest column tile of the in-LDS sweeps
   int splitk_below = 4096;   // backward levels with fewer wavefronts take split-K
   int wide_below = 321;      // factor levels with fewer workgroups take 1024 threads
+  int panel_width = 32;      // columns per panel of the in-LDS factor (16 or 32)
   // factor launch order: each level's supernodes by front height (ld)
   // descending, cut into buckets launched with their own LDS size
   int* d_flvl = nullptr;
@@ -2218,8 +2225,9 @@ inline int level_max_ld(const dcx::Symbolic& S, int d) {
   }
   return maxld;
 }
-inline size_t factor_lds(int maxld, size_t tsize) {
-  return (kNB * kLDG + kNB + static_cast<size_t>(maxld + 16) * kLDP) * tsize;
+inline size_t factor_lds(int maxld, size_t tsize, int nb = kNB) {
+  return (static_cast<size_t>(nb) * (nb + 4) + nb + static_cast<size_t>(maxld + 16) * (nb + 4)) *
+         tsize;
 }
 // also levels with fewer than DCX_FACTOR_GRID_BELOW (front x batch)
 // workgroups, where one workgroup per front leaves the chip idle
@@ -2268,13 +2276,13 @@ std::vector<int> build_factor_order(dcx_plan* P) {
   return order;
 }
 
-template <typename T, int NT, int RPT>
+template <typename T, int NT, int RPT, int NB = kNB>
 void factor_launch(dcx_plan* P, int d, const int* lvl, int cnt, size_t lds, hipStream_t st) {
   const auto& S = P->S;
   T* fr[2] = {static_cast<T*>(P->d_front[0]), static_cast<T*>(P->d_front[1])};
   const int64_t fbs[2] = {S.front_size[0], S.front_size[1]};
   dim3 grid(cnt, P->nbatch);
-  hipLaunchKernelGGL((k_factor_level<T, NT, RPT>), grid, dim3(NT), lds, st, P->d_sup,
+  hipLaunchKernelGGL((k_factor_level<T, NT, RPT, NB>), grid, dim3(NT), lds, st, P->d_sup,
                      lvl, P->d_relpos, P->d_cidx, P->d_a_e, P->d_a_row,
                      P->d_a_col, P->d_vals, static_cast<int64_t>(S.nnz), fr[d & 1], fbs[d & 1],
                      fr[(d + 1) & 1], fbs[(d + 1) & 1], static_cast<T*>(P->d_panels),
@@ -2371,8 +2379,14 @@ int factor_level(dcx_plan* P, int d, hipStream_t st) {
     const int b0 = P->fbkt[t];
     const int b1 = t + 1 < P->fbkt_ptr[d + 1] ? P->fbkt[t + 1] : S.lvl_ptr[d + 1];
     const int bld = P->flvl_ld[b0];
-    const size_t blds = factor_lds(bld, sizeof(T));
     const int* lvl = P->d_flvl + b0;
+    if (P->panel_width == 16) {
+      const size_t l16 = factor_lds(bld, sizeof(T), 16);
+      if (bld <= 256 && !wide) factor_launch<T, 256, 1, 16>(P, d, lvl, b1 - b0, l16, st);
+      else factor_launch<T, 1024, 1, 16>(P, d, lvl, b1 - b0, l16, st);
+      continue;
+    }
+    const size_t blds = factor_lds(bld, sizeof(T));
     if (bld <= 256 && !wide)
       factor_launch<T, 256, 1>(P, d, lvl, b1 - b0, blds, st);
     else
@@ -2822,6 +2836,7 @@ int dcx_plan_create(int n, const int32_t* crow, const int32_t* col, int nbatch,
     P->tc_cap = env_int("DCX_SWEEP_TC", 32);
     P->splitk_below = env_int("DCX_BWD_SPLITK_BELOW", 4096);
     P->wide_below = env_int("DCX_FACTOR_WIDE_BELOW", 321);
+    P->panel_width = env_int("DCX_FACTOR_PANEL_WIDTH", 32) == 16 ? 16 : 32;
     {
       int rc = upload_fused(P.get());
       if (rc) return rc;
@@ -2943,6 +2958,10 @@ int dcx_plan_set_tuning(dcx_plan* P, int key, int value) {
       if (value < 0) return fail(DCX_ERR_ARG, "wide-factor threshold must be >= 0");
       P->wide_below = value;
       break;
+    case DCX_TUNE_PANEL_WIDTH:
+      if (value != 16 && value != 32) return fail(DCX_ERR_ARG, "panel width must be 16 or 32");
+      P->panel_width = value;
+      break;
     default:
       return fail(DCX_ERR_ARG, "unknown tuning key");
   }
@@ -2955,6 +2974,7 @@ int dcx_plan_get_tuning(const dcx_plan* P, int key, int* value) {
     case DCX_TUNE_SWEEP_TC: *value = P->tc_cap; break;
     case DCX_TUNE_SPLITK_BELOW: *value = P->splitk_below; break;
     case DCX_TUNE_WIDE_BELOW: *value = P->wide_below; break;
+    case DCX_TUNE_PANEL_WIDTH: *value = P->panel_width; break;
     default: return fail(DCX_ERR_ARG, "unknown tuning key");
   }
   return DCX_OK;

@@ -24,7 +24,7 @@ launch economy:
   `LineBatch(leaf="auto")` builds every line with its class's choice;
   `autotune_tiles` then times the kernel-variant thresholds of a plan (sweep
   column tile, split-K threshold of the backward sweep, wide-factor
-  threshold; `dcx_plan_set_tuning`) by coordinate search, per size class in
+  threshold, factor panel width; `dcx_plan_set_tuning`) by coordinate search, per size class in
   `LineBatch(tiles="auto")` and for the one-plan batch
   (`LineForest.autotune_tiles`).  Tuning is not part of a timed step.
 
@@ -57,7 +57,7 @@ def _eval_grad(eng, x, dobs, w):
 
 #: candidate values of each kernel-variant threshold (dcx_plan_set_tuning)
 TILE_GRID = {"sweep_tc": (16, 32, 64), "splitk_below": (1024, 4096, 16384),
-             "wide_below": (0, 321, 4096)}
+             "wide_below": (0, 321, 4096), "panel_width": (16, 32)}
 
 
 def tile_key(setting):

@@ -115,11 +115,11 @@ class HipBatchSolver:
             _lib.check(self._lib.dcx_plan_set_fusion(self._h, int(rows)), "set_fusion")
         self.fuse_rows = int(rows)
 
-    _TUNE_KEYS = {"sweep_tc": 1, "splitk_below": 2, "wide_below": 3}
+    _TUNE_KEYS = {"sweep_tc": 1, "splitk_below": 2, "wide_below": 3, "panel_width": 4}
 
     def set_tuning(self, **values):
         """Kernel-variant thresholds of this plan: sweep_tc (16, 32, 64),
-        splitk_below, wide_below (see dcx_plan_set_tuning)."""
+        splitk_below, wide_below, panel_width (16, 32) (see dcx_plan_set_tuning)."""
         for name, v in values.items():
             if name not in self._TUNE_KEYS:
                 raise ValueError(f"unknown tuning key {name!r}")
