@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 import torch
 
-from helpers import noisy_data
+from helpers import case, noisy_data
 
 
 def _lines(n=4, cells=(3000, 8000), seed=5):
@@ -273,3 +273,36 @@ def test_line_forest_full_batch_fp64():
         assert float(phi) == pytest.approx(float(phi_r), rel=1e-11), i
         assert float((g - g_r).norm() / g_r.norm()) < 1e-10, i
     lf.free()
+
+
+@pytest.mark.gpu
+def test_grouped_map_arguments_and_refusals():
+    """dcx_lmap_set_groups rejects out-of-range groups and keeps the map
+    usable; a two-group table whose columns both hold the map's own
+    wavenumbers leaves the assembled values unchanged; dcx_jacobian_rows
+    refuses a map with more than one group (its kernel reads one wavenumber
+    per system) before launching anything."""
+    import ctypes as C
+    from simpeg_dc_torch_amd import TorchDC2D, _lib
+    print("grouped map arguments", flush=True)
+    mesh, survey, m = case("suite")
+    eng = TorchDC2D(mesh, survey, nky=6, device="cuda")
+    g = eng._gpu
+    sig = torch.exp(-torch.tensor(m, device="cuda"))
+    v0 = g.assemble(sig)
+    nnz = eng.nnzA
+    bad, badp = _lib.i32(np.full(nnz, 3))
+    ky2, ky2p = _lib.f64((eng.kys ** 2).reshape(-1, 1))
+    assert g.lib.dcx_lmap_set_groups(g.lmap, 1, badp, ky2p) != 0
+    ky22, ky22p = _lib.f64(np.repeat((eng.kys ** 2).reshape(-1, 1), 2, axis=1))
+    grp, grpp = _lib.i32(np.arange(nnz) % 2)
+    _lib.check(g.lib.dcx_lmap_set_groups(g.lmap, 2, grpp, ky22p), "set_groups")
+    assert torch.equal(g.assemble(sig), v0)
+    Y = torch.zeros(eng.nky, eng.nN, 1, dtype=torch.float64, device="cuda")
+    U = torch.zeros(eng.nky, eng.nN, eng.nS, dtype=torch.float64, device="cuda")
+    src = torch.zeros(1, dtype=torch.int32, device="cuda")
+    J = torch.zeros(eng.nC, 1, dtype=torch.float64, device="cuda")
+    rc = g.lib.dcx_jacobian_rows(g.lmap, g.solver._h, _lib.ptr(Y), 1, _lib.ptr(U), eng.nS,
+                                 _lib.ptr(src), _lib.ptr(g.wq), _lib.ptr(sig), _lib.ptr(J),
+                                 _lib.DCX_PERMUTED, g.stream())
+    assert rc != 0 and b"group" in g.lib.dcx_last_error()
