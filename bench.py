@@ -458,6 +458,23 @@ def lines_times(steps, warmup, autotune=True, per_line=True):
     ms_seq, ms_par, ms_thr = timed(1), timed(4), timed(4, threads=True)
     one = one_plan_times(lines, ms, data, steps, warmup, compare=lb)
 
+    # the same per-line batch with every engine in the electrode basis
+    lbe = LineBatch(lines, precision="mixed", streams=4, leaf=lb.leaf, rhs_basis="electrodes")
+    lbe.set_data([d for d, _ in data], [w for _, w in data])
+    for _ in range(warmup):
+        lbe.eval_grad(ms)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(steps):
+        lbe.eval_grad(ms)
+    torch.cuda.synchronize()
+    ms_elec = 1e3 * (time.perf_counter() - t1) / steps
+    oe, ob = lbe.eval_grad(ms), lb.eval_grad(ms)
+    torch.cuda.synchronize()
+    elec_diff = max(float(torch.linalg.norm(a[1] - b[1]) / torch.linalg.norm(b[1]))
+                    for a, b in zip(oe, ob))
+    del lbe
+
     # FP64 engines: eager on 4 streams, then each line's eval+grad replayed
     # as a captured HIP graph (capture is limited to FP64 engines)
     lb64 = LineBatch(lines, precision="fp64", streams=4)
@@ -488,9 +505,11 @@ def lines_times(steps, warmup, autotune=True, per_line=True):
             "precision": "mixed", "sequential_ms": round(ms_seq, 2),
             "streams4_ms": round(ms_par, 2), "streams4_threads4_ms": round(ms_thr, 2),
             "one_plan_ms": ms_forest,
+            "streams4_electrode_basis_ms": round(ms_elec, 2),
+            "electrode_basis_grad_max_rel_diff": float(f"{elec_diff:.1e}"),
             "fp64_streams4_ms": round(ms64_eager, 2), "fp64_graphs_ms": round(ms64_graph, 2),
             "fp64_graph_phi_max_rel_diff": float(f"{graph_diff:.1e}"),
-            "lines_per_s": round(1e3 * len(lines) / min(ms_par, ms_thr, ms_forest), 1),
+            "lines_per_s": round(1e3 * len(lines) / min(ms_par, ms_thr, ms_forest, ms_elec), 1),
             "one_plan": one["one_plan"],
             "leaf_by_size_class": {str(k): v for k, v in (lb.leaf or {}).items()},
             "tiles_by_size_class": {str(k): v for k, v in lb.tiles.items()},

@@ -145,10 +145,11 @@ class LineBatch:
     [(m, dobs, w) per line] to run `autotune_leaf` on one line per class;
     tiles: None, a {size_class: setting} dict, or "auto" (with `tune_data`:
     `autotune_tiles` on one line per class, its setting given to every line
-    of the class)."""
+    of the class); further keywords (e.g. rhs_basis="electrodes") go to every
+    TorchDC2D."""
 
     def __init__(self, lines, device="cuda", nky=12, precision="mixed", refine=0, streams=4,
-                 leaf=None, tune_data=None, threads=False, tiles=None):
+                 leaf=None, tune_data=None, threads=False, tiles=None, **engine_kw):
         self.device = torch.device(device)
         if self.device.type == "cuda" and self.device.index is None:
             self.device = torch.device("cuda", torch.cuda.current_device())
@@ -169,7 +170,8 @@ class LineBatch:
         for mesh, survey in lines:
             lf = leaf.get(size_class(mesh.nC)) if isinstance(leaf, dict) else leaf
             self.engines.append(TorchDC2D(mesh, survey, nky=nky, device=self.device,
-                                          precision=precision, refine=refine, leaf=lf))
+                                          precision=precision, refine=refine, leaf=lf,
+                                          **engine_kw))
         self.tiles = {}
         if tiles == "auto":
             if tune_data is None:

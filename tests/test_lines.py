@@ -77,6 +77,28 @@ def test_line_batch_streams_match_single(precision, threads):
             assert np.abs(out[i][1].cpu().numpy() - g).max() <= 1e-13 * np.abs(g).max(), (rep, i)
 
 
+@pytest.mark.gpu
+def test_line_batch_electrode_basis_matches_sources():
+    """Engine keywords reach every line: a batch built with
+    rhs_basis="electrodes" gives each line's misfit and gradient of the
+    source-basis batch (FP64; the two bases solve different right-hand sides,
+    so agreement is to round-off, not bit-exact)."""
+    from simpeg_dc_torch_amd.lines import LineBatch
+    print("line batch electrode basis fp64", flush=True)
+    L, data, lb = _setup(precision="fp64")
+    lbe = LineBatch([(mesh, survey) for mesh, survey, _ in L], nky=6, precision="fp64",
+                    streams=4, rhs_basis="electrodes")
+    lbe.set_data([d for d, _ in data], [w for _, w in data])
+    assert all(e.rhs_basis == "electrodes" for e in lbe.engines)
+    ms = [m for _, _, m in L]
+    a, b = lbe.eval_grad(ms), lb.eval_grad(ms)
+    torch.cuda.synchronize()
+    for i in range(len(L)):
+        assert float(a[i][0]) == pytest.approx(float(b[i][0]), rel=1e-10), i
+        g = b[i][1].cpu().numpy()
+        assert np.abs(a[i][1].cpu().numpy() - g).max() <= 1e-9 * np.abs(g).max(), i
+
+
 _MIXED_GRAPHS = pytest.mark.skip(
     reason="capture refuses mixed-precision engines: a replay of their graphs ended the "
            "process with a host segmentation fault inside hipGraphLaunch, cause not found")
