@@ -431,7 +431,10 @@ def lines_times(steps, warmup, autotune=True, per_line=True):
         data.append((dobs, 1.0 / (0.05 * dobs.abs())))
     del probe
     if not per_line:
-        return one_plan_times(lines, ms, data, steps, warmup)
+        one = one_plan_times(lines, ms, data, steps, warmup)
+        one_e = one_plan_times(lines, ms, data, steps, warmup, rhs_basis="electrodes")
+        return {**one, "one_plan_electrode_basis_ms": one_e["one_plan_ms"],
+                "one_plan_electrode_basis": one_e["one_plan"]}
     tune = [(m.cpu().numpy(), d.cpu().numpy(), w.cpu().numpy()) for m, (d, w) in zip(ms, data)]
     t0 = time.perf_counter()
     lb = LineBatch(lines, precision="mixed", streams=4, leaf="auto" if autotune else None,
@@ -457,6 +460,7 @@ def lines_times(steps, warmup, autotune=True, per_line=True):
 
     ms_seq, ms_par, ms_thr = timed(1), timed(4), timed(4, threads=True)
     one = one_plan_times(lines, ms, data, steps, warmup, compare=lb)
+    one_e = one_plan_times(lines, ms, data, steps, warmup, compare=lb, rhs_basis="electrodes")
 
     # the same per-line batch with every engine in the electrode basis
     lbe = LineBatch(lines, precision="mixed", streams=4, leaf=lb.leaf, rhs_basis="electrodes")
@@ -509,21 +513,25 @@ def lines_times(steps, warmup, autotune=True, per_line=True):
             "electrode_basis_grad_max_rel_diff": float(f"{elec_diff:.1e}"),
             "fp64_streams4_ms": round(ms64_eager, 2), "fp64_graphs_ms": round(ms64_graph, 2),
             "fp64_graph_phi_max_rel_diff": float(f"{graph_diff:.1e}"),
-            "lines_per_s": round(1e3 * len(lines) / min(ms_par, ms_thr, ms_forest, ms_elec), 1),
+            "one_plan_electrode_basis_ms": one_e["one_plan_ms"],
+            "lines_per_s": round(1e3 * len(lines) / min(ms_par, ms_thr, ms_forest, ms_elec,
+                                                        one_e["one_plan_ms"]), 1),
             "one_plan": one["one_plan"],
+            "one_plan_electrode_basis": one_e["one_plan"],
             "leaf_by_size_class": {str(k): v for k, v in (lb.leaf or {}).items()},
             "tiles_by_size_class": {str(k): v for k, v in lb.tiles.items()},
             "build_and_tune_s": round(t_tune, 1),
             "graph_capture": "FP64 engines (mixed-precision capture refused)"}
 
 
-def one_plan_times(lines, ms, data, steps, warmup, compare=None):
+def one_plan_times(lines, ms, data, steps, warmup, compare=None, rhs_basis="sources"):
     """Every line in one plan (a forest of the lines' separator trees), mixed
-    precision; with `compare` (a LineBatch) the largest relative differences
-    of phi and of the gradient against the per-line engines."""
+    precision, source or electrode basis; with `compare` (a LineBatch) the
+    largest relative differences of phi and of the gradient against the
+    per-line engines."""
     from simpeg_dc_torch_amd.lines import LineForest
     t0 = time.perf_counter()
-    lf = LineForest(lines, precision="mixed")
+    lf = LineForest(lines, precision="mixed", rhs_basis=rhs_basis)
     t_forest = time.perf_counter() - t0
     lf.set_data([d for d, _ in data], [w for _, w in data])
     t0 = time.perf_counter()
@@ -537,7 +545,7 @@ def one_plan_times(lines, ms, data, steps, warmup, compare=None):
         lf.eval_grad(ms)
     torch.cuda.synchronize()
     ms_forest = 1e3 * (time.perf_counter() - t1) / steps
-    one = {"source_columns": lf.K, "systems": lf.nbatch,
+    one = {"rhs_columns": lf.K, "systems": lf.nbatch,
            "level_launches": lf.solver.info().get("level_launches"),
            "build_s": round(t_forest, 1), "tiles": tiles, "tile_tuning_s": round(t_tiles, 1)}
     if compare is not None:

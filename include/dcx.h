@@ -216,6 +216,18 @@ int dcx_assemble(dcx_lmap* map, const double* sigma, double* values, void* strea
 int dcx_lmap_transpose(dcx_lmap* map, const double* gv, const double* sigma,
                        double* grad_m, void* stream);
 
+/* Row mix per segment: h[b][r][:] = g[b][r][:] w[b][l] for the rows r of
+ * segment l, rows [seg_rows[l], seg_rows[l+1]) (host array of nseg + 1
+ * ascending row offsets within 0..n; rows outside every segment are left
+ * untouched).  g, h: device (nbatch x n x k); w: device (nbatch x nseg x k x
+ * k), row-major; double, or float with DCX_F32_FIELDS.  k <= 128, nseg <=
+ * 256.  The adjoint field of the electrode basis when the systems hold
+ * several independent lines, each with its own k x k data matrix: replaces
+ * one small batched GEMM per line.  No reference counterpart (the reference
+ * simulates one line per call: dpred, dctorch/simulation2d.py:391-396). */
+int dcx_rows_mix(const void* g, const void* w, void* h, int nbatch, int n, int k, int nseg,
+                 const int32_t* seg_rows, int flags, void* stream);
+
 /* Rows of the Jacobian of dpred for r data at once, transposed:
  * J[c][j] = d dpred_j / d m_c (m = log rho on all cells).  Y: device
  * (nbatch x n x r), A_b^{-1} applied to the data's projection rows (Pd^T

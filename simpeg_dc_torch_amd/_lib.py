@@ -73,6 +73,8 @@ PROTOTYPES = {
     "dcx_jacobian_rows": (C.c_int, [_vp, _vp, _vp, C.c_int, _vp, C.c_int, _vp, _vp, _vp, _vp,
                                     C.c_int, _vp]),
     "dcx_set_rhs_support": (C.c_int, [_vp, _i32p, C.c_int]),
+    "dcx_rows_mix": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, C.c_int, C.c_int, _i32p,
+                               C.c_int, _vp]),
     "dcx_grad_values": (C.c_int, [_vp, _vp, _vp, C.c_int, _vp, _vp, C.c_int, _vp]),
     "dcx_mma_probe": (C.c_int, [C.c_int, _vp, _vp, _vp, _vp]),
     "dcx_permute_rows": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, C.c_int, _vp]),

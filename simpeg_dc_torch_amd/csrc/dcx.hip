@@ -2804,6 +2804,110 @@ void launch_permute_d2T(dcx_plan* P, const double* in, void* out, int k, int nbl
 // ===========================================================================
 // C-ABI
 // ===========================================================================
+namespace {
+
+constexpr int kMixMaxSeg = 256;  // segments (lines) per call
+constexpr int kMixRows = 64;     // rows per workgroup, 16 per wave
+constexpr int kMixMaxK = 128;    // columns: one or two per lane
+
+struct MixSegs {
+  int nseg;
+  int row[kMixMaxSeg + 1];   // segment l = rows [row[l], row[l+1])
+  int tile[kMixMaxSeg + 1];  // first workgroup of segment l
+};
+
+template <typename T>
+__device__ inline void load4(const T* p, T (&v)[4]) {
+  v[0] = p[0]; v[1] = p[1]; v[2] = p[2]; v[3] = p[3];
+}
+template <>
+__device__ inline void load4<float>(const float* p, float (&v)[4]) {
+  const float4 q = *reinterpret_cast<const float4*>(p);
+  v[0] = q.x; v[1] = q.y; v[2] = q.z; v[3] = q.w;
+}
+template <>
+__device__ inline void load4<double>(const double* p, double (&v)[4]) {
+  const double2 a = *reinterpret_cast<const double2*>(p);
+  const double2 b = *reinterpret_cast<const double2*>(p + 2);
+  v[0] = a.x; v[1] = a.y; v[2] = b.x; v[3] = b.y;
+}
+
+// H[b][r][:] = G[b][r][:] W[b][l] for the rows r of segment l (W[b][l]: k x k,
+// row-major).  A workgroup takes 64 rows of one segment of one system: the
+// rows are staged in LDS (k padded to kp, a multiple of 4, with zeros), lane
+// j owns columns j and j + 64 (NG = 2) for 16 rows of its wave, and W's row a
+// is read once per wave from L2 and used for all 16 rows.
+template <typename T, int NG>
+__global__ __launch_bounds__(256) void k_rows_mix(const T* __restrict__ G,
+                                                  const T* __restrict__ W, T* __restrict__ H,
+                                                  int n, int k, int kp, MixSegs segs) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* g = reinterpret_cast<T*>(smem_raw);
+  const int t = blockIdx.x, b = blockIdx.y;
+  int l = 0;
+  while (l + 1 < segs.nseg && segs.tile[l + 1] <= t) ++l;
+  const int r0 = segs.row[l] + (t - segs.tile[l]) * kMixRows;
+  const int nr = min(kMixRows, segs.row[l + 1] - r0);
+  const T* Gb = G + (static_cast<size_t>(b) * n + r0) * k;
+  for (int i = threadIdx.x; i < kMixRows * kp; i += blockDim.x) {
+    const int rr = i / kp, a = i - rr * kp;
+    g[i] = (rr < nr && a < k) ? Gb[static_cast<size_t>(rr) * k + a] : T(0);
+  }
+  __syncthreads();
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const T* Wl = W + (static_cast<size_t>(b) * segs.nseg + l) * k * k;
+  const T* gw = g + wv * 16 * kp;
+  T acc[NG][16];
+#pragma unroll
+  for (int c = 0; c < NG; ++c)
+#pragma unroll
+    for (int i = 0; i < 16; ++i) acc[c][i] = T(0);
+  for (int a = 0; a < k; a += 4) {
+    T w[NG][4];
+#pragma unroll
+    for (int c = 0; c < NG; ++c) {
+      const int j = lane + 64 * c;
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        w[c][q] = (j < k && a + q < k) ? Wl[static_cast<size_t>(a + q) * k + j] : T(0);
+    }
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      T gv[4];
+      load4(gw + i * kp + a, gv);
+#pragma unroll
+      for (int c = 0; c < NG; ++c)
+        acc[c][i] += gv[0] * w[c][0] + gv[1] * w[c][1] + gv[2] * w[c][2] + gv[3] * w[c][3];
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 16; ++i) {
+    const int rr = wv * 16 + i;
+    if (rr >= nr) break;
+    T* Hr = H + (static_cast<size_t>(b) * n + r0 + rr) * k;
+#pragma unroll
+    for (int c = 0; c < NG; ++c)
+      if (lane + 64 * c < k) Hr[lane + 64 * c] = acc[c][i];
+  }
+}
+
+template <typename T>
+void launch_rows_mix(const void* g, const void* w, void* h, int nbatch, int n, int k,
+                     const MixSegs& segs, hipStream_t st) {
+  const int kp = (k + 3) & ~3;
+  const dim3 grid(segs.tile[segs.nseg], nbatch);
+  const size_t lds = static_cast<size_t>(kMixRows) * kp * sizeof(T);
+  const T* G = static_cast<const T*>(g);
+  const T* W = static_cast<const T*>(w);
+  T* H = static_cast<T*>(h);
+  if (k <= 64)
+    hipLaunchKernelGGL((k_rows_mix<T, 1>), grid, dim3(256), lds, st, G, W, H, n, k, kp, segs);
+  else
+    hipLaunchKernelGGL((k_rows_mix<T, 2>), grid, dim3(256), lds, st, G, W, H, n, k, kp, segs);
+}
+
+}  // namespace
+
 extern "C" {
 
 const char* dcx_last_error(void) { return g_err.c_str(); }
@@ -3454,6 +3558,35 @@ int dcx_lmap_transpose(dcx_lmap* M, const double* gv, const double* sigma, doubl
                      M->nky, M->d_tptr, M->d_te, M->d_tle, M->d_tln, M->d_tbptr,
                      M->d_tbq, M->d_bc_e, M->d_bcoef, M->nbc, M->d_ge, M->d_gn, gv, sigma,
                      grad_m);
+  HIP_TRY(hipGetLastError());
+  return DCX_OK;
+}
+
+int dcx_rows_mix(const void* g, const void* w, void* h, int nbatch, int n, int k, int nseg,
+                 const int32_t* seg_rows, int flags, void* stream) {
+  if (!g || !w || !h || !seg_rows) return fail(DCX_ERR_ARG, "null argument");
+  if (nbatch < 1 || n < 1) return fail(DCX_ERR_ARG, "nbatch and n must be >= 1");
+  if (k < 1 || k > kMixMaxK) return fail(DCX_ERR_ARG, "k must be 1..128");
+  if (nseg < 1 || nseg > kMixMaxSeg) return fail(DCX_ERR_ARG, "nseg must be 1..256");
+  if (nbatch > 65535) return fail(DCX_ERR_ARG, "nbatch must be <= 65535");
+  MixSegs segs{};
+  segs.nseg = nseg;
+  int64_t tiles = 0;
+  for (int l = 0; l <= nseg; ++l) {
+    const int r = seg_rows[l];
+    if (r < 0 || r > n || (l > 0 && r < seg_rows[l - 1]))
+      return fail(DCX_ERR_ARG, "segment rows must ascend within 0..n");
+    segs.row[l] = r;
+    segs.tile[l] = static_cast<int>(tiles);
+    if (l < nseg) tiles += (static_cast<int64_t>(seg_rows[l + 1]) - r + kMixRows - 1) / kMixRows;
+  }
+  if (tiles == 0) return DCX_OK;
+  if (tiles > INT32_MAX) return fail(DCX_ERR_ARG, "too many rows");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (flags & DCX_F32_FIELDS)
+    launch_rows_mix<float>(g, w, h, nbatch, n, k, segs, st);
+  else
+    launch_rows_mix<double>(g, w, h, nbatch, n, k, segs, st);
   HIP_TRY(hipGetLastError());
   return DCX_OK;
 }

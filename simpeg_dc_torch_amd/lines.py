@@ -393,12 +393,26 @@ class LineForest:
     The weighted right-hand sides are kept on the device: nbatch x n x K FP64
     (1.7 GB for 16 lines of 251k cells in all, 10 systems, K = 81).
 
+    rhs_basis="electrodes" solves once, with the lines' electrode columns as
+    one right-hand side shared by every system (column j of the block holds
+    electrode j of every line that has one; K = max ne), and no adjoint
+    solve: with G_k = A_k^-1 E, per line i
+        K_i = sum_k w_(k,i) E_i^T G_k,  d_i = (R_i o (K_i S_i)[:, sidx_i]) summed
+        over electrodes,
+        dphi/dA_k = -G_k w_(k,i) M_i G_k^T on line i's rows,
+        M_i = T_i S_i^T,  T_i[:, s] = sum over the data of source s of R_i v,
+    so the adjoint field is H_k = G_k (w_(k,i) M_i)^T, one small product per
+    line over its rows of the solution (the lines' rows are contiguous in
+    the elimination order when the dissection orders component by component;
+    otherwise they are gathered).  The same operators as TorchDC2D's
+    electrode basis, per line.
+
     lines: list of (mesh, survey); nky: an int or one per line; precision:
     "mixed" (FP32 factor, sweeps and fields) or "fp64"; refine, leaf and
-    quadrature as TorchDC2D."""
+    quadrature as TorchDC2D; rhs_basis "sources" or "electrodes"."""
 
     def __init__(self, lines, device="cuda", nky=12, precision="mixed", refine=0, leaf=None,
-                 quadrature=None):
+                 quadrature=None, rhs_basis="sources"):
         from .solver import HipBatchSolver
         from . import _lib as lib_mod
         self._libm = lib_mod
@@ -406,6 +420,9 @@ class LineForest:
         if self.device.type == "cuda" and self.device.index is None:
             self.device = torch.device("cuda", torch.cuda.current_device())
         dev = self.device
+        if rhs_basis not in ("sources", "electrodes"):
+            raise ValueError(f"rhs_basis must be sources|electrodes, got {rhs_basis!r}")
+        self.rhs_basis = rhs_basis
         nkys = list(nky) if np.iterable(nky) else [int(nky)] * len(lines)
         maps, kys, wqs = [], [], []
         for (mesh, survey), nk in zip(lines, nkys):
@@ -454,14 +471,86 @@ class LineForest:
             L.check(lib.dcx_survey_create(self.solver._h, M["nD"], M["K"], ppp, pcp, pvp, sip,
                                           C.byref(h2)), "survey_create")
         self.survey = h2
+        self.ones = torch.ones(M["nbatch"], dtype=torch.float64, device=dev)
+        self.data_line = torch.tensor(M["data_line"], device=dev)
+        self._data = None
+        if rhs_basis == "electrodes":
+            self._electrode_basis(maps, perm)
+            return
         # weighted forward right-hand sides, elimination order: (nbatch, n, K)
         q = torch.tensor(M["q"][perm].toarray(), dtype=torch.float64, device=dev)
         wrow = torch.tensor(M["wq"][:, M["node_line"][perm]], dtype=torch.float64, device=dev)
         self.B = (wrow.unsqueeze(2) * q.unsqueeze(0)).contiguous()
         del q, wrow
-        self.ones = torch.ones(M["nbatch"], dtype=torch.float64, device=dev)
-        self.data_line = torch.tensor(M["data_line"], device=dev)
-        self._data = None
+
+    def _electrode_basis(self, maps, perm):
+        """Electrode-basis operators of every line, padded to common sizes:
+        the shared right-hand side E (elimination order, n x Ke), per line the
+        support rows of its electrodes and E_i^T on them, S_i, R_i and the
+        line's source index per datum, and the line's rows of the solution."""
+        M, dev, L = self.M, self.device, self.nlines
+        off = M["node_off"]
+        basis = [m.electrode_basis() for m in maps]
+        ne = [E.shape[1] for E, _, _ in basis]
+        Ke = self.K = max(ne)
+        pos = np.empty(M["n"], np.int64)
+        pos[perm] = np.arange(M["n"])
+        er, ec, ev = [], [], []
+        sups, esel = [], []
+        for i, (E, _, _) in enumerate(basis):
+            c = sp.coo_matrix(E)
+            prow = pos[c.row + off[i]]
+            er.append(prow); ec.append(c.col); ev.append(c.data)
+            sup = np.unique(prow)
+            sups.append(sup)
+            Es = sp.csr_matrix((c.data, (np.searchsorted(sup, prow), c.col)),
+                               shape=(sup.size, Ke)).toarray().T
+            esel.append(Es)
+        Ep = sp.csr_matrix((np.concatenate(ev), (np.concatenate(er), np.concatenate(ec))),
+                           shape=(M["n"], Ke))
+        self.E = torch.tensor(Ep.toarray(), dtype=torch.float64, device=dev)
+        smax = max(s.size for s in sups)
+        sup_pad = np.zeros((L, smax), np.int64)
+        esel_pad = np.zeros((L, Ke, smax))
+        for i, (sup, Es) in enumerate(zip(sups, esel)):
+            sup_pad[i, :sup.size] = sup
+            sup_pad[i, sup.size:] = sup[0]
+            esel_pad[i, :, :sup.size] = Es
+        self.e_sup = torch.tensor(sup_pad.reshape(-1), device=dev)
+        self.e_sel = torch.tensor(esel_pad, dtype=torch.float64, device=dev)
+        nS_max, nD_max = max(self.nS), max(self.nD)
+        S_pad = np.zeros((L, Ke, nS_max))
+        R_pad = np.zeros((L, Ke, nD_max))
+        sidx_pad = np.zeros((L, nD_max), np.int64)
+        dmask = np.zeros((L, nD_max), bool)
+        for i, ((_, S, R), m) in enumerate(zip(basis, maps)):
+            S_pad[i, :S.shape[0], :S.shape[1]] = S.toarray()
+            R_pad[i, :R.shape[0], :R.shape[1]] = R.toarray()
+            sidx_pad[i, :m.nD] = m.sidx
+            dmask[i, :m.nD] = True
+        self.S_pad = torch.tensor(S_pad, dtype=torch.float64, device=dev)
+        self.R_pad = torch.tensor(R_pad, dtype=torch.float64, device=dev)
+        self.sidx_pad = torch.tensor(sidx_pad, device=dev).unsqueeze(1).expand(L, Ke, nD_max)
+        self.dmask = torch.tensor(dmask, device=dev)
+        self.wq_lines = torch.tensor(M["wq"], dtype=torch.float64, device=dev)  # (nbatch, L)
+        # each line's rows in elimination order: one row segment per line
+        # (dcx_rows_mix) when every line's rows are contiguous, else gathers
+        self.line_rows = []
+        for i in range(L):
+            r = np.sort(pos[off[i]:off[i + 1]])
+            if r[-1] - r[0] + 1 == r.size:
+                self.line_rows.append(slice(int(r[0]), int(r[-1]) + 1))
+            else:
+                self.line_rows.append(torch.tensor(r, device=dev))
+        self._mix_segs = None
+        if all(isinstance(r, slice) for r in self.line_rows):
+            order = np.argsort([r.start for r in self.line_rows])
+            starts = [self.line_rows[i].start for i in order]
+            stops = [self.line_rows[i].stop for i in order]
+            if L <= 256 and Ke <= 128 and starts[0] == 0 and stops[-1] == M["n"] and all(
+                    a == b for a, b in zip(stops[:-1], starts[1:])):
+                self._mix_order = torch.tensor(order, device=dev)
+                self._mix_segs = np.ascontiguousarray(np.r_[starts, stops[-1]], np.int32)
 
     def _stream(self):
         return self._libm.stream_of(self.device)
@@ -486,6 +575,8 @@ class LineForest:
         one launch sequence on the current stream, no host synchronisation."""
         if self._data is None:
             raise RuntimeError("set_data first")
+        if self.rhs_basis == "electrodes":
+            return self._eval_grad_electrodes(ms)
         dobs, w = self._data
         L, lib, sv, dev = self._libm, self.lib, self.solver, self.device
         f32 = self.precision == "mixed"
@@ -521,6 +612,59 @@ class LineForest:
                                       L.DCX_PERMUTED | L.DCX_SHARED_RHS | L.DCX_SPARSE_RHS | fl32,
                                       st), "solve")
             gv = sv.grad_values(Lam, U, scale=self.ones, permuted=True)
+            grad = torch.empty_like(sig)
+            L.check(lib.dcx_lmap_transpose(self.lmap, L.ptr(gv), L.ptr(sig), L.ptr(grad), st),
+                    "lmap_transpose")
+        co = self.M["cell_off"]
+        return [(phi[i], grad[co[i]:co[i + 1]]) for i in range(self.nlines)]
+
+    def _eval_grad_electrodes(self, ms):
+        """eval_grad in the electrode basis: one solve, no adjoint solve."""
+        dobs, w = self._data
+        L, lib, sv, dev = self._libm, self.lib, self.solver, self.device
+        f32 = self.precision == "mixed"
+        fl32 = L.DCX_F32_FIELDS if f32 else 0
+        n, K, nb, nl = sv.n, self.K, self.nbatch, self.nlines
+        st = self._stream()
+        with torch.no_grad(), torch.cuda.device(dev):
+            m = torch.cat([torch.as_tensor(x, dtype=torch.float64).reshape(-1).to(dev)
+                           for x in ms])
+            sig = torch.exp(-m)
+            vals = torch.empty(nb, self.M["nnz"], dtype=torch.float64, device=dev)
+            L.check(lib.dcx_assemble(self.lmap, L.ptr(sig), L.ptr(vals), st), "assemble")
+            G = torch.empty(nb, n, K, dtype=torch.float32 if f32 else torch.float64, device=dev)
+            with sv._lock:
+                L.check(lib.dcx_factor_solve(sv._h, L.ptr(vals), L.ptr(self.E), L.ptr(G), int(K),
+                                             L.DCX_PERMUTED | L.DCX_SHARED_RHS |
+                                             L.DCX_SPARSE_RHS | fl32, st), "factor_solve")
+                sv._vals_keep = vals
+            # K_i = sum_k w_(k,i) E_i^T G_k on each line's electrode rows
+            Gs = G[:, self.e_sup, :].to(torch.float64).view(nb, nl, -1, K)
+            Ke = torch.matmul(self.e_sel, Gs)                      # (nb, nl, K, K)
+            Kl = (self.wq_lines.view(nb, nl, 1, 1) * Ke).sum(0)    # (nl, K, K)
+            KS = torch.matmul(Kl, self.S_pad)                      # (nl, K, nS_max)
+            dpad = (self.R_pad * torch.gather(KS, 2, self.sidx_pad)).sum(1)
+            d = dpad[self.dmask]
+            r = w * (d - dobs)
+            phi = torch.zeros(nl, dtype=torch.float64, device=dev)
+            phi.index_add_(0, self.data_line, r ** 2)
+            vpad = torch.zeros_like(dpad)
+            vpad[self.dmask] = (2.0 * r) * w
+            T = torch.zeros(nl, K, self.S_pad.shape[2], dtype=torch.float64, device=dev)
+            T.scatter_add_(2, self.sidx_pad, self.R_pad * vpad.unsqueeze(1))
+            Mt = torch.matmul(self.S_pad, T.transpose(1, 2))       # M_i^T = S_i T_i^T
+            Mw = (self.wq_lines.view(nb, nl, 1, 1) * Mt.unsqueeze(0)).to(G.dtype)
+            H = torch.empty_like(G)
+            if self._mix_segs is not None:
+                # one launch: each line's rows times its own matrix
+                Wseg = Mw[:, self._mix_order].contiguous()
+                L.check(lib.dcx_rows_mix(L.ptr(G), L.ptr(Wseg), L.ptr(H), nb, n, int(K), nl,
+                                         self._mix_segs.ctypes.data_as(L._i32p), fl32, st),
+                        "rows_mix")
+            else:
+                for i, rows in enumerate(self.line_rows):
+                    H[:, rows, :] = torch.matmul(G[:, rows, :], Mw[:, i])
+            gv = sv.grad_values(G, H, scale=self.ones, permuted=True)
             grad = torch.empty_like(sig)
             L.check(lib.dcx_lmap_transpose(self.lmap, L.ptr(gv), L.ptr(sig), L.ptr(grad), st),
                     "lmap_transpose")

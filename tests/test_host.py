@@ -73,6 +73,31 @@ def test_library_exports_header():
     assert set(_lib.PROTOTYPES) <= names
 
 
+def test_rows_mix_rejects_bad_arguments():
+    """dcx_rows_mix checks its sizes and segment table on the host and
+    returns before any device work (the pointers below are never read); a
+    table of empty segments is a no-op."""
+    import ctypes as C
+    lib = _lib.load()
+    fake = C.c_void_p(256)
+
+    def call(n, k, segs, nb=2):
+        a, p = _lib.i32(np.asarray(segs))
+        return lib.dcx_rows_mix(fake, fake, fake, nb, n, k, len(segs) - 1, p,
+                                _lib.DCX_F32_FIELDS, None)
+    assert call(100, 0, [0, 50, 100]) != 0 and b"k must" in lib.dcx_last_error()
+    assert call(100, 129, [0, 50, 100]) != 0
+    assert call(100, 8, [0]) != 0 and b"nseg" in lib.dcx_last_error()
+    assert call(100, 8, list(range(258))) != 0
+    assert call(100, 8, [0, 60, 40]) != 0 and b"ascend" in lib.dcx_last_error()
+    assert call(100, 8, [0, 50, 101]) != 0
+    assert call(100, 8, [-1, 50]) != 0
+    assert call(100, 8, [0, 50], nb=0) != 0
+    assert lib.dcx_rows_mix(None, fake, fake, 2, 100, 8, 1, _lib.i32(np.r_[0, 50])[1], 0,
+                            None) != 0
+    assert call(100, 8, [30, 30, 30]) == 0
+
+
 @pytest.mark.parametrize("coords", [False, True])
 def test_symbolic_structure_and_emulated_solve(coords):
     """Host analysis + a numpy run of the kernel algorithm == SciPy."""

@@ -201,17 +201,18 @@ def test_merge_line_maps_host():
 # mixed: with 3% noise the residual is ~0.03 |d|, so a relative data error e
 # moves phi by ~2 e / 0.03; the 1e-6 data bar allows phi to move by ~7e-5
 @pytest.mark.gpu
+@pytest.mark.parametrize("basis", ["sources", "electrodes"])
 @pytest.mark.parametrize("precision,tol_phi,tol_g", [("fp64", 1e-11, 1e-10),
                                                      ("mixed", 1e-4, 1e-3)])
-def test_line_forest_matches_lines(precision, tol_phi, tol_g):
-    """All lines in one plan against each line's own FP64 engine (autograd
-    eval+grad): three small lines with 6, 4 and 5 wavenumbers (two padded),
-    then the four-line batch of 3k-8k cells; twice in a row, the second at
-    other models."""
+def test_line_forest_matches_lines(precision, tol_phi, tol_g, basis):
+    """All lines in one plan (source or electrode basis) against each line's
+    own FP64 engine in the source basis (autograd eval+grad): three small
+    lines with 6, 4 and 5 wavenumbers (two padded), then the four-line batch
+    of 3k-8k cells; twice in a row, the second at other models."""
     from simpeg_dc_torch_amd import TorchDC2D
     from simpeg_dc_torch_amd.lines import LineForest
     for name, (L, nkys) in (("small", _small_lines()), ("batch4", (_lines(4), [6] * 4))):
-        print(f"line forest {precision} {name}", flush=True)
+        print(f"line forest {precision} {basis} {name}", flush=True)
         engs = [TorchDC2D(mesh, survey, nky=nk, device="cuda")
                 for (mesh, survey, _), nk in zip(L, nkys)]
         data = []
@@ -219,7 +220,8 @@ def test_line_forest_matches_lines(precision, tol_phi, tol_g):
             with torch.no_grad():
                 d = e.dpred(torch.tensor(m, device="cuda")).cpu().numpy()
             data.append(noisy_data(d, seed=9))
-        lf = LineForest([(mesh, survey) for mesh, survey, _ in L], nky=nkys, precision=precision)
+        lf = LineForest([(mesh, survey) for mesh, survey, _ in L], nky=nkys, precision=precision,
+                        rhs_basis=basis)
         lf.set_data([d for d, _ in data], [w for _, w in data])
         rng = np.random.default_rng(6)
         for rep in range(2):
@@ -232,6 +234,40 @@ def test_line_forest_matches_lines(precision, tol_phi, tol_g):
                 gi = out[i][1].cpu().numpy()
                 assert np.linalg.norm(gi - g) <= tol_g * np.linalg.norm(g), (name, rep, i)
         lf.free()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+@pytest.mark.parametrize("k", [1, 40, 84, 128])
+def test_rows_mix_matches_per_segment_products(dtype, k):
+    """dcx_rows_mix against torch per segment: one and two column groups
+    (k <= 64, k > 64), k not a multiple of 4, segments longer and shorter
+    than a workgroup's 64 rows, an empty segment, and rows outside every
+    segment left as they were."""
+    from simpeg_dc_torch_amd import _lib
+    print(f"rows mix {dtype} k={k}", flush=True)
+    lib = _lib.load()
+    nb, n = 3, 1000
+    segs = np.array([5, 6, 200, 200, 333, 900, 997], np.int32)
+    nseg = segs.size - 1
+    gen = torch.Generator(device="cpu").manual_seed(k)
+    G = torch.randn(nb, n, k, generator=gen, dtype=torch.float64).to("cuda", dtype)
+    W = torch.randn(nb, nseg, k, k, generator=gen, dtype=torch.float64).to("cuda", dtype)
+    H = torch.full_like(G, 7.0)
+    flags = _lib.DCX_F32_FIELDS if dtype == torch.float32 else 0
+    _lib.check(lib.dcx_rows_mix(_lib.ptr(G), _lib.ptr(W), _lib.ptr(H), nb, n, k, nseg,
+                                segs.ctypes.data_as(_lib._i32p), flags,
+                                _lib.stream_of(G.device)), "rows_mix")
+    torch.cuda.synchronize()
+    ref = torch.full_like(G, 7.0, dtype=torch.float64)
+    G64, W64 = G.double(), W.double()
+    for l in range(nseg):
+        a, b = int(segs[l]), int(segs[l + 1])
+        ref[:, a:b] = torch.matmul(G64[:, a:b], W64[:, l])
+    tol = 1e-5 if dtype == torch.float32 else 1e-13
+    scale = max(1.0, float(np.sqrt(k)))
+    assert torch.equal(H[:, :5], ref[:, :5].to(dtype)) and torch.equal(H[:, 997:], ref[:, 997:].to(dtype))
+    assert float((H.double() - ref).abs().max()) <= tol * scale * 4
 
 
 @pytest.mark.gpu
