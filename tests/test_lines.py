@@ -238,7 +238,7 @@ def test_line_forest_matches_lines(precision, tol_phi, tol_g, basis):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
-@pytest.mark.parametrize("k", [1, 40, 84, 128])
+@pytest.mark.parametrize("k", [1, 17, 40, 84, 128])
 def test_rows_mix_matches_per_segment_products(dtype, k):
     """dcx_rows_mix against torch per segment: one and two column groups
     (k <= 64, k > 64), k not a multiple of 4, segments longer and shorter
