@@ -389,7 +389,7 @@ class LineForest:
         U'_k = A_k^-1 B_k,  d = sum_k Pd U'_k,
         dphi/dA_k = -Lambda_k U'_k^T with Lambda_k = A_k^-1 Pd^T v.
     Every line is solved with K = max nS source columns (a line with fewer
-    sources carries zero columns).  Full-mesh models only (no active cells).
+    sources carries zero columns).
     The weighted right-hand sides are kept on the device: nbatch x n x K FP64
     (1.7 GB for 16 lines of 251k cells in all, 10 systems, K = 81).
 
@@ -409,10 +409,14 @@ class LineForest:
 
     lines: list of (mesh, survey); nky: an int or one per line; precision:
     "mixed" (FP32 factor, sweeps and fields) or "fp64"; refine, leaf and
-    quadrature as TorchDC2D; rhs_basis "sources" or "electrodes"."""
+    quadrature as TorchDC2D; rhs_basis "sources" or "electrodes";
+    active_cells: None or one entry per line (None or a boolean mask of the
+    line's cells), with val_inactive (a number or one per line) the fixed
+    log-resistivity of the inactive cells, as TorchDC2D: each line's model
+    and gradient then cover its active cells only."""
 
     def __init__(self, lines, device="cuda", nky=12, precision="mixed", refine=0, leaf=None,
-                 quadrature=None, rhs_basis="sources"):
+                 quadrature=None, rhs_basis="sources", active_cells=None, val_inactive=None):
         from .solver import HipBatchSolver
         from . import _lib as lib_mod
         self._libm = lib_mod
@@ -436,6 +440,7 @@ class LineForest:
         self.nD = [m.nD for m in maps]
         self.nC = [m.nC for m in maps]
         self.precision = precision
+        self._active_map(active_cells, val_inactive)
         self.solver = HipBatchSolver(
             torch.tensor(M["crow"]), torch.tensor(M["col"]), M["n"], M["K"], M["nbatch"],
             device=dev, dtype=torch.float64, precision=precision, refine=refine,
@@ -552,6 +557,55 @@ class LineForest:
                 self._mix_order = torch.tensor(order, device=dev)
                 self._mix_segs = np.ascontiguousarray(np.r_[starts, stops[-1]], np.int32)
 
+    def _active_map(self, active_cells, val_inactive):
+        """Indices of the active cells in the stacked full model and the fill
+        of the inactive ones (None when every line is full-mesh)."""
+        L, co = self.nlines, self.M["cell_off"]
+        self._act = None
+        self.n_model = list(self.nC)
+        if active_cells is None or all(a is None for a in active_cells):
+            return
+        if len(active_cells) != L:
+            raise ValueError("active_cells needs one entry (None or a mask) per line")
+        vals = list(val_inactive) if np.iterable(val_inactive) else [val_inactive] * L
+        fill = np.zeros(int(co[-1]))
+        idx = []
+        for i, a in enumerate(active_cells):
+            if a is None:
+                idx.append(np.arange(co[i], co[i + 1]))
+                continue
+            a = np.asarray(a, dtype=bool).reshape(-1)
+            if a.size != self.nC[i]:
+                raise ValueError(f"active_cells[{i}] has {a.size} entries, line has {self.nC[i]} cells")
+            if vals[i] is None:
+                raise ValueError(f"val_inactive is needed for line {i}")
+            fill[co[i]:co[i + 1]][~a] = float(vals[i])
+            idx.append(co[i] + np.where(a)[0])
+            self.n_model[i] = int(a.sum())
+        self._act = torch.tensor(np.concatenate(idx), dtype=torch.int64, device=self.device)
+        self._fill = torch.tensor(fill, dtype=torch.float64, device=self.device)
+        self._moff = np.r_[0, np.cumsum(self.n_model)]
+
+    def _full_model(self, ms):
+        """Stacked full-mesh model (log-resistivity) from the lines' models."""
+        m = torch.cat([torch.as_tensor(x, dtype=torch.float64).reshape(-1).to(self.device)
+                       for x in ms])
+        if self._act is None:
+            return m
+        if m.numel() != self._act.numel():
+            raise ValueError(f"models hold {m.numel()} values, the active cells {self._act.numel()}")
+        full = self._fill.clone()
+        full[self._act] = m
+        return full
+
+    def _per_line(self, phi, grad):
+        """[(phi_i, grad_i)], the gradient restricted to each line's model."""
+        if self._act is None:
+            co = self.M["cell_off"]
+        else:
+            grad, co = grad[self._act], self._moff
+        return [(phi[i], grad[co[i]:co[i + 1]]) for i in range(self.nlines)]
+
     def _stream(self):
         return self._libm.stream_of(self.device)
 
@@ -570,7 +624,8 @@ class LineForest:
         self._data = (cat(dobs), cat(w))
 
     def eval_grad(self, ms):
-        """[(phi_i, grad_i)] for the models `ms` (one per line, m = log rho):
+        """[(phi_i, grad_i)] for the models `ms` (one per line, m = log rho on
+        the line's active cells):
         phi_i = ||w_i (dpred_i - dobs_i)||^2 and its gradient, all lines in
         one launch sequence on the current stream, no host synchronisation."""
         if self._data is None:
@@ -584,9 +639,7 @@ class LineForest:
         n, K, nb = sv.n, self.K, self.nbatch
         st = self._stream()
         with torch.no_grad(), torch.cuda.device(dev):
-            m = torch.cat([torch.as_tensor(x, dtype=torch.float64).reshape(-1).to(dev)
-                           for x in ms])
-            sig = torch.exp(-m)
+            sig = torch.exp(-self._full_model(ms))
             vals = torch.empty(nb, self.M["nnz"], dtype=torch.float64, device=dev)
             L.check(lib.dcx_assemble(self.lmap, L.ptr(sig), L.ptr(vals), st), "assemble")
             U = torch.empty(nb, n, K, dtype=torch.float32 if f32 else torch.float64, device=dev)
@@ -615,8 +668,7 @@ class LineForest:
             grad = torch.empty_like(sig)
             L.check(lib.dcx_lmap_transpose(self.lmap, L.ptr(gv), L.ptr(sig), L.ptr(grad), st),
                     "lmap_transpose")
-        co = self.M["cell_off"]
-        return [(phi[i], grad[co[i]:co[i + 1]]) for i in range(self.nlines)]
+        return self._per_line(phi, grad)
 
     def _eval_grad_electrodes(self, ms):
         """eval_grad in the electrode basis: one solve, no adjoint solve."""
@@ -627,9 +679,7 @@ class LineForest:
         n, K, nb, nl = sv.n, self.K, self.nbatch, self.nlines
         st = self._stream()
         with torch.no_grad(), torch.cuda.device(dev):
-            m = torch.cat([torch.as_tensor(x, dtype=torch.float64).reshape(-1).to(dev)
-                           for x in ms])
-            sig = torch.exp(-m)
+            sig = torch.exp(-self._full_model(ms))
             vals = torch.empty(nb, self.M["nnz"], dtype=torch.float64, device=dev)
             L.check(lib.dcx_assemble(self.lmap, L.ptr(sig), L.ptr(vals), st), "assemble")
             G = torch.empty(nb, n, K, dtype=torch.float32 if f32 else torch.float64, device=dev)
@@ -668,8 +718,7 @@ class LineForest:
             grad = torch.empty_like(sig)
             L.check(lib.dcx_lmap_transpose(self.lmap, L.ptr(gv), L.ptr(sig), L.ptr(grad), st),
                     "lmap_transpose")
-        co = self.M["cell_off"]
-        return [(phi[i], grad[co[i]:co[i + 1]]) for i in range(self.nlines)]
+        return self._per_line(phi, grad)
 
     def free(self):
         if getattr(self, "lmap", None):

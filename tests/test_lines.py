@@ -237,6 +237,48 @@ def test_line_forest_matches_lines(precision, tol_phi, tol_g, basis):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("basis", ["sources", "electrodes"])
+def test_line_forest_active_cells(basis):
+    """Lines with inactive cells (two of three lines masked, one full-mesh,
+    per-line fill values) in one FP64 plan against each line's own engine
+    with the same mask: misfit and active-cell gradient; wrong model sizes
+    and a mask without its fill are refused."""
+    from simpeg_dc_torch_amd import TorchDC2D
+    from simpeg_dc_torch_amd.lines import LineForest
+    print(f"line forest active cells {basis}", flush=True)
+    L, nkys = _small_lines()
+    rng = np.random.default_rng(3)
+    masks = [rng.random(L[0][0].nC) > 0.15, None, rng.random(L[2][0].nC) > 0.3]
+    fills = [np.log(300.0), None, np.log(30.0)]
+    engs, data, ms = [], [], []
+    for (mesh, survey, m), nk, a, v in zip(L, nkys, masks, fills):
+        e = TorchDC2D(mesh, survey, nky=nk, device="cuda", active_cells=a, val_inactive=v)
+        mi = m if a is None else m[a]
+        with torch.no_grad():
+            d = e.dpred(torch.tensor(mi, device="cuda")).cpu().numpy()
+        engs.append(e)
+        data.append(noisy_data(d, seed=4))
+        ms.append(mi + 0.05 * rng.standard_normal(mi.size))
+    lf = LineForest([(mesh, survey) for mesh, survey, _ in L], nky=nkys, precision="fp64",
+                    rhs_basis=basis, active_cells=masks, val_inactive=fills)
+    assert lf.n_model == [m.size for m in ms]
+    lf.set_data([d for d, _ in data], [w for _, w in data])
+    out = lf.eval_grad(ms)
+    torch.cuda.synchronize()
+    for i, e in enumerate(engs):
+        phi, g = _single(e, ms[i], *data[i])
+        assert float(out[i][0]) == pytest.approx(phi, rel=1e-11), i
+        gi = out[i][1].cpu().numpy()
+        assert gi.shape == g.shape and np.linalg.norm(gi - g) <= 1e-10 * np.linalg.norm(g), i
+    with pytest.raises(ValueError):
+        lf.eval_grad([L[0][2], ms[1], ms[2]])
+    lf.free()
+    with pytest.raises(ValueError):
+        LineForest([(mesh, survey) for mesh, survey, _ in L], nky=nkys, precision="fp64",
+                   active_cells=masks, val_inactive=[None, None, 1.0])
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
 @pytest.mark.parametrize("k", [1, 17, 40, 84, 128])
 def test_rows_mix_matches_per_segment_products(dtype, k):
