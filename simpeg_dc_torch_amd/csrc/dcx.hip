@@ -2836,7 +2836,8 @@ __device__ inline void load4<double>(const double* p, double (&v)[4]) {
 // row-major).  A workgroup takes 64 rows of one segment of one system: the
 // rows are staged in LDS (k padded to kp, a multiple of 4, with zeros), lane
 // j owns columns j and j + 64 (NG = 2) for 16 rows of its wave, and W's row a
-// is read once per wave from L2 and used for all 16 rows.
+// is read once per wave from L2 and used for all 16 rows.  A wavefront stages
+// its own 16 rows only (no workgroup barrier).
 template <typename T, int NG>
 __global__ __launch_bounds__(256) void k_rows_mix(const T* __restrict__ G,
                                                   const T* __restrict__ W, T* __restrict__ H,
@@ -2848,29 +2849,63 @@ __global__ __launch_bounds__(256) void k_rows_mix(const T* __restrict__ G,
   while (l + 1 < segs.nseg && segs.tile[l + 1] <= t) ++l;
   const int r0 = segs.row[l] + (t - segs.tile[l]) * kMixRows;
   const int nr = min(kMixRows, segs.row[l + 1] - r0);
-  const T* Gb = G + (static_cast<size_t>(b) * n + r0) * k;
-  for (int i = threadIdx.x; i < kMixRows * kp; i += blockDim.x) {
-    const int rr = i / kp, a = i - rr * kp;
-    g[i] = (rr < nr && a < k) ? Gb[static_cast<size_t>(rr) * k + a] : T(0);
-  }
-  __syncthreads();
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  // each wavefront stages and then reads only its own 16 rows: a wavefront
+  // fence orders its LDS writes before its reads, and no workgroup barrier
+  // couples the wavefronts, so one's loads overlap another's FMAs
+  const T* Gb = G + (static_cast<size_t>(b) * n + r0 + wv * 16) * k;
+  T* gw_st = g + wv * 16 * kp;
+  const int nrw = nr - wv * 16;  // rows of this wavefront inside the tile
+  if (nrw <= 0) return;          // past a short tile's end: no barrier waits for it
+  {
+    // all 32 loads issued before any LDS write (a runtime loop waited on each
+    // load in turn); columns lane and lane + 64 cover kp <= 128
+    T v[16][2];
+#pragma unroll
+    for (int rr = 0; rr < 16; ++rr)
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        const int a = lane + 64 * c;
+        v[rr][c] = (rr < nrw && a < k) ? Gb[static_cast<size_t>(rr) * k + a] : T(0);
+      }
+#pragma unroll
+    for (int rr = 0; rr < 16; ++rr)
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        const int a = lane + 64 * c;
+        if (a < kp) gw_st[rr * kp + a] = v[rr][c];
+      }
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   const T* Wl = W + (static_cast<size_t>(b) * segs.nseg + l) * k * k;
-  const T* gw = g + wv * 16 * kp;
+  const T* gw = gw_st;
   T acc[NG][16];
 #pragma unroll
   for (int c = 0; c < NG; ++c)
 #pragma unroll
     for (int i = 0; i < 16; ++i) acc[c][i] = T(0);
-  for (int a = 0; a < k; a += 4) {
-    T w[NG][4];
+  // W's rows a..a+3 for the next step are loaded one step ahead, so the FMAs
+  // of a step do not wait on that step's L2 reads
+  T wn[NG][4];
+  auto load_w = [&](int a0) {
 #pragma unroll
     for (int c = 0; c < NG; ++c) {
       const int j = lane + 64 * c;
 #pragma unroll
       for (int q = 0; q < 4; ++q)
-        w[c][q] = (j < k && a + q < k) ? Wl[static_cast<size_t>(a + q) * k + j] : T(0);
+        wn[c][q] = (j < k && a0 + q < k) ? Wl[static_cast<size_t>(a0 + q) * k + j] : T(0);
     }
+  };
+  load_w(0);
+  for (int a = 0; a < k; a += 4) {
+    T w[NG][4];
+#pragma unroll
+    for (int c = 0; c < NG; ++c)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) w[c][q] = wn[c][q];
+    load_w(a + 4);  // rows past k load nothing
 #pragma unroll
     for (int i = 0; i < 16; ++i) {
       T gv[4];
