@@ -198,6 +198,36 @@ def test_merge_line_maps_host():
         assert np.abs(got - di).max() <= 1e-10 * np.abs(di).max(), i
 
 
+def test_merged_lines_order_component_by_component():
+    """The host symbolic analysis of a merged batch (the ordering the plan
+    builds) keeps each line's rows contiguous in the elimination order, keeps
+    every supernode and its parent inside one line (the elimination tree is a
+    forest of the lines' own trees), and has one root per line: the layout
+    the one-plan batch relies on for one row-mix launch per step."""
+    from simpeg_dc_torch_amd import TorchDC2D, _lib
+    from simpeg_dc_torch_amd.lines import merge_line_maps
+    from simpeg_dc_torch_amd.solver import DEFAULT_LEAF
+    for L in (_small_lines()[0], _lines(6, (2000, 9000), seed=2)):
+        engs = [TorchDC2D(mesh, survey, nky=4, device="cpu") for mesh, survey, _ in L]
+        M = merge_line_maps([e._maps for e in engs], [e.kys for e in engs],
+                            [e.wq_np for e in engs])
+        sym = _lib.symbolic(M["crow"], M["col"], coords=M["coords"], leaf_max=DEFAULT_LEAF)
+        pos = np.empty(M["n"], np.int64)
+        pos[sym["perm"]] = np.arange(M["n"])
+        line_of_pos = M["node_line"][sym["perm"]]
+        for i in range(len(L)):
+            r = np.sort(pos[M["node_off"][i]:M["node_off"][i + 1]])
+            assert r[-1] - r[0] + 1 == r.size, i
+        sup_line = line_of_pos[sym["first"]]
+        for s_, (f, ns) in enumerate(zip(sym["first"], sym["ns"])):
+            assert np.all(line_of_pos[f:f + ns] == sup_line[s_]), s_
+            p_ = sym["parent"][s_]
+            if p_ >= 0:
+                assert sup_line[p_] == sup_line[s_], s_
+        roots = sup_line[sym["parent"] < 0]
+        assert sorted(roots.tolist()) == list(range(len(L)))
+
+
 # mixed: with 3% noise the residual is ~0.03 |d|, so a relative data error e
 # moves phi by ~2 e / 0.03; the 1e-6 data bar allows phi to move by ~7e-5
 @pytest.mark.gpu
