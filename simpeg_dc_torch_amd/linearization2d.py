@@ -155,7 +155,8 @@ class Linearization2D:
                 Ge = self._electrode_fields()
                 T = torch.zeros(g.ne, eng.nS, dtype=torch.float64, device=eng.device)
                 T.index_add_(1, g.sidx, g.R * vv)
-                H = torch.matmul(Ge, (T @ g.S.T).T.contiguous())
+                from .simulation2d import _rows_times
+                H = _rows_times(g, Ge, (T @ g.S.T).T)
                 gv = g.solver.grad_values(Ge, H, scale=g.wq, permuted=True)
                 return self._to_model(g.lmap_t(gv, self.sig))
             G = g.adjoint_rhs(vv, eng.nS, 0, eng.nS)

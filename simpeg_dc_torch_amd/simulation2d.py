@@ -227,6 +227,23 @@ def _electrodes_forward(eng, m_full):
     return d, (sig, vals, G)
 
 
+def _rows_times(g, G, Wt):
+    """G_b Wt for every system b (G: nbatch x n x ne, Wt: ne x ne): one
+    dcx_rows_mix launch over all rows when ne <= 128, else a batched GEMM."""
+    nb, n, ne = G.shape
+    W = Wt.to(G.dtype)
+    if ne > 128:
+        return torch.matmul(G, W.contiguous())
+    W = W.unsqueeze(0).expand(nb, ne, ne).contiguous()
+    H = torch.empty_like(G)
+    seg = np.array([0, n], np.int32)
+    flags = _lib.DCX_F32_FIELDS if G.dtype == torch.float32 else 0
+    _lib.check(g.lib.dcx_rows_mix(_lib.ptr(G), _lib.ptr(W), _lib.ptr(H), nb, n, ne, 1,
+                                  seg.ctypes.data_as(_lib._i32p), flags, g.stream()),
+               "rows_mix")
+    return H
+
+
 def _electrodes_backward(eng, saved, stamp, gd):
     sig, vals, G = saved
     g = eng._gpu
@@ -237,7 +254,7 @@ def _electrodes_backward(eng, saved, stamp, gd):
     T = torch.zeros(g.ne, eng.nS, dtype=torch.float64, device=eng.device)
     T.index_add_(1, g.sidx, g.R * v)
     M = T @ g.S.T
-    H = torch.matmul(G, M.T.to(G.dtype).contiguous())
+    H = _rows_times(g, G, M.T)
     gv = g.solver.grad_values(G, H, scale=g.wq, permuted=True)
     return g.lmap_t(gv, sig)
 
