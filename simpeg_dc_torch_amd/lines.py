@@ -536,7 +536,9 @@ class LineForest:
         self.S_pad = torch.tensor(S_pad, dtype=torch.float64, device=dev)
         self.R_pad = torch.tensor(R_pad, dtype=torch.float64, device=dev)
         self.sidx_pad = torch.tensor(sidx_pad, device=dev).unsqueeze(1).expand(L, Ke, nD_max)
-        self.dmask = torch.tensor(dmask, device=dev)
+        # flat positions of the real data in the padded (line, datum) block:
+        # integer indexing, where a boolean mask would synchronise the host
+        self.dflat = torch.tensor(np.flatnonzero(dmask), dtype=torch.int64, device=dev)
         self.wq_lines = torch.tensor(M["wq"], dtype=torch.float64, device=dev)  # (nbatch, L)
         # each line's rows in elimination order: one row segment per line
         # (dcx_rows_mix) when every line's rows are contiguous, else gathers
@@ -694,12 +696,12 @@ class LineForest:
             Kl = (self.wq_lines.view(nb, nl, 1, 1) * Ke).sum(0)    # (nl, K, K)
             KS = torch.matmul(Kl, self.S_pad)                      # (nl, K, nS_max)
             dpad = (self.R_pad * torch.gather(KS, 2, self.sidx_pad)).sum(1)
-            d = dpad[self.dmask]
+            d = dpad.reshape(-1)[self.dflat]
             r = w * (d - dobs)
             phi = torch.zeros(nl, dtype=torch.float64, device=dev)
             phi.index_add_(0, self.data_line, r ** 2)
             vpad = torch.zeros_like(dpad)
-            vpad[self.dmask] = (2.0 * r) * w
+            vpad.view(-1)[self.dflat] = (2.0 * r) * w
             T = torch.zeros(nl, K, self.S_pad.shape[2], dtype=torch.float64, device=dev)
             T.scatter_add_(2, self.sidx_pad, self.R_pad * vpad.unsqueeze(1))
             Mt = torch.matmul(self.S_pad, T.transpose(1, 2))       # M_i^T = S_i T_i^T
