@@ -47,14 +47,32 @@ Pinning (what fixes each function independently of this file)
   dctorch/tests/test_solver.py:19-25;
 * gradient / Jtvec / Jmatrix: central finite differences of `dpred`, the
   adjoint identity, Jmatrix rows == Jtvec(e_i).
-* The Robin coefficient alpha(ky) and boundary-face layout are recalled from
-  SimPEG 0.25 (not importable here): parity with stock SimPEG itself is
-  unpinned; the half-space check bounds their effect.
+* the Robin term (alpha(ky) and the boundary-face layout): per wavenumber,
+  the solution for a surface pole at the Robin source point on an unpadded
+  mesh converges at second order to the analytic (rho/pi) K0(ky r) on the
+  side and bottom boundary nodes; alpha = 0, 2 alpha and alpha without
+  r_hat . n_hat stall at 16-300% errors there (tests/test_oracle.py).
+* Parity with stock SimPEG itself is unpinned (simpeg is not importable
+  here): the formulas are recalled from SimPEG 0.25 and pinned by the
+  physics above, not by SimPEG's own outputs.
 """
 import numpy as np
 import scipy.sparse as sp
 import scipy.sparse.linalg as spla
 from scipy.special import k0e, k1e
+
+
+# --------------------------------------------------------------------------
+# Robin coefficient (SimPEG 0.25 Simulation2DNodal, bc_type="Robin")
+# --------------------------------------------------------------------------
+def robin_alpha(ky, r, rdn):
+    """alpha of one boundary face: ky K1(ky r)/K0(ky r) (r_hat . n_hat).
+
+    The 2.5-D field of a surface pole at the source point decays as
+    K0(ky r), so d u/d n = -alpha u holds exactly for it on every boundary
+    face; tests/test_oracle.py pins this against the analytic K0 field
+    (with mutants of this function that must fail)."""
+    return ky * k1e(ky * r) / k0e(ky * r) * rdn
 
 
 # --------------------------------------------------------------------------
@@ -167,7 +185,7 @@ class Grid2D:
             dx, dz = fx - sx, fz - sz
             rr = np.hypot(dx, dz)
             rdn = (dx * nrm[0] + dz * nrm[1]) / rr
-            alpha = ky * k1e(ky * rr) / k0e(ky * rr) * rdn
+            alpha = robin_alpha(ky, rr, rdn)
             for nd in (na, nb):
                 r.append(nd); c.append(cc); v.append(0.5 * alpha * L)
         return sp.csr_matrix((v, (r, c)), shape=(self.nN, self.nC))

@@ -36,13 +36,24 @@ Pinning
   falling as the cells halve;
 * gradient and Jtvec: central finite differences of the misfit and of
   dpred (tests/test_host.py).
-* The Robin coefficient and the source point are recalled from SimPEG 0.25
-  (not importable here): parity with stock SimPEG is unpinned; the
-  half-space check bounds their effect.
+* the Robin term: a surface pole at the Robin source point on an unpadded
+  mesh converges at second order to rho / (2 pi r) on the side and bottom
+  boundary nodes; alpha = 0, 2 alpha and alpha without r_hat . n_hat stall
+  (tests/test_oracle.py).  Parity with stock SimPEG itself is unpinned
+  (simpeg is not importable here).
 """
 import numpy as np
 import scipy.sparse as sp
 import scipy.sparse.linalg as spla
+
+
+def robin_alpha_3d(r, rdn):
+    """alpha of a boundary face: (r_hat . n_hat) / r (SimPEG 0.25
+    Simulation3DNodal, bc_type="Robin").  The field of a surface pole at the
+    source point decays as 1/r, so d u/d n = -alpha u holds exactly for it;
+    tests/test_oracle.py pins this against rho / (2 pi r) with mutants that
+    must fail."""
+    return rdn / r
 
 
 class Grid3D:
@@ -161,7 +172,7 @@ class Grid3D:
                 continue
             rv = centre - src
             r = np.linalg.norm(rv, axis=1)
-            alpha = (rv @ normal) / r / r
+            alpha = robin_alpha_3d(r, (rv @ normal) / r)
             for j in range(4):
                 rows.append(nd4[:, j]); cols.append(cc); vals.append(0.25 * alpha * area)
         return sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),

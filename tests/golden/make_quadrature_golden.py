@@ -26,6 +26,11 @@ CASES = [  # (r_min, r_max, nky)
     (5.0, 120.0, 12),
     (5.0, 35.0, 11),
     (5.0, 120.0, 31),
+    (2.0, 48.0, 12),     # the 25,088-cell line (eval+grad benchmark): 10 points kept
+    (5.0, 180.0, 12),    # the 62,016-cell Gauss-Newton line
+    (2.0, 16.0, 12),     # the uneven batch of 16 lines
+    (3.0, 24.0, 11),     # the 5,000-cell regression line with the electrode fit
+    (2.857142857142854, 17.142857142857146, 6),  # the smoke / suite case
 ]
 
 
