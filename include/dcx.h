@@ -47,8 +47,11 @@ enum {
   DCX_SHARED_RHS = 1,  /* b is one (n x k) block shared by every batch member */
   DCX_PERMUTED = 2,    /* b and x are in the plan's elimination order */
   DCX_SPARSE_RHS = 4,  /* b is zero outside the rows set by dcx_set_rhs_support */
-  DCX_F32_FIELDS = 8   /* the field blocks (dcx_solve's x, dcx_grad_values' y and
+  DCX_F32_FIELDS = 8,  /* the field blocks (dcx_solve's x, dcx_grad_values' y and
                           x, dcx_survey_dpred's U) are float, not double */
+  DCX_CHECK_DEFER = 16 /* the solve's check is launched but not waited for: its
+                          verdict (and any refinement) comes from
+                          dcx_plan_check_wait; b and x must stay valid until then */
 };
 
 typedef struct {
@@ -62,6 +65,9 @@ typedef struct {
   int64_t panel_size;         /* factor panel elements per batch member */
   int level_launches;         /* level launches per sweep */
   int nfused;                 /* fused bottom subtrees */
+  int64_t buffer_generation;  /* bumped whenever a workspace buffer is reallocated: work
+                                 recorded in a HIP graph before a change holds stale
+                                 addresses */
 } dcx_plan_info;
 
 typedef struct dcx_symbolic dcx_symbolic;
@@ -128,6 +134,38 @@ int dcx_plan_get_tuning(const dcx_plan* plan, int key, int* value);
 int dcx_plan_set_fusion(dcx_plan* plan, int fuse_rows);
 /* number of FP64 refinement passes after the first FP32 solve (mixed mode) */
 int dcx_plan_set_refinement(dcx_plan* plan, int passes);
+
+/* Backward-error check of every solve (on by default; tol <= 0 turns it off;
+ * skipped while the stream is being captured).  After each dcx_solve /
+ * dcx_factor_solve, eta_b = |B - A_b X| / (|A_b| |X| + |B|) (Frobenius norms,
+ * FP64 residual, over the first sample_cols <= 64 columns) and the pivot flags
+ * of the factorisation are read back (one stream synchronisation).  An FP32
+ * plan whose eta_b exceeds tol for some b adds FP64-residual refinement
+ * passes, up to max_passes; then, or at once for a non-positive pivot or an
+ * FP64 plan, the call returns DCX_ERR_NUMERIC with the solution of the last
+ * pass in x, and dcx_plan_get_check names the failing systems (the caller
+ * refactorises those in FP64).  Defaults: tol 1e-5 (FP32) / 1e-10 (FP64), or
+ * DCX_CHECK_TOL at plan creation; max_passes 2; sample_cols 8.  Counterpart
+ * of the reference's guarded solve, a residual check followed by a fallback
+ * (dctorch/optimization.py:753-761); cuDSS reports pivot failures itself. */
+int dcx_plan_set_check(dcx_plan* plan, double tol, int max_passes, int sample_cols);
+/* Verdict of a deferred check (DCX_CHECK_DEFER): waits for it, runs the
+ * refinement passes a failing FP32 solve gets, and returns DCX_OK or
+ * DCX_ERR_NUMERIC as the solve itself would have; DCX_OK when nothing is
+ * pending.  The next solve or factorisation of the plan completes a pending
+ * check first.  No reference counterpart (see dcx_plan_set_check). */
+int dcx_plan_check_wait(dcx_plan* plan);
+/* Result of the last check: eta and pivot_fail (host, nbatch each, may be
+ * NULL), refinement passes run, status (0 none yet, 1 passed, 2 failed,
+ * 3 skipped under stream capture or with the check off). */
+int dcx_plan_get_check(const dcx_plan* plan, double* eta, int* pivot_fail, int* passes,
+                       int* status);
+/* Fault injection for tests: later factorisations of batch member `batch`
+ * factor (1 + rel) A_b instead of A_b (the solve's FP64 residual keeps A_b),
+ * a factor with backward error about rel / 2; rel = 0 disarms.  No reference
+ * counterpart (the reference's failure test clobbers a shared factor,
+ * dctorch/tests/test_linearization.py:56-57). */
+int dcx_plan_inject_fault(dcx_plan* plan, int batch, double rel);
 
 /* Numeric factorisation of all nbatch matrices (values: device, FP64,
  * nbatch x nnz).  Replaces CuDSSBatchSolver.factorize

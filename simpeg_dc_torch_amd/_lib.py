@@ -14,7 +14,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("DCX_LIBRARY") or os.path.join(_HERE, "libdcx.so")
 
 DCX_FP64, DCX_FP32 = 0, 1
-DCX_SHARED_RHS, DCX_PERMUTED, DCX_SPARSE_RHS, DCX_F32_FIELDS = 1, 2, 4, 8
+DCX_SHARED_RHS, DCX_PERMUTED, DCX_SPARSE_RHS, DCX_F32_FIELDS, DCX_CHECK_DEFER = 1, 2, 4, 8, 16
 DCX_TUNE_SWEEP_TC, DCX_TUNE_SPLITK_BELOW, DCX_TUNE_WIDE_BELOW, DCX_TUNE_PANEL_WIDTH = 1, 2, 3, 4
 
 _i32p = C.POINTER(C.c_int32)
@@ -28,7 +28,8 @@ class PlanInfo(C.Structure):
                 ("precision", C.c_int), ("nnz_L", C.c_int64), ("factor_flops", C.c_double),
                 ("solve_flops_per_rhs", C.c_double), ("device_bytes", C.c_int64),
                 ("sweep_rows_nf", C.c_int64), ("sweep_rows_ns", C.c_int64),
-                ("panel_size", C.c_int64), ("level_launches", C.c_int), ("nfused", C.c_int)]
+                ("panel_size", C.c_int64), ("level_launches", C.c_int), ("nfused", C.c_int),
+                ("buffer_generation", C.c_int64)]
 
 
 class SymbolicView(C.Structure):
@@ -61,6 +62,11 @@ PROTOTYPES = {
     "dcx_plan_get_perm": (C.c_int, [_vp, _i32p]),
     "dcx_plan_set_refinement": (C.c_int, [_vp, C.c_int]),
     "dcx_plan_set_fusion": (C.c_int, [_vp, C.c_int]),
+    "dcx_plan_set_check": (C.c_int, [_vp, C.c_double, C.c_int, C.c_int]),
+    "dcx_plan_get_check": (C.c_int, [_vp, _f64p, _i32p, C.POINTER(C.c_int),
+                                     C.POINTER(C.c_int)]),
+    "dcx_plan_inject_fault": (C.c_int, [_vp, C.c_int, C.c_double]),
+    "dcx_plan_check_wait": (C.c_int, [_vp]),
     "dcx_plan_set_timing": (C.c_int, [_vp, C.c_int]),
     "dcx_plan_set_tuning": (C.c_int, [_vp, C.c_int, C.c_int]),
     "dcx_plan_get_tuning": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_int)]),
@@ -121,10 +127,19 @@ def load():
     return lib
 
 
+DCX_ERR_NUMERIC = -4
+
+
+class NumericError(RuntimeError):
+    """A solve failed its backward-error check or met a non-positive pivot
+    (status DCX_ERR_NUMERIC); the plan's check report names the systems."""
+
+
 def check(status, what=""):
     if status != 0:
         msg = _lib.dcx_last_error().decode(errors="replace") if _lib else "unknown"
-        raise RuntimeError(f"dcx {what} failed ({status}): {msg}")
+        cls = NumericError if status == DCX_ERR_NUMERIC else RuntimeError
+        raise cls(f"dcx {what} failed ({status}): {msg}")
 
 
 def i32(a):

@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <memory>
@@ -346,7 +347,7 @@ __global__ __launch_bounds__(NT) void k_factor_level(
     }
     // (1) the jb x jb diagonal block, by wavefront 0 alone (lanes < jb hold
     // its rows)
-    if (wave == 0) diag_block<T, NB>(rv[0], jb, lane, ldg, dgi, info);
+    if (wave == 0) diag_block<T, NB>(rv[0], jb, lane, ldg, dgi, info + b);
     __syncthreads();
     // (2) every row below the block, on the thread's own registers
 #pragma unroll
@@ -628,7 +629,7 @@ __global__ __launch_bounds__(64) void k_diag_big(const SupDev* __restrict__ sup,
   const int f = blockIdx.x / nbatch, b = blockIdx.x % nbatch;
   const SupDev S = sup[lvl[f]];
   const int ld = 2 * S.ns + S.rs;
-  panel_diag<T>(fcur + b * fcur_bs + S.front_off, ld, min(kNB2, S.ns), sm, info);
+  panel_diag<T>(fcur + b * fcur_bs + S.front_off, ld, min(kNB2, S.ns), sm, info + b);
   T* o = dg + static_cast<int64_t>(blockIdx.x) * kDg;
   for (int i = threadIdx.x; i < kDg; i += 64) o[i] = sm[i];
 }
@@ -782,7 +783,8 @@ __global__ __launch_bounds__(256) void k_trail_outer(const SupDev* __restrict__ 
   if (blockIdx.x == 0 && blockIdx.y == 0 && j0 + jb < ns) {
     __shared__ T sm[kDg];
     __syncthreads();  // the tile's stores by every wavefront before wavefront 0 reads them
-    panel_diag<T>(Pn + jb + static_cast<int64_t>(jb) * ld, ld, min(kNB2, ns - j0 - jb), sm, info);
+    panel_diag<T>(Pn + jb + static_cast<int64_t>(jb) * ld, ld, min(kNB2, ns - j0 - jb), sm,
+                  info + b);
     T* o = dg + static_cast<int64_t>(blockIdx.z) * kDg;
     for (int q = threadIdx.x; q < kDg; q += 256) o[q] = sm[q];
   }
@@ -854,7 +856,8 @@ __global__ __launch_bounds__(256) void k_trail_big(const SupDev* __restrict__ su
   if (blockIdx.x == 0 && blockIdx.y == 0 && j0 + jb < ns) {
     __shared__ T sm[kDg];
     __syncthreads();  // the tile's stores by every wavefront before wavefront 0 reads them
-    panel_diag<T>(Pn + jb + static_cast<int64_t>(jb) * ld, ld, min(kNB2, ns - j0 - jb), sm, info);
+    panel_diag<T>(Pn + jb + static_cast<int64_t>(jb) * ld, ld, min(kNB2, ns - j0 - jb), sm,
+                  info + b);
     T* o = dg + static_cast<int64_t>(blockIdx.z) * kDg;
     for (int q = threadIdx.x; q < kDg; q += 256) o[q] = sm[q];
   }
@@ -1735,8 +1738,18 @@ __global__ void k_axpy_f32(const float* __restrict__ c, double* __restrict__ x, 
     x[i] += static_cast<double>(c[i]);
 }
 
-__global__ void k_zero_int(int* p) {
-  if (threadIdx.x == 0) p[0] = 0;
+__global__ void k_zero_int(int* p, int n) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0;
+}
+
+// fv[b] = vals[b] * (b == batch ? scale : 1): the values the factorisation
+// reads when a fault is injected into one batch member's factor
+__global__ void k_scale_vals(const double* __restrict__ vals, double* __restrict__ fv,
+                             int64_t nnz, int nb, int batch, double scale) {
+  const int64_t total = nnz * nb;
+  for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < total;
+       i += static_cast<int64_t>(gridDim.x) * blockDim.x)
+    fv[i] = i / nnz == batch ? vals[i] * scale : vals[i];
 }
 
 __global__ void k_fill(double* __restrict__ x, int64_t n, double v) {
@@ -1763,6 +1776,122 @@ __global__ void k_residual(const int* __restrict__ crow, const int* __restrict__
       acc -= vb[vidx ? vidx[p] : p] * X[xb + static_cast<int64_t>(ccol[p]) * k + c];
     R[xb + static_cast<int64_t>(row) * k + c] = acc;
   }
+}
+
+// Backward-error check of a solve on a sample of its columns: for batch
+// member b and the first cs columns (cs <= LPR), sums of r^2 (r = B - A_b X
+// in FP64), X^2 and B^2 over every row, and of A_b's squared entries.  LPR
+// lanes per row (32 or 64), kChkRows rows per thread; the first ME entries of
+// each row are loaded in three rounds for all of a thread's rows together
+// (indices, then values and fields, then the arithmetic), so the loads of
+// different rows overlap instead of chaining; entries past ME (longer rows
+// than the stencil's) take a plain loop.  Each workgroup writes its four
+// partial sums to part[b][blockIdx.x], summed in a fixed order by
+// k_check_sum.
+constexpr int kChkRows = 2;
+template <typename TX, int LPR, int ME>
+__global__ __launch_bounds__(256) void k_check(const int* __restrict__ crow,
+                                               const int* __restrict__ ccol,
+                                               const int* __restrict__ vidx,
+                                               const double* __restrict__ vals, int64_t nnz,
+                                               const double* __restrict__ B, int64_t b_bs,
+                                               const TX* __restrict__ X, int n, int k, int cs,
+                                               double* __restrict__ part) {
+  constexpr int RPB = 256 / LPR;  // rows per pass of the workgroup
+  constexpr int R = kChkRows;
+  __shared__ double red[4][4];
+  const int b = blockIdx.y, lane = threadIdx.x % LPR, sub = threadIdx.x / LPR;
+  const int64_t xb = static_cast<int64_t>(b) * n * k;
+  const double* vb = vals + b * nnz;
+  const int c = lane < cs ? lane : 0;   // lanes past the sample read column 0, count zero
+  int row[R], p0[R], p1[R];
+#pragma unroll
+  for (int i = 0; i < R; ++i) {
+    row[i] = (blockIdx.x * R + i) * RPB + sub;
+    const bool ok = row[i] < n;
+    p0[i] = ok ? crow[row[i]] : 0;
+    p1[i] = ok ? crow[row[i] + 1] : 0;
+  }
+  int ci[R][ME], vi[R][ME];
+#pragma unroll
+  for (int i = 0; i < R; ++i)
+#pragma unroll
+    for (int e = 0; e < ME; ++e) {
+      const int p = p0[i] + e;
+      const bool live = p < p1[i];
+      ci[i][e] = live ? ccol[p] : -1;
+      vi[i][e] = live ? (vidx ? vidx[p] : p) : -1;
+    }
+  double av[R][ME], bv[R];
+  TX xv[R][ME], xd[R];
+#pragma unroll
+  for (int i = 0; i < R; ++i) {
+    const bool ok = row[i] < n;
+    bv[i] = ok ? B[b * b_bs + static_cast<int64_t>(row[i]) * k + c] : 0.0;
+    xd[i] = ok ? X[xb + static_cast<int64_t>(row[i]) * k + c] : TX(0);
+#pragma unroll
+    for (int e = 0; e < ME; ++e) {
+      av[i][e] = vi[i][e] >= 0 ? vb[vi[i][e]] : 0.0;
+      xv[i][e] = ci[i][e] >= 0 ? X[xb + static_cast<int64_t>(ci[i][e]) * k + c] : TX(0);
+    }
+  }
+  double v[4] = {0.0, 0.0, 0.0, 0.0};
+  const bool in = lane < cs;
+#pragma unroll
+  for (int i = 0; i < R; ++i) {
+    if (row[i] >= n) continue;
+    double acc = bv[i];
+#pragma unroll
+    for (int e = 0; e < ME; ++e) acc -= av[i][e] * static_cast<double>(xv[i][e]);
+    double a2 = 0.0;
+#pragma unroll
+    for (int e = 0; e < ME; ++e) a2 += av[i][e] * av[i][e];
+    for (int p = p0[i] + ME; p < p1[i]; ++p) {  // rows longer than ME
+      const double a = vb[vidx ? vidx[p] : p];
+      acc -= a * static_cast<double>(X[xb + static_cast<int64_t>(ccol[p]) * k + c]);
+      a2 += a * a;
+    }
+    const double xr = static_cast<double>(xd[i]);
+    if (in) {
+      v[0] += acc * acc;
+      v[1] += xr * xr;
+      v[2] += bv[i] * bv[i];
+    }
+    if (lane == 0) v[3] += a2;
+  }
+  const int wl = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+    for (int o = 32; o > 0; o >>= 1) v[i] += __shfl_down(v[i], o, 64);
+  if (wl == 0)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) red[wave][i] = v[i];
+  __syncthreads();
+  if (threadIdx.x < 4)
+    part[(static_cast<int64_t>(b) * gridDim.x + blockIdx.x) * 4 + threadIdx.x] =
+        red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] + red[3][threadIdx.x];
+}
+
+// out[b][i] = sum over the nblk partial sums part[b][.][i] (one workgroup per
+// batch member; thread t sums blocks t, t + 256, ... then a fixed tree)
+__global__ __launch_bounds__(256) void k_check_sum(const double* __restrict__ part, int nblk,
+                                                   double* __restrict__ out) {
+  __shared__ double red[256][4];
+  const int b = blockIdx.x, t = threadIdx.x;
+  double v[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int q = t; q < nblk; q += 256)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) v[i] += part[(static_cast<int64_t>(b) * nblk + q) * 4 + i];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) red[t][i] = v[i];
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (t < h)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) red[t][i] += red[t + h][i];
+    __syncthreads();
+  }
+  if (t < 4) out[4 * b + t] = red[0][t];
 }
 
 // Y[b] = alpha A_b X[b] (FP64; A_b from vals[b], CSR rows in the storage
@@ -2097,7 +2226,37 @@ struct dcx_plan {
   int *d_perm = nullptr, *d_iperm = nullptr, *d_pcrow = nullptr, *d_pcol = nullptr,
       *d_pvidx = nullptr;
   int *d_crow = nullptr, *d_col = nullptr;
-  int* d_info = nullptr;
+  int* d_info = nullptr;  // per batch member: a pivot was not positive
+  // backward-error check of every solve (dcx_plan_set_check)
+  double chk_tol = 0.0;     // <= 0: off
+  int chk_passes = 2;       // refinement passes a failing check may add (FP32 plans)
+  int chk_cols = 8;         // sample columns (<= 64)
+  double* d_chk = nullptr;  // nbatch x 4 sums of k_check
+  void* h_chk = nullptr;    // pinned host: nbatch x 4 sums, then nbatch info ints
+  std::vector<double> chk_eta;
+  std::vector<int> chk_pivot;
+  int chk_passes_done = 0;
+  int chk_status = 0;       // 0 not run, 1 passed, 2 failed, 3 skipped (stream capture)
+  // a deferred check (DCX_CHECK_DEFER): launched, its verdict read by
+  // dcx_plan_check_wait; the solve it belongs to
+  bool chk_pending = false;
+  hipEvent_t chk_ev = nullptr;     // the check's results are on the host
+  hipEvent_t chk_src = nullptr;    // the solve is done (the check stream waits on it)
+  hipStream_t chk_st = nullptr;    // stream of deferred checks (default priority)
+  size_t chk_tot_off = 0;          // offset of the totals in d_chk
+  const double* pend_B = nullptr;
+  int64_t pend_bbs = 0;
+  void* pend_xv = nullptr;
+  int pend_k = 0, pend_flags = 0;
+  hipStream_t pend_st = nullptr;
+  // fault injection (tests): the factorisation reads d_fvals, in which batch
+  // member inj_batch is scaled by 1 + inj_rel
+  int inj_batch = -1;
+  double inj_rel = 0.0;
+  double* d_fvals = nullptr;
+  // incremented whenever a workspace buffer is reallocated (kernels recorded
+  // in a graph before that point hold stale addresses)
+  int64_t buf_gen = 0;
   // factor
   void* d_panels = nullptr;
   void* d_panelsT = nullptr;
@@ -2146,11 +2305,18 @@ struct dcx_plan {
                     d_perm, d_iperm, d_pcrow, d_pcol, d_pvidx, d_crow, d_col, d_info, d_panels,
                     d_front[0], d_front[1], d_vals, d_active, d_act_lvl, d_upd[0],
                     d_upd[1], d_w, d_r, d_b, d_x, d_fz, d_flv, d_fmem, d_fgptr, d_fgb,
-                    d_toff, d_lridx, d_ulvl, d_ufz, d_panelsT, d_fs_off, d_fs, d_dg};
+                    d_toff, d_lridx, d_ulvl, d_ufz, d_panelsT, d_fs_off, d_fs, d_dg,
+                    d_chk, d_fvals};
     for (void* p : ptrs)
       if (p) (void)hipFree(p);
+    if (h_chk) (void)hipHostFree(h_chk);
+    if (chk_ev) (void)hipEventDestroy(chk_ev);
+    if (chk_src) (void)hipEventDestroy(chk_src);
+    if (chk_st) (void)hipStreamDestroy(chk_st);
   }
 };
+
+static int complete_pending(dcx_plan* P);
 
 namespace {
 
@@ -2166,8 +2332,28 @@ int default_fuse_rows(int precision) {
   return 0;  // fused subtrees off by default (slower than the level kernels at this occupancy)
 }
 
-int ensure(void** p, size_t* cap, size_t bytes, hipStream_t st) {
+// a workspace buffer may not grow while the stream is being captured (the
+// synchronisation and the free are not capturable, and a graph recorded
+// before would keep the freed address)
+int refuse_in_capture(hipStream_t st) {
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  HIP_TRY(hipStreamIsCapturing(st, &cs));
+  if (cs != hipStreamCaptureStatusNone)
+    return fail(DCX_ERR_STATE,
+                "a workspace buffer would grow during stream capture: run the same call "
+                "once before capturing it");
+  return DCX_OK;
+}
+
+bool capturing(hipStream_t st) {
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  return hipStreamIsCapturing(st, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone;
+}
+
+int ensure(dcx_plan* P, void** p, size_t* cap, size_t bytes, hipStream_t st) {
   if (*cap >= bytes && *p) return DCX_OK;
+  if (int rc = refuse_in_capture(st)) return rc;
+  ++P->buf_gen;
   HIP_TRY(hipStreamSynchronize(st));
   if (*p) HIP_TRY(hipFree(*p));
   *p = nullptr;
@@ -2211,6 +2397,13 @@ int upload_fused(dcx_plan* P) {
   HIP_TRY(upload(&P->d_ulvl, S.ulvl_sup));
   return DCX_OK;
 }
+
+// default backward-error tolerances of the solve check (DCX_CHECK_TOL
+// overrides): FP32 factors reach about 1e-7 on the benchmark problems, FP64
+// ones about 1e-16; a failing check means a factor that lost accuracy or
+// definiteness, not an ordinary rounding error
+constexpr double kCheckTol32 = 1e-5;
+constexpr double kCheckTol64 = 1e-10;
 
 constexpr size_t kLdsBudget = 64 * 1024;
 constexpr size_t kFusedLdsBudget = 80 * 1024;  // two workgroups per CU
@@ -2276,6 +2469,11 @@ std::vector<int> build_factor_order(dcx_plan* P) {
   return order;
 }
 
+// the values the factorisation reads: the caller's, or the fault-injected copy
+inline const double* factor_vals(const dcx_plan* P) {
+  return P->d_fvals && P->inj_batch >= 0 ? P->d_fvals : P->d_vals;
+}
+
 template <typename T, int NT, int RPT, int NB = kNB>
 void factor_launch(dcx_plan* P, int d, const int* lvl, int cnt, size_t lds, hipStream_t st) {
   const auto& S = P->S;
@@ -2284,7 +2482,7 @@ void factor_launch(dcx_plan* P, int d, const int* lvl, int cnt, size_t lds, hipS
   dim3 grid(cnt, P->nbatch);
   hipLaunchKernelGGL((k_factor_level<T, NT, RPT, NB>), grid, dim3(NT), lds, st, P->d_sup,
                      lvl, P->d_relpos, P->d_cidx, P->d_a_e, P->d_a_row,
-                     P->d_a_col, P->d_vals, static_cast<int64_t>(S.nnz), fr[d & 1], fbs[d & 1],
+                     P->d_a_col, factor_vals(P), static_cast<int64_t>(S.nnz), fr[d & 1], fbs[d & 1],
                      fr[(d + 1) & 1], fbs[(d + 1) & 1], static_cast<T*>(P->d_panels),
                      static_cast<T*>(P->d_panelsT), S.panel_size, P->d_info);
 }
@@ -2311,7 +2509,7 @@ int factor_level_big(dcx_plan* P, int d, int cnt, hipStream_t st) {
   if (static_cast<int64_t>(cnt) * nb > 65535)
     return fail(DCX_ERR_ARG, "too many large fronts in one level");
   hipLaunchKernelGGL(k_front_zero_big<T>, dim3((maxnf + kZeroCols - 1) / kZeroCols, cnt * nb),
-                     dim3(256), 0, st, P->d_sup, lvl, P->d_a_e, P->d_a_row, P->d_a_col, P->d_vals,
+                     dim3(256), 0, st, P->d_sup, lvl, P->d_a_e, P->d_a_row, P->d_a_col, factor_vals(P),
                      static_cast<int64_t>(S.nnz), fr[d & 1], fbs[d & 1], nb);
   for (int c = 0; c < maxkids; ++c) {
     if (!maxrc[c]) continue;
@@ -2598,6 +2796,8 @@ int ensure_upd(dcx_plan* P, int k, hipStream_t st) {
   const auto& S = P->S;
   const size_t fsb = static_cast<size_t>(P->nbatch) * big_fwd_rows(P, sizeof(T)) * k * sizeof(T);
   if (fsb > P->fs_bytes) {
+    if (int rc = refuse_in_capture(st)) return rc;
+    ++P->buf_gen;
     HIP_TRY(hipStreamSynchronize(st));
     if (P->d_fs) HIP_TRY(hipFree(P->d_fs));
     P->d_fs = nullptr;
@@ -2609,6 +2809,8 @@ int ensure_upd(dcx_plan* P, int k, hipStream_t st) {
   const size_t ub0 = static_cast<size_t>(P->nbatch) * S.upd_rows[0] * k * sizeof(T);
   const size_t ub1 = static_cast<size_t>(P->nbatch) * S.upd_rows[1] * k * sizeof(T);
   const size_t ubf = static_cast<size_t>(P->nbatch) * S.f_upd_rows * k * sizeof(T);
+  if (int rc = refuse_in_capture(st)) return rc;
+  ++P->buf_gen;
   HIP_TRY(hipStreamSynchronize(st));
   void* olds[] = {P->d_upd[0], P->d_upd[1], P->d_ufz};
   for (void* q : olds)
@@ -3009,7 +3211,11 @@ int dcx_plan_create(int n, const int32_t* crow, const int32_t* col, int nbatch,
     for (int i = 0; i < n; ++i) P->max_row_nnz = std::max(P->max_row_nnz, crow[i + 1] - crow[i]);
     HIP_TRY(upload(&P->d_crow, vcrow));
     HIP_TRY(upload(&P->d_col, vcol));
-    HIP_TRY(hipMalloc(&P->d_info, sizeof(int)));
+    HIP_TRY(hipMalloc(&P->d_info, sizeof(int) * nbatch));
+    {
+      const char* e = std::getenv("DCX_CHECK_TOL");
+      P->chk_tol = e ? std::atof(e) : (precision == DCX_FP32 ? kCheckTol32 : kCheckTol64);
+    }
     const size_t pb = static_cast<size_t>(nbatch) * S.panel_size * P->tsize;
     const size_t f0 = static_cast<size_t>(nbatch) * S.front_size[0] * P->tsize;
     const size_t f1 = static_cast<size_t>(nbatch) * S.front_size[1] * P->tsize;
@@ -3030,6 +3236,43 @@ int dcx_plan_create(int n, const int32_t* crow, const int32_t* col, int nbatch,
   } catch (const std::exception& e) {
     return fail(DCX_ERR_INTERNAL, e.what());
   }
+}
+
+int dcx_plan_set_check(dcx_plan* P, double tol, int max_passes, int sample_cols) {
+  if (!P) return fail(DCX_ERR_ARG, "null plan");
+  if (max_passes < 0 || max_passes > 16) return fail(DCX_ERR_ARG, "max_passes must be 0..16");
+  if (sample_cols < 1 || sample_cols > 64) return fail(DCX_ERR_ARG, "sample_cols must be 1..64");
+  P->chk_tol = tol;
+  P->chk_passes = max_passes;
+  P->chk_cols = sample_cols;
+  return DCX_OK;
+}
+
+int dcx_plan_get_check(const dcx_plan* P, double* eta, int* pivot_fail, int* passes,
+                       int* status) {
+  if (!P) return fail(DCX_ERR_ARG, "null plan");
+  for (int b = 0; b < P->nbatch; ++b) {
+    const bool run = static_cast<int>(P->chk_eta.size()) == P->nbatch;
+    if (eta) eta[b] = run ? P->chk_eta[b] : 0.0;
+    if (pivot_fail) pivot_fail[b] = run ? P->chk_pivot[b] : 0;
+  }
+  if (passes) *passes = P->chk_passes_done;
+  if (status) *status = P->chk_status;
+  return DCX_OK;
+}
+
+int dcx_plan_check_wait(dcx_plan* P) {
+  if (!P) return fail(DCX_ERR_ARG, "null plan");
+  HIP_TRY(hipSetDevice(P->device));
+  return complete_pending(P);
+}
+
+int dcx_plan_inject_fault(dcx_plan* P, int batch, double rel) {
+  if (!P) return fail(DCX_ERR_ARG, "null plan");
+  if (batch >= P->nbatch) return fail(DCX_ERR_ARG, "batch member out of range");
+  P->inj_batch = rel != 0.0 ? batch : -1;
+  P->inj_rel = rel;
+  return DCX_OK;
 }
 
 int dcx_plan_destroy(dcx_plan* plan) {
@@ -3066,6 +3309,7 @@ int dcx_plan_get_info(const dcx_plan* P, dcx_plan_info* o) {
   o->panel_size = S.panel_size;
   o->level_launches = nl;
   o->nfused = S.nfused;
+  o->buffer_generation = P->buf_gen;
   return DCX_OK;
 }
 
@@ -3198,21 +3442,251 @@ int dcx_set_rhs_support(dcx_plan* P, const int32_t* rows, int nrows) {
   return DCX_OK;
 }
 
+namespace {
+// copy the caller's values, clear the pivot flags, and prepare the values the
+// factorisation reads (a fault-injected copy when one is armed)
+static int load_values(dcx_plan* P, const double* values, hipStream_t st) {
+  const size_t vb = static_cast<size_t>(P->nbatch) * P->S.nnz * sizeof(double);
+  HIP_TRY(hipMemcpyAsync(P->d_vals, values, vb, hipMemcpyDeviceToDevice, st));
+  hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(256), 0, st, P->d_info, P->nbatch);
+  if (P->inj_batch >= 0) {
+    if (!P->d_fvals) {
+      if (int rc = refuse_in_capture(st)) return rc;
+      HIP_TRY(hipMalloc(&P->d_fvals, std::max<size_t>(vb, 256)));
+    }
+    hipLaunchKernelGGL(k_scale_vals, dim3(grid_stride(static_cast<int64_t>(P->nbatch) * P->S.nnz)),
+                       dim3(256), 0, st, P->d_vals, P->d_fvals, static_cast<int64_t>(P->S.nnz),
+                       P->nbatch, P->inj_batch, 1.0 + P->inj_rel);
+  }
+  HIP_TRY(hipGetLastError());
+  return DCX_OK;
+}
+
+// The backward-error check of a finished solve (X: its solution, float or
+// double, rows in elimination order when `permuted`): k_check on the first
+// chk_cols columns, the sums and the pivot flags to pinned host memory, one
+// synchronisation.  eta_b = |R| / (|A_b| |X| + |B|) (Frobenius norms over the
+// sample) for every batch member; returns the number failing (eta > tol or a
+// pivot that was not positive).
+static int launch_check(dcx_plan* P, const double* B, int64_t b_bs, const void* X, bool xf32,
+                        bool permuted, int k, hipStream_t st) {
+  const auto& S = P->S;
+  const int nb = P->nbatch;
+  const int cs = std::min(k, std::min(P->chk_cols, 64));
+  const int lpr = cs > 32 ? 64 : cs > 16 ? 32 : cs > 8 ? 16 : 8;
+  const int nblk = grid1(S.n, (256 / lpr) * kChkRows);
+  if (!P->d_chk) {
+    // partial sums (nb x nblk x 4, nblk at its largest: 64 lanes per row)
+    // followed by the totals (nb x 4)
+    const size_t mx = grid1(S.n, 4 * kChkRows);
+    HIP_TRY(hipMalloc(&P->d_chk, sizeof(double) * 4 * nb * (mx + 1)));
+    HIP_TRY(hipHostMalloc(&P->h_chk, (sizeof(double) * 4 + sizeof(int)) * nb));
+    P->chk_tot_off = static_cast<size_t>(4) * nb * mx;
+  }
+  double* part = P->d_chk;
+  double* tot = P->d_chk + P->chk_tot_off;
+  const int* rp = permuted ? P->d_pcrow : P->d_crow;
+  const int* cl = permuted ? P->d_pcol : P->d_col;
+  const int* vi = permuted ? P->d_pvidx : nullptr;
+  dim3 grid(nblk, nb);
+  auto go = [&](auto kern, const auto* Xp) {
+    hipLaunchKernelGGL(kern, grid, dim3(256), 0, st, rp, cl, vi, P->d_vals,
+                       static_cast<int64_t>(S.nnz), B, b_bs, Xp, S.n, k, cs, part);
+  };
+  const float* Xf = static_cast<const float*>(X);
+  const double* Xd = static_cast<const double*>(X);
+  // entries per row loaded together: 5 (2-D stencils), else 8
+  const bool wide = P->max_row_nnz > 5;
+  auto pick = [&](auto tx, const auto* Xp) {
+    using TX = decltype(tx);
+    switch (lpr) {
+      case 8: wide ? go(k_check<TX, 8, 8>, Xp) : go(k_check<TX, 8, 5>, Xp); break;
+      case 16: wide ? go(k_check<TX, 16, 8>, Xp) : go(k_check<TX, 16, 5>, Xp); break;
+      case 32: wide ? go(k_check<TX, 32, 8>, Xp) : go(k_check<TX, 32, 5>, Xp); break;
+      default: wide ? go(k_check<TX, 64, 8>, Xp) : go(k_check<TX, 64, 5>, Xp); break;
+    }
+  };
+  if (xf32) pick(0.0f, Xf); else pick(0.0, Xd);
+  hipLaunchKernelGGL(k_check_sum, dim3(nb), dim3(256), 0, st, part, nblk, tot);
+  HIP_TRY(hipGetLastError());
+  double* hs = static_cast<double*>(P->h_chk);
+  int* hi = reinterpret_cast<int*>(hs + 4 * nb);
+  HIP_TRY(hipMemcpyAsync(hs, tot, sizeof(double) * 4 * nb, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipMemcpyAsync(hi, P->d_info, sizeof(int) * nb, hipMemcpyDeviceToHost, st));
+  return DCX_OK;
+}
+
+// the check's verdict from the host copies (after the stream or the check
+// event has been waited on); returns the number of failing systems
+static int eval_check(dcx_plan* P) {
+  const int nb = P->nbatch;
+  const double* hs = static_cast<const double*>(P->h_chk);
+  const int* hi = reinterpret_cast<const int*>(hs + 4 * nb);
+  P->chk_eta.assign(nb, 0.0);
+  P->chk_pivot.assign(nb, 0);
+  int bad = 0;
+  for (int b = 0; b < nb; ++b) {
+    const double r = std::sqrt(hs[4 * b]), x = std::sqrt(hs[4 * b + 1]),
+                 bn = std::sqrt(hs[4 * b + 2]), a = std::sqrt(hs[4 * b + 3]);
+    const double den = a * x + bn;
+    const double eta = den > 0.0 ? r / den : (r > 0.0 ? INFINITY : 0.0);
+    P->chk_eta[b] = std::isfinite(eta) ? eta : INFINITY;
+    P->chk_pivot[b] = hi[b] != 0;
+    if (P->chk_pivot[b] || !(P->chk_eta[b] <= P->chk_tol)) ++bad;
+  }
+  return bad;
+}
+
+// launch_check, one synchronisation, eval_check; a negative value is a
+// status (HIP error)
+static int run_check(dcx_plan* P, const double* B, int64_t b_bs, const void* X, bool xf32,
+                     bool permuted, int k, hipStream_t st) {
+  if (int rc = launch_check(P, B, b_bs, X, xf32, permuted, k, st)) return rc;
+  HIP_TRY(hipStreamSynchronize(st));
+  return eval_check(P);
+}
+
+static std::string check_message(const dcx_plan* P) {
+  std::string m;
+  for (int b = 0; b < P->nbatch; ++b) {
+    if (!P->chk_pivot[b] && P->chk_eta[b] <= P->chk_tol) continue;
+    char buf[96];
+    std::snprintf(buf, sizeof(buf), "%s system %d: %s backward error %.2e", m.empty() ? "" : ";",
+                  b, P->chk_pivot[b] ? "non-positive pivot," : "", P->chk_eta[b]);
+    m += buf;
+  }
+  return m;
+}
+}  // namespace
+
 int dcx_factorize(dcx_plan* P, const double* values, void* stream) {
   if (!P || !values) return fail(DCX_ERR_ARG, "null argument");
   hipStream_t st = static_cast<hipStream_t>(stream);
   HIP_TRY(hipSetDevice(P->device));
-  const size_t vb = static_cast<size_t>(P->nbatch) * P->S.nnz * sizeof(double);
-  HIP_TRY(hipMemcpyAsync(P->d_vals, values, vb, hipMemcpyDeviceToDevice, st));
-  hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, st, P->d_info);
+  if (int rc0 = complete_pending(P)) return rc0;
+  if (int rc = load_values(P, values, st)) return rc;
   int rc = P->precision == DCX_FP32 ? launch_factor<float>(P, st) : launch_factor<double>(P, st);
   if (rc) return rc;
   P->factored = true;
   return DCX_OK;
 }
 
+// one refinement pass of an FP32 plan: r = B - A x (FP64), W = A^{-1} r
+// (FP32 sweeps), x += W (FP64)
+static int mixed_refine_pass(dcx_plan* P, const double* B, int64_t bbs, double* x, float* W,
+                             int k, bool permuted, hipStream_t st) {
+  const auto& S = P->S;
+  const int nb = P->nbatch, n = S.n;
+  const int64_t blk = static_cast<int64_t>(n) * k;
+  int rc = ensure(P, reinterpret_cast<void**>(&P->d_r), &P->r_bytes,
+                  static_cast<size_t>(nb) * blk * 8, st);
+  if (rc) return rc;
+  dim3 rgrid(grid1(n, 4), nb);
+  if (permuted)
+    hipLaunchKernelGGL(k_residual, rgrid, dim3(256), 0, st, P->d_pcrow, P->d_pcol, P->d_pvidx,
+                       P->d_vals, static_cast<int64_t>(S.nnz), B, bbs, x, P->d_r, n, k);
+  else
+    hipLaunchKernelGGL(k_residual, rgrid, dim3(256), 0, st, P->d_crow, P->d_col,
+                       static_cast<const int*>(nullptr), P->d_vals,
+                       static_cast<int64_t>(S.nnz), B, bbs, x, P->d_r, n, k);
+  SweepIO ir;
+  ir.B = P->d_r;
+  ir.b_bs = blk;
+  ir.brow = permuted ? nullptr : P->d_perm;
+  ir.xrow = permuted ? nullptr : P->d_perm;
+  ir.X64 = x;
+  ir.xmode = 2;
+  return sweeps<float>(P, W, k, false, ir, st);
+}
+
+// After a check that found `bad` failing systems (the verdict in chk_*): an
+// FP64 plan or a non-positive pivot fails at once; an FP32 plan gets up to
+// chk_passes refinement passes, each followed by a check.  B, xv, k, flags:
+// the solve's right-hand side and solution as the caller passed them (B the
+// copy when b and x alias).
+static int finish_check(dcx_plan* P, int bad, const double* B, int64_t bbs, void* xv, int k,
+                        int flags, hipStream_t st) {
+  const bool permuted = flags & DCX_PERMUTED, f32x = (flags & DCX_F32_FIELDS) != 0;
+  const int nb = P->nbatch;
+  const int64_t blk = static_cast<int64_t>(P->S.n) * k;
+  double* x = f32x ? (P->refine > 0 ? P->d_x : nullptr) : static_cast<double*>(xv);
+  float* W = f32x ? static_cast<float*>(xv) : static_cast<float*>(P->d_w);
+  for (int extra = 0;; ++extra) {
+    if (bad < 0) return bad;
+    P->chk_passes_done = (P->precision == DCX_FP32 ? P->refine : 0) + extra;
+    if (bad == 0) {
+      P->chk_status = 1;
+      return DCX_OK;
+    }
+    bool pivot = false;
+    for (int v : P->chk_pivot) pivot = pivot || v;
+    if (P->precision == DCX_FP64) {
+      P->chk_status = 2;
+      return fail(DCX_ERR_NUMERIC,
+                  "FP64 solve failed its check (matrix not positive definite?):" +
+                      check_message(P));
+    }
+    if (pivot || extra >= P->chk_passes) {
+      P->chk_status = 2;
+      return fail(DCX_ERR_NUMERIC, "FP32 solve failed its check after " +
+                                       std::to_string(P->chk_passes_done) +
+                                       " refinement pass(es):" + check_message(P));
+    }
+    if (f32x && !x) {  // first pass on FP32 fields: an FP64 iterate from them
+      int rc = ensure(P, reinterpret_cast<void**>(&P->d_x), &P->x_bytes,
+                      static_cast<size_t>(nb) * blk * 8, st);
+      if (rc) return rc;
+      x = P->d_x;
+      hipLaunchKernelGGL((k_convert<float, double>), dim3(grid_stride(nb * blk)), dim3(256), 0,
+                         st, W, x, nb * blk);
+    }
+    int rc = mixed_refine_pass(P, B, bbs, x, W, k, permuted, st);
+    if (rc) return rc;
+    if (f32x)
+      hipLaunchKernelGGL((k_convert<double, float>), dim3(grid_stride(nb * blk)), dim3(256), 0,
+                         st, x, W, nb * blk);
+    HIP_TRY(hipGetLastError());
+    bad = run_check(P, B, bbs, f32x ? static_cast<const void*>(W) : x, f32x, permuted, k, st);
+  }
+}
+
+// the check of a finished solve: now (one synchronisation) or, with
+// DCX_CHECK_DEFER, launched with its verdict left to dcx_plan_check_wait
+static int start_check(dcx_plan* P, const double* B, int64_t bbs, void* xv, int k, int flags,
+                       hipStream_t st) {
+  const bool permuted = flags & DCX_PERMUTED, f32x = (flags & DCX_F32_FIELDS) != 0;
+  P->chk_status = 0;
+  if (flags & DCX_CHECK_DEFER) {
+    // on the plan's check stream, after the solve: the work the caller
+    // queues next on `st` does not wait for it
+    if (!P->chk_st) {
+      HIP_TRY(hipStreamCreateWithFlags(&P->chk_st, hipStreamNonBlocking));
+      HIP_TRY(hipEventCreateWithFlags(&P->chk_ev, hipEventDisableTiming));
+      HIP_TRY(hipEventCreateWithFlags(&P->chk_src, hipEventDisableTiming));
+    }
+    HIP_TRY(hipEventRecord(P->chk_src, st));
+    HIP_TRY(hipStreamWaitEvent(P->chk_st, P->chk_src, 0));
+    if (int rc = launch_check(P, B, bbs, xv, f32x, permuted, k, P->chk_st)) return rc;
+    HIP_TRY(hipEventRecord(P->chk_ev, P->chk_st));
+    P->chk_pending = true;
+    P->pend_B = B;
+    P->pend_bbs = bbs;
+    P->pend_xv = xv;
+    P->pend_k = k;
+    P->pend_flags = flags;
+    P->pend_st = st;
+    return DCX_OK;
+  }
+  return finish_check(P, run_check(P, B, bbs, xv, f32x, permuted, k, st), B, bbs, xv, k, flags,
+                      st);
+}
+
 // dcx_solve and dcx_factor_solve (with_factor: the factorisation runs here,
-// overlapped with the first forward sweep)
+// overlapped with the first forward sweep).  With the check on (and the
+// stream not being captured) every solve ends with run_check; an FP32 plan
+// whose check fails adds refinement passes (FP64 residual, FP32 correction)
+// up to chk_passes, then reports DCX_ERR_NUMERIC; an FP64 plan reports at
+// once.  A non-positive pivot is reported without refinement.
 static int solve_impl(dcx_plan* P, const double* b, void* xv, int k, int flags, hipStream_t st,
                bool with_factor) {
   const auto& S = P->S;
@@ -3225,6 +3699,9 @@ static int solve_impl(dcx_plan* P, const double* b, void* xv, int k, int flags, 
     return fail(DCX_ERR_ARG, "DCX_F32_FIELDS needs an FP32 plan and DCX_PERMUTED");
   if (f32x && static_cast<const void*>(b) == xv)
     return fail(DCX_ERR_ARG, "b and a float x must not alias");
+  if (int rc0 = complete_pending(P)) return rc0;
+  const bool check = P->chk_tol > 0.0 && !capturing(st);
+  P->chk_status = check ? 0 : 3;
   double* x = f32x ? nullptr : static_cast<double*>(xv);
   const int64_t blk = static_cast<int64_t>(S.n) * k;
   const int n = S.n;
@@ -3233,38 +3710,10 @@ static int solve_impl(dcx_plan* P, const double* b, void* xv, int k, int flags, 
   io.b_bs = shared ? 0 : blk;
   io.brow = permuted ? nullptr : P->d_perm;
   io.xrow = permuted ? nullptr : P->d_perm;
-  if (P->precision == DCX_FP64) {
-    double* W = x;
-    if (!permuted) {
-      int rc = ensure(&P->d_w, &P->w_bytes, static_cast<size_t>(nb) * blk * 8, st);
-      if (rc) return rc;
-      W = static_cast<double*>(P->d_w);
-      io.X64 = x;
-      io.xmode = 1;
-    }
-    return with_factor ? factor_and_sweeps<double>(P, W, k, sparse, io, st)
-                       : sweeps<double>(P, W, k, sparse, io, st);
-  }
-  // mixed: FP32 sweeps, FP64 residual refinement; the solution accumulates in
-  // x (double) or, for a float x, in a plan buffer rounded into x at the end
   int rc;
-  float* W;
-  if (f32x) {
-    W = static_cast<float*>(xv);
-    if (P->refine == 0)
-      return with_factor ? factor_and_sweeps<float>(P, W, k, sparse, io, st)
-                         : sweeps<float>(P, W, k, sparse, io, st);
-    rc = ensure(reinterpret_cast<void**>(&P->d_x), &P->x_bytes,
-                static_cast<size_t>(nb) * blk * 8, st);
-    if (rc) return rc;
-    x = P->d_x;
-  } else {
-    rc = ensure(&P->d_w, &P->w_bytes, static_cast<size_t>(nb) * blk * 4, st);
-    if (rc) return rc;
-    W = static_cast<float*>(P->d_w);
-  }
-  if (P->refine > 0 && b == x) {  // the residual needs the right-hand side intact
-    rc = ensure(reinterpret_cast<void**>(&P->d_b), &P->b_bytes,
+  // the residual (refinement, check) needs the right-hand side intact
+  if ((check || (P->precision == DCX_FP32 && P->refine > 0)) && b == x) {
+    rc = ensure(P, reinterpret_cast<void**>(&P->d_b), &P->b_bytes,
                 static_cast<size_t>(shared ? 1 : nb) * blk * 8, st);
     if (rc) return rc;
     HIP_TRY(hipMemcpyAsync(P->d_b, b, (shared ? 1 : nb) * blk * 8, hipMemcpyDeviceToDevice, st));
@@ -3272,37 +3721,60 @@ static int solve_impl(dcx_plan* P, const double* b, void* xv, int k, int flags, 
   }
   const double* Borig = io.B;
   const int64_t Bbs = io.b_bs;
+  if (P->precision == DCX_FP64) {
+    double* W = x;
+    if (!permuted) {
+      rc = ensure(P, &P->d_w, &P->w_bytes, static_cast<size_t>(nb) * blk * 8, st);
+      if (rc) return rc;
+      W = static_cast<double*>(P->d_w);
+      io.X64 = x;
+      io.xmode = 1;
+    }
+    rc = with_factor ? factor_and_sweeps<double>(P, W, k, sparse, io, st)
+                     : sweeps<double>(P, W, k, sparse, io, st);
+    if (rc || !check) return rc;
+    return start_check(P, Borig, Bbs, xv, k, flags, st);
+  }
+  // mixed: FP32 sweeps, FP64 residual refinement; the solution accumulates in
+  // x (double) or, for a float x, in a plan buffer rounded into x at the end
+  float* W;
+  if (f32x) {
+    W = static_cast<float*>(xv);
+    if (P->refine > 0) {
+      rc = ensure(P, reinterpret_cast<void**>(&P->d_x), &P->x_bytes,
+                  static_cast<size_t>(nb) * blk * 8, st);
+      if (rc) return rc;
+      x = P->d_x;
+    }
+  } else {
+    rc = ensure(P, &P->d_w, &P->w_bytes, static_cast<size_t>(nb) * blk * 4, st);
+    if (rc) return rc;
+    W = static_cast<float*>(P->d_w);
+  }
   io.X64 = x;
-  io.xmode = 1;
+  io.xmode = x ? 1 : 0;
   rc = with_factor ? factor_and_sweeps<float>(P, W, k, sparse, io, st)
                    : sweeps<float>(P, W, k, sparse, io, st);
   if (rc) return rc;
-  if (P->refine > 0) {
-    rc = ensure(reinterpret_cast<void**>(&P->d_r), &P->r_bytes, static_cast<size_t>(nb) * blk * 8, st);
-    if (rc) return rc;
-  }
-  dim3 rgrid(grid1(n, 4), nb);
   for (int it = 0; it < P->refine; ++it) {
-    if (permuted)
-      hipLaunchKernelGGL(k_residual, rgrid, dim3(256), 0, st, P->d_pcrow, P->d_pcol,
-                         P->d_pvidx, P->d_vals, static_cast<int64_t>(S.nnz), Borig, Bbs, x,
-                         P->d_r, n, k);
-    else
-      hipLaunchKernelGGL(k_residual, rgrid, dim3(256), 0, st, P->d_crow, P->d_col,
-                         static_cast<const int*>(nullptr), P->d_vals,
-                         static_cast<int64_t>(S.nnz), Borig, Bbs, x, P->d_r, n, k);
-    SweepIO ir = io;
-    ir.B = P->d_r;
-    ir.b_bs = blk;
-    ir.xmode = 2;
-    rc = sweeps<float>(P, W, k, false, ir, st);
+    rc = mixed_refine_pass(P, Borig, Bbs, x, W, k, permuted, st);
     if (rc) return rc;
   }
-  if (f32x)
+  if (f32x && x)
     hipLaunchKernelGGL((k_convert<double, float>), dim3(grid_stride(nb * blk)), dim3(256), 0, st,
                        x, W, nb * blk);
   HIP_TRY(hipGetLastError());
-  return DCX_OK;
+  if (!check) return DCX_OK;
+  return start_check(P, Borig, Bbs, xv, k, flags, st);
+}
+
+// complete a deferred check (its solve's verdict, refinement, status)
+static int complete_pending(dcx_plan* P) {
+  if (!P->chk_pending) return DCX_OK;
+  P->chk_pending = false;
+  HIP_TRY(hipEventSynchronize(P->chk_ev));
+  return finish_check(P, eval_check(P), P->pend_B, P->pend_bbs, P->pend_xv, P->pend_k,
+                      P->pend_flags, P->pend_st);
 }
 
 int dcx_solve(dcx_plan* P, const double* b, void* xv, int k, int flags, void* stream) {
@@ -3320,9 +3792,8 @@ int dcx_factor_solve(dcx_plan* P, const double* values, const double* b, void* x
   if (k < 1) return fail(DCX_ERR_ARG, "k must be >= 1");
   hipStream_t st = static_cast<hipStream_t>(stream);
   HIP_TRY(hipSetDevice(P->device));
-  const size_t vb = static_cast<size_t>(P->nbatch) * P->S.nnz * sizeof(double);
-  HIP_TRY(hipMemcpyAsync(P->d_vals, values, vb, hipMemcpyDeviceToDevice, st));
-  hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, st, P->d_info);
+  if (int rc0 = complete_pending(P)) return rc0;
+  if (int rc = load_values(P, values, st)) return rc;
   P->factored = true;
   return solve_impl(P, b, xv, k, flags, st, true);
 }

@@ -645,28 +645,36 @@ class LineForest:
             vals = torch.empty(nb, self.M["nnz"], dtype=torch.float64, device=dev)
             L.check(lib.dcx_assemble(self.lmap, L.ptr(sig), L.ptr(vals), st), "assemble")
             U = torch.empty(nb, n, K, dtype=torch.float32 if f32 else torch.float64, device=dev)
+            # the solves' checks are deferred and waited for where their results
+            # are used; systems refactorised in FP64 have their results redone
             with sv._lock:
-                L.check(lib.dcx_factor_solve(sv._h, L.ptr(vals), L.ptr(self.B), L.ptr(U), int(K),
-                                             L.DCX_PERMUTED | L.DCX_SPARSE_RHS | fl32, st),
-                        "factor_solve")
-                sv._vals_keep = vals
+                sv.run_factor_solve(vals, self.B, U, K, L.DCX_PERMUTED | L.DCX_SPARSE_RHS | fl32,
+                                    defer=True)
             d = torch.empty(self.M["nD"], dtype=torch.float64, device=dev)
-            L.check(lib.dcx_survey_dpred(self.survey, L.ptr(U), int(K), nb, L.ptr(self.ones), 0,
-                                         int(K), L.ptr(d), L.DCX_PERMUTED | fl32, st),
-                    "survey_dpred")
-            r = w * (d - dobs)
+            G = torch.empty(n, K, dtype=torch.float64, device=dev)
+
+            def project():
+                L.check(lib.dcx_survey_dpred(self.survey, L.ptr(U), int(K), nb, L.ptr(self.ones),
+                                             0, int(K), L.ptr(d), L.DCX_PERMUTED | fl32, st),
+                        "survey_dpred")
+                r = w * (d - dobs)
+                gd = ((2.0 * r) * w).contiguous()
+                L.check(lib.dcx_survey_adjoint_rhs(self.survey, L.ptr(gd), L.ptr(G), int(K), 0,
+                                                   int(K), L.DCX_PERMUTED, st), "adjoint_rhs")
+                return r
+            r = project()
+            if sv.check_wait():
+                r = project()
             phi = torch.zeros(self.nlines, dtype=torch.float64, device=dev)
             phi.index_add_(0, self.data_line, r ** 2)
-            gd = ((2.0 * r) * w).contiguous()
-            G = torch.empty(n, K, dtype=torch.float64, device=dev)
-            L.check(lib.dcx_survey_adjoint_rhs(self.survey, L.ptr(gd), L.ptr(G), int(K), 0,
-                                               int(K), L.DCX_PERMUTED, st), "adjoint_rhs")
             Lam = torch.empty_like(U)
             with sv._lock:
-                L.check(lib.dcx_solve(sv._h, L.ptr(G), L.ptr(Lam), int(K),
-                                      L.DCX_PERMUTED | L.DCX_SHARED_RHS | L.DCX_SPARSE_RHS | fl32,
-                                      st), "solve")
+                sv.run_solve(G, Lam, K,
+                             L.DCX_PERMUTED | L.DCX_SHARED_RHS | L.DCX_SPARSE_RHS | fl32,
+                             defer=True)
             gv = sv.grad_values(Lam, U, scale=self.ones, permuted=True)
+            if sv.check_wait():
+                gv = sv.grad_values(Lam, U, scale=self.ones, permuted=True)
             grad = torch.empty_like(sig)
             L.check(lib.dcx_lmap_transpose(self.lmap, L.ptr(gv), L.ptr(sig), L.ptr(grad), st),
                     "lmap_transpose")
@@ -686,10 +694,24 @@ class LineForest:
             L.check(lib.dcx_assemble(self.lmap, L.ptr(sig), L.ptr(vals), st), "assemble")
             G = torch.empty(nb, n, K, dtype=torch.float32 if f32 else torch.float64, device=dev)
             with sv._lock:
-                L.check(lib.dcx_factor_solve(sv._h, L.ptr(vals), L.ptr(self.E), L.ptr(G), int(K),
-                                             L.DCX_PERMUTED | L.DCX_SHARED_RHS |
-                                             L.DCX_SPARSE_RHS | fl32, st), "factor_solve")
-                sv._vals_keep = vals
+                sv.run_factor_solve(vals, self.E, G, K, L.DCX_PERMUTED | L.DCX_SHARED_RHS |
+                                    L.DCX_SPARSE_RHS | fl32, defer=True)
+            phi, gv = self._electrode_rest(G, dobs, w)
+            if sv.check_wait():     # systems of G were solved again in FP64
+                phi, gv = self._electrode_rest(G, dobs, w)
+            grad = torch.empty_like(sig)
+            L.check(lib.dcx_lmap_transpose(self.lmap, L.ptr(gv), L.ptr(sig), L.ptr(grad), st),
+                    "lmap_transpose")
+        return self._per_line(phi, grad)
+
+    def _electrode_rest(self, G, dobs, w):
+        """Everything of the electrode-basis eval+grad after the solve G:
+        per-line phi and the value gradient gv."""
+        L, lib, sv, dev = self._libm, self.lib, self.solver, self.device
+        fl32 = L.DCX_F32_FIELDS if G.dtype == torch.float32 else 0
+        n, K, nb, nl = sv.n, self.K, self.nbatch, self.nlines
+        st = self._stream()
+        with torch.no_grad(), torch.cuda.device(dev):
             # K_i = sum_k w_(k,i) E_i^T G_k on each line's electrode rows
             Gs = G[:, self.e_sup, :].to(torch.float64).view(nb, nl, -1, K)
             Ke = torch.matmul(self.e_sel, Gs)                      # (nb, nl, K, K)
@@ -717,10 +739,7 @@ class LineForest:
                 for i, rows in enumerate(self.line_rows):
                     H[:, rows, :] = torch.matmul(G[:, rows, :], Mw[:, i])
             gv = sv.grad_values(G, H, scale=self.ones, permuted=True)
-            grad = torch.empty_like(sig)
-            L.check(lib.dcx_lmap_transpose(self.lmap, L.ptr(gv), L.ptr(sig), L.ptr(grad), st),
-                    "lmap_transpose")
-        return self._per_line(phi, grad)
+        return phi, gv
 
     def free(self):
         if getattr(self, "lmap", None):

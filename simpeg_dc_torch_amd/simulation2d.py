@@ -132,18 +132,19 @@ class _GpuState:
                                                _lib.ptr(out), self.stream()), "lmap_transpose")
         return out
 
-    def solve_perm(self, b, k, sparse=True, shared=True, f32=False):
-        """Solve in elimination order; f32: FP32 solution block (mixed plans)."""
+    def solve_perm(self, b, k, sparse=True, shared=True, f32=False, defer=False):
+        """Solve in elimination order; f32: FP32 solution block (mixed plans);
+        defer: the check's verdict is left to solver.check_wait()."""
         sv = self.solver
         x = torch.empty(sv.nbatch, sv.n, k, dtype=torch.float32 if f32 else torch.float64,
                         device=b.device)
         flags = _lib.DCX_PERMUTED | (_lib.DCX_SHARED_RHS if shared else 0) | \
             (_lib.DCX_SPARSE_RHS if sparse else 0) | (_lib.DCX_F32_FIELDS if f32 else 0)
-        _lib.check(self.lib.dcx_solve(sv._h, _lib.ptr(b), _lib.ptr(x), int(k), flags,
-                                      self.stream()), "solve")
+        with sv._lock:
+            sv.run_solve(b, x, k, flags, defer=defer)
         return x
 
-    def factor_solve_perm(self, vals, b, k, f32=False):
+    def factor_solve_perm(self, vals, b, k, f32=False, defer=False):
         """Factor every wavenumber and solve the shared, sparse right-hand
         sides b in elimination order, the two overlapped (dcx_factor_solve)."""
         sv = self.solver
@@ -153,10 +154,7 @@ class _GpuState:
             (_lib.DCX_F32_FIELDS if f32 else 0)
         v = vals.contiguous()
         with sv._lock:
-            _lib.check(self.lib.dcx_factor_solve(sv._h, _lib.ptr(v), _lib.ptr(b), _lib.ptr(x),
-                                                 int(k), flags, self.stream()), "factor_solve")
-            sv._vals_keep = v
-        sv.stamp += 1
+            sv.run_factor_solve(v, b, x, k, flags, defer=defer)
         return x
 
     def dpred(self, U, k, s0, s1, d):
@@ -210,8 +208,10 @@ def _sources_backward(eng, saved, stamp, gd):
         g.solver.factorize(vals)
     v = gd.to(torch.float64).contiguous()
     G = g.adjoint_rhs(v, eng.nS, 0, eng.nS)
-    Lam = g.solve_perm(G, eng.nS, f32=g.fields32)
+    Lam = g.solve_perm(G, eng.nS, f32=g.fields32, defer=True)
     gv = g.solver.grad_values(Lam, U, scale=g.wq, permuted=True)
+    if g.solver.check_wait():        # systems of Lam were solved again in FP64
+        gv = g.solver.grad_values(Lam, U, scale=g.wq, permuted=True)
     return g.lmap_t(gv, sig)
 
 
@@ -259,23 +259,110 @@ def _electrodes_backward(eng, saved, stamp, gd):
     return g.lmap_t(gv, sig)
 
 
+def _misfit_body(eng, m_full, dobs, w, want_grad):
+    """phi = ||w (dpred - dobs)||^2 and, with want_grad, dphi/dm_full, in one
+    pass on the device: the forward solve, the projection, the misfit, the
+    adjoint solve and the gathers, with the solves' checks deferred and
+    waited for where their results are used (the forward's before the adjoint
+    solve, the adjoint's at the end), so the host never waits for the device
+    mid-pass; a system refactorised in FP64 by a failing check has everything
+    computed from its fields computed again.  Returns (phi, d, grad_full, r)
+    with r = w (d - dobs)."""
+    g = eng._gpu
+    sv = g.solver
+    sig = torch.exp(-m_full)
+    vals = g.assemble(sig)
+    if eng.rhs_basis == "electrodes":
+        G = g.factor_solve_perm(vals, g.E_perm, g.ne, f32=g.fields32, defer=True)
+
+        def rest():
+            Ke = torch.matmul(g.E_supT, G[:, g.e_sup, :].to(torch.float64))  # (nky, ne, ne)
+            K = (g.wq.view(-1, 1, 1) * Ke).sum(0)
+            d = (g.R * (K @ g.S)[:, g.sidx]).sum(0)
+            r = w * (d - dobs)
+            if not want_grad:
+                return d, r, None
+            T = torch.zeros(g.ne, eng.nS, dtype=torch.float64, device=eng.device)
+            T.index_add_(1, g.sidx, g.R * ((2.0 * r) * w))
+            H = _rows_times(g, G, (T @ g.S.T).T)
+            return d, r, sv.grad_values(G, H, scale=g.wq, permuted=True)
+        d, r, gv = rest()
+        if sv.check_wait():
+            d, r, gv = rest()
+    else:
+        U = g.factor_solve_perm(vals, g.q_perm, eng.nS, f32=g.fields32, defer=True)
+        d = torch.empty(eng.nD, dtype=torch.float64, device=eng.device)
+
+        def project():
+            g.dpred(U, eng.nS, 0, eng.nS, d)
+            r = w * (d - dobs)
+            G = g.adjoint_rhs(((2.0 * r) * w).contiguous(), eng.nS, 0, eng.nS) \
+                if want_grad else None
+            return r, G
+        r, G = project()
+        if sv.check_wait():
+            r, G = project()
+        gv = None
+        if want_grad:
+            Lam = g.solve_perm(G, eng.nS, f32=g.fields32, defer=True)
+            gv = sv.grad_values(Lam, U, scale=g.wq, permuted=True)
+            if sv.check_wait():
+                gv = sv.grad_values(Lam, U, scale=g.wq, permuted=True)
+    phi = (r ** 2).sum()
+    return phi, d, (g.lmap_t(gv, sig) if want_grad else None), r
+
+
+class _Misfit(torch.autograd.Function):
+    """(m_full, dobs, w) -> phi on the GPU, the gradient computed with the
+    value (one fused eval+grad, `_misfit_body`); backward scales it."""
+
+    @staticmethod
+    def forward(ctx, m_full, dobs, w, eng):
+        with torch.cuda.device(eng.device):
+            phi, d, grad_full, r = _misfit_body(eng, m_full.detach(), dobs.detach(),
+                                                w.detach(), True)
+        ctx.save_for_backward(grad_full, r, d, dobs, w)
+        return phi
+
+    @staticmethod
+    def backward(ctx, gphi):
+        grad_full, r, d, dobs, w = ctx.saved_tensors
+        gm = grad_full * gphi if ctx.needs_input_grad[0] else None
+        gdobs = (-2.0 * w * r) * gphi if ctx.needs_input_grad[1] else None
+        gw = (2.0 * r * (d - dobs)) * gphi if ctx.needs_input_grad[2] else None
+        return gm, gdobs, gw, None
+
+
+def _gpu_misfit(eng, m, dobs, w):
+    """Body of TorchDC2D/TorchDC3D.misfit on a GPU engine: the fused pass,
+    through autograd when a gradient is wanted."""
+    out_dt = m.dtype
+    for t in (dobs, w):
+        if torch.is_tensor(t):
+            out_dt = torch.promote_types(out_dt, t.dtype)
+    m_full = eng._model_to_full(m).to(torch.float64)
+    dobs_t = torch.as_tensor(dobs).to(device=eng.device, dtype=torch.float64).reshape(-1)
+    w_t = torch.as_tensor(w).to(device=eng.device, dtype=torch.float64).reshape(-1)
+    if torch.is_grad_enabled() and (m_full.requires_grad or dobs_t.requires_grad or
+                                    w_t.requires_grad):
+        phi = _Misfit.apply(m_full, dobs_t, w_t, eng)
+    else:
+        with torch.no_grad(), torch.cuda.device(eng.device):
+            phi = _misfit_body(eng, m_full, dobs_t, w_t, False)[0]
+    return phi.to(device=m.device, dtype=out_dt)
+
+
 def _eval_grad_device(eng, m, dobs, w):
-    """Body of TorchDC2D/TorchDC3D.eval_grad_device: forward, the misfit's
-    derivative in autograd's order (d phi/d d = (2 r) w with r = w (d - dobs)),
-    backward."""
+    """Body of TorchDC2D/TorchDC3D.eval_grad_device: the fused pass of
+    `misfit` (`_misfit_body`) without autograd."""
     if eng._gpu is None:
         raise RuntimeError("eval_grad_device needs a GPU engine")
-    fwd, bwd = ((_electrodes_forward, _electrodes_backward)
-                if eng.rhs_basis == "electrodes" else (_sources_forward, _sources_backward))
     with torch.no_grad(), torch.cuda.device(eng.device):
         m_full = eng._model_to_full(torch.as_tensor(m).detach()).to(torch.float64)
         dobs = torch.as_tensor(dobs, dtype=torch.float64, device=eng.device)
         w = torch.as_tensor(w, dtype=torch.float64, device=eng.device)
-        d, saved = fwd(eng, m_full)
-        stamp = eng._gpu.solver.stamp
-        r = w * (d - dobs)
-        phi = (r ** 2).sum()
-        grad = eng._model_from_full(bwd(eng, saved, stamp, (2.0 * r) * w))
+        phi, _, grad_full, _ = _misfit_body(eng, m_full, dobs, w, True)
+        grad = eng._model_from_full(grad_full)
     return phi, grad
 
 
@@ -504,17 +591,22 @@ class TorchDC2D:
         return torch.sparse.mm(self.Pd, Ubar)[gi, li]
 
     def misfit(self, m, dobs, w):
-        """phi_d = ||w (dpred - dobs)||^2 (no 1/2 factor)."""
+        """phi_d = ||w (dpred - dobs)||^2 (no 1/2 factor).  On a GPU engine the
+        value and, when a gradient is wanted, dphi/dm (and dphi/ddobs,
+        dphi/dw) come from one fused pass; backward() then only scales it."""
+        if self._gpu is not None:
+            return _gpu_misfit(self, m, dobs, w)
         r = w * (self.dpred(m) - dobs)
         return (r ** 2).sum()
 
     def eval_grad_device(self, m, dobs, w):
-        """(phi_d, dphi_d/dm) as device tensors, with no autograd graph and no
-        host synchronisation: the same kernels, in the same order, as
-        `misfit(m, dobs, w).backward()` (bit-identical results), issued on the
-        calling thread's current stream.  Meant for callers that drive many
-        engines from several host threads (torch's autograd engine runs every
-        CUDA backward of a device on one worker thread).  GPU engines only."""
+        """(phi_d, dphi_d/dm) as device tensors, with no autograd graph: the
+        same fused pass, kernels and order as `misfit(m, dobs, w).backward()`
+        (bit-identical results), issued on the calling thread's current
+        stream; the host waits only for the solves' checks.  Meant for callers
+        that drive many engines from several host threads (torch's autograd
+        engine runs every CUDA backward of a device on one worker thread).
+        GPU engines only."""
         return _eval_grad_device(self, m, dobs, w)
 
     # -- chunked reverse passes ---------------------------------------------------

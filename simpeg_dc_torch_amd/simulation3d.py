@@ -22,7 +22,8 @@ import scipy.sparse as sp
 import torch
 
 from .operators3d import Problem3DMaps
-from .simulation2d import _Forward2D, _ForwardElectrodes2D, _GpuState, _eval_grad_device
+from .simulation2d import (_Forward2D, _ForwardElectrodes2D, _GpuState, _eval_grad_device,
+                           _gpu_misfit)
 from .solver import make_solver, sparse_solve
 
 
@@ -160,14 +161,17 @@ class TorchDC3D:
         return torch.autograd.grad(d, mc, grad_outputs=v_t)[0]
 
     def misfit(self, m, dobs, w):
-        """phi_d = ||w (dpred - dobs)||^2 (no 1/2 factor)."""
+        """phi_d = ||w (dpred - dobs)||^2 (no 1/2 factor); on a GPU engine one
+        fused eval+grad pass (see TorchDC2D.misfit)."""
+        if self._gpu is not None:
+            return _gpu_misfit(self, m, dobs, w)
         r = w * (self.dpred(m) - dobs)
         return (r ** 2).sum()
 
     def eval_grad_device(self, m, dobs, w):
-        """(phi_d, dphi_d/dm) as device tensors without an autograd graph or a
-        host synchronisation, bit-identical to `misfit(...).backward()`, on
-        the calling thread's current stream.  GPU engines only."""
+        """(phi_d, dphi_d/dm) as device tensors without an autograd graph,
+        bit-identical to `misfit(...).backward()`, on the calling thread's
+        current stream.  GPU engines only."""
         return _eval_grad_device(self, m, dobs, w)
 
     def misfit_and_grad(self, m, dobs, w):

@@ -22,6 +22,7 @@ computed by one gather-dot kernel on the GPU.
 import ctypes as C
 import os
 import threading
+import warnings
 
 import numpy as np
 import torch
@@ -80,10 +81,12 @@ class HipBatchSolver:
             dim = co.shape[1]
             self._coords = co
         fp = _lib.DCX_FP64 if precision == "fp64" else _lib.DCX_FP32
+        self._coords_arg = coords
+        self.leaf = int(leaf or DEFAULT_LEAF)
         h = C.c_void_p()
         with torch.cuda.device(self.device):
             _lib.check(lib.dcx_plan_create(self.n, crow_p, col_p, self.nbatch, co_p, dim,
-                                           int(leaf or DEFAULT_LEAF), fp, C.byref(h)),
+                                           self.leaf, fp, C.byref(h)),
                        "plan_create")
         self._h = h
         self._lib = lib
@@ -92,6 +95,11 @@ class HipBatchSolver:
         self.stamp = 0
         self._lock = threading.Lock()
         self._support = None
+        self._twin = None          # FP64 plan of the same pattern (fallback)
+        self._twin_stamp = -1
+        self.fallbacks = []        # (stamp, systems) of every FP64 fallback
+        self._pending = None       # the solve whose check is deferred
+        self.set_check()
 
     # -- info ----------------------------------------------------------------
     def info(self):
@@ -151,6 +159,144 @@ class HipBatchSolver:
         _lib.check(self._lib.dcx_set_rhs_support(self._h, rp, int(r.size)), "set_rhs_support")
         self._support = r
 
+    # -- solve check and FP64 fallback ----------------------------------------
+    def set_check(self, tol=None, max_passes=2, sample_cols=8):
+        """Backward-error check of every solve (dcx_plan_set_check): tol None
+        takes the default (1e-5 for FP32 factors, 1e-10 for FP64, or
+        DCX_CHECK_TOL), tol <= 0 turns the check off.  A mixed-precision
+        solve failing it gets up to `max_passes` FP64-residual refinement
+        passes, then its failing systems are refactorised and solved in FP64
+        (see `_fp64_fallback`); an FP64 solve failing it raises
+        `NumericError`."""
+        if tol is None:
+            env = os.environ.get("DCX_CHECK_TOL")
+            tol = float(env) if env else (1e-10 if self.precision == "fp64" else 1e-5)
+        _lib.check(self._lib.dcx_plan_set_check(self._h, float(tol), int(max_passes),
+                                                int(sample_cols)), "plan_set_check")
+        self.check_tol = float(tol)
+
+    def check_report(self):
+        """The last solve's check: eta (backward error per system), pivot_fail,
+        passes (refinement passes run), status."""
+        eta = np.zeros(self.nbatch)
+        piv = np.zeros(self.nbatch, np.int32)
+        passes, status = C.c_int(), C.c_int()
+        _lib.check(self._lib.dcx_plan_get_check(
+            self._h, eta.ctypes.data_as(_lib._f64p), piv.ctypes.data_as(_lib._i32p),
+            C.byref(passes), C.byref(status)), "plan_get_check")
+        return {"eta": eta, "pivot_fail": piv.astype(bool), "passes": passes.value,
+                "status": {0: "none", 1: "passed", 2: "failed", 3: "skipped"}[status.value]}
+
+    def inject_fault(self, batch, rel):
+        """Tests: later factorisations of system `batch` factor (1 + rel) A
+        (dcx_plan_inject_fault); rel = 0 disarms."""
+        _lib.check(self._lib.dcx_plan_inject_fault(self._h, int(batch), float(rel)),
+                   "plan_inject_fault")
+
+    def _fp64_twin(self):
+        if self._twin is None:
+            t = HipBatchSolver(self.crow.cpu(), self.col.cpu(), self.n, self.n_rhs, self.nbatch,
+                               device=self.device, precision="fp64",
+                               coords=self._coords_arg, leaf=self.leaf)
+            if not np.array_equal(t.perm(), self.perm()):
+                raise RuntimeError("FP64 fallback plan has another elimination order")
+            if self._support is not None:
+                t.set_rhs_support(self._support)
+            self._twin = t
+        return self._twin
+
+    def _fp64_fallback(self, vals, b, x, k, flags, exc):
+        """The systems whose FP32 solve failed its check (or met a non-positive
+        pivot) are factorised and solved again in FP64 on a second plan of
+        the same pattern and order; their rows of x are replaced.  Returns
+        those systems; raises NumericError when the FP64 factorisation fails
+        too (the matrix is not positive definite)."""
+        rep = self.check_report()
+        bad = np.nonzero(rep["pivot_fail"] | ~(rep["eta"] <= self.check_tol))[0]
+        if bad.size == 0:
+            raise exc
+        twin = self._fp64_twin()
+        st = self._stream()
+        flags = int(flags) & ~(_lib.DCX_F32_FIELDS | _lib.DCX_CHECK_DEFER)
+        try:
+            if self._twin_stamp != self.stamp:
+                _lib.check(self._lib.dcx_factorize(twin._h, _lib.ptr(vals), st), "factorize")
+                self._twin_stamp = self.stamp
+            x64 = torch.empty(self.nbatch, self.n, k, dtype=torch.float64, device=self.device)
+            _lib.check(self._lib.dcx_solve(twin._h, _lib.ptr(b), _lib.ptr(x64), int(k), flags,
+                                           st), "solve")
+        except _lib.NumericError as e2:
+            raise _lib.NumericError(
+                f"systems {bad.tolist()} failed the FP32 solve check ({exc}) and their FP64 "
+                f"factorisation failed too: not positive definite ({e2})") from e2
+        idx = torch.as_tensor(bad, device=self.device)
+        x[idx] = x64[idx].to(x.dtype)
+        self.fallbacks.append((self.stamp, bad.tolist()))
+        warnings.warn(f"mixed-precision solve: systems {bad.tolist()} failed the backward-error "
+                      f"check (eta {rep['eta'][bad].tolist()}) after {rep['passes']} "
+                      "refinement pass(es); refactorised and solved in FP64", RuntimeWarning,
+                      stacklevel=3)
+        return bad.tolist()
+
+    def check_wait(self):
+        """Verdict of the deferred check of the last solve run with
+        ``defer=True`` (dcx_plan_check_wait): [] when it passed (after any
+        refinement passes), else the systems that were refactorised and
+        solved in FP64, whose rows of that solve's x were replaced (results
+        computed from that x before must be computed again).  Raises
+        NumericError for an FP64 plan or a system that is not SPD."""
+        p = self._pending
+        if p is None:
+            return []
+        self._pending = None
+        try:
+            with torch.cuda.device(self.device):
+                _lib.check(self._lib.dcx_plan_check_wait(self._h), "check_wait")
+        except _lib.NumericError as exc:
+            if self.precision == "fp64":
+                raise
+            with torch.cuda.device(self.device):
+                return self._fp64_fallback(*p, exc)
+        return []
+
+    def run_solve(self, b, x, k, flags, defer=False):
+        """dcx_solve on tensors (b FP64, x FP64 or FP32 with DCX_F32_FIELDS),
+        with the FP64 fallback of failing systems; with defer, the check's
+        verdict is left to `check_wait` (b and x must stay as they are until
+        then).  The caller holds the lock and the device."""
+        self.check_wait()
+        fl = int(flags) | (_lib.DCX_CHECK_DEFER if defer else 0)
+        try:
+            _lib.check(self._lib.dcx_solve(self._h, _lib.ptr(b), _lib.ptr(x), int(k), fl,
+                                           self._stream()), "solve")
+        except _lib.NumericError as exc:
+            if self.precision == "fp64":
+                raise
+            self._fp64_fallback(self._vals_keep, b, x, k, flags, exc)
+            return
+        if defer:
+            self._pending = (self._vals_keep, b, x, k, flags)
+
+    def run_factor_solve(self, vals, b, x, k, flags, defer=False):
+        """dcx_factor_solve on tensors (vals FP64 (nbatch, nnz)), with the FP64
+        fallback of failing systems (defer: as `run_solve`); bumps the stamp.
+        The caller holds the lock and the device."""
+        self.check_wait()
+        self._vals_keep = vals
+        self.stamp += 1
+        fl = int(flags) | (_lib.DCX_CHECK_DEFER if defer else 0)
+        try:
+            _lib.check(self._lib.dcx_factor_solve(self._h, _lib.ptr(vals), _lib.ptr(b),
+                                                  _lib.ptr(x), int(k), fl, self._stream()),
+                       "factor_solve")
+        except _lib.NumericError as exc:
+            if self.precision == "fp64":
+                raise
+            self._fp64_fallback(vals, b, x, k, flags, exc)
+            return
+        if defer:
+            self._pending = (vals, b, x, k, flags)
+
     # -- numeric -------------------------------------------------------------
     def _stream(self):
         return _lib.stream_of(self.device)
@@ -161,6 +307,7 @@ class HipBatchSolver:
         if v.shape != (self.nbatch, self.col.numel()):
             v = v.reshape(self.nbatch, self.col.numel())
         with self._lock, torch.cuda.device(self.device):
+            self.check_wait()
             _lib.check(self._lib.dcx_factorize(self._h, _lib.ptr(v), self._stream()),
                        "factorize")
             self._vals_keep = v
@@ -178,8 +325,7 @@ class HipBatchSolver:
             bb = bb.reshape(self.nbatch, self.n, k)
         x = torch.empty(self.nbatch, self.n, k, dtype=torch.float64, device=self.device)
         with self._lock, torch.cuda.device(self.device):
-            _lib.check(self._lib.dcx_solve(self._h, _lib.ptr(bb), _lib.ptr(x), int(k),
-                                           int(flags), self._stream()), "solve")
+            self.run_solve(bb, x, k, flags)
         return x.to(self.dtype)
 
     def factor_solve(self, values, b, flags=0):
@@ -198,11 +344,7 @@ class HipBatchSolver:
             bb = bb.reshape(self.nbatch, self.n, k)
         x = torch.empty(self.nbatch, self.n, k, dtype=torch.float64, device=self.device)
         with self._lock, torch.cuda.device(self.device):
-            _lib.check(self._lib.dcx_factor_solve(self._h, _lib.ptr(v), _lib.ptr(bb),
-                                                  _lib.ptr(x), int(k), int(flags),
-                                                  self._stream()), "factor_solve")
-            self._vals_keep = v
-        self.stamp += 1
+            self.run_factor_solve(v, bb, x, k, flags)
         return x.to(self.dtype)
 
     def grad_values(self, adj, x, scale=None, permuted=False):
@@ -223,6 +365,9 @@ class HipBatchSolver:
         return gv
 
     def free(self):
+        if getattr(self, "_twin", None) is not None:
+            self._twin.free()
+            self._twin = None
         if getattr(self, "_h", None):
             self._lib.dcx_plan_destroy(self._h)
             self._h = None
