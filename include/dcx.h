@@ -135,19 +135,23 @@ int dcx_plan_set_fusion(dcx_plan* plan, int fuse_rows);
 /* number of FP64 refinement passes after the first FP32 solve (mixed mode) */
 int dcx_plan_set_refinement(dcx_plan* plan, int passes);
 
-/* Backward-error check of every solve (on by default; tol <= 0 turns it off;
+/* Residual check of every solve (on by default; tol <= 0 turns it off;
  * skipped while the stream is being captured).  After each dcx_solve /
- * dcx_factor_solve, eta_b = |B - A_b X| / (|A_b| |X| + |B|) (Frobenius norms,
- * FP64 residual, over the first sample_cols <= 64 columns) and the pivot flags
- * of the factorisation are read back (one stream synchronisation).  An FP32
- * plan whose eta_b exceeds tol for some b adds FP64-residual refinement
- * passes, up to max_passes; then, or at once for a non-positive pivot or an
- * FP64 plan, the call returns DCX_ERR_NUMERIC with the solution of the last
- * pass in x, and dcx_plan_get_check names the failing systems (the caller
- * refactorises those in FP64).  Defaults: tol 1e-5 (FP32) / 1e-10 (FP64), or
- * DCX_CHECK_TOL at plan creation; max_passes 2; sample_cols 8.  Counterpart
- * of the reference's guarded solve, a residual check followed by a fallback
- * (dctorch/optimization.py:753-761); cuDSS reports pivot failures itself. */
+ * dcx_factor_solve the FP64 residual R = B - A_b X of the first sample_cols
+ * <= 64 columns is formed and rho_b = |R| / |B| (Frobenius norms over the
+ * sample) and the pivot flags of the factorisation are read back (one stream
+ * synchronisation, or none with DCX_CHECK_DEFER).  An FP32 plan whose rho_b
+ * exceeds tol for some b adds FP64-residual refinement passes, up to
+ * max_passes; then, or at once for a non-positive pivot or an FP64 plan, the
+ * call returns DCX_ERR_NUMERIC with the solution of the last pass in x, and
+ * dcx_plan_get_check names the failing systems (the caller refactorises
+ * those in FP64).  The residual bounds the backward error, not the forward
+ * error: the check catches a factorisation that broke down or lost accuracy
+ * well beyond rounding, not a 1e-6 change in the data.  Defaults: tol 1e-3
+ * (FP32) / 1e-8 (FP64), or DCX_CHECK_TOL at plan creation; max_passes 2;
+ * sample_cols 8.  Counterpart of the reference's guarded solve, a residual
+ * check followed by a fallback (dctorch/optimization.py:753-761); cuDSS
+ * reports pivot failures itself. */
 int dcx_plan_set_check(dcx_plan* plan, double tol, int max_passes, int sample_cols);
 /* Verdict of a deferred check (DCX_CHECK_DEFER): waits for it, runs the
  * refinement passes a failing FP32 solve gets, and returns DCX_OK or
@@ -155,11 +159,13 @@ int dcx_plan_set_check(dcx_plan* plan, double tol, int max_passes, int sample_co
  * pending.  The next solve or factorisation of the plan completes a pending
  * check first.  No reference counterpart (see dcx_plan_set_check). */
 int dcx_plan_check_wait(dcx_plan* plan);
-/* Result of the last check: eta and pivot_fail (host, nbatch each, may be
- * NULL), refinement passes run, status (0 none yet, 1 passed, 2 failed,
- * 3 skipped under stream capture or with the check off). */
-int dcx_plan_get_check(const dcx_plan* plan, double* eta, int* pivot_fail, int* passes,
-                       int* status);
+/* Result of the last check (host arrays of nbatch, each may be NULL): rho
+ * (the tested relative residual), eta (the normwise backward error
+ * |R| / (|A| |X| + |B|), for information), pivot_fail; refinement passes run;
+ * status (0 none yet, 1 passed, 2 failed, 3 skipped under stream capture or
+ * with the check off). */
+int dcx_plan_get_check(const dcx_plan* plan, double* rho, double* eta, int* pivot_fail,
+                       int* passes, int* status);
 /* Fault injection for tests: later factorisations of batch member `batch`
  * factor (1 + rel) A_b instead of A_b (the solve's FP64 residual keeps A_b),
  * a factor with backward error about rel / 2; rel = 0 disarms.  No reference

@@ -229,22 +229,29 @@ class Problem2D:
         nS = len(survey.source_list)
         q = np.zeros((g.nN, nS))
         rows, sidx = [], []
+        self._src_at, self._rx_at = [], []   # [(sign, (x, z))] per source / datum
         for s, src in enumerate(survey.source_list):
             for nd, w in g.interp_row(*src.location_a[:2]).items():
                 q[nd, s] += w
+            at = [(1.0, tuple(float(v) for v in src.location_a[:2]))]
             if getattr(src, "location_b", None) is not None:
                 for nd, w in g.interp_row(*src.location_b[:2]).items():
                     q[nd, s] -= w
+                at.append((-1.0, tuple(float(v) for v in src.location_b[:2])))
+            self._src_at.append(at)
             for rx in src.receiver_list:
                 M = np.atleast_2d(rx.locations_m)
                 N = None if getattr(rx, "locations_n", None) is None \
                     else np.atleast_2d(rx.locations_n)
                 for i in range(M.shape[0]):
                     row = dict(g.interp_row(*M[i, :2]))
+                    at = [(1.0, tuple(float(v) for v in M[i, :2]))]
                     if N is not None:
                         for nd, w in g.interp_row(*N[i, :2]).items():
                             row[nd] = row.get(nd, 0.0) - w
+                        at.append((-1.0, tuple(float(v) for v in N[i, :2])))
                     rows.append(row)
+                    self._rx_at.append(at)
                     sidx.append(s)
         self.q = q
         self.nS = nS
@@ -348,6 +355,82 @@ class Problem2D:
                     + self.AvgBC_[k].T @ (lam * u))
             J[i] = sigma * acc
         return J
+
+
+# --------------------------------------------------------------------------
+# the same quantities through the electrode positions (exact by linearity)
+# --------------------------------------------------------------------------
+def _electrode_basis(P):
+    """(E, S, R): E (nN x ne) the interpolation columns of the ne distinct
+    electrode positions, S (ne x nS) and R (ne x nD) their signed sums, so
+    that q = E S and Pd = (E R)^T exactly (the same interpolation rows)."""
+    pos = {}
+    for at in P._src_at + P._rx_at:
+        for _, loc in at:
+            pos.setdefault(loc, len(pos))
+    E = np.zeros((P.g.nN, len(pos)))
+    for loc, j in pos.items():
+        for nd, w in P.g.interp_row(*loc).items():
+            E[nd, j] += w
+
+    def combo(ats):
+        M = np.zeros((len(pos), len(ats)))
+        for c, at in enumerate(ats):
+            for sign, loc in at:
+                M[pos[loc], c] += sign
+        return M
+    return E, combo(P._src_at), combo(P._rx_at)
+
+
+def misfit_and_grad_electrodes(P, m, dobs, w):
+    """Problem2D.misfit_and_grad through the electrode positions: one SuperLU
+    solve per wavenumber with the ne electrode columns, G_k = A_k^{-1} E,
+    instead of the nS source and nS adjoint columns.  Exact by linearity:
+    U_k = G_k S, the adjoint fields Lam_k = G_k T with T[:, s] the receiver
+    combinations R weighted by the data residual of source s, and
+    sum_s f(Lam_k[:, s]) g(U_k[:, s]) = rowsum((f(G_k) M) o g(G_k)) with
+    M = T S^T.  Pinned against misfit_and_grad (tests/test_oracle.py); used
+    where the source-column restatement would take minutes (the 25,088-cell
+    line: 849 sources, 61 electrodes).  Returns (phi, grad_m, dpred)."""
+    E, S, R = _electrode_basis(P)
+    sigma = np.exp(-m)
+    G = [spla.splu(P.A(sigma, k).tocsc()).solve(E) for k in range(P.nky)]
+    K = sum(P.wq[k] * (E.T @ G[k]) for k in range(P.nky))          # ne x ne
+    d = np.einsum("ei,ei->i", R, (K @ S)[:, P.sidx])
+    r = d - dobs
+    phi = float(((w * r) ** 2).sum())
+    v = 2.0 * w ** 2 * r
+    T = np.zeros((E.shape[1], P.nS))
+    np.add.at(T.T, P.sidx, (R * v).T)
+    M = T @ S.T
+    out = np.zeros(P.g.nC)
+    for k in range(P.nky):
+        gG = P.Grad @ G[k]
+        edge = ((gG @ M) * gG).sum(1)
+        nodal = ((G[k] @ M) * G[k]).sum(1)
+        out += P.wq[k] * (P.Qe_.T @ edge + P.kys[k] ** 2 * (P.Qn_.T @ nodal)
+                          + P.AvgBC_[k].T @ nodal)
+    return phi, sigma * out, d
+
+
+def jacobian_rows_electrodes(P, m, rows):
+    """Rows `rows` of J = d dpred / d m through the electrode positions (as
+    misfit_and_grad_electrodes): J[i] = sigma * sum_k w_k [Qe^T((Grad G R_i)
+    o (Grad G S_s)) + ky^2 Qn^T((G R_i) o (G S_s)) + AvgBC_k^T((G R_i) o
+    (G S_s))], s the source of datum i.  Pinned against Jmatrix
+    (tests/test_oracle.py)."""
+    E, S, R = _electrode_basis(P)
+    sigma = np.exp(-m)
+    rows = np.asarray(rows)
+    out = np.zeros((rows.size, P.g.nC))
+    for k in range(P.nky):
+        G = spla.splu(P.A(sigma, k).tocsc()).solve(E)
+        gG = P.Grad @ G
+        lam, u = G @ R[:, rows], G @ S[:, P.sidx[rows]]
+        glam, gu = gG @ R[:, rows], gG @ S[:, P.sidx[rows]]
+        out += P.wq[k] * ((P.Qe_.T @ (glam * gu)).T + P.kys[k] ** 2 * (P.Qn_.T @ (lam * u)).T
+                          + (P.AvgBC_[k].T @ (lam * u)).T)
+    return out * sigma[None, :]
 
 
 def halfspace_dipole(rho, a, b, m, n):

@@ -63,7 +63,7 @@ PROTOTYPES = {
     "dcx_plan_set_refinement": (C.c_int, [_vp, C.c_int]),
     "dcx_plan_set_fusion": (C.c_int, [_vp, C.c_int]),
     "dcx_plan_set_check": (C.c_int, [_vp, C.c_double, C.c_int, C.c_int]),
-    "dcx_plan_get_check": (C.c_int, [_vp, _f64p, _i32p, C.POINTER(C.c_int),
+    "dcx_plan_get_check": (C.c_int, [_vp, _f64p, _f64p, _i32p, C.POINTER(C.c_int),
                                      C.POINTER(C.c_int)]),
     "dcx_plan_inject_fault": (C.c_int, [_vp, C.c_int, C.c_double]),
     "dcx_plan_check_wait": (C.c_int, [_vp]),

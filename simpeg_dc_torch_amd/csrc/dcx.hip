@@ -2233,7 +2233,8 @@ struct dcx_plan {
   int chk_cols = 8;         // sample columns (<= 64)
   double* d_chk = nullptr;  // nbatch x 4 sums of k_check
   void* h_chk = nullptr;    // pinned host: nbatch x 4 sums, then nbatch info ints
-  std::vector<double> chk_eta;
+  std::vector<double> chk_eta;   // normwise backward error |R| / (|A| |X| + |B|)
+  std::vector<double> chk_rho;   // relative residual |R| / |B| (the test)
   std::vector<int> chk_pivot;
   int chk_passes_done = 0;
   int chk_status = 0;       // 0 not run, 1 passed, 2 failed, 3 skipped (stream capture)
@@ -2398,12 +2399,12 @@ int upload_fused(dcx_plan* P) {
   return DCX_OK;
 }
 
-// default backward-error tolerances of the solve check (DCX_CHECK_TOL
-// overrides): FP32 factors reach about 1e-7 on the benchmark problems, FP64
-// ones about 1e-16; a failing check means a factor that lost accuracy or
-// definiteness, not an ordinary rounding error
-constexpr double kCheckTol32 = 1e-5;
-constexpr double kCheckTol64 = 1e-10;
+// default tolerances of the solve check on the relative residual |R| / |B|
+// (DCX_CHECK_TOL overrides): FP32 factors stay below about 1e-5 on the
+// benchmark problems, FP64 ones below 1e-12; a failing check means a factor
+// that broke down or lost accuracy far beyond rounding
+constexpr double kCheckTol32 = 1e-3;
+constexpr double kCheckTol64 = 1e-8;
 
 constexpr size_t kLdsBudget = 64 * 1024;
 constexpr size_t kFusedLdsBudget = 80 * 1024;  // two workgroups per CU
@@ -3248,11 +3249,12 @@ int dcx_plan_set_check(dcx_plan* P, double tol, int max_passes, int sample_cols)
   return DCX_OK;
 }
 
-int dcx_plan_get_check(const dcx_plan* P, double* eta, int* pivot_fail, int* passes,
-                       int* status) {
+int dcx_plan_get_check(const dcx_plan* P, double* rho, double* eta, int* pivot_fail,
+                       int* passes, int* status) {
   if (!P) return fail(DCX_ERR_ARG, "null plan");
   for (int b = 0; b < P->nbatch; ++b) {
     const bool run = static_cast<int>(P->chk_eta.size()) == P->nbatch;
+    if (rho) rho[b] = run ? P->chk_rho[b] : 0.0;
     if (eta) eta[b] = run ? P->chk_eta[b] : 0.0;
     if (pivot_fail) pivot_fail[b] = run ? P->chk_pivot[b] : 0;
   }
@@ -3523,16 +3525,20 @@ static int eval_check(dcx_plan* P) {
   const double* hs = static_cast<const double*>(P->h_chk);
   const int* hi = reinterpret_cast<const int*>(hs + 4 * nb);
   P->chk_eta.assign(nb, 0.0);
+  P->chk_rho.assign(nb, 0.0);
   P->chk_pivot.assign(nb, 0);
   int bad = 0;
+  auto ratio = [](double num, double den) {
+    const double q = den > 0.0 ? num / den : (num > 0.0 ? INFINITY : 0.0);
+    return std::isfinite(q) ? q : INFINITY;
+  };
   for (int b = 0; b < nb; ++b) {
     const double r = std::sqrt(hs[4 * b]), x = std::sqrt(hs[4 * b + 1]),
                  bn = std::sqrt(hs[4 * b + 2]), a = std::sqrt(hs[4 * b + 3]);
-    const double den = a * x + bn;
-    const double eta = den > 0.0 ? r / den : (r > 0.0 ? INFINITY : 0.0);
-    P->chk_eta[b] = std::isfinite(eta) ? eta : INFINITY;
+    P->chk_rho[b] = ratio(r, bn);
+    P->chk_eta[b] = ratio(r, a * x + bn);
     P->chk_pivot[b] = hi[b] != 0;
-    if (P->chk_pivot[b] || !(P->chk_eta[b] <= P->chk_tol)) ++bad;
+    if (P->chk_pivot[b] || !(P->chk_rho[b] <= P->chk_tol)) ++bad;
   }
   return bad;
 }
@@ -3549,10 +3555,10 @@ static int run_check(dcx_plan* P, const double* B, int64_t b_bs, const void* X, 
 static std::string check_message(const dcx_plan* P) {
   std::string m;
   for (int b = 0; b < P->nbatch; ++b) {
-    if (!P->chk_pivot[b] && P->chk_eta[b] <= P->chk_tol) continue;
-    char buf[96];
-    std::snprintf(buf, sizeof(buf), "%s system %d: %s backward error %.2e", m.empty() ? "" : ";",
-                  b, P->chk_pivot[b] ? "non-positive pivot," : "", P->chk_eta[b]);
+    if (!P->chk_pivot[b] && P->chk_rho[b] <= P->chk_tol) continue;
+    char buf[112];
+    std::snprintf(buf, sizeof(buf), "%s system %d: %s relative residual %.2e", m.empty() ? "" : ";",
+                  b, P->chk_pivot[b] ? "non-positive pivot," : "", P->chk_rho[b]);
     m += buf;
   }
   return m;

@@ -161,30 +161,33 @@ class HipBatchSolver:
 
     # -- solve check and FP64 fallback ----------------------------------------
     def set_check(self, tol=None, max_passes=2, sample_cols=8):
-        """Backward-error check of every solve (dcx_plan_set_check): tol None
-        takes the default (1e-5 for FP32 factors, 1e-10 for FP64, or
-        DCX_CHECK_TOL), tol <= 0 turns the check off.  A mixed-precision
+        """Residual check of every solve (dcx_plan_set_check): tol None takes
+        the default (relative residual 1e-3 for FP32 factors, 1e-8 for FP64,
+        or DCX_CHECK_TOL), tol <= 0 turns the check off.  A mixed-precision
         solve failing it gets up to `max_passes` FP64-residual refinement
         passes, then its failing systems are refactorised and solved in FP64
         (see `_fp64_fallback`); an FP64 solve failing it raises
         `NumericError`."""
         if tol is None:
             env = os.environ.get("DCX_CHECK_TOL")
-            tol = float(env) if env else (1e-10 if self.precision == "fp64" else 1e-5)
+            tol = float(env) if env else (1e-8 if self.precision == "fp64" else 1e-3)
         _lib.check(self._lib.dcx_plan_set_check(self._h, float(tol), int(max_passes),
                                                 int(sample_cols)), "plan_set_check")
         self.check_tol = float(tol)
 
     def check_report(self):
-        """The last solve's check: eta (backward error per system), pivot_fail,
-        passes (refinement passes run), status."""
+        """The last solve's check: rho (relative residual per system, the
+        tested quantity), eta (normwise backward error), pivot_fail, passes
+        (refinement passes run), status."""
+        rho = np.zeros(self.nbatch)
         eta = np.zeros(self.nbatch)
         piv = np.zeros(self.nbatch, np.int32)
         passes, status = C.c_int(), C.c_int()
         _lib.check(self._lib.dcx_plan_get_check(
-            self._h, eta.ctypes.data_as(_lib._f64p), piv.ctypes.data_as(_lib._i32p),
-            C.byref(passes), C.byref(status)), "plan_get_check")
-        return {"eta": eta, "pivot_fail": piv.astype(bool), "passes": passes.value,
+            self._h, rho.ctypes.data_as(_lib._f64p), eta.ctypes.data_as(_lib._f64p),
+            piv.ctypes.data_as(_lib._i32p), C.byref(passes), C.byref(status)),
+            "plan_get_check")
+        return {"rho": rho, "eta": eta, "pivot_fail": piv.astype(bool), "passes": passes.value,
                 "status": {0: "none", 1: "passed", 2: "failed", 3: "skipped"}[status.value]}
 
     def inject_fault(self, batch, rel):
@@ -212,7 +215,7 @@ class HipBatchSolver:
         those systems; raises NumericError when the FP64 factorisation fails
         too (the matrix is not positive definite)."""
         rep = self.check_report()
-        bad = np.nonzero(rep["pivot_fail"] | ~(rep["eta"] <= self.check_tol))[0]
+        bad = np.nonzero(rep["pivot_fail"] | ~(rep["rho"] <= self.check_tol))[0]
         if bad.size == 0:
             raise exc
         twin = self._fp64_twin()
@@ -233,7 +236,7 @@ class HipBatchSolver:
         x[idx] = x64[idx].to(x.dtype)
         self.fallbacks.append((self.stamp, bad.tolist()))
         warnings.warn(f"mixed-precision solve: systems {bad.tolist()} failed the backward-error "
-                      f"check (eta {rep['eta'][bad].tolist()}) after {rep['passes']} "
+                      f"check (relative residual {rep['rho'][bad].tolist()}) after {rep['passes']} "
                       "refinement pass(es); refactorised and solved in FP64", RuntimeWarning,
                       stacklevel=3)
         return bad.tolist()

@@ -257,3 +257,21 @@ def test_robin_3d_mutants_fail(monkeypatch, name):
     monkeypatch.setattr(dc3d, "robin_alpha_3d", _MUTANTS_3D[name])
     e1, e2 = _robin_error_3d(2.0), _robin_error_3d(1.0)
     assert not _converges(e1, e2, 2e-2), (name, e1, e2)
+
+
+def test_electrode_path_equals_source_path():
+    """The oracle's electrode-column path (used at the 25,088-cell and
+    62,016-cell sizes) against its source-column restatement: misfit,
+    gradient, dpred and Jacobian rows, to rounding."""
+    from oracle.dc25d import jacobian_rows_electrodes, misfit_and_grad_electrodes
+    mesh, survey, m, O = _tiny(nky=4)
+    d = O.dpred(m)
+    rng = np.random.default_rng(4)
+    dobs = d * (1 + 0.03 * rng.standard_normal(d.size))
+    w = 1.0 / (0.03 * np.abs(d))
+    phi, g, d0 = O.misfit_and_grad(m, dobs, w)
+    phi_e, g_e, d_e = misfit_and_grad_electrodes(O, m, dobs, w)
+    assert phi_e == pytest.approx(phi, rel=1e-12)
+    assert rel(g_e, g) < 1e-12 and rel(d_e, d) < 1e-12
+    rows = np.array([0, 3, O.nD // 2, O.nD - 1])
+    assert rel(jacobian_rows_electrodes(O, m, rows), O.Jmatrix(m)[rows]) < 1e-12
