@@ -10,15 +10,21 @@ misfit, adjoint solve, gradient gather and transposed map -- all resident on
 the GPU, inputs in HBM before the timed region.
 
 Prints one JSON line.  `value` is the mixed-precision eval+grad time in ms
-(FP32 factorisation and sweeps, FP64 assembly, projection, residual and
-gather; `--refine N` adds N FP64-residual refinement passes per solve).  The
-FP64 time, the errors of the mixed result against the FP64 one, the roofline
-of the dominant kernel (the backward sweep) from HIP events on the solve
-stream, and the CPU baseline (the oracle's SuperLU restatement of the
-reference CPU path on a bounded sample) are reported beside it, with the
-Gauss-Newton building blocks (`linearization`), the 3-D engine on a
-56 x 56 x 38-cell grid (`grid3d`) and an uneven batch of 16 independent lines
-(`uneven_lines`).
+(FP32 factorisation, sweeps and fields; FP64 assembly, projection, sums and
+the residual check of every solve, which refines or refactorises in FP64 a
+wavenumber whose check fails; `--refine N` adds N fixed FP64-residual
+refinement passes per solve).  Beside it: the FP64 time, the errors of the
+mixed result against the FP64 one and the check's report, the roofline of
+the dominant kernel (the backward sweep: its algorithmic bytes per launch
+over its average launch time, from HIP events on the solve stream; HBM
+traffic per launch from the committed rocprofv3 counter summary), and the
+CPU baseline (the oracle's restatement of the reference CPU path, the whole
+eval+grad at every wavenumber, median of 3 runs, sequential and with the
+wavenumbers on threads), with the Gauss-Newton building blocks
+(`linearization`), the 3-D engine on a 56 x 56 x 38-cell grid (`grid3d`) and
+an uneven batch of 16 independent lines (`uneven_lines`).
+`--summarize-trace` and `--summarize-pmc` turn rocprofv3 output of this
+script into the tables kept under profiles/.
 """
 import argparse
 import json
@@ -101,11 +107,17 @@ def sweep_roofline(eng, mt, dobs, w, refine, reps=3):
     sv.set_timing(False)
     launches = nsolve * info["level_launches"]
     ts = 4 if info["precision"] == 1 else 8
-    nb, k = info["nbatch"], eng.nS
-    # per backward sweep: read y_J and x_R rows, write x_J rows (and an FP64
-    # copy when x64_copy), read every panel once
-    per_sweep = nb * k * (ts * (info["sweep_rows_nf"] + info["sweep_rows_ns"])
-                          + (8 * info["sweep_rows_ns"] if x64_copy else 0)) \
+    fs = 4 if (info["precision"] == 1 and eng._gpu.fields32) else 8
+    nb, k, n = info["nbatch"], eng.nS, info["n"]
+    # algorithmic bytes of one backward sweep: the factor once (nnz_L entries
+    # of s_f bytes) and the field block in and out (n x k of s bytes each way)
+    # per wavenumber, plus the FP64 copy of x when the plan writes one
+    per_sweep = nb * (ts * info["nnz_L"] + 2 * fs * n * k + (8 * n * k if x64_copy else 0))
+    # the bytes the kernels address: every supernode's front rows (x_R
+    # re-read by each supernode that borders them, mostly from L2) and the
+    # stored sweep panels
+    access = nb * k * (ts * (info["sweep_rows_nf"] + info["sweep_rows_ns"])
+                       + (8 * info["sweep_rows_ns"] if x64_copy else 0)) \
         + nb * info["panel_size"] * ts
     flops = 2.0 * nb * k * info["solve_flops_per_rhs"] / 4.0
     avg_ms = b_ms / launches
@@ -114,6 +126,8 @@ def sweep_roofline(eng, mt, dobs, w, refine, reps=3):
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": None, "avg_ms": round(avg_ms, 4),
             "bytes_per_launch": int(per_sweep / info["level_launches"]),
+            "bytes_per_sweep": int(per_sweep),
+            "access_bytes_per_launch": int(access / info["level_launches"]),
             "tflops": round(flops / (b_ms / nsolve * 1e-3) / 1e12, 2),
             "tflops_peak": MFMA_PEAK["fp32" if ts == 4 else "fp64"],
             "launches_per_sweep": info["level_launches"],
@@ -560,24 +574,196 @@ def one_plan_times(lines, ms, data, steps, warmup, compare=None, rhs_basis="sour
     return {"one_plan_ms": round(ms_forest, 2), "one_plan": one}
 
 
-def cpu_baseline(mesh, survey, m, kys, wq, sample_ky=1):
-    """The oracle's SuperLU path (reference CPU solver semantics) on a sample
-    of `sample_ky` wavenumbers, scaled to all of them."""
-    from oracle.dc25d import Problem2D
-    import scipy.sparse.linalg as spla
+def host_cores():
+    """(cores this process may use, CPU model name)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return n, model
+
+
+def cpu_baseline(mesh, survey, m, kys, wq, dobs, w, reps=3):
+    """The reference CPU path restated by the oracle (SuperLU with SciPy
+    defaults per wavenumber, forward and adjoint solves of all 849 source
+    columns, projection, misfit and the gradient) on the whole eval+grad at
+    every wavenumber: the median of `reps` runs one wavenumber after another
+    (the reference's loop, one core for the factorisations and solves), and
+    the median of `reps` runs with the independent wavenumbers on a thread
+    pool (min(nky, 16, cores) threads)."""
+    from oracle.dc25d import Problem2D, misfit_and_grad_by_ky
     O = Problem2D(mesh.h[0], mesh.h[1], mesh.origin, survey, kys, wq)
-    sigma = np.exp(-m)
-    rng = np.random.default_rng(0)
-    G = O._adjoint_rhs(rng.standard_normal(O.nD))
-    t0 = time.perf_counter()
-    for k in range(sample_ky):
-        lu = spla.splu(O.A(sigma, k).tocsc())
-        U = lu.solve(np.ascontiguousarray(O.q))
-        Lam = lu.solve(np.ascontiguousarray(G))
-        gl, gu = O.Grad @ Lam, O.Grad @ U
-        _ = O.Qe_.T @ (gl * gu).sum(1) + O.Qn_.T @ (Lam * U).sum(1)
-    t = time.perf_counter() - t0
-    return 1e3 * t * len(kys) / sample_ky, t
+    ncores, model = host_cores()
+    workers = max(1, min(len(kys), 16, ncores))
+    out = {}
+    for label, nw in (("sequential", 1), ("threaded", workers)):
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            misfit_and_grad_by_ky(O, m, dobs, w, workers=nw)
+            ts.append(1e3 * (time.perf_counter() - t0))
+        out[label] = (sorted(ts)[len(ts) // 2], ts, nw)
+    return out, ncores, model
+
+
+def summarize_trace(path, steps):
+    """Per-kernel and per-family tables of the mixed-precision eval+grad steps
+    that bench.py timed, from a rocprofv3 --kernel-trace CSV of a bench.py run
+    (markdown on stdout).  A step starts at k_assemble; steps are classed by
+    their kernels (FP64 or FP32 factor, electrode basis or source basis); the
+    timed mixed steps are the last `steps` of the first run of source-basis
+    FP32 steps that follows a run of source-basis FP64 steps after the tile
+    tuning (the tuner's trial runs, which come first, are excluded)."""
+    import csv
+    import re
+    rows = sorted(csv.DictReader(open(path)), key=lambda r: int(r["Start_Timestamp"]))
+    # a step: k_assemble up to k_lmap_t (the transposed map closes the
+    # gradient) and the kernels that follow it back to back (autograd's tail)
+    steps_all, cur, closed, last_end = [], None, False, 0
+    for r in rows:
+        name = r["Kernel_Name"]
+        if "k_assemble" in name:
+            cur, closed = [], False
+            steps_all.append(cur)
+        elif cur is not None and closed and int(r["Start_Timestamp"]) - last_end > 30_000:
+            cur = None
+        if cur is not None:
+            cur.append(r)
+            last_end = max(last_end, int(r["End_Timestamp"])) if closed else int(r["End_Timestamp"])
+            closed = closed or "k_lmap_t" in name
+
+    def kind(st):
+        names = " ".join(r["Kernel_Name"] for r in st)
+        if "k_rows_mix" in names or "k_jpair" in names or "k_spmm" in names:
+            return "other"
+        if "k_factor_level<float" in names:
+            return "mixed"
+        if "k_factor_level<double" in names:
+            return "fp64"
+        return "other"
+    kinds = [kind(st) for st in steps_all]
+    # runs of equal kind
+    runs, i = [], 0
+    while i < len(kinds):
+        j = i
+        while j < len(kinds) and kinds[j] == kinds[i]:
+            j += 1
+        runs.append((kinds[i], i, j))
+        i = j
+    # the timed pair is the last fp64 run followed by a mixed run of at least
+    # `steps` steps (tuning: fp64 trials, mixed trials; timing: fp64, mixed)
+    sel = None
+    for a, b in zip(runs, runs[1:]):
+        if a[0] == "fp64" and b[0] == "mixed" and b[2] - b[1] >= steps:
+            sel = b
+    if sel is None:
+        raise SystemExit("no timed mixed steps found in the trace")
+    chosen = steps_all[sel[2] - steps:sel[2]]
+
+    def short(n):
+        n = re.sub(r"^void |\(anonymous namespace\)::", "", n)
+        return n.split("(")[0]
+    per, fam = {}, {}
+    for st in chosen:
+        for r in st:
+            d = (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3
+            k = short(r["Kernel_Name"])
+            f = k.split("<")[0] + ("<float>" if "<float" in k else "<double>" if "<double" in k else "")
+            for tab, key in ((per, k), (fam, f)):
+                c, t = tab.get(key, (0, 0.0))
+                tab[key] = (c + 1, t + d)
+    span = [(int(st[-1]["End_Timestamp"]) - int(st[0]["Start_Timestamp"])) / 1e3 for st in chosen]
+    print(f"Timed mixed steps: {steps} (trace steps {sel[2] - steps}..{sel[2] - 1}); "
+          f"span per step {np.mean(span):.1f} us (first kernel start to last kernel end)\n")
+    for title, tab in (("Per kernel family (all template instances)", fam),
+                       ("Per kernel instance", per)):
+        print(f"### {title}\n")
+        print("| kernel | dispatches per step | us per step | average us per dispatch |")
+        print("|---|---|---|---|")
+        for k, (c, t) in sorted(tab.items(), key=lambda kv: -kv[1][1]):
+            print(f"| `{k}` | {c / steps:g} | {t / steps:.1f} | {t / c:.2f} |")
+        print()
+
+
+def summarize_pmc(fetch_csv, write_csv, out_json):
+    """HBM bytes per dispatch from two rocprofv3 --pmc runs of `bench.py
+    --pmc-pass` (FETCH_SIZE, then WRITE_SIZE; counter_collection CSVs): each
+    counter (KiB) scaled by what it reads for the pass's 1 GiB calibration
+    copy (the largest copy dispatch moves 2^30 bytes each way); per kernel and
+    per family (name up to its template arguments, all instances).  Writes
+    `out_json` (read by the benchmark's roofline) and prints markdown."""
+    import csv
+    import re
+
+    def load(path, counter):
+        vals, names = {}, {}
+        for r in csv.DictReader(open(path)):
+            if r.get("Counter_Name") != counter:
+                continue
+            d = r["Dispatch_Id"]
+            vals[d] = vals.get(d, 0.0) + float(r["Counter_Value"])
+            names[d] = r["Kernel_Name"]
+        return vals, names
+    fv, fn = load(fetch_csv, "FETCH_SIZE")
+    wv, wn = load(write_csv, "WRITE_SIZE")
+
+    def calib(vals, names):
+        copies = [vals[d] for d in vals if "copyBuffer" in names[d] or "copy" in names[d].lower()]
+        return (1 << 30) / (1024.0 * max(copies))
+    fs, ws = calib(fv, fn), calib(wv, wn)
+
+    def short(n):
+        n = re.sub(r"^void |\(anonymous namespace\)::", "", n)
+        return n.split("(")[0]
+
+    def per_kernel(vals, names, scale):
+        out = {}
+        for d, v in vals.items():
+            k = short(names[d])
+            c, t = out.get(k, (0, 0.0))
+            out[k] = (c + 1, t + v * 1024.0 * scale)
+        return out
+    pf, pw = per_kernel(fv, fn, fs), per_kernel(wv, wn, ws)
+    kernels, fams = {}, {}
+    for k in set(pf) | set(pw):
+        cf, tf = pf.get(k, (0, 0.0))
+        cw, tw = pw.get(k, (0, 0.0))
+        c = max(cf, cw)
+        f = tf / cf if cf else 0.0
+        w_ = tw / cw if cw else 0.0
+        kernels[k] = {"fetch_bytes": f, "write_bytes": w_, "traffic_bytes": f + w_,
+                      "dispatches": c}
+        fam = k.split("<")[0] + ("<float>" if "<float" in k else "<double>" if "<double" in k else "")
+        a = fams.setdefault(fam, {"fetch": 0.0, "write": 0.0, "dispatches": 0})
+        a["fetch"] += tf
+        a["write"] += tw
+        a["dispatches"] += c
+    families = {k: {"fetch_bytes": v["fetch"] / v["dispatches"],
+                    "write_bytes": v["write"] / v["dispatches"],
+                    "traffic_bytes": (v["fetch"] + v["write"]) / v["dispatches"],
+                    "dispatches": v["dispatches"]} for k, v in fams.items()}
+    with open(out_json, "w") as fh:
+        json.dump({"calibration": {"fetch_scale": fs, "write_scale": ws},
+                   "kernels": kernels, "families": families}, fh, indent=1)
+    print(f"Calibration (1 GiB copy): FETCH_SIZE x {fs:.3f}, WRITE_SIZE x {ws:.3f}\n")
+    for title, tab in (("Per kernel family (all template instances)", families),
+                       ("Per kernel instance", kernels)):
+        print(f"### {title}\n")
+        print("| kernel | dispatches | fetch MB/dispatch | write MB/dispatch | traffic MB/dispatch |")
+        print("|---|---|---|---|---|")
+        for k, v in sorted(tab.items(), key=lambda kv: -kv[1]["traffic_bytes"] * kv[1]["dispatches"]):
+            print(f"| `{k}` | {v['dispatches']} | {v['fetch_bytes'] / 1e6:.1f} | "
+                  f"{v['write_bytes'] / 1e6:.1f} | {v['traffic_bytes'] / 1e6:.1f} |")
+        print()
 
 
 def pmc_pass(args):
@@ -607,6 +793,12 @@ def main():
     ap.add_argument("--refine", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--pmc-pass", action="store_true")
+    ap.add_argument("--summarize-pmc", nargs=3, metavar=("FETCH_CSV", "WRITE_CSV", "OUT_JSON"),
+                    help="HBM traffic per dispatch from the two --pmc-pass counter CSVs "
+                         "(FETCH_SIZE, WRITE_SIZE); writes OUT_JSON, prints markdown, exits")
+    ap.add_argument("--summarize-trace", metavar="CSV",
+                    help="print per-kernel tables of the timed mixed steps from a rocprofv3 "
+                         "kernel-trace CSV of a bench.py run (--steps as in that run) and exit")
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the default kernel-variant thresholds (no tile autotuning)")
     ap.add_argument("--workload", choices=("all", "line25k", "grid3d", "lines",
@@ -620,6 +812,12 @@ def main():
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported "
               "(the path runs on one GPU; there is no multi-GPU code path)", file=sys.stderr)
         sys.exit(2)
+    if args.summarize_pmc:
+        summarize_pmc(*args.summarize_pmc)
+        return
+    if args.summarize_trace:
+        summarize_trace(args.summarize_trace, args.steps)
+        return
     if args.pmc_pass:
         pmc_pass(args)
         return
@@ -662,7 +860,11 @@ def main():
         dmx = emx.dpred(mt)
     err_d = float(torch.linalg.norm(dmx - d64) / torch.linalg.norm(d64))
     err_g = float(torch.linalg.norm(gmx - g64) / torch.linalg.norm(g64))
-    log(f"mixed vs fp64: dpred {err_d:.2e} grad {err_g:.2e}")
+    rep = emx.solver.check_report()
+    check = {"status": rep["status"], "refinement_passes": rep["passes"],
+             "relative_residual_max": float(rep["rho"].max()),
+             "tolerance": emx.solver.check_tol, "fp64_fallbacks": len(emx.solver.fallbacks)}
+    log(f"mixed vs fp64: dpred {err_d:.2e} grad {err_g:.2e}; solve check {check}")
     tiles = None
     if not args.no_tune:
         tiles, t_tune = tune_tiles({"fp64": e64, "mixed": emx}, mt, dobs, w)
@@ -703,12 +905,21 @@ def main():
         log(f"readme case {rc}")
     cpu = None
     if not args.no_cpu:
-        log("cpu baseline (oracle SuperLU, 1 wavenumber sample)")
-        cms, ct = cpu_baseline(mesh, survey, m, e64.kys, e64.wq_np)
-        cpu = {"value": round(cms, 1), "unit": "ms", "cores": 1, "kind": "port",
-               "sample": f"1 of {e64.nky} wavenumbers (SuperLU factor + forward and adjoint "
-                         f"solves of all {e64.nS} sources + gradient gather), {ct:.1f} s, "
-                         f"scaled x{e64.nky}"}
+        log("cpu baseline (the oracle's restatement of the reference CPU path, all "
+            "wavenumbers, median of 3, sequential and threaded)")
+        res, ncores, model = cpu_baseline(mesh, survey, m, e64.kys, e64.wq_np,
+                                          dobs.cpu().numpy(), w.cpu().numpy())
+        seq, thr = res["sequential"], res["threaded"]
+        cpu = {"value": round(seq[0], 1), "unit": "ms", "cores": 1, "kind": "port",
+               "sample": f"the whole eval+grad: SuperLU (SciPy defaults) factorisation, forward "
+                         f"and adjoint solves of all {e64.nS} source columns and the gradient "
+                         f"at all {e64.nky} wavenumbers, one after another; median of "
+                         f"{len(seq[1])} runs {[round(t) for t in seq[1]]} ms",
+               "threaded": {"value": round(thr[0], 1), "unit": "ms", "cores": thr[2],
+                            "runs_ms": [round(t) for t in thr[1]],
+                            "sample": f"the same with the wavenumbers on {thr[2]} threads"},
+               "host": {"cores_available": ncores, "cpu_model": model}}
+        log(f"cpu baseline {cpu}")
     out = {
         "metric": METRIC, "value": round(msmx, 3), "unit": "ms", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(msmx, 3),
@@ -722,6 +933,8 @@ def main():
                    "nky": e64.nky, "refine_passes": args.refine},
         "fp64_ms": round(ms64, 3), "mixed_speedup_vs_fp64": round(ms64 / msmx, 3),
         "mixed_dpred_rel_err": err_d, "mixed_grad_rel_err": err_g,
+        "mixed_dpred_within_1e-6": bool(err_d <= 1e-6),
+        "mixed_solve_check": check,
         "build_s": round(t_build, 2),
         "tile_tuning": tiles,
         "roofline": roof, "roofline_fp64": roof64,

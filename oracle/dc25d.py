@@ -358,6 +358,48 @@ class Problem2D:
 
 
 # --------------------------------------------------------------------------
+# the reference CPU chain wavenumber by wavenumber (the CPU baseline)
+# --------------------------------------------------------------------------
+def misfit_and_grad_by_ky(P, m, dobs, w, workers=1):
+    """Problem2D.misfit_and_grad organised as the reference's CPU path runs it
+    (dctorch/solver.py:314-374 with simulation2d.py:345-375): per wavenumber
+    one SuperLU factorisation (SciPy defaults) and the forward solve of every
+    source column; the projection and misfit; per wavenumber the adjoint
+    solve on the kept factor and the gradient contribution.  workers > 1
+    runs the independent wavenumbers of each phase on a thread pool (SuperLU
+    releases the interpreter lock); workers = 1 is the reference's sequential
+    loop.  Returns (phi, grad_m, dpred)."""
+    from concurrent.futures import ThreadPoolExecutor
+    sigma = np.exp(-m)
+    q = np.ascontiguousarray(P.q)
+
+    def run(fn):
+        if workers <= 1:
+            return [fn(k) for k in range(P.nky)]
+        with ThreadPoolExecutor(min(workers, P.nky)) as ex:
+            return list(ex.map(fn, range(P.nky)))
+
+    def fwd(k):
+        lu = spla.splu(P.A(sigma, k).tocsc())
+        return lu, lu.solve(q)
+    F = run(fwd)
+    Ubar = sum(P.wq[k] * F[k][1] for k in range(P.nky))
+    d = (P.Pd @ Ubar)[np.arange(P.nD), P.sidx]
+    r = d - dobs
+    phi = float(((w * r) ** 2).sum())
+    G = np.ascontiguousarray(P._adjoint_rhs(2.0 * w ** 2 * r))
+
+    def adj(k):
+        lu, U = F[k]
+        Lam = lu.solve(G)
+        gl, gu = P.Grad @ Lam, P.Grad @ U
+        nodal = (Lam * U).sum(1)
+        return P.wq[k] * (P.Qe_.T @ (gl * gu).sum(1) + P.kys[k] ** 2 * (P.Qn_.T @ nodal)
+                          + P.AvgBC_[k].T @ nodal)
+    return phi, sigma * sum(run(adj)), d
+
+
+# --------------------------------------------------------------------------
 # the same quantities through the electrode positions (exact by linearity)
 # --------------------------------------------------------------------------
 def _electrode_basis(P):

@@ -275,3 +275,17 @@ def test_electrode_path_equals_source_path():
     assert rel(g_e, g) < 1e-12 and rel(d_e, d) < 1e-12
     rows = np.array([0, 3, O.nD // 2, O.nD - 1])
     assert rel(jacobian_rows_electrodes(O, m, rows), O.Jmatrix(m)[rows]) < 1e-12
+
+
+@pytest.mark.parametrize("workers", [1, 3])
+def test_by_ky_chain_equals_misfit_and_grad(workers):
+    """The CPU-baseline organisation of the oracle (wavenumber by wavenumber,
+    optionally threaded) gives misfit_and_grad's results."""
+    from oracle.dc25d import misfit_and_grad_by_ky
+    mesh, survey, m, O = _tiny(nky=4)
+    d = O.dpred(m)
+    dobs, w = d * 1.03, 1.0 / (0.03 * np.abs(d))
+    phi, g, d0 = O.misfit_and_grad(m, dobs, w)
+    phi_k, g_k, d_k = misfit_and_grad_by_ky(O, m, dobs, w, workers=workers)
+    assert phi_k == pytest.approx(phi, rel=1e-13)
+    assert rel(g_k, g) < 1e-13 and rel(d_k, d0) < 1e-14
