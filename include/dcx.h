@@ -286,6 +286,20 @@ int dcx_jacobian_rows(dcx_lmap* map, dcx_plan* plan, const double* Y, int r, con
                       int nS, const int32_t* src, const double* wq, const double* sigma,
                       double* J, int flags, void* stream);
 
+/* dcx_jacobian_rows without allocating: the same arguments, kernels and
+ * results (bit for bit), with the pairwise-sum workspace in a fixed 64 MiB
+ * device array of the library (one per GPU, held until the process ends)
+ * instead of a buffer grown with hipMalloc.  Refuses (DCX_ERR_ARG, naming the
+ * largest r that fits) a block needing more than 64 MiB:
+ * (2 nnz + max(nbc, 1)) r 8 bytes, e.g. 2 MB per datum on the 25,088-cell
+ * line, at most 33 data per call.  Calls that overlap on one GPU share the
+ * array and race: order them on one stream.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, serving loops, harnesses that forbid hipMalloc). */
+int dcx_jacobian_rows_noalloc(dcx_lmap* map, dcx_plan* plan, const double* Y, int r,
+                              const double* U, int nS, const int32_t* src, const double* wq,
+                              const double* sigma, double* J, int flags, void* stream);
+
 /* ---- receivers and adjoint sources ----------------------------------------
  * Pd (nD x nN, CSR, host) and the source index of each datum.  Replaces the
  * projection torch.sparse.mm(Pd, Ubar)[iD, sidx] and its backward

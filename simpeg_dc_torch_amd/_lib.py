@@ -78,6 +78,8 @@ PROTOTYPES = {
     "dcx_spmm": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, C.c_double, C.c_int, _vp]),
     "dcx_jacobian_rows": (C.c_int, [_vp, _vp, _vp, C.c_int, _vp, C.c_int, _vp, _vp, _vp, _vp,
                                     C.c_int, _vp]),
+    "dcx_jacobian_rows_noalloc": (C.c_int, [_vp, _vp, _vp, C.c_int, _vp, C.c_int, _vp, _vp,
+                                            _vp, _vp, C.c_int, _vp]),
     "dcx_set_rhs_support": (C.c_int, [_vp, _i32p, C.c_int]),
     "dcx_rows_mix": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, C.c_int, C.c_int, _i32p,
                                C.c_int, _vp]),

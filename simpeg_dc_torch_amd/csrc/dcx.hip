@@ -17,6 +17,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <tuple>
 #include <type_traits>
@@ -4103,9 +4104,14 @@ int dcx_rows_mix(const void* g, const void* w, void* h, int nbatch, int n, int k
   return DCX_OK;
 }
 
-int dcx_jacobian_rows(dcx_lmap* M, dcx_plan* P, const double* Y, int r, const double* U, int nS,
-                      const int32_t* src, const double* wq, const double* sigma, double* J,
-                      int flags, void* stream) {
+}  // extern "C"
+
+namespace {
+// checks shared by dcx_jacobian_rows and its allocation-free variant; the
+// workspace they need, in bytes, for r data
+int jacobian_rows_args(const dcx_lmap* M, const dcx_plan* P, const void* Y, int r,
+                       const void* U, int nS, const void* src, const void* wq,
+                       const void* sigma, const void* J, size_t* need) {
   if (!M || !P || !Y || !U || !src || !wq || !sigma || !J)
     return fail(DCX_ERR_ARG, "null argument");
   if (r < 1 || nS < 1) return fail(DCX_ERR_ARG, "r and nS must be >= 1");
@@ -4115,18 +4121,17 @@ int dcx_jacobian_rows(dcx_lmap* M, dcx_plan* P, const double* Y, int r, const do
     return fail(DCX_ERR_ARG, "more boundary terms per entry than k_jpair holds");
   if (M->ngroup != 1)
     return fail(DCX_ERR_ARG, "Jacobian rows of a map with wavenumber groups are not supported");
-  hipStream_t st = static_cast<hipStream_t>(stream);
-  HIP_TRY(hipSetDevice(M->device));
-  const size_t need = (2 * static_cast<size_t>(M->nnz) + std::max(M->nbc, 1)) * r * sizeof(double);
-  if (M->jws_bytes < need) {
-    HIP_TRY(hipStreamSynchronize(st));
-    if (M->d_jws) HIP_TRY(hipFree(M->d_jws));
-    M->d_jws = nullptr;
-    M->jws_bytes = 0;
-    HIP_TRY(hipMalloc(&M->d_jws, need));
-    M->jws_bytes = need;
-  }
-  double* ge = M->d_jws;
+  *need = (2 * static_cast<size_t>(M->nnz) + std::max(M->nbc, 1)) * r * sizeof(double);
+  return DCX_OK;
+}
+
+// the two kernels of a block of Jacobian rows, with the pairwise sums in the
+// workspace ws (need bytes; every entry of it is written before it is read)
+void jacobian_rows_launch(const dcx_lmap* M, const dcx_plan* P, const double* Y, int r,
+                          const double* U, int nS, const int32_t* src, const double* wq,
+                          const double* sigma, double* J, int flags, double* ws,
+                          hipStream_t st) {
+  double* ge = ws;
   double* gn = ge + static_cast<size_t>(M->nnz) * r;
   double* gb = gn + static_cast<size_t>(M->nnz) * r;
   const int n = P->S.n;
@@ -4137,6 +4142,78 @@ int dcx_jacobian_rows(dcx_lmap* M, dcx_plan* P, const double* Y, int r, const do
   hipLaunchKernelGGL(k_jmap, dim3(grid1(M->ncell, 4)), dim3(256), 0, st, M->ncell, r, M->d_tptr,
                      M->d_tent, M->d_te, M->d_le, M->d_ln, M->d_tbptr, M->d_tbq, ge, gn, gb,
                      sigma, J);
+}
+}  // namespace
+
+// The allocation-free variant's workspace: one file-scope device array per
+// GPU, held until the process ends.  In host code `scratch` names a host
+// stand-in, not a device address: its device address is taken once per
+// device with hipGetSymbolAddress (scratch_ptr) and only that address is
+// passed to kernels.
+constexpr size_t kScratchBytes = static_cast<size_t>(64) << 20;
+static __device__ __align__(256) double scratch[kScratchBytes / sizeof(double)];
+
+namespace {
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_dev[kMaxScratchDevices] = {};
+
+// the device address of `scratch` on the current device `dev`
+int scratch_ptr(int dev, double** out) {
+  if (dev < 0 || dev >= kMaxScratchDevices) return fail(DCX_ERR_ARG, "device index out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(DCX_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") +
+                                   hipGetErrorString(e));
+    g_scratch_dev[dev] = p;
+  }
+  *out = static_cast<double*>(g_scratch_dev[dev]);
+  return DCX_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int dcx_jacobian_rows(dcx_lmap* M, dcx_plan* P, const double* Y, int r, const double* U, int nS,
+                      const int32_t* src, const double* wq, const double* sigma, double* J,
+                      int flags, void* stream) {
+  size_t need = 0;
+  if (int rc = jacobian_rows_args(M, P, Y, r, U, nS, src, wq, sigma, J, &need)) return rc;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  HIP_TRY(hipSetDevice(M->device));
+  if (M->jws_bytes < need) {
+    HIP_TRY(hipStreamSynchronize(st));
+    if (M->d_jws) HIP_TRY(hipFree(M->d_jws));
+    M->d_jws = nullptr;
+    M->jws_bytes = 0;
+    HIP_TRY(hipMalloc(&M->d_jws, need));
+    M->jws_bytes = need;
+  }
+  jacobian_rows_launch(M, P, Y, r, U, nS, src, wq, sigma, J, flags, M->d_jws, st);
+  HIP_TRY(hipGetLastError());
+  return DCX_OK;
+}
+
+int dcx_jacobian_rows_noalloc(dcx_lmap* M, dcx_plan* P, const double* Y, int r,
+                              const double* U, int nS, const int32_t* src, const double* wq,
+                              const double* sigma, double* J, int flags, void* stream) {
+  size_t need = 0;
+  if (int rc = jacobian_rows_args(M, P, Y, r, U, nS, src, wq, sigma, J, &need)) return rc;
+  if (need > kScratchBytes) {
+    const size_t per_row = need / static_cast<size_t>(r);
+    return fail(DCX_ERR_ARG, std::to_string(r) + " rows need " + std::to_string(need >> 20) +
+                                 " MiB of workspace; the allocation-free variant holds " +
+                                 std::to_string(kScratchBytes >> 20) + " MiB: at most " +
+                                 std::to_string(kScratchBytes / per_row) + " rows per call");
+  }
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  HIP_TRY(hipSetDevice(M->device));
+  double* ws = nullptr;
+  if (int rc = scratch_ptr(M->device, &ws)) return rc;
+  jacobian_rows_launch(M, P, Y, r, U, nS, src, wq, sigma, J, flags, ws, st);
   HIP_TRY(hipGetLastError());
   return DCX_OK;
 }
