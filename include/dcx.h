@@ -17,6 +17,7 @@
 #ifndef DCX_H_
 #define DCX_H_
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -88,6 +89,21 @@ typedef struct {
 } dcx_symbolic_view;
 
 const char* dcx_last_error(void);
+
+/* Device memory of plans, maps and surveys (factors, fronts, workspaces,
+ * device tables) comes from hipMalloc unless an allocator is registered:
+ * alloc(bytes, device, ctx) returns device memory on `device` (the current
+ * device) or NULL, and free_fn(ptr, bytes, device, ctx) takes it back after
+ * the library has waited for the device to be idle.  Registering (or
+ * clearing, with two NULLs) affects later allocations; each allocation is
+ * freed the way it was made.  The ctypes layer registers torch's caching
+ * allocator, so plan memory shows in torch.cuda.memory_allocated and
+ * torch.cuda.empty_cache can reclaim it: the ownership the reference could not
+ * have with cuDSS, whose memory sat outside torch (dctorch/simulation3d.py:
+ * 160-169).  The fixed array of dcx_jacobian_rows_noalloc is not allocated. */
+typedef void* (*dcx_alloc_fn)(size_t bytes, int device, void* ctx);
+typedef void (*dcx_free_fn)(void* ptr, size_t bytes, int device, void* ctx);
+int dcx_set_allocator(dcx_alloc_fn alloc, dcx_free_fn free_fn, void* ctx);
 
 /* Host-only symbolic analysis (no device work): the same ordering and
  * supernode structure dcx_plan_create builds, for inspection and tests. */

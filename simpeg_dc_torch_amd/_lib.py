@@ -50,6 +50,7 @@ class SymbolicView(C.Structure):
 # name -> (restype, argtypes); every entry point returns an int status
 PROTOTYPES = {
     "dcx_last_error": (C.c_char_p, []),
+    "dcx_set_allocator": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "dcx_device_count": (C.c_int, [C.POINTER(C.c_int)]),
     "dcx_symbolic_analyse": (C.c_int, [C.c_int, _i32p, _i32p, _f64p, C.c_int, C.c_int,
                                        C.c_int, C.POINTER(_vp)]),
@@ -104,6 +105,31 @@ PROTOTYPES = {
 _lib = None
 _load_error = None
 
+# the library's device memory through torch's caching allocator
+# (dcx_set_allocator); DCX_TORCH_ALLOCATOR=0 keeps plain hipMalloc
+_ALLOC_T = C.CFUNCTYPE(C.c_void_p, C.c_size_t, C.c_int, C.c_void_p)
+_FREE_T = C.CFUNCTYPE(None, C.c_void_p, C.c_size_t, C.c_int, C.c_void_p)
+
+
+def _torch_alloc(nbytes, device, ctx):
+    try:
+        import torch
+        return int(torch.cuda.caching_allocator_alloc(int(nbytes), int(device)))
+    except Exception:
+        return None
+
+
+def _torch_free(ptr, nbytes, device, ctx):
+    try:
+        import torch
+        torch.cuda.caching_allocator_delete(int(ptr))
+    except Exception:
+        pass
+
+
+_ALLOC_CB = _ALLOC_T(_torch_alloc)   # kept alive for the library's lifetime
+_FREE_CB = _FREE_T(_torch_free)
+
 
 def load():
     """The loaded library; ImportError (with the reason) when it is absent."""
@@ -125,6 +151,8 @@ def load():
         fn = getattr(lib, name)
         fn.restype = res
         fn.argtypes = args
+    if os.environ.get("DCX_TORCH_ALLOCATOR", "1") != "0":
+        lib.dcx_set_allocator(C.cast(_ALLOC_CB, C.c_void_p), C.cast(_FREE_CB, C.c_void_p), None)
     _lib = lib
     return lib
 

@@ -18,6 +18,7 @@
 #include <cstring>
 #include <memory>
 #include <mutex>
+#include <unordered_map>
 #include <string>
 #include <tuple>
 #include <type_traits>
@@ -54,11 +55,91 @@ struct FusedDev {
   int64_t uoff;
 };
 
+// ---------------------------------------------------------------------------
+// device memory of plans, maps and surveys: hipMalloc, or the caller's
+// allocator once one is registered (dcx_set_allocator); each allocation
+// remembers how it was made, so it is freed the same way
+// ---------------------------------------------------------------------------
+struct AllocRec {
+  size_t bytes;
+  int device;
+  dcx_free_fn free_fn;  // null: hipMalloc
+  void* ctx;
+};
+std::mutex g_alloc_mu;
+dcx_alloc_fn g_alloc_fn = nullptr;
+dcx_free_fn g_free_fn = nullptr;
+void* g_alloc_ctx = nullptr;
+std::unordered_map<void*, AllocRec> g_allocs;
+
+hipError_t dev_malloc_bytes(void** p, size_t bytes) {
+  *p = nullptr;
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  dcx_alloc_fn af;
+  dcx_free_fn ff;
+  void* ctx;
+  {
+    std::lock_guard<std::mutex> lk(g_alloc_mu);
+    af = g_alloc_fn;
+    ff = g_free_fn;
+    ctx = g_alloc_ctx;
+  }
+  if (af) {
+    void* q = af(bytes, dev, ctx);  // outside the lock: the callback may take others
+    if (!q) return hipErrorOutOfMemory;
+    *p = q;
+  } else {
+    e = hipMalloc(p, bytes);
+    if (e != hipSuccess) return e;
+    ff = nullptr;
+  }
+  std::lock_guard<std::mutex> lk(g_alloc_mu);
+  g_allocs[*p] = AllocRec{bytes, dev, ff, ctx};
+  return hipSuccess;
+}
+
+template <typename T>
+hipError_t dev_malloc(T** p, size_t bytes) {
+  void* q = nullptr;
+  const hipError_t e = dev_malloc_bytes(&q, bytes);
+  *p = static_cast<T*>(q);
+  return e;
+}
+
+// memory from the caller's allocator goes back to it after the device is
+// idle (the allocator may hand it to other work at once)
+hipError_t dev_free(void* p) {
+  if (!p) return hipSuccess;
+  AllocRec r{0, 0, nullptr, nullptr};
+  bool known = false;
+  {
+    std::lock_guard<std::mutex> lk(g_alloc_mu);
+    auto it = g_allocs.find(p);
+    if (it != g_allocs.end()) {
+      r = it->second;
+      known = true;
+      g_allocs.erase(it);
+    }
+  }
+  if (known && r.free_fn) {
+    int cur = 0;
+    (void)hipGetDevice(&cur);
+    if (cur != r.device) (void)hipSetDevice(r.device);
+    const hipError_t e = hipDeviceSynchronize();
+    r.free_fn(p, r.bytes, r.device, r.ctx);
+    if (cur != r.device) (void)hipSetDevice(cur);
+    return e;
+  }
+  return hipFree(p);
+}
+
 template <typename T>
 hipError_t upload(T** dst, const std::vector<T>& v) {
   *dst = nullptr;
   if (v.empty()) return hipSuccess;
-  hipError_t e = hipMalloc(dst, v.size() * sizeof(T));
+  hipError_t e = dev_malloc(dst, v.size() * sizeof(T));
   if (e != hipSuccess) return e;
   return hipMemcpy(*dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
 }
@@ -2310,7 +2391,7 @@ struct dcx_plan {
                     d_toff, d_lridx, d_ulvl, d_ufz, d_panelsT, d_fs_off, d_fs, d_dg,
                     d_chk, d_fvals};
     for (void* p : ptrs)
-      if (p) (void)hipFree(p);
+      if (p) (void)dev_free(p);
     if (h_chk) (void)hipHostFree(h_chk);
     if (chk_ev) (void)hipEventDestroy(chk_ev);
     if (chk_src) (void)hipEventDestroy(chk_src);
@@ -2357,10 +2438,10 @@ int ensure(dcx_plan* P, void** p, size_t* cap, size_t bytes, hipStream_t st) {
   if (int rc = refuse_in_capture(st)) return rc;
   ++P->buf_gen;
   HIP_TRY(hipStreamSynchronize(st));
-  if (*p) HIP_TRY(hipFree(*p));
+  if (*p) HIP_TRY(dev_free(*p));
   *p = nullptr;
   *cap = 0;
-  HIP_TRY(hipMalloc(p, std::max<size_t>(bytes, 256)));
+  HIP_TRY(dev_malloc(p, std::max<size_t>(bytes, 256)));
   *cap = bytes;
   return DCX_OK;
 }
@@ -2371,7 +2452,7 @@ int upload_fused(dcx_plan* P) {
   void* olds[] = {P->d_fz, P->d_flv, P->d_fmem, P->d_fgptr, P->d_fgb, P->d_toff,
                   P->d_lridx, P->d_ulvl, P->d_sup};
   for (void* q : olds)
-    if (q) HIP_TRY(hipFree(q));
+    if (q) HIP_TRY(dev_free(q));
   P->d_fz = nullptr;
   P->d_flv = P->d_fmem = P->d_fgptr = P->d_fgb = P->d_toff = P->d_lridx = P->d_ulvl = nullptr;
   P->d_sup = nullptr;
@@ -2801,10 +2882,10 @@ int ensure_upd(dcx_plan* P, int k, hipStream_t st) {
     if (int rc = refuse_in_capture(st)) return rc;
     ++P->buf_gen;
     HIP_TRY(hipStreamSynchronize(st));
-    if (P->d_fs) HIP_TRY(hipFree(P->d_fs));
+    if (P->d_fs) HIP_TRY(dev_free(P->d_fs));
     P->d_fs = nullptr;
     P->fs_bytes = 0;
-    HIP_TRY(hipMalloc(&P->d_fs, fsb));
+    HIP_TRY(dev_malloc(&P->d_fs, fsb));
     P->fs_bytes = fsb;
   }
   if (P->upd_k >= k) return DCX_OK;
@@ -2816,11 +2897,11 @@ int ensure_upd(dcx_plan* P, int k, hipStream_t st) {
   HIP_TRY(hipStreamSynchronize(st));
   void* olds[] = {P->d_upd[0], P->d_upd[1], P->d_ufz};
   for (void* q : olds)
-    if (q) HIP_TRY(hipFree(q));
+    if (q) HIP_TRY(dev_free(q));
   P->d_upd[0] = P->d_upd[1] = P->d_ufz = nullptr;
-  HIP_TRY(hipMalloc(&P->d_upd[0], std::max<size_t>(ub0, 256)));
-  HIP_TRY(hipMalloc(&P->d_upd[1], std::max<size_t>(ub1, 256)));
-  HIP_TRY(hipMalloc(&P->d_ufz, std::max<size_t>(ubf, 256)));
+  HIP_TRY(dev_malloc(&P->d_upd[0], std::max<size_t>(ub0, 256)));
+  HIP_TRY(dev_malloc(&P->d_upd[1], std::max<size_t>(ub1, 256)));
+  HIP_TRY(dev_malloc(&P->d_ufz, std::max<size_t>(ubf, 256)));
   P->upd_k = k;
   return DCX_OK;
 }
@@ -3157,6 +3238,16 @@ int dcx_device_count(int* out) {
   return DCX_OK;
 }
 
+int dcx_set_allocator(dcx_alloc_fn alloc, dcx_free_fn free_fn, void* ctx) {
+  if ((alloc == nullptr) != (free_fn == nullptr))
+    return fail(DCX_ERR_ARG, "give both an allocation and a free function, or neither");
+  std::lock_guard<std::mutex> lk(g_alloc_mu);
+  g_alloc_fn = alloc;
+  g_free_fn = free_fn;
+  g_alloc_ctx = ctx;
+  return DCX_OK;
+}
+
 int dcx_plan_create(int n, const int32_t* crow, const int32_t* col, int nbatch,
                     const double* coords, int dim, int leaf_max, int precision,
                     dcx_plan** out) {
@@ -3213,7 +3304,7 @@ int dcx_plan_create(int n, const int32_t* crow, const int32_t* col, int nbatch,
     for (int i = 0; i < n; ++i) P->max_row_nnz = std::max(P->max_row_nnz, crow[i + 1] - crow[i]);
     HIP_TRY(upload(&P->d_crow, vcrow));
     HIP_TRY(upload(&P->d_col, vcol));
-    HIP_TRY(hipMalloc(&P->d_info, sizeof(int) * nbatch));
+    HIP_TRY(dev_malloc(&P->d_info, sizeof(int) * nbatch));
     {
       const char* e = std::getenv("DCX_CHECK_TOL");
       P->chk_tol = e ? std::atof(e) : (precision == DCX_FP32 ? kCheckTol32 : kCheckTol64);
@@ -3222,16 +3313,16 @@ int dcx_plan_create(int n, const int32_t* crow, const int32_t* col, int nbatch,
     const size_t f0 = static_cast<size_t>(nbatch) * S.front_size[0] * P->tsize;
     const size_t f1 = static_cast<size_t>(nbatch) * S.front_size[1] * P->tsize;
     const size_t vb = static_cast<size_t>(nbatch) * S.nnz * sizeof(double);
-    HIP_TRY(hipMalloc(&P->d_panels, std::max<size_t>(pb, 256)));
-    HIP_TRY(hipMalloc(&P->d_panelsT, std::max<size_t>(pb, 256)));
-    HIP_TRY(hipMalloc(&P->d_front[0], std::max<size_t>(f0, 256)));
-    HIP_TRY(hipMalloc(&P->d_front[1], std::max<size_t>(f1, 256)));
-    HIP_TRY(hipMalloc(&P->d_vals, std::max<size_t>(vb, 256)));
+    HIP_TRY(dev_malloc(&P->d_panels, std::max<size_t>(pb, 256)));
+    HIP_TRY(dev_malloc(&P->d_panelsT, std::max<size_t>(pb, 256)));
+    HIP_TRY(dev_malloc(&P->d_front[0], std::max<size_t>(f0, 256)));
+    HIP_TRY(dev_malloc(&P->d_front[1], std::max<size_t>(f1, 256)));
+    HIP_TRY(dev_malloc(&P->d_vals, std::max<size_t>(vb, 256)));
     int maxbig = 0;
     for (int d = 0; d < S.ndepth; ++d)
       if (level_is_big(S, d, P->tsize, nbatch)) maxbig = std::max(maxbig, S.lvl_ptr[d + 1] - S.lvl_ptr[d]);
     const size_t dgb = static_cast<size_t>(nbatch) * maxbig * kDg * P->tsize;
-    HIP_TRY(hipMalloc(&P->d_dg, std::max<size_t>(dgb, 256)));
+    HIP_TRY(dev_malloc(&P->d_dg, std::max<size_t>(dgb, 256)));
     P->device_bytes = static_cast<int64_t>(2 * pb + f0 + f1 + vb + dgb);
     *out = P.release();
     return DCX_OK;
@@ -3412,7 +3503,7 @@ int dcx_plan_set_fusion(dcx_plan* P, int fuse_rows) {
         if (act[P->S.ulvl_sup[q]]) lst.push_back(P->S.ulvl_sup[q]);
       ptr[d + 1] = static_cast<int>(lst.size());
     }
-    if (P->d_act_lvl) HIP_TRY(hipFree(P->d_act_lvl));
+    if (P->d_act_lvl) HIP_TRY(dev_free(P->d_act_lvl));
     P->d_act_lvl = nullptr;
     if (lst.empty()) lst.push_back(0);
     HIP_TRY(upload(&P->d_act_lvl, lst));
@@ -3435,8 +3526,8 @@ int dcx_set_rhs_support(dcx_plan* P, const int32_t* rows, int nrows) {
       if (act[S.ulvl_sup[q]]) lst.push_back(S.ulvl_sup[q]);
     ptr[d + 1] = static_cast<int>(lst.size());
   }
-  if (P->d_active) HIP_TRY(hipFree(P->d_active));
-  if (P->d_act_lvl) HIP_TRY(hipFree(P->d_act_lvl));
+  if (P->d_active) HIP_TRY(dev_free(P->d_active));
+  if (P->d_act_lvl) HIP_TRY(dev_free(P->d_act_lvl));
   P->d_active = P->d_act_lvl = nullptr;
   HIP_TRY(upload(&P->d_active, act));
   if (lst.empty()) lst.push_back(0);
@@ -3455,7 +3546,7 @@ static int load_values(dcx_plan* P, const double* values, hipStream_t st) {
   if (P->inj_batch >= 0) {
     if (!P->d_fvals) {
       if (int rc = refuse_in_capture(st)) return rc;
-      HIP_TRY(hipMalloc(&P->d_fvals, std::max<size_t>(vb, 256)));
+      HIP_TRY(dev_malloc(&P->d_fvals, std::max<size_t>(vb, 256)));
     }
     hipLaunchKernelGGL(k_scale_vals, dim3(grid_stride(static_cast<int64_t>(P->nbatch) * P->S.nnz)),
                        dim3(256), 0, st, P->d_vals, P->d_fvals, static_cast<int64_t>(P->S.nnz),
@@ -3482,7 +3573,7 @@ static int launch_check(dcx_plan* P, const double* B, int64_t b_bs, const void* 
     // partial sums (nb x nblk x 4, nblk at its largest: 64 lanes per row)
     // followed by the totals (nb x 4)
     const size_t mx = grid1(S.n, 4 * kChkRows);
-    HIP_TRY(hipMalloc(&P->d_chk, sizeof(double) * 4 * nb * (mx + 1)));
+    HIP_TRY(dev_malloc(&P->d_chk, sizeof(double) * 4 * nb * (mx + 1)));
     HIP_TRY(hipHostMalloc(&P->h_chk, (sizeof(double) * 4 + sizeof(int)) * nb));
     P->chk_tot_off = static_cast<size_t>(4) * nb * mx;
   }
@@ -3923,7 +4014,7 @@ struct dcx_lmap {
     void* ptrs[] = {d_ky2, d_grp, d_ptr, d_cell, d_le, d_ln, d_bptr, d_bcell, d_bc_e, d_bcoef,
                     d_tptr, d_tent, d_te, d_tbptr, d_tbq, d_ge, d_gn, d_jws, d_tle, d_tln};
     for (void* p : ptrs)
-      if (p) (void)hipFree(p);
+      if (p) (void)dev_free(p);
   }
 };
 
@@ -4014,8 +4105,8 @@ int dcx_lmap_create(int nnz, int ncell, int nky, const double* ky2, const int32_
     HIP_TRY(upload(&M->d_tent, tent));
     HIP_TRY(upload(&M->d_te, te));
     HIP_TRY(upload(&M->d_tbptr, tbptr));
-    HIP_TRY(hipMalloc(&M->d_ge, std::max<size_t>(8, nnz * sizeof(double))));
-    HIP_TRY(hipMalloc(&M->d_gn, std::max<size_t>(8, nnz * sizeof(double))));
+    HIP_TRY(dev_malloc(&M->d_ge, std::max<size_t>(8, nnz * sizeof(double))));
+    HIP_TRY(dev_malloc(&M->d_gn, std::max<size_t>(8, nnz * sizeof(double))));
     *out = M.release();
     return DCX_OK;
   } catch (const std::exception& e) {
@@ -4035,8 +4126,8 @@ int dcx_lmap_set_groups(dcx_lmap* M, int ngroup, const int32_t* group, const dou
       if (group[e] < 0 || group[e] >= ngroup) return fail(DCX_ERR_ARG, "group out of range");
     HIP_TRY(hipSetDevice(M->device));
     HIP_TRY(hipDeviceSynchronize());  // earlier launches may still read the old table
-    if (M->d_ky2) HIP_TRY(hipFree(M->d_ky2));
-    if (M->d_grp) HIP_TRY(hipFree(M->d_grp));
+    if (M->d_ky2) HIP_TRY(dev_free(M->d_ky2));
+    if (M->d_grp) HIP_TRY(dev_free(M->d_grp));
     M->d_ky2 = nullptr;
     M->d_grp = nullptr;
     HIP_TRY(upload(&M->d_ky2,
@@ -4186,10 +4277,10 @@ int dcx_jacobian_rows(dcx_lmap* M, dcx_plan* P, const double* Y, int r, const do
   HIP_TRY(hipSetDevice(M->device));
   if (M->jws_bytes < need) {
     HIP_TRY(hipStreamSynchronize(st));
-    if (M->d_jws) HIP_TRY(hipFree(M->d_jws));
+    if (M->d_jws) HIP_TRY(dev_free(M->d_jws));
     M->d_jws = nullptr;
     M->jws_bytes = 0;
-    HIP_TRY(hipMalloc(&M->d_jws, need));
+    HIP_TRY(dev_malloc(&M->d_jws, need));
     M->jws_bytes = need;
   }
   jacobian_rows_launch(M, P, Y, r, U, nS, src, wq, sigma, J, flags, M->d_jws, st);
@@ -4237,7 +4328,7 @@ struct dcx_survey {
     void* ptrs[] = {d_ptr, d_rows, d_rows_p, d_sidx, d_val, d_tptr, d_tdat, d_tcoef,
                     d_trow, d_trow_p, d_tsrc};
     for (void* p : ptrs)
-      if (p) (void)hipFree(p);
+      if (p) (void)dev_free(p);
   }
 };
 

@@ -625,3 +625,27 @@ def test_eval_grad_device_equals_autograd_3d():
         phi.backward()
         phi2, g2 = eng.eval_grad_device(torch.tensor(m, device=DEV), dobs_t, w_t)
         assert torch.equal(phi.detach(), phi2) and torch.equal(x.grad, g2)
+
+
+def test_plan_memory_through_torch_allocator():
+    """The library's device memory comes from torch's caching allocator
+    (dcx_set_allocator, registered by the ctypes layer): a plan's factor and
+    workspaces show in torch.cuda.memory_allocated while it lives and go back
+    when it is freed (the reference's cuDSS memory sat outside torch,
+    dctorch/simulation3d.py:160-169)."""
+    A, b = laplacian_fixture(nx=60, n_rhs=16)
+    torch.cuda.synchronize()
+    before = torch.cuda.memory_allocated()
+    slv = _solver(A, 16, nbatch=4, precision="mixed")
+    slv.factorize(torch.tensor(np.stack([A.data] * 4), device=DEV))
+    x = slv.solve(torch.tensor(b, device=DEV))
+    torch.cuda.synchronize()
+    held = torch.cuda.memory_allocated() - before - x.numel() * x.element_size()
+    slv_bytes = slv.info()["device_bytes"]
+    _say(f"plan memory through torch: {held} bytes (plan reports {slv_bytes})")
+    assert held >= slv_bytes
+    assert rel(x[0].cpu().numpy(), spla.splu(A.tocsc()).solve(b)) < 1e-5
+    mid = torch.cuda.memory_allocated()
+    slv.free()
+    torch.cuda.synchronize()
+    assert mid - torch.cuda.memory_allocated() >= slv_bytes
