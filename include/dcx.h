@@ -50,9 +50,17 @@ enum {
   DCX_SPARSE_RHS = 4,  /* b is zero outside the rows set by dcx_set_rhs_support */
   DCX_F32_FIELDS = 8,  /* the field blocks (dcx_solve's x, dcx_grad_values' y and
                           x, dcx_survey_dpred's U) are float, not double */
-  DCX_CHECK_DEFER = 16 /* the solve's check is launched but not waited for: its
+  DCX_CHECK_DEFER = 16, /* the solve's check is launched but not waited for: its
                           verdict (and any refinement) comes from
                           dcx_plan_check_wait; b and x must stay valid until then */
+  DCX_SPLIT_BWD = 32   /* with DCX_SPARSE_RHS: the backward sweep computes the
+                          supernodes whose subtree holds a support row on the call's
+                          stream, the others on the plan's second stream after them;
+                          x is final on the support rows (and their supernodes) when
+                          the call's stream gets there, on every row only after
+                          dcx_plan_join.  Ignored (one stream) unless x is the
+                          sweeps' own block: FP64 with DCX_PERMUTED, or DCX_F32_FIELDS
+                          without fixed refinement passes */
 };
 
 typedef struct {
@@ -169,12 +177,20 @@ int dcx_plan_set_refinement(dcx_plan* plan, int passes);
  * check followed by a fallback (dctorch/optimization.py:753-761); cuDSS
  * reports pivot failures itself. */
 int dcx_plan_set_check(dcx_plan* plan, double tol, int max_passes, int sample_cols);
-/* Verdict of a deferred check (DCX_CHECK_DEFER): waits for it, runs the
- * refinement passes a failing FP32 solve gets, and returns DCX_OK or
+/* Verdict of the oldest pending deferred check (DCX_CHECK_DEFER; at most two
+ * are pending, a third solve completes the oldest first): waits for it, runs
+ * the refinement passes a failing FP32 solve gets, and returns DCX_OK or
  * DCX_ERR_NUMERIC as the solve itself would have; DCX_OK when nothing is
- * pending.  The next solve or factorisation of the plan completes a pending
- * check first.  No reference counterpart (see dcx_plan_set_check). */
+ * pending.  A factorisation of the plan (and a solve of an FP32 plan with
+ * fixed refinement passes) completes the pending checks first.  No reference
+ * counterpart (see dcx_plan_set_check). */
 int dcx_plan_check_wait(dcx_plan* plan);
+/* Orders `stream` after the second half of the last split backward sweep
+ * (DCX_SPLIT_BWD) of the plan, so every row of its x may be read there; no
+ * host synchronisation.  No reference counterpart: the work split is this
+ * library's (dpred needs the fields on the receivers only, which the first
+ * half gives; the rest overlaps the misfit and the adjoint solve). */
+int dcx_plan_join(dcx_plan* plan, void* stream);
 /* Result of the last check (host arrays of nbatch, each may be NULL): rho
  * (the tested relative residual), eta (the normwise backward error
  * |R| / (|A| |X| + |B|), for information), pivot_fail; refinement passes run;

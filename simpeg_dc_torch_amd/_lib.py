@@ -14,7 +14,8 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("DCX_LIBRARY") or os.path.join(_HERE, "libdcx.so")
 
 DCX_FP64, DCX_FP32 = 0, 1
-DCX_SHARED_RHS, DCX_PERMUTED, DCX_SPARSE_RHS, DCX_F32_FIELDS, DCX_CHECK_DEFER = 1, 2, 4, 8, 16
+DCX_SHARED_RHS, DCX_PERMUTED, DCX_SPARSE_RHS, DCX_F32_FIELDS = 1, 2, 4, 8
+DCX_CHECK_DEFER, DCX_SPLIT_BWD = 16, 32
 DCX_TUNE_SWEEP_TC, DCX_TUNE_SPLITK_BELOW, DCX_TUNE_WIDE_BELOW, DCX_TUNE_PANEL_WIDTH = 1, 2, 3, 4
 
 _i32p = C.POINTER(C.c_int32)
@@ -68,6 +69,7 @@ PROTOTYPES = {
                                      C.POINTER(C.c_int)]),
     "dcx_plan_inject_fault": (C.c_int, [_vp, C.c_int, C.c_double]),
     "dcx_plan_check_wait": (C.c_int, [_vp]),
+    "dcx_plan_join": (C.c_int, [_vp, _vp]),
     "dcx_plan_set_timing": (C.c_int, [_vp, C.c_int]),
     "dcx_plan_set_tuning": (C.c_int, [_vp, C.c_int, C.c_int]),
     "dcx_plan_get_tuning": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_int)]),

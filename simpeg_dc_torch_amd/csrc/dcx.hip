@@ -2283,6 +2283,21 @@ inline int grid_stride(int64_t n) {
 // ===========================================================================
 // plan
 // ===========================================================================
+// buffers and context of one solve check: the device partial sums and totals,
+// their pinned host copy (nbatch x 4 sums, then nbatch pivot flags), the
+// events of a deferred check, and the solve it belongs to
+struct CheckSlot {
+  double* d_chk = nullptr;
+  void* h_chk = nullptr;
+  hipEvent_t ev = nullptr;   // the results are on the host
+  hipEvent_t src = nullptr;  // the solve is done (the check stream waits on it)
+  const double* B = nullptr;
+  int64_t bbs = 0;
+  void* xv = nullptr;
+  int k = 0, flags = 0;
+  hipStream_t st = nullptr;
+};
+
 struct dcx_plan {
   dcx::Symbolic S;
   int device = 0;
@@ -2313,25 +2328,23 @@ struct dcx_plan {
   double chk_tol = 0.0;     // <= 0: off
   int chk_passes = 2;       // refinement passes a failing check may add (FP32 plans)
   int chk_cols = 8;         // sample columns (<= 64)
-  double* d_chk = nullptr;  // nbatch x 4 sums of k_check
-  void* h_chk = nullptr;    // pinned host: nbatch x 4 sums, then nbatch info ints
+  // check buffers: slots 0 and 1 hold up to two deferred checks (a FIFO),
+  // slot 2 the immediate one
+  CheckSlot slot[3];
+  int slot_head = 0, slot_count = 0;
   std::vector<double> chk_eta;   // normwise backward error |R| / (|A| |X| + |B|)
   std::vector<double> chk_rho;   // relative residual |R| / |B| (the test)
   std::vector<int> chk_pivot;
   int chk_passes_done = 0;
   int chk_status = 0;       // 0 not run, 1 passed, 2 failed, 3 skipped (stream capture)
-  // a deferred check (DCX_CHECK_DEFER): launched, its verdict read by
-  // dcx_plan_check_wait; the solve it belongs to
-  bool chk_pending = false;
-  hipEvent_t chk_ev = nullptr;     // the check's results are on the host
-  hipEvent_t chk_src = nullptr;    // the solve is done (the check stream waits on it)
   hipStream_t chk_st = nullptr;    // stream of deferred checks (default priority)
-  size_t chk_tot_off = 0;          // offset of the totals in d_chk
-  const double* pend_B = nullptr;
-  int64_t pend_bbs = 0;
-  void* pend_xv = nullptr;
-  int pend_k = 0, pend_flags = 0;
-  hipStream_t pend_st = nullptr;
+  size_t chk_tot_off = 0;          // offset of the totals in a slot's d_chk
+  // split backward sweep (DCX_SPLIT_BWD): the supernodes outside the support
+  // run on the side stream; rest_ev marks their end
+  int* d_inact_lvl = nullptr;
+  std::vector<int> inact_lvl_ptr;
+  hipEvent_t rest_ev = nullptr, split_ev = nullptr;
+  bool rest_pending = false;
   // fault injection (tests): the factorisation reads d_fvals, in which batch
   // member inj_batch is scaled by 1 + inj_rel
   int inj_batch = -1;
@@ -2389,17 +2402,23 @@ struct dcx_plan {
                     d_front[0], d_front[1], d_vals, d_active, d_act_lvl, d_upd[0],
                     d_upd[1], d_w, d_r, d_b, d_x, d_fz, d_flv, d_fmem, d_fgptr, d_fgb,
                     d_toff, d_lridx, d_ulvl, d_ufz, d_panelsT, d_fs_off, d_fs, d_dg,
-                    d_chk, d_fvals};
+                    d_fvals, d_inact_lvl};
     for (void* p : ptrs)
       if (p) (void)dev_free(p);
-    if (h_chk) (void)hipHostFree(h_chk);
-    if (chk_ev) (void)hipEventDestroy(chk_ev);
-    if (chk_src) (void)hipEventDestroy(chk_src);
+    for (auto& c : slot) {
+      if (c.d_chk) (void)dev_free(c.d_chk);
+      if (c.h_chk) (void)hipHostFree(c.h_chk);
+      if (c.ev) (void)hipEventDestroy(c.ev);
+      if (c.src) (void)hipEventDestroy(c.src);
+    }
+    if (rest_ev) (void)hipEventDestroy(rest_ev);
+    if (split_ev) (void)hipEventDestroy(split_ev);
     if (chk_st) (void)hipStreamDestroy(chk_st);
   }
 };
 
 static int complete_pending(dcx_plan* P);
+static int complete_all(dcx_plan* P);
 
 namespace {
 
@@ -2951,15 +2970,20 @@ int fwd_big(dcx_plan* P, T* W, int k, int d, const int* lvl, int cnt, int maxnf,
 }
 
 // one level of one sweep direction (sparse forward: active supernodes only)
+// subset (backward sweep only): 0 every supernode of the level, 1 those of
+// the right-hand side's support, 2 the others
 template <typename T>
 int sweep_level(dcx_plan* P, T* W, int k, bool fwd, int d, const int* act, const SweepIO& io,
-                hipStream_t st) {
+                hipStream_t st, int subset = 0) {
   const auto& S = P->S;
   const int* lvl;
   int cnt;
-  if (fwd && act) {
+  if ((fwd && act) || subset == 1) {
     lvl = P->d_act_lvl + P->act_lvl_ptr[d];
     cnt = P->act_lvl_ptr[d + 1] - P->act_lvl_ptr[d];
+  } else if (subset == 2) {
+    lvl = P->d_inact_lvl + P->inact_lvl_ptr[d];
+    cnt = P->inact_lvl_ptr[d + 1] - P->inact_lvl_ptr[d];
   } else {
     lvl = P->d_ulvl + S.ulvl_ptr[d];
     cnt = S.ulvl_ptr[d + 1] - S.ulvl_ptr[d];
@@ -2990,7 +3014,7 @@ int sweep_level(dcx_plan* P, T* W, int k, bool fwd, int d, const int* act, const
 // run (overlapped with the factorisation), only the backward one remains
 template <typename T>
 int sweeps(dcx_plan* P, T* W, int k, bool sparse_rhs, const SweepIO& io, hipStream_t st,
-           bool skip_fwd = false) {
+           bool skip_fwd = false, bool split = false) {
   const auto& S = P->S;
   int rc = ensure_upd<T>(P, k, st);
   if (rc) return rc;
@@ -3012,10 +3036,23 @@ int sweeps(dcx_plan* P, T* W, int k, bool sparse_rhs, const SweepIO& io, hipStre
       rc = launch_fused<T>(P, W, k, true, act, io, st);
       if (rc) return rc;
     }
+    const bool sp = split && !fwd;
     for (int dd = 0; dd < S.ndepth; ++dd) {
       const int d = fwd ? S.ndepth - 1 - dd : dd;
-      rc = sweep_level<T>(P, W, k, fwd, d, act, io, st);
+      rc = sweep_level<T>(P, W, k, fwd, d, act, io, st, sp ? 1 : 0);
       if (rc) return rc;
+    }
+    if (sp) {
+      // the supernodes outside the support, root first, on the side stream
+      // after the support's ones; rest_ev marks their end (dcx_plan_join)
+      HIP_TRY(hipEventRecord(P->split_ev, st));
+      HIP_TRY(hipStreamWaitEvent(P->side, P->split_ev, 0));
+      for (int d = 0; d < S.ndepth; ++d) {
+        rc = sweep_level<T>(P, W, k, false, d, act, io, P->side, 2);
+        if (rc) return rc;
+      }
+      HIP_TRY(hipEventRecord(P->rest_ev, P->side));
+      P->rest_pending = true;
     }
     if (!fwd && S.nfused) {
       rc = launch_fused<T>(P, W, k, false, act, io, st);
@@ -3033,6 +3070,8 @@ int ensure_side(dcx_plan* P) {
   HIP_TRY(hipStreamCreateWithFlags(&P->side, hipStreamNonBlocking));
   P->lev_ev.assign(P->S.ndepth + 2, nullptr);
   for (auto& e : P->lev_ev) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  HIP_TRY(hipEventCreateWithFlags(&P->rest_ev, hipEventDisableTiming));
+  HIP_TRY(hipEventCreateWithFlags(&P->split_ev, hipEventDisableTiming));
   return DCX_OK;
 }
 
@@ -3044,7 +3083,7 @@ int ensure_side(dcx_plan* P) {
 // plain sequence.
 template <typename T>
 int factor_and_sweeps(dcx_plan* P, T* W, int k, bool sparse_rhs, const SweepIO& io,
-                      hipStream_t st) {
+                      hipStream_t st, bool split = false) {
   const auto& S = P->S;
   if (S.nfused) {
     int rc = launch_factor<T>(P, st);
@@ -3069,7 +3108,7 @@ int factor_and_sweeps(dcx_plan* P, T* W, int k, bool sparse_rhs, const SweepIO& 
   }
   HIP_TRY(hipEventRecord(join, P->side));
   HIP_TRY(hipStreamWaitEvent(st, join, 0));
-  return sweeps<T>(P, W, k, sparse_rhs, io, st, true);
+  return sweeps<T>(P, W, k, sparse_rhs, io, st, true, split);
 }
 
 void launch_permute_d2T(dcx_plan* P, const double* in, void* out, int k, int nblocks,
@@ -3355,6 +3394,13 @@ int dcx_plan_get_check(const dcx_plan* P, double* rho, double* eta, int* pivot_f
   return DCX_OK;
 }
 
+int dcx_plan_join(dcx_plan* P, void* stream) {
+  if (!P) return fail(DCX_ERR_ARG, "null plan");
+  if (P->rest_pending)
+    HIP_TRY(hipStreamWaitEvent(static_cast<hipStream_t>(stream), P->rest_ev, 0));
+  return DCX_OK;
+}
+
 int dcx_plan_check_wait(dcx_plan* P) {
   if (!P) return fail(DCX_ERR_ARG, "null plan");
   HIP_TRY(hipSetDevice(P->device));
@@ -3520,19 +3566,25 @@ int dcx_set_rhs_support(dcx_plan* P, const int32_t* rows, int nrows) {
     if (rows[i] < 0 || rows[i] >= S.n) return fail(DCX_ERR_ARG, "support row out of range");
     for (int s = S.snode_of[S.iperm[rows[i]]]; s >= 0 && !act[s]; s = S.parent[s]) act[s] = 1;
   }
-  std::vector<int> ptr(S.ndepth + 1, 0), lst;
+  std::vector<int> ptr(S.ndepth + 1, 0), lst, iptr(S.ndepth + 1, 0), ilst;
   for (int d = 0; d < S.ndepth; ++d) {
     for (int q = S.ulvl_ptr[d]; q < S.ulvl_ptr[d + 1]; ++q)
-      if (act[S.ulvl_sup[q]]) lst.push_back(S.ulvl_sup[q]);
+      (act[S.ulvl_sup[q]] ? lst : ilst).push_back(S.ulvl_sup[q]);
     ptr[d + 1] = static_cast<int>(lst.size());
+    iptr[d + 1] = static_cast<int>(ilst.size());
   }
+  HIP_TRY(hipDeviceSynchronize());  // the old tables may still be read
   if (P->d_active) HIP_TRY(dev_free(P->d_active));
   if (P->d_act_lvl) HIP_TRY(dev_free(P->d_act_lvl));
-  P->d_active = P->d_act_lvl = nullptr;
+  if (P->d_inact_lvl) HIP_TRY(dev_free(P->d_inact_lvl));
+  P->d_active = P->d_act_lvl = P->d_inact_lvl = nullptr;
   HIP_TRY(upload(&P->d_active, act));
   if (lst.empty()) lst.push_back(0);
   HIP_TRY(upload(&P->d_act_lvl, lst));
+  if (ilst.empty()) ilst.push_back(0);
+  HIP_TRY(upload(&P->d_inact_lvl, ilst));
   P->act_lvl_ptr = ptr;
+  P->inact_lvl_ptr = iptr;
   return DCX_OK;
 }
 
@@ -3540,6 +3592,10 @@ namespace {
 // copy the caller's values, clear the pivot flags, and prepare the values the
 // factorisation reads (a fault-injected copy when one is armed)
 static int load_values(dcx_plan* P, const double* values, hipStream_t st) {
+  if (P->rest_pending) {  // the rest of a split sweep still reads the factor
+    HIP_TRY(hipStreamWaitEvent(st, P->rest_ev, 0));
+    P->rest_pending = false;
+  }
   const size_t vb = static_cast<size_t>(P->nbatch) * P->S.nnz * sizeof(double);
   HIP_TRY(hipMemcpyAsync(P->d_vals, values, vb, hipMemcpyDeviceToDevice, st));
   hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(256), 0, st, P->d_info, P->nbatch);
@@ -3562,23 +3618,23 @@ static int load_values(dcx_plan* P, const double* values, hipStream_t st) {
 // synchronisation.  eta_b = |R| / (|A_b| |X| + |B|) (Frobenius norms over the
 // sample) for every batch member; returns the number failing (eta > tol or a
 // pivot that was not positive).
-static int launch_check(dcx_plan* P, const double* B, int64_t b_bs, const void* X, bool xf32,
-                        bool permuted, int k, hipStream_t st) {
+static int launch_check(dcx_plan* P, CheckSlot& C, const double* B, int64_t b_bs,
+                        const void* X, bool xf32, bool permuted, int k, hipStream_t st) {
   const auto& S = P->S;
   const int nb = P->nbatch;
   const int cs = std::min(k, std::min(P->chk_cols, 64));
   const int lpr = cs > 32 ? 64 : cs > 16 ? 32 : cs > 8 ? 16 : 8;
   const int nblk = grid1(S.n, (256 / lpr) * kChkRows);
-  if (!P->d_chk) {
+  if (!C.d_chk) {
     // partial sums (nb x nblk x 4, nblk at its largest: 64 lanes per row)
     // followed by the totals (nb x 4)
     const size_t mx = grid1(S.n, 4 * kChkRows);
-    HIP_TRY(dev_malloc(&P->d_chk, sizeof(double) * 4 * nb * (mx + 1)));
-    HIP_TRY(hipHostMalloc(&P->h_chk, (sizeof(double) * 4 + sizeof(int)) * nb));
+    HIP_TRY(dev_malloc(&C.d_chk, sizeof(double) * 4 * nb * (mx + 1)));
+    HIP_TRY(hipHostMalloc(&C.h_chk, (sizeof(double) * 4 + sizeof(int)) * nb));
     P->chk_tot_off = static_cast<size_t>(4) * nb * mx;
   }
-  double* part = P->d_chk;
-  double* tot = P->d_chk + P->chk_tot_off;
+  double* part = C.d_chk;
+  double* tot = C.d_chk + P->chk_tot_off;
   const int* rp = permuted ? P->d_pcrow : P->d_crow;
   const int* cl = permuted ? P->d_pcol : P->d_col;
   const int* vi = permuted ? P->d_pvidx : nullptr;
@@ -3603,7 +3659,7 @@ static int launch_check(dcx_plan* P, const double* B, int64_t b_bs, const void* 
   if (xf32) pick(0.0f, Xf); else pick(0.0, Xd);
   hipLaunchKernelGGL(k_check_sum, dim3(nb), dim3(256), 0, st, part, nblk, tot);
   HIP_TRY(hipGetLastError());
-  double* hs = static_cast<double*>(P->h_chk);
+  double* hs = static_cast<double*>(C.h_chk);
   int* hi = reinterpret_cast<int*>(hs + 4 * nb);
   HIP_TRY(hipMemcpyAsync(hs, tot, sizeof(double) * 4 * nb, hipMemcpyDeviceToHost, st));
   HIP_TRY(hipMemcpyAsync(hi, P->d_info, sizeof(int) * nb, hipMemcpyDeviceToHost, st));
@@ -3612,9 +3668,9 @@ static int launch_check(dcx_plan* P, const double* B, int64_t b_bs, const void* 
 
 // the check's verdict from the host copies (after the stream or the check
 // event has been waited on); returns the number of failing systems
-static int eval_check(dcx_plan* P) {
+static int eval_check(dcx_plan* P, const CheckSlot& C) {
   const int nb = P->nbatch;
-  const double* hs = static_cast<const double*>(P->h_chk);
+  const double* hs = static_cast<const double*>(C.h_chk);
   const int* hi = reinterpret_cast<const int*>(hs + 4 * nb);
   P->chk_eta.assign(nb, 0.0);
   P->chk_rho.assign(nb, 0.0);
@@ -3639,9 +3695,9 @@ static int eval_check(dcx_plan* P) {
 // status (HIP error)
 static int run_check(dcx_plan* P, const double* B, int64_t b_bs, const void* X, bool xf32,
                      bool permuted, int k, hipStream_t st) {
-  if (int rc = launch_check(P, B, b_bs, X, xf32, permuted, k, st)) return rc;
+  if (int rc = launch_check(P, P->slot[2], B, b_bs, X, xf32, permuted, k, st)) return rc;
   HIP_TRY(hipStreamSynchronize(st));
-  return eval_check(P);
+  return eval_check(P, P->slot[2]);
 }
 
 static std::string check_message(const dcx_plan* P) {
@@ -3661,7 +3717,7 @@ int dcx_factorize(dcx_plan* P, const double* values, void* stream) {
   if (!P || !values) return fail(DCX_ERR_ARG, "null argument");
   hipStream_t st = static_cast<hipStream_t>(stream);
   HIP_TRY(hipSetDevice(P->device));
-  if (int rc0 = complete_pending(P)) return rc0;
+  if (int rc0 = complete_all(P)) return rc0;
   if (int rc = load_values(P, values, st)) return rc;
   int rc = P->precision == DCX_FP32 ? launch_factor<float>(P, st) : launch_factor<double>(P, st);
   if (rc) return rc;
@@ -3730,6 +3786,7 @@ static int finish_check(dcx_plan* P, int bad, const double* B, int64_t bbs, void
                                        std::to_string(P->chk_passes_done) +
                                        " refinement pass(es):" + check_message(P));
     }
+    if (P->rest_pending) HIP_TRY(hipStreamWaitEvent(st, P->rest_ev, 0));  // all rows of x
     if (f32x && !x) {  // first pass on FP32 fields: an FP64 iterate from them
       int rc = ensure(P, reinterpret_cast<void**>(&P->d_x), &P->x_bytes,
                       static_cast<size_t>(nb) * blk * 8, st);
@@ -3755,26 +3812,32 @@ static int start_check(dcx_plan* P, const double* B, int64_t bbs, void* xv, int 
   const bool permuted = flags & DCX_PERMUTED, f32x = (flags & DCX_F32_FIELDS) != 0;
   P->chk_status = 0;
   if (flags & DCX_CHECK_DEFER) {
-    // on the plan's check stream, after the solve: the work the caller
-    // queues next on `st` does not wait for it
-    if (!P->chk_st) {
-      HIP_TRY(hipStreamCreateWithFlags(&P->chk_st, hipStreamNonBlocking));
-      HIP_TRY(hipEventCreateWithFlags(&P->chk_ev, hipEventDisableTiming));
-      HIP_TRY(hipEventCreateWithFlags(&P->chk_src, hipEventDisableTiming));
+    // on the plan's check stream, after the solve (and the rest of a split
+    // backward sweep): the work the caller queues next on `st` does not wait
+    // for it.  At most two checks are pending; a third completes the oldest.
+    if (P->slot_count == 2)
+      if (int rc = complete_pending(P)) return rc;
+    if (!P->chk_st) HIP_TRY(hipStreamCreateWithFlags(&P->chk_st, hipStreamNonBlocking));
+    CheckSlot& C = P->slot[(P->slot_head + P->slot_count) % 2];
+    if (!C.ev) {
+      HIP_TRY(hipEventCreateWithFlags(&C.ev, hipEventDisableTiming));
+      HIP_TRY(hipEventCreateWithFlags(&C.src, hipEventDisableTiming));
     }
-    HIP_TRY(hipEventRecord(P->chk_src, st));
-    HIP_TRY(hipStreamWaitEvent(P->chk_st, P->chk_src, 0));
-    if (int rc = launch_check(P, B, bbs, xv, f32x, permuted, k, P->chk_st)) return rc;
-    HIP_TRY(hipEventRecord(P->chk_ev, P->chk_st));
-    P->chk_pending = true;
-    P->pend_B = B;
-    P->pend_bbs = bbs;
-    P->pend_xv = xv;
-    P->pend_k = k;
-    P->pend_flags = flags;
-    P->pend_st = st;
+    HIP_TRY(hipEventRecord(C.src, st));
+    HIP_TRY(hipStreamWaitEvent(P->chk_st, C.src, 0));
+    if (P->rest_pending) HIP_TRY(hipStreamWaitEvent(P->chk_st, P->rest_ev, 0));
+    if (int rc = launch_check(P, C, B, bbs, xv, f32x, permuted, k, P->chk_st)) return rc;
+    HIP_TRY(hipEventRecord(C.ev, P->chk_st));
+    C.B = B;
+    C.bbs = bbs;
+    C.xv = xv;
+    C.k = k;
+    C.flags = flags;
+    C.st = st;
+    ++P->slot_count;
     return DCX_OK;
   }
+  if (P->rest_pending) HIP_TRY(hipStreamWaitEvent(st, P->rest_ev, 0));
   return finish_check(P, run_check(P, B, bbs, xv, f32x, permuted, k, st), B, bbs, xv, k, flags,
                       st);
 }
@@ -3797,9 +3860,19 @@ static int solve_impl(dcx_plan* P, const double* b, void* xv, int k, int flags, 
     return fail(DCX_ERR_ARG, "DCX_F32_FIELDS needs an FP32 plan and DCX_PERMUTED");
   if (f32x && static_cast<const void*>(b) == xv)
     return fail(DCX_ERR_ARG, "b and a float x must not alias");
-  if (int rc0 = complete_pending(P)) return rc0;
+  // a pending check's refinement would find the FP64 iterate of fixed passes
+  // overwritten: with fixed passes, complete them first
+  if (P->refine > 0 && P->precision == DCX_FP32)
+    if (int rc0 = complete_all(P)) return rc0;
   const bool check = P->chk_tol > 0.0 && !capturing(st);
   P->chk_status = check ? 0 : 3;
+  // split backward sweep: only where it is safe and useful (see DCX_SPLIT_BWD):
+  // a sparse right-hand side, the caller's block as the working block (FP64,
+  // or FP32 fields without fixed refinement), no fused subtrees, no
+  // row-grouped backward levels (their scratch is shared), no sweep timing
+  const bool split = (flags & DCX_SPLIT_BWD) && sparse && P->d_inact_lvl && !S.nfused &&
+                     S.max_ns <= 128 && !P->timing &&
+                     (P->precision == DCX_FP64 ? permuted : (f32x && P->refine == 0));
   double* x = f32x ? nullptr : static_cast<double*>(xv);
   const int64_t blk = static_cast<int64_t>(S.n) * k;
   const int n = S.n;
@@ -3828,8 +3901,9 @@ static int solve_impl(dcx_plan* P, const double* b, void* xv, int k, int flags, 
       io.X64 = x;
       io.xmode = 1;
     }
-    rc = with_factor ? factor_and_sweeps<double>(P, W, k, sparse, io, st)
-                     : sweeps<double>(P, W, k, sparse, io, st);
+    if (split && (rc = ensure_side(P))) return rc;
+    rc = with_factor ? factor_and_sweeps<double>(P, W, k, sparse, io, st, split)
+                     : sweeps<double>(P, W, k, sparse, io, st, false, split);
     if (rc || !check) return rc;
     return start_check(P, Borig, Bbs, xv, k, flags, st);
   }
@@ -3851,8 +3925,9 @@ static int solve_impl(dcx_plan* P, const double* b, void* xv, int k, int flags, 
   }
   io.X64 = x;
   io.xmode = x ? 1 : 0;
-  rc = with_factor ? factor_and_sweeps<float>(P, W, k, sparse, io, st)
-                   : sweeps<float>(P, W, k, sparse, io, st);
+  if (split && (rc = ensure_side(P))) return rc;
+  rc = with_factor ? factor_and_sweeps<float>(P, W, k, sparse, io, st, split)
+                   : sweeps<float>(P, W, k, sparse, io, st, false, split);
   if (rc) return rc;
   for (int it = 0; it < P->refine; ++it) {
     rc = mixed_refine_pass(P, Borig, Bbs, x, W, k, permuted, st);
@@ -3867,12 +3942,25 @@ static int solve_impl(dcx_plan* P, const double* b, void* xv, int k, int flags, 
 }
 
 // complete a deferred check (its solve's verdict, refinement, status)
+static int complete_all(dcx_plan* P) {
+  while (P->slot_count > 0)
+    if (int rc = complete_pending(P)) {
+      while (P->slot_count > 0) {  // drop the rest: their solves are superseded
+        P->slot_head = (P->slot_head + 1) % 2;
+        --P->slot_count;
+      }
+      return rc;
+    }
+  return DCX_OK;
+}
+
 static int complete_pending(dcx_plan* P) {
-  if (!P->chk_pending) return DCX_OK;
-  P->chk_pending = false;
-  HIP_TRY(hipEventSynchronize(P->chk_ev));
-  return finish_check(P, eval_check(P), P->pend_B, P->pend_bbs, P->pend_xv, P->pend_k,
-                      P->pend_flags, P->pend_st);
+  if (P->slot_count == 0) return DCX_OK;
+  CheckSlot& C = P->slot[P->slot_head];
+  P->slot_head = (P->slot_head + 1) % 2;
+  --P->slot_count;
+  HIP_TRY(hipEventSynchronize(C.ev));
+  return finish_check(P, eval_check(P, C), C.B, C.bbs, C.xv, C.k, C.flags, C.st);
 }
 
 int dcx_solve(dcx_plan* P, const double* b, void* xv, int k, int flags, void* stream) {
@@ -3890,7 +3978,7 @@ int dcx_factor_solve(dcx_plan* P, const double* values, const double* b, void* x
   if (k < 1) return fail(DCX_ERR_ARG, "k must be >= 1");
   hipStream_t st = static_cast<hipStream_t>(stream);
   HIP_TRY(hipSetDevice(P->device));
-  if (int rc0 = complete_pending(P)) return rc0;
+  if (int rc0 = complete_all(P)) return rc0;
   if (int rc = load_values(P, values, st)) return rc;
   P->factored = true;
   return solve_impl(P, b, xv, k, flags, st, true);

@@ -144,14 +144,14 @@ class _GpuState:
             sv.run_solve(b, x, k, flags, defer=defer)
         return x
 
-    def factor_solve_perm(self, vals, b, k, f32=False, defer=False):
+    def factor_solve_perm(self, vals, b, k, f32=False, defer=False, split=False):
         """Factor every wavenumber and solve the shared, sparse right-hand
         sides b in elimination order, the two overlapped (dcx_factor_solve)."""
         sv = self.solver
         x = torch.empty(sv.nbatch, sv.n, k, dtype=torch.float32 if f32 else torch.float64,
                         device=b.device)
         flags = _lib.DCX_PERMUTED | _lib.DCX_SHARED_RHS | _lib.DCX_SPARSE_RHS | \
-            (_lib.DCX_F32_FIELDS if f32 else 0)
+            (_lib.DCX_F32_FIELDS if f32 else 0) | (_lib.DCX_SPLIT_BWD if split else 0)
         v = vals.contiguous()
         with sv._lock:
             sv.run_factor_solve(v, b, x, k, flags, defer=defer)
@@ -290,7 +290,12 @@ def _misfit_body(eng, m_full, dobs, w, want_grad):
         if sv.check_wait():
             d, r, gv = rest()
     else:
-        U = g.factor_solve_perm(vals, g.q_perm, eng.nS, f32=g.fields32, defer=True)
+        # the forward solve's backward sweep is split (DCX_SPLIT_BWD): the
+        # receivers' supernodes first, which is all the projection reads; the
+        # rest overlaps the misfit and the adjoint solve and is joined before
+        # the gather.  Both checks are waited for after the gather is queued.
+        U = g.factor_solve_perm(vals, g.q_perm, eng.nS, f32=g.fields32, defer=True,
+                                split=want_grad)
         d = torch.empty(eng.nD, dtype=torch.float64, device=eng.device)
 
         def project():
@@ -299,14 +304,23 @@ def _misfit_body(eng, m_full, dobs, w, want_grad):
             G = g.adjoint_rhs(((2.0 * r) * w).contiguous(), eng.nS, 0, eng.nS) \
                 if want_grad else None
             return r, G
-        r, G = project()
-        if sv.check_wait():
-            r, G = project()
-        gv = None
-        if want_grad:
+
+        def adjoint(G):
             Lam = g.solve_perm(G, eng.nS, f32=g.fields32, defer=True)
-            gv = sv.grad_values(Lam, U, scale=g.wq, permuted=True)
+            sv.join()
+            return Lam, sv.grad_values(Lam, U, scale=g.wq, permuted=True)
+        r, G = project()
+        gv = None
+        if not want_grad:
             if sv.check_wait():
+                r, _ = project()
+        else:
+            Lam, gv = adjoint(G)
+            if sv.check_wait():          # the forward's: U changed, all again
+                sv.check_discard()       # the adjoint's, of a superseded solve
+                r, G = project()
+                Lam, gv = adjoint(G)
+            if sv.check_wait():          # the adjoint's: Lam changed
                 gv = sv.grad_values(Lam, U, scale=g.wq, permuted=True)
     phi = (r ** 2).sum()
     return phi, d, (g.lmap_t(gv, sig) if want_grad else None), r

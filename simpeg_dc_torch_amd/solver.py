@@ -98,7 +98,7 @@ class HipBatchSolver:
         self._twin = None          # FP64 plan of the same pattern (fallback)
         self._twin_stamp = -1
         self.fallbacks = []        # (stamp, systems) of every FP64 fallback
-        self._pending = None       # the solve whose check is deferred
+        self._pending = []         # the solves whose checks are deferred, oldest first
         self.set_check()
 
     # -- info ----------------------------------------------------------------
@@ -220,7 +220,7 @@ class HipBatchSolver:
             raise exc
         twin = self._fp64_twin()
         st = self._stream()
-        flags = int(flags) & ~(_lib.DCX_F32_FIELDS | _lib.DCX_CHECK_DEFER)
+        flags = int(flags) & ~(_lib.DCX_F32_FIELDS | _lib.DCX_CHECK_DEFER | _lib.DCX_SPLIT_BWD)
         try:
             if self._twin_stamp != self.stamp:
                 _lib.check(self._lib.dcx_factorize(twin._h, _lib.ptr(vals), st), "factorize")
@@ -242,32 +242,59 @@ class HipBatchSolver:
         return bad.tolist()
 
     def check_wait(self):
-        """Verdict of the deferred check of the last solve run with
-        ``defer=True`` (dcx_plan_check_wait): [] when it passed (after any
+        """Verdict of the oldest deferred check (solves run with
+        ``defer=True``; dcx_plan_check_wait): [] when it passed (after any
         refinement passes), else the systems that were refactorised and
         solved in FP64, whose rows of that solve's x were replaced (results
         computed from that x before must be computed again).  Raises
         NumericError for an FP64 plan or a system that is not SPD."""
-        p = self._pending
-        if p is None:
+        if not self._pending:
             return []
-        self._pending = None
+        p = self._pending.pop(0)
         try:
             with torch.cuda.device(self.device):
                 _lib.check(self._lib.dcx_plan_check_wait(self._h), "check_wait")
         except _lib.NumericError as exc:
             if self.precision == "fp64":
+                self._pending.clear()
+                self._lib.dcx_plan_check_wait(self._h)   # drop the rest as well
                 raise
             with torch.cuda.device(self.device):
                 return self._fp64_fallback(*p, exc)
         return []
 
+    def check_discard(self):
+        """Complete the oldest deferred check without acting on its verdict
+        (its solve was superseded: its results are computed again)."""
+        if not self._pending:
+            return
+        self._pending.pop(0)
+        with torch.cuda.device(self.device):
+            self._lib.dcx_plan_check_wait(self._h)
+
+    def check_wait_all(self):
+        """check_wait for every pending deferred check."""
+        out = []
+        while self._pending:
+            out.append(self.check_wait())
+        return out
+
+    def join(self):
+        """Order the current stream after the second half of the last split
+        backward sweep (dcx_plan_join): every row of that solve's x is then
+        final."""
+        _lib.check(self._lib.dcx_plan_join(self._h, self._stream()), "plan_join")
+
     def run_solve(self, b, x, k, flags, defer=False):
         """dcx_solve on tensors (b FP64, x FP64 or FP32 with DCX_F32_FIELDS),
         with the FP64 fallback of failing systems; with defer, the check's
         verdict is left to `check_wait` (b and x must stay as they are until
-        then).  The caller holds the lock and the device."""
-        self.check_wait()
+        then; at most two are pending, a third completes the oldest).  The
+        caller holds the lock and the device."""
+        if self.refine > 0 and self.precision != "fp64":
+            self.check_wait_all()        # the library would complete them first
+        elif defer and len(self._pending) >= 2:
+            self.check_wait()
         fl = int(flags) | (_lib.DCX_CHECK_DEFER if defer else 0)
         try:
             _lib.check(self._lib.dcx_solve(self._h, _lib.ptr(b), _lib.ptr(x), int(k), fl,
@@ -278,13 +305,13 @@ class HipBatchSolver:
             self._fp64_fallback(self._vals_keep, b, x, k, flags, exc)
             return
         if defer:
-            self._pending = (self._vals_keep, b, x, k, flags)
+            self._pending.append((self._vals_keep, b, x, k, flags))
 
     def run_factor_solve(self, vals, b, x, k, flags, defer=False):
         """dcx_factor_solve on tensors (vals FP64 (nbatch, nnz)), with the FP64
         fallback of failing systems (defer: as `run_solve`); bumps the stamp.
         The caller holds the lock and the device."""
-        self.check_wait()
+        self.check_wait_all()            # the factor they refer to is replaced
         self._vals_keep = vals
         self.stamp += 1
         fl = int(flags) | (_lib.DCX_CHECK_DEFER if defer else 0)
@@ -295,10 +322,10 @@ class HipBatchSolver:
         except _lib.NumericError as exc:
             if self.precision == "fp64":
                 raise
-            self._fp64_fallback(vals, b, x, k, flags, exc)
+            self._fp64_fallback(vals, b, x, k, flags & ~_lib.DCX_SPLIT_BWD, exc)
             return
         if defer:
-            self._pending = (vals, b, x, k, flags)
+            self._pending.append((vals, b, x, k, flags & ~_lib.DCX_SPLIT_BWD))
 
     # -- numeric -------------------------------------------------------------
     def _stream(self):
@@ -310,7 +337,7 @@ class HipBatchSolver:
         if v.shape != (self.nbatch, self.col.numel()):
             v = v.reshape(self.nbatch, self.col.numel())
         with self._lock, torch.cuda.device(self.device):
-            self.check_wait()
+            self.check_wait_all()
             _lib.check(self._lib.dcx_factorize(self._h, _lib.ptr(v), self._stream()),
                        "factorize")
             self._vals_keep = v
