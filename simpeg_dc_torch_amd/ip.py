@@ -30,8 +30,20 @@ forward solves and both adjoint solves are the HIP eval+grad path and autograd
 differentiates both blocks of the model.
 
 `TorchCR2D` is the complex-resistivity forward of dctorch/cr2d.py
-(sigma* = sigma (1 - i m), complex-symmetric solves) on the cpu device only:
-libdcx.so factors real SPD systems, so a GPU device raises.
+(sigma* = sigma (1 - i m), complex-symmetric solves).  On the cpu it runs
+the reference's chain (complex SuperLU through the complex-safe autograd
+op).  On a GPU it needs no complex factorisation: A(sigma*) = A_r + i A_i
+with A_r = A(sigma) real SPD and A_i = A(-sigma m), and the complex system
+is solved by block Gauss-Seidel on the real factor of A_r,
+
+    x_r = A_r^{-1} (b_r + s A_i x_i),   x_i = A_r^{-1} (b_i - s A_i x_r)
+
+(s = +1 for A, -1 for its conjugate), whose error contracts by
+rho(A_r^{-1} A_i)^2 <= max|m|^2 per pass; the adjoint is the same
+iteration on the conjugate system, and the value gradients are four entry
+gathers (`_ComplexResistivity`).  Every step is a libdcx.so call
+(dcx_assemble, dcx_factorize, dcx_solve, dcx_spmm, dcx_survey_dpred,
+dcx_survey_adjoint_rhs, dcx_grad_values, dcx_lmap_transpose).
 """
 import numpy as np
 import torch
@@ -319,26 +331,107 @@ class TorchDCIP2D:
         return (r0 ** 2).sum() + lam * (r1 ** 2).sum()
 
 
+def _cr_solve(g, vim, br, bi, sign, k, tol=1e-13, maxit=60):
+    """(x_r, x_i) with (A_r + i sign A_i)(x_r + i x_i) = b_r + i b_i, A_r the
+    engine's factorised systems (elimination order, every wavenumber), A_i
+    the values vim, by block Gauss-Seidel on A_r (module docstring).  b_r,
+    b_i: (n, k) shared or (nbatch, n, k); None is zero.  Stops when both
+    parts change by at most tol (relative) in one pass."""
+    from . import _lib
+    sv = g.solver
+    nb, n = sv.nbatch, sv.n
+
+    def full(b):
+        if b is None:
+            return torch.zeros(nb, n, k, dtype=torch.float64, device=vim.device)
+        return b.expand(nb, n, k) if b.dim() == 2 else b
+
+    def spmm(x, alpha):
+        y = torch.empty_like(x)
+        _lib.check(g.lib.dcx_spmm(sv._h, _lib.ptr(vim), _lib.ptr(x), _lib.ptr(y), int(k),
+                                  float(alpha), _lib.DCX_PERMUTED, g.stream()), "spmm")
+        return y
+    br, bi = full(br), full(bi)
+    xi = torch.zeros(nb, n, k, dtype=torch.float64, device=vim.device)
+    xr = None
+    for it in range(maxit):
+        rr = (br + spmm(xi, sign)).contiguous() if it else br.contiguous()
+        xr_new = g.solve_perm(rr, k, sparse=False, shared=False)
+        xi_new = g.solve_perm((bi + spmm(xr_new, -sign)).contiguous(), k, sparse=False,
+                              shared=False)
+        if xr is not None:
+            dr = float(torch.linalg.norm(xr_new - xr) / torch.linalg.norm(xr_new))
+            di = float(torch.linalg.norm(xi_new - xi) / max(float(torch.linalg.norm(xi_new)),
+                                                              1e-300))
+            if max(dr, di) <= tol:
+                return xr_new, xi_new
+        xr, xi = xr_new, xi_new
+    raise RuntimeError(f"complex-resistivity iteration did not converge in {maxit} passes "
+                       "(it contracts by max|m|^2 per pass: |m| must stay below 1)")
+
+
+class _ComplexResistivity(torch.autograd.Function):
+    """(sigma, s2 = -sigma m) -> (Re V*, Im V*) on the GPU: V* = sum_k w_k
+    P A_k(sigma*)^{-1} q with A(sigma*) = A(sigma) + i A(s2).  Backward: the
+    adjoint fields lam = conj(A)^{-1} (G_r + i G_i), G the projection's
+    transpose applied to the data gradients; then per entry
+    dL/dA_r = -(lam_r u_r + lam_i u_i), dL/dA_i = lam_r u_i - lam_i u_r
+    (summed over sources, times w_k), and the transposed map."""
+
+    @staticmethod
+    def forward(ctx, sig, s2, cr):
+        g = cr.dc._gpu
+        nS, nD = cr.dc.nS, cr.dc.nD
+        with torch.cuda.device(cr.device):
+            vre, vim = g.assemble(sig), g.assemble(s2)
+            g.solver.factorize(vre)
+            ur, ui = _cr_solve(g, vim, g.q_perm, None, 1.0, nS)
+            Vr = torch.empty(nD, dtype=torch.float64, device=cr.device)
+            Vi = torch.empty_like(Vr)
+            g.dpred(ur, nS, 0, nS, Vr)
+            g.dpred(ui, nS, 0, nS, Vi)
+        ctx.cr, ctx.stamp = cr, g.solver.stamp
+        ctx.save_for_backward(vre, vim, ur, ui)
+        return Vr, Vi
+
+    @staticmethod
+    def backward(ctx, gVr, gVi):
+        cr = ctx.cr
+        g = cr.dc._gpu
+        nS = cr.dc.nS
+        vre, vim, ur, ui = ctx.saved_tensors
+        with torch.cuda.device(cr.device):
+            if g.solver.stamp != ctx.stamp:
+                g.solver.factorize(vre)
+
+            def rhs(gv):
+                return None if gv is None else \
+                    g.adjoint_rhs(gv.to(torch.float64).contiguous(), nS, 0, nS)
+            lr, li = _cr_solve(g, vim, rhs(gVr), rhs(gVi), -1.0, nS)
+            gvv = lambda y, x: g.solver.grad_values(y, x, scale=g.wq, permuted=True)
+            gv_re = gvv(lr, ur) + gvv(li, ui)
+            gv_im = gvv(li, ur) - gvv(lr, ui)
+            minus_one = -torch.ones(cr.dc.nC, dtype=torch.float64, device=cr.device)
+            return g.lmap_t(gv_re, minus_one), g.lmap_t(gv_im, minus_one), None
+
+
 class TorchCR2D:
     """Complex-resistivity 2.5-D nodal forward (dctorch/cr2d.py:13-108):
     sigma* = sigma_DC (1 - i m), one complex-symmetric solve per wavenumber,
     d_rho = |V*| ("magnitude") or Re V* ("real"), d_ip = ip_scale Im V* /
     Re V*; model cat(log rho_DC, m) on the model cells.
 
-    The cpu device only: the complex-symmetric solves are SciPy SuperLU (as
-    the reference's cpu path), through the complex-safe autograd op
-    `sparse_solve_batch`.  libdcx.so has no complex-symmetric factorisation,
-    so a GPU device raises NotImplementedError (no fallback)."""
+    On the cpu the complex-symmetric solves are SciPy SuperLU (as the
+    reference's cpu path), through the complex-safe autograd op
+    `sparse_solve_batch`.  On a GPU the forward and its gradient are
+    `_ComplexResistivity`: block Gauss-Seidel on the real FP64 factor of
+    A(sigma) (converges for |m| < 1, by max|m|^2 per pass)."""
 
     def __init__(self, mesh, survey, nky=12, device="cuda", dtype=torch.complex128,
                  active_cells=None, val_inactive=None, ip_scale=1.0, dc_datum="magnitude",
                  quadrature=None):
         from .simulation2d import TorchDC2D
         from .solver import make_batch_solver
-        if torch.device(device).type != "cpu":
-            raise NotImplementedError(
-                "TorchCR2D needs a complex-symmetric factorisation, which libdcx.so does "
-                "not have: use device='cpu'")
         if dc_datum not in ("magnitude", "real"):
             raise ValueError(f"dc_datum must be magnitude|real, got {dc_datum!r}")
         rdt = torch.float64 if dtype == torch.complex128 else torch.float32
@@ -351,6 +444,11 @@ class TorchCR2D:
         self.nP = dc.nC if dc._P_inj is None else int(dc._act.numel())
         self.ip_scale = float(ip_scale)
         self.dc_datum = dc_datum
+        if dc._gpu is not None:
+            if dtype != torch.complex128:
+                raise TypeError("TorchCR2D on a GPU runs in complex128 (FP64 real factor)")
+            self.solver = dc.solver
+            return
         self.solver = make_batch_solver(dc._crow, dc._ccol, dc.nN, dc.nS, dc.nky,
                                         device=self.device, dtype=dtype,
                                         matrix_type="symmetric")
@@ -377,6 +475,11 @@ class TorchCR2D:
         mi = m_ip.to(device=self.device, dtype=self.rdtype).reshape(-1)
         sig = torch.exp(-dc._model_to_full(mr))
         chg = self._inject_ip(mi)
+        if dc._gpu is not None:
+            Vr, Vi = _ComplexResistivity.apply(sig, -sig * chg, self)
+            d_rho = torch.sqrt(Vr ** 2 + Vi ** 2) if self.dc_datum == "magnitude" else Vr
+            d_ip = self.ip_scale * Vi / Vr
+            return d_rho.to(device=dev0, dtype=dt0), d_ip.to(device=dev0, dtype=dt0)
         vre = torch.sparse.mm(dc.Lstack, sig.unsqueeze(1)).view(dc.nky, dc.nnzA)
         vim = torch.sparse.mm(dc.Lstack, (-sig * chg).unsqueeze(1)).view(dc.nky, dc.nnzA)
         U = sparse_solve_batch(torch.complex(vre, vim), self.q, self.solver)

@@ -12,10 +12,11 @@ What pins each piece:
   and the "total" DC datum is V0 / (1 - eta); its gradient against central
   differences of its own misfit; to first order in eta it equals the
   linearised IP (the difference is O(eta)).
-* TorchCR2D (cpu): m = 0 gives the DC data; a uniform m gives V* = V_dc /
+* TorchCR2D (cpu and GPU): m = 0 gives the DC data; a uniform m gives V* = V_dc /
   (1 - i m) exactly (phase ratio m, magnitude |V_dc| / sqrt(1 + m^2)); the
-  complex-safe autograd solve against central differences; a GPU device is
-  refused.
+  complex-safe autograd solve against central differences; the GPU path
+  (block Gauss-Seidel on the real factor) against the cpu path's complex
+  SuperLU, data and gradient.
 * GPU (marked gpu): every GPU class on the HIP path against the same class on
   the CPU path.
 """
@@ -239,8 +240,45 @@ def test_cr2d_gradient_vs_fd():
         assert float(xg.grad @ v) == pytest.approx(float(fd), rel=1e-5)
 
 
-def test_cr2d_refuses_gpu():
+@pytest.mark.gpu
+def test_cr2d_gpu_matches_cpu():
+    """The GPU complex-resistivity forward (block Gauss-Seidel on the real
+    FP64 factor, _ComplexResistivity) against the cpu path's complex SuperLU
+    solves: both data and the misfit gradient with respect to both model
+    halves (log rho and m), for chargeabilities up to 0.3."""
     from simpeg_dc_torch_amd import TorchCR2D
-    mesh, survey, _ = case("suite")
-    with pytest.raises(NotImplementedError):
-        TorchCR2D(mesh, survey, nky=6, device="cuda")
+    mesh, survey, m = case("suite")
+    rng = np.random.default_rng(11)
+    x = np.r_[m, 0.3 * rng.random(mesh.nC)]
+    outs = []
+    for dev in ("cpu", "cuda"):
+        cr = TorchCR2D(mesh, survey, nky=6, device=dev, ip_scale=1000.0)
+        xt = torch.tensor(x, device=dev, requires_grad=True)
+        d_r, d_i = cr.dpred(xt)
+        if not outs:
+            dobs = [torch.tensor(a) for a in (*noisy_data(d_r.detach().numpy(), seed=1),
+                                              *noisy_data(d_i.detach().numpy(), seed=2))]
+        args = [a.to(dev) for a in dobs]
+        phi = ((args[1] * (d_r - args[0])) ** 2).sum() + ((args[3] * (d_i - args[2])) ** 2).sum()
+        phi.backward()
+        outs.append((d_r.detach().cpu(), d_i.detach().cpu(), xt.grad.cpu()))
+    for a, b in zip(outs[1], outs[0]):
+        assert _rel(a, b) < 1e-10
+
+
+@pytest.mark.gpu
+def test_cr2d_gpu_closed_forms():
+    """On the GPU: m = 0 gives the DC data with Im V* = 0; a uniform m gives
+    d_ip = ip_scale m exactly."""
+    from simpeg_dc_torch_amd import TorchCR2D, TorchDC2D
+    mesh, survey, m = case("suite")
+    mt = torch.tensor(m, device="cuda")
+    with torch.no_grad():
+        v_dc = TorchDC2D(mesh, survey, nky=6, device="cuda").dpred(mt)
+    cr = TorchCR2D(mesh, survey, nky=6, device="cuda", ip_scale=1000.0, dc_datum="real")
+    with torch.no_grad():
+        d_r, d_i = cr.dpred(torch.cat([mt, torch.zeros_like(mt)]))
+    assert float((d_r - v_dc).norm() / v_dc.norm()) < 1e-12 and float(d_i.abs().max()) < 1e-12
+    with torch.no_grad():
+        d_r, d_i = cr.dpred(torch.cat([mt, torch.full_like(mt, 0.05)]))
+    assert float((d_i - 50.0).abs().max()) < 1e-8
