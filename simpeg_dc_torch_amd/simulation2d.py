@@ -259,6 +259,57 @@ def _electrodes_backward(eng, saved, stamp, gd):
     return g.lmap_t(gv, sig)
 
 
+def _field_chunk(eng, want_grad):
+    """Source columns per pass of the fused eval+grad: all of them when the
+    fields of one pass (U and, with the gradient, Lambda: nky x n x nS each)
+    fit the budget -- the engine's fields_max_gib, else half of the free
+    device memory -- otherwise as many as fit (fields recomputed per chunk
+    instead of stored, the reference's memory bound by source chunks,
+    dctorch/simulation2d.py:345-375)."""
+    g = eng._gpu
+    s = 4 if g.fields32 else 8
+    per_col = (2 if want_grad else 1) * eng.nky * eng.nN * s + eng.nN * 8
+    budget = getattr(eng, "fields_max_gib", None)
+    if budget is None:
+        free, _ = torch.cuda.mem_get_info(eng.device)
+        budget_b = 0.5 * free
+    else:
+        budget_b = float(budget) * 2 ** 30
+    width = int(budget_b // per_col)
+    if eng.chunk:
+        width = min(width, int(eng.chunk))
+    return max(1, min(eng.nS, width))
+
+
+def _misfit_body_chunked(eng, m_full, dobs, w, want_grad, width):
+    """_misfit_body one source chunk of `width` columns at a time: one
+    factorisation, then per chunk the forward solve, the projection of its
+    data, its adjoint solve and the gather accumulated over the chunks."""
+    g = eng._gpu
+    sv = g.solver
+    sig = torch.exp(-m_full)
+    vals = g.assemble(sig)
+    sv.factorize(vals)
+    d = torch.empty(eng.nD, dtype=torch.float64, device=eng.device)
+    gv = torch.zeros(eng.nky, eng.nnzA, dtype=torch.float64, device=eng.device) \
+        if want_grad else None
+    for s0 in range(0, eng.nS, width):
+        s1 = min(s0 + width, eng.nS)
+        k = s1 - s0
+        U = g.solve_perm(g.q_perm[:, s0:s1].contiguous(), k, f32=g.fields32)
+        g.dpred(U, k, s0, s1, d)
+        if want_grad:
+            rows, _, _ = eng._rows_of(s0, s1)
+            v = torch.zeros(eng.nD, dtype=torch.float64, device=eng.device)
+            v[rows] = (2.0 * w[rows]) * (w[rows] * (d[rows] - dobs[rows]))
+            Lam = g.solve_perm(g.adjoint_rhs(v, k, s0, s1), k, f32=g.fields32)
+            gv += sv.grad_values(Lam, U, scale=g.wq, permuted=True)
+            del Lam
+        del U
+    r = w * (d - dobs)
+    return (r ** 2).sum(), d, (g.lmap_t(gv, sig) if want_grad else None), r
+
+
 def _misfit_body(eng, m_full, dobs, w, want_grad):
     """phi = ||w (dpred - dobs)||^2 and, with want_grad, dphi/dm_full, in one
     pass on the device: the forward solve, the projection, the misfit, the
@@ -270,6 +321,10 @@ def _misfit_body(eng, m_full, dobs, w, want_grad):
     with r = w (d - dobs)."""
     g = eng._gpu
     sv = g.solver
+    if eng.rhs_basis != "electrodes":
+        width = _field_chunk(eng, want_grad)
+        if width < eng.nS:
+            return _misfit_body_chunked(eng, m_full, dobs, w, want_grad, width)
     sig = torch.exp(-m_full)
     vals = g.assemble(sig)
     if eng.rhs_basis == "electrodes":
@@ -434,7 +489,8 @@ class TorchDC2D:
     def __init__(self, mesh, survey, nky=12, device="cuda", dtype=torch.float64,
                  active_cells=None, val_inactive=None, chunk=None,
                  singularity_removal=False, rho_primary=None, quadrature=None,
-                 precision="fp64", refine=1, leaf=None, rhs_basis="sources"):
+                 precision="fp64", refine=1, leaf=None, rhs_basis="sources",
+                 fields_max_gib=None):
         if singularity_removal:
             raise NotImplementedError(
                 "singularity removal is not part of this engine (off by default in the "
@@ -453,6 +509,9 @@ class TorchDC2D:
         if rhs_basis not in ("sources", "electrodes"):
             raise ValueError(f"rhs_basis must be sources|electrodes, got {rhs_basis!r}")
         self.rhs_basis = rhs_basis
+        # device memory budget of the fields one eval+grad keeps (None: half
+        # of the free memory); beyond it the sources go in chunks
+        self.fields_max_gib = fields_max_gib
 
         rmin, rmax = electrode_distance_range(survey)
         sp_pts, sp_wts, fell_back = simpeg_quadrature(mesh, nky)

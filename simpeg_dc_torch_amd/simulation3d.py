@@ -55,7 +55,7 @@ class TorchDC3D:
 
     def __init__(self, mesh, survey, device="cuda", dtype=torch.float64, active_cells=None,
                  val_inactive=None, bc_type="Robin", precision="fp64", refine=1, leaf=None,
-                 rhs_basis="sources"):
+                 rhs_basis="sources", fields_max_gib=None):
         if bc_type != "Robin":
             raise NotImplementedError("only the Robin boundary condition (the default) is built")
         self.device = torch.device(device)
@@ -66,6 +66,10 @@ class TorchDC3D:
         if rhs_basis not in ("sources", "electrodes"):
             raise ValueError(f"rhs_basis must be sources|electrodes, got {rhs_basis!r}")
         self.rhs_basis = rhs_basis
+        # device memory budget of the fields one eval+grad keeps (None: half
+        # of the free memory); beyond it the sources go in chunks
+        self.fields_max_gib = fields_max_gib
+        self.chunk = None
         maps = Problem3DMaps(mesh, survey)
         self._maps = maps
         self.kys = np.zeros(1)
@@ -136,6 +140,15 @@ class TorchDC3D:
 
     def _model_from_full(self, g_full):
         return g_full[self._act] if self._P_inj is not None else g_full
+
+    def _rows_of(self, s0, s1):
+        """(data rows, their indices, their source offsets) of sources s0..s1."""
+        key = (s0, s1)
+        cache = self.__dict__.setdefault("_rowcache", {})
+        if key not in cache:
+            sel = (self.sidx >= s0) & (self.sidx < s1)
+            cache[key] = (torch.nonzero(sel, as_tuple=True)[0], self.iD[sel], self.sidx[sel] - s0)
+        return cache[key]
 
     # ---------------- runtime --------------------------------------------------
     def dpred(self, m):

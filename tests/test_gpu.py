@@ -649,3 +649,46 @@ def test_plan_memory_through_torch_allocator():
     slv.free()
     torch.cuda.synchronize()
     assert mid - torch.cuda.memory_allocated() >= slv_bytes
+
+
+@pytest.mark.parametrize("dim", ["2d", "3d"])
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_fields_budget_chunks_the_eval_grad(precision, dim):
+    """A field-memory budget smaller than the fields of one eval+grad makes
+    the fused misfit go through source chunks (fields recomputed per chunk
+    instead of stored); the misfit and gradient equal the one-pass ones (to
+    rounding), through autograd and through eval_grad_device."""
+    from simpeg_dc_torch_amd import TorchDC2D, TorchDC3D
+    from simpeg_dc_torch_amd.simulation2d import _field_chunk
+    _say(f"fields budget {dim} {precision}")
+    if dim == "2d":
+        mesh, survey, m = case("suite")
+        make = lambda **kw: TorchDC2D(mesh, survey, nky=6, device=DEV, precision=precision,
+                                      refine=0, **kw)
+    else:
+        mesh, survey, m = case("small_3d")
+        make = lambda **kw: TorchDC3D(mesh, survey, device=DEV, precision=precision,
+                                      refine=0, **kw)
+    full = make()
+    per_col = 2 * full.nky * full.nN * (4 if precision == "mixed" else 8) + full.nN * 8
+    want = max(1, full.nS // 3)
+    small = make(fields_max_gib=(want + 0.5) * per_col / 2 ** 30)
+    assert _field_chunk(full, True) == full.nS and _field_chunk(small, True) == want < full.nS
+    mt = torch.tensor(m, device=DEV)
+    with torch.no_grad():
+        d = full.dpred(mt).cpu().numpy()
+    dobs, w = (torch.tensor(a, device=DEV) for a in noisy_data(d))
+    res = []
+    for eng in (full, small):
+        x = mt.clone().requires_grad_(True)
+        phi = eng.misfit(x, dobs, w)
+        phi.backward()
+        p2, g2 = eng.eval_grad_device(mt, dobs, w)
+        with torch.no_grad():
+            p3 = eng.misfit(mt, dobs, w)
+        res.append((float(phi), x.grad.cpu().numpy(), float(p2), g2.cpu().numpy(), float(p3)))
+    tol = 1e-12 if precision == "fp64" else 1e-6
+    assert res[1][0] == pytest.approx(res[0][0], rel=tol)
+    assert rel(res[1][1], res[0][1]) < tol and rel(res[1][3], res[0][3]) < tol
+    assert res[1][2] == pytest.approx(res[1][0], rel=1e-14)
+    assert res[1][4] == pytest.approx(res[0][4], rel=tol)
