@@ -239,6 +239,13 @@ class LineBatch:
             graphs.append((g, x, phi, x.grad))
         torch.cuda.synchronize(self.device)
         self._graphs = graphs
+        self._graph_gens = self._buffer_generations()
+
+    def _buffer_generations(self):
+        # a plan reallocates a workspace buffer when a later call needs more
+        # (a wider solve, a Jacobian block): graphs recorded before hold the
+        # old addresses, so the generations are compared before each replay
+        return [e._gpu.solver.info()["buffer_generation"] for e in self.engines]
 
     def _issue(self, t, ms, out):
         """The lines of stream t, one after another, on that stream."""
@@ -264,6 +271,10 @@ class LineBatch:
         cur = torch.cuda.current_stream(self.device)
         for s in self.streams:
             s.wait_stream(cur)
+        if self._graphs is not None and self._buffer_generations() != self._graph_gens:
+            self._graphs = None
+            raise RuntimeError("a plan reallocated its workspace after capture(): the recorded "
+                               "graphs are dropped; call capture() again")
         if self._graphs is not None:
             out = []
             for i, (g, x, phi, grad) in enumerate(self._graphs):

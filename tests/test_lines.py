@@ -128,6 +128,33 @@ def test_line_batch_graph_replays(precision):
 
 
 @pytest.mark.gpu
+def test_graphs_dropped_when_a_plan_reallocates():
+    """A wider solve after capture (an explicit Jacobian in blocks wider
+    than the sources) reallocates plan workspace: the next eval_grad refuses
+    to replay (the graphs hold the old addresses) instead of replaying, and a
+    new capture replays correctly again."""
+    print("graphs dropped on reallocation", flush=True)
+    L, data, lb = _setup(precision="fp64")
+    ms0 = [m for _, _, m in L]
+    lb.capture(ms0)
+    eng = lb.engines[0]
+    gen0 = eng._gpu.solver.info()["buffer_generation"]
+    eng.linearize(torch.tensor(ms0[0], device="cuda")).Jmatrix(sub=eng.nD)
+    torch.cuda.synchronize()
+    assert eng._gpu.solver.info()["buffer_generation"] > gen0
+    with pytest.raises(RuntimeError, match="capture"):
+        lb.eval_grad(ms0)
+    assert lb._graphs is None
+    lb.capture(ms0)
+    ref = [_single(lb.engines[i], ms0[i], *data[i]) for i in range(len(L))]
+    torch.cuda.synchronize()
+    out = lb.eval_grad(ms0)
+    torch.cuda.synchronize()
+    for i in range(len(L)):
+        assert float(out[i][0]) == pytest.approx(ref[i][0], rel=1e-13), i
+
+
+@pytest.mark.gpu
 def test_capture_refuses_mixed_engines():
     from simpeg_dc_torch_amd.lines import LineBatch
     print("capture refuses mixed", flush=True)
