@@ -252,24 +252,42 @@ __device__ void front_init(const SupDev& S, int b, const SupDev* __restrict__ su
     __syncthreads();  // F writes before, rpl reads of the previous child
     for (int i = tid; i < rc; i += NT) rpl[i] = relpos[C.rptr + i];
     __syncthreads();
-    const int64_t rc2 = static_cast<int64_t>(rc) * rc;
-    for (int64_t t0 = tid; t0 < rc2; t0 += NT * kU) {
+    if (rc == 0) continue;  // uniform
+    // entry t = i + j rc of the child's square update block; (i, j) is
+    // carried along from one entry of the thread to the next (steps of NT
+    // and NT kU entries) instead of divided out per entry
+    const int rc2 = rc * rc;  // in-LDS fronts: rc <= 1024
+    const int sq = NT / rc, sr = NT % rc;
+    const int bq = (NT * kU) / rc, br = (NT * kU) % rc;
+    int ib = tid % rc, jbk = tid / rc;
+    for (int t0 = tid; t0 < rc2; t0 += NT * kU) {
       T cv[kU], fv[kU];
-      int64_t fa[kU];
+      int fa[kU];  // front offsets: in-LDS fronts hold < 2^31 entries
+      int i = ib, j = jbk;
 #pragma unroll
       for (int u = 0; u < kU; ++u) {
-        const int64_t t = t0 + u * NT;
-        const int i = static_cast<int>(t % rc), j = static_cast<int>(t / rc);
         fa[u] = -1;
-        if (t < rc2 && i >= j) {
-          fa[u] = rpl[i] + static_cast<int64_t>(rpl[j]) * ld;
+        if (t0 + u * NT < rc2 && i >= j) {
+          fa[u] = rpl[i] + rpl[j] * ld;
           cv[u] = Fc[(C.ns + i) + static_cast<int64_t>(C.ns + j) * ldc];
           fv[u] = F[fa[u]];
+        }
+        i += sr;
+        j += sq;
+        if (i >= rc) {
+          i -= rc;
+          ++j;
         }
       }
 #pragma unroll
       for (int u = 0; u < kU; ++u)
         if (fa[u] >= 0) F[fa[u]] = fv[u] + cv[u];
+      ib += br;
+      jbk += bq;
+      if (ib >= rc) {
+        ib -= rc;
+        ++jbk;
+      }
     }
   }
   __syncthreads();
@@ -364,25 +382,47 @@ __device__ void panel_out(const SupDev& S, const T* __restrict__ F, T* __restric
   constexpr int kU = kGather;
   const int tid = threadIdx.x;
   const int ns = S.ns, nf = S.ns + S.rs, ld = nf + S.ns;
-  const int64_t np = static_cast<int64_t>(nf) * ns;
-  for (int64_t i0 = tid; i0 < np; i0 += NT * kU) {
+  // entry idx = c + r ns, with (c, r) carried along as in front_init
+  const int np = nf * ns;  // in-LDS fronts: < 2^31
+  const int sq = NT / ns, sr = NT % ns;
+  const int bq = (NT * kU) / ns, br = (NT * kU) % ns;
+  int cb = tid % ns, rb = tid / ns;
+  for (int i0 = tid; i0 < np; i0 += NT * kU) {
     T pv[kU];
+    int c = cb, r = rb;
 #pragma unroll
     for (int u = 0; u < kU; ++u) {
-      const int64_t idx = i0 + u * NT;
-      const int c = static_cast<int>(idx % ns), r = static_cast<int>(idx / ns);
-      pv[u] = idx < np ? F[(nf + c) + static_cast<int64_t>(r) * ld] : T(0);
+      pv[u] = i0 + u * NT < np ? F[(nf + c) + static_cast<int64_t>(r) * ld] : T(0);
       if (r < ns && r < c) pv[u] = T(0);
       if (r >= ns) pv[u] = -pv[u];
+      c += sr;
+      r += sq;
+      if (c >= ns) {
+        c -= ns;
+        ++r;
+      }
     }
+    c = cb;
+    r = rb;
 #pragma unroll
     for (int u = 0; u < kU; ++u) {
-      const int64_t idx = i0 + u * NT;
+      const int idx = i0 + u * NT;
       if (idx < np) {
-        const int c = static_cast<int>(idx % ns), r = static_cast<int>(idx / ns);
         PT[idx] = pv[u];
         P[r + static_cast<int64_t>(c) * nf] = pv[u];
       }
+      c += sr;
+      r += sq;
+      if (c >= ns) {
+        c -= ns;
+        ++r;
+      }
+    }
+    cb += br;
+    rb += bq;
+    if (cb >= ns) {
+      cb -= ns;
+      ++rb;
     }
   }
 }
@@ -547,11 +587,10 @@ __global__ __launch_bounds__(256) void k_front_zero_big(
   if (c0 >= nf) return;
   const int c1 = min(nf, c0 + kZeroCols);
   T* F = fcur + b * fcur_bs + S.front_off;
-  const int64_t base = static_cast<int64_t>(c0) * ld, n = static_cast<int64_t>(c1 - c0) * ld;
-  for (int64_t i = threadIdx.x; i < n; i += 256) {
-    const int64_t idx = base + i;
-    const int col = static_cast<int>(idx / ld), row = static_cast<int>(idx - int64_t(col) * ld);
-    F[idx] = row == nf + col ? T(1) : T(0);  // E row nf + j holds the identity column j
+  for (int col = c0; col < c1; ++col) {
+    T* Fcol = F + static_cast<int64_t>(col) * ld;
+    for (int row = threadIdx.x; row < ld; row += 256)
+      Fcol[row] = row == nf + col ? T(1) : T(0);  // E row nf + j holds the identity column j
   }
   // the entries of columns [c0, c1): a_col ascends within a supernode
   int lo = S.a_begin, hi = S.a_end;
