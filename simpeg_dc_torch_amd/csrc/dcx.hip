@@ -1282,14 +1282,22 @@ __global__ __launch_bounds__(kThreads) void k_bwd_wave(
     int64_t xs_bs) {
   constexpr int CW = 16 * V;
   const SweepBlock sb = sweep_block(lvl, cnt, nzb, total);
-  if (!sb.live) return;
+  if (!sb.live) return;  // uniform over the workgroup
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const SupDev S = sup[sb.s];
+  const int ns = S.ns, nf = S.ns + S.rs;
+  // the R rows' positions, staged in LDS once per workgroup (every wavefront
+  // of a workgroup works on the same supernode), so that the x_R loads of a
+  // k-step wait for one memory round trip, not for an index load first
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  constexpr size_t kRedBytes = KS > 1 ? static_cast<size_t>(KS) * MT * V * 4 * 64 * sizeof(T) : 0;
+  int* rl = reinterpret_cast<int*>(smem_raw + kRedBytes);
+  for (int i = threadIdx.x; i < S.rs; i += kThreads) rl[i] = ridx[S.rptr + i];
+  __syncthreads();
   // z = column group + nzc * row group; a row group is 16 MT output rows
   const int zc = sb.z % nzc, r0 = (sb.z / nzc) * 16 * MT;
   const int col0 = (KS == 1 ? zc * 4 + wave : zc) * CW;
   if (col0 >= k) return;  // uniform over the workgroup when KS > 1
-  const SupDev S = sup[sb.s];
-  const int ns = S.ns, nf = S.ns + S.rs;
   if (r0 >= ns) return;  // uniform
   const bool act = !active || active[sb.s];
   const T* PT = panelsT + sb.b * panel_bs + S.panel_off;
@@ -1321,8 +1329,7 @@ __global__ __launch_bounds__(kThreads) void k_bwd_wave(
         if (kk < ns) {
           if (act) load_cols<T, V>(Wb + static_cast<int64_t>(S.first + kk) * k + cl, navail, bv[j]);
         } else {
-          load_cols<T, V>(Wb + static_cast<int64_t>(ridx[S.rptr + kk - ns]) * k + cl, navail,
-                          bv[j]);
+          load_cols<T, V>(Wb + static_cast<int64_t>(rl[kk - ns]) * k + cl, navail, bv[j]);
 #pragma unroll
           for (int v = 0; v < V; ++v) bv[j][v] = -bv[j][v];
         }
@@ -1353,7 +1360,6 @@ __global__ __launch_bounds__(kThreads) void k_bwd_wave(
   if (KS > 1) {
     // partial tiles through LDS: wave w adds up and stores the tiles t with
     // t % KS == w
-    extern __shared__ __align__(16) unsigned char smem_raw[];
     T* red = reinterpret_cast<T*>(smem_raw);  // [KS][MT][V][4][64]
 #pragma unroll
     for (int t = 0; t < MT; ++t)
@@ -2810,14 +2816,16 @@ inline int64_t big_fwd_rows(const dcx_plan* P, size_t tsize) {
 
 template <typename T, int MT, int V, int KS, int PF>
 void bwd_wave_launch(dcx_plan* P, T* W, int k, const int* lvl, int cnt, const int* active,
-                     const SweepIO& io, hipStream_t st, int nrg) {
+                     const SweepIO& io, hipStream_t st, int nrg, int maxrs) {
   constexpr int CW = 16 * V;
   const int ncb = (k + CW - 1) / CW;              // column blocks
   const int nzc = KS == 1 ? (ncb + 3) / 4 : ncb;  // column groups per supernode and batch member
   const int nzb = nzc * nrg;                      // x row groups of 16 MT rows
   const int total = cnt * P->nbatch * nzb;
   dim3 grid(((total + 7) / 8) * 8);
-  const size_t lds = KS > 1 ? static_cast<size_t>(KS) * MT * V * 4 * 64 * sizeof(T) : 0;
+  // split-K partial tiles, then the R rows of the level's widest front
+  const size_t lds = (KS > 1 ? static_cast<size_t>(KS) * MT * V * 4 * 64 * sizeof(T) : 0) +
+                     static_cast<size_t>(std::max(maxrs, 1)) * sizeof(int);
   const int64_t w_bs = static_cast<int64_t>(P->S.n) * k;
   T* xs = nrg > 1 ? static_cast<T*>(P->d_fs) : nullptr;
   const int64_t xs_bs = big_fwd_rows(P, sizeof(T)) * k;
@@ -2835,16 +2843,16 @@ void bwd_wave_launch(dcx_plan* P, T* W, int k, const int* lvl, int cnt, const in
 }
 
 template <typename T, int V, int KS, int PF>
-bool bwd_wave_mt(dcx_plan* P, T* W, int k, int maxns, const int* lvl, int cnt,
+bool bwd_wave_mt(dcx_plan* P, T* W, int k, int maxns, int maxrs, const int* lvl, int cnt,
                  const int* active, const SweepIO& io, hipStream_t st) {
   const int mt = (maxns + 15) / 16;
-  if (mt <= 1) bwd_wave_launch<T, 1, V, KS, PF>(P, W, k, lvl, cnt, active, io, st, 1);
-  else if (mt <= 2) bwd_wave_launch<T, 2, V, KS, PF>(P, W, k, lvl, cnt, active, io, st, 1);
-  else if (mt <= 3) bwd_wave_launch<T, 3, V, KS, PF>(P, W, k, lvl, cnt, active, io, st, 1);
-  else if (mt <= 4) bwd_wave_launch<T, 4, V, KS, PF>(P, W, k, lvl, cnt, active, io, st, 1);
-  else if (mt <= 6) bwd_wave_launch<T, 6, V, KS, PF>(P, W, k, lvl, cnt, active, io, st, 1);
+  if (mt <= 1) bwd_wave_launch<T, 1, V, KS, PF>(P, W, k, lvl, cnt, active, io, st, 1, maxrs);
+  else if (mt <= 2) bwd_wave_launch<T, 2, V, KS, PF>(P, W, k, lvl, cnt, active, io, st, 1, maxrs);
+  else if (mt <= 3) bwd_wave_launch<T, 3, V, KS, PF>(P, W, k, lvl, cnt, active, io, st, 1, maxrs);
+  else if (mt <= 4) bwd_wave_launch<T, 4, V, KS, PF>(P, W, k, lvl, cnt, active, io, st, 1, maxrs);
+  else if (mt <= 6) bwd_wave_launch<T, 6, V, KS, PF>(P, W, k, lvl, cnt, active, io, st, 1, maxrs);
   else  // taller fronts: row groups of 128 rows
-    bwd_wave_launch<T, 8, V, KS, PF>(P, W, k, lvl, cnt, active, io, st, (mt + 7) / 8);
+    bwd_wave_launch<T, 8, V, KS, PF>(P, W, k, lvl, cnt, active, io, st, (mt + 7) / 8, maxrs);
   return true;
 }
 
@@ -2852,15 +2860,16 @@ bool bwd_wave_mt(dcx_plan* P, T* W, int k, int maxns, const int* lvl, int cnt,
 // wide vector columns where the level has enough wavefronts to fill the
 // chip, split-K otherwise
 template <typename T>
-bool bwd_wave(dcx_plan* P, T* W, int k, int maxns, const int* lvl, int cnt, const int* active,
-              const SweepIO& io, hipStream_t st) {
+bool bwd_wave(dcx_plan* P, T* W, int k, int maxns, int maxrs, const int* lvl, int cnt,
+              const int* active, const SweepIO& io, hipStream_t st) {
   static const int use = env_int("DCX_BWD_WAVE", 1);
   const int min_waves = P->splitk_below;
   if (!use) return false;
   constexpr int VW = 16 / sizeof(T);  // 16-byte vectors
   const int64_t waves = static_cast<int64_t>(cnt) * P->nbatch * ((k + 16 * VW - 1) / (16 * VW));
-  if (waves < min_waves) return bwd_wave_mt<T, 1, 4, 4>(P, W, k, maxns, lvl, cnt, active, io, st);
-  return bwd_wave_mt<T, VW, 1, 4>(P, W, k, maxns, lvl, cnt, active, io, st);
+  if (waves < min_waves)
+    return bwd_wave_mt<T, 1, 4, 4>(P, W, k, maxns, maxrs, lvl, cnt, active, io, st);
+  return bwd_wave_mt<T, VW, 1, 4>(P, W, k, maxns, maxrs, lvl, cnt, active, io, st);
 }
 
 template <typename T, int TC>
@@ -3028,14 +3037,15 @@ int sweep_level(dcx_plan* P, T* W, int k, bool fwd, int d, const int* act, const
     cnt = S.ulvl_ptr[d + 1] - S.ulvl_ptr[d];
   }
   if (!cnt) return DCX_OK;
-  int maxnf = 0, maxns = 0, maxpanel = 0;
+  int maxnf = 0, maxns = 0, maxrs = 0, maxpanel = 0;
   for (int q = S.ulvl_ptr[d]; q < S.ulvl_ptr[d + 1]; ++q) {
     int s = S.ulvl_sup[q];
     maxnf = std::max(maxnf, S.ns[s] + S.rs[s]);
     maxns = std::max(maxns, S.ns[s]);
+    maxrs = std::max(maxrs, S.rs[s]);
     maxpanel = std::max(maxpanel, (S.ns[s] + S.rs[s]) * S.ns[s]);
   }
-  if (!fwd && bwd_wave<T>(P, W, k, maxns, lvl, cnt, act, io, st)) return DCX_OK;
+  if (!fwd && bwd_wave<T>(P, W, k, maxns, maxrs, lvl, cnt, act, io, st)) return DCX_OK;
   const int tc = pick_tc(maxnf, maxns, sizeof(T), P->tc_cap);
   if (!tc) {
     if (fwd) return fwd_big<T>(P, W, k, d, lvl, cnt, maxnf, act, io, st);
