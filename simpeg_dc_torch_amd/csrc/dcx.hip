@@ -1089,6 +1089,7 @@ __device__ inline SweepBlock sweep_block(const int* lvl, int cnt, int nzb, int t
 // updates] in LDS, then [y_J; w] = P_s f_J: y_J to W, u_s = f_R - w to the
 // update buffer of this depth.  PL: panel staged in LDS, `ct` column tiles
 // per workgroup.
+constexpr int kFwdKids = 8;  // children whose extend-add data k_fwd_level stages
 template <typename T, int TC, bool PL>
 __global__ __launch_bounds__(kThreads) void k_fwd_level(
     const SupDev* __restrict__ sup, const int* __restrict__ lvl,
@@ -1119,6 +1120,20 @@ __global__ __launch_bounds__(kThreads) void k_fwd_level(
   T* Wb = W + b * w_bs;
   const double* Bb = Bsrc + b * b_bs;
   T* uo = ucur + b * ucur_bs + S.upd_off * k;
+  // the first kFwdKids children: update-block offset (ufz-based ones stored
+  // as -(offset + 1)), R rows (0: inactive, skipped) and relative positions,
+  // read once per workgroup, so that each child's extend-add waits for one
+  // memory round trip instead of a chain of index loads
+  __shared__ int64_t kid_u[kFwdKids];
+  __shared__ int kid_rs[kFwdKids], kid_rptr[kFwdKids];
+  const int nkid = S.c_end - S.c_begin;
+  if (tid < min(nkid, kFwdKids)) {
+    const int c = cidx[S.c_begin + tid];
+    const SupDev C = sup[c];
+    kid_rs[tid] = (!active || active[c]) ? C.rs : 0;
+    kid_rptr[tid] = C.rptr;
+    kid_u[tid] = C.fz_uoff >= 0 ? -(C.fz_uoff * k + 1) : C.upd_off * k;
+  }
   constexpr int NCT = TC / 16;
   const int nt = ((nf + 15) / 16) * NCT;
   for (int tile = 0; tile < ct; ++tile) {
@@ -1135,14 +1150,25 @@ __global__ __launch_bounds__(kThreads) void k_fwd_level(
       f[idx] = v;
     }
     __syncthreads();
-    for (int cc = S.c_begin; cc < S.c_end; ++cc) {
-      const int c = cidx[cc];
-      if (active && !active[c]) continue;
-      const SupDev C = sup[c];
-      const T* u = C.fz_uoff >= 0 ? ufz + b * ufz_bs + C.fz_uoff * k
-                                  : uprev + b * uprev_bs + C.upd_off * k;
-      const int* rp = relpos + C.rptr;
-      for (int idx = tid; idx < C.rs * TC; idx += kThreads) {
+    for (int q = 0; q < nkid; ++q) {
+      int rsc, rptr;
+      const T* u;
+      if (q < kFwdKids) {
+        rsc = kid_rs[q];
+        rptr = kid_rptr[q];
+        const int64_t uo_c = kid_u[q];
+        u = uo_c < 0 ? ufz + b * ufz_bs + (-uo_c - 1) : uprev + b * uprev_bs + uo_c;
+      } else {
+        const int c = cidx[S.c_begin + q];
+        const SupDev C = sup[c];
+        rsc = (!active || active[c]) ? C.rs : 0;
+        rptr = C.rptr;
+        u = C.fz_uoff >= 0 ? ufz + b * ufz_bs + C.fz_uoff * k
+                           : uprev + b * uprev_bs + C.upd_off * k;
+      }
+      if (!rsc) continue;  // inactive child (uniform)
+      const int* rp = relpos + rptr;
+      for (int idx = tid; idx < rsc * TC; idx += kThreads) {
         const int r = idx / TC, cl = idx % TC;
         if (col0 + cl < k) f[rp[r] * TC + cl] += u[static_cast<int64_t>(r) * k + col0 + cl];
       }
@@ -2120,10 +2146,13 @@ __global__ void k_jmap(int ncell, int r, const int* __restrict__ tptr,
 // (its own L2).  gridDim.x must be a multiple of 8.  Up to MAXE entries of a
 // row at a time; each lane loads 4 columns of the Y row and of every X row
 // before it multiplies, so that 4 (1 + MAXE) loads per lane are in flight,
-// the last partial block of columns included; FP64 sums.
-template <typename TI, int MAXE, int LPR>
+// the last partial block of columns included; FP64 sums.  With pad_col /
+// pad_out (rows of at most MAXE entries, padded with -1) a row's columns and
+// output positions come from one load each, with no row-pointer load first.
+template <typename TI, int MAXE, int LPR, bool PAD>
 __global__ void k_grad_values(const int* __restrict__ crow, const int* __restrict__ ccol,
-                              const int* __restrict__ oidx, const TI* __restrict__ Y,
+                              const int* __restrict__ oidx, const int* __restrict__ pad_col,
+                              const int* __restrict__ pad_out, const TI* __restrict__ Y,
                               const TI* __restrict__ X, int n, int k,
                               const double* __restrict__ scale, double* __restrict__ gv,
                               int64_t nnz) {
@@ -2141,12 +2170,19 @@ __global__ void k_grad_values(const int* __restrict__ crow, const int* __restric
   const int64_t xb = static_cast<int64_t>(b) * n * k;
   const TI* y = Y + xb + static_cast<int64_t>(row) * k;
   const double sc = scale ? scale[b] : 1.0;
-  for (int p0 = crow[row]; p0 < crow[row + 1]; p0 += MAXE) {
-    const int ne = min(MAXE, crow[row + 1] - p0);
+  const int pbeg = PAD ? 0 : crow[row], pend = PAD ? MAXE : crow[row + 1];
+  for (int p0 = pbeg; p0 < pend; p0 += MAXE) {
+    const int ne = min(MAXE, pend - p0);
     const TI* xr[MAXE];
 #pragma unroll
     for (int q = 0; q < MAXE; ++q) {
-      const int cq = ccol[p0 + min(q, ne - 1)];  // surplus slots repeat the last entry
+      int cq;
+      if constexpr (PAD) {
+        cq = pad_col[static_cast<int64_t>(row) * MAXE + q];
+        if (cq < 0) cq = row;  // empty slot: any valid row, its sum is dropped
+      } else {
+        cq = ccol[p0 + min(q, ne - 1)];  // surplus slots repeat the last entry
+      }
       xr[q] = X + xb + static_cast<int64_t>(cq) * k;
     }
     double acc[MAXE];
@@ -2202,7 +2238,14 @@ __global__ void k_grad_values(const int* __restrict__ crow, const int* __restric
       double a = acc[q];
 #pragma unroll
       for (int off = LPR / 2; off > 0; off >>= 1) a += __shfl_xor(a, off, 64);
-      if (sl == 0 && q < ne) gv[b * nnz + (oidx ? oidx[p0 + q] : p0 + q)] = -sc * a;
+      if constexpr (PAD) {
+        if (sl == 0) {
+          const int o = pad_out[static_cast<int64_t>(row) * MAXE + q];
+          if (o >= 0) gv[b * nnz + o] = -sc * a;
+        }
+      } else {
+        if (sl == 0 && q < ne) gv[b * nnz + (oidx ? oidx[p0 + q] : p0 + q)] = -sc * a;
+      }
     }
   }
 }
@@ -2426,6 +2469,12 @@ struct dcx_plan {
   size_t r_bytes = 0, b_bytes = 0, x_bytes = 0;
   int64_t device_bytes = 0;
   int max_row_nnz = 0;  // longest row of the caller's pattern
+  // the gather's padded rows (gpad_me = 5 or 8 entries per row, 0: rows too
+  // long): columns and output positions, in elimination order (p) and in the
+  // caller's order (u)
+  int gpad_me = 0;
+  int *d_gpad_col_p = nullptr, *d_gpad_out_p = nullptr, *d_gpad_col_u = nullptr,
+      *d_gpad_out_u = nullptr;
   // forward-sweep scratch of the levels whose fronts exceed the in-LDS kernel
   int64_t* d_fs_off = nullptr;  // row offset of each supernode within its level
   void* d_fs = nullptr;
@@ -2447,7 +2496,8 @@ struct dcx_plan {
                     d_front[0], d_front[1], d_vals, d_active, d_act_lvl, d_upd[0],
                     d_upd[1], d_w, d_r, d_b, d_x, d_fz, d_flv, d_fmem, d_fgptr, d_fgb,
                     d_toff, d_lridx, d_ulvl, d_ufz, d_panelsT, d_fs_off, d_fs, d_dg,
-                    d_fvals, d_inact_lvl};
+                    d_fvals, d_inact_lvl, d_gpad_col_p, d_gpad_out_p, d_gpad_col_u,
+                    d_gpad_out_u};
     for (void* p : ptrs)
       if (p) (void)dev_free(p);
     for (auto& c : slot) {
@@ -3392,6 +3442,26 @@ int dcx_plan_create(int n, const int32_t* crow, const int32_t* col, int nbatch,
     for (int i = 0; i < n; ++i) P->max_row_nnz = std::max(P->max_row_nnz, crow[i + 1] - crow[i]);
     HIP_TRY(upload(&P->d_crow, vcrow));
     HIP_TRY(upload(&P->d_col, vcol));
+    P->gpad_me = P->max_row_nnz <= 5 ? 5 : P->max_row_nnz <= 8 ? 8 : 0;
+    if (P->gpad_me) {
+      const int me = P->gpad_me;
+      std::vector<int> pc(static_cast<size_t>(n) * me, -1), po(pc.size(), -1);
+      std::vector<int> uc(pc.size(), -1), uo(pc.size(), -1);
+      for (int i = 0; i < n; ++i) {
+        for (int p = S.pcrow[i]; p < S.pcrow[i + 1]; ++p) {
+          pc[static_cast<size_t>(i) * me + (p - S.pcrow[i])] = S.pcol[p];
+          po[static_cast<size_t>(i) * me + (p - S.pcrow[i])] = S.pval_idx[p];
+        }
+        for (int p = crow[i]; p < crow[i + 1]; ++p) {
+          uc[static_cast<size_t>(i) * me + (p - crow[i])] = col[p];
+          uo[static_cast<size_t>(i) * me + (p - crow[i])] = p;
+        }
+      }
+      HIP_TRY(upload(&P->d_gpad_col_p, pc));
+      HIP_TRY(upload(&P->d_gpad_out_p, po));
+      HIP_TRY(upload(&P->d_gpad_col_u, uc));
+      HIP_TRY(upload(&P->d_gpad_out_u, uo));
+    }
     HIP_TRY(dev_malloc(&P->d_info, sizeof(int) * nbatch));
     {
       const char* e = std::getenv("DCX_CHECK_TOL");
@@ -4054,19 +4124,30 @@ int dcx_grad_values(dcx_plan* P, const void* y, const void* x, int k,
   const int* rp = perm ? P->d_pcrow : P->d_crow;
   const int* cl = perm ? P->d_pcol : P->d_col;
   const int* oi = perm ? P->d_pvidx : nullptr;
+  const int* pcol = perm ? P->d_gpad_col_p : P->d_gpad_col_u;
+  const int* pout = perm ? P->d_gpad_out_p : P->d_gpad_out_u;
   // entries per pass: the longest row up to 5 (2-D stencils), else 8
   const bool wide = P->max_row_nnz > 5;
   auto go = [&](auto tag, auto me) {
     using TI = decltype(tag);
     constexpr int ME = decltype(me)::value;
+    // padded rows when every row fits the pass width
+    const bool padded = P->gpad_me == ME;
     auto launch = [&](auto kern) {
-      hipLaunchKernelGGL(kern, grid, dim3(256), 0, st, rp, cl, oi, static_cast<const TI*>(y),
+      hipLaunchKernelGGL(kern, grid, dim3(256), 0, st, rp, cl, oi, padded ? pcol : nullptr,
+                         padded ? pout : nullptr, static_cast<const TI*>(y),
                          static_cast<const TI*>(x), S.n, k, scale, gv,
                          static_cast<int64_t>(S.nnz));
     };
-    if (lpr == 16) launch(k_grad_values<TI, ME, 16>);
-    else if (lpr == 32) launch(k_grad_values<TI, ME, 32>);
-    else launch(k_grad_values<TI, ME, 64>);
+    if (padded) {
+      if (lpr == 16) launch(k_grad_values<TI, ME, 16, true>);
+      else if (lpr == 32) launch(k_grad_values<TI, ME, 32, true>);
+      else launch(k_grad_values<TI, ME, 64, true>);
+    } else {
+      if (lpr == 16) launch(k_grad_values<TI, ME, 16, false>);
+      else if (lpr == 32) launch(k_grad_values<TI, ME, 32, false>);
+      else launch(k_grad_values<TI, ME, 64, false>);
+    }
   };
   using E5 = std::integral_constant<int, 5>;
   using E8 = std::integral_constant<int, 8>;
