@@ -41,7 +41,7 @@ enum {
 enum { DCX_FP64 = 0, DCX_FP32 = 1 };
 /* keys of dcx_plan_set_tuning / dcx_plan_get_tuning */
 enum { DCX_TUNE_SWEEP_TC = 1, DCX_TUNE_SPLITK_BELOW = 2, DCX_TUNE_WIDE_BELOW = 3,
-       DCX_TUNE_PANEL_WIDTH = 4 };
+       DCX_TUNE_PANEL_WIDTH = 4, DCX_TUNE_ROWSPLIT_BELOW = 5 };
 
 /* dcx_solve flags */
 enum {
@@ -146,7 +146,10 @@ int dcx_plan_sweep_times(dcx_plan* plan, double* fwd_ms, double* bwd_ms, int* ns
  * sweeps (16, 32, 64); DCX_TUNE_SPLITK_BELOW, backward-sweep levels with fewer
  * wavefronts take the split-K variant; DCX_TUNE_WIDE_BELOW, factor levels with
  * fewer workgroups take 1024-thread fronts; DCX_TUNE_PANEL_WIDTH (16 or 32),
- * columns per panel of the in-LDS factor kernel.  Every value selects among
+ * columns per panel of the in-LDS factor kernel; DCX_TUNE_ROWSPLIT_BELOW,
+ * backward-sweep levels of supernodes of at most 64 rows with fewer wavefronts
+ * (of the column-split kernel) take the row-split variant, four wavefronts per
+ * column block (checked before the split-K threshold).  Every value selects among
  * kernels that are correct for any level; only speed and summation order
  * change.  No counterpart in the reference (cuDSS chooses internally behind
  * dctorch/solver.py:187-213); a tile autotuner times them per size class. */

@@ -17,6 +17,7 @@ DCX_FP64, DCX_FP32 = 0, 1
 DCX_SHARED_RHS, DCX_PERMUTED, DCX_SPARSE_RHS, DCX_F32_FIELDS = 1, 2, 4, 8
 DCX_CHECK_DEFER, DCX_SPLIT_BWD = 16, 32
 DCX_TUNE_SWEEP_TC, DCX_TUNE_SPLITK_BELOW, DCX_TUNE_WIDE_BELOW, DCX_TUNE_PANEL_WIDTH = 1, 2, 3, 4
+DCX_TUNE_ROWSPLIT_BELOW = 5
 
 _i32p = C.POINTER(C.c_int32)
 _f64p = C.POINTER(C.c_double)

@@ -1269,7 +1269,12 @@ __global__ __launch_bounds__(kThreads) void k_bwd_level(
 // and the column set {c0 + V j + v : j < 16} is the v-th 16-column matrix
 // tile.  KS = 1: the four wavefronts of a workgroup take four adjacent column
 // blocks; KS = 4: they split the K range (the front rows) of one column block
-// and add their partial tiles through LDS (levels with few supernodes).
+// and add their partial tiles through LDS (levels with few supernodes);
+// KS = 0: they take the four 16-row tiles (MT = 1, ns <= 64) of one column
+// block, each walking the whole K range, and store after a barrier (the other
+// wavefronts still read y_J from those rows until then): four times the
+// wavefronts of KS = 1 for the upper levels, where a level holds too few
+// wavefronts to hide the latency of its K loop.
 // MT: 16-row output tiles (ns <= 16 MT); PF: k-steps whose loads are in
 // flight together.  Grid as sweep_block, nzb = column groups per workgroup.
 template <typename T, int V>
@@ -1321,10 +1326,15 @@ __global__ __launch_bounds__(kThreads) void k_bwd_wave(
   for (int i = threadIdx.x; i < S.rs; i += kThreads) rl[i] = ridx[S.rptr + i];
   __syncthreads();
   // z = column group + nzc * row group; a row group is 16 MT output rows
-  const int zc = sb.z % nzc, r0 = (sb.z / nzc) * 16 * MT;
+  // (KS = 0: 64 rows, wavefront w taking rows [16 w, 16 w + 16))
+  const int zc = sb.z % nzc;
+  const int r0 = KS == 0 ? 16 * wave : (sb.z / nzc) * 16 * MT;
   const int col0 = (KS == 1 ? zc * 4 + wave : zc) * CW;
-  if (col0 >= k) return;  // uniform over the workgroup when KS > 1
-  if (r0 >= ns) return;  // uniform
+  if (col0 >= k) return;  // uniform over the workgroup when KS != 1
+  // uniform over the workgroup, except KS = 0, whose idle wavefronts still
+  // take part in the barrier below
+  if (KS != 0 && r0 >= ns) return;
+  const bool rows = r0 < ns;
   const bool act = !active || active[sb.s];
   const T* PT = panelsT + sb.b * panel_bs + S.panel_off;
   T* Wb = W + sb.b * w_bs;
@@ -1343,7 +1353,7 @@ __global__ __launch_bounds__(kThreads) void k_bwd_wave(
   for (int t = 0; t < MT; ++t)
 #pragma unroll
     for (int v = 0; v < V; ++v) acc[t][v] = {T(0), T(0), T(0), T(0)};
-  for (int k0 = kbeg; k0 < kend; k0 += 4 * PF) {
+  for (int k0 = kbeg; k0 < (rows ? kend : 0); k0 += 4 * PF) {
     T bv[PF][V], av[PF][MT];
 #pragma unroll
     for (int j = 0; j < PF; ++j) {
@@ -1409,7 +1419,8 @@ __global__ __launch_bounds__(kThreads) void k_bwd_wave(
         }
     }
   }
-  if (navail <= 0) return;
+  if (KS == 0) __syncthreads();  // every wavefront has read y_J
+  if (navail <= 0 || !rows) return;
 #pragma unroll
   for (int t = 0; t < MT; ++t) {
     if (KS > 1 && t % KS != wave) continue;
@@ -2400,6 +2411,7 @@ struct dcx_plan {
   // environment at plan creation)
   int tc_cap = 32;           // widest column tile of the in-LDS sweeps
   int splitk_below = 4096;   // backward levels with fewer wavefronts take split-K
+  int rowsplit_below = 0;    // ... and those with ns <= 64 the row-split variant
   int wide_below = 321;      // factor levels with fewer workgroups take 1024 threads
   int panel_width = 32;      // columns per panel of the in-LDS factor (16 or 32)
   // factor launch order: each level's supernodes by front height (ld)
@@ -2870,6 +2882,7 @@ void bwd_wave_launch(dcx_plan* P, T* W, int k, const int* lvl, int cnt, const in
   constexpr int CW = 16 * V;
   const int ncb = (k + CW - 1) / CW;              // column blocks
   const int nzc = KS == 1 ? (ncb + 3) / 4 : ncb;  // column groups per supernode and batch member
+  static_assert(KS != 0 || MT == 1, "row-split tiles are 16 rows per wavefront");
   const int nzb = nzc * nrg;                      // x row groups of 16 MT rows
   const int total = cnt * P->nbatch * nzb;
   dim3 grid(((total + 7) / 8) * 8);
@@ -2917,6 +2930,10 @@ bool bwd_wave(dcx_plan* P, T* W, int k, int maxns, int maxrs, const int* lvl, in
   if (!use) return false;
   constexpr int VW = 16 / sizeof(T);  // 16-byte vectors
   const int64_t waves = static_cast<int64_t>(cnt) * P->nbatch * ((k + 16 * VW - 1) / (16 * VW));
+  if (maxns <= 64 && waves < P->rowsplit_below) {  // four wavefronts per column block
+    bwd_wave_launch<T, 1, VW, 0, 4>(P, W, k, lvl, cnt, active, io, st, 1, maxrs);
+    return true;
+  }
   if (waves < min_waves)
     return bwd_wave_mt<T, 1, 4, 4>(P, W, k, maxns, maxrs, lvl, cnt, active, io, st);
   return bwd_wave_mt<T, VW, 1, 4>(P, W, k, maxns, maxrs, lvl, cnt, active, io, st);
@@ -3407,6 +3424,7 @@ int dcx_plan_create(int n, const int32_t* crow, const int32_t* col, int nbatch,
     P->tsize = precision == DCX_FP32 ? 4 : 8;
     P->tc_cap = env_int("DCX_SWEEP_TC", 32);
     P->splitk_below = env_int("DCX_BWD_SPLITK_BELOW", 4096);
+    P->rowsplit_below = env_int("DCX_BWD_ROWSPLIT_BELOW", 0);
     P->wide_below = env_int("DCX_FACTOR_WIDE_BELOW", 321);
     P->panel_width = env_int("DCX_FACTOR_PANEL_WIDTH", 32) == 16 ? 16 : 32;
     {
@@ -3604,6 +3622,10 @@ int dcx_plan_set_tuning(dcx_plan* P, int key, int value) {
       if (value != 16 && value != 32) return fail(DCX_ERR_ARG, "panel width must be 16 or 32");
       P->panel_width = value;
       break;
+    case DCX_TUNE_ROWSPLIT_BELOW:
+      if (value < 0) return fail(DCX_ERR_ARG, "row-split threshold must be >= 0");
+      P->rowsplit_below = value;
+      break;
     default:
       return fail(DCX_ERR_ARG, "unknown tuning key");
   }
@@ -3617,6 +3639,7 @@ int dcx_plan_get_tuning(const dcx_plan* P, int key, int* value) {
     case DCX_TUNE_SPLITK_BELOW: *value = P->splitk_below; break;
     case DCX_TUNE_WIDE_BELOW: *value = P->wide_below; break;
     case DCX_TUNE_PANEL_WIDTH: *value = P->panel_width; break;
+    case DCX_TUNE_ROWSPLIT_BELOW: *value = P->rowsplit_below; break;
     default: return fail(DCX_ERR_ARG, "unknown tuning key");
   }
   return DCX_OK;

@@ -57,6 +57,7 @@ def _eval_grad(eng, x, dobs, w):
 
 #: candidate values of each kernel-variant threshold (dcx_plan_set_tuning)
 TILE_GRID = {"sweep_tc": (16, 32, 64), "splitk_below": (1024, 4096, 16384),
+             "rowsplit_below": (0, 4096, 16384, 65536),
              "wide_below": (0, 321, 4096), "panel_width": (16, 32)}
 
 

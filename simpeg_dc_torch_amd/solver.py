@@ -123,7 +123,8 @@ class HipBatchSolver:
             _lib.check(self._lib.dcx_plan_set_fusion(self._h, int(rows)), "set_fusion")
         self.fuse_rows = int(rows)
 
-    _TUNE_KEYS = {"sweep_tc": 1, "splitk_below": 2, "wide_below": 3, "panel_width": 4}
+    _TUNE_KEYS = {"sweep_tc": 1, "splitk_below": 2, "wide_below": 3, "panel_width": 4,
+                  "rowsplit_below": 5}
 
     def set_tuning(self, **values):
         """Kernel-variant thresholds of this plan: sweep_tc (16, 32, 64),
