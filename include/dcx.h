@@ -352,7 +352,10 @@ int dcx_survey_dpred(dcx_survey* sv, const void* U, int k, int nky,
                      void* stream);
 /* G[j][s - s0] = sum over data i of source s in [s0, s1) of Pd[i,j] v[i]:
  * device (n x k), zero elsewhere; rows in elimination order (DCX_PERMUTED) or
- * original order. */
+ * original order.  With DCX_SPARSE_RHS the block is not cleared first: only
+ * the survey's target entries (the receiver rows of each source's data, the
+ * same set on every call with the same k, s0, s1 and order) are written, for
+ * a G that is already zero elsewhere -- a block cleared once and reused. */
 int dcx_survey_adjoint_rhs(dcx_survey* sv, const double* v, double* G, int k,
                            int s0, int s1, int flags, void* stream);
 

@@ -2338,23 +2338,30 @@ __global__ void k_lmap_t(int ncell, int nnz, int nky, const int* __restrict__ tp
 // ---------------------------------------------------------------------------
 // survey: projection and adjoint sources
 // ---------------------------------------------------------------------------
+// 16 lanes per datum: lane j takes the wavenumbers j, j + 16, ..., so that the
+// field loads of every wavenumber are in flight together; the lanes' sums are
+// added by a fixed butterfly (deterministic).
+constexpr int kDpredLanes = 16;
 template <typename TU>
 __global__ void k_dpred(int nD, const int* __restrict__ ptr, const int* __restrict__ rows,
                         const double* __restrict__ val, const int* __restrict__ sidx,
                         const TU* __restrict__ U, int64_t bs, int k, int nky,
                         const double* __restrict__ w, int s0, int s1, double* __restrict__ d) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nD) return;
+  const int i = (blockIdx.x * blockDim.x + threadIdx.x) / kDpredLanes;
+  const int j = threadIdx.x % kDpredLanes;
+  if (i >= nD) return;  // uniform over the datum's lanes
   const int s = sidx[i] - s0;
-  if (sidx[i] < s0 || sidx[i] >= s1) return;
+  if (sidx[i] < s0 || sidx[i] >= s1) return;  // uniform over the datum's lanes
   double acc = 0.0;
-  for (int kk = 0; kk < nky; ++kk) {
+  for (int kk = j; kk < nky; kk += kDpredLanes) {
     double a = 0.0;
     for (int p = ptr[i]; p < ptr[i + 1]; ++p)
       a += val[p] * static_cast<double>(U[kk * bs + static_cast<int64_t>(rows[p]) * k + s]);
     acc += w[kk] * a;
   }
-  d[i] = acc;
+#pragma unroll
+  for (int off = kDpredLanes / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, kDpredLanes);
+  if (j == 0) d[i] = acc;
 }
 
 // G[row][src - s0] = sum coef * v[datum] for each (row, source) target
@@ -3214,7 +3221,7 @@ int factor_and_sweeps(dcx_plan* P, T* W, int k, bool sparse_rhs, const SweepIO& 
   hipEvent_t start = P->lev_ev[S.ndepth], join = P->lev_ev[S.ndepth + 1];
   HIP_TRY(hipEventRecord(start, st));
   HIP_TRY(hipStreamWaitEvent(P->side, start, 0));
-  for (int d = S.ndepth - 1; d >= 0; --d) {
+  for (int d = S.ndepth - 1; d >= 1; --d) {
     rc = factor_level<T>(P, d, st);
     if (rc) return rc;
     HIP_TRY(hipEventRecord(P->lev_ev[d], st));
@@ -3222,8 +3229,15 @@ int factor_and_sweeps(dcx_plan* P, T* W, int k, bool sparse_rhs, const SweepIO& 
     rc = sweep_level<T>(P, W, k, true, d, act, io, P->side);
     if (rc) return rc;
   }
+  // the root level: factored, then its forward sweep on `st` after the side
+  // stream's levels (finished by then, while the root was factored) -- no
+  // event between the root's factorisation and its sweep
   HIP_TRY(hipEventRecord(join, P->side));
+  rc = factor_level<T>(P, 0, st);
+  if (rc) return rc;
   HIP_TRY(hipStreamWaitEvent(st, join, 0));
+  rc = sweep_level<T>(P, W, k, true, 0, act, io, st);
+  if (rc) return rc;
   return sweeps<T>(P, W, k, sparse_rhs, io, st, true, split);
 }
 
@@ -4652,12 +4666,13 @@ int dcx_survey_dpred(dcx_survey* V, const void* U, int k, int nky, const double*
   HIP_TRY(hipSetDevice(V->device));
   const int64_t bs = static_cast<int64_t>(V->n) * k;
   const int* rows = (flags & DCX_PERMUTED) ? V->d_rows_p : V->d_rows;
+  const int blocks = grid1(static_cast<int64_t>(V->nD) * kDpredLanes, 256);
   if (flags & DCX_F32_FIELDS)
-    hipLaunchKernelGGL(k_dpred<float>, dim3(grid1(V->nD, 256)), dim3(256), 0, st, V->nD,
+    hipLaunchKernelGGL(k_dpred<float>, dim3(blocks), dim3(256), 0, st, V->nD,
                        V->d_ptr, rows, V->d_val, V->d_sidx, static_cast<const float*>(U), bs, k,
                        nky, w, s0, s1, d);
   else
-    hipLaunchKernelGGL(k_dpred<double>, dim3(grid1(V->nD, 256)), dim3(256), 0, st, V->nD,
+    hipLaunchKernelGGL(k_dpred<double>, dim3(blocks), dim3(256), 0, st, V->nD,
                        V->d_ptr, rows, V->d_val, V->d_sidx, static_cast<const double*>(U), bs,
                        k, nky, w, s0, s1, d);
   HIP_TRY(hipGetLastError());
@@ -4672,7 +4687,9 @@ int dcx_survey_adjoint_rhs(dcx_survey* V, const double* v, double* G, int k, int
   hipStream_t st = static_cast<hipStream_t>(stream);
   HIP_TRY(hipSetDevice(V->device));
   const int64_t tot = static_cast<int64_t>(V->n) * k;
-  hipLaunchKernelGGL(k_fill, dim3(grid_stride(tot)), dim3(256), 0, st, G, tot, 0.0);
+  // DCX_SPARSE_RHS: G is already zero outside the target entries
+  if (!(flags & DCX_SPARSE_RHS))
+    hipLaunchKernelGGL(k_fill, dim3(grid_stride(tot)), dim3(256), 0, st, G, tot, 0.0);
   if (V->ntgt)
     hipLaunchKernelGGL(k_adjoint_rhs, dim3(grid1(V->ntgt, 256)), dim3(256), 0, st, V->ntgt,
                        V->d_tptr, (flags & DCX_PERMUTED) ? V->d_trow_p : V->d_trow, V->d_tsrc,
