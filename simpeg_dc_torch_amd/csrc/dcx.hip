@@ -2261,6 +2261,86 @@ __global__ void k_grad_values(const int* __restrict__ crow, const int* __restric
   }
 }
 
+// The gather for long rows (padded entries) with a whole pass of a row's loads
+// in flight at once: one wavefront per row, each lane loading 16-byte vectors
+// (VW = 4 floats or 2 doubles) at columns c0 + VW (64 j + lane), j < NV, of the
+// Y row and of each of the row's MAXE X rows -- (1 + MAXE) NV loads per lane
+// issued before the first product, so that a row of up to 64 VW NV columns
+// costs one memory round trip (k_grad_values' lanes walk a row in four).
+// Every load is unconditional: a vector that would cross the end of the row
+// is read at the last full vector of the row instead and its columns already
+// counted are masked, so no load sits behind a branch.  k >= VW.  FP64 sums.
+template <typename TI, int MAXE, int NV>
+__global__ __launch_bounds__(256) void k_grad_rows(const int* __restrict__ pad_col,
+                                                   const int* __restrict__ pad_out,
+                                                   const TI* __restrict__ Y,
+                                                   const TI* __restrict__ X, int n, int k,
+                                                   const double* __restrict__ scale,
+                                                   double* __restrict__ gv, int64_t nnz) {
+  constexpr int VW = 16 / sizeof(TI);
+  constexpr int CP = 64 * VW * NV;  // columns per pass
+  const int per = gridDim.x >> 3;
+  const int lb = (blockIdx.x & 7) * per + (blockIdx.x >> 3);  // XCD remap as k_grad_values
+  const int lane = threadIdx.x & 63;
+  const int row = lb * 4 + (threadIdx.x >> 6);
+  const int b = blockIdx.y;
+  if (row >= n) return;  // uniform over the wavefront
+  const int64_t xb = static_cast<int64_t>(b) * n * k;
+  const TI* y = Y + xb + static_cast<int64_t>(row) * k;
+  const TI* xr[MAXE];
+#pragma unroll
+  for (int q = 0; q < MAXE; ++q) {
+    int cq = pad_col[static_cast<int64_t>(row) * MAXE + q];
+    if (cq < 0) cq = row;  // empty slot: any valid row, its sum is dropped
+    xr[q] = X + xb + static_cast<int64_t>(cq) * k;
+  }
+  double acc[MAXE];
+#pragma unroll
+  for (int q = 0; q < MAXE; ++q) acc[q] = 0.0;
+  for (int c0 = 0; c0 < k; c0 += CP) {
+    int ce[NV];
+    TI yv[NV][VW], xv[MAXE][NV][VW];
+#pragma unroll
+    for (int j = 0; j < NV; ++j) {
+      const int c = c0 + VW * (64 * j + lane);
+      ce[j] = min(c, k - VW);
+      typename Vec<TI, VW>::v t;
+      __builtin_memcpy(&t, y + ce[j], sizeof(t));
+#pragma unroll
+      for (int v = 0; v < VW; ++v) {
+        // column ce + v counts here if no earlier vector of this lane's
+        // pass or of another lane covered it: ce + v >= c
+        yv[j][v] = ce[j] + v >= c ? t[v] : TI(0);
+      }
+#pragma unroll
+      for (int q = 0; q < MAXE; ++q) {
+        typename Vec<TI, VW>::v u;
+        __builtin_memcpy(&u, xr[q] + ce[j], sizeof(u));
+#pragma unroll
+        for (int v = 0; v < VW; ++v) xv[q][j][v] = u[v];
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < MAXE; ++q)
+#pragma unroll
+      for (int j = 0; j < NV; ++j)
+#pragma unroll
+        for (int v = 0; v < VW; ++v)
+          acc[q] += static_cast<double>(yv[j][v]) * static_cast<double>(xv[q][j][v]);
+  }
+  const double sc = scale ? scale[b] : 1.0;
+#pragma unroll
+  for (int q = 0; q < MAXE; ++q) {
+    double a = acc[q];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) a += __shfl_xor(a, off, 64);
+    if (lane == 0) {
+      const int o = pad_out[static_cast<int64_t>(row) * MAXE + q];
+      if (o >= 0) gv[b * nnz + o] = -sc * a;
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // assembly map
 // ---------------------------------------------------------------------------
@@ -4176,7 +4256,26 @@ int dcx_grad_values(dcx_plan* P, const void* y, const void* x, int k,
                          static_cast<const TI*>(x), S.n, k, scale, gv,
                          static_cast<int64_t>(S.nnz));
     };
-    if (padded) {
+    // k_grad_rows for long padded FP64 rows, four vectors per lane and pass
+    // (two passes on the 25k-cell line, 849 columns): eval+grad 6.20 ->
+    // 6.10 ms there.  FP32 rows keep k_grad_values: k_grad_rows (two vectors
+    // per lane) timed faster alone but made the mixed eval+grad slower,
+    // 3.78 -> 3.94 ms.  DCX_GATHER_NV (vectors per lane, 0: k_grad_values)
+    // overrides both.
+    static const int nv_env = env_int("DCX_GATHER_NV", -1);
+    const int nv = nv_env >= 0 ? nv_env : (sizeof(TI) == 4 ? 0 : 4);
+    constexpr int VWI = 16 / static_cast<int>(sizeof(TI));
+    if (padded && lpr == 64 && nv > 0 && k >= VWI) {
+      // whole rows in flight (k_grad_rows), NV vectors of each row per lane
+      auto rl = [&](auto kern) {
+        hipLaunchKernelGGL(kern, grid, dim3(256), 0, st, pcol, pout, static_cast<const TI*>(y),
+                           static_cast<const TI*>(x), S.n, k, scale, gv,
+                           static_cast<int64_t>(S.nnz));
+      };
+      if (nv <= 2) rl(k_grad_rows<TI, ME, 2>);
+      else if (nv <= 4) rl(k_grad_rows<TI, ME, 4>);
+      else rl(k_grad_rows<TI, ME, 8>);
+    } else if (padded) {
       if (lpr == 16) launch(k_grad_values<TI, ME, 16, true>);
       else if (lpr == 32) launch(k_grad_values<TI, ME, 32, true>);
       else launch(k_grad_values<TI, ME, 64, true>);

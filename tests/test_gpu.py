@@ -160,6 +160,31 @@ def test_f32_fields_match_f64_output(refine):
         assert rel(xo, ref) < (1e-5 if refine == 0 else 1e-6)
 
 
+@pytest.mark.parametrize("k", [129, 301, 849, 1103])
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_gather_long_rows_against_reference(k, dtype):
+    """The gather of rows longer than 128 columns (whole passes of each row's
+    loads in flight, the last vector of a row read in place and masked):
+    gv[b][e] = -scale[b] sum_c y[b][row_e][c] x[b][col_e][c] against the same
+    sums in FP64 on the host, for widths that are not multiples of the vector
+    width and wider than one pass."""
+    _say(f"long-row gather k={k} {dtype}")
+    A, _ = laplacian_fixture(nx=20, n_rhs=1)
+    n, nb = A.shape[0], 2
+    slv = _solver(A, k, nbatch=nb)
+    rng = np.random.default_rng(k)
+    y = torch.tensor(rng.standard_normal((nb, n, k)), dtype=dtype, device=DEV)
+    x = torch.tensor(rng.standard_normal((nb, n, k)), dtype=dtype, device=DEV)
+    sc = torch.tensor([0.5, 2.0], dtype=torch.float64, device=DEV)
+    gv = slv.grad_values(y, x, scale=sc, permuted=True).cpu().numpy()
+    iperm = np.argsort(slv.perm())  # original row -> elimination row
+    Ac = A.tocoo()
+    yy, xx = y.double().cpu().numpy(), x.double().cpu().numpy()
+    ref = np.stack([-s * np.einsum("ec,ec->e", yy[b][iperm[Ac.row]], xx[b][iperm[Ac.col]])
+                    for b, s in enumerate([0.5, 2.0])])
+    assert rel(gv, ref) < 1e-14
+
+
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_factor_solve_matches_factorize_then_solve(precision):
     """dcx_factor_solve (factorisation overlapped with the forward sweep on a
