@@ -2391,6 +2391,11 @@ __global__ void k_reduce_ky(int nnz, int nky, const double* __restrict__ ky2,
 // grad_m[c] = -sigma[c] * sum over the cell's map entries (transposed CSR;
 // tle / tln: the map coefficients in the same cell-major order, read
 // contiguously)
+// four lanes per cell: lane j takes the cell's entries j, j + 4, ... (and the
+// boundary terms' wavenumbers j, j + 4, ...), so that a cell's loads are in
+// flight together instead of a chain of dependent loads; the lanes' sums are
+// added in a fixed order
+constexpr int kLmapLanes = 4;
 __global__ void k_lmap_t(int ncell, int nnz, int nky, const int* __restrict__ tptr,
                          const int* __restrict__ te,
                          const double* __restrict__ tle, const double* __restrict__ tln,
@@ -2399,20 +2404,23 @@ __global__ void k_lmap_t(int ncell, int nnz, int nky, const int* __restrict__ tp
                          int nbc, const double* __restrict__ ge, const double* __restrict__ gn,
                          const double* __restrict__ gv, const double* __restrict__ sigma,
                          double* __restrict__ grad_m) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= ncell) return;
+  const int c = (blockIdx.x * blockDim.x + threadIdx.x) / kLmapLanes;
+  const int j = threadIdx.x % kLmapLanes;
+  if (c >= ncell) return;  // uniform over the cell's lanes
   double acc = 0.0;
-  for (int t = tptr[c]; t < tptr[c + 1]; ++t) {
+  for (int t = tptr[c] + j; t < tptr[c + 1]; t += kLmapLanes) {
     const int e = te[t];
     acc += tle[t] * ge[e] + tln[t] * gn[e];
   }
   for (int t = tbptr[c]; t < tbptr[c + 1]; ++t) {
     const int q = tbq[t];
     const int e = bc_e[q];
-    for (int k = 0; k < nky; ++k)
+    for (int k = j; k < nky; k += kLmapLanes)
       acc += bcoef[static_cast<int64_t>(k) * nbc + q] * gv[static_cast<int64_t>(k) * nnz + e];
   }
-  grad_m[c] = -sigma[c] * acc;
+#pragma unroll
+  for (int off = 1; off < kLmapLanes; off <<= 1) acc += __shfl_xor(acc, off, kLmapLanes);
+  if (j == 0) grad_m[c] = -sigma[c] * acc;
 }
 
 // ---------------------------------------------------------------------------
@@ -4512,7 +4520,8 @@ int dcx_lmap_transpose(dcx_lmap* M, const double* gv, const double* sigma, doubl
   HIP_TRY(hipSetDevice(M->device));
   hipLaunchKernelGGL(k_reduce_ky, dim3(grid1(M->nnz, 256)), dim3(256), 0, st, M->nnz, M->nky,
                      M->d_ky2, M->d_grp, M->ngroup, gv, M->d_ge, M->d_gn);
-  hipLaunchKernelGGL(k_lmap_t, dim3(grid1(M->ncell, 256)), dim3(256), 0, st, M->ncell, M->nnz,
+  hipLaunchKernelGGL(k_lmap_t, dim3(grid1(static_cast<int64_t>(M->ncell) * kLmapLanes, 256)),
+                     dim3(256), 0, st, M->ncell, M->nnz,
                      M->nky, M->d_tptr, M->d_te, M->d_tle, M->d_tln, M->d_tbptr,
                      M->d_tbq, M->d_bc_e, M->d_bcoef, M->nbc, M->d_ge, M->d_gn, gv, sigma,
                      grad_m);
