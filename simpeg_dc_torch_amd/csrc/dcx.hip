@@ -9,6 +9,7 @@
 // Fronts and update rows are double-buffered by tree depth parity; each
 // depth level is one launch over (supernodes x batch [x column tiles]).
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include <algorithm>
 #include <cmath>
@@ -2779,16 +2780,20 @@ inline const double* factor_vals(const dcx_plan* P) {
 }
 
 template <typename T, int NT, int RPT, int NB = kNB>
-void factor_launch(dcx_plan* P, int d, const int* lvl, int cnt, size_t lds, hipStream_t st) {
+void factor_launch(dcx_plan* P, int d, const int* lvl, int cnt, size_t lds, hipStream_t st,
+                   hipEvent_t done = nullptr) {
   const auto& S = P->S;
   T* fr[2] = {static_cast<T*>(P->d_front[0]), static_cast<T*>(P->d_front[1])};
   const int64_t fbs[2] = {S.front_size[0], S.front_size[1]};
   dim3 grid(cnt, P->nbatch);
-  hipLaunchKernelGGL((k_factor_level<T, NT, RPT, NB>), grid, dim3(NT), lds, st, P->d_sup,
-                     lvl, P->d_relpos, P->d_cidx, P->d_a_e, P->d_a_row,
-                     P->d_a_col, factor_vals(P), static_cast<int64_t>(S.nnz), fr[d & 1], fbs[d & 1],
-                     fr[(d + 1) & 1], fbs[(d + 1) & 1], static_cast<T*>(P->d_panels),
-                     static_cast<T*>(P->d_panelsT), S.panel_size, P->d_info);
+  // `done`: an event that the launch itself signals when the kernel ends (no
+  // separate marker between this level and the next)
+  hipExtLaunchKernelGGL((k_factor_level<T, NT, RPT, NB>), grid, dim3(NT), lds, st, nullptr, done,
+                        0, P->d_sup, lvl, P->d_relpos, P->d_cidx, P->d_a_e, P->d_a_row,
+                        P->d_a_col, factor_vals(P), static_cast<int64_t>(S.nnz), fr[d & 1],
+                        fbs[d & 1], fr[(d + 1) & 1], fbs[(d + 1) & 1],
+                        static_cast<T*>(P->d_panels), static_cast<T*>(P->d_panelsT),
+                        S.panel_size, P->d_info);
 }
 
 // a level with fronts too tall for k_factor_level: grid-wide launches
@@ -2855,9 +2860,14 @@ int factor_level_big(dcx_plan* P, int d, int cnt, hipStream_t st) {
 
 // one launch per level; the threads (x rows per thread) cover the tallest
 // panel of the level
+// done (optional): an event the level's last launch signals at its end;
+// *signalled tells whether it did (levels through the grid-wide kernels and
+// empty levels do not: the caller records the event itself)
 template <typename T>
-int factor_level(dcx_plan* P, int d, hipStream_t st) {
+int factor_level(dcx_plan* P, int d, hipStream_t st, hipEvent_t done = nullptr,
+                 bool* signalled = nullptr) {
   const auto& S = P->S;
+  if (signalled) *signalled = false;
   const int cnt = S.lvl_ptr[d + 1] - S.lvl_ptr[d];
   if (!cnt) return DCX_OK;
   const int maxld = level_max_ld(S, d);
@@ -2867,12 +2877,13 @@ int factor_level(dcx_plan* P, int d, hipStream_t st) {
   // front for the trailing update
   static const int buckets = env_int("DCX_FACTOR_BUCKETS", 1);
   const bool wide = static_cast<int64_t>(cnt) * P->nbatch < P->wide_below;
+  if (signalled) *signalled = done != nullptr;
   if (!buckets) {
     const int* lvl = P->d_lvl + S.lvl_ptr[d];
     if (maxld <= 256 && !wide)
-      factor_launch<T, 256, 1>(P, d, lvl, cnt, lds, st);
+      factor_launch<T, 256, 1>(P, d, lvl, cnt, lds, st, done);
     else
-      factor_launch<T, 1024, 1>(P, d, lvl, cnt, lds, st);
+      factor_launch<T, 1024, 1>(P, d, lvl, cnt, lds, st, done);
     return DCX_OK;
   }
   // one launch per bucket of the level (fronts of one occupancy class), each
@@ -2882,17 +2893,18 @@ int factor_level(dcx_plan* P, int d, hipStream_t st) {
     const int b1 = t + 1 < P->fbkt_ptr[d + 1] ? P->fbkt[t + 1] : S.lvl_ptr[d + 1];
     const int bld = P->flvl_ld[b0];
     const int* lvl = P->d_flvl + b0;
+    hipEvent_t ev = t + 1 == P->fbkt_ptr[d + 1] ? done : nullptr;  // the level's last launch
     if (P->panel_width == 16) {
       const size_t l16 = factor_lds(bld, sizeof(T), 16);
-      if (bld <= 256 && !wide) factor_launch<T, 256, 1, 16>(P, d, lvl, b1 - b0, l16, st);
-      else factor_launch<T, 1024, 1, 16>(P, d, lvl, b1 - b0, l16, st);
+      if (bld <= 256 && !wide) factor_launch<T, 256, 1, 16>(P, d, lvl, b1 - b0, l16, st, ev);
+      else factor_launch<T, 1024, 1, 16>(P, d, lvl, b1 - b0, l16, st, ev);
       continue;
     }
     const size_t blds = factor_lds(bld, sizeof(T));
     if (bld <= 256 && !wide)
-      factor_launch<T, 256, 1>(P, d, lvl, b1 - b0, blds, st);
+      factor_launch<T, 256, 1>(P, d, lvl, b1 - b0, blds, st, ev);
     else
-      factor_launch<T, 1024, 1>(P, d, lvl, b1 - b0, blds, st);
+      factor_launch<T, 1024, 1>(P, d, lvl, b1 - b0, blds, st, ev);
   }
   return DCX_OK;
 }
@@ -3310,9 +3322,10 @@ int factor_and_sweeps(dcx_plan* P, T* W, int k, bool sparse_rhs, const SweepIO& 
   HIP_TRY(hipEventRecord(start, st));
   HIP_TRY(hipStreamWaitEvent(P->side, start, 0));
   for (int d = S.ndepth - 1; d >= 1; --d) {
-    rc = factor_level<T>(P, d, st);
+    bool signalled = false;
+    rc = factor_level<T>(P, d, st, P->lev_ev[d], &signalled);
     if (rc) return rc;
-    HIP_TRY(hipEventRecord(P->lev_ev[d], st));
+    if (!signalled) HIP_TRY(hipEventRecord(P->lev_ev[d], st));
     HIP_TRY(hipStreamWaitEvent(P->side, P->lev_ev[d], 0));
     rc = sweep_level<T>(P, W, k, true, d, act, io, P->side);
     if (rc) return rc;
