@@ -3318,12 +3318,16 @@ int factor_and_sweeps(dcx_plan* P, T* W, int k, bool sparse_rhs, const SweepIO& 
   rc = ensure_side(P);
   if (rc) return rc;
   const int* act = sparse_rhs && P->d_active ? P->d_active : nullptr;
-  hipEvent_t start = P->lev_ev[S.ndepth], join = P->lev_ev[S.ndepth + 1];
-  HIP_TRY(hipEventRecord(start, st));
-  HIP_TRY(hipStreamWaitEvent(P->side, start, 0));
+  // the side stream's first wait (the deepest level's event) orders it after
+  // everything queued on `st` before this call.  Outside stream capture the
+  // level events are signalled by the factor launches themselves; under
+  // capture they are recorded (a launch-signalled event is no capture
+  // dependency: the side stream's sweep would be left out of the graph)
+  const bool cap = capturing(st);
+  hipEvent_t join = P->lev_ev[S.ndepth + 1];
   for (int d = S.ndepth - 1; d >= 1; --d) {
     bool signalled = false;
-    rc = factor_level<T>(P, d, st, P->lev_ev[d], &signalled);
+    rc = factor_level<T>(P, d, st, cap ? nullptr : P->lev_ev[d], &signalled);
     if (rc) return rc;
     if (!signalled) HIP_TRY(hipEventRecord(P->lev_ev[d], st));
     HIP_TRY(hipStreamWaitEvent(P->side, P->lev_ev[d], 0));
