@@ -692,6 +692,27 @@ def summarize_trace(path, steps):
         for k, (c, t) in sorted(tab.items(), key=lambda kv: -kv[1][1]):
             print(f"| `{k}` | {c / steps:g} | {t / steps:.1f} | {t / c:.2f} |")
         print()
+    # the sweep-roofline reps (sweep timing on: one backward launch per tree
+    # level, no split sweep): the last 3 steps of the first mixed run after
+    # the timed one that an FP64 run follows (the electrode basis comes
+    # between), and the first 3 steps of that FP64 run
+    reps = 3
+    after = [r for r in runs if r[1] >= sel[2]]
+    pair = next(((a, b) for a, b in zip(after, after[1:])
+                 if a[0] == "mixed" and b[0] == "fp64" and a[2] - a[1] >= reps
+                 and b[2] - b[1] >= reps), None)
+    if pair is None:
+        return
+    print("### Sweep-roofline reps (sweep timing on, one backward launch per level)\n")
+    print("| kernel | steps | dispatches per step | average us per dispatch |")
+    print("|---|---|---|---|")
+    for fam_name, sl in (("k_bwd_wave<float>", steps_all[pair[0][2] - reps:pair[0][2]]),
+                         ("k_bwd_wave<double>", steps_all[pair[1][1]:pair[1][1] + reps])):
+        ds = [(int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3
+              for st in sl for r in st if short(r["Kernel_Name"]).startswith(fam_name[:-1])]
+        if ds:
+            print(f"| `{fam_name}` | {reps} | {len(ds) / reps:g} | {np.mean(ds):.2f} |")
+    print()
 
 
 def summarize_pmc(fetch_csv, write_csv, out_json):
@@ -768,7 +789,8 @@ def summarize_pmc(fetch_csv, write_csv, out_json):
 
 def pmc_pass(args):
     """Short run for hardware-counter passes: a 1 GiB device copy (the
-    counters' calibration) and two mixed eval+grad steps."""
+    counters' calibration) and two eval+grad steps of each engine (mixed,
+    then FP64), with sweep timing on as in the roofline's reps."""
     log("pmc pass: calibration copy of 1 GiB")
     a = torch.empty(1 << 27, dtype=torch.float64, device="cuda:0").fill_(1.0)
     b = torch.empty_like(a)
@@ -776,10 +798,15 @@ def pmc_pass(args):
     torch.cuda.synchronize()
     mesh, survey, m, e64, emx, mt, dobs, w, _ = build_engines(args.refine)
     # default kernel-variant thresholds: a tuning search here would add its
-    # trial runs to the per-dispatch counter averages
-    for _ in range(2):
-        eval_grad(emx, mt, dobs, w)
-    torch.cuda.synchronize()
+    # trial runs to the per-dispatch counter averages.  Sweep timing on, as in
+    # the roofline's timed reps: one backward launch per tree level (no split
+    # sweep), so a dispatch's bytes are a level's bytes
+    for eng in (emx, e64):
+        eng.solver.set_timing(True)
+        for _ in range(2):
+            eval_grad(eng, mt, dobs, w)
+        torch.cuda.synchronize()
+        eng.solver.set_timing(False)
     info = emx.solver.info()
     log(f"pmc pass done: nbatch={info['nbatch']} k={emx.nS} rows_nf={info['sweep_rows_nf']} "
         f"rows_ns={info['sweep_rows_ns']} panel={info['panel_size']}")
