@@ -141,6 +141,16 @@ int dcx_plan_get_perm(const dcx_plan* plan, int32_t* perm);
  * the last event) over the solves since timing was enabled. */
 int dcx_plan_set_timing(dcx_plan* plan, int enable);
 int dcx_plan_sweep_times(dcx_plan* plan, double* fwd_ms, double* bwd_ms, int* nsweeps);
+/* The plan's own streams beside the caller's (default on): the factorisation
+ * of dcx_factor_solve overlapped with the first forward sweep on a second
+ * stream, the split backward sweep (DCX_SPLIT_BWD) and deferred checks on a
+ * check stream.  Off (enable = 0), every kernel of a call goes on the caller's
+ * stream in order (DCX_SPLIT_BWD is ignored, a deferred check runs after the
+ * solve on that stream); for callers that run many plans on streams of their
+ * own, where each plan's two extra streams would share the device's few
+ * hardware queues with them.  It may be switched between any two calls.  It
+ * has no counterpart in the reference. */
+int dcx_plan_set_overlap(dcx_plan* plan, int enable);
 /* Kernel-variant thresholds of one plan (defaults from the environment at
  * dcx_plan_create): DCX_TUNE_SWEEP_TC, the widest column tile of the in-LDS
  * sweeps (16, 32, 64); DCX_TUNE_SPLITK_BELOW, backward-sweep levels with fewer

@@ -72,6 +72,7 @@ PROTOTYPES = {
     "dcx_plan_check_wait": (C.c_int, [_vp]),
     "dcx_plan_join": (C.c_int, [_vp, _vp]),
     "dcx_plan_set_timing": (C.c_int, [_vp, C.c_int]),
+    "dcx_plan_set_overlap": (C.c_int, [_vp, C.c_int]),
     "dcx_plan_set_tuning": (C.c_int, [_vp, C.c_int, C.c_int]),
     "dcx_plan_get_tuning": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_int)]),
     "dcx_plan_sweep_times": (C.c_int, [_vp, C.POINTER(C.c_double), C.POINTER(C.c_double),

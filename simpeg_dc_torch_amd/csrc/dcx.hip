@@ -2593,6 +2593,11 @@ struct dcx_plan {
   std::vector<hipEvent_t> lev_ev;
   // optional event timing of the sweeps (forward, backward) of every solve
   bool timing = false;
+  // the plan's own streams beside the caller's (dcx_plan_set_overlap): the
+  // factorisation overlapped with the first forward sweep, the split backward
+  // sweep and the deferred checks; off, a plan's work stays on the caller's
+  // stream in order
+  bool overlap = true;
   std::vector<hipEvent_t> ev;  // triples per sweep pair: start, mid, end
 
   ~dcx_plan() {
@@ -3308,7 +3313,7 @@ template <typename T>
 int factor_and_sweeps(dcx_plan* P, T* W, int k, bool sparse_rhs, const SweepIO& io,
                       hipStream_t st, bool split = false) {
   const auto& S = P->S;
-  if (S.nfused) {
+  if (S.nfused || !P->overlap) {
     int rc = launch_factor<T>(P, st);
     if (rc) return rc;
     return sweeps<T>(P, W, k, sparse_rhs, io, st);
@@ -3773,6 +3778,14 @@ int dcx_plan_set_timing(dcx_plan* P, int enable) {
   return DCX_OK;
 }
 
+int dcx_plan_set_overlap(dcx_plan* P, int enable) {
+  if (!P) return fail(DCX_ERR_ARG, "null argument");
+  // pending checks keep their events, and the rest of a split sweep stays
+  // joined through rest_ev: nothing to wait for here
+  P->overlap = enable != 0;
+  return DCX_OK;
+}
+
 int dcx_plan_sweep_times(dcx_plan* P, double* fwd_ms, double* bwd_ms, int* nsweeps) {
   if (!P || !fwd_ms || !bwd_ms || !nsweeps) return fail(DCX_ERR_ARG, "null argument");
   HIP_TRY(hipSetDevice(P->device));
@@ -4075,20 +4088,26 @@ static int start_check(dcx_plan* P, const double* B, int64_t bbs, void* xv, int 
   if (flags & DCX_CHECK_DEFER) {
     // on the plan's check stream, after the solve (and the rest of a split
     // backward sweep): the work the caller queues next on `st` does not wait
-    // for it.  At most two checks are pending; a third completes the oldest.
+    // for it.  Without the plan's own streams (dcx_plan_set_overlap) the
+    // check runs on `st` in order.  At most two checks are pending; a third
+    // completes the oldest.
     if (P->slot_count == 2)
       if (int rc = complete_pending(P)) return rc;
-    if (!P->chk_st) HIP_TRY(hipStreamCreateWithFlags(&P->chk_st, hipStreamNonBlocking));
+    if (P->overlap && !P->chk_st)
+      HIP_TRY(hipStreamCreateWithFlags(&P->chk_st, hipStreamNonBlocking));
+    hipStream_t cs = P->overlap ? P->chk_st : st;
     CheckSlot& C = P->slot[(P->slot_head + P->slot_count) % 2];
     if (!C.ev) {
       HIP_TRY(hipEventCreateWithFlags(&C.ev, hipEventDisableTiming));
       HIP_TRY(hipEventCreateWithFlags(&C.src, hipEventDisableTiming));
     }
-    HIP_TRY(hipEventRecord(C.src, st));
-    HIP_TRY(hipStreamWaitEvent(P->chk_st, C.src, 0));
-    if (P->rest_pending) HIP_TRY(hipStreamWaitEvent(P->chk_st, P->rest_ev, 0));
-    if (int rc = launch_check(P, C, B, bbs, xv, f32x, permuted, k, P->chk_st)) return rc;
-    HIP_TRY(hipEventRecord(C.ev, P->chk_st));
+    if (cs != st) {
+      HIP_TRY(hipEventRecord(C.src, st));
+      HIP_TRY(hipStreamWaitEvent(cs, C.src, 0));
+    }
+    if (P->rest_pending) HIP_TRY(hipStreamWaitEvent(cs, P->rest_ev, 0));
+    if (int rc = launch_check(P, C, B, bbs, xv, f32x, permuted, k, cs)) return rc;
+    HIP_TRY(hipEventRecord(C.ev, cs));
     C.B = B;
     C.bbs = bbs;
     C.xv = xv;
@@ -4132,7 +4151,7 @@ static int solve_impl(dcx_plan* P, const double* b, void* xv, int k, int flags, 
   // or FP32 fields without fixed refinement), no fused subtrees, no
   // row-grouped backward levels (their scratch is shared), no sweep timing
   const bool split = (flags & DCX_SPLIT_BWD) && sparse && P->d_inact_lvl && !S.nfused &&
-                     S.max_ns <= 128 && !P->timing &&
+                     S.max_ns <= 128 && !P->timing && P->overlap &&
                      (P->precision == DCX_FP64 ? permuted : (f32x && P->refine == 0));
   double* x = f32x ? nullptr : static_cast<double*>(xv);
   const int64_t blk = static_cast<int64_t>(S.n) * k;

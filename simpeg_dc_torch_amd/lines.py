@@ -146,11 +146,16 @@ class LineBatch:
     [(m, dobs, w) per line] to run `autotune_leaf` on one line per class;
     tiles: None, a {size_class: setting} dict, or "auto" (with `tune_data`:
     `autotune_tiles` on one line per class, its setting given to every line
-    of the class); further keywords (e.g. rhs_basis="electrodes") go to every
-    TorchDC2D."""
+    of the class); overlap: whether each line's plan uses its own extra
+    streams (factor / forward-sweep overlap, split backward sweep, check
+    stream; `set_overlap`), None = only when the batch runs on one stream
+    (several batch streams already overlap the lines, and each plan's two
+    extra streams would share the device's few hardware queues with them);
+    further keywords (e.g. rhs_basis="electrodes") go to every TorchDC2D."""
 
     def __init__(self, lines, device="cuda", nky=12, precision="mixed", refine=0, streams=4,
-                 leaf=None, tune_data=None, threads=False, tiles=None, **engine_kw):
+                 leaf=None, tune_data=None, threads=False, tiles=None, overlap=None,
+                 **engine_kw):
         self.device = torch.device(device)
         if self.device.type == "cuda" and self.device.index is None:
             self.device = torch.device("cuda", torch.cuda.current_device())
@@ -192,6 +197,16 @@ class LineBatch:
         self._pool = ThreadPoolExecutor(max_workers=len(self.streams)) if threads else None
         self._data = None
         self._graphs = None
+        self.overlap = overlap
+        self._overlap_set = None
+        self._apply_overlap()
+
+    def _apply_overlap(self):
+        want = self.overlap if self.overlap is not None else len(self.streams) == 1
+        if want != self._overlap_set:
+            for e in self.engines:
+                e._gpu.solver.set_overlap(want)
+            self._overlap_set = want
 
     def set_data(self, dobs, w):
         """Observed data and weights per line (kept on the device)."""
@@ -221,6 +236,7 @@ class LineBatch:
         library keeps, so nothing is allocated during capture)."""
         if self._data is None:
             raise RuntimeError("set_data first")
+        self._apply_overlap()
         graphs = []
         for i, eng in enumerate(self.engines):
             s = self._stream(i)
@@ -249,16 +265,19 @@ class LineBatch:
         return [e._gpu.solver.info()["buffer_generation"] for e in self.engines]
 
     def _issue(self, t, ms, out):
-        """The lines of stream t, one after another, on that stream."""
+        """The lines of stream t on that stream: every line's kernels queued
+        first, then their solve checks waited for."""
         s = self.streams[t]
         with torch.cuda.device(self.device), torch.cuda.stream(s):
-            for i in range(t, len(self.engines), len(self.streams)):
-                self._line(i, ms, out)
+            idx = range(t, len(self.engines), len(self.streams))
+            pending = [self._begin(i, ms) for i in idx]
+            for i, p in zip(idx, pending):
+                out[i] = p.finish()
 
-    def _line(self, i, ms, out):
+    def _begin(self, i, ms):
         x = torch.as_tensor(ms[i], dtype=torch.float64, device=self.device)
         dobs, w = self._data[i]
-        out[i] = self.engines[i].eval_grad_device(x, dobs, w)
+        return self.engines[i].eval_grad_device_begin(x, dobs, w)
 
     def eval_grad(self, ms):
         """[(phi, grad)] for the models `ms` (one per line), line i on stream
@@ -269,6 +288,8 @@ class LineBatch:
         call."""
         if self._data is None:
             raise RuntimeError("set_data first")
+        if self._graphs is None:
+            self._apply_overlap()
         cur = torch.cuda.current_stream(self.device)
         for s in self.streams:
             s.wait_stream(cur)
@@ -291,9 +312,14 @@ class LineBatch:
                           for t in range(len(self.streams))]:
                     f.result()
             else:
+                # every line queued before the host waits for any line's checks
+                pending = []
                 for i in range(len(self.engines)):
                     with torch.cuda.stream(self._stream(i)):
-                        self._line(i, ms, out)
+                        pending.append(self._begin(i, ms))
+                for i, p in enumerate(pending):
+                    with torch.cuda.stream(self._stream(i)):
+                        out[i] = p.finish()
             for phi, grad in out:
                 phi.record_stream(cur)
                 grad.record_stream(cur)

@@ -325,15 +325,33 @@ def _misfit_body_chunked(eng, m_full, dobs, w, want_grad, width):
     return (r ** 2).sum(), d, (g.lmap_t(gv, sig) if want_grad else None), r
 
 
+def _drive(steps):
+    """Run a generator of `_misfit_steps` form to its end; its return value."""
+    try:
+        while True:
+            next(steps)
+    except StopIteration as e:
+        return e.value
+
+
 def _misfit_body(eng, m_full, dobs, w, want_grad):
+    """`_misfit_steps` run to its end: (phi, d, grad_full, r)."""
+    return _drive(_misfit_steps(eng, m_full, dobs, w, want_grad))
+
+
+def _misfit_steps(eng, m_full, dobs, w, want_grad):
     """phi = ||w (dpred - dobs)||^2 and, with want_grad, dphi/dm_full, in one
     pass on the device: the forward solve, the projection, the misfit, the
     adjoint solve and the gathers, with the solves' checks deferred and
     waited for where their results are used (the forward's before the adjoint
     solve, the adjoint's at the end), so the host never waits for the device
     mid-pass; a system refactorised in FP64 by a failing check has everything
-    computed from its fields computed again.  Returns (phi, d, grad_full, r)
-    with r = w (d - dobs)."""
+    computed from its fields computed again.  A generator: it yields once,
+    when every kernel of the pass is queued and the host is about to wait for
+    the solves' checks, so that a caller driving several engines can queue
+    all of their passes before it waits for any (it resumes each on the
+    stream and device it started on; no context is held across the yield).
+    Its return value is (phi, d, grad_full, r) with r = w (d - dobs)."""
     g = eng._gpu
     sv = g.solver
     if eng.rhs_basis != "electrodes":
@@ -357,6 +375,7 @@ def _misfit_body(eng, m_full, dobs, w, want_grad):
             H = _rows_times(g, G, (T @ g.S.T).T)
             return d, r, sv.grad_values(G, H, scale=g.wq, permuted=True)
         d, r, gv = rest()
+        yield
         if sv.check_wait():
             d, r, gv = rest()
     else:
@@ -382,10 +401,12 @@ def _misfit_body(eng, m_full, dobs, w, want_grad):
         r, G = project()
         gv = None
         if not want_grad:
+            yield
             if sv.check_wait():
                 r, _ = project()
         else:
             Lam, gv = adjoint(G)
+            yield
             if sv.check_wait():          # the forward's: U changed, all again
                 sv.check_discard()       # the adjoint's, of a superseded solve
                 r, G = project()
@@ -442,12 +463,43 @@ def _eval_grad_device(eng, m, dobs, w):
     if eng._gpu is None:
         raise RuntimeError("eval_grad_device needs a GPU engine")
     with torch.no_grad(), torch.cuda.device(eng.device):
-        m_full = eng._model_to_full(torch.as_tensor(m).detach()).to(torch.float64)
-        dobs = torch.as_tensor(dobs, dtype=torch.float64, device=eng.device)
-        w = torch.as_tensor(w, dtype=torch.float64, device=eng.device)
-        phi, _, grad_full, _ = _misfit_body(eng, m_full, dobs, w, True)
-        grad = eng._model_from_full(grad_full)
-    return phi, grad
+        return _drive(_eval_grad_device_steps(eng, m, dobs, w))
+
+
+class PendingEvalGrad:
+    """An eval+grad whose kernels are all queued and whose solve checks are
+    not yet waited for (`eval_grad_device_begin`); `finish()` waits for the
+    checks (recomputing what a failing check's FP64 re-solve changed) and
+    returns (phi, grad).  Finish it on the stream it was begun on."""
+
+    def __init__(self, eng, m, dobs, w):
+        self._eng = eng
+        self._steps = _eval_grad_device_steps(eng, m, dobs, w)
+        self._out = None
+        self._step()
+
+    def _step(self):
+        with torch.no_grad(), torch.cuda.device(self._eng.device):
+            try:
+                next(self._steps)
+            except StopIteration as e:
+                self._out, self._steps = e.value, None
+
+    def finish(self):
+        while self._steps is not None:
+            self._step()
+        return self._out
+
+
+def _eval_grad_device_steps(eng, m, dobs, w):
+    """`_eval_grad_device` as a generator of `_misfit_steps` form (one yield,
+    before the host waits for the checks); returns (phi, grad).  The caller
+    runs each step under torch.no_grad() and the engine's device."""
+    m_full = eng._model_to_full(torch.as_tensor(m).detach()).to(torch.float64)
+    dobs = torch.as_tensor(dobs, dtype=torch.float64, device=eng.device)
+    w = torch.as_tensor(w, dtype=torch.float64, device=eng.device)
+    phi, _, grad_full, _ = yield from _misfit_steps(eng, m_full, dobs, w, True)
+    return phi, eng._model_from_full(grad_full)
 
 
 class _Forward2D(torch.autograd.Function):
@@ -696,6 +748,16 @@ class TorchDC2D:
         engine runs every CUDA backward of a device on one worker thread).
         GPU engines only."""
         return _eval_grad_device(self, m, dobs, w)
+
+    def eval_grad_device_begin(self, m, dobs, w):
+        """`eval_grad_device` in two halves: every kernel of the pass is
+        queued on the current stream and the host has not waited for
+        anything yet; the returned PendingEvalGrad's `finish()` waits for the
+        solves' checks and gives (phi_d, dphi_d/dm).  A caller driving several
+        engines from one thread begins all of them before it finishes any."""
+        if self._gpu is None:
+            raise RuntimeError("eval_grad_device_begin needs a GPU engine")
+        return PendingEvalGrad(self, m, dobs, w)
 
     # -- chunked reverse passes ---------------------------------------------------
     def _gpu_reverse(self, m, weights_of, chunk):

@@ -22,8 +22,8 @@ import scipy.sparse as sp
 import torch
 
 from .operators3d import Problem3DMaps
-from .simulation2d import (_Forward2D, _ForwardElectrodes2D, _GpuState, _eval_grad_device,
-                           _gpu_misfit)
+from .simulation2d import (PendingEvalGrad, _Forward2D, _ForwardElectrodes2D, _GpuState,
+                           _eval_grad_device, _gpu_misfit)
 from .solver import make_solver, sparse_solve
 
 
@@ -186,6 +186,13 @@ class TorchDC3D:
         bit-identical to `misfit(...).backward()`, on the calling thread's
         current stream.  GPU engines only."""
         return _eval_grad_device(self, m, dobs, w)
+
+    def eval_grad_device_begin(self, m, dobs, w):
+        """`eval_grad_device` in two halves (kernels queued now, the solve
+        checks waited for by the returned PendingEvalGrad's `finish()`)."""
+        if self._gpu is None:
+            raise RuntimeError("eval_grad_device_begin needs a GPU engine")
+        return PendingEvalGrad(self, m, dobs, w)
 
     def misfit_and_grad(self, m, dobs, w):
         """(phi_d, dphi_d/dm) by one reverse pass."""

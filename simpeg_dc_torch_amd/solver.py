@@ -148,6 +148,14 @@ class HipBatchSolver:
         """Record HIP events around the sweeps of every solve (on its stream)."""
         _lib.check(self._lib.dcx_plan_set_timing(self._h, int(bool(enable))), "set_timing")
 
+    def set_overlap(self, enable=True):
+        """The plan's own streams beside the caller's (default on): factor /
+        forward-sweep overlap, split backward sweep, deferred checks on a
+        check stream.  Off, every kernel goes on the caller's stream in order
+        (for many plans driven on streams of their own, which would otherwise
+        share the device's few hardware queues with the plans' extra ones)."""
+        _lib.check(self._lib.dcx_plan_set_overlap(self._h, int(bool(enable))), "set_overlap")
+
     def sweep_times(self):
         """(forward ms, backward ms, number of solves) since timing was enabled."""
         f, b, n = C.c_double(), C.c_double(), C.c_int()

@@ -615,11 +615,14 @@ def test_lbfgs_inversion_protocol_recovers_block():
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_eval_grad_device_equals_autograd(precision, basis):
     """TorchDC2D.eval_grad_device (no autograd graph, no host sync) gives the
-    autograd misfit and gradient bit for bit, on the active-cell engine too."""
+    autograd misfit and gradient bit for bit, on the active-cell engine too;
+    so does the two-phase form with both engines' passes queued before
+    either is finished (eval_grad_device_begin, then finish)."""
     from simpeg_dc_torch_amd import TorchDC2D
     _say(f"eval_grad_device {precision} {basis}")
     mesh, survey, m = case("suite")
     act = mesh.cell_centers[:, 1] < -1.0
+    runs = []
     for active in (None, act):
         eng = TorchDC2D(mesh, survey, nky=6, device=DEV, precision=precision, refine=0,
                         rhs_basis=basis, active_cells=active, val_inactive=float(np.log(100.0)))
@@ -633,6 +636,39 @@ def test_eval_grad_device_equals_autograd(precision, basis):
         phi.backward()
         phi2, g2 = eng.eval_grad_device(torch.tensor(mm, device=DEV), dobs_t, w_t)
         assert torch.equal(phi.detach(), phi2) and torch.equal(x.grad, g2)
+        runs.append((eng, torch.tensor(mm, device=DEV), dobs_t, w_t, phi.detach(), x.grad))
+    pending = [eng.eval_grad_device_begin(mm, dobs_t, w_t) for eng, mm, dobs_t, w_t, _, _ in runs]
+    for p, (_, _, _, _, phi, g) in zip(pending, runs):
+        phi3, g3 = p.finish()
+        assert torch.equal(phi, phi3) and torch.equal(g, g3)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_plan_without_overlap_matches(precision):
+    """A plan with its extra streams off (dcx_plan_set_overlap: factorisation,
+    split backward sweep and deferred checks all on the caller's stream)
+    gives the eval+grad of the default plan to rounding (the unsplit backward
+    sweep may pick other kernel variants), with its checks passed; switched
+    back on, the default results again, bit for bit."""
+    from simpeg_dc_torch_amd import TorchDC2D
+    _say(f"plan without overlap {precision}")
+    mesh, survey, m = case("suite")
+    eng = TorchDC2D(mesh, survey, nky=6, device=DEV, precision=precision, refine=0)
+    mt = torch.tensor(m, device=DEV)
+    with torch.no_grad():
+        d = eng.dpred(mt).cpu().numpy()
+    dobs, w = (torch.tensor(a, device=DEV) for a in noisy_data(d, seed=5))
+    sv = eng.solver
+    out = []
+    for ov in (True, False, True):
+        sv.set_overlap(ov)
+        phi, g = eng.eval_grad_device(mt, dobs, w)
+        torch.cuda.synchronize()
+        assert sv.check_report()["status"] == "passed" and sv.fallbacks == []
+        out.append((float(phi), g.cpu().numpy()))
+    tol = 1e-12 if precision == "fp64" else 1e-5
+    assert out[1][0] == pytest.approx(out[0][0], rel=tol) and rel(out[1][1], out[0][1]) < tol
+    assert out[2][0] == out[0][0] and np.array_equal(out[2][1], out[0][1])
 
 
 def test_eval_grad_device_equals_autograd_3d():
@@ -709,6 +745,8 @@ def test_fields_budget_chunks_the_eval_grad(precision, dim):
         phi = eng.misfit(x, dobs, w)
         phi.backward()
         p2, g2 = eng.eval_grad_device(mt, dobs, w)
+        p4, g4 = eng.eval_grad_device_begin(mt, dobs, w).finish()   # chunked: no yield
+        assert torch.equal(p4, p2) and torch.equal(g4, g2)
         with torch.no_grad():
             p3 = eng.misfit(mt, dobs, w)
         res.append((float(phi), x.grad.cpu().numpy(), float(p2), g2.cpu().numpy(), float(p3)))
