@@ -1304,6 +1304,17 @@ __device__ inline void load_cols(const T* p, int navail, T (&out)[V]) {
   }
 }
 
+// x with its sign flipped where m holds the sign bit (m is 0 or 0x80000000):
+// one integer xor where the value is used.  A load negated in place would
+// make the wavefront wait for it right there, before the loads that follow
+// it are issued.
+__device__ inline float flip_sign(float x, uint32_t m) {
+  return __uint_as_float(__float_as_uint(x) ^ m);
+}
+__device__ inline double flip_sign(double x, uint32_t m) {
+  return __longlong_as_double(__double_as_longlong(x) ^ (static_cast<long long>(m) << 32));
+}
+
 template <typename T, int MT, int V, int KS, int PF>
 __global__ __launch_bounds__(kThreads) void k_bwd_wave(
     const SupDev* __restrict__ sup, const int* __restrict__ lvl,
@@ -1356,21 +1367,22 @@ __global__ __launch_bounds__(kThreads) void k_bwd_wave(
     for (int v = 0; v < V; ++v) acc[t][v] = {T(0), T(0), T(0), T(0)};
   for (int k0 = kbeg; k0 < (rows ? kend : 0); k0 += 4 * PF) {
     T bv[PF][V], av[PF][MT];
+    uint32_t sg[PF];  // the R rows enter negated: sign flipped at the product
 #pragma unroll
     for (int j = 0; j < PF; ++j) {
       const int kk = k0 + 4 * j + kq;
       const bool kok = kk < kend;
+      // one load path for J and R rows (row chosen by a select): loads into
+      // the same registers on two branches would make the second branch wait
+      // for the first one's loads
+      const int kr = kk - ns;
+      const int ri = kr >= 0 && kok ? rl[kr] : 0;
+      const int row = kr >= 0 ? ri : S.first + kk;
+      sg[j] = kr >= 0 ? 0x80000000u : 0u;
 #pragma unroll
       for (int v = 0; v < V; ++v) bv[j][v] = T(0);
-      if (kok && navail > 0) {
-        if (kk < ns) {
-          if (act) load_cols<T, V>(Wb + static_cast<int64_t>(S.first + kk) * k + cl, navail, bv[j]);
-        } else {
-          load_cols<T, V>(Wb + static_cast<int64_t>(rl[kk - ns]) * k + cl, navail, bv[j]);
-#pragma unroll
-          for (int v = 0; v < V; ++v) bv[j][v] = -bv[j][v];
-        }
-      }
+      if (kok && navail > 0 && (kr >= 0 || act))
+        load_cols<T, V>(Wb + static_cast<int64_t>(row) * k + cl, navail, bv[j]);
       // L_JJ^{-1} (the J rows of P_s) is lower triangular: k-steps wholly
       // above a tile's first row multiply zeros and are skipped, and so are
       // tiles wholly past this supernode's ns rows (the level's tile count
@@ -1386,11 +1398,14 @@ __global__ __launch_bounds__(kThreads) void k_bwd_wave(
 #pragma unroll
     for (int j = 0; j < PF; ++j)
       if (k0 + 4 * j < kend) {
+        T b[V];
+#pragma unroll
+        for (int v = 0; v < V; ++v) b[v] = flip_sign(bv[j][v], sg[j]);
 #pragma unroll
         for (int t = 0; t < MT; ++t)
           if (KS > 1 || (k0 + 4 * j + 4 > r0 + 16 * t && r0 + 16 * t < ns)) {
 #pragma unroll
-            for (int v = 0; v < V; ++v) acc[t][v] = Mma<T>::op(av[j][t], bv[j][v], acc[t][v]);
+            for (int v = 0; v < V; ++v) acc[t][v] = Mma<T>::op(av[j][t], b[v], acc[t][v]);
           }
       }
   }
