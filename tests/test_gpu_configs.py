@@ -6,6 +6,9 @@
   1e-10 (the reference's parity bar, dctorch/tests/test_simulation2d.py:80),
   and the benchmark's own mixed setting (FP32 factor and fields, no fixed
   refinement pass, source basis) with dpred to 1e-6.
+* The reference's CPU-runnable case (4,800 cells, nky = 11 SimPEG
+  quadrature): FP64 against the oracle and the cpu engine to 1e-10, mixed
+  dpred to 1e-6.
 * The 62,016-cell Gauss-Newton line (270 sources, 2,430 data, 9
   wavenumbers): rows of the explicit J against the oracle's, J rows equal
   to J^T e_i through the engine's own adjoint (dctorch/tests/
@@ -75,6 +78,48 @@ def test_line25k_full_parity():
     assert abs(phi_m / phi_o - 1) < 1e-5
     assert rel(g_m, g_o) < 1e-3
     assert rep["status"] == "passed" and emx.solver.fallbacks == []
+
+
+def test_reference_case_parity():
+    """The reference's CPU-runnable case as stated (80 x 40 core of 2.5 m
+    cells plus padding, 4,800 cells; 21 electrodes dipole-dipole n = 1..6;
+    two Gaussian blobs; nky = 11 SimPEG quadrature; dobs = dpred (1 + 0.05
+    N(0, 1)) seed 0): FP64 dpred, misfit and gradient on the GPU against the
+    oracle to 1e-10, the cpu engine (the reference chain, SciPy SuperLU) to
+    the same bar, and the mixed engine's dpred to 1e-6."""
+    from simpeg_dc_torch_amd import TorchDC2D, workloads
+    mesh, survey, m = workloads.small_padded_case()
+    _say("reference case: FP64 engine, nky 11 SimPEG quadrature")
+    e64 = TorchDC2D(mesh, survey, nky=11, device=DEV, quadrature="simpeg")
+    assert (e64.nC, e64.nky, e64.nD) == (4800, 11, 93)
+    O = oracle_for(mesh, survey, e64.kys, e64.wq_np)
+    d_o = O.dpred(m)
+    rng = np.random.default_rng(0)
+    dobs = d_o * (1 + 0.05 * rng.standard_normal(d_o.size))
+    w = 1.0 / (0.05 * np.abs(dobs))
+    phi_o, g_o, _ = O.misfit_and_grad(m, dobs, w)
+    mt = torch.tensor(m, device=DEV)
+    dt, wt = torch.tensor(dobs, device=DEV), torch.tensor(w, device=DEV)
+    with torch.no_grad():
+        d64 = e64.dpred(mt).cpu().numpy()
+    phi, g = _eval_grad(e64, mt, dt, wt)
+    _say(f"FP64: dpred {rel(d64, d_o):.1e} phi {abs(phi / phi_o - 1):.1e} grad {rel(g, g_o):.1e}")
+    assert rel(d64, d_o) < 1e-10
+    assert abs(phi / phi_o - 1) < 1e-10
+    assert rel(g, g_o) < 1e-10
+    ecpu = TorchDC2D(mesh, survey, nky=11, device="cpu", quadrature="simpeg")
+    assert np.array_equal(ecpu.kys, e64.kys)
+    phi_c, g_c = _eval_grad(ecpu, torch.tensor(m), torch.tensor(dobs), torch.tensor(w))
+    assert abs(phi / phi_c - 1) < 1e-10
+    assert rel(g, g_c) < 1e-10
+    emx = TorchDC2D(mesh, survey, nky=11, device=DEV, quadrature="simpeg", precision="mixed",
+                    refine=0)
+    with torch.no_grad():
+        dmx = emx.dpred(mt).cpu().numpy()
+    phi_m, g_m = _eval_grad(emx, mt, dt, wt)
+    _say(f"mixed: dpred {rel(dmx, d_o):.1e} grad {rel(g_m, g_o):.1e}")
+    assert rel(dmx, d_o) < 1e-6
+    assert rel(g_m, g_o) < 1e-3
 
 
 def test_gn_line_jacobian_and_step():

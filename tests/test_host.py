@@ -206,6 +206,32 @@ def test_cpu_engine_dpred_and_gradient():
     assert rel(g.numpy(), mt.grad.numpy()) < 1e-12
 
 
+def _config1_data(d):
+    """dobs = dpred (1 + 0.05 N(0, 1)), seed 0; w = 1 / (0.05 |dobs|)."""
+    rng = np.random.default_rng(0)
+    dobs = d * (1 + 0.05 * rng.standard_normal(d.size))
+    return dobs, 1.0 / (0.05 * np.abs(dobs))
+
+
+def test_cpu_engine_reference_case():
+    """The reference's CPU-runnable case as stated: the 80 x 40 core of 2.5 m
+    cells with 8 (10 at depth) padding cells growing 1.3x (4,800 cells), 21
+    electrodes dipole-dipole n = 1..6, two Gaussian blobs, nky = 11 SimPEG
+    quadrature, dobs = dpred (1 + 0.05 N(0, 1)) seed 0: dpred, misfit and
+    gradient of the cpu engine (the reference chain) against the oracle."""
+    eng, O, m = _engine_and_oracle("small_padded", nky=11, quadrature="simpeg")
+    assert (eng.nC, eng.nky, eng.nD) == (4800, 11, 93)
+    d_o = O.dpred(m)
+    dobs, w = _config1_data(d_o)
+    mt = torch.tensor(m, requires_grad=True)
+    assert rel(eng.dpred(mt).detach().numpy(), d_o) < 1e-10
+    phi = eng.misfit(mt, torch.tensor(dobs), torch.tensor(w))
+    phi.backward()
+    phi_o, g_o, _ = O.misfit_and_grad(m, dobs, w)
+    assert float(phi) == pytest.approx(phi_o, rel=1e-10)
+    assert rel(mt.grad.numpy(), g_o) < 1e-10
+
+
 def test_cpu_engine_jtvec_jmatrix_jvec():
     eng, O, m = _engine_and_oracle("suite", nky=4)
     mt = torch.tensor(m)
