@@ -221,12 +221,12 @@ __device__ void front_init(const SupDev& S, int b, const SupDev* __restrict__ su
                            const int* __restrict__ a_e, const int* __restrict__ a_row,
                            const int* __restrict__ a_col, const double* __restrict__ vals,
                            int64_t nnz, T* __restrict__ F, const T* __restrict__ fprev,
-                           int64_t fprev_bs, int* rpl, int zero_lower) {
+                           int64_t fprev_bs, int* rpl) {
   constexpr int kU = kGather;
   const int tid = threadIdx.x;
   const int ns = S.ns, nf = S.ns + S.rs, ld = nf + S.ns;
   const int64_t fsz = static_cast<int64_t>(ld) * nf;
-  if (zero_lower) {
+  {
     // the strict upper triangle of the J/R rows is never read (panel loads
     // use the diagonal block's lower triangle, the trailing update and the
     // extend-add write i >= l only), so it is not written either: the front's
@@ -242,8 +242,6 @@ __device__ void front_init(const SupDev& S, int b, const SupDev* __restrict__ su
         ++zc;
       }
     }
-  } else {
-    for (int64_t i = tid; i < fsz; i += NT) F[i] = T(0);
   }
   __syncthreads();
   for (int j = tid; j < ns; j += NT) F[(nf + j) + static_cast<int64_t>(j) * ld] = T(1);
@@ -456,7 +454,7 @@ __global__ __launch_bounds__(NT) void k_factor_level(
     const int* __restrict__ a_col, const double* __restrict__ vals, int64_t nnz,
     T* __restrict__ fcur, int64_t fcur_bs, const T* __restrict__ fprev,
     int64_t fprev_bs, T* __restrict__ panels, T* __restrict__ panelsT, int64_t panel_bs,
-    int* __restrict__ info, int zero_lower) {
+    int* __restrict__ info) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   constexpr int LDP = NB + 4, LDG = NB + 4;
   const int s = lvl[blockIdx.x];
@@ -470,7 +468,7 @@ __global__ __launch_bounds__(NT) void k_factor_level(
   T* pan = dgi + NB;                         // panel rows, stride LDP
   T* F = fcur + b * fcur_bs + S.front_off;
   front_init<T, NT>(S, b, sup, relpos, cidx, a_e, a_row, a_col, vals, nnz, F, fprev, fprev_bs,
-                    reinterpret_cast<int*>(pan), zero_lower);
+                    reinterpret_cast<int*>(pan));
   for (int j0 = 0; j0 < ns; j0 += NB) {
     const int jb = min(NB, ns - j0);
     // panel rows: J, R and E rows from j0 on.  E row e (front row nf + e)
@@ -1109,9 +1107,6 @@ __device__ inline SweepBlock sweep_block(const int* lvl, int cnt, int nzb, int t
 // update buffer of this depth.  PL: panel staged in LDS, `ct` column tiles
 // per workgroup.
 constexpr int kFwdKids = 8;  // children whose extend-add data k_fwd_level stages
-constexpr int kFwdMT = 4;    // row tiles of a wavefront in one k-loop (mt != 0)
-constexpr int kFwdPF = 4;    // k-steps of operands loaded ahead in that loop
-constexpr int kFwdGather = 8;  // staging loads in flight per thread (mt != 0)
 template <typename T, int TC, bool PL>
 __global__ __launch_bounds__(kThreads) void k_fwd_level(
     const SupDev* __restrict__ sup, const int* __restrict__ lvl,
@@ -1121,7 +1116,7 @@ __global__ __launch_bounds__(kThreads) void k_fwd_level(
     const int* __restrict__ brow, T* __restrict__ W, int64_t w_bs, int k,
     T* __restrict__ ucur, int64_t ucur_bs, const T* __restrict__ uprev,
     int64_t uprev_bs, const T* __restrict__ ufz, int64_t ufz_bs, int ct, int pmax, int cnt,
-    int nzb, int total, int mt) {
+    int nzb, int total) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const SweepBlock sb = sweep_block(lvl, cnt, nzb, total);
   if (!sb.live) return;
@@ -1162,40 +1157,14 @@ __global__ __launch_bounds__(kThreads) void k_fwd_level(
     const int col0 = (sb.z * ct + tile) * TC;
     if (col0 >= k) break;
     __syncthreads();
-    if (mt) {
-      // b_J: kFwdGather entries per thread with their loads in flight
-      // together (one round trip per batch, not per entry); the R rows of f
-      // start at zero
-      for (int idx = ns * TC + tid; idx < nf * TC; idx += kThreads) f[idx] = T(0);
-      const int nj = ns * TC;
-      for (int i0 = tid; i0 < nj; i0 += kThreads * kFwdGather) {
-        double v[kFwdGather];
-#pragma unroll
-        for (int u = 0; u < kFwdGather; ++u) {
-          const int idx = i0 + u * kThreads;
-          const int r = idx / TC, c = idx % TC;
-          v[u] = 0.0;
-          if (idx < nj && col0 + c < k) {
-            const int gr = brow ? brow[S.first + r] : S.first + r;
-            v[u] = Bb[static_cast<int64_t>(gr) * k + col0 + c];
-          }
-        }
-#pragma unroll
-        for (int u = 0; u < kFwdGather; ++u) {
-          const int idx = i0 + u * kThreads;
-          if (idx < nj) f[idx] = static_cast<T>(v[u]);
-        }
+    for (int idx = tid; idx < nf * TC; idx += kThreads) {
+      const int r = idx / TC, c = idx % TC;
+      T v = T(0);
+      if (r < ns && col0 + c < k) {
+        const int gr = brow ? brow[S.first + r] : S.first + r;
+        v = static_cast<T>(Bb[static_cast<int64_t>(gr) * k + col0 + c]);
       }
-    } else {
-      for (int idx = tid; idx < nf * TC; idx += kThreads) {
-        const int r = idx / TC, c = idx % TC;
-        T v = T(0);
-        if (r < ns && col0 + c < k) {
-          const int gr = brow ? brow[S.first + r] : S.first + r;
-          v = static_cast<T>(Bb[static_cast<int64_t>(gr) * k + col0 + c]);
-        }
-        f[idx] = v;
-      }
+      f[idx] = v;
     }
     __syncthreads();
     for (int q = 0; q < nkid; ++q) {
@@ -1216,102 +1185,11 @@ __global__ __launch_bounds__(kThreads) void k_fwd_level(
       }
       if (!rsc) continue;  // inactive child (uniform)
       const int* rp = relpos + rptr;
-      if (mt) {
-        // kFwdGather entries per thread, their index and value loads issued
-        // before any is added (one child's positions are distinct)
-        const int nu = rsc * TC;
-        for (int i0 = tid; i0 < nu; i0 += kThreads * kFwdGather) {
-          T uv[kFwdGather];
-          int pos[kFwdGather];
-#pragma unroll
-          for (int e = 0; e < kFwdGather; ++e) {
-            const int idx = i0 + e * kThreads;
-            const int r = idx / TC, cl = idx % TC;
-            pos[e] = -1;
-            if (idx < nu && col0 + cl < k) {
-              pos[e] = rp[r] * TC + cl;
-              uv[e] = u[static_cast<int64_t>(r) * k + col0 + cl];
-            }
-          }
-#pragma unroll
-          for (int e = 0; e < kFwdGather; ++e)
-            if (pos[e] >= 0) f[pos[e]] += uv[e];
-        }
-      } else {
-        for (int idx = tid; idx < rsc * TC; idx += kThreads) {
-          const int r = idx / TC, cl = idx % TC;
-          if (col0 + cl < k) f[rp[r] * TC + cl] += u[static_cast<int64_t>(r) * k + col0 + cl];
-        }
+      for (int idx = tid; idx < rsc * TC; idx += kThreads) {
+        const int r = idx / TC, cl = idx % TC;
+        if (col0 + cl < k) f[rp[r] * TC + cl] += u[static_cast<int64_t>(r) * k + col0 + cl];
       }
       __syncthreads();
-    }
-    if (mt) {
-      // every row tile of a wavefront (tiles wave, wave + 4, ...; up to
-      // kFwdMT at a time) and every column tile of the block in one k-loop,
-      // so that the panel loads of all of them are in flight together: one
-      // chain of dependent L2 round trips per wavefront instead of one per
-      // output tile (a depth-1 front of the 25k-cell line has 30 tiles)
-      const int ntr = (nf + 15) / 16;
-      for (int g0 = wave; g0 < ntr; g0 += 4 * kFwdMT) {
-        int kt[kFwdMT];
-        int kend = 0;
-#pragma unroll
-        for (int m = 0; m < kFwdMT; ++m) {
-          const int tr = (g0 + 4 * m) * 16;
-          // the J rows of P_s hold L_JJ^{-1}, lower triangular: row tile tr
-          // needs columns up to its last row only
-          kt[m] = tr >= nf ? 0 : tr < ns ? min(ns, tr + 16) : ns;
-          kend = max(kend, kt[m]);
-        }
-        typename Mma<T>::v4 acc[kFwdMT][NCT];
-#pragma unroll
-        for (int m = 0; m < kFwdMT; ++m)
-#pragma unroll
-          for (int c = 0; c < NCT; ++c) acc[m][c] = {T(0), T(0), T(0), T(0)};
-        const int kq = lane >> 4;
-        for (int k0 = 0; k0 < kend; k0 += 4 * kFwdPF) {
-          T av[kFwdPF][kFwdMT], bv[kFwdPF][NCT];
-#pragma unroll
-          for (int j = 0; j < kFwdPF; ++j) {
-            const int kk = k0 + 4 * j + kq;
-#pragma unroll
-            for (int m = 0; m < kFwdMT; ++m) {
-              const int rr = (g0 + 4 * m) * 16 + (lane & 15);
-              av[j][m] = (kk < kt[m] && rr < nf) ? Pa(rr, kk) : T(0);
-            }
-#pragma unroll
-            for (int c = 0; c < NCT; ++c)
-              bv[j][c] = kk < kend ? f[kk * TC + c * 16 + (lane & 15)] : T(0);
-          }
-#pragma unroll
-          for (int j = 0; j < kFwdPF; ++j)
-#pragma unroll
-            for (int m = 0; m < kFwdMT; ++m)
-              if (k0 + 4 * j < kt[m]) {  // uniform over the wavefront
-#pragma unroll
-                for (int c = 0; c < NCT; ++c) acc[m][c] = Mma<T>::op(av[j][m], bv[j][c], acc[m][c]);
-              }
-        }
-#pragma unroll
-        for (int m = 0; m < kFwdMT; ++m) {
-          const int tr = (g0 + 4 * m) * 16;
-          if (tr >= nf) continue;  // uniform
-#pragma unroll
-          for (int cti = 0; cti < NCT; ++cti) {
-            const int c = cti * 16 + (lane & 15);
-            if (col0 + c >= k) continue;
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-              const int r = tr + Mma<T>::row(lane, i);
-              if (r < ns)
-                Wb[static_cast<int64_t>(S.first + r) * k + col0 + c] = acc[m][cti][i];
-              else if (r < nf)
-                uo[static_cast<int64_t>(r - ns) * k + col0 + c] = f[r * TC + c] - acc[m][cti][i];
-            }
-          }
-        }
-      }
-      continue;
     }
     for (int t = wave; t < nt; t += 4) {
       const int tr = (t / NCT) * 16, tc = (t % NCT) * 16;
@@ -2944,9 +2822,6 @@ void factor_launch(dcx_plan* P, int d, const int* lvl, int cnt, size_t lds, hipS
   T* fr[2] = {static_cast<T*>(P->d_front[0]), static_cast<T*>(P->d_front[1])};
   const int64_t fbs[2] = {S.front_size[0], S.front_size[1]};
   dim3 grid(cnt, P->nbatch);
-  // zero only the lower triangle of the fronts' J/R rows (DCX_FRONT_ZERO_ALL=1:
-  // the whole front, the earlier form)
-  static const int zero_lower = !env_int("DCX_FRONT_ZERO_ALL", 0);
   // `done`: an event that the launch itself signals when the kernel ends (no
   // separate marker between this level and the next)
   hipExtLaunchKernelGGL((k_factor_level<T, NT, RPT, NB>), grid, dim3(NT), lds, st, nullptr, done,
@@ -2954,7 +2829,7 @@ void factor_launch(dcx_plan* P, int d, const int* lvl, int cnt, size_t lds, hipS
                         P->d_a_col, factor_vals(P), static_cast<int64_t>(S.nnz), fr[d & 1],
                         fbs[d & 1], fr[(d + 1) & 1], fbs[(d + 1) & 1],
                         static_cast<T*>(P->d_panels), static_cast<T*>(P->d_panelsT),
-                        S.panel_size, P->d_info, zero_lower);
+                        S.panel_size, P->d_info);
 }
 
 // a level with fronts too tall for k_factor_level: grid-wide launches
@@ -3104,16 +2979,13 @@ void sweep_launch(dcx_plan* P, T* W, int k, int maxnf, int maxns, int pmax, int 
   T* panels = static_cast<T*>(P->d_panels);
   const int64_t w_bs = static_cast<int64_t>(S.n) * k;
   if (fwd) {
-    // every row tile of a wavefront in one k-loop (DCX_FWD_MT=0: one output
-    // tile after another, the earlier form)
-    static const int fwd_mt = env_int("DCX_FWD_MT", 1);
     T* u[2] = {static_cast<T*>(P->d_upd[0]), static_cast<T*>(P->d_upd[1])};
     const int64_t ubs[2] = {S.upd_rows[0] * k, S.upd_rows[1] * k};
     hipLaunchKernelGGL((k_fwd_level<T, TC, PL>), grid, dim3(kThreads), lds, st, P->d_sup, lvl,
                        P->d_relpos, P->d_cidx, active, panels, S.panel_size, io.B, io.b_bs,
                        io.brow, W, w_bs, k, u[d & 1], ubs[d & 1], u[(d + 1) & 1],
                        ubs[(d + 1) & 1], static_cast<const T*>(P->d_ufz), P->S.f_upd_rows * k,
-                       ct, pmax, cnt, nzb, total, fwd_mt);
+                       ct, pmax, cnt, nzb, total);
   } else {
     hipLaunchKernelGGL((k_bwd_level<T, TC, PL>), grid, dim3(kThreads), lds, st, P->d_sup, lvl,
                        P->d_ridx, active, static_cast<const T*>(P->d_panelsT), S.panel_size, W,
