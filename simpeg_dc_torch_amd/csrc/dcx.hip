@@ -2192,6 +2192,7 @@ __global__ void k_jmap(int ncell, int r, const int* __restrict__ tptr,
 // the last partial block of columns included; FP64 sums.  With pad_col /
 // pad_out (rows of at most MAXE entries, padded with -1) a row's columns and
 // output positions come from one load each, with no row-pointer load first.
+constexpr size_t kGatherLds = 25 * 1024;  // occupancy cap of k_grad_values (see its launch)
 template <typename TI, int MAXE, int LPR, bool PAD>
 __global__ void k_grad_values(const int* __restrict__ crow, const int* __restrict__ ccol,
                               const int* __restrict__ oidx, const int* __restrict__ pad_col,
@@ -2205,7 +2206,12 @@ __global__ void k_grad_values(const int* __restrict__ crow, const int* __restric
   const int lb = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
   const int lane = threadIdx.x & 63;
   const int sl = lane % LPR;  // lane within the row's segment
-  const int row = (lb * (blockDim.x / 64) + threadIdx.x / 64) * RPW + lane / LPR;
+  int row = (lb * (blockDim.x / 64) + threadIdx.x / 64) * RPW + lane / LPR;
+  // one row per wavefront: the row and its neighbours' rows are the same in
+  // every lane; said so, their bases live in scalar registers and each field
+  // load takes one 32-bit lane offset (61 instead of 88 VGPRs; the launch
+  // caps the wavefronts per SIMD at 6, see kGatherLds)
+  if constexpr (LPR == 64) row = __builtin_amdgcn_readfirstlane(row);
   const int b = blockIdx.y;
   // a row's segment is wholly active or wholly gone, and the reductions
   // below stay inside one segment
@@ -2226,6 +2232,7 @@ __global__ void k_grad_values(const int* __restrict__ crow, const int* __restric
       } else {
         cq = ccol[p0 + min(q, ne - 1)];  // surplus slots repeat the last entry
       }
+      if constexpr (LPR == 64) cq = __builtin_amdgcn_readfirstlane(cq);
       xr[q] = X + xb + static_cast<int64_t>(cq) * k;
     }
     double acc[MAXE];
@@ -2235,11 +2242,11 @@ __global__ void k_grad_values(const int* __restrict__ crow, const int* __restric
     for (; c + LPR * (U - 1) < k; c += LPR * U) {
       TI yv[U], xv[MAXE][U];
 #pragma unroll
-      for (int u = 0; u < U; ++u) yv[u] = y[c + LPR * u];
+      for (int u = 0; u < U; ++u) yv[u] = y[static_cast<uint32_t>(c + LPR * u)];
 #pragma unroll
       for (int q = 0; q < MAXE; ++q)
 #pragma unroll
-        for (int u = 0; u < U; ++u) xv[q][u] = xr[q][c + LPR * u];
+        for (int u = 0; u < U; ++u) xv[q][u] = xr[q][static_cast<uint32_t>(c + LPR * u)];
 #pragma unroll
       for (int q = 0; q < MAXE; ++q)
 #pragma unroll
@@ -2262,11 +2269,11 @@ __global__ void k_grad_values(const int* __restrict__ crow, const int* __restric
 #pragma unroll
       for (int u = 0; u < U; ++u) cu[u] = min(c + LPR * u, k - 1);
 #pragma unroll
-      for (int u = 0; u < U; ++u) yv[u] = y[cu[u]];
+      for (int u = 0; u < U; ++u) yv[u] = y[static_cast<uint32_t>(cu[u])];
 #pragma unroll
       for (int q = 0; q < MAXE; ++q)
 #pragma unroll
-        for (int u = 0; u < U; ++u) xv[q][u] = xr[q][cu[u]];
+        for (int u = 0; u < U; ++u) xv[q][u] = xr[q][static_cast<uint32_t>(cu[u])];
 #pragma unroll
       for (int u = 0; u < U; ++u)
         if (c + LPR * u >= k) yv[u] = TI(0);
@@ -4325,8 +4332,16 @@ int dcx_grad_values(dcx_plan* P, const void* y, const void* x, int k,
     constexpr int ME = decltype(me)::value;
     // padded rows when every row fits the pass width
     const bool padded = P->gpad_me == ME;
+    // k_grad_values reserves kGatherLds bytes of LDS it does not use: at most
+    // 6 workgroups (wavefronts per SIMD) per CU.  With its row bases in
+    // scalar registers the one-row-per-wavefront form needs 61 VGPRs, i.e. 8
+    // wavefronts per SIMD, and at 8 the neighbour rows that several rows
+    // re-read are evicted before the re-reads: mixed eval+grad of the 25k-cell
+    // line 3.54 ms at 8, 3.40 at 7, 3.33 at 6 and 5, 3.35 at 4, 3.40 at 3
+    // (3.37 for the earlier per-lane-address form, 88 VGPRs, 5 per SIMD)
     auto launch = [&](auto kern) {
-      hipLaunchKernelGGL(kern, grid, dim3(256), 0, st, rp, cl, oi, padded ? pcol : nullptr,
+      hipLaunchKernelGGL(kern, grid, dim3(256), kGatherLds, st, rp, cl, oi,
+                         padded ? pcol : nullptr,
                          padded ? pout : nullptr, static_cast<const TI*>(y),
                          static_cast<const TI*>(x), S.n, k, scale, gv,
                          static_cast<int64_t>(S.nnz));
