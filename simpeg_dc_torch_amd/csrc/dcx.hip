@@ -1331,8 +1331,13 @@ __device__ inline double flip_sign(double x, uint32_t m) {
   return __longlong_as_double(__double_as_longlong(x) ^ (static_cast<long long>(m) << 32));
 }
 
+// The FP32 split-K variants ask the register allocator for 5 wavefronts per
+// SIMD: their accumulators then stay in VGPRs instead of adding AGPRs (MT 8:
+// 87 + 32 -> 87 registers, 4 -> 5 wavefronts per SIMD; MT 6: 4 -> 6), with
+// no spills; the FP64 ones would spill at MT 6 and 8, the wide variants at
+// MT >= 3 (both measured slower).
 template <typename T, int MT, int V, int KS, int PF>
-__global__ __launch_bounds__(kThreads) void k_bwd_wave(
+__global__ __launch_bounds__(kThreads, (KS > 1 && sizeof(T) == 4 ? 5 : 1)) void k_bwd_wave(
     const SupDev* __restrict__ sup, const int* __restrict__ lvl,
     const int* __restrict__ ridx, const int* __restrict__ active,
     const T* __restrict__ panelsT, int64_t panel_bs, T* __restrict__ W, int64_t w_bs, int k,
