@@ -213,8 +213,10 @@ __device__ inline double lane_value(double v, int l) {
 // zero the front, seed the E rows with the identity, scatter the A entries
 // and extend-add the children's update blocks (lower triangles; relative
 // positions staged in the LDS array rpl).  Global gathers take kGather
-// entries per thread at a time, loads first, so that their latencies overlap.
-constexpr int kGather = 16;
+// entries per thread at a time, loads first, so that their latencies overlap
+// (8, not 16: the FP64 narrow-panel factor then needs 85 instead of 107
+// VGPRs; FP64 eval+grad 5.41 -> 5.39 ms, the 16-line batch 9.96 -> 9.90 ms).
+constexpr int kGather = 8;
 template <typename T, int NT>
 __device__ void front_init(const SupDev& S, int b, const SupDev* __restrict__ sup,
                            const int* __restrict__ relpos, const int* __restrict__ cidx,
@@ -248,13 +250,13 @@ __device__ void front_init(const SupDev& S, int b, const SupDev* __restrict__ su
   const double* v = vals + b * nnz;
   for (int q0 = S.a_begin + tid; q0 < S.a_end; q0 += NT * kU) {
     T av[kU];
-    int64_t ad[kU];
+    int ad[kU];  // front offsets: in-LDS fronts hold < 2^31 entries
 #pragma unroll
     for (int u = 0; u < kU; ++u) {
       const int q = q0 + u * NT;
       ad[u] = -1;
       if (q < S.a_end) {
-        ad[u] = a_row[q] + static_cast<int64_t>(a_col[q]) * ld;
+        ad[u] = a_row[q] + a_col[q] * ld;
         av[u] = static_cast<T>(v[a_e[q]]);
       }
     }
@@ -445,9 +447,11 @@ __device__ void panel_out(const SupDev& S, const T* __restrict__ F, T* __restric
 }
 
 // NB: panel width (32; 16 in the narrow-panel variant: shorter register
-// rows and diagonal-block steps, one more trailing pass per 16 columns)
+// rows and diagonal-block steps, one more trailing pass per 16 columns).
+// The FP32 256-thread variants are built for 6 wavefronts per SIMD (80
+// VGPRs, no spills; 5 at their natural 81-83); the FP64 ones would spill.
 template <typename T, int NT, int RPT, int NB>
-__global__ __launch_bounds__(NT) void k_factor_level(
+__global__ __launch_bounds__(NT, (NT == 256 && sizeof(T) == 4 ? 6 : 1)) void k_factor_level(
     const SupDev* __restrict__ sup, const int* __restrict__ lvl,
     const int* __restrict__ relpos, const int* __restrict__ cidx,
     const int* __restrict__ a_e, const int* __restrict__ a_row,
@@ -522,7 +526,7 @@ __global__ __launch_bounds__(NT) void k_factor_level(
     constexpr int TB = 2;  // tiles per wavefront whose loads are in flight together
     for (int tt0 = wave; tt0 < ntri + nte; tt0 += NW * TB) {
       T cold[TB][4];
-      int64_t addr[TB][4];
+      int addr[TB][4];  // front offsets (< 2^31 entries)
       bool ok[TB][4];
       int i0[TB], l0[TB];
 #pragma unroll
@@ -549,7 +553,7 @@ __global__ __launch_bounds__(NT) void k_factor_level(
         for (int e = 0; e < 4; ++e) {
           const int l = l0[u] + Mma<T>::row(lane, e);
           ok[u][e] = i < imax && l < n1 && (!diag || i >= l);
-          addr[u][e] = (j0 + i) + static_cast<int64_t>(j0 + l) * ld;
+          addr[u][e] = (j0 + i) + (j0 + l) * ld;
           cold[u][e] = ok[u][e] ? F[addr[u][e]] : T(0);
         }
       }
