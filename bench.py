@@ -428,8 +428,10 @@ def lines_times(steps, warmup, autotune=True, per_line=True):
     issued from 4 host threads (leaf size autotuned per size class, tuning
     outside the timed region); and all lines in one plan (`LineForest`: one
     launch sequence per step).  The one-plan results are compared with the
-    per-line engines' (both mixed precision).  per_line=False: the one-plan
-    leg alone (for profiling it)."""
+    per-line engines' (both mixed precision).  Beside them, FP64: the
+    per-line batch eager and as one captured graph per line, and the one-plan
+    batch eager and as one captured graph for every line.  per_line=False:
+    the mixed one-plan leg alone (for profiling it)."""
     from simpeg_dc_torch_amd import workloads
     from simpeg_dc_torch_amd.lines import LineBatch
     L = workloads.uneven_lines_case()
@@ -516,6 +518,7 @@ def lines_times(steps, warmup, autotune=True, per_line=True):
     torch.cuda.synchronize()
     graph_diff = max(abs(float(p) / r[0] - 1.0) for (p, _), r in zip(out64, ref64))
     del lb64
+    forest64 = forest_graph_times(lines, ms, data, steps, warmup)
     ms_forest = one["one_plan_ms"]
     cells = sum(mesh.nC for mesh, _ in lines)
     return {"lines": len(lines), "cells_total": cells,
@@ -527,6 +530,7 @@ def lines_times(steps, warmup, autotune=True, per_line=True):
             "electrode_basis_grad_max_rel_diff": float(f"{elec_diff:.1e}"),
             "fp64_streams4_ms": round(ms64_eager, 2), "fp64_graphs_ms": round(ms64_graph, 2),
             "fp64_graph_phi_max_rel_diff": float(f"{graph_diff:.1e}"),
+            **forest64,
             "one_plan_electrode_basis_ms": one_e["one_plan_ms"],
             "lines_per_s": round(1e3 * len(lines) / min(ms_par, ms_thr, ms_forest, ms_elec,
                                                         one_e["one_plan_ms"]), 1),
@@ -535,7 +539,37 @@ def lines_times(steps, warmup, autotune=True, per_line=True):
             "leaf_by_size_class": {str(k): v for k, v in (lb.leaf or {}).items()},
             "tiles_by_size_class": {str(k): v for k, v in lb.tiles.items()},
             "build_and_tune_s": round(t_tune, 1),
-            "graph_capture": "FP64 engines (mixed-precision capture refused)"}
+            "graph_capture": "FP64 engines and FP64 one-plan batches (mixed-precision "
+                             "capture refused)"}
+
+
+def forest_graph_times(lines, ms, data, steps, warmup):
+    """Every line in one FP64 plan (`LineForest`), eager and as one captured
+    HIP graph replayed per step (tiles at their defaults)."""
+    from simpeg_dc_torch_amd.lines import LineForest
+    lf = LineForest(lines, precision="fp64")
+    lf.set_data([d for d, _ in data], [w for _, w in data])
+
+    def timed():
+        for _ in range(warmup):
+            lf.eval_grad(ms)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(steps):
+            lf.eval_grad(ms)
+        torch.cuda.synchronize()
+        return 1e3 * (time.perf_counter() - t1) / steps
+
+    eager = timed()
+    ref = [float(p) for p, _ in lf.eval_grad(ms)]
+    lf.capture([m.cpu().numpy() for m in ms])
+    graph = timed()
+    out = lf.eval_grad(ms)
+    torch.cuda.synchronize()
+    diff = max(abs(float(p) / r - 1.0) for (p, _), r in zip(out, ref))
+    lf.free()
+    return {"one_plan_fp64_ms": round(eager, 2), "one_plan_fp64_graph_ms": round(graph, 2),
+            "one_plan_fp64_graph_phi_max_rel_diff": float(f"{diff:.1e}")}
 
 
 def one_plan_times(lines, ms, data, steps, warmup, compare=None, rhs_basis="sources"):

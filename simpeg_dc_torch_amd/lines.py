@@ -18,7 +18,8 @@ launch economy:
   engines only: replays of FP64 graphs match the eager results bit for bit,
   while a replay of mixed-precision graphs ended the process with a host
   segmentation fault inside hipGraphLaunch whose cause is not found, so
-  capture refuses mixed-precision engines;
+  capture refuses mixed-precision engines (`LineForest.capture` records the
+  one-plan batch as a single graph, under the same restriction);
 * autotuning: `autotune_leaf` times the nested-dissection leaf size per size
   class (cells rounded to a power of two) on one line of that class and
   `LineBatch(leaf="auto")` builds every line with its class's choice;
@@ -478,6 +479,7 @@ class LineForest:
         self.nD = [m.nD for m in maps]
         self.nC = [m.nC for m in maps]
         self.precision = precision
+        self._graph = None
         self._active_map(active_cells, val_inactive)
         self.solver = HipBatchSolver(
             torch.tensor(M["crow"]), torch.tensor(M["col"]), M["n"], M["K"], M["nbatch"],
@@ -626,10 +628,9 @@ class LineForest:
         self._fill = torch.tensor(fill, dtype=torch.float64, device=self.device)
         self._moff = np.r_[0, np.cumsum(self.n_model)]
 
-    def _full_model(self, ms):
-        """Stacked full-mesh model (log-resistivity) from the lines' models."""
-        m = torch.cat([torch.as_tensor(x, dtype=torch.float64).reshape(-1).to(self.device)
-                       for x in ms])
+    def _full_model(self, m):
+        """Stacked full-mesh model (log-resistivity) from the stacked models
+        of the lines' active cells (`_model_vector`)."""
         if self._act is None:
             return m
         if m.numel() != self._act.numel():
@@ -662,16 +663,74 @@ class LineForest:
         cat = lambda xs: torch.cat([torch.as_tensor(x, dtype=torch.float64).reshape(-1).to(
             self.device) for x in xs])
         self._data = (cat(dobs), cat(w))
+        self._graph = None  # a captured graph reads the data it was recorded with
+
+    def capture(self, ms, warmup=2):
+        """Record the eval+grad of every line at the models `ms` as ONE HIP
+        graph (through `torch.cuda.CUDAGraph`, after `warmup` eager runs on
+        its stream that size every workspace the plan keeps); later
+        `eval_grad` calls copy their models into the graph's input tensor and
+        replay it.  FP64 plans only, as `LineBatch.capture` (a replay of
+        mixed-precision graphs ended a process inside hipGraphLaunch, cause
+        not found).  The graph takes the stacked model from a device tensor:
+        the host-side stacking of `ms` stays outside it (a captured copy from
+        pageable host memory would re-read that host buffer at every
+        replay).  The solve checks do not run inside a captured graph."""
+        if self.precision != "fp64":
+            raise NotImplementedError(
+                "graph capture is limited to FP64 plans: a replay of mixed-precision graphs "
+                "ended in a host segmentation fault inside hipGraphLaunch (cause not found)")
+        if self._data is None:
+            raise RuntimeError("set_data first")
+        dev = self.device
+        self._graph = None
+        m_in = self._model_vector(ms).clone()
+        s = torch.cuda.Stream(device=dev)  # default priority
+        s.wait_stream(torch.cuda.current_stream(dev))
+        with torch.cuda.stream(s):
+            for _ in range(warmup):
+                self._eval_grad_vec(m_in)
+        s.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=s):
+            out = self._eval_grad_vec(m_in)
+        torch.cuda.synchronize(dev)
+        self._graph = (g, m_in, out, self.solver.info()["buffer_generation"])
+
+    def _model_vector(self, ms):
+        """The lines' models (active cells) stacked on the device."""
+        return torch.cat([torch.as_tensor(x, dtype=torch.float64).reshape(-1).to(self.device)
+                          for x in ms])
 
     def eval_grad(self, ms):
         """[(phi_i, grad_i)] for the models `ms` (one per line, m = log rho on
         the line's active cells):
         phi_i = ||w_i (dpred_i - dobs_i)||^2 and its gradient, all lines in
-        one launch sequence on the current stream, no host synchronisation."""
+        one launch sequence on the current stream, no host synchronisation.
+        After `capture` it replays the graph: phi_i and grad_i are then views
+        of the graph's output buffers, overwritten by the next call."""
         if self._data is None:
             raise RuntimeError("set_data first")
+        m = self._model_vector(ms)
+        graph = getattr(self, "_graph", None)
+        if graph is not None:
+            g, m_in, out, gen = graph
+            if self.solver.info()["buffer_generation"] != gen:
+                self._graph = None
+                raise RuntimeError("the plan reallocated its workspace after capture(): the "
+                                   "recorded graph is dropped; call capture() again")
+            if m.numel() != m_in.numel():
+                raise ValueError(f"models hold {m.numel()} values, the graph {m_in.numel()}")
+            with torch.no_grad():
+                m_in.copy_(m)
+            g.replay()  # on torch's current stream, after the copy
+            return self._per_line(*out)
+        return self._per_line(*self._eval_grad_vec(m))
+
+    def _eval_grad_vec(self, m):
+        """(phi per line, full-mesh gradient) for the stacked active model m."""
         if self.rhs_basis == "electrodes":
-            return self._eval_grad_electrodes(ms)
+            return self._eval_grad_electrodes(m)
         dobs, w = self._data
         L, lib, sv, dev = self._libm, self.lib, self.solver, self.device
         f32 = self.precision == "mixed"
@@ -679,7 +738,7 @@ class LineForest:
         n, K, nb = sv.n, self.K, self.nbatch
         st = self._stream()
         with torch.no_grad(), torch.cuda.device(dev):
-            sig = torch.exp(-self._full_model(ms))
+            sig = torch.exp(-self._full_model(m))
             vals = torch.empty(nb, self.M["nnz"], dtype=torch.float64, device=dev)
             L.check(lib.dcx_assemble(self.lmap, L.ptr(sig), L.ptr(vals), st), "assemble")
             U = torch.empty(nb, n, K, dtype=torch.float32 if f32 else torch.float64, device=dev)
@@ -716,9 +775,9 @@ class LineForest:
             grad = torch.empty_like(sig)
             L.check(lib.dcx_lmap_transpose(self.lmap, L.ptr(gv), L.ptr(sig), L.ptr(grad), st),
                     "lmap_transpose")
-        return self._per_line(phi, grad)
+        return phi, grad
 
-    def _eval_grad_electrodes(self, ms):
+    def _eval_grad_electrodes(self, m):
         """eval_grad in the electrode basis: one solve, no adjoint solve."""
         dobs, w = self._data
         L, lib, sv, dev = self._libm, self.lib, self.solver, self.device
@@ -727,7 +786,7 @@ class LineForest:
         n, K, nb, nl = sv.n, self.K, self.nbatch, self.nlines
         st = self._stream()
         with torch.no_grad(), torch.cuda.device(dev):
-            sig = torch.exp(-self._full_model(ms))
+            sig = torch.exp(-self._full_model(m))
             vals = torch.empty(nb, self.M["nnz"], dtype=torch.float64, device=dev)
             L.check(lib.dcx_assemble(self.lmap, L.ptr(sig), L.ptr(vals), st), "assemble")
             G = torch.empty(nb, n, K, dtype=torch.float32 if f32 else torch.float64, device=dev)
@@ -740,7 +799,7 @@ class LineForest:
             grad = torch.empty_like(sig)
             L.check(lib.dcx_lmap_transpose(self.lmap, L.ptr(gv), L.ptr(sig), L.ptr(grad), st),
                     "lmap_transpose")
-        return self._per_line(phi, grad)
+        return phi, grad
 
     def _electrode_rest(self, G, dobs, w):
         """Everything of the electrode-basis eval+grad after the solve G:

@@ -2,9 +2,10 @@
 
 CPU: the workload generator's sizes and electrode placement.  GPU: the
 stream-concurrent batch (one issuing host thread, or one per stream) against
-each line's own eval+grad on the default stream, the captured graphs against eager runs at several models with eager
-work and synchronisations between the replays (every replay checked), and
-the leaf autotuner's contract.
+each line's own eval+grad on the default stream, the captured graphs (one per
+line, and one for every line of the one-plan batch) against eager runs at
+several models with eager work and synchronisations between the replays
+(every replay checked), and the leaf autotuner's contract.
 """
 import numpy as np
 import pytest
@@ -291,6 +292,55 @@ def test_line_forest_matches_lines(precision, tol_phi, tol_g, basis):
                 gi = out[i][1].cpu().numpy()
                 assert np.linalg.norm(gi - g) <= tol_g * np.linalg.norm(g), (name, rep, i)
         lf.free()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("basis", ["sources", "electrodes"])
+def test_line_forest_graph_replays(basis):
+    """Every line's eval+grad as one captured graph (FP64 plan): captured at
+    one model, replayed at three, each replay preceded by an eager eval+grad
+    of the same plan at another model and a synchronisation; every replay
+    equals the eager result at its model.  A mixed-precision plan's capture
+    is refused."""
+    from simpeg_dc_torch_amd import TorchDC2D
+    from simpeg_dc_torch_amd.lines import LineForest
+    print(f"line forest graphs {basis}", flush=True)
+    L, nkys = _lines(4), [6] * 4
+    data = []
+    for mesh, survey, m in L:
+        e = TorchDC2D(mesh, survey, nky=6, device="cuda")
+        with torch.no_grad():
+            d = e.dpred(torch.tensor(m, device="cuda")).cpu().numpy()
+        data.append(noisy_data(d, seed=9))
+    lf = LineForest([(mesh, survey) for mesh, survey, _ in L], nky=nkys, precision="fp64",
+                    rhs_basis=basis)
+    lf.set_data([d for d, _ in data], [w for _, w in data])
+    rng = np.random.default_rng(2)
+    ms0 = [m for _, _, m in L]
+    lf.capture(ms0)
+
+    def eager(ms):
+        phi, grad = lf._eval_grad_vec(lf._model_vector(ms))
+        return [(float(p), g.cpu().numpy()) for p, g in lf._per_line(phi, grad)]
+    for rep in range(3):
+        ms = [m + 0.1 * (rep + 1) * rng.standard_normal(m.size) for m in ms0]
+        other = [m - 0.05 * rng.standard_normal(m.size) for m in ms0]
+        eager(other)                      # eager work on the same plan between replays
+        ref = eager(ms)
+        torch.cuda.synchronize()
+        out = lf.eval_grad(ms)
+        torch.cuda.synchronize()
+        for i in range(len(L)):
+            assert float(out[i][0]) == pytest.approx(ref[i][0], rel=1e-13), (rep, i)
+            g = out[i][1].cpu().numpy()
+            assert np.abs(g - ref[i][1]).max() <= 1e-13 * np.abs(ref[i][1]).max(), (rep, i)
+    lf.free()
+    lm = LineForest([(mesh, survey) for mesh, survey, _ in L[:2]], nky=nkys[:2],
+                    precision="mixed", rhs_basis=basis)
+    lm.set_data([d for d, _ in data[:2]], [w for _, w in data[:2]])
+    with pytest.raises(NotImplementedError):
+        lm.capture(ms0[:2])
+    lm.free()
 
 
 @pytest.mark.gpu
