@@ -134,6 +134,39 @@ def sweep_roofline(eng, mt, dobs, w, refine, reps=3):
             "fwd_sweep_ms": round(f_ms / nsolve, 3), "bwd_sweep_ms": round(b_ms / nsolve, 3)}
 
 
+def factor_roofline(eng, mt, reps=10):
+    """The numeric factorisation (K2: `k_factor_level`, one launch per tree
+    level, all wavenumbers at once) alone: its time per factorisation from
+    HIP events on the solve stream (torch's current stream, where
+    dcx_factorize runs every level), its flops (partial Cholesky of every
+    front plus the inverse of its diagonal block, the symbolic phase's
+    count) against the factor precision's MFMA peak, and its algorithmic
+    bytes nky (8 nnzA + s_f nnz(L)) against HBM peak."""
+    sv, g = eng.solver, eng._gpu
+    info = sv.info()
+    with torch.no_grad():
+        vals = g.assemble(torch.exp(-mt))
+    sv.factorize(vals)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        sv.factorize(vals)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    fp32 = info["precision"] == 1
+    nb = info["nbatch"]
+    flops = nb * info["factor_flops"]
+    bytes_ = nb * (8 * sv.col.numel() + (4 if fp32 else 8) * info["nnz_L"])
+    peak = MFMA_PEAK["fp32" if fp32 else "fp64"]
+    tflops = flops / (ms * 1e-3) / 1e12
+    return {"kernel": "k_factor_level", "bound": "mfma", "ms": round(ms, 3),
+            "gflop": round(flops / 1e9, 3), "tflops": round(tflops, 2), "peak_tflops": peak,
+            "frac": round(tflops / peak, 4), "bytes": int(bytes_),
+            "achieved_gbs": round(bytes_ / (ms * 1e-3) / 1e9, 1), "launches": info["ndepth"]}
+
+
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_line25k_eval_grad.json")
 
 
@@ -944,6 +977,8 @@ def main():
     roof64 = sweep_roofline(e64, mt, dobs, w, 0)
     roof["traffic"] = measured_traffic("k_bwd_wave<float>")
     roof64["traffic"] = measured_traffic("k_bwd_wave<double>")
+    froof, froof64 = factor_roofline(emx, mt), factor_roofline(e64, mt)
+    log(f"factorisation: mixed {froof}, fp64 {froof64}")
     log("linearisation (FP64): J v, J^T v, J, diag(J^T D J)")
     lin = linearization_times(e64, mt)
     log(f"linearisation {lin}")
@@ -999,6 +1034,7 @@ def main():
         "build_s": round(t_build, 2),
         "tile_tuning": tiles,
         "roofline": roof, "roofline_fp64": roof64,
+        "factor_roofline": froof, "factor_roofline_fp64": froof64,
         "electrode_basis": eb,
         "linearization": lin,
         "grid3d": g3,
