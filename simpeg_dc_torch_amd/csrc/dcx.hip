@@ -1335,15 +1335,23 @@ __device__ inline double flip_sign(double x, uint32_t m) {
   return __longlong_as_double(__double_as_longlong(x) ^ (static_cast<long long>(m) << 32));
 }
 
-// The FP32 split-K variants, and the wide ones of at most 3 row tiles with 3
-// k-steps loaded ahead (bwd_wave_mt), ask the register allocator for 5
-// wavefronts per SIMD: their accumulators then stay in VGPRs instead of
-// adding AGPRs, with no spills (split-K MT 8: 87 + 32 -> 87 registers, 4 -> 5
-// wavefronts per SIMD; MT 6: 4 -> 6; wide MT 3, the leaf level: 62 + 52 -> 94,
-// 4 -> 5; wide MT 2: 5 -> 6).  The FP64 ones would spill at MT 6 and 8, the
-// wide ones at MT >= 4 or with 4 k-steps ahead (all measured slower).
+// Wavefronts per SIMD asked of the register allocator (kBwdWaves): 5 for the
+// FP32 split-K variants and the wide ones of at most 3 row tiles with 3
+// k-steps loaded ahead (bwd_wave_mt), 4 for the wide 4-row-tile one; their
+// accumulators then stay in VGPRs instead of adding AGPRs, with no spills
+// (split-K MT 8: 87 + 32 -> 87 registers, 4 -> 5 wavefronts per SIMD; MT 6:
+// 4 -> 6; wide MT 3, the leaf level: 62 + 52 -> 94, 4 -> 5; wide MT 2: 5 -> 6;
+// wide MT 4: 78 + 68 -> 113, 3 -> 4).  The FP64 ones would spill at MT 6 and
+// 8, the wide ones beyond these counts or with 4 k-steps ahead (all measured
+// slower).
 template <typename T, int MT, int V, int KS, int PF>
-__global__ __launch_bounds__(kThreads, (sizeof(T) == 4 && (KS > 1 || (KS == 1 && MT <= 3 && PF <= 3)) ? 5 : 1)) void k_bwd_wave(
+constexpr int kBwdWaves = sizeof(T) != 4 ? 1
+                          : KS > 1 ? 5
+                          : KS == 1 && MT <= 3 && PF <= 3 ? 5
+                          : KS == 1 && MT == 4 && PF <= 3 ? 4
+                          : 1;
+template <typename T, int MT, int V, int KS, int PF>
+__global__ __launch_bounds__(kThreads, (kBwdWaves<T, MT, V, KS, PF>)) void k_bwd_wave(
     const SupDev* __restrict__ sup, const int* __restrict__ lvl,
     const int* __restrict__ ridx, const int* __restrict__ active,
     const T* __restrict__ panelsT, int64_t panel_bs, T* __restrict__ W, int64_t w_bs, int k,
@@ -3070,15 +3078,15 @@ template <typename T, int V, int KS, int PF>
 bool bwd_wave_mt(dcx_plan* P, T* W, int k, int maxns, int maxrs, const int* lvl, int cnt,
                  const int* active, const SweepIO& io, hipStream_t st) {
   const int mt = (maxns + 15) / 16;
-  // FP32 wide tiles of up to 3 row tiles: 3 k-steps ahead, built for 5
-  // wavefronts per SIMD (k_bwd_wave): mixed eval+grad 3.337 -> 3.318 ms, the
-  // backward sweep 0.805 -> 0.785 ms, the 16-line batch 9.88 -> 9.69 ms (2 k-steps
-  // ahead: 3.324-3.337 ms)
+  // FP32 wide tiles of up to 4 row tiles: 3 k-steps ahead, built for 5 (4 for
+  // MT 4) wavefronts per SIMD (kBwdWaves): mixed eval+grad 3.337 -> 3.318 ms
+  // with MT <= 3, 3.33 -> 3.28 ms with MT 4 added; the 16-line batch 9.88 ->
+  // 9.60 ms (2 k-steps ahead: within 0.01 ms of 3)
   constexpr int PFN = (KS == 1 && sizeof(T) == 4) ? 3 : PF;
   if (mt <= 1) bwd_wave_launch<T, 1, V, KS, PFN>(P, W, k, lvl, cnt, active, io, st, 1, maxrs);
   else if (mt <= 2) bwd_wave_launch<T, 2, V, KS, PFN>(P, W, k, lvl, cnt, active, io, st, 1, maxrs);
   else if (mt <= 3) bwd_wave_launch<T, 3, V, KS, PFN>(P, W, k, lvl, cnt, active, io, st, 1, maxrs);
-  else if (mt <= 4) bwd_wave_launch<T, 4, V, KS, PF>(P, W, k, lvl, cnt, active, io, st, 1, maxrs);
+  else if (mt <= 4) bwd_wave_launch<T, 4, V, KS, PFN>(P, W, k, lvl, cnt, active, io, st, 1, maxrs);
   else if (mt <= 6) bwd_wave_launch<T, 6, V, KS, PF>(P, W, k, lvl, cnt, active, io, st, 1, maxrs);
   else  // taller fronts: row groups of 128 rows
     bwd_wave_launch<T, 8, V, KS, PF>(P, W, k, lvl, cnt, active, io, st, (mt + 7) / 8, maxrs);
