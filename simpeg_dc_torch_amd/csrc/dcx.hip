@@ -2212,8 +2212,13 @@ __global__ void k_jmap(int ncell, int r, const int* __restrict__ tptr,
 // pad_out (rows of at most MAXE entries, padded with -1) a row's columns and
 // output positions come from one load each, with no row-pointer load first.
 constexpr size_t kGatherLds = 25 * 1024;  // occupancy cap of k_grad_values (see its launch)
+// FP32 rows of several per wavefront (LPR < 64) and up to 5 entries: built
+// for 6 wavefronts per SIMD (88 -> 76 VGPRs, no spills), the launch's LDS cap:
+// 16-line batch 9.62 -> 9.51 ms, its electrode basis 8.66 -> 8.54 ms (8-entry
+// rows would spill)
 template <typename TI, int MAXE, int LPR, bool PAD>
-__global__ void k_grad_values(const int* __restrict__ crow, const int* __restrict__ ccol,
+__global__ __launch_bounds__(256, (LPR < 64 && sizeof(TI) == 4 && MAXE <= 5 ? 6 : 1))
+void k_grad_values(const int* __restrict__ crow, const int* __restrict__ ccol,
                               const int* __restrict__ oidx, const int* __restrict__ pad_col,
                               const int* __restrict__ pad_out, const TI* __restrict__ Y,
                               const TI* __restrict__ X, int n, int k,
