@@ -3614,7 +3614,10 @@ int dcx_plan_create(int n, const int32_t* crow, const int32_t* col, int nbatch,
     P->precision = precision;
     P->tsize = precision == DCX_FP32 ? 4 : 8;
     P->tc_cap = env_int("DCX_SWEEP_TC", 32);
-    P->splitk_below = env_int("DCX_BWD_SPLITK_BELOW", 4096);
+    // split-K backward levels below 1024 wavefronts: with tile tuning off the
+    // 25k-cell line's eval+grad takes 3.29 ms (FP64 5.51) against 3.37 (5.57)
+    // at 4096; the sweep column tile (16 or 32) measured equal there
+    P->splitk_below = env_int("DCX_BWD_SPLITK_BELOW", 1024);
     P->rowsplit_below = env_int("DCX_BWD_ROWSPLIT_BELOW", 0);
     P->wide_below = env_int("DCX_FACTOR_WIDE_BELOW", 321);
     P->panel_width = env_int("DCX_FACTOR_PANEL_WIDTH", 32) == 16 ? 16 : 32;
